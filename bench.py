@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Benchmark of the 2D Euler FV time-step path on one MI355X.
+
+A step is one full SSP-RK3 time step (3 fused residual+stage kernels, the
+CFL reduction fused into stage 3, and two one-thread control kernels) over
+the whole grid.  Default workload: the periodic isentropic vortex on
+[0,10]^2 at 1024 x 1024, gamma 1.4, plus 8 seeded Fourier density modes,
+TENO5 characteristic reconstruction + 3-point transverse quadrature + HLLC,
+FP64, CFL 0.5.  The state is generated on the device, so inputs are resident
+in HBM when the timed region starts.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hermite-teno_amd"))
+
+METRIC = "2D Euler cell-updates/s (TENO5/HWENO+HLLC), learned+FP32 vs plain FP64 at equal L1"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+# Algorithmic HBM bytes per cell-update of the FP64 4-field state: stage 1
+# reads u^n and writes u^1, stages 2 and 3 read the stage input and u^n and
+# write their result: (2 + 3 + 3) touches x 4 fields x 8 B.
+BYTES_PER_CELL_UPDATE = 256.0
+# Hand count of FP64 flops per cell-update (characteristic projections, two
+# TENO5 sides per field and direction, 8 transverse quadratures per face and
+# direction, back-projection, flux at 3 Gauss points, divergence, RK3).
+FLOPS_PER_CELL_UPDATE = 18.2e3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n, seconds=12.0):
+    """Time the OpenMP build of the FP64 CPU oracle on the same workload
+    (1 warm-up step, then whole steps until ~`seconds` have elapsed)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np  # noqa: F401
+    from oracle import Oracle, params
+    from htx.ic import vortex_workload
+
+    w = vortex_workload(n)
+    orc = Oracle(omp=True)
+    prm = params(n, n, w["xl"], w["xr"], w["yb"], w["yt"], w["gamma"], "periodic", "hllc", True)
+    q = orc.ic_vortex(prm, w["centre"], w["modes"])
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"[cpu] oracle OpenMP build, {threads} threads, {n}x{n}: warm-up step")
+    q, t, _ = orc.fv2d_advance(prm, q, w["cfl"], 0.0, w["tmax"], 1)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        q, t, k = orc.fv2d_advance(prm, q, w["cfl"], t, w["tmax"], 1)
+        steps += k
+        el = time.perf_counter() - t0
+        log(f"[cpu] step {steps}: {el:.1f} s")
+        if el > seconds or steps >= 20:
+            break
+    rate = n * n * steps / el
+    return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} SSP-RK3 steps of the {n}x{n} vortex workload after 1 warm-up step, "
+                      f"FP64 CPU oracle (gcc -O3 -fopenmp), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--flux", default="hllc")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this path runs on one GPU; multi-GPU sharding is designed "
+                                   "(see DESIGN.md) but not built, so --gpus must be 1"}))
+        sys.exit(2)
+
+    import htx
+    from htx.ic import vortex_workload
+
+    n = args.n
+    w = vortex_workload(n)
+    log(f"[gpu] creating {n}x{n} solver ({args.flux}, fp64)")
+    s = htx.System2D(w["xl"], w["xr"], n, w["yb"], w["yt"], n, w["cfl"], w["tmax"], 0.0, w["gamma"],
+                     flux=args.flux, bc="periodic", precision="fp64")
+    s.isentropicVortex(w["centre"], w["modes"])
+    log(f"[gpu] warm-up {args.warmup} steps")
+    s.steps_async(args.warmup)
+    s.sync()
+    s.timing_enable(True)
+    s.timing_read()  # clear
+    log(f"[gpu] timing {args.steps} steps")
+    s.sync()
+    t0 = time.perf_counter()
+    s.steps_async(args.steps)
+    s.sync()
+    t1 = time.perf_counter()
+    kernel_ms, nlaunch = s.timing_read()
+    s.timing_enable(False)
+    steps_done = s.nsteps - args.warmup
+    if steps_done != args.steps:
+        raise RuntimeError(f"only {steps_done} of {args.steps} steps ran (t_end reached)")
+    el = t1 - t0
+    value = n * n * args.steps / el
+    avg_launch_s = kernel_ms / 1e3 / max(nlaunch, 1)
+    cells = n * n
+    bytes_per_launch = cells * BYTES_PER_CELL_UPDATE / 3.0
+    flops_per_launch = cells * FLOPS_PER_CELL_UPDATE / 3.0
+    achieved_gbs = bytes_per_launch / avg_launch_s / 1e9
+    achieved_tf = flops_per_launch / avg_launch_s / 1e12
+    log(f"[gpu] {value:.4e} cell-updates/s, stage kernel {avg_launch_s * 1e3:.3f} ms avg over {nlaunch}")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded isentropic vortex + Fourier density modes, generated on device)",
+        "config": {"workload": f"2D Euler periodic isentropic vortex + 8 seeded Fourier modes, {n}x{n} on "
+                               f"[0,10]^2, TENO5 char. + 3-pt transverse quadrature + {args.flux.upper()}, "
+                               f"SSP-RK3 CFL 0.5, FP64 (plain drop-in path)",
+                   "nx": n, "ny": n, "scheme": "teno5", "flux": args.flux, "precision": "fp64",
+                   "parallelism": "single GPU"},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved_gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "fv2d_stage_kernel",
+            "avg_launch_ms": avg_launch_s * 1e3,
+            "bytes_per_launch": bytes_per_launch,
+            "valu": {"achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tf / FP64_PEAK_TFLOPS, "flops_per_launch": flops_per_launch,
+                     "note": "binding roof: FP64 VALU (about 71 flop per algorithmic byte)"},
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
+        except Exception as e:  # the baseline is reported, never required for the GPU figure
+            log(f"[cpu] baseline failed: {e}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,407 @@
+// htx_api.hip -- the C ABI of libhtx.so (include/htx.h): handle lifetime,
+// state transfer, the device-resident time loop and stage timing.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/htx.h"
+#include "htx_internal.h"
+
+using namespace htx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(HTX_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+// Gauss-Legendre 9-point rule in the reference's node order
+// (Schemes_Lagrange.cpp:22-42).
+void gauss9(double x[9], double w[9]) {
+    const double a[4] = {0.8360311073266358, 0.9681602395076261, 0.3242534234038089, 0.6133714327005904};
+    const double wa[4] = {0.1806481606948574, 0.08127438836157441, 0.31234707704000286, 0.26061069640293544};
+    x[0] = 0.0;
+    w[0] = 0.3302393550012598;
+    for (int k = 0; k < 4; ++k) {
+        x[1 + 2 * k] = -a[k];
+        x[2 + 2 * k] = a[k];
+        w[1 + 2 * k] = wa[k];
+        w[2 + 2 * k] = wa[k];
+    }
+}
+
+}  // namespace
+
+struct htx_solver {
+    htx_params p{};
+    int nx = 0, ny = 0, nfield = 4, pitch = 0;
+    long long fstride = 0;
+    double dx = 0, dy = 0;
+    double* dA = nullptr;  // u^n / current state
+    double* dB = nullptr;  // stage 1 result
+    double* dC = nullptr;  // stage 2 result
+    double* dR = nullptr;  // residual output
+    Ctl* ctl = nullptr;
+    hipStream_t stream = nullptr;
+    QuadConsts qc{};
+    bool have_state = false;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> event_pool;
+};
+
+namespace {
+
+int check_params(const htx_params* p) {
+    if (!p) return fail(HTX_EINVAL, "params is null");
+    if (p->struct_size != (int)sizeof(htx_params)) return fail(HTX_EINVAL, "params.struct_size mismatch");
+    if (p->dim != 2) return fail(HTX_ENOTSUP, "only dim = 2 is built");
+    if (p->nx <= 0 || p->ny <= 0) return fail(HTX_EINVAL, "grid size must be positive");
+    if (!(p->xr > p->xl) || !(p->yt > p->yb)) return fail(HTX_EINVAL, "domain must have positive extent");
+    if (!(p->gamma > 1.0)) return fail(HTX_EINVAL, "gamma must exceed 1");
+    if (!(p->cfl > 0.0)) return fail(HTX_EINVAL, "cfl must be positive");
+    if (p->scheme != HTX_SCHEME_TENO5 && p->scheme != HTX_SCHEME_HTENO) return fail(HTX_EINVAL, "unknown scheme");
+    if (p->flux != HTX_FLUX_RUSANOV_CHAR && p->flux != HTX_FLUX_HLLC) return fail(HTX_EINVAL, "unknown flux");
+    if (p->bc != HTX_BC_ZERO_GRAD && p->bc != HTX_BC_PERIODIC) return fail(HTX_EINVAL, "unknown bc");
+    if (p->precision != HTX_PREC_FP64 && p->precision != HTX_PREC_FP32_MIXED && p->precision != HTX_PREC_LEARNED)
+        return fail(HTX_EINVAL, "unknown precision");
+    if (p->scheme != HTX_SCHEME_TENO5) return fail(HTX_ENOTSUP, "HTENO scheme not built in this library");
+    if (p->precision != HTX_PREC_FP64) return fail(HTX_ENOTSUP, "only FP64 precision is built in this library");
+    return HTX_OK;
+}
+
+Fv2dArgs stage_args(htx_solver* h, const double* qin, const double* q0, double* qout, int mode) {
+    Fv2dArgs a{};
+    a.qin = qin;
+    a.q0 = q0;
+    a.qout = qout;
+    a.ctl = h->ctl;
+    a.nx = h->nx;
+    a.ny = h->ny;
+    a.pitch = h->pitch;
+    a.fstride = h->fstride;
+    a.bc = h->p.bc;
+    a.mode = mode;
+    a.quirks = h->p.ref_quirks ? 1 : 0;
+    a.gam = h->p.gamma;
+    a.g1 = h->p.gamma - 1.0;
+    a.dx = h->dx;
+    a.dy = h->dy;
+    a.area = h->dx * h->dy;
+    a.qc = h->qc;
+    return a;
+}
+
+int flux_kind(const htx_solver* h) {
+    if (h->p.flux == HTX_FLUX_RUSANOV_CHAR) return FLUX_RUSANOV_CHAR;
+    return h->p.ref_quirks ? FLUX_HLLC : FLUX_HLLC_TORO;
+}
+
+int launch_stage(htx_solver* h, const double* qin, const double* q0, double* qout, int mode) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (h->timing) {
+        if (!h->event_pool.empty()) {
+            e0 = h->event_pool.back().first;
+            e1 = h->event_pool.back().second;
+            h->event_pool.pop_back();
+        } else {
+            HIP_TRY(hipEventCreate(&e0));
+            HIP_TRY(hipEventCreate(&e1));
+        }
+        HIP_TRY(hipEventRecord(e0, h->stream));
+    }
+    HIP_TRY(launch_fv2d_stage(stage_args(h, qin, q0, qout, mode), flux_kind(h), h->p.tile_x, h->p.tile_y,
+                              h->stream));
+    if (h->timing) {
+        HIP_TRY(hipEventRecord(e1, h->stream));
+        h->events.emplace_back(e0, e1);
+    }
+    return HTX_OK;
+}
+
+int enqueue_step(htx_solver* h) {
+    HIP_TRY(launch_dt(h->ctl, h->dx, h->dy, h->stream));
+    int rc;
+    if ((rc = launch_stage(h, h->dA, h->dA, h->dB, 1))) return rc;
+    if ((rc = launch_stage(h, h->dB, h->dA, h->dC, 2))) return rc;
+    if ((rc = launch_stage(h, h->dC, h->dA, h->dA, 3))) return rc;
+    HIP_TRY(launch_tadvance(h->ctl, h->stream));
+    return HTX_OK;
+}
+
+int read_ctl(htx_solver* h, Ctl* c) {
+    HIP_TRY(hipMemcpyAsync(c, h->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return HTX_OK;
+}
+
+int refresh_cfl(htx_solver* h) {
+    HIP_TRY(launch_reset_smax(h->ctl, h->stream));
+    HIP_TRY(launch_max_speed(h->dA, h->nx, h->ny, h->pitch, h->fstride, h->p.gamma, h->p.ref_quirks ? 1 : 0,
+                             h->ctl, h->stream));
+    return HTX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int htx_abi_version(void) { return HTX_ABI_VERSION; }
+
+const char* htx_last_error(void) { return g_err.c_str(); }
+
+int htx_params_default(htx_params* p) {
+    if (!p) return fail(HTX_EINVAL, "params is null");
+    std::memset(p, 0, sizeof(*p));
+    p->struct_size = (int)sizeof(htx_params);
+    p->dim = 2;
+    p->xl = 0.0;
+    p->xr = 1.0;
+    p->nx = 64;
+    p->yb = 0.0;
+    p->yt = 1.0;
+    p->ny = 64;
+    p->cfl = 0.5;
+    p->tmax = 1.0;
+    p->tinit = 0.0;
+    p->gamma = 1.4;
+    p->scheme = HTX_SCHEME_TENO5;
+    p->flux = HTX_FLUX_HLLC;
+    p->bc = HTX_BC_ZERO_GRAD;
+    p->precision = HTX_PREC_FP64;
+    p->ref_quirks = 1;
+    return HTX_OK;
+}
+
+int htx_create(const htx_params* p, htx_solver** out) {
+    if (!out) return fail(HTX_EINVAL, "out is null");
+    *out = nullptr;
+    int rc = check_params(p);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(HTX_EDEVICE, "no HIP device");
+    htx_solver* h = new htx_solver();
+    h->p = *p;
+    h->nx = p->nx;
+    h->ny = p->ny;
+    h->nfield = 4;
+    h->pitch = p->nx;
+    h->fstride = (long long)h->pitch * p->ny;
+    h->dx = (p->xr - p->xl) / p->nx;
+    h->dy = (p->yt - p->yb) / p->ny;
+    fill_quad_consts(h->qc);
+    size_t bytes = sizeof(double) * (size_t)h->fstride * h->nfield;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(HTX_EDEVICE, "hipStreamCreate failed");
+    }
+    if (hipMalloc(&h->dA, bytes) != hipSuccess || hipMalloc(&h->dB, bytes) != hipSuccess ||
+        hipMalloc(&h->dC, bytes) != hipSuccess || hipMalloc(&h->dR, bytes) != hipSuccess ||
+        hipMalloc(&h->ctl, sizeof(Ctl)) != hipSuccess) {
+        htx_destroy(h);
+        return fail(HTX_ENOMEM, "device allocation failed");
+    }
+    Ctl c{};
+    c.t = p->tinit;
+    c.tfinal = p->tmax;
+    c.cfl = p->cfl;
+    if (hipMemcpy(h->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess) {
+        htx_destroy(h);
+        return fail(HTX_EDEVICE, "control block upload failed");
+    }
+    *out = h;
+    return HTX_OK;
+}
+
+void htx_destroy(htx_solver* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto& e : h->events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    for (auto& e : h->event_pool) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    if (h->dA) (void)hipFree(h->dA);
+    if (h->dB) (void)hipFree(h->dB);
+    if (h->dC) (void)hipFree(h->dC);
+    if (h->dR) (void)hipFree(h->dR);
+    if (h->ctl) (void)hipFree(h->ctl);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int htx_set_state(htx_solver* h, const double* q, size_t ld) {
+    if (!h || !q) return fail(HTX_EINVAL, "null argument");
+    if (ld < (size_t)h->nx) return fail(HTX_EINVAL, "ld < nx");
+    HIP_TRY(hipMemcpy2DAsync(h->dA, sizeof(double) * h->pitch, q, sizeof(double) * ld, sizeof(double) * h->nx,
+                             (size_t)h->ny * h->nfield, hipMemcpyHostToDevice, h->stream));
+    int rc = refresh_cfl(h);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->have_state = true;
+    return HTX_OK;
+}
+
+int htx_get_state(htx_solver* h, double* q, size_t ld) {
+    if (!h || !q) return fail(HTX_EINVAL, "null argument");
+    if (ld < (size_t)h->nx) return fail(HTX_EINVAL, "ld < nx");
+    if (!h->have_state) return fail(HTX_ESTATE, "no state set");
+    HIP_TRY(hipMemcpy2DAsync(q, sizeof(double) * ld, h->dA, sizeof(double) * h->pitch, sizeof(double) * h->nx,
+                             (size_t)h->ny * h->nfield, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return HTX_OK;
+}
+
+int htx_init_vortex(htx_solver* h, double cx, double cy, const double* modes, int nmodes) {
+    if (!h) return fail(HTX_EINVAL, "null handle");
+    if (nmodes < 0 || nmodes > 16 || (nmodes > 0 && !modes)) return fail(HTX_EINVAL, "nmodes must be 0..16");
+    VortexArgs a{};
+    a.q = h->dA;
+    a.nx = h->nx;
+    a.ny = h->ny;
+    a.pitch = h->pitch;
+    a.fstride = h->fstride;
+    a.xl = h->p.xl;
+    a.yb = h->p.yb;
+    a.dx = h->dx;
+    a.dy = h->dy;
+    a.gam = h->p.gamma;
+    a.cx = cx;
+    a.cy = cy;
+    a.nmodes = nmodes;
+    for (int m = 0; m < nmodes; ++m)
+        for (int k = 0; k < 6; ++k) a.modes[m][k] = modes[6 * m + k];
+    gauss9(a.gx, a.gw);
+    HIP_TRY(launch_ic_vortex(a, h->stream));
+    int rc = refresh_cfl(h);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->have_state = true;
+    return HTX_OK;
+}
+
+int htx_rhs(htx_solver* h, double* rhs, size_t ld) {
+    if (!h || !rhs) return fail(HTX_EINVAL, "null argument");
+    if (ld < (size_t)h->nx) return fail(HTX_EINVAL, "ld < nx");
+    if (!h->have_state) return fail(HTX_ESTATE, "no state set");
+    int rc = launch_stage(h, h->dA, h->dA, h->dR, 0);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy2DAsync(rhs, sizeof(double) * ld, h->dR, sizeof(double) * h->pitch, sizeof(double) * h->nx,
+                             (size_t)h->ny * h->nfield, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return HTX_OK;
+}
+
+int htx_compute_dt(htx_solver* h, double* dt) {
+    if (!h || !dt) return fail(HTX_EINVAL, "null argument");
+    if (!h->have_state) return fail(HTX_ESTATE, "no state set");
+    Ctl c;
+    int rc = read_ctl(h, &c);
+    if (rc) return rc;
+    double sx = 0, sy = 0;
+    std::memcpy(&sx, &c.smax_x, sizeof(double));
+    std::memcpy(&sy, &c.smax_y, sizeof(double));
+    double d = c.cfl * std::fmin(h->dx, h->dy) / std::fmax(sx, sy);
+    *dt = (c.t + d > c.tfinal) ? c.tfinal - c.t : d;
+    return HTX_OK;
+}
+
+int htx_step(htx_solver* h, double* dt_taken) {
+    if (!h) return fail(HTX_EINVAL, "null handle");
+    if (!h->have_state) return fail(HTX_ESTATE, "no state set");
+    int rc = enqueue_step(h);
+    if (rc) return rc;
+    Ctl c;
+    if ((rc = read_ctl(h, &c))) return rc;
+    if (dt_taken) *dt_taken = c.active ? c.dt : 0.0;
+    return HTX_OK;
+}
+
+int htx_steps_async(htx_solver* h, int n) {
+    if (!h) return fail(HTX_EINVAL, "null handle");
+    if (!h->have_state) return fail(HTX_ESTATE, "no state set");
+    if (n < 0) return fail(HTX_EINVAL, "n < 0");
+    for (int k = 0; k < n; ++k) {
+        int rc = enqueue_step(h);
+        if (rc) return rc;
+    }
+    return HTX_OK;
+}
+
+int htx_advance(htx_solver* h, double tfinal, int* nsteps) {
+    if (!h) return fail(HTX_EINVAL, "null handle");
+    if (!h->have_state) return fail(HTX_ESTATE, "no state set");
+    Ctl c;
+    int rc = read_ctl(h, &c);
+    if (rc) return rc;
+    int n0 = c.nsteps;
+    c.tfinal = tfinal;
+    HIP_TRY(hipMemcpyAsync(&h->ctl->tfinal, &c.tfinal, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->p.tmax = tfinal;
+    // Steps past tfinal are skipped on the device, so batches may overshoot.
+    int batch = 4;
+    while (c.t < tfinal) {
+        for (int k = 0; k < batch; ++k)
+            if ((rc = enqueue_step(h))) return rc;
+        if ((rc = read_ctl(h, &c))) return rc;
+        if (batch < 64) batch *= 2;
+    }
+    if (nsteps) *nsteps = c.nsteps - n0;
+    return HTX_OK;
+}
+
+int htx_get_time(htx_solver* h, double* t, int* nsteps) {
+    if (!h) return fail(HTX_EINVAL, "null handle");
+    Ctl c;
+    int rc = read_ctl(h, &c);
+    if (rc) return rc;
+    if (t) *t = c.t;
+    if (nsteps) *nsteps = c.nsteps;
+    return HTX_OK;
+}
+
+int htx_sync(htx_solver* h) {
+    if (!h) return fail(HTX_EINVAL, "null handle");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return HTX_OK;
+}
+
+int htx_timing_enable(htx_solver* h, int enable) {
+    if (!h) return fail(HTX_EINVAL, "null handle");
+    h->timing = enable != 0;
+    return HTX_OK;
+}
+
+int htx_timing_read(htx_solver* h, double* kernel_ms, int* nlaunch) {
+    if (!h) return fail(HTX_EINVAL, "null handle");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    double total = 0.0;
+    for (auto& e : h->events) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+        total += ms;
+        h->event_pool.push_back(e);
+    }
+    if (kernel_ms) *kernel_ms = total;
+    if (nlaunch) *nlaunch = (int)h->events.size();
+    h->events.clear();
+    return HTX_OK;
+}
+
+}  // extern "C"

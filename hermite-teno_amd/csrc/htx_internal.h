@@ -1,0 +1,57 @@
+// htx_internal.h -- shared definitions between the HIP kernels and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "htx_numerics.h"
+
+namespace htx {
+
+// Device-resident time-stepping control block.  The host never needs to read
+// it between steps: dt comes from the max wave speed that the previous step's
+// last stage accumulated, and inactive steps (t >= tfinal) are skipped by the
+// stage kernels themselves.
+struct Ctl {
+    double t, dt, tfinal, cfl;
+    unsigned long long smax_x, smax_y;  // bit patterns of non-negative doubles
+    int active;
+    int nsteps;
+};
+
+enum FluxKind { FLUX_RUSANOV_CHAR = 0, FLUX_HLLC = 1, FLUX_HLLC_TORO = 2 };
+
+struct Fv2dArgs {
+    const double* qin;  // state whose residual is taken
+    const double* q0;   // u^n (stages 2 and 3)
+    double* qout;       // residual (mode 0) or the stage result
+    Ctl* ctl;
+    int nx, ny, pitch;  // pitch: elements between rows
+    long long fstride;  // elements between fields
+    int bc;             // 0 zero-gradient, 1 periodic
+    int mode;           // 0 residual only, 1..3 SSP-RK3 stage
+    int quirks;         // reference maxSpeed quirk in the CFL epilogue
+    int tiles_x, tiles_y;
+    double gam, g1, dx, dy, area;
+    QuadConsts qc;
+};
+
+struct VortexArgs {
+    double* q;
+    int nx, ny, pitch;
+    long long fstride;
+    double xl, yb, dx, dy, gam, cx, cy;
+    int nmodes;
+    double modes[16][6];  // (k, l, a, phi, Lx, Ly)
+    double gx[9], gw[9];
+};
+
+// launchers (htx_fv2d.hip)
+hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y, hipStream_t s);
+hipError_t launch_dt(Ctl* c, double dx, double dy, hipStream_t s);
+hipError_t launch_tadvance(Ctl* c, hipStream_t s);
+hipError_t launch_reset_smax(Ctl* c, hipStream_t s);
+hipError_t launch_max_speed(const double* q, int nx, int ny, int pitch, long long fstride, double gam,
+                            int quirks, Ctl* c, hipStream_t s);
+hipError_t launch_ic_vortex(const VortexArgs& a, hipStream_t s);
+void fill_quad_consts(QuadConsts& C);
+
+}  // namespace htx

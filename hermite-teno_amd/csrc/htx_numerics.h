@@ -1,0 +1,299 @@
+// htx_numerics.h -- device numerics of the 2D Euler FV path (gfx950).
+//
+// Reconstruction, transverse quadrature, Roe eigensystem and Riemann fluxes,
+// written for registers: no Eigen temporaries, no per-object scratch.  Each
+// routine follows one reference routine; the citation is at the definition.
+// Divisions by a common denominator are kept as divisions (IEEE, correctly
+// rounded) so the FP64 path matches the FP64 CPU restatement to rounding
+// of fused multiply-adds only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace htx {
+
+// Runtime-computed coefficients of Schemes::Lagrange that involve sqrt(15)
+// (Schemes_Lagrange.cpp:64-107).  Filled on the host with the reference's own
+// expressions and passed to kernels by value.
+struct QuadConsts {
+    double gp[3][3];   // crjgp3L(r, k)
+    double gm[3][3];   // crjgm3L(r, k)
+    double linwp[3], linwm[3], linw0p[3], linw0m[3];
+    double sumw0p, sumw0m;
+};
+
+struct Prim {  // per-cell quantities used by the Roe average
+    double sr, u, v, h;
+};
+
+// Schemes::Lagrange::teno5, one side (Schemes_Lagrange.cpp:481-511 with
+// smoothnessMeasure :1013-1026, fullpointsSmootness :1029-1034, cutOff
+// :1141-1149, catchnan :1151-1153).  v0..v4 = 5-point window, p = the three
+// candidate values, d = linear weights.  Returns the TENO value; *cut gets a
+// bitmask of discarded stencils.
+__device__ __forceinline__ double teno5_side(double v0, double v1, double v2, double v3, double v4,
+                                             double p0, double p1, double p2, double d0, double d1,
+                                             double d2, unsigned* cut) {
+    const double c1 = 13.0 / 12.0;
+    double c_0 = v2 - 2.0 * v3 + v4, b_0 = (3.0 * v2 - 4.0 * v3 + v4) / 2.0;
+    double c_1 = v1 - 2.0 * v2 + v3, b_1 = (v1 - v3) / 2.0;
+    double c_2 = v0 - 2.0 * v1 + v2, b_2 = (v0 - 4.0 * v1 + 3.0 * v2) / 2.0;
+    double s0 = b_0 * b_0 + c1 * c_0 * c_0;
+    double s1 = b_1 * b_1 + c1 * c_1 * c_1;
+    double s2 = b_2 * b_2 + c1 * c_2 * c_2;
+    double b35 = v4 * (6908.0 * v4 - 51001.0 * v3 + 67923.0 * v2 - 38947.0 * v1 + 8209.0 * v0) +
+                 v3 * (104963.0 * v3 - 299076.0 * v2 + 179098.0 * v1 - 38947.0 * v0) +
+                 v2 * (231153.0 * v2 - 299076.0 * v1 + 67923.0 * v0) +
+                 v1 * (104963.0 * v1 - 51001.0 * v0) + 6908.0 * v0 * v0;
+    b35 = b35 / 5040.0;
+    double tau = fabs(b35 - (s0 + s2 + 4.0 * s1) / 6.0);
+    double g0 = 1.0 + tau / (s0 + 1e-40);
+    double g1 = 1.0 + tau / (s1 + 1e-40);
+    double g2 = 1.0 + tau / (s2 + 1e-40);
+    g0 = g0 * g0 * g0; g0 = g0 * g0;
+    g1 = g1 * g1 * g1; g1 = g1 * g1;
+    g2 = g2 * g2 * g2; g2 = g2 * g2;
+    double gs = (g0 + g1) + g2;
+    // cutOff: w/sum < C_T -> discard.  NaN (inf/inf) compares false -> kept.
+    bool k0 = !(g0 / gs < 5e-4), k1 = !(g1 / gs < 5e-4), k2 = !(g2 / gs < 5e-4);
+    double w0 = k0 ? d0 : 0.0, w1 = k1 ? d1 : 0.0, w2 = k2 ? d2 : 0.0;
+    double ws = (w0 + w1) + w2;
+    w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
+    double out = (p0 * w0 + p1 * w1) + p2 * w2;
+    if (cut) *cut = (k0 ? 0u : 1u) | (k1 ? 0u : 2u) | (k2 ? 0u : 4u);
+    return isnan(out) ? 1e-15 : out;
+}
+
+// Schemes::Lagrange::teno5 (Schemes_Lagrange.cpp:481-511): u[0..5] =
+// u_{i-2..i+3} -> (u^-_{i+1/2}, u^+_{i+1/2}).
+__device__ __forceinline__ void teno5(const double u[6], double& um, double& up, unsigned* cut = nullptr) {
+    const double a = 1.0 / 3.0, b = 5.0 / 6.0, c = -1.0 / 6.0, d = 11.0 / 6.0, e = -7.0 / 6.0;
+    // crj columns: col0 = (d, e, a), col1 = (a, b, c), col2 = (c, b, a), col3 = (a, e, d)
+    double p0 = (a * u[2] + b * u[3]) + c * u[4];
+    double p1 = (c * u[1] + b * u[2]) + a * u[3];
+    double p2 = (a * u[0] + e * u[1]) + d * u[2];
+    unsigned cm = 0, cp = 0;
+    um = teno5_side(u[0], u[1], u[2], u[3], u[4], p0, p1, p2, 0.3, 0.6, 0.1, cut ? &cm : nullptr);
+    double q0 = (d * u[3] + e * u[4]) + a * u[5];
+    double q1 = (a * u[2] + b * u[3]) + c * u[4];
+    double q2 = (c * u[1] + b * u[2]) + a * u[3];
+    up = teno5_side(u[1], u[2], u[3], u[4], u[5], q0, q1, q2, 0.1, 0.6, 0.3, cut ? &cp : nullptr);
+    if (cut) *cut = cm | (cp << 3);
+}
+
+// Schemes::Lagrange::weno5JS3PointsQuadrature + smoothnessMeasure3L
+// (Schemes_Lagrange.cpp:1155-1217): 5 transverse values -> values at the
+// Gauss points (+xi, 0, -xi).
+__device__ __forceinline__ void weno3q(const QuadConsts& C, double u0, double u1, double u2, double u3,
+                                       double u4, double& o0, double& o1, double& o2) {
+    const double c1 = 13.0 / 12.0;
+    double bg0 = u0 - 4.0 * u1 + 3.0 * u2, cg0 = u0 - 2.0 * u1 + u2;
+    double bg1 = u1 - u3, cg1 = u1 - 2.0 * u2 + u3;
+    double bg2 = 3.0 * u2 - 4.0 * u3 + u4, cg2 = u2 - 2.0 * u3 + u4;
+    double s0 = 0.25 * (bg0 * bg0) + c1 * (cg0 * cg0);
+    double s1 = 0.25 * (bg1 * bg1) + c1 * (cg1 * cg1);
+    double s2 = 0.25 * (bg2 * bg2) + c1 * (cg2 * cg2);
+    double e0 = 1e-6 + s0, e1 = 1e-6 + s1, e2 = 1e-6 + s2;
+    e0 = e0 * e0; e1 = e1 * e1; e2 = e2 * e2;
+    double w0, w1, w2, ws, p0, p1, p2;
+    // +xi
+    p0 = (C.gp[0][0] * u0 + C.gp[1][0] * u1) + C.gp[2][0] * u2;
+    p1 = (C.gp[0][1] * u1 + C.gp[1][1] * u2) + C.gp[2][1] * u3;
+    p2 = (C.gp[0][2] * u2 + C.gp[1][2] * u3) + C.gp[2][2] * u4;
+    w0 = C.linwp[0] / e0; w1 = C.linwp[1] / e1; w2 = C.linwp[2] / e2;
+    ws = (w0 + w1) + w2;
+    w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
+    o0 = (p0 * w0 + p1 * w1) + p2 * w2;
+    // centre, positive/negative split
+    p0 = (-u0 + 2.0 * u1 + 23.0 * u2) / 24.0;
+    p1 = (-u1 + 26.0 * u2 - u3) / 24.0;
+    p2 = (23.0 * u2 + 2.0 * u3 - u4) / 24.0;
+    w0 = C.linw0p[0] / e0; w1 = C.linw0p[1] / e1; w2 = C.linw0p[2] / e2;
+    ws = (w0 + w1) + w2;
+    w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
+    o1 = C.sumw0p * ((p0 * w0 + p1 * w1) + p2 * w2);
+    w0 = C.linw0m[0] / e0; w1 = C.linw0m[1] / e1; w2 = C.linw0m[2] / e2;
+    ws = (w0 + w1) + w2;
+    w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
+    o1 -= C.sumw0m * ((p0 * w0 + p1 * w1) + p2 * w2);
+    // -xi
+    p0 = (C.gm[0][0] * u0 + C.gm[1][0] * u1) + C.gm[2][0] * u2;
+    p1 = (C.gm[0][1] * u1 + C.gm[1][1] * u2) + C.gm[2][1] * u3;
+    p2 = (C.gm[0][2] * u2 + C.gm[1][2] * u3) + C.gm[2][2] * u4;
+    w0 = C.linwm[0] / e0; w1 = C.linwm[1] / e1; w2 = C.linwm[2] / e2;
+    ws = (w0 + w1) + w2;
+    w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
+    o2 = (p0 * w0 + p1 * w1) + p2 * w2;
+    o0 = isnan(o0) ? 1e-15 : o0;
+    o1 = isnan(o1) ? 1e-15 : o1;
+    o2 = isnan(o2) ? 1e-15 : o2;
+}
+
+// Per-cell quantities of calcPrimitiveVariable (FV_System2D.cpp:226-263).
+__device__ __forceinline__ Prim cell_prim(double g1, double q0, double q1, double q2, double q3) {
+    Prim P;
+    double u = q1 / q0, v = q2 / q0;
+    double p = (q3 - 0.5 * (q1 * u + q2 * v)) * g1;
+    P.sr = sqrt(fabs(q0));
+    P.u = u;
+    P.v = v;
+    P.h = (q3 + p) / q0;
+    return P;
+}
+
+// Roe-averaged eigensystem of one face (FV_System2D.cpp:271-298 / :332-358,
+// Equations_Euler2D.cpp:100-205), kept as the 9 scalars the 4x4 matrices
+// are built from.
+struct Eig {
+    double vn, vt, hm, qm, cm, rcm, b1, b2, t2;  // vn: normal, vt: tangential velocity
+};
+
+__device__ __forceinline__ Eig roe_eig(double g1, bool ydir, const Prim& a, const Prim& b) {
+    Eig E;
+    double t0 = a.sr / (a.sr + b.sr);
+    double t1 = 1.0 - t0;
+    double vxm = t0 * a.u + t1 * b.u;
+    double vym = t0 * a.v + t1 * b.v;
+    E.hm = t0 * a.h + t1 * b.h;
+    E.qm = 0.5 * (vxm * vxm + vym * vym);
+    E.cm = sqrt(fabs(g1 * (E.hm - E.qm)));
+    E.rcm = 1.0 / E.cm;
+    E.b1 = g1 * E.rcm * E.rcm;
+    E.b2 = E.qm * E.b1;
+    E.t2 = 0.5 * E.b1;
+    E.vn = ydir ? vym : vxm;
+    E.vt = ydir ? vxm : vym;
+    return E;
+}
+
+// Characteristic projection L*q.  Components are ordered (rho, normal
+// momentum, tangential momentum, E) via (qn, qt); the x and y left
+// eigenvector matrices are the same matrix in that ordering.  Products with
+// the matrices' exact 0 and 1 entries are not formed.
+__device__ __forceinline__ void eig_left(const Eig& E, double q0, double qn, double qt, double q3,
+                                         double& c0, double& c1, double& c2, double& c3) {
+    double tr = E.vn * E.rcm, tb = E.b1 * E.vn;
+    c0 = ((0.5 * (E.b2 + tr) * q0 + (-0.5 * (tb + E.rcm)) * qn) + (-E.t2 * E.vt) * qt) + E.t2 * q3;
+    c1 = -E.vt * q0 + qt;
+    c2 = (((1.0 - E.b2) * q0 + tb * qn) + (E.b1 * E.vt) * qt) + (-E.b1) * q3;
+    c3 = ((0.5 * (E.b2 - tr) * q0 + (-0.5 * (tb - E.rcm)) * qn) + (-E.t2 * E.vt) * qt) + E.t2 * q3;
+}
+
+// Back-projection R*c, returned in (rho, normal, tangential, E) ordering.
+__device__ __forceinline__ void eig_right(const Eig& E, double c0, double c1, double c2, double c3,
+                                          double& q0, double& qn, double& qt, double& q3) {
+    double tc = E.vn * E.cm;
+    q0 = (c0 + c2) + c3;
+    qn = ((E.vn - E.cm) * c0 + E.vn * c2) + (E.vn + E.cm) * c3;
+    qt = ((E.vt * c0 + c1) + E.vt * c2) + E.vt * c3;
+    q3 = (((E.hm - tc) * c0 + E.vt * c1) + E.qm * c2) + (E.hm + tc) * c3;
+}
+
+// Physical normal flux (Equations_Euler2D.cpp:36-50) in (rho, n, t, E) order.
+__device__ __forceinline__ void phys_flux(double g1, double q0, double qn, double qt, double q3,
+                                          double& f0, double& fn, double& ft, double& f3, double& wn,
+                                          double& p, double& rho) {
+    rho = q0;
+    wn = qn / q0;
+    double wt = qt / q0;
+    p = (q3 - 0.5 * (qn * wn + qt * wt)) * g1;
+    f0 = qn;
+    fn = qn * wn + p;
+    ft = qn * wt;
+    f3 = wn * (q3 + p);
+}
+
+// maxLocalWaveSpeed (Equations_Euler2D.cpp:221-246) + characteristic Rusanov
+// (:310-330), all in (rho, n, t, E) ordering.
+__device__ __forceinline__ void rusanov_char(double gam, double g1, const Eig& E, const double ql[4],
+                                             const double qr[4], double f[4]) {
+    double fl0, fln, flt, fl3, wnl, pl, rl;
+    double fr0, frn, frt, fr3, wnr, pr, rr;
+    phys_flux(g1, ql[0], ql[1], ql[2], ql[3], fl0, fln, flt, fl3, wnl, pl, rl);
+    phys_flux(g1, qr[0], qr[1], qr[2], qr[3], fr0, frn, frt, fr3, wnr, pr, rr);
+    double cl = sqrt(fabs(gam * pl / rl));
+    double cr = sqrt(fabs(gam * pr / rr));
+    double a0 = fmax(fabs(wnl - cl), fabs(wnr - cr));
+    double a1 = fmax(fabs(wnl), fabs(wnr));
+    double a3 = fmax(fabs(wnl + cl), fabs(wnr + cr));
+    double d0, d1, d2, d3;
+    eig_left(E, qr[0] - ql[0], qr[1] - ql[1], qr[2] - ql[2], qr[3] - ql[3], d0, d1, d2, d3);
+    d0 *= a0; d1 *= a1; d2 *= a1; d3 *= a3;
+    double r0, rn, rt, r3;
+    eig_right(E, d0, d1, d2, d3, r0, rn, rt, r3);
+    f[0] = 0.5 * ((fl0 + fr0) - r0);
+    f[1] = 0.5 * ((fln + frn) - rn);
+    f[2] = 0.5 * ((flt + frt) - rt);
+    f[3] = 0.5 * ((fl3 + fr3) - r3);
+}
+
+// Equations::Euler2D::HLLC (Equations_Euler2D.cpp:372-428) in (rho, n, t, E)
+// ordering.  QUIRKS keeps c = sqrt(|p/rho|) (:393-394) and p_L in the right
+// star energy (:423).
+template <bool QUIRKS>
+__device__ __forceinline__ void hllc(double gam, double g1, const double ql[4], const double qr[4],
+                                     double f[4]) {
+    double wl0 = ql[0], wln = ql[1] / ql[0], wlt = ql[2] / ql[0];
+    double wr0 = qr[0], wrn = qr[1] / qr[0], wrt = qr[2] / qr[0];
+    double pl = (ql[3] - 0.5 * (ql[1] * wln + ql[2] * wlt)) * g1;
+    double pr = (qr[3] - 0.5 * (qr[1] * wrn + qr[2] * wrt)) * g1;
+    double hl = (ql[3] + pl) / wl0;
+    double hr = (qr[3] + pr) / wr0;
+    double rho_ratio = sqrt(fabs(wr0 / wl0));
+    double rrr = 1.0 / (1.0 + rho_ratio);
+    double vn = (wln + wrn * rho_ratio) * rrr;
+    double vt = (wlt + wrt * rho_ratio) * rrr;
+    double qm = 0.5 * (vn * vn + vt * vt);
+    double hm = (hl + hr * rho_ratio) * rrr;
+    double cm = sqrt(fabs(g1 * (hm - qm)));
+    double cl = QUIRKS ? sqrt(fabs(pl / wl0)) : sqrt(fabs(gam * pl / wl0));
+    double cr = QUIRKS ? sqrt(fabs(pr / wr0)) : sqrt(fabs(gam * pr / wr0));
+    double sl = fmin(wln - cl, vn - cm);
+    double sr = fmax(wrn + cr, vn + cm);
+    double sm = (qr[1] * (sr - wrn) - ql[1] * (sl - wln) + pl - pr) /
+                (wr0 * (sr - wrn) - wl0 * (sl - wln));
+    double ps = pl + wl0 * (wln - sl) * (wln - sm);
+    if (sl > 0.0) {
+        f[0] = ql[1];
+        f[1] = ql[1] * wln + pl;
+        f[2] = ql[1] * wlt;
+        f[3] = wln * (ql[3] + pl);
+    } else if (sl <= 0.0 && sm > 0.0) {
+        double nom = sl - wln, den = sl - sm;
+        double s0 = wl0 * nom / den;
+        double sn = (nom * ql[1] + ps - pl) / den;
+        double st = nom * ql[2] / den;
+        double s3 = (nom * ql[3] - pl * wln + ps * sm) / den;
+        f[0] = ql[1] + sl * (s0 - ql[0]);
+        f[1] = (ql[1] * wln + pl) + sl * (sn - ql[1]);
+        f[2] = ql[1] * wlt + sl * (st - ql[2]);
+        f[3] = wln * (ql[3] + pl) + sl * (s3 - ql[3]);
+    } else if (sm <= 0.0 && sr >= 0.0) {
+        double nom = sr - wrn, den = sr - sm;
+        double pe = QUIRKS ? pl : pr;
+        double s0 = wr0 * nom / den;
+        double sn = (nom * qr[1] + ps - pr) / den;
+        double st = nom * qr[2] / den;
+        double s3 = (nom * qr[3] - pe * wrn + ps * sm) / den;
+        f[0] = qr[1] + sr * (s0 - qr[0]);
+        f[1] = (qr[1] * wrn + pr) + sr * (sn - qr[1]);
+        f[2] = qr[1] * wrt + sr * (st - qr[2]);
+        f[3] = wrn * (qr[3] + pr) + sr * (s3 - qr[3]);
+    } else {
+        f[0] = qr[1];
+        f[1] = qr[1] * wrn + pr;
+        f[2] = qr[1] * wrt;
+        f[3] = wrn * (qr[3] + pr);
+    }
+}
+
+// Euler2D::maxSpeed (Equations_Euler2D.cpp:207-219); QUIRKS drops the fabs
+// on u - c (:218).  Negative/NaN results are clamped to 0, which is what the
+// reference's fmax(0, .) accumulation does with them.
+template <bool QUIRKS>
+__device__ __forceinline__ double max_speed(double gam, double wn, double p, double rho) {
+    double c = sqrt(fabs(gam * p / rho));
+    double s = QUIRKS ? fmax(wn - c, fabs(wn + c)) : fmax(fabs(wn - c), fabs(wn + c));
+    return s > 0.0 ? s : 0.0;
+}
+
+}  // namespace htx

@@ -1,0 +1,109 @@
+"""ctypes binding of libhtx.so (include/htx.h).
+
+The library is loaded from this package directory only.  If it is missing the
+import fails loudly: there is no CPU fallback for the product path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhtx.so")
+
+HTX_OK = 0
+ERRORS = {-1: "HTX_EINVAL", -2: "HTX_EDEVICE", -3: "HTX_ENOMEM", -4: "HTX_ESTATE", -5: "HTX_ENOTSUP"}
+
+SCHEMES = {"teno5": 0, "hteno": 1}
+FLUXES = {"rusanov": 0, "rusanov_char": 0, "hllc": 1}
+BCS = {"zero_grad": 0, "zerograd": 0, "periodic": 1}
+PRECISIONS = {"fp64": 0, "fp32_mixed": 1, "learned": 2}
+
+
+class HtxParams(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_int),
+        ("dim", ctypes.c_int),
+        ("xl", ctypes.c_double),
+        ("xr", ctypes.c_double),
+        ("nx", ctypes.c_int),
+        ("yb", ctypes.c_double),
+        ("yt", ctypes.c_double),
+        ("ny", ctypes.c_int),
+        ("cfl", ctypes.c_double),
+        ("tmax", ctypes.c_double),
+        ("tinit", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("scheme", ctypes.c_int),
+        ("flux", ctypes.c_int),
+        ("bc", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("ref_quirks", ctypes.c_int),
+        ("tile_x", ctypes.c_int),
+        ("tile_y", ctypes.c_int),
+    ]
+
+
+class HtxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+_S = ctypes.c_size_t
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+
+# name -> (restype, argtypes); every symbol include/htx.h declares
+SIGNATURES = {
+    "htx_params_default": (_I, [ctypes.POINTER(HtxParams)]),
+    "htx_create": (_I, [ctypes.POINTER(HtxParams), ctypes.POINTER(_P)]),
+    "htx_destroy": (None, [_P]),
+    "htx_set_state": (_I, [_P, _DP, _S]),
+    "htx_get_state": (_I, [_P, _DP, _S]),
+    "htx_init_vortex": (_I, [_P, _D, _D, _DP, _I]),
+    "htx_rhs": (_I, [_P, _DP, _S]),
+    "htx_compute_dt": (_I, [_P, _DP]),
+    "htx_step": (_I, [_P, _DP]),
+    "htx_steps_async": (_I, [_P, _I]),
+    "htx_advance": (_I, [_P, _D, _IP]),
+    "htx_get_time": (_I, [_P, _DP, _IP]),
+    "htx_sync": (_I, [_P]),
+    "htx_timing_enable": (_I, [_P, _I]),
+    "htx_timing_read": (_I, [_P, _DP, _IP]),
+    "htx_last_error": (ctypes.c_char_p, []),
+    "htx_abi_version": (_I, []),
+}
+
+_lib = None
+
+
+def load():
+    """Load libhtx.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != HTX_OK:
+        msg = load().htx_last_error().decode(errors="replace")
+        raise HtxError(rc, msg)
+    return rc
+
+
+def dptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_DP)

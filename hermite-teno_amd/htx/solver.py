@@ -1,0 +1,155 @@
+"""Host-side mirror of the reference's 2D solver interface.
+
+``System2D`` keeps the constructor and method names of the reference's
+``FV::System2D`` (FV_System2D.h:12-20) and ``FVHermite::System2D``
+(FVHermite_System2D.h:16-24); the numerics run in libhtx.so on the GPU.
+Method choices that the reference makes by commenting code in or out
+(flux, boundary condition, scheme) are constructor keywords here.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr
+
+
+class System2D:
+    """FV::System2D(xl, xr, nx, yb, yt, ny, cflnumber, tmax, tInit, specificHeatRatio)."""
+
+    def __init__(self, xl, xr, nx, yb, yt, ny, cflnumber, tmax, tInit, specificHeatRatio, *,
+                 scheme="teno5", flux="hllc", bc="zero_grad", precision="fp64", ref_quirks=True,
+                 tile=(0, 0)):
+        lib = _lib.load()
+        p = _lib.HtxParams()
+        check(lib.htx_params_default(ctypes.byref(p)))
+        p.xl, p.xr, p.nx = float(xl), float(xr), int(nx)
+        p.yb, p.yt, p.ny = float(yb), float(yt), int(ny)
+        p.cfl, p.tmax, p.tinit, p.gamma = float(cflnumber), float(tmax), float(tInit), float(specificHeatRatio)
+        p.scheme = _lib.SCHEMES[scheme]
+        p.flux = _lib.FLUXES[flux]
+        p.bc = _lib.BCS[bc]
+        p.precision = _lib.PRECISIONS[precision]
+        p.ref_quirks = 1 if ref_quirks else 0
+        p.tile_x, p.tile_y = tile
+        self.params = p
+        self._h = None
+        h = ctypes.c_void_p()
+        check(lib.htx_create(ctypes.byref(p), ctypes.byref(h)))
+        self._h = h
+        self.nx, self.ny = int(nx), int(ny)
+        self.nfield = 12 if scheme == "hteno" else 4
+        self.dx = (p.xr - p.xl) / p.nx
+        self.dy = (p.yt - p.yb) / p.ny
+        self.gamma = p.gamma
+        self.tFinal = p.tmax
+        self.exact = None
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().htx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- geometry ---------------------------------------------------------
+    def cell_centres(self):
+        """Cell centres as FV_System2D.cpp:37-38 (interior cells)."""
+        p = self.params
+        x = (np.arange(self.nx, dtype=np.float64) + 0.5) * self.dx + p.xl
+        y = (np.arange(self.ny, dtype=np.float64) + 0.5) * self.dy + p.yb
+        return x, y
+
+    # -- state ------------------------------------------------------------
+    def set_state(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(self.nfield, self.ny, self.nx)
+        check(_lib.load().htx_set_state(self._h, dptr(q), self.nx))
+
+    def get_state(self):
+        q = np.empty((self.nfield, self.ny, self.nx), dtype=np.float64)
+        check(_lib.load().htx_get_state(self._h, dptr(q), self.nx))
+        return q
+
+    # -- initial conditions (reference method names) ----------------------
+    def isentropicVortex(self, centre=(0.0, 0.0), modes=None):
+        """FV::System2D::isentropicVortex (FV_System2D.cpp:76-99).  ``centre``
+        moves the vortex off the origin; ``modes`` is an (m, 6) array of
+        density modes (k, l, amplitude, phase, Lx, Ly)."""
+        m = np.zeros((0, 6)) if modes is None else np.ascontiguousarray(modes, dtype=np.float64)
+        mm = np.ascontiguousarray(m.reshape(-1), dtype=np.float64)
+        ptr = dptr(mm) if mm.size else None
+        check(_lib.load().htx_init_vortex(self._h, float(centre[0]), float(centre[1]), ptr, int(m.shape[0])))
+
+    # -- the hot path -----------------------------------------------------
+    def rhs(self):
+        """One residual evaluation L(q) (FV::System2D::rhs)."""
+        r = np.empty((self.nfield, self.ny, self.nx), dtype=np.float64)
+        check(_lib.load().htx_rhs(self._h, dptr(r), self.nx))
+        return r
+
+    def compute_dt(self):
+        d = ctypes.c_double()
+        check(_lib.load().htx_compute_dt(self._h, ctypes.byref(d)))
+        return d.value
+
+    def step(self):
+        """One SSP-RK3 step (FV::System2D::RungeKutta3); returns dt taken."""
+        d = ctypes.c_double()
+        check(_lib.load().htx_step(self._h, ctypes.byref(d)))
+        return d.value
+
+    def steps_async(self, n):
+        check(_lib.load().htx_steps_async(self._h, int(n)))
+
+    def sync(self):
+        check(_lib.load().htx_sync(self._h))
+
+    def timeStepping(self, tfinal=None):
+        """FV::System2D::timeStepping: advance to tFinal; returns steps taken."""
+        n = ctypes.c_int()
+        t = self.tFinal if tfinal is None else float(tfinal)
+        check(_lib.load().htx_advance(self._h, t, ctypes.byref(n)))
+        self.tFinal = t
+        return n.value
+
+    @property
+    def time(self):
+        t = ctypes.c_double()
+        n = ctypes.c_int()
+        check(_lib.load().htx_get_time(self._h, ctypes.byref(t), ctypes.byref(n)))
+        return t.value
+
+    @property
+    def nsteps(self):
+        t = ctypes.c_double()
+        n = ctypes.c_int()
+        check(_lib.load().htx_get_time(self._h, ctypes.byref(t), ctypes.byref(n)))
+        return n.value
+
+    def timing_enable(self, on=True):
+        check(_lib.load().htx_timing_enable(self._h, 1 if on else 0))
+
+    def timing_read(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int()
+        check(_lib.load().htx_timing_read(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def printResult(self):
+        """FV::System2D::printResult (FV_System2D.cpp:653-666): mean |rho -
+        rho_exact| over interior cells, when an exact field has been set."""
+        if self.exact is None:
+            return None
+        q = self.get_state()
+        return float(np.abs(q[0] - self.exact[0]).sum() / (self.nx * self.ny))

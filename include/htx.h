@@ -1,0 +1,134 @@
+/*
+ * htx.h -- C ABI of the MI355X-native Hermite-TENO finite-volume residual /
+ * time-step path (libhtx.so).
+ *
+ * The reference (indrow/Hermite-TENO) has no FFI: its solver classes are
+ * driven from C++ (main.cpp).  This ABI is what those classes bind to: the
+ * C++ facade headers in this directory (FV_System2D.h, FVHermite_System2D.h)
+ * implement the reference's class interface over these calls, and Python
+ * binds them with ctypes.  Each entry point names the reference member it
+ * replaces.
+ *
+ * Conventions
+ *   - Return 0 on success, a negative HTX_E* code on failure; the message is
+ *     available from htx_last_error() (per host thread).  Nothing exits the
+ *     process and no C++ exception crosses this boundary.
+ *   - A handle owns its device buffers, its HIP stream and its control block.
+ *     One handle per host thread; handles are not internally synchronised.
+ *   - Host state arrays are field-major, interior cells only:
+ *     q[(f * ny + j) * ld + i], f over the conserved fields (rho, rho u,
+ *     rho v, E), then for the Hermite scheme the x- and y-derivative moments
+ *     of the same four fields (12 fields in all).  ld >= nx.
+ */
+#ifndef HTX_H
+#define HTX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HTX_ABI_VERSION 1
+
+enum htx_error {
+    HTX_OK = 0,
+    HTX_EINVAL = -1,   /* bad argument (grid size 0, unknown enum, null pointer) */
+    HTX_EDEVICE = -2,  /* HIP runtime error, or no device */
+    HTX_ENOMEM = -3,   /* device allocation failed */
+    HTX_ESTATE = -4,   /* call out of order (no state set) */
+    HTX_ENOTSUP = -5   /* combination not built yet */
+};
+
+enum htx_scheme { HTX_SCHEME_TENO5 = 0, HTX_SCHEME_HTENO = 1 };
+enum htx_flux { HTX_FLUX_RUSANOV_CHAR = 0, HTX_FLUX_HLLC = 1 };
+enum htx_bc { HTX_BC_ZERO_GRAD = 0, HTX_BC_PERIODIC = 1 };
+enum htx_precision { HTX_PREC_FP64 = 0, HTX_PREC_FP32_MIXED = 1, HTX_PREC_LEARNED = 2 };
+
+typedef struct htx_params {
+    int struct_size;    /* sizeof(htx_params), for ABI checks */
+    int dim;            /* 2 */
+    double xl, xr;      /* ctor (xl, xr, nx, ...), FV_System2D.h:12-15 */
+    int nx;
+    double yb, yt;
+    int ny;
+    double cfl, tmax, tinit, gamma;  /* cflnumber, tmax, tInit, specificHeatRatio */
+    int scheme;         /* htx_scheme: FV::System2D / FVHermite::System2D */
+    int flux;           /* htx_flux: FV_System2D.cpp:462-466 (Rusanov) / :465-466 (HLLC) */
+    int bc;             /* htx_bc: FV_System2D.cpp:365-366 */
+    int precision;      /* htx_precision */
+    int ref_quirks;     /* 1: reproduce the reference's HLLC and maxSpeed quirks */
+    int tile_x, tile_y; /* 0: default tile shape */
+} htx_params;
+
+typedef struct htx_solver htx_solver;
+
+/* Fill p with the defaults of a 2D TENO5/HLLC FP64 solver (quirks on). */
+int htx_params_default(htx_params* p);
+
+/* FV::System2D::System2D / FVHermite::System2D::System2D (FV_System2D.cpp:3-47,
+ * FVHermite_System2D.cpp:3-50): validates the parameters (a zero grid size is
+ * HTX_EINVAL where the reference calls exit(1), Schemes_Hermite.cpp:16-25),
+ * allocates the device state and sets t = tinit. */
+int htx_create(const htx_params* p, htx_solver** out);
+
+/* Releases every device resource of the handle.  Null is ignored. */
+void htx_destroy(htx_solver* h);
+
+/* Replaces the cell(i,j).u assignment of the reference's initial-condition
+ * methods: copies host state (layout above) to the device.  Also resets the
+ * CFL accumulator from the new state. */
+int htx_set_state(htx_solver* h, const double* q, size_t ld);
+
+/* Copies the device state to host memory (the reference's cell(i,j).u). */
+int htx_get_state(htx_solver* h, double* q, size_t ld);
+
+/* FV::System2D::isentropicVortex (FV_System2D.cpp:76-99 with
+ * Equations_Euler2D.cpp:513-528, 656-666), generated on the device: vortex
+ * of strength 5 centred at (cx, cy) plus nmodes density modes, each 6
+ * doubles (k, l, amplitude, phase, Lx, Ly) adding
+ * amplitude*sin(2 pi (k x / Lx + l y / Ly) + phase) at fixed u, v, p.
+ * nmodes <= 16. */
+int htx_init_vortex(htx_solver* h, double cx, double cy, const double* modes, int nmodes);
+
+/* One residual evaluation L(q) of the current state, FV::System2D::rhs
+ * (FV_System2D.cpp:363-573); written to host memory in the state layout. */
+int htx_rhs(htx_solver* h, double* rhs, size_t ld);
+
+/* The dt that the next step would take (FV_System2D.cpp:623-645). */
+int htx_compute_dt(htx_solver* h, double* dt);
+
+/* One SSP-RK3 step, FV::System2D::RungeKutta3 (:575-612) with the dt of
+ * timeStepping (:622-648).  Blocks; returns the dt taken (0 if t >= tmax). */
+int htx_step(htx_solver* h, double* dt_taken);
+
+/* Enqueues n steps without blocking the host; steps past tmax are no-ops. */
+int htx_steps_async(htx_solver* h, int n);
+
+/* FV::System2D::timeStepping (:614-651): advance until t >= tfinal (tfinal
+ * replaces tmax).  *nsteps receives the number of steps taken (may be null). */
+int htx_advance(htx_solver* h, double tfinal, int* nsteps);
+
+/* Simulation time and number of steps taken so far (blocks). */
+int htx_get_time(htx_solver* h, double* t, int* nsteps);
+
+/* Waits for all work enqueued on the handle's stream. */
+int htx_sync(htx_solver* h);
+
+/* Stage-kernel timing with HIP events on the handle's stream.  While
+ * enabled, every stage-kernel launch is bracketed by an event pair;
+ * htx_timing_read blocks, returns the summed kernel milliseconds and the
+ * number of launches since the last read, and clears them. */
+int htx_timing_enable(htx_solver* h, int enable);
+int htx_timing_read(htx_solver* h, double* kernel_ms, int* nlaunch);
+
+/* Message of the last failed call on this host thread. */
+const char* htx_last_error(void);
+
+/* HTX_ABI_VERSION of the loaded library. */
+int htx_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HTX_H */

@@ -1,0 +1,770 @@
+/*
+ * htx_oracle.c -- FP64 CPU restatement of the Hermite-TENO FV residual /
+ * time-step path (reference: indrow/Hermite-TENO, C++/Eigen, CPU only).
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the checker for the HIP product path.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * it; the product library never links or calls it.
+ *
+ * The reference cannot be compiled here (it needs Eigen3, absent offline), so
+ * this file restates it operation by operation from the source text.  Each
+ * function cites the reference lines it follows.  Evaluation order follows the
+ * source text left to right (Eigen's own dot/mat-vec order is not pinned by
+ * any reference test); the parity build uses -ffp-contract=off.
+ *
+ * Pinning (tests/test_oracle.py): closed-form known answers (polynomial
+ * exactness of teno5/linear/quadrature, Gauss-9 integrals), invariants
+ * (L.R = I, R.diag(lambda).L = dF/dq, flux consistency F(q,q) = f(q),
+ * Toro HLLC on exact Riemann data with quirks off, conservation with periodic
+ * boundaries, free-stream preservation, mirror and x<->y transpose symmetry,
+ * SSP-RK3 amplification polynomial) and brute-force CFL on tiny grids.
+ */
+#include "htx_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Constants of Schemes::Lagrange (Schemes_Lagrange.cpp:6-142)               */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double crj[3][4];                 /* crj(r,c)            :125-136 */
+    double dkm[3], dkp[3];            /* 0.3,0.6,0.1 / mirror :140-142 */
+    double crjgp3L[3][3], crjgm3L[3][3]; /* :66-84 */
+    double linwp[3], linwm[3];        /* :95-105 */
+    double linw0p[3], linw0m[3];      /* :86-93 */
+    double sumw0p, sumw0m;
+    double c1, eps, epsz, ct, machine_eps;
+} lag_consts;
+
+static lag_consts LC;
+static int LC_ready = 0;
+
+static void lag_init(void) {
+    if (LC_ready) return;
+    double sq15 = sqrt(15.0);
+    LC.c1 = 13.0 / 12.0;
+    LC.eps = 1E-6;
+    LC.epsz = 1E-40;
+    LC.ct = 5E-4;
+    LC.machine_eps = 1E-15;
+
+    /* crj: Schemes_Lagrange.cpp:125-136 */
+    LC.crj[0][0] = 11.0 / 6.0; LC.crj[1][0] = -7.0 / 6.0; LC.crj[2][0] = 1.0 / 3.0;
+    LC.crj[0][1] = 1.0 / 3.0;  LC.crj[1][1] = 5.0 / 6.0;  LC.crj[2][1] = -1.0 / 6.0;
+    LC.crj[0][2] = -1.0 / 6.0; LC.crj[1][2] = 5.0 / 6.0;  LC.crj[2][2] = 1.0 / 3.0;
+    LC.crj[0][3] = 1.0 / 3.0;  LC.crj[1][3] = -7.0 / 6.0; LC.crj[2][3] = 11.0 / 6.0;
+    LC.dkm[0] = 0.3; LC.dkm[1] = 0.6; LC.dkm[2] = 0.1;
+    LC.dkp[0] = 0.1; LC.dkp[1] = 0.6; LC.dkp[2] = 0.3;
+
+    /* crjgp3L / crjgm3L: :66-84 */
+    LC.crjgp3L[0][0] = 1.0 / 30.0 + sq15 / 20.0;
+    LC.crjgp3L[1][0] = -1.0 / 15.0 - sq15 / 5.0;
+    LC.crjgp3L[2][0] = 31.0 / 30.0 + 3.0 * sq15 / 20.0;
+    LC.crjgp3L[0][1] = 1.0 / 30.0 - sq15 / 20.0;
+    LC.crjgp3L[1][1] = 14.0 / 15.0;
+    LC.crjgp3L[2][1] = 1.0 / 30.0 + sq15 / 20.0;
+    LC.crjgp3L[0][2] = 31.0 / 30.0 - 3.0 * sq15 / 20.0;
+    LC.crjgp3L[1][2] = -1.0 / 15.0 + sq15 / 5.0;
+    LC.crjgp3L[2][2] = 1.0 / 30.0 - sq15 / 20.0;
+
+    LC.crjgm3L[0][0] = 1.0 / 30.0 - sq15 / 20.0;
+    LC.crjgm3L[1][0] = -1.0 / 15.0 + sq15 / 5.0;
+    LC.crjgm3L[2][0] = 31.0 / 30.0 - 3.0 * sq15 / 20.0;
+    LC.crjgm3L[0][1] = 1.0 / 30.0 + sq15 / 20.0;
+    LC.crjgm3L[1][1] = 14.0 / 15.0;
+    LC.crjgm3L[2][1] = 1.0 / 30.0 - sq15 / 20.0;
+    LC.crjgm3L[0][2] = 31.0 / 30.0 + 3.0 * sq15 / 20.0;
+    LC.crjgm3L[1][2] = -1.0 / 15.0 - sq15 / 5.0;
+    LC.crjgm3L[2][2] = 1.0 / 30.0 + sq15 / 20.0;
+
+    /* centre-point split into positive / negative groups: :86-93 */
+    double lin0[3] = {-9.0 / 80.0, 49.0 / 40.0, -9.0 / 80.0};
+    for (int i = 0; i < 3; ++i) {
+        LC.linw0p[i] = 0.5 * (lin0[i] + 3.0 * fabs(lin0[i]));
+        LC.linw0m[i] = LC.linw0p[i] - lin0[i];
+    }
+    LC.sumw0p = (LC.linw0p[0] + LC.linw0p[1]) + LC.linw0p[2];
+    LC.sumw0m = (LC.linw0m[0] + LC.linw0m[1]) + LC.linw0m[2];
+    for (int i = 0; i < 3; ++i) {
+        LC.linw0p[i] = LC.linw0p[i] / LC.sumw0p;
+        LC.linw0m[i] = LC.linw0m[i] / LC.sumw0m;
+    }
+    /* +xi / -xi linear weights: :95-105 */
+    double denom = 15720.0 * sq15 + 10480.0;
+    LC.linwp[0] = (2882.0 * sq15 - 1179.0) / denom;
+    LC.linwp[1] = (9672.0 * sq15 + 6448.0) / denom;
+    LC.linwp[2] = (3166.0 * sq15 + 5211.0) / denom;
+    denom = 15720.0 * sq15 - 10480;
+    LC.linwm[0] = (2882.0 * sq15 + 1179.0) / denom;
+    LC.linwm[1] = (9672.0 * sq15 - 6448.0) / denom;
+    LC.linwm[2] = (3166.0 * sq15 - 5211.0) / denom;
+    LC_ready = 1;
+}
+
+static inline double dot3(const double a[3], const double b[3]) {
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+static inline double sum3(const double a[3]) { return (a[0] + a[1]) + a[2]; }
+static inline double catchnan(double x) { return isnan(x) ? LC.machine_eps : x; }
+
+/* Schemes::Lagrange::smoothnessMeasure, Schemes_Lagrange.cpp:1013-1026 */
+static void smoothness5(const double* v, double SI[3]) {
+    double c[3], b[3];
+    c[0] = v[2] - 2.0 * v[3] + v[4];
+    b[0] = 3.0 * v[2] - 4.0 * v[3] + v[4];
+    c[1] = v[1] - 2.0 * v[2] + v[3];
+    b[1] = v[1] - v[3];
+    c[2] = v[0] - 2.0 * v[1] + v[2];
+    b[2] = v[0] - 4.0 * v[1] + 3.0 * v[2];
+    for (int k = 0; k < 3; ++k) {
+        b[k] = b[k] / 2.0;
+        SI[k] = b[k] * b[k] + LC.c1 * c[k] * c[k];
+    }
+}
+
+/* Schemes::Lagrange::fullpointsSmootness, Schemes_Lagrange.cpp:1029-1034 */
+static double b35_of(const double* u) {
+    double b35 = u[4] * (6908.0 * u[4] - 51001.0 * u[3] + 67923.0 * u[2] - 38947.0 * u[1] + 8209.0 * u[0]) +
+                 u[3] * (104963.0 * u[3] - 299076.0 * u[2] + 179098.0 * u[1] - 38947.0 * u[0]) +
+                 u[2] * (231153.0 * u[2] - 299076.0 * u[1] + 67923.0 * u[0]) +
+                 u[1] * (104963.0 * u[1] - 51001.0 * u[0]) + 6908.0 * u[0] * u[0];
+    return b35 / 5040.0;
+}
+
+/* One side of teno5: smoothness, tau, gamma^6, cutOff (Schemes_Lagrange.cpp
+ * :486-494 and :1141-1149).  v = the 5-point window, p = candidates,
+ * d = linear weights.  Returns the reconstructed value; writes delta flags. */
+static double teno5_side(const double* v, const double p[3], const double d[3], int sig_out[3]) {
+    double SI[3], w[3], sigm[3];
+    smoothness5(v, SI);
+    double b35 = b35_of(v);
+    double tau = fabs(b35 - (SI[0] + SI[2] + 4.0 * SI[1]) / 6.0);
+    for (int k = 0; k < 3; ++k) w[k] = pow(1.0 + tau / (SI[k] + LC.epsz), 6);
+    /* cutOff(): normalise, then threshold at C_T */
+    double s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    for (int k = 0; k < 3; ++k) sigm[k] = w[k] < LC.ct ? 0.0 : 1.0;
+    for (int k = 0; k < 3; ++k) w[k] = sigm[k] * d[k];
+    s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    if (sig_out)
+        for (int k = 0; k < 3; ++k) sig_out[k] = (int)sigm[k];
+    return catchnan(dot3(p, w));
+}
+
+/* Schemes::Lagrange::teno5, Schemes_Lagrange.cpp:481-511.
+ * u = 6 consecutive values u_{i-2..i+3}; out = (u^-_{i+1/2}, u^+_{i+1/2}). */
+static void teno5_impl(const double u[6], double out[2], int flags[6]) {
+    double p[3], c[3];
+    /* left-biased value at x_{i+1/2}: :482-494 */
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][1];
+    p[0] = dot3(c, u + 2);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][2];
+    p[1] = dot3(c, u + 1);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][3];
+    p[2] = dot3(c, u + 0);
+    out[0] = teno5_side(u + 0, p, LC.dkm, flags ? flags : NULL);
+    /* right-biased value at x_{i+1/2}: :496-508 */
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][0];
+    p[0] = dot3(c, u + 3);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][1];
+    p[1] = dot3(c, u + 2);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][2];
+    p[2] = dot3(c, u + 1);
+    out[1] = teno5_side(u + 1, p, LC.dkp, flags ? flags + 3 : NULL);
+}
+
+void orc_teno5(const double u[6], double out[2]) {
+    lag_init();
+    teno5_impl(u, out, NULL);
+}
+
+void orc_teno5_flags(const double u[6], int flags[6]) {
+    double out[2];
+    lag_init();
+    teno5_impl(u, out, flags);
+}
+
+/* Schemes::Lagrange::linear, Schemes_Lagrange.cpp:1219-1233 */
+void orc_linear5(const double u[6], double out[2]) {
+    double p[3], c[3];
+    lag_init();
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][1];
+    p[0] = dot3(c, u + 2);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][2];
+    p[1] = dot3(c, u + 1);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][3];
+    p[2] = dot3(c, u + 0);
+    out[0] = dot3(p, LC.dkm);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][0];
+    p[0] = dot3(c, u + 3);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][1];
+    p[1] = dot3(c, u + 2);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][2];
+    p[2] = dot3(c, u + 1);
+    out[1] = dot3(p, LC.dkp);
+}
+
+/* Schemes::Lagrange::weno5JS3PointsQuadrature + smoothnessMeasure3L,
+ * Schemes_Lagrange.cpp:1155-1217.  u = 5 transverse face values (rows
+ * j-2..j+2); out = values at the Gauss points (+xi, 0, -xi), xi = sqrt(15)/10. */
+void orc_weno5js3q(const double u[5], double out[3]) {
+    double bg[3], cg[3], SI[3], p[3], w[3], c[3], s;
+    lag_init();
+    bg[0] = u[0] - 4.0 * u[1] + 3.0 * u[2];
+    cg[0] = u[0] - 2.0 * u[1] + u[2];
+    bg[1] = u[1] - u[3];
+    cg[1] = u[1] - 2.0 * u[2] + u[3];
+    bg[2] = 3.0 * u[2] - 4.0 * u[3] + u[4];
+    cg[2] = u[2] - 2.0 * u[3] + u[4];
+    for (int k = 0; k < 3; ++k) {
+        bg[k] = 0.25 * (bg[k] * bg[k]);
+        cg[k] = LC.c1 * (cg[k] * cg[k]);
+        SI[k] = bg[k] + cg[k];
+    }
+    double den[3];
+    for (int k = 0; k < 3; ++k) den[k] = (LC.eps + SI[k]) * (LC.eps + SI[k]);
+
+    /* +xi point: :1178-1186 */
+    for (int k = 0; k < 3; ++k) {
+        for (int r = 0; r < 3; ++r) c[r] = LC.crjgp3L[r][k];
+        p[k] = dot3(c, u + k);
+    }
+    for (int k = 0; k < 3; ++k) w[k] = LC.linwp[k] / den[k];
+    s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    out[0] = dot3(p, w);
+
+    /* centre point with positive/negative weight split: :1188-1200 */
+    p[0] = (-u[0] + 2 * u[1] + 23.0 * u[2]) / 24.0;
+    p[1] = (-u[1] + 26.0 * u[2] - u[3]) / 24.0;
+    p[2] = (23.0 * u[2] + 2.0 * u[3] - u[4]) / 24.0;
+    for (int k = 0; k < 3; ++k) w[k] = LC.linw0p[k] / den[k];
+    s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    out[1] = LC.sumw0p * dot3(p, w);
+    for (int k = 0; k < 3; ++k) w[k] = LC.linw0m[k] / den[k];
+    s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    out[1] -= LC.sumw0m * dot3(p, w);
+
+    /* -xi point: :1202-1210 */
+    for (int k = 0; k < 3; ++k) {
+        for (int r = 0; r < 3; ++r) c[r] = LC.crjgm3L[r][k];
+        p[k] = dot3(c, u + k);
+    }
+    for (int k = 0; k < 3; ++k) w[k] = LC.linwm[k] / den[k];
+    s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    out[2] = dot3(p, w);
+
+    for (int k = 0; k < 3; ++k) out[k] = catchnan(out[k]);
+}
+
+/* Gauss-Legendre 9-point rule in the reference's node order
+ * (Schemes_Lagrange.cpp:22-42): 0, -a1, +a1, -a2, +a2, -a3, +a3, -a4, +a4. */
+void orc_gauss9(double nodes[9], double weights[9]) {
+    static const double a[4] = {0.8360311073266358, 0.9681602395076261, 0.3242534234038089,
+                                0.6133714327005904};
+    static const double wa[4] = {0.1806481606948574, 0.08127438836157441, 0.31234707704000286,
+                                 0.26061069640293544};
+    nodes[0] = 0.0;
+    weights[0] = 0.3302393550012598;
+    for (int k = 0; k < 4; ++k) {
+        nodes[1 + 2 * k] = -a[k];
+        nodes[2 + 2 * k] = a[k];
+        weights[1 + 2 * k] = wa[k];
+        weights[2 + 2 * k] = wa[k];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Equations::Euler2D (Equations_Euler2D.cpp)                                */
+/* ------------------------------------------------------------------------ */
+
+/* primitive: Equations_Euler2D.cpp:20-26 */
+void orc_primitive(double gamma, const double q[4], double w[4]) {
+    double g1 = gamma - 1.0;
+    w[0] = q[0];
+    w[1] = q[1] / q[0];
+    w[2] = q[2] / q[0];
+    w[3] = (q[3] - 0.5 * (q[1] * w[1] + q[2] * w[2])) * g1;
+}
+
+/* conservative: Equations_Euler2D.cpp:28-34 */
+void orc_conservative(double gamma, const double w[4], double q[4]) {
+    double g1 = gamma - 1.0;
+    q[0] = w[0];
+    q[1] = w[0] * w[1];
+    q[2] = w[0] * w[2];
+    q[3] = w[3] / g1 + 0.5 * (q[1] * w[1] + q[2] * w[2]);
+}
+
+/* fluxx / fluxy: Equations_Euler2D.cpp:36-50 */
+static void phys_flux(int dir, const double q[4], const double w[4], double f[4]) {
+    if (dir == 1) {
+        f[0] = q[1];
+        f[1] = q[1] * w[1] + w[3];
+        f[2] = q[1] * w[2];
+        f[3] = w[1] * (q[3] + w[3]);
+    } else {
+        f[0] = q[2];
+        f[1] = q[2] * w[1];
+        f[2] = q[2] * w[2] + w[3];
+        f[3] = w[2] * (q[3] + w[3]);
+    }
+}
+
+/* Roe-averaged eigenvectors of the face between cells a and b in direction
+ * dir: FV::System2D::eigen2DEuler (FV_System2D.cpp:265-300 for x, :329-360 for
+ * y) with Euler2D::right/leftEigenVectorx/y (Equations_Euler2D.cpp:100-205).
+ * The per-cell sqrt(|rho|) and H come from calcPrimitiveVariable
+ * (FV_System2D.cpp:226-263).  L and R are row-major 4x4. */
+void orc_eigen(double gamma, int dir, const double qa[4], const double qb[4], double L[16],
+               double R[16]) {
+    double g1 = gamma - 1.0;
+    double wa[4], wb[4];
+    orc_primitive(gamma, qa, wa);
+    orc_primitive(gamma, qb, wb);
+    double sa = sqrt(fabs(wa[0])), sb = sqrt(fabs(wb[0]));
+    double ha = (qa[3] + wa[3]) / wa[0], hb = (qb[3] + wb[3]) / wb[0];
+    double t0 = sa / (sa + sb);
+    double t1 = 1.0 - t0;
+    double vxm = t0 * wa[1] + t1 * wb[1];
+    double vym = t0 * wa[2] + t1 * wb[2];
+    double hm = t0 * ha + t1 * hb;
+    double qm = 0.5 * (vxm * vxm + vym * vym);
+    double cm = sqrt(fabs(g1 * (hm - qm)));
+    double rcm = 1.0 / cm;
+    double b1 = g1 * rcm * rcm;
+    double b2 = qm * b1;
+    double t2 = 0.5 * b1;
+    if (dir == 1) {
+        double tc = vxm * cm;
+        double Rx[16] = {1.0, 0.0, 1.0, 1.0,
+                         vxm - cm, 0.0, vxm, vxm + cm,
+                         vym, 1.0, vym, vym,
+                         hm - tc, vym, qm, hm + tc};
+        double tr = vxm * rcm, tb = b1 * vxm;
+        double Lx[16] = {0.5 * (b2 + tr), -0.5 * (tb + rcm), -t2 * vym, t2,
+                         -vym, 0.0, 1.0, 0.0,
+                         1.0 - b2, tb, b1 * vym, -b1,
+                         0.5 * (b2 - tr), -0.5 * (tb - rcm), -t2 * vym, t2};
+        memcpy(R, Rx, sizeof Rx);
+        memcpy(L, Lx, sizeof Lx);
+    } else {
+        double tc = vym * cm;
+        double Ry[16] = {1.0, 0.0, 1.0, 1.0,
+                         vxm, 1.0, vxm, vxm,
+                         vym - cm, 0.0, vym, vym + cm,
+                         hm - tc, vxm, qm, hm + tc};
+        double tr = vym * rcm, tb = b1 * vym;
+        double Ly[16] = {0.5 * (b2 + tr), -t2 * vxm, -0.5 * (tb + rcm), t2,
+                         -vxm, 1.0, 0.0, 0.0,
+                         1.0 - b2, b1 * vxm, tb, -b1,
+                         0.5 * (b2 - tr), -t2 * vxm, -0.5 * (tb - rcm), t2};
+        memcpy(R, Ry, sizeof Ry);
+        memcpy(L, Ly, sizeof Ly);
+    }
+}
+
+static inline void matvec4(const double M[16], const double x[4], double y[4]) {
+    for (int r = 0; r < 4; ++r)
+        y[r] = ((M[4 * r] * x[0] + M[4 * r + 1] * x[1]) + M[4 * r + 2] * x[2]) + M[4 * r + 3] * x[3];
+}
+
+/* maxLocalWaveSpeed (Equations_Euler2D.cpp:221-246) + characteristic Rusanov
+ * (:310-330): F = 0.5 (f(qL) + f(qR) - R diag(a) L (qR - qL)). */
+void orc_rusanov_char(double gamma, int dir, const double ql[4], const double qr[4],
+                      const double L[16], const double R[16], double f[4]) {
+    double wl[4], wr[4], a[4], fl[4], fr[4], dq[4], ld[4], rd[4];
+    orc_primitive(gamma, ql, wl);
+    orc_primitive(gamma, qr, wr);
+    double cl = sqrt(fabs(gamma * wl[3] / wl[0]));
+    double cr = sqrt(fabs(gamma * wr[3] / wr[0]));
+    double vl = wl[dir], vr = wr[dir];
+    a[0] = fmax(fabs(vl - cl), fabs(vr - cr));
+    a[1] = fmax(fabs(vl), fabs(vr));
+    a[2] = a[1];
+    a[3] = fmax(fabs(vl + cl), fabs(vr + cr));
+    /* aM = ... / 1.0 (FV_System2D.cpp:460,485) is a no-op */
+    phys_flux(dir, ql, wl, fl);
+    phys_flux(dir, qr, wr, fr);
+    for (int k = 0; k < 4; ++k) dq[k] = qr[k] - ql[k];
+    matvec4(L, dq, ld);
+    for (int k = 0; k < 4; ++k) ld[k] = a[k] * ld[k];
+    matvec4(R, ld, rd);
+    for (int k = 0; k < 4; ++k) f[k] = 0.5 * ((fl[k] + fr[k]) - rd[k]);
+}
+
+/* Equations::Euler2D::HLLC, Equations_Euler2D.cpp:372-428.  quirks = 1 keeps
+ * the reference's c = sqrt(|p/rho|) (:393-394) and p_L in the right star
+ * energy (:423); quirks = 0 is Toro's HLLC with the same Roe-Einfeldt speeds. */
+void orc_hllc(double gamma, int dir, int quirks, const double ql[4], const double qr[4],
+              double f[4]) {
+    double g1 = gamma - 1.0;
+    int d1 = dir - 1;
+    double wl[4], wr[4], fl[4], fr[4], qs[4], velm[2];
+    orc_primitive(gamma, ql, wl);
+    orc_primitive(gamma, qr, wr);
+    double hl = (ql[3] + wl[3]) / wl[0];
+    double hr = (qr[3] + wr[3]) / wr[0];
+    double rhoRatio = sqrt(fabs(wr[0] / wl[0]));
+    double rr = 1.0 / (1.0 + rhoRatio);
+    velm[0] = (wl[1] + wr[1] * rhoRatio) * rr;
+    velm[1] = (wl[2] + wr[2] * rhoRatio) * rr;
+    double qm = 0.5 * (velm[0] * velm[0] + velm[1] * velm[1]);
+    double hm = (hl + hr * rhoRatio) * rr;
+    double cm = sqrt(fabs(g1 * (hm - qm)));
+    double cl, cr;
+    if (quirks) {
+        cl = sqrt(fabs(wl[3] / wl[0]));
+        cr = sqrt(fabs(wr[3] / wr[0]));
+    } else {
+        cl = sqrt(fabs(gamma * wl[3] / wl[0]));
+        cr = sqrt(fabs(gamma * wr[3] / wr[0]));
+    }
+    double sl = fmin(wl[dir] - cl, velm[d1] - cm);
+    double sr = fmax(wr[dir] + cr, velm[d1] + cm);
+    double sm = (qr[dir] * (sr - wr[dir]) - ql[dir] * (sl - wl[dir]) + wl[3] - wr[3]) /
+                (wr[0] * (sr - wr[dir]) - wl[0] * (sl - wl[dir]));
+    double ps = wl[3] + wl[0] * (wl[dir] - sl) * (wl[dir] - sm);
+    phys_flux(dir, ql, wl, fl);
+    phys_flux(dir, qr, wr, fr);
+    if (sl > 0.0) {
+        memcpy(f, fl, sizeof fl);
+    } else if (sl <= 0.0 && sm > 0.0) {
+        double nom = sl - wl[dir], den = sl - sm;
+        qs[0] = wl[0] * nom / den;
+        qs[1] = dir == 1 ? (nom * ql[1] + ps - wl[3]) / den : nom * ql[1] / den;
+        qs[2] = dir == 2 ? (nom * ql[2] + ps - wl[3]) / den : nom * ql[2] / den;
+        qs[3] = (nom * ql[3] - wl[3] * wl[dir] + ps * sm) / den;
+        for (int k = 0; k < 4; ++k) f[k] = fl[k] + sl * (qs[k] - ql[k]);
+    } else if (sm <= 0.0 && sr >= 0.0) {
+        double nom = sr - wr[dir], den = sr - sm;
+        double pe = quirks ? wl[3] : wr[3];
+        qs[0] = wr[0] * nom / den;
+        qs[1] = dir == 1 ? (nom * qr[1] + ps - wr[3]) / den : nom * qr[1] / den;
+        qs[2] = dir == 2 ? (nom * qr[2] + ps - wr[3]) / den : nom * qr[2] / den;
+        qs[3] = (nom * qr[3] - pe * wr[dir] + ps * sm) / den;
+        for (int k = 0; k < 4; ++k) f[k] = fr[k] + sr * (qs[k] - qr[k]);
+    } else {
+        memcpy(f, fr, sizeof fr);
+    }
+}
+
+/* Euler2D::maxSpeed, Equations_Euler2D.cpp:207-219 (quirk: no fabs on u-c) */
+double orc_max_speed(double gamma, int dir, int quirks, const double w[4]) {
+    double c = sqrt(fabs(gamma * w[3] / w[0]));
+    if (quirks) return fmax((w[dir] - c), fabs(w[dir] + c));
+    return fmax(fabs(w[dir] - c), fabs(w[dir] + c));
+}
+
+/* SSPRK::RungeKuttaIII, Schemes_SSPRK.cpp:3-15 */
+void orc_rk3_combine(int stage, const double* q0, const double* qs, const double* rhs, double dt,
+                     double* out, long n) {
+    for (long i = 0; i < n; ++i) {
+        if (stage == 1)
+            out[i] = q0[i] + dt * rhs[i];
+        else if (stage == 2)
+            out[i] = 0.75 * q0[i] + 0.25 * (qs[i] + rhs[i] * dt);
+        else
+            out[i] = (q0[i] + 2.0 * (qs[i] + dt * rhs[i])) / 3.0;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* FV::System2D (FV_System2D.cpp)                                            */
+/* ------------------------------------------------------------------------ */
+
+#define FV_K 6 /* ghost width, FV_System2D.cpp:20 */
+#define FV_S 2 /* (k-1)/2, :21 */
+
+typedef struct {
+    int nx, ny, k, nxx, nyy;
+    double *q[4];   /* ghost-padded state, idx = j*nxx + i */
+} padded;
+
+static void pad_alloc(padded* P, int nx, int ny, int k, int nf) {
+    P->nx = nx; P->ny = ny; P->k = k;
+    P->nxx = nx + 2 * k; P->nyy = ny + 2 * k;
+    for (int f = 0; f < nf; ++f) P->q[f] = (double*)malloc(sizeof(double) * (size_t)P->nxx * P->nyy);
+}
+static void pad_free(padded* P, int nf) {
+    for (int f = 0; f < nf; ++f) free(P->q[f]);
+}
+#define PIX(P, i, j) ((size_t)(j) * (P)->nxx + (i))
+
+/* zeroGradBc (FV_System2D.cpp:134-152) / periodicBc (:114-132) on one field */
+static void fill_ghosts(padded* P, double* a, int bc) {
+    int k = P->k, nx = P->nx, ny = P->ny, nxx = P->nxx, nyy = P->nyy;
+    for (int j = k; j < ny + k; ++j)
+        for (int m = 0; m < k; ++m) {
+            if (bc == ORC_BC_ZERO_GRAD) {
+                a[PIX(P, m, j)] = a[PIX(P, k, j)];
+                a[PIX(P, nx + k + m, j)] = a[PIX(P, nx + k - 1, j)];
+            } else {
+                a[PIX(P, k - 1 - m, j)] = a[PIX(P, nx + k - 1 - m, j)];
+                a[PIX(P, nxx - 1 - m, j)] = a[PIX(P, k + k - 1 - m, j)];
+            }
+        }
+    for (int i = 0; i < nxx; ++i)
+        for (int m = 0; m < k; ++m) {
+            if (bc == ORC_BC_ZERO_GRAD) {
+                a[PIX(P, i, m)] = a[PIX(P, i, k)];
+                a[PIX(P, i, ny + k + m)] = a[PIX(P, i, ny + k - 1)];
+            } else {
+                a[PIX(P, i, k - 1 - m)] = a[PIX(P, i, ny + k - 1 - m)];
+                a[PIX(P, i, nyy - 1 - m)] = a[PIX(P, i, k + k - 1 - m)];
+            }
+        }
+}
+
+static void load_padded(padded* P, const double* q, int nf, int bc) {
+    size_t n = (size_t)P->nx * P->ny;
+    for (int f = 0; f < nf; ++f) {
+        for (int j = 0; j < P->ny; ++j)
+            memcpy(P->q[f] + PIX(P, P->k, j + P->k), q + f * n + (size_t)j * P->nx,
+                   sizeof(double) * P->nx);
+        fill_ghosts(P, P->q[f], bc);
+    }
+}
+
+static inline void cell_q(const padded* P, int i, int j, double out[4]) {
+    size_t id = PIX(P, i, j);
+    for (int f = 0; f < 4; ++f) out[f] = P->q[f][id];
+}
+
+/* FV::System2D::rhs (FV_System2D.cpp:363-573): characteristic TENO5 in x and
+ * y (:372-400), 3-point transverse quadrature of the characteristic face
+ * values of rows j-2..j+2 (:402-423), back-projection with the face's own R
+ * (:425-430, FV_System2D.h:50-58), flux at the 3 Gauss points weighted by
+ * (5/18, 4/9, 5/18) (:432-497), divergence (:562-572). */
+void orc_fv2d_rhs(const orc_params* p, const double* q, double* rhs) {
+    lag_init();
+    padded P;
+    int nx = p->nx, ny = p->ny, k = FV_K, s = FV_S;
+    double gamma = p->gamma;
+    double dx = (p->xr - p->xl) / nx, dy = (p->yt - p->yb) / ny, area = dx * dy;
+    double gw[3] = {5.0 / 9.0 / 2.0, 8.0 / 9.0 / 2.0, 5.0 / 9.0 / 2.0}; /* :28 */
+    pad_alloc(&P, nx, ny, k, 4);
+    load_padded(&P, q, 4, p->bc);
+    int nxx = P.nxx, nyy = P.nyy;
+    size_t nt = (size_t)nxx * nyy;
+    /* face arrays (stored at the left/lower cell, like the reference) */
+    double* uxl = (double*)calloc(4 * nt, sizeof(double));
+    double* uxr = (double*)calloc(4 * nt, sizeof(double));
+    double* uyl = (double*)calloc(4 * nt, sizeof(double));
+    double* uyr = (double*)calloc(4 * nt, sizeof(double));
+    double* fx = (double*)calloc(4 * nt, sizeof(double));
+    double* fy = (double*)calloc(4 * nt, sizeof(double));
+
+    /* x reconstruction: FV_System2D.cpp:372-385 */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nyy; ++j) {
+        for (int i = s; i < nxx - s - 1; ++i) {
+            double qa[4], qb[4], L[16], R[16], C[4][6], cm[4], out[2];
+            cell_q(&P, i, j, qa);
+            cell_q(&P, i + 1, j, qb);
+            orc_eigen(gamma, 1, qa, qb, L, R);
+            for (int m = 0; m < 6; ++m) {
+                cell_q(&P, i - 2 + m, j, cm);
+                double y[4];
+                matvec4(L, cm, y);
+                for (int e = 0; e < 4; ++e) C[e][m] = y[e];
+            }
+            for (int e = 0; e < 4; ++e) {
+                teno5_impl(C[e], out, NULL);
+                uxl[e * nt + PIX(&P, i, j)] = out[0];
+                uxr[e * nt + PIX(&P, i, j)] = out[1];
+            }
+        }
+    }
+    /* y reconstruction: :387-400 */
+#pragma omp parallel for schedule(static)
+    for (int j = s; j < nyy - s - 1; ++j) {
+        for (int i = 0; i < nxx; ++i) {
+            double qa[4], qb[4], L[16], R[16], C[4][6], cm[4], out[2];
+            cell_q(&P, i, j, qa);
+            cell_q(&P, i, j + 1, qb);
+            orc_eigen(gamma, 2, qa, qb, L, R);
+            for (int m = 0; m < 6; ++m) {
+                cell_q(&P, i, j - 2 + m, cm);
+                double y[4];
+                matvec4(L, cm, y);
+                for (int e = 0; e < 4; ++e) C[e][m] = y[e];
+            }
+            for (int e = 0; e < 4; ++e) {
+                teno5_impl(C[e], out, NULL);
+                uyl[e * nt + PIX(&P, i, j)] = out[0];
+                uyr[e * nt + PIX(&P, i, j)] = out[1];
+            }
+        }
+    }
+    /* transverse quadrature + back-projection + flux: :402-497.  The
+     * quadrature loop runs over [s+2, n-s-3) and the flux loop over
+     * [k-1, n-k); only the latter range is consumed, so both are fused. */
+#pragma omp parallel for schedule(static)
+    for (int j = k - 1; j < nyy - k; ++j) {
+        for (int i = k - 1; i < nxx - k; ++i) {
+            for (int dir = 1; dir <= 2; ++dir) {
+                double qa[4], qb[4], L[16], R[16];
+                double gl[4][3], gr[4][3];
+                cell_q(&P, i, j, qa);
+                if (dir == 1) cell_q(&P, i + 1, j, qb); else cell_q(&P, i, j + 1, qb);
+                orc_eigen(gamma, dir, qa, qb, L, R);
+                for (int e = 0; e < 4; ++e) {
+                    double a5[5], b5[5];
+                    for (int m = 0; m < 5; ++m) {
+                        size_t id = dir == 1 ? PIX(&P, i, j - 2 + m) : PIX(&P, i - 2 + m, j);
+                        a5[m] = (dir == 1 ? uxl : uyl)[e * nt + id];
+                        b5[m] = (dir == 1 ? uxr : uyr)[e * nt + id];
+                    }
+                    orc_weno5js3q(a5, gl[e]);
+                    orc_weno5js3q(b5, gr[e]);
+                }
+                double tf[3][4];
+                for (int m = 0; m < 3; ++m) {
+                    double cl[4], cr[4], ql[4], qr[4], F[4];
+                    for (int e = 0; e < 4; ++e) { cl[e] = gl[e][m]; cr[e] = gr[e][m]; }
+                    matvec4(R, cl, ql);
+                    matvec4(R, cr, qr);
+                    if (p->flux == ORC_FLUX_HLLC)
+                        orc_hllc(gamma, dir, p->quirks, ql, qr, F);
+                    else
+                        orc_rusanov_char(gamma, dir, ql, qr, L, R, F);
+                    for (int e = 0; e < 4; ++e) tf[m][e] = gw[m] * F[e];
+                }
+                double* fd = dir == 1 ? fx : fy;
+                for (int e = 0; e < 4; ++e)
+                    fd[e * nt + PIX(&P, i, j)] = (tf[0][e] + tf[1][e]) + tf[2][e];
+            }
+        }
+    }
+    /* divergence: :562-572 */
+    size_t n = (size_t)nx * ny;
+    for (int j = k; j < nyy - k; ++j)
+        for (int i = k; i < nxx - k; ++i)
+            for (int e = 0; e < 4; ++e) {
+                double v = ((fx[e * nt + PIX(&P, i - 1, j)] - fx[e * nt + PIX(&P, i, j)]) * dy +
+                            (fy[e * nt + PIX(&P, i, j - 1)] - fy[e * nt + PIX(&P, i, j)]) * dx) /
+                           area;
+                rhs[e * n + (size_t)(j - k) * nx + (i - k)] = v;
+            }
+    free(uxl); free(uxr); free(uyl); free(uyr); free(fx); free(fy);
+    pad_free(&P, 4);
+}
+
+/* CFL reduction: FV::System2D::timeStepping (FV_System2D.cpp:623-631) */
+void orc_fv2d_max_speed(const orc_params* p, const double* q, double* sx, double* sy) {
+    size_t n = (size_t)p->nx * p->ny;
+    double mx = 0.0, my = 0.0;
+    for (size_t c = 0; c < n; ++c) {
+        double qc[4] = {q[c], q[n + c], q[2 * n + c], q[3 * n + c]}, w[4];
+        orc_primitive(p->gamma, qc, w);
+        mx = fmax(mx, orc_max_speed(p->gamma, 1, p->quirks, w));
+        my = fmax(my, orc_max_speed(p->gamma, 2, p->quirks, w));
+    }
+    *sx = mx;
+    *sy = my;
+}
+
+/* dt = cfl * min(dx,dy)^(3/3) / max(sx,sy), clamped to tFinal (:633,645) */
+double orc_fv2d_dt(const orc_params* p, const double* q, double cfl, double t, double tfinal) {
+    double sx, sy;
+    double dx = (p->xr - p->xl) / p->nx, dy = (p->yt - p->yb) / p->ny;
+    orc_fv2d_max_speed(p, q, &sx, &sy);
+    double dt = cfl * pow(fmin(dx, dy), 3.0 / 3.0) / fmax(sx, sy);
+    return t + dt > tfinal ? tfinal - t : dt;
+}
+
+/* FV::System2D::RungeKutta3, FV_System2D.cpp:575-612 */
+void orc_fv2d_rk3(const orc_params* p, double* q, double dt) {
+    size_t n = 4 * (size_t)p->nx * p->ny;
+    double* q0 = (double*)malloc(n * sizeof(double));
+    double* r = (double*)malloc(n * sizeof(double));
+    double* qs = (double*)malloc(n * sizeof(double));
+    memcpy(q0, q, n * sizeof(double));
+    orc_fv2d_rhs(p, q, r);
+    orc_rk3_combine(1, q0, q0, r, dt, q, (long)n);
+    memcpy(qs, q, n * sizeof(double));
+    orc_fv2d_rhs(p, q, r);
+    orc_rk3_combine(2, q0, qs, r, dt, q, (long)n);
+    memcpy(qs, q, n * sizeof(double));
+    orc_fv2d_rhs(p, q, r);
+    orc_rk3_combine(3, q0, qs, r, dt, q, (long)n);
+    free(q0); free(r); free(qs);
+}
+
+/* FV::System2D::timeStepping, FV_System2D.cpp:614-651 */
+int orc_fv2d_advance(const orc_params* p, double* q, double cfl, double t0, double tfinal,
+                     int max_steps, double* t_out) {
+    double t = t0;
+    int nsteps = 0;
+    while (t < tfinal && (max_steps <= 0 || nsteps < max_steps)) {
+        double dt = orc_fv2d_dt(p, q, cfl, t, tfinal);
+        orc_fv2d_rk3(p, q, dt);
+        t += dt;
+        ++nsteps;
+    }
+    if (t_out) *t_out = t;
+    return nsteps;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Initial conditions                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* conservativeOfIsentropicVortex (Equations_Euler2D.cpp:656-666), centred at
+ * (cx, cy), plus an optional density perturbation sum_m a_m sin(2 pi (k_m x +
+ * l_m y) / Lx + phi_m) at fixed velocity and pressure.  modes = nmodes rows of
+ * (k, l, a, phi, Lx, Ly). */
+static void vortex_point(double gamma, double x, double y, double cx, double cy,
+                         const double* modes, int nmodes, double q[4]) {
+    const double pi = 4.0 * atan(1.0);
+    double g1 = gamma - 1.0;
+    double xr = x - cx, yr = y - cy;
+    double w[4];
+    double r = pow(xr, 2) + pow(yr, 2), b = 5.0;
+    w[0] = 1.0 - g1 * pow(b, 2) / 8.0 / gamma / pow(pi, 2) * exp(1 - r);
+    w[0] = pow(w[0], 1.0 / g1);
+    w[1] = 1.0 - b * yr / 2.0 / pi * exp(0.5 * (1 - r));
+    w[2] = 1.0 + b * xr / 2.0 / pi * exp(0.5 * (1 - r));
+    w[3] = pow(w[0], gamma);
+    for (int m = 0; m < nmodes; ++m) {
+        const double* md = modes + 6 * m;
+        w[0] += md[2] * sin(2.0 * pi * (md[0] * x / md[4] + md[1] * y / md[5]) + md[3]);
+    }
+    orc_conservative(gamma, w, q);
+}
+
+/* Euler2D::isentropicVortex: 9x9 Gauss cell average (Equations_Euler2D.cpp
+ * :513-528), cell centres as FV_System2D.cpp:37-38. */
+void orc_ic_vortex(const orc_params* p, double cx, double cy, const double* modes, int nmodes,
+                   double* q) {
+    double a[9], wt[9];
+    orc_gauss9(a, wt);
+    int nx = p->nx, ny = p->ny;
+    double dx = (p->xr - p->xl) / nx, dy = (p->yt - p->yb) / ny;
+    size_t n = (size_t)nx * ny;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double x = ((double)i + 0.5) * dx + p->xl;
+            double y = ((double)j + 0.5) * dy + p->yb;
+            double acc[4] = {0, 0, 0, 0};
+            for (int ii = 0; ii < 9; ++ii)
+                for (int jj = 0; jj < 9; ++jj) {
+                    double qq[4];
+                    vortex_point(p->gamma, x + dx * a[ii] / 2.0, y + dy * a[jj] / 2.0, cx, cy, modes,
+                                 nmodes, qq);
+                    for (int e = 0; e < 4; ++e) acc[e] += wt[ii] * wt[jj] * qq[e] / 4.0;
+                }
+            for (int e = 0; e < 4; ++e) q[e * n + (size_t)j * nx + i] = acc[e];
+        }
+}

@@ -1,0 +1,66 @@
+/*
+ * htx_oracle.h -- FP64 CPU restatement of the Hermite-TENO finite-volume
+ * residual / time-step path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (libhtx.so) never links or calls it.
+ *
+ * Every function restates a reference routine; the citation is given at the
+ * definition in htx_oracle.c.  State arrays are structure-of-arrays, interior
+ * cells only, field-major: q[f*nx*ny + j*nx + i] (x fastest).  Ghost layers
+ * are built internally, exactly as the reference's zeroGradBc()/periodicBc().
+ */
+#ifndef HTX_ORACLE_H
+#define HTX_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_BC_ZERO_GRAD = 0, ORC_BC_PERIODIC = 1 };
+enum { ORC_FLUX_RUSANOV_CHAR = 0, ORC_FLUX_HLLC = 1 };
+
+typedef struct {
+    int nx, ny;
+    double xl, xr, yb, yt;
+    double gamma;
+    int bc;          /* ORC_BC_* */
+    int flux;        /* ORC_FLUX_* */
+    int quirks;      /* 1: reproduce reference quirks (HLLC c, HLLC E*, maxSpeed) */
+} orc_params;
+
+/* ---- scalar building blocks (unit-test surface) ---- */
+void orc_teno5(const double u[6], double out[2]);
+void orc_linear5(const double u[6], double out[2]);
+void orc_teno5_flags(const double u[6], int flags[6]);
+void orc_weno5js3q(const double u[5], double out[3]);
+void orc_primitive(double gamma, const double q[4], double w[4]);
+void orc_conservative(double gamma, const double w[4], double q[4]);
+void orc_eigen(double gamma, int dir, const double qa[4], const double qb[4],
+               double L[16], double R[16]);
+void orc_rusanov_char(double gamma, int dir, const double ql[4], const double qr[4],
+                      const double L[16], const double R[16], double f[4]);
+void orc_hllc(double gamma, int dir, int quirks, const double ql[4], const double qr[4],
+              double f[4]);
+double orc_max_speed(double gamma, int dir, int quirks, const double w[4]);
+void orc_rk3_combine(int stage, const double* q0, const double* qs, const double* rhs,
+                     double dt, double* out, long n);
+void orc_gauss9(double nodes[9], double weights[9]);
+
+/* ---- 2D FV (TENO5 + 3-point transverse quadrature) ---- */
+void orc_fv2d_rhs(const orc_params* p, const double* q, double* rhs);
+void orc_fv2d_max_speed(const orc_params* p, const double* q, double* sx, double* sy);
+double orc_fv2d_dt(const orc_params* p, const double* q, double cfl, double t, double tfinal);
+void orc_fv2d_rk3(const orc_params* p, double* q, double dt);
+int orc_fv2d_advance(const orc_params* p, double* q, double cfl, double t0, double tfinal,
+                     int max_steps, double* t_out);
+
+/* ---- initial conditions (restated generators) ---- */
+void orc_ic_vortex(const orc_params* p, double cx, double cy, const double* modes,
+                   int nmodes, double* q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

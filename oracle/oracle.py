@@ -1,0 +1,198 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  Arrays are numpy float64,
+field-major, interior cells only: q[f, j, i].
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "liboracle.so")
+LIB_OMP = os.path.join(_HERE, "liboracle_omp.so")
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", _I), ("ny", _I),
+        ("xl", _D), ("xr", _D), ("yb", _D), ("yt", _D),
+        ("gamma", _D),
+        ("bc", _I), ("flux", _I), ("quirks", _I),
+    ]
+
+
+_SIG = {
+    "orc_teno5": (None, [_DP, _DP]),
+    "orc_linear5": (None, [_DP, _DP]),
+    "orc_teno5_flags": (None, [_DP, _IP]),
+    "orc_weno5js3q": (None, [_DP, _DP]),
+    "orc_primitive": (None, [_D, _DP, _DP]),
+    "orc_conservative": (None, [_D, _DP, _DP]),
+    "orc_eigen": (None, [_D, _I, _DP, _DP, _DP, _DP]),
+    "orc_rusanov_char": (None, [_D, _I, _DP, _DP, _DP, _DP, _DP]),
+    "orc_hllc": (None, [_D, _I, _I, _DP, _DP, _DP]),
+    "orc_max_speed": (_D, [_D, _I, _I, _DP]),
+    "orc_rk3_combine": (None, [_I, _DP, _DP, _DP, _D, _DP, ctypes.c_long]),
+    "orc_gauss9": (None, [_DP, _DP]),
+    "orc_fv2d_rhs": (None, [ctypes.POINTER(OrcParams), _DP, _DP]),
+    "orc_fv2d_max_speed": (None, [ctypes.POINTER(OrcParams), _DP, _DP, _DP]),
+    "orc_fv2d_dt": (_D, [ctypes.POINTER(OrcParams), _DP, _D, _D, _D]),
+    "orc_fv2d_rk3": (None, [ctypes.POINTER(OrcParams), _DP, _D]),
+    "orc_fv2d_advance": (_I, [ctypes.POINTER(OrcParams), _DP, _D, _D, _D, _I, _DP]),
+    "orc_ic_vortex": (None, [ctypes.POINTER(OrcParams), _D, _D, _DP, _I, _DP]),
+}
+
+_libs = {}
+
+
+def load(omp=False):
+    path = LIB_OMP if omp else LIB
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        raise ImportError(f"{path} missing: build it with __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIG.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _libs[path] = lib
+    return lib
+
+
+def _p(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_DP)
+
+
+def params(nx, ny, xl=0.0, xr=1.0, yb=0.0, yt=1.0, gamma=1.4, bc="zero_grad", flux="hllc", quirks=True):
+    return OrcParams(nx, ny, xl, xr, yb, yt, gamma, {"zero_grad": 0, "periodic": 1}[bc],
+                     {"rusanov": 0, "hllc": 1}[flux], 1 if quirks else 0)
+
+
+class Oracle:
+    def __init__(self, omp=False):
+        self.lib = load(omp)
+
+    # scalar kernels
+    def teno5(self, u):
+        u, pu = _p(u)
+        out = np.empty(2)
+        self.lib.orc_teno5(pu, out.ctypes.data_as(_DP))
+        return out
+
+    def linear5(self, u):
+        u, pu = _p(u)
+        out = np.empty(2)
+        self.lib.orc_linear5(pu, out.ctypes.data_as(_DP))
+        return out
+
+    def teno5_flags(self, u):
+        u, pu = _p(u)
+        f = np.zeros(6, dtype=np.int32)
+        self.lib.orc_teno5_flags(pu, f.ctypes.data_as(_IP))
+        return f
+
+    def weno3q(self, u):
+        u, pu = _p(u)
+        out = np.empty(3)
+        self.lib.orc_weno5js3q(pu, out.ctypes.data_as(_DP))
+        return out
+
+    def primitive(self, gamma, q):
+        q, pq = _p(q)
+        w = np.empty(4)
+        self.lib.orc_primitive(gamma, pq, w.ctypes.data_as(_DP))
+        return w
+
+    def conservative(self, gamma, w):
+        w, pw = _p(w)
+        q = np.empty(4)
+        self.lib.orc_conservative(gamma, pw, q.ctypes.data_as(_DP))
+        return q
+
+    def eigen(self, gamma, direction, qa, qb):
+        qa, pa = _p(qa)
+        qb, pb = _p(qb)
+        L = np.empty(16)
+        R = np.empty(16)
+        self.lib.orc_eigen(gamma, direction, pa, pb, L.ctypes.data_as(_DP), R.ctypes.data_as(_DP))
+        return L.reshape(4, 4), R.reshape(4, 4)
+
+    def rusanov_char(self, gamma, direction, ql, qr, L, R):
+        ql, pl = _p(ql)
+        qr, pr = _p(qr)
+        L, pL = _p(np.asarray(L).reshape(16))
+        R, pR = _p(np.asarray(R).reshape(16))
+        f = np.empty(4)
+        self.lib.orc_rusanov_char(gamma, direction, pl, pr, pL, pR, f.ctypes.data_as(_DP))
+        return f
+
+    def hllc(self, gamma, direction, ql, qr, quirks=True):
+        ql, pl = _p(ql)
+        qr, pr = _p(qr)
+        f = np.empty(4)
+        self.lib.orc_hllc(gamma, direction, 1 if quirks else 0, pl, pr, f.ctypes.data_as(_DP))
+        return f
+
+    def max_speed(self, gamma, direction, w, quirks=True):
+        w, pw = _p(w)
+        return self.lib.orc_max_speed(gamma, direction, 1 if quirks else 0, pw)
+
+    def rk3_combine(self, stage, q0, qs, rhs, dt):
+        q0, p0 = _p(q0)
+        qs, ps = _p(qs)
+        rhs, pr = _p(rhs)
+        out = np.empty_like(q0)
+        self.lib.orc_rk3_combine(stage, p0, ps, pr, dt, out.ctypes.data_as(_DP), q0.size)
+        return out
+
+    def gauss9(self):
+        x = np.empty(9)
+        w = np.empty(9)
+        self.lib.orc_gauss9(x.ctypes.data_as(_DP), w.ctypes.data_as(_DP))
+        return x, w
+
+    # 2D FV
+    def fv2d_rhs(self, prm, q):
+        q, pq = _p(q)
+        r = np.empty_like(q)
+        self.lib.orc_fv2d_rhs(ctypes.byref(prm), pq, r.ctypes.data_as(_DP))
+        return r
+
+    def fv2d_max_speed(self, prm, q):
+        q, pq = _p(q)
+        sx, sy = _D(), _D()
+        self.lib.orc_fv2d_max_speed(ctypes.byref(prm), pq, ctypes.byref(sx), ctypes.byref(sy))
+        return sx.value, sy.value
+
+    def fv2d_dt(self, prm, q, cfl, t, tfinal):
+        q, pq = _p(q)
+        return self.lib.orc_fv2d_dt(ctypes.byref(prm), pq, cfl, t, tfinal)
+
+    def fv2d_rk3(self, prm, q, dt):
+        q = np.array(q, dtype=np.float64, copy=True, order="C")
+        self.lib.orc_fv2d_rk3(ctypes.byref(prm), q.ctypes.data_as(_DP), dt)
+        return q
+
+    def fv2d_advance(self, prm, q, cfl, t0, tfinal, max_steps=0):
+        q = np.array(q, dtype=np.float64, copy=True, order="C")
+        t = _D()
+        n = self.lib.orc_fv2d_advance(ctypes.byref(prm), q.ctypes.data_as(_DP), cfl, t0, tfinal, max_steps,
+                                      ctypes.byref(t))
+        return q, t.value, n
+
+    def ic_vortex(self, prm, centre=(0.0, 0.0), modes=None):
+        m = np.zeros((0, 6)) if modes is None else np.asarray(modes, dtype=np.float64).reshape(-1, 6)
+        mm = np.ascontiguousarray(m.reshape(-1))
+        q = np.empty((4, prm.ny, prm.nx))
+        self.lib.orc_ic_vortex(ctypes.byref(prm), centre[0], centre[1],
+                               mm.ctypes.data_as(_DP) if mm.size else None, m.shape[0], q.ctypes.data_as(_DP))
+        return q

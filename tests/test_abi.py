@@ -1,0 +1,77 @@
+"""The C ABI library loads and exports every symbol include/htx.h declares;
+parameter validation runs on the host and needs no GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "htx.h")
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(htx_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from htx import _lib
+
+    lib = _lib.load()
+    syms = declared_symbols()
+    assert len(syms) >= 15
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    # the ctypes table covers the same set
+    assert sorted(_lib.SIGNATURES) == syms
+
+
+def test_abi_version_and_params_struct():
+    from htx import _lib
+
+    lib = _lib.load()
+    assert lib.htx_abi_version() == 1
+    p = _lib.HtxParams()
+    assert lib.htx_params_default(ctypes.byref(p)) == 0
+    assert p.struct_size == ctypes.sizeof(_lib.HtxParams)
+    assert (p.dim, p.nx, p.ny, p.gamma, p.cfl, p.ref_quirks) == (2, 64, 64, 1.4, 0.5, 1)
+
+
+@pytest.mark.parametrize("field,value", [("nx", 0), ("ny", -3), ("gamma", 1.0), ("cfl", 0.0),
+                                         ("flux", 7), ("bc", 9), ("scheme", 5), ("struct_size", 4)])
+def test_invalid_params_rejected_without_exit(field, value):
+    """Where the reference calls exit(1) (Schemes_Hermite.cpp:16-25 on a zero
+    grid size) or prints and continues (bad dir codes), the ABI returns
+    HTX_EINVAL with a message and the process lives on."""
+    from htx import _lib
+
+    lib = _lib.load()
+    p = _lib.HtxParams()
+    lib.htx_params_default(ctypes.byref(p))
+    setattr(p, field, value)
+    h = ctypes.c_void_p()
+    rc = lib.htx_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == -1
+    assert h.value is None
+    assert lib.htx_last_error()
+
+
+def test_null_arguments():
+    from htx import _lib
+
+    lib = _lib.load()
+    assert lib.htx_create(None, None) == -1
+    assert lib.htx_step(None, None) == -1
+    assert lib.htx_sync(None) == -1
+    lib.htx_destroy(None)  # ignored
+
+
+def test_python_mirror_raises_typed_error():
+    import htx
+
+    with pytest.raises(htx.HtxError) as ei:
+        htx.System2D(0.0, 1.0, 0, 0.0, 1.0, 8, 0.5, 1.0, 0.0, 1.4)
+    assert ei.value.code == -1

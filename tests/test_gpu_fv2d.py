@@ -1,0 +1,196 @@
+"""Parity of the HIP FP64 2D TENO5 path (libhtx.so) against the FP64 CPU
+oracle on identical inputs.
+
+Tolerances (from the project's parity statement): relative L1 <= 1e-12 per
+residual / per step, <= 1e-9 at end time; the only allowed differences come
+from FMA contraction, reduction order and reciprocal rounding.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l1
+from oracle import params
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 1.4
+TOL_STEP = 1e-12
+TOL_END = 1e-9
+
+
+def make(nx, ny, bc="periodic", flux="hllc", quirks=True, dom=(0.0, 10.0, 0.0, 10.0), tmax=10.0, cfl=0.5):
+    import htx
+
+    print(f"[case] nx={nx} ny={ny} bc={bc} flux={flux} quirks={quirks}", flush=True)
+    s = htx.System2D(dom[0], dom[1], nx, dom[2], dom[3], ny, cfl, tmax, 0.0, GAMMA, flux=flux, bc=bc,
+                     ref_quirks=quirks)
+    prm = params(nx, ny, dom[0], dom[1], dom[2], dom[3], GAMMA, bc, flux, quirks)
+    return s, prm
+
+
+def quadrant_state(nx, ny, seed=3):
+    """Seeded four-quadrant Riemann data, point-sampled at cell centres."""
+    from htx.ic import SplitMix64
+
+    rng = SplitMix64(seed)
+    xs, ys = rng.uniform_in(0.3, 0.7), rng.uniform_in(0.3, 0.7)
+    states = [[rng.uniform_in(0.5, 2.0), rng.uniform_in(-0.5, 0.5), rng.uniform_in(-0.5, 0.5),
+               rng.uniform_in(0.2, 2.0)] for _ in range(4)]
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y)
+    k = (X >= xs).astype(int) + 2 * (Y >= ys).astype(int)
+    w = np.zeros((4, ny, nx))
+    for q in range(4):
+        for e in range(4):
+            w[e][k == q] = states[q][e]
+    q = np.stack([w[0], w[0] * w[1], w[0] * w[2], w[3] / (GAMMA - 1) + 0.5 * w[0] * (w[1] ** 2 + w[2] ** 2)])
+    return q
+
+
+@pytest.mark.parametrize("nx,ny", [(32, 32), (48, 40), (17, 23)])
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+@pytest.mark.parametrize("bc", ["periodic", "zero_grad"])
+def test_rhs_parity_vortex(orc, nx, ny, flux, bc):
+    from htx.ic import vortex_modes
+
+    s, prm = make(nx, ny, bc, flux)
+    q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
+    s.set_state(q)
+    r_gpu = s.rhs()
+    r_cpu = orc.fv2d_rhs(prm, q)
+    for e in range(4):
+        assert rel_l1(r_gpu[e], r_cpu[e]) < TOL_STEP, (e, rel_l1(r_gpu[e], r_cpu[e]))
+
+
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+@pytest.mark.parametrize("quirks", [True, False])
+def test_rhs_parity_riemann(orc, flux, quirks):
+    nx, ny = 40, 36
+    s, prm = make(nx, ny, "zero_grad", flux, quirks, dom=(0.0, 1.0, 0.0, 1.0))
+    q = quadrant_state(nx, ny)
+    s.set_state(q)
+    r_gpu = s.rhs()
+    r_cpu = orc.fv2d_rhs(prm, q)
+    assert rel_l1(r_gpu, r_cpu) < TOL_STEP
+
+
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+def test_step_parity_10_steps(orc, flux):
+    """Ten SSP-RK3 steps, compared after every step (dt included)."""
+    from htx.ic import vortex_modes
+
+    n = 64
+    s, prm = make(n, n, "periodic", flux)
+    q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
+    s.set_state(q)
+    qc = q.copy()
+    t = 0.0
+    for k in range(10):
+        dt_cpu = orc.fv2d_dt(prm, qc, 0.5, t, 10.0)
+        dt_gpu = s.step()
+        assert dt_gpu == pytest.approx(dt_cpu, rel=1e-13)
+        qc = orc.fv2d_rk3(prm, qc, dt_cpu)
+        t += dt_cpu
+        err = rel_l1(s.get_state(), qc)
+        print(f"step {k}: rel L1 {err:.3e}", flush=True)
+        assert err < TOL_STEP
+    assert s.time == pytest.approx(t, rel=1e-13)
+    assert s.nsteps == 10
+
+
+def test_end_time_parity_riemann(orc):
+    """Shock-bearing quadrant problem to t = 0.05 at 32^2 (zero-gradient)."""
+    n = 32
+    s, prm = make(n, n, "zero_grad", "hllc", True, dom=(0.0, 1.0, 0.0, 1.0), tmax=0.05)
+    q = quadrant_state(n, n, seed=11)
+    s.set_state(q)
+    nsteps = s.timeStepping(0.05)
+    qc, t, nc = orc.fv2d_advance(prm, q, 0.5, 0.0, 0.05)
+    assert nsteps == nc
+    assert s.time == pytest.approx(t, rel=1e-14)
+    assert rel_l1(s.get_state(), qc) < TOL_END
+
+
+def test_vortex_ic_generator_matches_oracle(orc):
+    from htx.ic import vortex_modes
+
+    n = 40
+    s, prm = make(n, n)
+    s.isentropicVortex((5.0, 5.0), vortex_modes())
+    q_gpu = s.get_state()
+    q_cpu = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
+    np.testing.assert_allclose(q_gpu, q_cpu, rtol=1e-13, atol=1e-15)
+
+
+def test_dt_parity(orc):
+    n = 24
+    s, prm = make(n, n, "zero_grad", dom=(0.0, 1.0, 0.0, 1.0))
+    q = quadrant_state(n, n, seed=5)
+    s.set_state(q)
+    assert s.compute_dt() == pytest.approx(orc.fv2d_dt(prm, q, 0.5, 0.0, 10.0), rel=1e-15)
+
+
+def test_tiny_and_ragged_grids(orc):
+    """Grids smaller than one tile and than the stencil still match."""
+    for nx, ny in [(1, 1), (3, 2), (5, 7), (16, 1)]:
+        s, prm = make(nx, ny, "zero_grad", dom=(0.0, 1.0, 0.0, 1.0))
+        q = quadrant_state(nx, ny, seed=nx + ny)
+        s.set_state(q)
+        assert rel_l1(s.rhs(), orc.fv2d_rhs(prm, q)) < TOL_STEP
+        s.close()
+
+
+def test_free_stream_exact_full_size():
+    """A uniform state stays uniform to the last bit at 1024^2 (the residual
+    is exactly zero)."""
+    import htx
+
+    n = 1024
+    s = htx.System2D(0.0, 10.0, n, 0.0, 10.0, n, 0.5, 10.0, 0.0, GAMMA, flux="hllc", bc="periodic")
+    q = np.empty((4, n, n))
+    w = [1.3, 0.4, -0.2, 0.9]
+    q0 = [w[0], w[0] * w[1], w[0] * w[2], w[3] / (GAMMA - 1) + 0.5 * w[0] * (w[1] ** 2 + w[2] ** 2)]
+    for e in range(4):
+        q[e] = q0[e]
+    s.set_state(q)
+    assert np.abs(s.rhs()).max() < 1e-13
+    s.steps_async(3)
+    out = s.get_state()
+    for e in range(4):
+        assert np.abs(out[e] - q0[e]).max() <= 1e-14 * abs(q0[e]) + 1e-15
+
+
+def test_conservation_full_size():
+    """Periodic config-2 workload at 1024^2: mass, momentum and energy are
+    conserved over 5 steps to rounding."""
+    import htx
+    from htx.ic import vortex_workload
+
+    w = vortex_workload(1024)
+    s = htx.System2D(w["xl"], w["xr"], w["nx"], w["yb"], w["yt"], w["ny"], w["cfl"], w["tmax"], 0.0, w["gamma"],
+                     flux="hllc", bc="periodic")
+    s.isentropicVortex(w["centre"], w["modes"])
+    q0 = s.get_state()
+    s.steps_async(5)
+    q1 = s.get_state()
+    for e in range(4):
+        tot0, tot1 = q0[e].sum(), q1[e].sum()
+        assert abs(tot1 - tot0) <= 1e-12 * np.abs(q0[e]).sum()
+    assert np.isfinite(q1).all()
+
+
+def test_transpose_symmetry_full_size():
+    """x <-> y mirror of a 512^2 quadrant problem gives the transposed residual."""
+    import htx
+
+    n = 512
+    q = quadrant_state(n, n, seed=21)
+    qt = np.stack([q[0].T, q[2].T, q[1].T, q[3].T]).copy()
+    s = htx.System2D(0.0, 1.0, n, 0.0, 1.0, n, 0.5, 1.0, 0.0, GAMMA, flux="hllc", bc="zero_grad")
+    s.set_state(q)
+    r = s.rhs()
+    s.set_state(qt)
+    rt = s.rhs()
+    back = np.stack([rt[0].T, rt[2].T, rt[1].T, rt[3].T])
+    assert rel_l1(back, r) < 1e-12
