@@ -1,0 +1,221 @@
+// htx_fv2d_tile.h -- the phases of the fused 2D FV stage kernel, written as
+// host+device functions of (thread id, shared tile) so that the kernel and a
+// CPU build of the same tiling (used by the tests) run identical code.
+//
+// Tile geometry (TX x TY cells, NT threads):
+//   halo tile   (TX+6) x (TY+6) cells, LDS cell (lx, ly) = global (i0-3+lx, j0-3+ly)
+//   x faces     recon (TX+1) x (TY+4): face fi sits between LDS cols 2+fi and 3+fi,
+//               row fj at LDS row 1+fj; flux (TX+1) x TY at rows 3..TY+2
+//   y faces     recon (TX+4) x (TY+1): column fi at LDS col 1+fi, face fj between
+//               LDS rows 2+fj and 3+fj; flux TX x (TY+1) at cols 3..TX+2
+#pragma once
+#include "htx_internal.h"
+
+namespace htx {
+
+template <int TX, int TY>
+struct Fv2dTile {
+    static constexpr int NT = 256;
+    static constexpr int HX = TX + 6, HY = TY + 6, HC = HX * HY;
+    static constexpr int NXF = (TX + 1) * (TY + 4);
+    static constexpr int NYF = (TX + 4) * (TY + 1);
+    static constexpr int NF = NXF > NYF ? NXF : NYF;
+    static constexpr int NXQ = (TX + 1) * TY;
+    static constexpr int NYQ = TX * (TY + 1);
+    static constexpr int NQ = NXQ > NYQ ? NXQ : NYQ;
+    static constexpr int NC = TX * TY;
+    static constexpr int KC = (NC + NT - 1) / NT;
+    double sQ[4][HC];  // conserved state
+    double sP[4][HC];  // sqrt|rho|, u, v, H
+    double sF[8][NF];  // characteristic face values (left/right per field)
+    double sFl[4][NQ]; // face fluxes
+    double sRed[2][NT / 64];
+};
+
+HTX_HD int map_index(int g, int n, int bc) {
+    if (bc == 1) {
+        g = g % n;
+        return g < 0 ? g + n : g;
+    }
+    return g < 0 ? 0 : (g >= n ? n - 1 : g);
+}
+
+// halo load with the boundary condition applied through the index, plus the
+// per-cell Roe inputs of calcPrimitiveVariable (FV_System2D.cpp:226-263)
+template <int TX, int TY>
+HTX_HD void ph_load(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int i0, int j0, int tid) {
+    using T = Fv2dTile<TX, TY>;
+    const long long fs = a.fstride;
+    for (int w = tid; w < T::HC; w += T::NT) {
+        int lx = w % T::HX, ly = w / T::HX;
+        int gi = map_index(i0 - 3 + lx, a.nx, a.bc);
+        int gj = map_index(j0 - 3 + ly, a.ny, a.bc);
+        long long off = (long long)gj * a.pitch + gi;
+        double q0 = a.qin[off], q1 = a.qin[off + fs], q2 = a.qin[off + 2 * fs], q3 = a.qin[off + 3 * fs];
+        s.sQ[0][w] = q0;
+        s.sQ[1][w] = q1;
+        s.sQ[2][w] = q2;
+        s.sQ[3][w] = q3;
+        Prim P = cell_prim(a.g1, q0, q1, q2, q3);
+        s.sP[0][w] = P.sr;
+        s.sP[1][w] = P.u;
+        s.sP[2][w] = P.v;
+        s.sP[3][w] = P.h;
+    }
+}
+
+template <int TX, int TY>
+HTX_HD Prim prim_at(const Fv2dTile<TX, TY>& s, int id) {
+    return Prim{s.sP[0][id], s.sP[1][id], s.sP[2][id], s.sP[3][id]};
+}
+
+// characteristic TENO5 on faces (FV_System2D.cpp:372-400).  YDIR selects the
+// face family; the state is read in (rho, normal, tangential, E) order.
+template <int TX, int TY, bool YDIR>
+HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
+    using T = Fv2dTile<TX, TY>;
+    constexpr int NW = YDIR ? T::NYF : T::NXF;
+    constexpr int RW = YDIR ? (TX + 4) : (TX + 1);   // faces per row of the face grid
+    constexpr int STEP = YDIR ? T::HX : 1;           // LDS stride along the normal
+    for (int w = tid; w < NW; w += T::NT) {
+        int fi = w % RW, fj = w / RW;
+        int ca = YDIR ? (2 + fj) * T::HX + (1 + fi) : (1 + fj) * T::HX + (2 + fi);
+        Eig E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
+        double c[4][6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            int id = ca + (m - 2) * STEP;
+            eig_left<YDIR>(E, s.sQ[0][id], s.sQ[1][id], s.sQ[2][id], s.sQ[3][id], c[0][m], c[1][m], c[2][m], c[3][m]);
+        }
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            double um, up;
+            teno5(c[f], um, up);
+            s.sF[2 * f][w] = um;
+            s.sF[2 * f + 1][w] = up;
+        }
+    }
+}
+
+// transverse 3-point quadrature, back-projection with the face's own R,
+// Riemann flux at the Gauss points weighted by (5/18, 4/9, 5/18)
+// (FV_System2D.cpp:402-497); the flux is stored in (rho, x, y, E) order.
+template <int TX, int TY, bool YDIR, int FLUX>
+HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
+    using T = Fv2dTile<TX, TY>;
+    constexpr int NW = YDIR ? T::NYQ : T::NXQ;
+    constexpr int RW = YDIR ? TX : (TX + 1);          // flux faces per row
+    constexpr int RR = YDIR ? (TX + 4) : (TX + 1);    // recon faces per row
+    constexpr int TS = YDIR ? 1 : (TX + 1);           // recon stride along the transverse direction
+    constexpr int STEP = YDIR ? T::HX : 1;
+    for (int w = tid; w < NW; w += T::NT) {
+        int fi = w % RW, fj = w / RW;
+        // first of the 5 transverse recon faces, and the face's left/lower cell
+        int b = YDIR ? fj * RR + fi : fj * RR + fi;
+        int ca = YDIR ? (2 + fj) * T::HX + (3 + fi) : (3 + fj) * T::HX + (2 + fi);
+        double gl[4][3], gr[4][3];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const double* L = s.sF[2 * f];
+            const double* R = s.sF[2 * f + 1];
+            weno3q(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1], gl[f][2]);
+            weno3q(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1], gr[f][2]);
+        }
+        Eig E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
+        double acc[3][4];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            double ql[4], qr[4], fl[4];
+            eig_right(E, gl[0][m], gl[1][m], gl[2][m], gl[3][m], ql[0], ql[1], ql[2], ql[3]);
+            eig_right(E, gr[0][m], gr[1][m], gr[2][m], gr[3][m], qr[0], qr[1], qr[2], qr[3]);
+            if (FLUX == FLUX_RUSANOV_CHAR)
+                rusanov_char<YDIR>(a.gam, a.g1, E, ql, qr, fl);
+            else if (FLUX == FLUX_HLLC)
+                hllc<true>(a.gam, a.g1, ql, qr, fl);
+            else
+                hllc<false>(a.gam, a.g1, ql, qr, fl);
+            const double gw = (m == 1) ? 8.0 / 9.0 / 2.0 : 5.0 / 9.0 / 2.0;
+            acc[m][0] = gw * fl[0];
+            acc[m][1] = gw * (YDIR ? fl[2] : fl[1]);
+            acc[m][2] = gw * (YDIR ? fl[1] : fl[2]);
+            acc[m][3] = gw * fl[3];
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) s.sFl[e][w] = (acc[0][e] + acc[1][e]) + acc[2][e];
+    }
+}
+
+// x part of the divergence, (fx_{i-1} - fx_i) * dy, kept per thread
+template <int TX, int TY>
+HTX_HD void ph_xdiv(const Fv2dArgs& a, const Fv2dTile<TX, TY>& s, int tid, double dxp[][4]) {
+    using T = Fv2dTile<TX, TY>;
+#pragma unroll
+    for (int k = 0; k < T::KC; ++k) {
+        int w = tid + k * T::NT;
+        if (w < T::NC) {
+            int c = w % TX, r = w / TX;
+            int b = r * (TX + 1) + c;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) dxp[k][e] = (s.sFl[e][b] - s.sFl[e][b + 1]) * a.dy;
+        }
+    }
+}
+
+// residual (FV_System2D.cpp:562-572), SSP-RK3 stage (:575-612 with
+// Schemes_SSPRK.cpp:3-15) and, in stage 3, the per-thread CFL maxima
+// (:623-631 with Equations_Euler2D.cpp:207-219)
+template <int TX, int TY>
+HTX_HD void ph_epilogue(const Fv2dArgs& a, const Fv2dTile<TX, TY>& s, int i0, int j0, int tid,
+                        const double dxp[][4], double dt, double& smx, double& smy) {
+    using T = Fv2dTile<TX, TY>;
+    const long long fs = a.fstride;
+#pragma unroll
+    for (int k = 0; k < T::KC; ++k) {
+        int w = tid + k * T::NT;
+        if (w >= T::NC) continue;
+        int c = w % TX, r = w / TX;
+        int gi = i0 + c, gj = j0 + r;
+        if (gi >= a.nx || gj >= a.ny) continue;
+        long long off = (long long)gj * a.pitch + gi;
+        int lc = (3 + r) * T::HX + (3 + c);
+        double unew[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            double rhs = (dxp[k][e] + (s.sFl[e][r * TX + c] - s.sFl[e][(r + 1) * TX + c]) * a.dx) / a.area;
+            double us = s.sQ[e][lc];
+            double v;
+            if (a.mode == 0)
+                v = rhs;
+            else if (a.mode == 1)
+                v = us + dt * rhs;
+            else if (a.mode == 2)
+                v = 0.75 * a.q0[off + e * fs] + 0.25 * (us + rhs * dt);
+            else
+                v = (a.q0[off + e * fs] + 2.0 * (us + dt * rhs)) / 3.0;
+            unew[e] = v;
+            a.qout[off + e * fs] = v;
+        }
+        if (a.mode == 3) {
+            double wu = unew[1] / unew[0], wv = unew[2] / unew[0];
+            double p = (unew[3] - 0.5 * (unew[1] * wu + unew[2] * wv)) * a.g1;
+            if (a.quirks) {
+                smx = fmax(smx, max_speed<true>(a.gam, wu, p, unew[0]));
+                smy = fmax(smy, max_speed<true>(a.gam, wv, p, unew[0]));
+            } else {
+                smx = fmax(smx, max_speed<false>(a.gam, wu, p, unew[0]));
+                smy = fmax(smy, max_speed<false>(a.gam, wv, p, unew[0]));
+            }
+        }
+    }
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin; give each XCD one
+// contiguous run of tiles so neighbouring tiles share that XCD's L2.
+HTX_HD int xcd_remap(int b, int nb) {
+    int per = nb / 8, rem = nb % 8;
+    int x = b % 8, k = b / 8;
+    int start = x * per + (x < rem ? x : rem);
+    return start + k;
+}
+
+}  // namespace htx

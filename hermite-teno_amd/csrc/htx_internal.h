@@ -52,6 +52,5 @@ hipError_t launch_reset_smax(Ctl* c, hipStream_t s);
 hipError_t launch_max_speed(const double* q, int nx, int ny, int pitch, long long fstride, double gam,
                             int quirks, Ctl* c, hipStream_t s);
 hipError_t launch_ic_vortex(const VortexArgs& a, hipStream_t s);
-void fill_quad_consts(QuadConsts& C);
 
 }  // namespace htx

@@ -9,6 +9,17 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// Host+device so that the same numerics can be exercised by CPU builds.
+#define HTX_HD __host__ __device__ __forceinline__
+
+// The reconstruction chain (cell primitives, Roe average, characteristic
+// projection, TENO5) is evaluated without FMA contraction, in the reference's
+// operation order.  TENO5's full-stencil indicator sums terms of size ~1e5 u^2
+// down to ~beta, so a one-ulp change in any input moves the cut-off decision
+// near its threshold; keeping this chain uncontracted makes the FP64 face
+// states bit-identical to an IEEE evaluation of the reference formulas.
+#define HTX_NO_CONTRACT _Pragma("clang fp contract(off)")
+
 namespace htx {
 
 // Runtime-computed coefficients of Schemes::Lagrange that involve sqrt(15)
@@ -21,6 +32,49 @@ struct QuadConsts {
     double sumw0p, sumw0m;
 };
 
+inline void fill_quad_consts(QuadConsts& C) {
+    // Schemes_Lagrange.cpp:64-105, same expressions, evaluated on the host.
+    double sq15 = sqrt(15.0);
+    C.gp[0][0] = 1.0 / 30.0 + sq15 / 20.0;
+    C.gp[1][0] = -1.0 / 15.0 - sq15 / 5.0;
+    C.gp[2][0] = 31.0 / 30.0 + 3.0 * sq15 / 20.0;
+    C.gp[0][1] = 1.0 / 30.0 - sq15 / 20.0;
+    C.gp[1][1] = 14.0 / 15.0;
+    C.gp[2][1] = 1.0 / 30.0 + sq15 / 20.0;
+    C.gp[0][2] = 31.0 / 30.0 - 3.0 * sq15 / 20.0;
+    C.gp[1][2] = -1.0 / 15.0 + sq15 / 5.0;
+    C.gp[2][2] = 1.0 / 30.0 - sq15 / 20.0;
+    C.gm[0][0] = 1.0 / 30.0 - sq15 / 20.0;
+    C.gm[1][0] = -1.0 / 15.0 + sq15 / 5.0;
+    C.gm[2][0] = 31.0 / 30.0 - 3.0 * sq15 / 20.0;
+    C.gm[0][1] = 1.0 / 30.0 + sq15 / 20.0;
+    C.gm[1][1] = 14.0 / 15.0;
+    C.gm[2][1] = 1.0 / 30.0 - sq15 / 20.0;
+    C.gm[0][2] = 31.0 / 30.0 + 3.0 * sq15 / 20.0;
+    C.gm[1][2] = -1.0 / 15.0 - sq15 / 5.0;
+    C.gm[2][2] = 1.0 / 30.0 + sq15 / 20.0;
+    double lin0[3] = {-9.0 / 80.0, 49.0 / 40.0, -9.0 / 80.0};
+    double p[3], m[3];
+    for (int i = 0; i < 3; ++i) {
+        p[i] = 0.5 * (lin0[i] + 3.0 * fabs(lin0[i]));
+        m[i] = p[i] - lin0[i];
+    }
+    C.sumw0p = (p[0] + p[1]) + p[2];
+    C.sumw0m = (m[0] + m[1]) + m[2];
+    for (int i = 0; i < 3; ++i) {
+        C.linw0p[i] = p[i] / C.sumw0p;
+        C.linw0m[i] = m[i] / C.sumw0m;
+    }
+    double den = 15720.0 * sq15 + 10480.0;
+    C.linwp[0] = (2882.0 * sq15 - 1179.0) / den;
+    C.linwp[1] = (9672.0 * sq15 + 6448.0) / den;
+    C.linwp[2] = (3166.0 * sq15 + 5211.0) / den;
+    den = 15720.0 * sq15 - 10480;
+    C.linwm[0] = (2882.0 * sq15 + 1179.0) / den;
+    C.linwm[1] = (9672.0 * sq15 - 6448.0) / den;
+    C.linwm[2] = (3166.0 * sq15 - 5211.0) / den;
+}
+
 struct Prim {  // per-cell quantities used by the Roe average
     double sr, u, v, h;
 };
@@ -30,9 +84,10 @@ struct Prim {  // per-cell quantities used by the Roe average
 // :1141-1149, catchnan :1151-1153).  v0..v4 = 5-point window, p = the three
 // candidate values, d = linear weights.  Returns the TENO value; *cut gets a
 // bitmask of discarded stencils.
-__device__ __forceinline__ double teno5_side(double v0, double v1, double v2, double v3, double v4,
+HTX_HD double teno5_side(double v0, double v1, double v2, double v3, double v4,
                                              double p0, double p1, double p2, double d0, double d1,
                                              double d2, unsigned* cut) {
+    HTX_NO_CONTRACT
     const double c1 = 13.0 / 12.0;
     double c_0 = v2 - 2.0 * v3 + v4, b_0 = (3.0 * v2 - 4.0 * v3 + v4) / 2.0;
     double c_1 = v1 - 2.0 * v2 + v3, b_1 = (v1 - v3) / 2.0;
@@ -60,12 +115,13 @@ __device__ __forceinline__ double teno5_side(double v0, double v1, double v2, do
     w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
     double out = (p0 * w0 + p1 * w1) + p2 * w2;
     if (cut) *cut = (k0 ? 0u : 1u) | (k1 ? 0u : 2u) | (k2 ? 0u : 4u);
-    return isnan(out) ? 1e-15 : out;
+    return out != out ? 1e-15 : out;  // catchnan
 }
 
 // Schemes::Lagrange::teno5 (Schemes_Lagrange.cpp:481-511): u[0..5] =
 // u_{i-2..i+3} -> (u^-_{i+1/2}, u^+_{i+1/2}).
-__device__ __forceinline__ void teno5(const double u[6], double& um, double& up, unsigned* cut = nullptr) {
+HTX_HD void teno5(const double u[6], double& um, double& up, unsigned* cut = nullptr) {
+    HTX_NO_CONTRACT
     const double a = 1.0 / 3.0, b = 5.0 / 6.0, c = -1.0 / 6.0, d = 11.0 / 6.0, e = -7.0 / 6.0;
     // crj columns: col0 = (d, e, a), col1 = (a, b, c), col2 = (c, b, a), col3 = (a, e, d)
     double p0 = (a * u[2] + b * u[3]) + c * u[4];
@@ -83,7 +139,7 @@ __device__ __forceinline__ void teno5(const double u[6], double& um, double& up,
 // Schemes::Lagrange::weno5JS3PointsQuadrature + smoothnessMeasure3L
 // (Schemes_Lagrange.cpp:1155-1217): 5 transverse values -> values at the
 // Gauss points (+xi, 0, -xi).
-__device__ __forceinline__ void weno3q(const QuadConsts& C, double u0, double u1, double u2, double u3,
+HTX_HD void weno3q(const QuadConsts& C, double u0, double u1, double u2, double u3,
                                        double u4, double& o0, double& o1, double& o2) {
     const double c1 = 13.0 / 12.0;
     double bg0 = u0 - 4.0 * u1 + 3.0 * u2, cg0 = u0 - 2.0 * u1 + u2;
@@ -123,13 +179,14 @@ __device__ __forceinline__ void weno3q(const QuadConsts& C, double u0, double u1
     ws = (w0 + w1) + w2;
     w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
     o2 = (p0 * w0 + p1 * w1) + p2 * w2;
-    o0 = isnan(o0) ? 1e-15 : o0;
-    o1 = isnan(o1) ? 1e-15 : o1;
-    o2 = isnan(o2) ? 1e-15 : o2;
+    o0 = o0 != o0 ? 1e-15 : o0;  // catchnan
+    o1 = o1 != o1 ? 1e-15 : o1;
+    o2 = o2 != o2 ? 1e-15 : o2;
 }
 
 // Per-cell quantities of calcPrimitiveVariable (FV_System2D.cpp:226-263).
-__device__ __forceinline__ Prim cell_prim(double g1, double q0, double q1, double q2, double q3) {
+HTX_HD Prim cell_prim(double g1, double q0, double q1, double q2, double q3) {
+    HTX_NO_CONTRACT
     Prim P;
     double u = q1 / q0, v = q2 / q0;
     double p = (q3 - 0.5 * (q1 * u + q2 * v)) * g1;
@@ -147,7 +204,8 @@ struct Eig {
     double vn, vt, hm, qm, cm, rcm, b1, b2, t2;  // vn: normal, vt: tangential velocity
 };
 
-__device__ __forceinline__ Eig roe_eig(double g1, bool ydir, const Prim& a, const Prim& b) {
+HTX_HD Eig roe_eig(double g1, bool ydir, const Prim& a, const Prim& b) {
+    HTX_NO_CONTRACT
     Eig E;
     double t0 = a.sr / (a.sr + b.sr);
     double t1 = 1.0 - t0;
@@ -165,21 +223,34 @@ __device__ __forceinline__ Eig roe_eig(double g1, bool ydir, const Prim& a, cons
     return E;
 }
 
-// Characteristic projection L*q.  Components are ordered (rho, normal
-// momentum, tangential momentum, E) via (qn, qt); the x and y left
-// eigenvector matrices are the same matrix in that ordering.  Products with
-// the matrices' exact 0 and 1 entries are not formed.
-__device__ __forceinline__ void eig_left(const Eig& E, double q0, double qn, double qt, double q3,
-                                         double& c0, double& c1, double& c2, double& c3) {
+// Characteristic projection L*q of a state given in (rho, rho u, rho v, E)
+// order.  The x and y left eigenvector matrices are one matrix once the
+// momenta are read as (normal, tangential); the sums run over the natural
+// component order, as Eigen's row-times-vector does.  Products with the
+// matrices' exact 0 and 1 entries are not formed.
+template <bool YDIR>
+HTX_HD void eig_left(const Eig& E, double q0, double qx, double qy, double q3, double& c0, double& c1,
+                     double& c2, double& c3) {
+    HTX_NO_CONTRACT
     double tr = E.vn * E.rcm, tb = E.b1 * E.vn;
-    c0 = ((0.5 * (E.b2 + tr) * q0 + (-0.5 * (tb + E.rcm)) * qn) + (-E.t2 * E.vt) * qt) + E.t2 * q3;
-    c1 = -E.vt * q0 + qt;
-    c2 = (((1.0 - E.b2) * q0 + tb * qn) + (E.b1 * E.vt) * qt) + (-E.b1) * q3;
-    c3 = ((0.5 * (E.b2 - tr) * q0 + (-0.5 * (tb - E.rcm)) * qn) + (-E.t2 * E.vt) * qt) + E.t2 * q3;
+    double ln0 = -0.5 * (tb + E.rcm), lt0 = -E.t2 * E.vt;  // row 0: normal / tangential entries
+    double ln3 = -0.5 * (tb - E.rcm);                      // row 3 normal entry (tangential = lt0)
+    double lt2 = E.b1 * E.vt;                              // row 2 tangential entry (normal = tb)
+    if (!YDIR) {
+        c0 = ((0.5 * (E.b2 + tr) * q0 + ln0 * qx) + lt0 * qy) + E.t2 * q3;
+        c1 = -E.vt * q0 + qy;
+        c2 = (((1.0 - E.b2) * q0 + tb * qx) + lt2 * qy) + (-E.b1) * q3;
+        c3 = ((0.5 * (E.b2 - tr) * q0 + ln3 * qx) + lt0 * qy) + E.t2 * q3;
+    } else {
+        c0 = ((0.5 * (E.b2 + tr) * q0 + lt0 * qx) + ln0 * qy) + E.t2 * q3;
+        c1 = -E.vt * q0 + qx;
+        c2 = (((1.0 - E.b2) * q0 + lt2 * qx) + tb * qy) + (-E.b1) * q3;
+        c3 = ((0.5 * (E.b2 - tr) * q0 + lt0 * qx) + ln3 * qy) + E.t2 * q3;
+    }
 }
 
 // Back-projection R*c, returned in (rho, normal, tangential, E) ordering.
-__device__ __forceinline__ void eig_right(const Eig& E, double c0, double c1, double c2, double c3,
+HTX_HD void eig_right(const Eig& E, double c0, double c1, double c2, double c3,
                                           double& q0, double& qn, double& qt, double& q3) {
     double tc = E.vn * E.cm;
     q0 = (c0 + c2) + c3;
@@ -189,7 +260,7 @@ __device__ __forceinline__ void eig_right(const Eig& E, double c0, double c1, do
 }
 
 // Physical normal flux (Equations_Euler2D.cpp:36-50) in (rho, n, t, E) order.
-__device__ __forceinline__ void phys_flux(double g1, double q0, double qn, double qt, double q3,
+HTX_HD void phys_flux(double g1, double q0, double qn, double qt, double q3,
                                           double& f0, double& fn, double& ft, double& f3, double& wn,
                                           double& p, double& rho) {
     rho = q0;
@@ -203,8 +274,9 @@ __device__ __forceinline__ void phys_flux(double g1, double q0, double qn, doubl
 }
 
 // maxLocalWaveSpeed (Equations_Euler2D.cpp:221-246) + characteristic Rusanov
-// (:310-330), all in (rho, n, t, E) ordering.
-__device__ __forceinline__ void rusanov_char(double gam, double g1, const Eig& E, const double ql[4],
+// (:310-330); states and flux in (rho, n, t, E) ordering.
+template <bool YDIR>
+HTX_HD void rusanov_char(double gam, double g1, const Eig& E, const double ql[4],
                                              const double qr[4], double f[4]) {
     double fl0, fln, flt, fl3, wnl, pl, rl;
     double fr0, frn, frt, fr3, wnr, pr, rr;
@@ -216,7 +288,10 @@ __device__ __forceinline__ void rusanov_char(double gam, double g1, const Eig& E
     double a1 = fmax(fabs(wnl), fabs(wnr));
     double a3 = fmax(fabs(wnl + cl), fabs(wnr + cr));
     double d0, d1, d2, d3;
-    eig_left(E, qr[0] - ql[0], qr[1] - ql[1], qr[2] - ql[2], qr[3] - ql[3], d0, d1, d2, d3);
+    if (YDIR)
+        eig_left<true>(E, qr[0] - ql[0], qr[2] - ql[2], qr[1] - ql[1], qr[3] - ql[3], d0, d1, d2, d3);
+    else
+        eig_left<false>(E, qr[0] - ql[0], qr[1] - ql[1], qr[2] - ql[2], qr[3] - ql[3], d0, d1, d2, d3);
     d0 *= a0; d1 *= a1; d2 *= a1; d3 *= a3;
     double r0, rn, rt, r3;
     eig_right(E, d0, d1, d2, d3, r0, rn, rt, r3);
@@ -230,7 +305,7 @@ __device__ __forceinline__ void rusanov_char(double gam, double g1, const Eig& E
 // ordering.  QUIRKS keeps c = sqrt(|p/rho|) (:393-394) and p_L in the right
 // star energy (:423).
 template <bool QUIRKS>
-__device__ __forceinline__ void hllc(double gam, double g1, const double ql[4], const double qr[4],
+HTX_HD void hllc(double gam, double g1, const double ql[4], const double qr[4],
                                      double f[4]) {
     double wl0 = ql[0], wln = ql[1] / ql[0], wlt = ql[2] / ql[0];
     double wr0 = qr[0], wrn = qr[1] / qr[0], wrt = qr[2] / qr[0];
@@ -290,7 +365,7 @@ __device__ __forceinline__ void hllc(double gam, double g1, const double ql[4], 
 // on u - c (:218).  Negative/NaN results are clamped to 0, which is what the
 // reference's fmax(0, .) accumulation does with them.
 template <bool QUIRKS>
-__device__ __forceinline__ double max_speed(double gam, double wn, double p, double rho) {
+HTX_HD double max_speed(double gam, double wn, double p, double rho) {
     double c = sqrt(fabs(gam * p / rho));
     double s = QUIRKS ? fmax(wn - c, fabs(wn + c)) : fmax(fabs(wn - c), fabs(wn + c));
     return s > 0.0 ? s : 0.0;

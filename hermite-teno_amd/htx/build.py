@@ -6,6 +6,8 @@ Everything builds in-tree with plain hipcc / gcc command lines:
   oracle/liboracle.so                the FP64 checker (-O2, no FP contraction)
   oracle/liboracle_omp.so            the same source, OpenMP, -O3, for the
                                      CPU baseline timing
+  tests/host/libfv2d_host.so         CPU build of the stage kernel's tiling
+                                     (test infrastructure)
 """
 import os
 import subprocess
@@ -28,6 +30,9 @@ HIPCC_FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-shared",
+    # contract a*b+c into FMA, except where a function asks not to (the
+    # reconstruction chain, see HTX_NO_CONTRACT in htx_numerics.h)
+    "-ffp-contract=fast-honor-pragmas",
     # packed FP32 instructions stay off: some of their forms have produced
     # silently wrong results on this hardware pool
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -63,12 +68,25 @@ def build_oracle():
     return LIBORACLE
 
 
+HOST_HARNESS = os.path.join(ROOT, "tests", "host", "libfv2d_host.so")
+
+
+def build_host_harness():
+    """CPU build of the stage kernel's tiling (tests/host/fv2d_host.hip), with
+    contraction off so it can be compared bit for bit with the oracle."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    src = os.path.join(ROOT, "tests", "host", "fv2d_host.hip")
+    _run([hipcc, "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-o", HOST_HARNESS, src])
+    return HOST_HARNESS
+
+
 def build_all():
     build_oracle()
+    build_host_harness()
     build_htx()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "htx": build_htx, "oracle": build_oracle}[w]()
+        {"all": build_all, "htx": build_htx, "oracle": build_oracle, "harness": build_host_harness}[w]()

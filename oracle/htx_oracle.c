@@ -494,7 +494,7 @@ typedef struct {
 static void pad_alloc(padded* P, int nx, int ny, int k, int nf) {
     P->nx = nx; P->ny = ny; P->k = k;
     P->nxx = nx + 2 * k; P->nyy = ny + 2 * k;
-    for (int f = 0; f < nf; ++f) P->q[f] = (double*)malloc(sizeof(double) * (size_t)P->nxx * P->nyy);
+    for (int f = 0; f < nf; ++f) P->q[f] = (double*)calloc((size_t)P->nxx * P->nyy, sizeof(double));
 }
 static void pad_free(padded* P, int nf) {
     for (int f = 0; f < nf; ++f) free(P->q[f]);
