@@ -80,13 +80,28 @@ def build_host_harness():
     return HOST_HARNESS
 
 
+EXAMPLES = os.path.join(os.path.dirname(PKG_DIR), "examples")
+EXAMPLE_BIN = os.path.join(EXAMPLES, "build", "main_fv2d")
+
+
+def build_examples():
+    """The main.cpp-style C++ driver over the facade headers (C++11, g++)."""
+    os.makedirs(os.path.dirname(EXAMPLE_BIN), exist_ok=True)
+    _run(["g++", "-std=c++11", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", EXAMPLE_BIN,
+          os.path.join(EXAMPLES, "main_fv2d.cpp"), "-L", PKG_DIR, "-lhtx", "-Wl,-rpath," + PKG_DIR,
+          "-Wl,-rpath,$ORIGIN/../../htx"])
+    return EXAMPLE_BIN
+
+
 def build_all():
     build_oracle()
     build_host_harness()
     build_htx()
+    build_examples()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "htx": build_htx, "oracle": build_oracle, "harness": build_host_harness}[w]()
+        {"all": build_all, "htx": build_htx, "oracle": build_oracle, "harness": build_host_harness,
+         "examples": build_examples}[w]()

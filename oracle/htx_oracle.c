@@ -488,7 +488,7 @@ void orc_rk3_combine(int stage, const double* q0, const double* qs, const double
 
 typedef struct {
     int nx, ny, k, nxx, nyy;
-    double *q[4];   /* ghost-padded state, idx = j*nxx + i */
+    double *q[12];  /* ghost-padded fields, idx = j*nxx + i */
 } padded;
 
 static void pad_alloc(padded* P, int nx, int ny, int k, int nf) {
@@ -767,4 +767,753 @@ void orc_ic_vortex(const orc_params* p, double cx, double cy, const double* mode
                 }
             for (int e = 0; e < 4; ++e) q[e * n + (size_t)j * nx + i] = acc[e];
         }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Schemes::Hermite (Schemes_Hermite.cpp)                                    */
+/* ------------------------------------------------------------------------ */
+
+/* coefficients of setGridSize (Schemes_Hermite.cpp:31-66) for grid size dif */
+typedef struct {
+    double crj[3][3], crjd[4][3];
+    double dkm[3], dkp[3], dkmd[3], dkpd[3];
+    double dif;
+} herm_consts;
+
+static void herm_init(herm_consts* H, double dif) {
+    H->dif = dif;
+    H->crj[0][0] = -7.0 / 6.0; H->crj[1][0] = 13.0 / 6.0; H->crj[2][0] = -2.0 / 3.0 * dif;
+    H->crj[0][1] = 1.0 / 6.0;  H->crj[1][1] = 5.0 / 6.0;  H->crj[2][1] = -1.0 / 3.0 * dif;
+    H->crj[0][2] = -1.0 / 6.0; H->crj[1][2] = 5.0 / 6.0;  H->crj[2][2] = 1.0 / 3.0;
+    H->crjd[0][0] = 4.0 / dif;  H->crjd[1][0] = -4.0 / dif; H->crjd[2][0] = 1.5;        H->crjd[3][0] = 3.5;
+    H->crjd[0][1] = -2.0 / dif; H->crjd[1][1] = 2.0 / dif;  H->crjd[2][1] = -0.5;       H->crjd[3][1] = -0.5;
+    H->crjd[0][2] = 0.25 / dif; H->crjd[1][2] = -1.0 / dif; H->crjd[2][2] = 0.75 / dif; H->crjd[3][2] = 0.5;
+    H->dkm[0] = 9.0 / 80.0; H->dkm[1] = 42.0 / 80.0; H->dkm[2] = 29.0 / 80.0;
+    H->dkp[0] = H->dkm[1]; H->dkp[1] = H->dkm[0]; H->dkp[2] = H->dkm[2];
+    H->dkmd[0] = 1.0 / 18.0; H->dkmd[1] = 5.0 / 6.0; H->dkmd[2] = 1.0 / 9.0;
+    H->dkpd[0] = H->dkmd[1]; H->dkpd[1] = H->dkmd[0]; H->dkpd[2] = H->dkmd[2];
+}
+
+/* Hermite::smoothnessMeasure, Schemes_Hermite.cpp:515-526 */
+static void herm_si(const double* u, const double* v, double SI[3]) {
+    double b[3], c[3];
+    b[0] = -2.0 * (u[0] - u[1]) - v[0];
+    c[0] = -2.0 * (u[0] - u[1] + v[0]);
+    b[1] = -2.0 * (u[1] - u[2]) - v[2];
+    c[1] = 2.0 * (u[1] - u[2]) + 2.0 * v[2];
+    b[2] = (-u[0] + u[2]) / 2.0;
+    c[2] = u[0] - 2.0 * u[1] + u[2];
+    for (int k = 0; k < 3; ++k) SI[k] = LC.c1 * (c[k] * c[k]) + b[k] * b[k];
+}
+
+/* Hermite::smoothnessMeasureDerivative, :528-541 */
+static void herm_sid(const double* u, const double* d, double SI[3]) {
+    double b[3], c[3];
+    b[0] = 6.0 * (u[0] - u[1]) + (2.0 * d[0] + 4.0 * d[1]);
+    c[0] = 12.0 * (u[0] - u[1]) + 6.0 * (d[0] + d[1]);
+    b[1] = 6.0 * (u[2] - u[1]) - (4.0 * d[1] + 2.0 * d[2]);
+    c[1] = 12.0 * (u[1] - u[2]) + 6.0 * (d[1] + d[2]);
+    b[2] = u[0] - 2.0 * u[1] + u[2];
+    c[2] = 3.0 * (u[2] - u[0]) - 6.0 * d[1];
+    for (int k = 0; k < 3; ++k) SI[k] = LC.c1 * (c[k] * c[k]) + b[k] * b[k];
+}
+
+/* Hermite::fullPointsSI, :597-603 */
+static double herm_brk(const double* u, const double* d) {
+    return fabs((1029270.0 * u[2] - 2129184.0 * u[1] + 1099914.0 * u[0] + 281121.0 * d[0] - 494958.0 * d[2]) * d[0] +
+                (-1029270.0 * u[0] + 2129184.0 * u[1] - 1099914.0 * u[2] + 281121.0 * d[2]) * d[2] +
+                (1099445.0 * u[2] - 4317056.0 * u[1] + 2118166.0 * u[0]) * u[2] +
+                4317056.0 * (u[1] - u[0]) * u[1] + 1099445.0 * u[0] * u[0]) /
+           6720.0;
+}
+
+/* Hermite::fullPointsSId, :605-612 */
+static double herm_brkd(const double* u, const double* d) {
+    return fabs((3156862.0 * d[2] + 13189240.0 * d[1] + 1710479.0 * d[0] - 9617754.0 * u[2] - 531552.0 * u[1] +
+                 10149306.0 * u[0]) * d[0] +
+                (13189240.0 * d[2] + 26511200.0 * d[1] - 39700440.0 * u[2] + 39700440.0 * u[0]) * d[1] +
+                (1710479.0 * d[2] - 10149306.0 * u[2] + 531552.0 * u[1] + 9617754.0 * u[0]) * d[2] +
+                (15136011.0 * u[2] - 1076544.0 * u[1] - 29195478.0 * u[0]) * u[2] +
+                1076544.0 * (u[1] - u[0]) * u[1] + 15136011.0 * u[0] * u[0]) /
+           1680.0;
+}
+
+/* one HTENO side: tau from the full-stencil indicator, gamma^6, cutOff at
+ * C_T = 1e-3 (:614-628), weights, catchnan */
+static double hteno_side(double brk, const double SI[3], const double p[3], const double d[3]) {
+    double w[3], sigm[3];
+    double tau = fabs(brk - 0.25 * (SI[0] + SI[1] + 2.0 * SI[2]));
+    for (int k = 0; k < 3; ++k) w[k] = pow(1.0 + tau / (SI[k] + 1E-40), 6);
+    double s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    for (int k = 0; k < 3; ++k) sigm[k] = w[k] < 1E-3 ? 0.0 : 1.0;
+    for (int k = 0; k < 3; ++k) w[k] = sigm[k] * d[k];
+    s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    return catchnan(dot3(p, w));
+}
+
+/* Hermite::hteno, Schemes_Hermite.cpp:295-379: cell averages u[0..3] =
+ * u_{i-1..i+2} and derivative moments d[0..3] -> (u^-, u^+, d^-, d^+) at
+ * x_{i+1/2}. */
+static void hteno_impl(const herm_consts* H, const double u[4], const double d[4], double out[4]) {
+    double ud[4], p[3], SI[3];
+    for (int k = 0; k < 4; ++k) ud[k] = d[k] * H->dif;
+    /* u^- :304-320 */
+    p[0] = H->crj[0][0] * u[0] + H->crj[1][0] * u[1] + H->crj[2][0] * d[0];
+    p[1] = H->crj[0][1] * u[1] + H->crj[1][1] * u[2] + H->crj[2][1] * d[2];
+    p[2] = H->crj[0][2] * u[0] + H->crj[1][2] * u[1] + H->crj[2][2] * u[2];
+    herm_si(u, ud, SI);
+    out[0] = hteno_side(herm_brk(u, ud), SI, p, H->dkm);
+    /* d^- :323-339 */
+    p[0] = H->crjd[0][0] * u[0] + H->crjd[1][0] * u[1] + H->crjd[2][0] * d[0] + H->crjd[3][0] * d[1];
+    p[1] = H->crjd[0][1] * u[1] + H->crjd[1][1] * u[2] + H->crjd[2][1] * d[1] + H->crjd[3][1] * d[2];
+    p[2] = H->crjd[0][2] * u[0] + H->crjd[1][2] * u[1] + H->crjd[2][2] * u[2] + H->crjd[3][2] * d[1];
+    herm_sid(u, ud, SI);
+    out[2] = hteno_side(herm_brkd(u, ud), SI, p, H->dkmd);
+    /* u^+ :342-357 */
+    p[0] = H->crj[1][1] * u[1] + H->crj[0][1] * u[2] - H->crj[2][1] * d[1];
+    p[1] = H->crj[1][0] * u[2] + H->crj[0][0] * u[3] - H->crj[2][0] * d[3];
+    p[2] = H->crj[2][2] * u[1] + H->crj[1][2] * u[2] + H->crj[0][2] * u[3];
+    herm_si(u + 1, ud + 1, SI);
+    out[1] = hteno_side(herm_brk(u + 1, ud + 1), SI, p, H->dkp);
+    /* d^+ :360-376 */
+    p[0] = H->crjd[0][1] * u[1] + H->crjd[1][1] * u[2] + H->crjd[3][1] * d[1] + H->crjd[2][1] * d[2];
+    p[1] = H->crjd[0][0] * u[2] + H->crjd[1][0] * u[3] + H->crjd[3][0] * d[2] + H->crjd[2][0] * d[3];
+    p[2] = -(H->crjd[2][2] * u[1] + H->crjd[1][2] * u[2] + H->crjd[0][2] * u[3]) + H->crjd[3][2] * d[2];
+    herm_sid(u + 1, ud + 1, SI);
+    out[3] = hteno_side(herm_brkd(u + 1, ud + 1), SI, p, H->dkpd);
+}
+
+void orc_hteno(double dif, const double u[4], const double d[4], double out[4]) {
+    herm_consts H;
+    lag_init();
+    herm_init(&H, dif);
+    hteno_impl(&H, u, d, out);
+}
+
+/* Hermite::hermiteLimiter, Schemes_Hermite.cpp:634-662 */
+static double herm_limiter(double dif, const double u[3], const double d[3]) {
+    double p[3], lin[3], sm[3], w[3], ud[3];
+    p[0] = 0.75 * (u[2] - u[0]) / dif - 0.25 * (d[0] + d[2]);
+    p[1] = (u[1] - u[0]) / dif;
+    p[2] = (u[2] - u[1]) / dif;
+    lin[0] = 1.0 - 2E-5; lin[1] = 1E-5; lin[2] = 1E-5;
+    for (int k = 0; k < 3; ++k) ud[k] = d[k] * dif;
+    sm[0] = herm_brk(u, ud);
+    sm[1] = (u[0] - u[1]) * (u[0] - u[1]);
+    sm[2] = (u[1] - u[2]) * (u[1] - u[2]);
+    double a = fabs(sm[0] - sm[1]) + fabs(sm[0] - sm[2]);
+    double tau = a * a / 4.0;
+    for (int k = 0; k < 3; ++k) w[k] = lin[k] * (1.0 + tau / (sm[k] + 1E-6));
+    double s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    return w[0] * (p[0] / lin[0] - ((lin[1] / lin[0]) * p[1] + (lin[2] / lin[0]) * p[2])) + (w[1] * p[1] + w[2] * p[2]);
+}
+
+double orc_hermite_limiter(double dif, const double u[3], const double d[3]) {
+    lag_init();
+    return herm_limiter(dif, u, d);
+}
+
+/* Euler2D::dfluxx / dfluxy, Equations_Euler2D.cpp:52-98 (row-major) */
+void orc_dflux(double gamma, int dir, const double q[4], const double w[4], double M[16]) {
+    double g1 = gamma - 1.0, g3 = gamma - 3.0;
+    if (dir == 1) {
+        double sq = w[1] * w[1];
+        double dyn = 0.5 * g1 * (sq + w[2] * w[2]);
+        double h = (q[3] + w[3]) / w[0];
+        double m[16] = {0.0, 1.0, 0.0, 0.0,
+                        dyn - sq, -g3 * w[1], -g1 * w[2], g1,
+                        -w[1] * w[2], w[2], w[1], 0.0,
+                        (dyn - h) * w[1], h - g1 * sq, -g1 * w[1] * w[2], gamma * w[1]};
+        memcpy(M, m, sizeof m);
+    } else {
+        double sq = w[2] * w[2];
+        double dyn = 0.5 * g1 * (sq + w[1] * w[1]);
+        double h = (q[3] + w[3]) / w[0];
+        double m[16] = {0.0, 0.0, 1.0, 0.0,
+                        -w[1] * w[2], w[2], w[1], 0.0,
+                        dyn - sq, -g1 * w[1], -g3 * w[2], g1,
+                        (dyn - h) * w[2], -g1 * w[1] * w[2], h - g1 * sq, gamma * w[2]};
+        memcpy(M, m, sizeof m);
+    }
+}
+
+/* Euler2D::dRusanov, Equations_Euler2D.cpp:352-360:
+ * 0.5 (Jl dql + Jr dqr - R diag(a) L (dqr - dql)) */
+static void drusanov(const double dql[4], const double dqr[4], const double L[16], const double R[16],
+                     const double Jl[16], const double Jr[16], const double a[4], double f[4]) {
+    double t1[4], t2[4], dd[4], ld[4], rd[4];
+    matvec4(Jl, dql, t1);
+    matvec4(Jr, dqr, t2);
+    for (int k = 0; k < 4; ++k) dd[k] = dqr[k] - dql[k];
+    matvec4(L, dd, ld);
+    for (int k = 0; k < 4; ++k) ld[k] = a[k] * ld[k];
+    matvec4(R, ld, rd);
+    for (int k = 0; k < 4; ++k) f[k] = 0.5 * ((t1[k] + t2[k]) - rd[k]);
+}
+
+static void max_local_wave(double gamma, int dir, const double wl[4], const double wr[4], double a[4]) {
+    double cl = sqrt(fabs(gamma * wl[3] / wl[0]));
+    double cr = sqrt(fabs(gamma * wr[3] / wr[0]));
+    double vl = wl[dir], vr = wr[dir];
+    a[0] = fmax(fabs(vl - cl), fabs(vr - cr));
+    a[1] = fmax(fabs(vl), fabs(vr));
+    a[2] = a[1];
+    a[3] = fmax(fabs(vl + cl), fabs(vr + cr));
+}
+
+/* ------------------------------------------------------------------------ */
+/* FVHermite::System2D (FVHermite_System2D.cpp)                              */
+/* ------------------------------------------------------------------------ */
+#define FVH_K 8 /* k = 5 + 3, FVHermite_System2D.cpp:20-23 */
+#define FVH_S 2 /* s = (5 - 1) / 2, computed before k += 3 */
+
+/* limitDerivatives, FVHermite_System2D.cpp:393-427, single-threaded order:
+ * the x sweep replaces v in place with i ascending (reading the already
+ * updated v_{i-1}), the y sweep replaces w with j ascending. */
+static void fvh_limit(const padded* P, double** u, double** v, double** w, double dif) {
+    int k = P->k, nxx = P->nxx, nyy = P->nyy;
+    for (int i = k; i < nxx - k + 1; ++i)
+        for (int j = 0; j < nyy; ++j)
+            for (int e = 0; e < 4; ++e) {
+                double uu[3], vv[3];
+                for (int m = 0; m < 3; ++m) {
+                    uu[m] = u[e][PIX(P, i - 1 + m, j)];
+                    vv[m] = v[e][PIX(P, i - 1 + m, j)];
+                }
+                v[e][PIX(P, i, j)] = herm_limiter(dif, uu, vv);
+            }
+    for (int i = 0; i < nxx; ++i)
+        for (int j = k; j < nyy - k + 1; ++j)
+            for (int e = 0; e < 4; ++e) {
+                double uu[3], ww[3];
+                for (int m = 0; m < 3; ++m) {
+                    uu[m] = u[e][PIX(P, i, j - 1 + m)];
+                    ww[m] = w[e][PIX(P, i, j - 1 + m)];
+                }
+                w[e][PIX(P, i, j)] = herm_limiter(dif, uu, ww);
+            }
+}
+
+/* FVHermite::System2D::rhs, FVHermite_System2D.cpp:429-731.  q holds 12
+ * fields: u (4), v = <dq/dx> (4), w = <dq/dy> (4). */
+void orc_fvh2d_rhs(const orc_params* p, const double* q, double* rhs) {
+    lag_init();
+    padded P;
+    int nx = p->nx, ny = p->ny, k = FVH_K, s = FVH_S;
+    double gamma = p->gamma;
+    double dx = (p->xr - p->xl) / nx, dy = (p->yt - p->yb) / ny, area = dx * dy;
+    double gw[3] = {5.0 / 9.0 / 2.0, 8.0 / 9.0 / 2.0, 5.0 / 9.0 / 2.0};
+    herm_consts H;
+    herm_init(&H, dx); /* setGridSize(dx), :28 (used for y as well) */
+    pad_alloc(&P, nx, ny, k, 12);
+    load_padded(&P, q, 12, p->bc);           /* first BC call, :432 */
+    double** U = P.q;
+    double** V = P.q + 4;
+    double** W = P.q + 8;
+    fvh_limit(&P, U, V, W, dx);              /* :435 */
+    for (int f = 0; f < 12; ++f) fill_ghosts(&P, P.q[f], p->bc); /* :437 */
+    int nxx = P.nxx, nyy = P.nyy;
+    size_t nt = (size_t)nxx * nyy;
+    /* face states, 6 series (u, v, w left/right) x 4 fields per direction */
+    double* FX = (double*)calloc(24 * nt, sizeof(double)); /* uxl uxr vxl vxr wxl wxr */
+    double* FY = (double*)calloc(24 * nt, sizeof(double)); /* uyl uyr vyl vyr wyl wyr */
+    double* flx = (double*)calloc(12 * nt, sizeof(double)); /* fx gx hx */
+    double* fly = (double*)calloc(12 * nt, sizeof(double)); /* fy gy hy */
+#define SER(A, s_, e_, id) A[((size_t)(s_) * 4 + (e_)) * nt + (id)]
+
+    /* x reconstruction :443-468 */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nyy; ++j)
+        for (int i = s; i < nxx - s - 1; ++i) {
+            double qa[4], qb[4], L[16], R[16], A[4][4], B[4][4], C[4][6], cc[4], y[4], o4[4], o2[2];
+            cell_q(&P, i, j, qa);
+            cell_q(&P, i + 1, j, qb);
+            orc_eigen(gamma, 1, qa, qb, L, R);
+            for (int m = 0; m < 4; ++m) {
+                for (int e = 0; e < 4; ++e) cc[e] = U[e][PIX(&P, i - 1 + m, j)];
+                matvec4(L, cc, y);
+                for (int e = 0; e < 4; ++e) A[e][m] = y[e];
+                for (int e = 0; e < 4; ++e) cc[e] = V[e][PIX(&P, i - 1 + m, j)];
+                matvec4(L, cc, y);
+                for (int e = 0; e < 4; ++e) B[e][m] = y[e];
+            }
+            for (int m = 0; m < 6; ++m) {
+                for (int e = 0; e < 4; ++e) cc[e] = W[e][PIX(&P, i - 2 + m, j)];
+                matvec4(L, cc, y);
+                for (int e = 0; e < 4; ++e) C[e][m] = y[e];
+            }
+            size_t id = PIX(&P, i, j);
+            for (int e = 0; e < 4; ++e) {
+                hteno_impl(&H, A[e], B[e], o4);
+                SER(FX, 0, e, id) = o4[0];
+                SER(FX, 1, e, id) = o4[1];
+                SER(FX, 2, e, id) = o4[2];
+                SER(FX, 3, e, id) = o4[3];
+                teno5_impl(C[e], o2, NULL);
+                SER(FX, 4, e, id) = o2[0];
+                SER(FX, 5, e, id) = o2[1];
+            }
+        }
+    /* y reconstruction :470-495 (hteno on u and w = d/dy, teno5 on v) */
+#pragma omp parallel for schedule(static)
+    for (int j = s; j < nyy - s - 1; ++j)
+        for (int i = 0; i < nxx; ++i) {
+            double qa[4], qb[4], L[16], R[16], A[4][4], B[4][4], C[4][6], cc[4], y[4], o4[4], o2[2];
+            cell_q(&P, i, j, qa);
+            cell_q(&P, i, j + 1, qb);
+            orc_eigen(gamma, 2, qa, qb, L, R);
+            for (int m = 0; m < 4; ++m) {
+                for (int e = 0; e < 4; ++e) cc[e] = U[e][PIX(&P, i, j - 1 + m)];
+                matvec4(L, cc, y);
+                for (int e = 0; e < 4; ++e) A[e][m] = y[e];
+                for (int e = 0; e < 4; ++e) cc[e] = W[e][PIX(&P, i, j - 1 + m)];
+                matvec4(L, cc, y);
+                for (int e = 0; e < 4; ++e) B[e][m] = y[e];
+            }
+            for (int m = 0; m < 6; ++m) {
+                for (int e = 0; e < 4; ++e) cc[e] = V[e][PIX(&P, i, j - 2 + m)];
+                matvec4(L, cc, y);
+                for (int e = 0; e < 4; ++e) C[e][m] = y[e];
+            }
+            size_t id = PIX(&P, i, j);
+            for (int e = 0; e < 4; ++e) {
+                hteno_impl(&H, A[e], B[e], o4);
+                SER(FY, 0, e, id) = o4[0];  /* uyl */
+                SER(FY, 1, e, id) = o4[1];  /* uyr */
+                SER(FY, 4, e, id) = o4[2];  /* wyl */
+                SER(FY, 5, e, id) = o4[3];  /* wyr */
+                teno5_impl(C[e], o2, NULL);
+                SER(FY, 2, e, id) = o2[0];  /* vyl */
+                SER(FY, 3, e, id) = o2[1];  /* vyr */
+            }
+        }
+    /* quadrature + transform + fluxes :497-640 (fused over the flux range) */
+#pragma omp parallel for schedule(static)
+    for (int j = k - 1; j < nyy - k; ++j)
+        for (int i = k - 1; i < nxx - k; ++i)
+            for (int dir = 1; dir <= 2; ++dir) {
+                double qa[4], qb[4], L[16], R[16], G[6][4][3];
+                cell_q(&P, i, j, qa);
+                if (dir == 1) cell_q(&P, i + 1, j, qb); else cell_q(&P, i, j + 1, qb);
+                orc_eigen(gamma, dir, qa, qb, L, R);
+                double* F = dir == 1 ? FX : FY;
+                for (int sr = 0; sr < 6; ++sr)
+                    for (int e = 0; e < 4; ++e) {
+                        double a5[5];
+                        for (int m = 0; m < 5; ++m)
+                            a5[m] = SER(F, sr, e, dir == 1 ? PIX(&P, i, j - 2 + m) : PIX(&P, i - 2 + m, j));
+                        orc_weno5js3q(a5, G[sr][e]);
+                    }
+                double tu[3][4], tv[3][4], tw[3][4];
+                for (int m = 0; m < 3; ++m) {
+                    double c[6][4], Q[6][4];
+                    for (int sr = 0; sr < 6; ++sr) {
+                        for (int e = 0; e < 4; ++e) c[sr][e] = G[sr][e][m];
+                        matvec4(R, c[sr], Q[sr]);
+                    }
+                    /* Q: 0 ul, 1 ur, 2 vl, 3 vr, 4 wl, 5 wr (conservative) */
+                    double wl[4], wr[4], Jl[16], Jr[16], a[4], Fu[4], Fv[4], Fw[4];
+                    orc_primitive(gamma, Q[0], wl);
+                    orc_dflux(gamma, dir, Q[0], wl, Jl);
+                    orc_primitive(gamma, Q[1], wr);
+                    orc_dflux(gamma, dir, Q[1], wr, Jr);
+                    max_local_wave(gamma, dir, wl, wr, a);
+                    if (p->flux == ORC_FLUX_HLLC)
+                        orc_hllc(gamma, dir, p->quirks, Q[0], Q[1], Fu);
+                    else
+                        orc_rusanov_char(gamma, dir, Q[0], Q[1], L, R, Fu);
+                    drusanov(Q[2], Q[3], L, R, Jl, Jr, a, Fv);
+                    drusanov(Q[4], Q[5], L, R, Jl, Jr, a, Fw);
+                    for (int e = 0; e < 4; ++e) {
+                        tu[m][e] = gw[m] * Fu[e];
+                        tv[m][e] = gw[m] * Fv[e];
+                        tw[m][e] = gw[m] * Fw[e];
+                    }
+                }
+                double* fl = dir == 1 ? flx : fly;
+                size_t id = PIX(&P, i, j);
+                for (int e = 0; e < 4; ++e) {
+                    fl[(0 * 4 + e) * nt + id] = (tu[0][e] + tu[1][e]) + tu[2][e];
+                    fl[(1 * 4 + e) * nt + id] = (tv[0][e] + tv[1][e]) + tv[2][e];
+                    fl[(2 * 4 + e) * nt + id] = (tw[0][e] + tw[1][e]) + tw[2][e];
+                }
+            }
+    /* divergence :710-730 */
+    size_t n = (size_t)nx * ny;
+    for (int j = k; j < nyy - k; ++j)
+        for (int i = k; i < nxx - k; ++i)
+            for (int g = 0; g < 12; ++g) {
+                double v = ((flx[g * nt + PIX(&P, i - 1, j)] - flx[g * nt + PIX(&P, i, j)]) * dy +
+                            (fly[g * nt + PIX(&P, i, j - 1)] - fly[g * nt + PIX(&P, i, j)]) * dx) /
+                           area;
+                rhs[g * n + (size_t)(j - k) * nx + (i - k)] = v;
+            }
+#undef SER
+    free(FX); free(FY); free(flx); free(fly);
+    pad_free(&P, 12);
+}
+
+/* FVHermite::System2D::RungeKutta3, :733-782 (u, v and w) */
+void orc_fvh2d_rk3(const orc_params* p, double* q, double dt) {
+    size_t n = 12 * (size_t)p->nx * p->ny;
+    double* q0 = (double*)malloc(n * sizeof(double));
+    double* r = (double*)malloc(n * sizeof(double));
+    double* qs = (double*)malloc(n * sizeof(double));
+    memcpy(q0, q, n * sizeof(double));
+    orc_fvh2d_rhs(p, q, r);
+    orc_rk3_combine(1, q0, q0, r, dt, q, (long)n);
+    memcpy(qs, q, n * sizeof(double));
+    orc_fvh2d_rhs(p, q, r);
+    orc_rk3_combine(2, q0, qs, r, dt, q, (long)n);
+    memcpy(qs, q, n * sizeof(double));
+    orc_fvh2d_rhs(p, q, r);
+    orc_rk3_combine(3, q0, qs, r, dt, q, (long)n);
+    free(q0); free(r); free(qs);
+}
+
+/* FVHermite::System2D::timeStepping, :784-819 (dt from u only) */
+int orc_fvh2d_advance(const orc_params* p, double* q, double cfl, double t0, double tfinal, int max_steps,
+                      double* t_out) {
+    double t = t0;
+    int nsteps = 0;
+    while (t < tfinal && (max_steps <= 0 || nsteps < max_steps)) {
+        double dt = orc_fv2d_dt(p, q, cfl, t, tfinal);
+        orc_fvh2d_rk3(p, q, dt);
+        t += dt;
+        ++nsteps;
+    }
+    if (t_out) *t_out = t;
+    return nsteps;
+}
+
+/* Isentropic vortex with derivative moments (FVHermite_System2D.cpp:885-900,
+ * Equations_Euler2D.cpp:668-745), centred at (cx, cy), including the
+ * reference's /pi where /pi^2 is meant in the x-derivative (:678).  The
+ * density modes (k, l, a, phi, Lx, Ly) add a sin(theta) to rho at fixed u, v,
+ * p, and their analytic x/y derivatives to the moments. */
+static void vortex_point_derivs(double gamma, double x, double y, double cx, double cy, const double* modes,
+                                int nmodes, double q[4], double qx[4], double qy[4]) {
+    const double pi = 4.0 * atan(1.0);
+    double g1 = gamma - 1.0;
+    double X = x - cx, Y = y - cy;
+    double r = pow(X, 2) + pow(Y, 2);
+    double w[4];
+    w[0] = 1.0 - g1 * 25.0 / 8.0 / gamma / pow(pi, 2) * exp(1 - r);
+    w[0] = pow(w[0], 1.0 / g1);
+    w[1] = 1.0 - 5.0 * Y / 2.0 / pi * exp(0.5 * (1 - r));
+    w[2] = 1.0 + 5.0 * X / 2.0 / pi * exp(0.5 * (1 - r));
+    w[3] = pow(w[0], gamma);
+    /* :678-689 */
+    double a0 = 1.0 - g1 * 25.0 / 8.0 / gamma / pi * exp(1 - r);
+    double rx = 25.0 * X * exp(1 - r) / 4.0 / pow(pi, 2) / gamma * pow(a0, (2.0 - gamma) / g1);
+    double ux = 5.0 * X * Y / 2.0 / pi * exp(0.5 * (1 - r));
+    double vx = 5.0 * (1 - pow(X, 2)) / 2.0 / pi * exp(0.5 * (1 - r));
+    /* :702-713 */
+    double b0 = 1.0 - g1 * 25.0 / 8.0 / gamma / pow(pi, 2) * exp(1 - r);
+    double ry = 25.0 * Y * exp(1 - r) / 4.0 / pow(pi, 2) / gamma * pow(b0, (2.0 - gamma) / g1);
+    double uy = 5.0 * (pow(Y, 2) - 1.0) / 2.0 / pi * exp(0.5 * (1 - r));
+    double vy = -5.0 * X * Y / 2.0 / pi * exp(0.5 * (1 - r));
+    double pxv = gamma * pow(w[0], g1) * rx; /* dp/dx of the isentropic vortex */
+    double pyv = gamma * pow(w[0], g1) * ry;
+    /* density modes, accumulated onto rho in the order of vortex_point */
+    double rho = w[0], ddx = 0.0, ddy = 0.0;
+    for (int m = 0; m < nmodes; ++m) {
+        const double* md = modes + 6 * m;
+        double th = 2.0 * pi * (md[0] * x / md[4] + md[1] * y / md[5]) + md[3];
+        rho += md[2] * sin(th);
+        ddx += md[2] * cos(th) * (2.0 * pi * md[0] / md[4]);
+        ddy += md[2] * cos(th) * (2.0 * pi * md[1] / md[5]);
+    }
+    double wp[4] = {rho, w[1], w[2], w[3]};
+    orc_conservative(gamma, wp, q);
+    double Rx = rx + ddx, Ry = ry + ddy;
+    qx[0] = Rx;
+    qx[1] = rho * ux + Rx * w[1];
+    qx[2] = rho * vx + Rx * w[2];
+    qx[3] = pxv / g1 + 0.5 * Rx * (pow(w[1], 2) + pow(w[2], 2)) + 0.5 * rho * (2.0 * w[1] * ux + 2.0 * w[2] * vx);
+    qy[0] = Ry;
+    qy[1] = rho * uy + Ry * w[1];
+    qy[2] = rho * vy + Ry * w[2];
+    qy[3] = pyv / g1 + 0.5 * Ry * (pow(w[1], 2) + pow(w[2], 2)) + 0.5 * rho * (2.0 * w[1] * uy + 2.0 * w[2] * vy);
+}
+
+void orc_ic_vortex_hermite(const orc_params* p, double cx, double cy, const double* modes, int nmodes,
+                           double* q) {
+    double a[9], wt[9];
+    orc_gauss9(a, wt);
+    int nx = p->nx, ny = p->ny;
+    double dx = (p->xr - p->xl) / nx, dy = (p->yt - p->yb) / ny;
+    size_t n = (size_t)nx * ny;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double x = ((double)i + 0.5) * dx + p->xl;
+            double y = ((double)j + 0.5) * dy + p->yb;
+            double acc[12] = {0};
+            for (int ii = 0; ii < 9; ++ii)
+                for (int jj = 0; jj < 9; ++jj) {
+                    double qq[4], qx[4], qy[4];
+                    vortex_point_derivs(p->gamma, x + dx * a[ii] / 2.0, y + dy * a[jj] / 2.0, cx, cy, modes, nmodes,
+                                        qq, qx, qy);
+                    for (int e = 0; e < 4; ++e) {
+                        acc[e] += wt[ii] * wt[jj] * qq[e] / 4.0;
+                        acc[4 + e] += wt[ii] * wt[jj] * qx[e] / 4.0;
+                        acc[8 + e] += wt[ii] * wt[jj] * qy[e] / 4.0;
+                    }
+                }
+            for (int g = 0; g < 12; ++g) q[g * n + (size_t)j * nx + i] = acc[g];
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* FV::System1D + Equations::Euler1D (FV_System1D.cpp, Equations_Euler1D.cpp) */
+/* ------------------------------------------------------------------------ */
+#define FV1_K 5 /* FV_System1D.cpp:15 */
+
+static void e1_primitive(double g1, const double q[3], double w[3]) {
+    /* Equations_Euler1D.cpp:18-23 */
+    w[0] = q[0];
+    w[1] = q[1] / q[0];
+    w[2] = (q[2] - 0.5 * q[1] * w[1]) * g1;
+}
+
+static void e1_conservative(double g1, const double w[3], double q[3]) {
+    /* :25-30 */
+    q[0] = w[0];
+    q[1] = w[0] * w[1];
+    q[2] = w[2] / g1 + 0.5 * q[1] * w[1];
+}
+
+static void e1_flux(const double q[3], const double w[3], double f[3]) {
+    /* :32-37 */
+    f[0] = q[1];
+    f[1] = q[1] * w[1] + w[2];
+    f[2] = w[1] * (q[2] + w[2]);
+}
+
+static inline void matvec3(const double M[9], const double x[3], double y[3]) {
+    for (int r = 0; r < 3; ++r) y[r] = (M[3 * r] * x[0] + M[3 * r + 1] * x[1]) + M[3 * r + 2] * x[2];
+}
+
+/* Euler1D::HLLC, Equations_Euler1D.cpp:156-210 (quirks: c without gamma
+ * :174-175, p_L in the right star energy :204) */
+static void e1_hllc(double gamma, int quirks, const double ql[3], const double qr[3], double f[3]) {
+    double g1 = gamma - 1.0, wl[3], wr[3], fl[3], fr[3], qs[3];
+    e1_primitive(g1, ql, wl);
+    e1_primitive(g1, qr, wr);
+    double hl = (ql[2] + wl[2]) / wl[0], hr = (qr[2] + wr[2]) / wr[0];
+    double rr = sqrt(fabs(wr[0] / wl[0]));
+    double rrr = 1.0 / (1.0 + rr);
+    double vm = (wl[1] + wr[1] * rr) * rrr;
+    double qm = 0.5 * vm * vm;
+    double hm = (hl + hr * rr) * rrr;
+    double cm = sqrt(fabs(g1 * (hm - qm)));
+    double cl = quirks ? sqrt(fabs(wl[2] / wl[0])) : sqrt(fabs(gamma * wl[2] / wl[0]));
+    double cr = quirks ? sqrt(fabs(wr[2] / wr[0])) : sqrt(fabs(gamma * wr[2] / wr[0]));
+    double sl = fmin(wl[1] - cl, vm - cm);
+    double sr = fmax(wr[1] + cr, vm + cm);
+    double sm = (qr[1] * (sr - wr[1]) - ql[1] * (sl - wl[1]) + wl[2] - wr[2]) /
+                (wr[0] * (sr - wr[1]) - wl[0] * (sl - wl[1]));
+    double ps = wl[2] + wl[0] * (wl[1] - sl) * (wl[1] - sm);
+    e1_flux(ql, wl, fl);
+    e1_flux(qr, wr, fr);
+    if (sl > 0.0) {
+        memcpy(f, fl, sizeof fl);
+    } else if (sl <= 0.0 && sm > 0.0) {
+        double nom = sl - wl[1], den = sl - sm;
+        qs[0] = wl[0] * nom / den;
+        qs[1] = (nom * ql[1] + ps - wl[2]) / den;
+        qs[2] = (nom * ql[2] - wl[2] * wl[1] + ps * sm) / den;
+        for (int k = 0; k < 3; ++k) f[k] = fl[k] + sl * (qs[k] - ql[k]);
+    } else if (sm <= 0.0 && sr >= 0.0) {
+        double nom = sr - wr[1], den = sr - sm;
+        qs[0] = wr[0] * nom / den;
+        qs[1] = (nom * qr[1] + ps - wr[2]) / den;
+        qs[2] = (nom * qr[2] - (quirks ? wl[2] : wr[2]) * wr[1] + ps * sm) / den;
+        for (int k = 0; k < 3; ++k) f[k] = fr[k] + sr * (qs[k] - qr[k]);
+    } else {
+        memcpy(f, fr, sizeof fr);
+    }
+}
+
+/* Lagrange::teno5i, Schemes_Lagrange.cpp:1235-1263: tau = max(beta) */
+static double teno5i_side(const double* v, const double p[3], const double d[3]) {
+    double SI[3], w[3], sigm[3];
+    smoothness5(v, SI);
+    double tau = SI[0];
+    if (SI[1] > tau) tau = SI[1];
+    if (SI[2] > tau) tau = SI[2];
+    for (int k = 0; k < 3; ++k) w[k] = pow(1.0 + tau / (SI[k] + LC.epsz), 6);
+    double s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    for (int k = 0; k < 3; ++k) sigm[k] = w[k] < LC.ct ? 0.0 : 1.0;
+    for (int k = 0; k < 3; ++k) w[k] = sigm[k] * d[k];
+    s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    return catchnan(dot3(p, w));
+}
+
+static void teno5i_impl(const double u[6], double out[2]) {
+    double p[3], c[3];
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][1];
+    p[0] = dot3(c, u + 2);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][2];
+    p[1] = dot3(c, u + 1);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][3];
+    p[2] = dot3(c, u + 0);
+    out[0] = teno5i_side(u, p, LC.dkm);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][0];
+    p[0] = dot3(c, u + 3);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][1];
+    p[1] = dot3(c, u + 2);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][2];
+    p[2] = dot3(c, u + 1);
+    out[1] = teno5i_side(u + 1, p, LC.dkp);
+}
+
+void orc_teno5i(const double u[6], double out[2]) {
+    lag_init();
+    teno5i_impl(u, out);
+}
+
+/* FV::System1D::rhs, FV_System1D.cpp:158-241 (zero-gradient BC :74-80,
+ * eigen1DEuler :121-156; the KXRCF flags :166-192 feed nothing).
+ * q = 3 fields x nx.  recon: 0 teno5, 1 teno5i (shipped); flux: ORC_FLUX_*. */
+void orc_fv1d_rhs(int nx, double xl, double xr, double gamma, int recon, int flux, int quirks, const double* q,
+                  double* rhs) {
+    lag_init();
+    int k = FV1_K, nxx = nx + 2 * k;
+    double g1 = gamma - 1.0, dx = (xr - xl) / nx;
+    double* U = (double*)calloc(3 * (size_t)nxx, sizeof(double));
+    double* fx = (double*)calloc(3 * (size_t)nxx, sizeof(double));
+    for (int e = 0; e < 3; ++e) {
+        for (int i = 0; i < nx; ++i) U[e * nxx + k + i] = q[e * nx + i];
+        for (int m = 0; m < k; ++m) {
+            U[e * nxx + k - 1 - m] = U[e * nxx + k];
+            U[e * nxx + nxx - 1 - m] = U[e * nxx + nxx - k - 1];
+        }
+    }
+    for (int i = k - 1; i < nxx - k; ++i) {
+        double wa[3], wb[3], qa[3] = {U[i], U[nxx + i], U[2 * nxx + i]};
+        double qb[3] = {U[i + 1], U[nxx + i + 1], U[2 * nxx + i + 1]};
+        e1_primitive(g1, qa, wa);
+        e1_primitive(g1, qb, wb);
+        double sa = sqrt(fabs(wa[0])), sb = sqrt(fabs(wb[0]));
+        double ha = (qa[2] + wa[2]) / wa[0], hb = (qb[2] + wb[2]) / wb[0];
+        double t0 = sa / (sa + sb), t1 = 1.0 - t0;
+        double vxm = t0 * wa[1] + t1 * wb[1];
+        double hm = t0 * ha + t1 * hb;
+        double qm = 0.5 * vxm * vxm;
+        double cm = sqrt(fabs(g1 * (hm - qm)));
+        double tc = vxm * cm;
+        double R[9] = {1.0, 1.0, 1.0, vxm - cm, vxm, vxm + cm, hm - tc, qm, hm + tc};
+        double rcm = 1.0 / cm, b1 = g1 * rcm * rcm, b2 = qm * b1, tr = vxm * rcm, tb = b1 * vxm, t2 = 0.5 * b1;
+        double L[9] = {0.5 * (b2 + tr), -0.5 * (tb + rcm), t2, 1.0 - b2, tb, -b1, 0.5 * (b2 - tr), -0.5 * (tb - rcm), t2};
+        double C[3][6], cc[3], y[3], ul[3], ur[3], o[2];
+        for (int m = 0; m < 6; ++m) {
+            for (int e = 0; e < 3; ++e) cc[e] = U[e * nxx + i - 2 + m];
+            matvec3(L, cc, y);
+            for (int e = 0; e < 3; ++e) C[e][m] = y[e];
+        }
+        for (int e = 0; e < 3; ++e) {
+            if (recon == 1) teno5i_impl(C[e], o); else teno5_impl(C[e], o, NULL);
+            ul[e] = o[0];
+            ur[e] = o[1];
+        }
+        double ql[3], qr[3], F[3];
+        matvec3(R, ul, ql);  /* transformToConservative, FV_System1D.h:48-51 */
+        matvec3(R, ur, qr);
+        if (flux == ORC_FLUX_HLLC) {
+            e1_hllc(gamma, quirks, ql, qr, F);
+        } else {
+            double wl[3], wr[3], fl[3], fr[3], a[3], dd[3], ld[3], rd[3];
+            e1_primitive(g1, ql, wl);
+            e1_primitive(g1, qr, wr);
+            double cl = sqrt(fabs(gamma * wl[2] / wl[0])), cr = sqrt(fabs(gamma * wr[2] / wr[0]));
+            a[0] = fmax(fabs(wl[1] - cl), fabs(wr[1] - cr));
+            a[1] = fmax(fabs(wl[1]), fabs(wr[1]));
+            a[2] = fmax(fabs(wl[1] + cl), fabs(wr[1] + cr));
+            e1_flux(ql, wl, fl);
+            e1_flux(qr, wr, fr);
+            for (int e = 0; e < 3; ++e) dd[e] = qr[e] - ql[e];
+            matvec3(L, dd, ld);
+            for (int e = 0; e < 3; ++e) ld[e] = a[e] * ld[e];
+            matvec3(R, ld, rd);
+            for (int e = 0; e < 3; ++e) F[e] = 0.5 * ((fl[e] + fr[e]) - rd[e]);
+        }
+        for (int e = 0; e < 3; ++e) fx[e * nxx + i] = F[e];
+    }
+    for (int i = k; i < nxx - k; ++i)
+        for (int e = 0; e < 3; ++e) rhs[e * nx + i - k] = (fx[e * nxx + i - 1] - fx[e * nxx + i]) / dx;
+    free(U);
+    free(fx);
+}
+
+/* FV::System1D::timeStepping + RungeKutta3, FV_System1D.cpp:243-292 */
+int orc_fv1d_advance(int nx, double xl, double xr, double gamma, int recon, int flux, int quirks, double* q,
+                     double cfl, double t0, double tfinal, int max_steps, double* t_out) {
+    double g1 = gamma - 1.0, dx = (xr - xl) / nx, t = t0;
+    size_t n = 3 * (size_t)nx;
+    double* q0 = (double*)malloc(n * sizeof(double));
+    double* qs = (double*)malloc(n * sizeof(double));
+    double* r = (double*)malloc(n * sizeof(double));
+    int nsteps = 0;
+    while (t < tfinal && (max_steps <= 0 || nsteps < max_steps)) {
+        double smax = 0.0;
+        for (int i = 0; i < nx; ++i) {
+            double qq[3] = {q[i], q[nx + i], q[2 * nx + i]}, w[3];
+            e1_primitive(g1, qq, w);
+            double c = sqrt(fabs(gamma * w[2] / w[0]));
+            smax = fmax(smax, fmax(fabs(w[1] - c), fabs(w[1] + c))); /* Equations_Euler1D.cpp:91-94 */
+        }
+        double dt = cfl * pow(dx, 3.0 / 3.0) / smax;
+        dt = t + dt > tfinal ? tfinal - t : dt;
+        memcpy(q0, q, n * sizeof(double));
+        orc_fv1d_rhs(nx, xl, xr, gamma, recon, flux, quirks, q, r);
+        orc_rk3_combine(1, q0, q0, r, dt, q, (long)n);
+        memcpy(qs, q, n * sizeof(double));
+        orc_fv1d_rhs(nx, xl, xr, gamma, recon, flux, quirks, q, r);
+        orc_rk3_combine(2, q0, qs, r, dt, q, (long)n);
+        memcpy(qs, q, n * sizeof(double));
+        orc_fv1d_rhs(nx, xl, xr, gamma, recon, flux, quirks, q, r);
+        orc_rk3_combine(3, q0, qs, r, dt, q, (long)n);
+        t += dt;
+        ++nsteps;
+    }
+    free(q0); free(qs); free(r);
+    if (t_out) *t_out = t;
+    return nsteps;
+}
+
+/* Euler1D::sodShock (Equations_Euler1D.cpp:317-332) point-sampled at the
+ * cell centres of FV_System1D.cpp:25. */
+void orc_ic_sod(int nx, double xl, double xr, double gamma, double* q) {
+    double dx = (xr - xl) / nx, g1 = gamma - 1.0;
+    for (int i = 0; i < nx; ++i) {
+        double x = ((double)(i + FV1_K) - (double)FV1_K + 0.5) * dx + xl;
+        double w[3], qq[3];
+        if (x < 0.5) { w[0] = 1.0; w[1] = 0.0; w[2] = 1.0; }
+        else { w[0] = 0.125; w[1] = 0.0; w[2] = 0.1; }
+        e1_conservative(g1, w, qq);
+        for (int e = 0; e < 3; ++e) q[e * nx + i] = qq[e];
+    }
+}
+
+/* ---- test hooks: smoothness indicators, exported for independent pinning ---- */
+void orc_teno5_indicators(const double u[6], double SI[6], double b35[2]) {
+    lag_init();
+    smoothness5(u, SI);
+    smoothness5(u + 1, SI + 3);
+    b35[0] = b35_of(u);
+    b35[1] = b35_of(u + 1);
+}
+
+void orc_hteno_indicators(const double u[3], const double ud[3], double SI[3], double SId[3], double brk[2]) {
+    lag_init();
+    herm_si(u, ud, SI);
+    herm_sid(u, ud, SId);
+    brk[0] = herm_brk(u, ud);
+    brk[1] = herm_brkd(u, ud);
 }

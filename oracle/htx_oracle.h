@@ -56,6 +56,29 @@ void orc_fv2d_rk3(const orc_params* p, double* q, double dt);
 int orc_fv2d_advance(const orc_params* p, double* q, double cfl, double t0, double tfinal,
                      int max_steps, double* t_out);
 
+/* ---- 2D Hermite-TENO (FVHermite::System2D): 12 fields u, <dq/dx>, <dq/dy> ---- */
+void orc_hteno(double dif, const double u[4], const double d[4], double out[4]);
+double orc_hermite_limiter(double dif, const double u[3], const double d[3]);
+void orc_dflux(double gamma, int dir, const double q[4], const double w[4], double M[16]);
+void orc_fvh2d_rhs(const orc_params* p, const double* q, double* rhs);
+void orc_fvh2d_rk3(const orc_params* p, double* q, double dt);
+int orc_fvh2d_advance(const orc_params* p, double* q, double cfl, double t0, double tfinal,
+                      int max_steps, double* t_out);
+void orc_ic_vortex_hermite(const orc_params* p, double cx, double cy, const double* modes,
+                           int nmodes, double* q);
+
+/* ---- 1D FV (FV::System1D): 3 fields ---- */
+void orc_teno5i(const double u[6], double out[2]);
+void orc_fv1d_rhs(int nx, double xl, double xr, double gamma, int recon, int flux, int quirks,
+                  const double* q, double* rhs);
+int orc_fv1d_advance(int nx, double xl, double xr, double gamma, int recon, int flux, int quirks,
+                     double* q, double cfl, double t0, double tfinal, int max_steps, double* t_out);
+void orc_ic_sod(int nx, double xl, double xr, double gamma, double* q);
+
+/* ---- test hooks: smoothness indicators ---- */
+void orc_teno5_indicators(const double u[6], double SI[6], double b35[2]);
+void orc_hteno_indicators(const double u[3], const double ud[3], double SI[3], double SId[3], double brk[2]);
+
 /* ---- initial conditions (restated generators) ---- */
 void orc_ic_vortex(const orc_params* p, double cx, double cy, const double* modes,
                    int nmodes, double* q);

@@ -47,6 +47,19 @@ _SIG = {
     "orc_fv2d_rk3": (None, [ctypes.POINTER(OrcParams), _DP, _D]),
     "orc_fv2d_advance": (_I, [ctypes.POINTER(OrcParams), _DP, _D, _D, _D, _I, _DP]),
     "orc_ic_vortex": (None, [ctypes.POINTER(OrcParams), _D, _D, _DP, _I, _DP]),
+    "orc_hteno": (None, [_D, _DP, _DP, _DP]),
+    "orc_hermite_limiter": (_D, [_D, _DP, _DP]),
+    "orc_dflux": (None, [_D, _I, _DP, _DP, _DP]),
+    "orc_fvh2d_rhs": (None, [ctypes.POINTER(OrcParams), _DP, _DP]),
+    "orc_fvh2d_rk3": (None, [ctypes.POINTER(OrcParams), _DP, _D]),
+    "orc_fvh2d_advance": (_I, [ctypes.POINTER(OrcParams), _DP, _D, _D, _D, _I, _DP]),
+    "orc_ic_vortex_hermite": (None, [ctypes.POINTER(OrcParams), _D, _D, _DP, _I, _DP]),
+    "orc_teno5i": (None, [_DP, _DP]),
+    "orc_fv1d_rhs": (None, [_I, _D, _D, _D, _I, _I, _I, _DP, _DP]),
+    "orc_fv1d_advance": (_I, [_I, _D, _D, _D, _I, _I, _I, _DP, _D, _D, _D, _I, _DP]),
+    "orc_ic_sod": (None, [_I, _D, _D, _D, _DP]),
+    "orc_teno5_indicators": (None, [_DP, _DP, _DP]),
+    "orc_hteno_indicators": (None, [_DP, _DP, _DP, _DP, _DP]),
 }
 
 _libs = {}
@@ -196,3 +209,93 @@ class Oracle:
         self.lib.orc_ic_vortex(ctypes.byref(prm), centre[0], centre[1],
                                mm.ctypes.data_as(_DP) if mm.size else None, m.shape[0], q.ctypes.data_as(_DP))
         return q
+
+    # Hermite-TENO
+    def hteno(self, dif, u, d):
+        u, pu = _p(u)
+        d, pd = _p(d)
+        out = np.empty(4)
+        self.lib.orc_hteno(dif, pu, pd, out.ctypes.data_as(_DP))
+        return out
+
+    def hermite_limiter(self, dif, u, d):
+        u, pu = _p(u)
+        d, pd = _p(d)
+        return self.lib.orc_hermite_limiter(dif, pu, pd)
+
+    def dflux(self, gamma, direction, q, w):
+        q, pq = _p(q)
+        w, pw = _p(w)
+        M = np.empty(16)
+        self.lib.orc_dflux(gamma, direction, pq, pw, M.ctypes.data_as(_DP))
+        return M.reshape(4, 4)
+
+    def fvh2d_rhs(self, prm, q):
+        q, pq = _p(q)
+        r = np.empty_like(q)
+        self.lib.orc_fvh2d_rhs(ctypes.byref(prm), pq, r.ctypes.data_as(_DP))
+        return r
+
+    def fvh2d_rk3(self, prm, q, dt):
+        q = np.array(q, dtype=np.float64, copy=True, order="C")
+        self.lib.orc_fvh2d_rk3(ctypes.byref(prm), q.ctypes.data_as(_DP), dt)
+        return q
+
+    def fvh2d_advance(self, prm, q, cfl, t0, tfinal, max_steps=0):
+        q = np.array(q, dtype=np.float64, copy=True, order="C")
+        t = _D()
+        n = self.lib.orc_fvh2d_advance(ctypes.byref(prm), q.ctypes.data_as(_DP), cfl, t0, tfinal, max_steps,
+                                       ctypes.byref(t))
+        return q, t.value, n
+
+    def ic_vortex_hermite(self, prm, centre=(0.0, 0.0), modes=None):
+        m = np.zeros((0, 6)) if modes is None else np.asarray(modes, dtype=np.float64).reshape(-1, 6)
+        mm = np.ascontiguousarray(m.reshape(-1))
+        q = np.empty((12, prm.ny, prm.nx))
+        self.lib.orc_ic_vortex_hermite(ctypes.byref(prm), centre[0], centre[1],
+                                       mm.ctypes.data_as(_DP) if mm.size else None, m.shape[0],
+                                       q.ctypes.data_as(_DP))
+        return q
+
+    # 1D
+    def teno5i(self, u):
+        u, pu = _p(u)
+        out = np.empty(2)
+        self.lib.orc_teno5i(pu, out.ctypes.data_as(_DP))
+        return out
+
+    def fv1d_rhs(self, nx, xl, xr, gamma, q, recon="teno5", flux="hllc", quirks=True):
+        q, pq = _p(q)
+        r = np.empty_like(q)
+        self.lib.orc_fv1d_rhs(nx, xl, xr, gamma, {"teno5": 0, "teno5i": 1}[recon], {"rusanov": 0, "hllc": 1}[flux],
+                              1 if quirks else 0, pq, r.ctypes.data_as(_DP))
+        return r
+
+    def fv1d_advance(self, nx, xl, xr, gamma, q, cfl, t0, tfinal, recon="teno5", flux="hllc", quirks=True,
+                     max_steps=0):
+        q = np.array(q, dtype=np.float64, copy=True, order="C")
+        t = _D()
+        n = self.lib.orc_fv1d_advance(nx, xl, xr, gamma, {"teno5": 0, "teno5i": 1}[recon],
+                                      {"rusanov": 0, "hllc": 1}[flux], 1 if quirks else 0, q.ctypes.data_as(_DP),
+                                      cfl, t0, tfinal, max_steps, ctypes.byref(t))
+        return q, t.value, n
+
+    def ic_sod(self, nx, xl=0.0, xr=1.0, gamma=1.4):
+        q = np.empty((3, nx))
+        self.lib.orc_ic_sod(nx, xl, xr, gamma, q.ctypes.data_as(_DP))
+        return q
+
+    # indicators (test hooks)
+    def teno5_indicators(self, u):
+        u, pu = _p(u)
+        SI = np.empty(6)
+        b = np.empty(2)
+        self.lib.orc_teno5_indicators(pu, SI.ctypes.data_as(_DP), b.ctypes.data_as(_DP))
+        return SI, b
+
+    def hteno_indicators(self, u, ud):
+        u, pu = _p(u)
+        ud, pd = _p(ud)
+        SI, SId, b = np.empty(3), np.empty(3), np.empty(2)
+        self.lib.orc_hteno_indicators(pu, pd, SI.ctypes.data_as(_DP), SId.ctypes.data_as(_DP), b.ctypes.data_as(_DP))
+        return SI, SId, b
