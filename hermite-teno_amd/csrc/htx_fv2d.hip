@@ -25,7 +25,7 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 template <int TX, int TY, int FLUX>
-__global__ void __launch_bounds__(256) fv2d_stage_kernel(Fv2dArgs a) {
+__global__ void __launch_bounds__(256, 2) fv2d_stage_kernel(Fv2dArgs a) {
     using T = Fv2dTile<TX, TY>;
     __shared__ T s;
     double dt = 0.0;

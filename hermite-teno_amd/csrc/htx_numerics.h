@@ -20,6 +20,15 @@
 // states bit-identical to an IEEE evaluation of the reference formulas.
 #define HTX_NO_CONTRACT _Pragma("clang fp contract(off)")
 
+// Divisions that feed no discontinuous decision (normalisations, the
+// quadrature's nonlinear weights, the HLLC star state) are strength-reduced
+// to one reciprocal and multiplies, which moves results by a few ulp.  The
+// CPU tiling harness builds with HTX_EXACT_DIVISION=1 to reproduce the
+// reference's division-by-division evaluation bit for bit.
+#ifndef HTX_EXACT_DIVISION
+#define HTX_EXACT_DIVISION 0
+#endif
+
 namespace htx {
 
 // Runtime-computed coefficients of Schemes::Lagrange that involve sqrt(15)
@@ -112,8 +121,12 @@ HTX_HD double teno5_side(double v0, double v1, double v2, double v3, double v4,
     bool k0 = !(g0 / gs < 5e-4), k1 = !(g1 / gs < 5e-4), k2 = !(g2 / gs < 5e-4);
     double w0 = k0 ? d0 : 0.0, w1 = k1 ? d1 : 0.0, w2 = k2 ? d2 : 0.0;
     double ws = (w0 + w1) + w2;
+#if HTX_EXACT_DIVISION
     w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
     double out = (p0 * w0 + p1 * w1) + p2 * w2;
+#else
+    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * (1.0 / ws);  // all cut: 0 * inf = NaN -> catchnan
+#endif
     if (cut) *cut = (k0 ? 0u : 1u) | (k1 ? 0u : 2u) | (k2 ? 0u : 4u);
     return out != out ? 1e-15 : out;  // catchnan
 }
@@ -151,6 +164,7 @@ HTX_HD void weno3q(const QuadConsts& C, double u0, double u1, double u2, double 
     double e0 = 1e-6 + s0, e1 = 1e-6 + s1, e2 = 1e-6 + s2;
     e0 = e0 * e0; e1 = e1 * e1; e2 = e2 * e2;
     double w0, w1, w2, ws, p0, p1, p2;
+#if HTX_EXACT_DIVISION
     // +xi
     p0 = (C.gp[0][0] * u0 + C.gp[1][0] * u1) + C.gp[2][0] * u2;
     p1 = (C.gp[0][1] * u1 + C.gp[1][1] * u2) + C.gp[2][1] * u3;
@@ -179,6 +193,36 @@ HTX_HD void weno3q(const QuadConsts& C, double u0, double u1, double u2, double 
     ws = (w0 + w1) + w2;
     w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
     o2 = (p0 * w0 + p1 * w1) + p2 * w2;
+#else
+    // 3 reciprocals of (eps + beta_k)^2 shared by the four weight sets, one
+    // reciprocal per normalisation
+    double r0 = 1.0 / e0, r1 = 1.0 / e1, r2 = 1.0 / e2;
+    // +xi
+    p0 = (C.gp[0][0] * u0 + C.gp[1][0] * u1) + C.gp[2][0] * u2;
+    p1 = (C.gp[0][1] * u1 + C.gp[1][1] * u2) + C.gp[2][1] * u3;
+    p2 = (C.gp[0][2] * u2 + C.gp[1][2] * u3) + C.gp[2][2] * u4;
+    w0 = C.linwp[0] * r0; w1 = C.linwp[1] * r1; w2 = C.linwp[2] * r2;
+    ws = (w0 + w1) + w2;
+    o0 = ((p0 * w0 + p1 * w1) + p2 * w2) / ws;
+    // centre, positive/negative split
+    const double r24 = 1.0 / 24.0;
+    p0 = (-u0 + 2.0 * u1 + 23.0 * u2) * r24;
+    p1 = (-u1 + 26.0 * u2 - u3) * r24;
+    p2 = (23.0 * u2 + 2.0 * u3 - u4) * r24;
+    w0 = C.linw0p[0] * r0; w1 = C.linw0p[1] * r1; w2 = C.linw0p[2] * r2;
+    ws = (w0 + w1) + w2;
+    o1 = C.sumw0p * (((p0 * w0 + p1 * w1) + p2 * w2) / ws);
+    w0 = C.linw0m[0] * r0; w1 = C.linw0m[1] * r1; w2 = C.linw0m[2] * r2;
+    ws = (w0 + w1) + w2;
+    o1 -= C.sumw0m * (((p0 * w0 + p1 * w1) + p2 * w2) / ws);
+    // -xi
+    p0 = (C.gm[0][0] * u0 + C.gm[1][0] * u1) + C.gm[2][0] * u2;
+    p1 = (C.gm[0][1] * u1 + C.gm[1][1] * u2) + C.gm[2][1] * u3;
+    p2 = (C.gm[0][2] * u2 + C.gm[1][2] * u3) + C.gm[2][2] * u4;
+    w0 = C.linwm[0] * r0; w1 = C.linwm[1] * r1; w2 = C.linwm[2] * r2;
+    ws = (w0 + w1) + w2;
+    o2 = ((p0 * w0 + p1 * w1) + p2 * w2) / ws;
+#endif
     o0 = o0 != o0 ? 1e-15 : o0;  // catchnan
     o1 = o1 != o1 ? 1e-15 : o1;
     o2 = o2 != o2 ? 1e-15 : o2;
@@ -264,8 +308,14 @@ HTX_HD void phys_flux(double g1, double q0, double qn, double qt, double q3,
                                           double& f0, double& fn, double& ft, double& f3, double& wn,
                                           double& p, double& rho) {
     rho = q0;
+#if HTX_EXACT_DIVISION
     wn = qn / q0;
     double wt = qt / q0;
+#else
+    double iq = 1.0 / q0;
+    wn = qn * iq;
+    double wt = qt * iq;
+#endif
     p = (q3 - 0.5 * (qn * wn + qt * wt)) * g1;
     f0 = qn;
     fn = qn * wn + p;
@@ -282,8 +332,13 @@ HTX_HD void rusanov_char(double gam, double g1, const Eig& E, const double ql[4]
     double fr0, frn, frt, fr3, wnr, pr, rr;
     phys_flux(g1, ql[0], ql[1], ql[2], ql[3], fl0, fln, flt, fl3, wnl, pl, rl);
     phys_flux(g1, qr[0], qr[1], qr[2], qr[3], fr0, frn, frt, fr3, wnr, pr, rr);
+#if HTX_EXACT_DIVISION
     double cl = sqrt(fabs(gam * pl / rl));
     double cr = sqrt(fabs(gam * pr / rr));
+#else
+    double cl = sqrt(fabs(gam * pl * (1.0 / rl)));
+    double cr = sqrt(fabs(gam * pr * (1.0 / rr)));
+#endif
     double a0 = fmax(fabs(wnl - cl), fabs(wnr - cr));
     double a1 = fmax(fabs(wnl), fabs(wnr));
     double a3 = fmax(fabs(wnl + cl), fabs(wnr + cr));
@@ -307,6 +362,7 @@ HTX_HD void rusanov_char(double gam, double g1, const Eig& E, const double ql[4]
 template <bool QUIRKS>
 HTX_HD void hllc(double gam, double g1, const double ql[4], const double qr[4],
                                      double f[4]) {
+#if HTX_EXACT_DIVISION
     double wl0 = ql[0], wln = ql[1] / ql[0], wlt = ql[2] / ql[0];
     double wr0 = qr[0], wrn = qr[1] / qr[0], wrt = qr[2] / qr[0];
     double pl = (ql[3] - 0.5 * (ql[1] * wln + ql[2] * wlt)) * g1;
@@ -314,14 +370,26 @@ HTX_HD void hllc(double gam, double g1, const double ql[4], const double qr[4],
     double hl = (ql[3] + pl) / wl0;
     double hr = (qr[3] + pr) / wr0;
     double rho_ratio = sqrt(fabs(wr0 / wl0));
+    double cl = QUIRKS ? sqrt(fabs(pl / wl0)) : sqrt(fabs(gam * pl / wl0));
+    double cr = QUIRKS ? sqrt(fabs(pr / wr0)) : sqrt(fabs(gam * pr / wr0));
+#else
+    double il = 1.0 / ql[0], ir = 1.0 / qr[0];
+    double wl0 = ql[0], wln = ql[1] * il, wlt = ql[2] * il;
+    double wr0 = qr[0], wrn = qr[1] * ir, wrt = qr[2] * ir;
+    double pl = (ql[3] - 0.5 * (ql[1] * wln + ql[2] * wlt)) * g1;
+    double pr = (qr[3] - 0.5 * (qr[1] * wrn + qr[2] * wrt)) * g1;
+    double hl = (ql[3] + pl) * il;
+    double hr = (qr[3] + pr) * ir;
+    double rho_ratio = sqrt(fabs(wr0 * il));
+    double cl = QUIRKS ? sqrt(fabs(pl * il)) : sqrt(fabs(gam * pl * il));
+    double cr = QUIRKS ? sqrt(fabs(pr * ir)) : sqrt(fabs(gam * pr * ir));
+#endif
     double rrr = 1.0 / (1.0 + rho_ratio);
     double vn = (wln + wrn * rho_ratio) * rrr;
     double vt = (wlt + wrt * rho_ratio) * rrr;
     double qm = 0.5 * (vn * vn + vt * vt);
     double hm = (hl + hr * rho_ratio) * rrr;
     double cm = sqrt(fabs(g1 * (hm - qm)));
-    double cl = QUIRKS ? sqrt(fabs(pl / wl0)) : sqrt(fabs(gam * pl / wl0));
-    double cr = QUIRKS ? sqrt(fabs(pr / wr0)) : sqrt(fabs(gam * pr / wr0));
     double sl = fmin(wln - cl, vn - cm);
     double sr = fmax(wrn + cr, vn + cm);
     double sm = (qr[1] * (sr - wrn) - ql[1] * (sl - wln) + pl - pr) /
@@ -334,10 +402,18 @@ HTX_HD void hllc(double gam, double g1, const double ql[4], const double qr[4],
         f[3] = wln * (ql[3] + pl);
     } else if (sl <= 0.0 && sm > 0.0) {
         double nom = sl - wln, den = sl - sm;
+#if HTX_EXACT_DIVISION
         double s0 = wl0 * nom / den;
         double sn = (nom * ql[1] + ps - pl) / den;
         double st = nom * ql[2] / den;
         double s3 = (nom * ql[3] - pl * wln + ps * sm) / den;
+#else
+        double id = 1.0 / den;
+        double s0 = wl0 * nom * id;
+        double sn = (nom * ql[1] + ps - pl) * id;
+        double st = nom * ql[2] * id;
+        double s3 = (nom * ql[3] - pl * wln + ps * sm) * id;
+#endif
         f[0] = ql[1] + sl * (s0 - ql[0]);
         f[1] = (ql[1] * wln + pl) + sl * (sn - ql[1]);
         f[2] = ql[1] * wlt + sl * (st - ql[2]);
@@ -345,10 +421,18 @@ HTX_HD void hllc(double gam, double g1, const double ql[4], const double qr[4],
     } else if (sm <= 0.0 && sr >= 0.0) {
         double nom = sr - wrn, den = sr - sm;
         double pe = QUIRKS ? pl : pr;
+#if HTX_EXACT_DIVISION
         double s0 = wr0 * nom / den;
         double sn = (nom * qr[1] + ps - pr) / den;
         double st = nom * qr[2] / den;
         double s3 = (nom * qr[3] - pe * wrn + ps * sm) / den;
+#else
+        double id = 1.0 / den;
+        double s0 = wr0 * nom * id;
+        double sn = (nom * qr[1] + ps - pr) * id;
+        double st = nom * qr[2] * id;
+        double s3 = (nom * qr[3] - pe * wrn + ps * sm) * id;
+#endif
         f[0] = qr[1] + sr * (s0 - qr[0]);
         f[1] = (qr[1] * wrn + pr) + sr * (sn - qr[1]);
         f[2] = qr[1] * wrt + sr * (st - qr[2]);

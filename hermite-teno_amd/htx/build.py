@@ -69,14 +69,19 @@ def build_oracle():
 
 
 HOST_HARNESS = os.path.join(ROOT, "tests", "host", "libfv2d_host.so")
+HOST_HARNESS_FAST = os.path.join(ROOT, "tests", "host", "libfv2d_host_fast.so")
 
 
 def build_host_harness():
-    """CPU build of the stage kernel's tiling (tests/host/fv2d_host.hip), with
-    contraction off so it can be compared bit for bit with the oracle."""
+    """CPU builds of the stage kernel's tiling (tests/host/fv2d_host.hip), with
+    contraction off: one with the reference's division-by-division evaluation
+    (compared bit for bit with the oracle) and one with the product's
+    strength-reduced divisions (compared within tolerance)."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     src = os.path.join(ROOT, "tests", "host", "fv2d_host.hip")
-    _run([hipcc, "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-o", HOST_HARNESS, src])
+    base = [hipcc, "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off"]
+    _run(base + ["-DHTX_EXACT_DIVISION=1", "-o", HOST_HARNESS, src])
+    _run(base + ["-o", HOST_HARNESS_FAST, src])
     return HOST_HARNESS
 
 
