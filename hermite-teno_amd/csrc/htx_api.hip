@@ -7,7 +7,12 @@
 #include <vector>
 
 #include "../../include/htx.h"
-#include "htx_internal.h"
+#include "htx_hermite.h"
+
+namespace htx {
+hipError_t launch_fvh2d_stage(const FvhArgs& a, int flux, hipStream_t s);
+hipError_t launch_ic_vortex_hermite(const VortexArgs& a, hipStream_t s);
+}  // namespace htx
 
 using namespace htx;
 
@@ -53,6 +58,10 @@ struct htx_solver {
     double* dB = nullptr;  // stage 1 result
     double* dC = nullptr;  // stage 2 result
     double* dR = nullptr;  // residual output
+    // Hermite scheme only: limited moments, face states and face fluxes
+    double* dLV = nullptr;
+    double *dFX = nullptr, *dFY = nullptr, *dGX = nullptr, *dGY = nullptr;
+    HermConsts hc{};
     Ctl* ctl = nullptr;
     hipStream_t stream = nullptr;
     QuadConsts qc{};
@@ -77,7 +86,6 @@ int check_params(const htx_params* p) {
     if (p->bc != HTX_BC_ZERO_GRAD && p->bc != HTX_BC_PERIODIC) return fail(HTX_EINVAL, "unknown bc");
     if (p->precision != HTX_PREC_FP64 && p->precision != HTX_PREC_FP32_MIXED && p->precision != HTX_PREC_LEARNED)
         return fail(HTX_EINVAL, "unknown precision");
-    if (p->scheme != HTX_SCHEME_TENO5) return fail(HTX_ENOTSUP, "HTENO scheme not built in this library");
     if (p->precision != HTX_PREC_FP64) return fail(HTX_ENOTSUP, "only FP64 precision is built in this library");
     return HTX_OK;
 }
@@ -109,6 +117,36 @@ int flux_kind(const htx_solver* h) {
     return h->p.ref_quirks ? FLUX_HLLC : FLUX_HLLC_TORO;
 }
 
+FvhArgs hermite_args(htx_solver* h, const double* qin, const double* q0, double* qout, int mode) {
+    FvhArgs a{};
+    a.q = qin;
+    a.lv = h->dLV;
+    a.lv_out = h->dLV;
+    a.fx = h->dFX;
+    a.fy = h->dFY;
+    a.gx = h->dGX;
+    a.gy = h->dGY;
+    a.q0 = q0;
+    a.qout = qout;
+    a.ctl = h->ctl;
+    a.nx = h->nx;
+    a.ny = h->ny;
+    a.pitch = h->pitch;
+    a.fstride = h->fstride;
+    a.bc = h->p.bc;
+    a.mode = mode;
+    a.quirks = h->p.ref_quirks ? 1 : 0;
+    a.flux = h->p.flux;
+    a.gam = h->p.gamma;
+    a.g1 = h->p.gamma - 1.0;
+    a.dx = h->dx;
+    a.dy = h->dy;
+    a.area = h->dx * h->dy;
+    a.qc = h->qc;
+    a.hc = h->hc;
+    return a;
+}
+
 int launch_stage(htx_solver* h, const double* qin, const double* q0, double* qout, int mode) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (h->timing) {
@@ -122,8 +160,11 @@ int launch_stage(htx_solver* h, const double* qin, const double* q0, double* qou
         }
         HIP_TRY(hipEventRecord(e0, h->stream));
     }
-    HIP_TRY(launch_fv2d_stage(stage_args(h, qin, q0, qout, mode), flux_kind(h), h->p.tile_x, h->p.tile_y,
-                              h->stream));
+    if (h->p.scheme == HTX_SCHEME_HTENO)
+        HIP_TRY(launch_fvh2d_stage(hermite_args(h, qin, q0, qout, mode), flux_kind(h), h->stream));
+    else
+        HIP_TRY(launch_fv2d_stage(stage_args(h, qin, q0, qout, mode), flux_kind(h), h->p.tile_x, h->p.tile_y,
+                                  h->stream));
     if (h->timing) {
         HIP_TRY(hipEventRecord(e1, h->stream));
         h->events.emplace_back(e0, e1);
@@ -196,12 +237,13 @@ int htx_create(const htx_params* p, htx_solver** out) {
     h->p = *p;
     h->nx = p->nx;
     h->ny = p->ny;
-    h->nfield = 4;
+    h->nfield = p->scheme == HTX_SCHEME_HTENO ? 12 : 4;
     h->pitch = p->nx;
     h->fstride = (long long)h->pitch * p->ny;
     h->dx = (p->xr - p->xl) / p->nx;
     h->dy = (p->yt - p->yb) / p->ny;
     fill_quad_consts(h->qc);
+    fill_herm_consts(h->hc, h->dx);  // Hermite::setGridSize(dx), FVHermite_System2D.cpp:28
     size_t bytes = sizeof(double) * (size_t)h->fstride * h->nfield;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
@@ -212,6 +254,17 @@ int htx_create(const htx_params* p, htx_solver** out) {
         hipMalloc(&h->ctl, sizeof(Ctl)) != hipSuccess) {
         htx_destroy(h);
         return fail(HTX_ENOMEM, "device allocation failed");
+    }
+    if (p->scheme == HTX_SCHEME_HTENO) {
+        size_t nxf = (size_t)(p->nx + 1) * p->ny, nyf = (size_t)p->nx * (p->ny + 1);
+        if (hipMalloc(&h->dLV, sizeof(double) * 8 * (size_t)h->fstride) != hipSuccess ||
+            hipMalloc(&h->dFX, sizeof(double) * 24 * nxf) != hipSuccess ||
+            hipMalloc(&h->dFY, sizeof(double) * 24 * nyf) != hipSuccess ||
+            hipMalloc(&h->dGX, sizeof(double) * 12 * nxf) != hipSuccess ||
+            hipMalloc(&h->dGY, sizeof(double) * 12 * nyf) != hipSuccess) {
+            htx_destroy(h);
+            return fail(HTX_ENOMEM, "device allocation failed");
+        }
     }
     Ctl c{};
     c.t = p->tinit;
@@ -241,6 +294,8 @@ void htx_destroy(htx_solver* h) {
     if (h->dC) (void)hipFree(h->dC);
     if (h->dR) (void)hipFree(h->dR);
     if (h->ctl) (void)hipFree(h->ctl);
+    for (double* b : {h->dLV, h->dFX, h->dFY, h->dGX, h->dGY})
+        if (b) (void)hipFree(b);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -287,7 +342,10 @@ int htx_init_vortex(htx_solver* h, double cx, double cy, const double* modes, in
     for (int m = 0; m < nmodes; ++m)
         for (int k = 0; k < 6; ++k) a.modes[m][k] = modes[6 * m + k];
     gauss9(a.gx, a.gw);
-    HIP_TRY(launch_ic_vortex(a, h->stream));
+    if (h->p.scheme == HTX_SCHEME_HTENO)
+        HIP_TRY(launch_ic_vortex_hermite(a, h->stream));
+    else
+        HIP_TRY(launch_ic_vortex(a, h->stream));
     int rc = refresh_cfl(h);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));

@@ -32,14 +32,6 @@ struct Fv2dTile {
     double sRed[2][NT / 64];
 };
 
-HTX_HD int map_index(int g, int n, int bc) {
-    if (bc == 1) {
-        g = g % n;
-        return g < 0 ? g + n : g;
-    }
-    return g < 0 ? 0 : (g >= n ? n - 1 : g);
-}
-
 // halo load with the boundary condition applied through the index, plus the
 // per-cell Roe inputs of calcPrimitiveVariable (FV_System2D.cpp:226-263)
 template <int TX, int TY>

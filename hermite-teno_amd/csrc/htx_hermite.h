@@ -1,0 +1,419 @@
+// htx_hermite.h -- device numerics of the Hermite-TENO scheme
+// (Schemes::Hermite, Schemes_Hermite.cpp) and the per-face / per-line
+// bodies of the FVHermite 2D kernels (FVHermite_System2D.cpp:393-731),
+// written as host+device functions of one work item so that the GPU kernels
+// and a CPU build of them run the same code.
+#pragma once
+#include "htx_internal.h"
+
+namespace htx {
+
+// Coefficients of Hermite::setGridSize (Schemes_Hermite.cpp:31-66) for the
+// grid size dif, filled on the host with the reference's expressions.
+struct HermConsts {
+    double crj[3][3], crjd[4][3];
+    double dkm[3], dkp[3], dkmd[3], dkpd[3];
+    double dif;
+};
+
+inline void fill_herm_consts(HermConsts& H, double dif) {
+    H.dif = dif;
+    H.crj[0][0] = -7.0 / 6.0; H.crj[1][0] = 13.0 / 6.0; H.crj[2][0] = -2.0 / 3.0 * dif;
+    H.crj[0][1] = 1.0 / 6.0;  H.crj[1][1] = 5.0 / 6.0;  H.crj[2][1] = -1.0 / 3.0 * dif;
+    H.crj[0][2] = -1.0 / 6.0; H.crj[1][2] = 5.0 / 6.0;  H.crj[2][2] = 1.0 / 3.0;
+    H.crjd[0][0] = 4.0 / dif;  H.crjd[1][0] = -4.0 / dif; H.crjd[2][0] = 1.5;        H.crjd[3][0] = 3.5;
+    H.crjd[0][1] = -2.0 / dif; H.crjd[1][1] = 2.0 / dif;  H.crjd[2][1] = -0.5;       H.crjd[3][1] = -0.5;
+    H.crjd[0][2] = 0.25 / dif; H.crjd[1][2] = -1.0 / dif; H.crjd[2][2] = 0.75 / dif; H.crjd[3][2] = 0.5;
+    H.dkm[0] = 9.0 / 80.0; H.dkm[1] = 42.0 / 80.0; H.dkm[2] = 29.0 / 80.0;
+    H.dkp[0] = H.dkm[1]; H.dkp[1] = H.dkm[0]; H.dkp[2] = H.dkm[2];
+    H.dkmd[0] = 1.0 / 18.0; H.dkmd[1] = 5.0 / 6.0; H.dkmd[2] = 1.0 / 9.0;
+    H.dkpd[0] = H.dkmd[1]; H.dkpd[1] = H.dkmd[0]; H.dkpd[2] = H.dkmd[2];
+}
+
+// Hermite::fullPointsSI (Schemes_Hermite.cpp:597-603); u, d are 3 consecutive
+// cell averages and scaled derivative moments.
+HTX_HD double herm_brk(double u0, double u1, double u2, double d0, double d2) {
+    HTX_NO_CONTRACT
+    return fabs((1029270.0 * u2 - 2129184.0 * u1 + 1099914.0 * u0 + 281121.0 * d0 - 494958.0 * d2) * d0 +
+                (-1029270.0 * u0 + 2129184.0 * u1 - 1099914.0 * u2 + 281121.0 * d2) * d2 +
+                (1099445.0 * u2 - 4317056.0 * u1 + 2118166.0 * u0) * u2 + 4317056.0 * (u1 - u0) * u1 +
+                1099445.0 * u0 * u0) /
+           6720.0;
+}
+
+// Hermite::fullPointsSId (:605-612)
+HTX_HD double herm_brkd(double u0, double u1, double u2, double d0, double d1, double d2) {
+    HTX_NO_CONTRACT
+    return fabs((3156862.0 * d2 + 13189240.0 * d1 + 1710479.0 * d0 - 9617754.0 * u2 - 531552.0 * u1 +
+                 10149306.0 * u0) * d0 +
+                (13189240.0 * d2 + 26511200.0 * d1 - 39700440.0 * u2 + 39700440.0 * u0) * d1 +
+                (1710479.0 * d2 - 10149306.0 * u2 + 531552.0 * u1 + 9617754.0 * u0) * d2 +
+                (15136011.0 * u2 - 1076544.0 * u1 - 29195478.0 * u0) * u2 + 1076544.0 * (u1 - u0) * u1 +
+                15136011.0 * u0 * u0) /
+           1680.0;
+}
+
+// one HTENO side (Schemes_Hermite.cpp:311-320 pattern): tau from the full
+// indicator, gamma^6, cutOff at C_T = 1e-3 (:614-628), weights, catchnan
+HTX_HD double hteno_side(double brk, double s0, double s1, double s2, double p0, double p1, double p2, double d0,
+                         double d1, double d2) {
+    HTX_NO_CONTRACT
+    double tau = fabs(brk - 0.25 * (s0 + s1 + 2.0 * s2));
+    double g0 = 1.0 + tau / (s0 + 1e-40);
+    double g1 = 1.0 + tau / (s1 + 1e-40);
+    double g2 = 1.0 + tau / (s2 + 1e-40);
+    g0 = g0 * g0 * g0; g0 = g0 * g0;
+    g1 = g1 * g1 * g1; g1 = g1 * g1;
+    g2 = g2 * g2 * g2; g2 = g2 * g2;
+    double gs = (g0 + g1) + g2;
+    bool k0 = !(g0 / gs < 1e-3), k1 = !(g1 / gs < 1e-3), k2 = !(g2 / gs < 1e-3);
+    double w0 = k0 ? d0 : 0.0, w1 = k1 ? d1 : 0.0, w2 = k2 ? d2 : 0.0;
+    double ws = (w0 + w1) + w2;
+#if HTX_EXACT_DIVISION
+    w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
+    double out = (p0 * w0 + p1 * w1) + p2 * w2;
+#else
+    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * (1.0 / ws);
+#endif
+    return out != out ? 1e-15 : out;
+}
+
+// Hermite::smoothnessMeasure (:515-526) and smoothnessMeasureDerivative (:528-541)
+HTX_HD void herm_si(double u0, double u1, double u2, double v0, double v2, double& s0, double& s1, double& s2) {
+    HTX_NO_CONTRACT
+    const double c1 = 13.0 / 12.0;
+    double b0 = -2.0 * (u0 - u1) - v0, c0 = -2.0 * (u0 - u1 + v0);
+    double b1 = -2.0 * (u1 - u2) - v2, cc1 = 2.0 * (u1 - u2) + 2.0 * v2;
+    double b2 = (-u0 + u2) / 2.0, c2 = u0 - 2.0 * u1 + u2;
+    s0 = c1 * (c0 * c0) + b0 * b0;
+    s1 = c1 * (cc1 * cc1) + b1 * b1;
+    s2 = c1 * (c2 * c2) + b2 * b2;
+}
+
+HTX_HD void herm_sid(double u0, double u1, double u2, double d0, double d1, double d2, double& s0, double& s1,
+                     double& s2) {
+    HTX_NO_CONTRACT
+    const double c1 = 13.0 / 12.0;
+    double b0 = 6.0 * (u0 - u1) + (2.0 * d0 + 4.0 * d1), c0 = 12.0 * (u0 - u1) + 6.0 * (d0 + d1);
+    double b1 = 6.0 * (u2 - u1) - (4.0 * d1 + 2.0 * d2), cc1 = 12.0 * (u1 - u2) + 6.0 * (d1 + d2);
+    double b2 = u0 - 2.0 * u1 + u2, c2 = 3.0 * (u2 - u0) - 6.0 * d1;
+    s0 = c1 * (c0 * c0) + b0 * b0;
+    s1 = c1 * (cc1 * cc1) + b1 * b1;
+    s2 = c1 * (c2 * c2) + b2 * b2;
+}
+
+// Hermite::hteno (Schemes_Hermite.cpp:295-379): cell averages u_{i-1..i+2} and
+// derivative moments d -> (u^-, u^+, d^-, d^+) at x_{i+1/2}.
+HTX_HD void hteno(const HermConsts& H, const double u[4], const double d[4], double out[4]) {
+    HTX_NO_CONTRACT
+    const double dif = H.dif;
+    double ud0 = d[0] * dif, ud1 = d[1] * dif, ud2 = d[2] * dif, ud3 = d[3] * dif;
+    double s0, s1, s2, p0, p1, p2;
+    // u^-
+    p0 = H.crj[0][0] * u[0] + H.crj[1][0] * u[1] + H.crj[2][0] * d[0];
+    p1 = H.crj[0][1] * u[1] + H.crj[1][1] * u[2] + H.crj[2][1] * d[2];
+    p2 = H.crj[0][2] * u[0] + H.crj[1][2] * u[1] + H.crj[2][2] * u[2];
+    herm_si(u[0], u[1], u[2], ud0, ud2, s0, s1, s2);
+    out[0] = hteno_side(herm_brk(u[0], u[1], u[2], ud0, ud2), s0, s1, s2, p0, p1, p2, H.dkm[0], H.dkm[1], H.dkm[2]);
+    // d^-
+    p0 = H.crjd[0][0] * u[0] + H.crjd[1][0] * u[1] + H.crjd[2][0] * d[0] + H.crjd[3][0] * d[1];
+    p1 = H.crjd[0][1] * u[1] + H.crjd[1][1] * u[2] + H.crjd[2][1] * d[1] + H.crjd[3][1] * d[2];
+    p2 = H.crjd[0][2] * u[0] + H.crjd[1][2] * u[1] + H.crjd[2][2] * u[2] + H.crjd[3][2] * d[1];
+    herm_sid(u[0], u[1], u[2], ud0, ud1, ud2, s0, s1, s2);
+    out[2] = hteno_side(herm_brkd(u[0], u[1], u[2], ud0, ud1, ud2), s0, s1, s2, p0, p1, p2, H.dkmd[0], H.dkmd[1],
+                        H.dkmd[2]);
+    // u^+
+    p0 = H.crj[1][1] * u[1] + H.crj[0][1] * u[2] - H.crj[2][1] * d[1];
+    p1 = H.crj[1][0] * u[2] + H.crj[0][0] * u[3] - H.crj[2][0] * d[3];
+    p2 = H.crj[2][2] * u[1] + H.crj[1][2] * u[2] + H.crj[0][2] * u[3];
+    herm_si(u[1], u[2], u[3], ud1, ud3, s0, s1, s2);
+    out[1] = hteno_side(herm_brk(u[1], u[2], u[3], ud1, ud3), s0, s1, s2, p0, p1, p2, H.dkp[0], H.dkp[1], H.dkp[2]);
+    // d^+
+    p0 = H.crjd[0][1] * u[1] + H.crjd[1][1] * u[2] + H.crjd[3][1] * d[1] + H.crjd[2][1] * d[2];
+    p1 = H.crjd[0][0] * u[2] + H.crjd[1][0] * u[3] + H.crjd[3][0] * d[2] + H.crjd[2][0] * d[3];
+    p2 = -(H.crjd[2][2] * u[1] + H.crjd[1][2] * u[2] + H.crjd[0][2] * u[3]) + H.crjd[3][2] * d[2];
+    herm_sid(u[1], u[2], u[3], ud1, ud2, ud3, s0, s1, s2);
+    out[3] = hteno_side(herm_brkd(u[1], u[2], u[3], ud1, ud2, ud3), s0, s1, s2, p0, p1, p2, H.dkpd[0], H.dkpd[1],
+                        H.dkpd[2]);
+}
+
+// Hermite::hermiteLimiter (Schemes_Hermite.cpp:634-662): u and derivative
+// moments of cells i-1, i, i+1 -> the limited moment of cell i.
+HTX_HD double herm_limiter(double dif, double u0, double u1, double u2, double d0, double d2) {
+    HTX_NO_CONTRACT
+    double p0 = 0.75 * (u2 - u0) / dif - 0.25 * (d0 + d2);
+    double p1 = (u1 - u0) / dif;
+    double p2 = (u2 - u1) / dif;
+    const double l0 = 1.0 - 2E-5, l1 = 1E-5, l2 = 1E-5;
+    double sm0 = herm_brk(u0, u1, u2, d0 * dif, d2 * dif);
+    double sm1 = (u0 - u1) * (u0 - u1);
+    double sm2 = (u1 - u2) * (u1 - u2);
+    double a = fabs(sm0 - sm1) + fabs(sm0 - sm2);
+    double tau = a * a / 4.0;
+    double w0 = l0 * (1.0 + tau / (sm0 + 1E-6));
+    double w1 = l1 * (1.0 + tau / (sm1 + 1E-6));
+    double w2 = l2 * (1.0 + tau / (sm2 + 1E-6));
+    double ws = (w0 + w1) + w2;
+    w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
+    return w0 * (p0 / l0 - ((l1 / l0) * p1 + (l2 / l0) * p2)) + (w1 * p1 + w2 * p2);
+}
+
+// Flux Jacobian times a moment vector, Euler2D::dfluxx/dfluxy
+// (Equations_Euler2D.cpp:52-98), in (rho, normal, tangential, E) order.
+HTX_HD void dflux_mul(double gam, double g1, double q3, double rho, double un, double ut, double p,
+                      const double dq[4], double out[4]) {
+    double g3 = gam - 3.0;
+    double sq = un * un;
+    double dyn = 0.5 * g1 * (sq + ut * ut);
+    double h = (q3 + p) / rho;
+    out[0] = dq[1];
+    out[1] = (((dyn - sq) * dq[0] + (-g3 * un) * dq[1]) + (-g1 * ut) * dq[2]) + g1 * dq[3];
+    out[2] = ((-un * ut) * dq[0] + ut * dq[1]) + un * dq[2];
+    out[3] = ((((dyn - h) * un) * dq[0] + (h - g1 * sq) * dq[1]) + (-g1 * un * ut) * dq[2]) + (gam * un) * dq[3];
+}
+
+// Hermite state layout on the device: 12 fields, u (4), v = <dq/dx> (4),
+// w = <dq/dy> (4), each nx*ny with pitch; face buffers hold 24 values per face
+// (series-major: u^-, u^+, v^-, v^+, w^-, w^+ for each of 4 characteristic
+// fields); flux buffers 12 per face (u, v, w fluxes).
+struct FvhArgs {
+    const double* q;     // current 12-field state (u, v, w)
+    const double* lv;    // limited moments: 8 fields (v', w')
+    double* lv_out;      // limiter output
+    double* fx;          // x-face states, 24 fields of ny x (nx+1)
+    double* fy;          // y-face states, 24 fields of (ny+1) x nx
+    double* gx;          // x-face fluxes, 12 fields of ny x (nx+1)
+    double* gy;          // y-face fluxes, 12 fields of (ny+1) x nx
+    const double* q0;    // u^n (stages 2, 3)
+    double* qout;        // residual (mode 0) or stage result
+    Ctl* ctl;
+    int nx, ny, pitch;
+    long long fstride;   // per field of the state
+    int bc, mode, quirks, flux;
+    double gam, g1, dx, dy, area;
+    QuadConsts qc;
+    HermConsts hc;
+};
+
+HTX_HD long long fvh_off(const FvhArgs& a, int i, int j) {
+    return (long long)map_index(j, a.ny, a.bc) * a.pitch + map_index(i, a.nx, a.bc);
+}
+
+// limitDerivatives, FVHermite_System2D.cpp:393-427: one sequential chain.
+// dir 0: row j, field e, v replaced with i ascending (Gauss-Seidel: the new
+// v_{i-1} is used, ghost neighbours keep their pre-limiting values); dir 1:
+// column i, w replaced with j ascending.
+HTX_HD void fvh_limit_line(const FvhArgs& a, int dir, int line, int e) {
+    HTX_NO_CONTRACT
+    const long long fs = a.fstride;
+    const double* U = a.q + e * fs;
+    const double* D = a.q + (dir == 0 ? 4 + e : 8 + e) * fs;
+    double* O = a.lv_out + (dir == 0 ? e : 4 + e) * fs;
+    int n = dir == 0 ? a.nx : a.ny;
+    auto at = [&](int k) { return dir == 0 ? fvh_off(a, k, line) : fvh_off(a, line, k); };
+    double prev = D[at(-1)];  // ghost: pre-limiting value
+    for (int k = 0; k < n; ++k) {
+        double v = herm_limiter(a.hc.dif, U[at(k - 1)], U[at(k)], U[at(k + 1)], prev, D[at(k + 1)]);
+        O[at(k)] = v;
+        prev = v;
+    }
+}
+
+// limited moment of the state at (i, j): the limited interior value at the
+// boundary-mapped cell (the reference re-applies the boundary condition
+// after limiting, FVHermite_System2D.cpp:437)
+HTX_HD double fvh_lim(const FvhArgs& a, int comp, int i, int j) { return a.lv[comp * a.fstride + fvh_off(a, i, j)]; }
+
+// x or y face reconstruction (FVHermite_System2D.cpp:443-495) for the face
+// between cell (i, j) and its +1 neighbour in direction YDIR; writes the 24
+// characteristic face values at face index fid of the face buffer.
+template <bool YDIR>
+HTX_HD void fvh_recon_face(const FvhArgs& a, int i, int j, double* F, long long fid, long long fstr) {
+    const long long fs = a.fstride;
+    auto cellq = [&](int k, double q[4]) {  // k: offset along the normal
+        long long o = YDIR ? fvh_off(a, i, j + k) : fvh_off(a, i + k, j);
+        q[0] = a.q[o]; q[1] = a.q[o + fs]; q[2] = a.q[o + 2 * fs]; q[3] = a.q[o + 3 * fs];
+    };
+    double qa[4], qb[4];
+    cellq(0, qa);
+    cellq(1, qb);
+    Eig E = roe_eig(a.g1, YDIR, cell_prim(a.g1, qa[0], qa[1], qa[2], qa[3]), cell_prim(a.g1, qb[0], qb[1], qb[2], qb[3]));
+    // hteno on (L u, L d_normal) over cells -1..2, teno5 on L d_tangential over -2..3
+    double A[4][4], B[4][4], C[4][6];
+    const int nc = YDIR ? 4 : 0;  // normal-derivative block: v (x) or w (y) -> limited comp offset
+    for (int m = 0; m < 6; ++m) {
+        int k = m - 2;
+        int ci = YDIR ? i : i + k, cj = YDIR ? j + k : j;
+        double q[4];
+        cellq(k, q);
+        double dn[4], dt[4];
+        for (int e = 0; e < 4; ++e) {
+            dn[e] = fvh_lim(a, nc + e, ci, cj);        // normal derivative moment
+            dt[e] = fvh_lim(a, (4 - nc) + e, ci, cj);  // tangential derivative moment
+        }
+        double c[4];
+        if (m >= 1 && m <= 4) {
+            eig_left<YDIR>(E, q[0], q[1], q[2], q[3], c[0], c[1], c[2], c[3]);
+            for (int e = 0; e < 4; ++e) A[e][m - 1] = c[e];
+            eig_left<YDIR>(E, dn[0], dn[1], dn[2], dn[3], c[0], c[1], c[2], c[3]);
+            for (int e = 0; e < 4; ++e) B[e][m - 1] = c[e];
+        }
+        eig_left<YDIR>(E, dt[0], dt[1], dt[2], dt[3], c[0], c[1], c[2], c[3]);
+        for (int e = 0; e < 4; ++e) C[e][m] = c[e];
+    }
+    for (int e = 0; e < 4; ++e) {
+        double o[4], um, up;
+        hteno(a.hc, A[e], B[e], o);
+        teno5(C[e], um, up);
+        // series: 0 u^-, 1 u^+, 2 vn^-, 3 vn^+, 4 vt^-, 5 vt^+ (vn: normal moment)
+        F[(0 * 4 + e) * fstr + fid] = o[0];
+        F[(1 * 4 + e) * fstr + fid] = o[1];
+        F[(2 * 4 + e) * fstr + fid] = o[2];
+        F[(3 * 4 + e) * fstr + fid] = o[3];
+        F[(4 * 4 + e) * fstr + fid] = um;
+        F[(5 * 4 + e) * fstr + fid] = up;
+    }
+}
+
+}  // namespace htx
+
+namespace htx {
+
+template <bool YDIR>
+HTX_HD void rot_to_nat(const double r[4], double& q0, double& qx, double& qy, double& q3) {
+    q0 = r[0];
+    qx = YDIR ? r[2] : r[1];
+    qy = YDIR ? r[1] : r[2];
+    q3 = r[3];
+}
+
+// Euler2D::dRusanov (Equations_Euler2D.cpp:352-360) in (rho, n, t, E) order:
+// 0.5 (Jl dql + Jr dqr - R diag(a) L (dqr - dql)).
+template <bool YDIR>
+HTX_HD void drusanov(double gam, double g1, const Eig& E, const double ql[4], const double qr[4], double pl,
+                     double pr, const double a[4], const double dql[4], const double dqr[4], double f[4]) {
+    double jl[4], jr[4], dd[4], c0, c1, c2, c3, r0, rn, rt, r3;
+    double il = 1.0 / ql[0], ir = 1.0 / qr[0];
+    dflux_mul(gam, g1, ql[3], ql[0], ql[1] * il, ql[2] * il, pl, dql, jl);
+    dflux_mul(gam, g1, qr[3], qr[0], qr[1] * ir, qr[2] * ir, pr, dqr, jr);
+    for (int e = 0; e < 4; ++e) dd[e] = dqr[e] - dql[e];
+    double n0, nx, ny, n3;
+    rot_to_nat<YDIR>(dd, n0, nx, ny, n3);
+    eig_left<YDIR>(E, n0, nx, ny, n3, c0, c1, c2, c3);
+    eig_right(E, a[0] * c0, a[1] * c1, a[1] * c2, a[3] * c3, r0, rn, rt, r3);
+    f[0] = 0.5 * ((jl[0] + jr[0]) - r0);
+    f[1] = 0.5 * ((jl[1] + jr[1]) - rn);
+    f[2] = 0.5 * ((jl[2] + jr[2]) - rt);
+    f[3] = 0.5 * ((jl[3] + jr[3]) - r3);
+}
+
+// Transverse quadrature, back-projection and the u, v, w fluxes of one face
+// at one Gauss point m (FVHermite_System2D.cpp:497-640).  The face lies
+// between cell (i, j) and its +1 neighbour in direction YDIR; the 5
+// transverse face-state rows are read at boundary-mapped positions.  Returns
+// the 12 weighted fluxes (u, v, w; each in (rho, x, y, E) order); the caller
+// sums the three Gauss points in order.
+template <bool YDIR, int FLUX>
+HTX_HD void fvh_flux_gp(const FvhArgs& a, int i, int j, int m, const double* F, long long fstr, double out[12]) {
+    long long fid[5];
+    for (int k = 0; k < 5; ++k) {
+        if (YDIR)
+            fid[k] = (long long)(j + 1) * a.nx + map_index(i - 2 + k, a.nx, a.bc);
+        else
+            fid[k] = (long long)map_index(j - 2 + k, a.ny, a.bc) * (a.nx + 1) + (i + 1);
+    }
+    double qa[4], qb[4];
+    {
+        const long long fs = a.fstride;
+        long long oa = fvh_off(a, i, j), ob = YDIR ? fvh_off(a, i, j + 1) : fvh_off(a, i + 1, j);
+        for (int e = 0; e < 4; ++e) {
+            qa[e] = a.q[oa + e * fs];
+            qb[e] = a.q[ob + e * fs];
+        }
+    }
+    Eig E = roe_eig(a.g1, YDIR, cell_prim(a.g1, qa[0], qa[1], qa[2], qa[3]), cell_prim(a.g1, qb[0], qb[1], qb[2], qb[3]));
+    double Q[6][4];
+    for (int sr = 0; sr < 6; ++sr) {
+        double c[4];
+        for (int e = 0; e < 4; ++e) {
+            const double* S = F + (long long)(sr * 4 + e) * fstr;
+            c[e] = weno3q_gp(a.qc, m, S[fid[0]], S[fid[1]], S[fid[2]], S[fid[3]], S[fid[4]]);
+        }
+        eig_right(E, c[0], c[1], c[2], c[3], Q[sr][0], Q[sr][1], Q[sr][2], Q[sr][3]);
+    }
+    // primitive states at the Gauss point and maxLocalWaveSpeed (Equations_Euler2D.cpp:221-246)
+    double fl0, fln, flt, fl3, wnl, pl, rl, fr0, frn, frt, fr3, wnr, pr, rr;
+    phys_flux(a.g1, Q[0][0], Q[0][1], Q[0][2], Q[0][3], fl0, fln, flt, fl3, wnl, pl, rl);
+    phys_flux(a.g1, Q[1][0], Q[1][1], Q[1][2], Q[1][3], fr0, frn, frt, fr3, wnr, pr, rr);
+    double cl = sqrt(fabs(a.gam * pl * (1.0 / rl))), cr = sqrt(fabs(a.gam * pr * (1.0 / rr)));
+    double am[4];
+    am[0] = fmax(fabs(wnl - cl), fabs(wnr - cr));
+    am[1] = fmax(fabs(wnl), fabs(wnr));
+    am[2] = am[1];
+    am[3] = fmax(fabs(wnl + cl), fabs(wnr + cr));
+    double fu[4], fv[4], fw[4];
+    if (FLUX == FLUX_RUSANOV_CHAR)
+        rusanov_char<YDIR>(a.gam, a.g1, E, Q[0], Q[1], fu);
+    else if (FLUX == FLUX_HLLC)
+        hllc<true>(a.gam, a.g1, Q[0], Q[1], fu);
+    else
+        hllc<false>(a.gam, a.g1, Q[0], Q[1], fu);
+    // series 2,3 carry the normal moment (v in x, w in y), 4,5 the tangential one
+    const int sv = YDIR ? 4 : 2, sw = YDIR ? 2 : 4;
+    drusanov<YDIR>(a.gam, a.g1, E, Q[0], Q[1], pl, pr, am, Q[sv], Q[sv + 1], fv);
+    drusanov<YDIR>(a.gam, a.g1, E, Q[0], Q[1], pl, pr, am, Q[sw], Q[sw + 1], fw);
+    const double gw = (m == 1) ? 8.0 / 9.0 / 2.0 : 5.0 / 9.0 / 2.0;
+    const double* fs3[3] = {fu, fv, fw};
+    for (int g = 0; g < 3; ++g) {
+        const double* f = fs3[g];
+        out[4 * g + 0] = gw * f[0];
+        out[4 * g + 1] = gw * (YDIR ? f[2] : f[1]);
+        out[4 * g + 2] = gw * (YDIR ? f[1] : f[2]);
+        out[4 * g + 3] = gw * f[3];
+    }
+}
+
+// divergence of the 12 fluxes (FVHermite_System2D.cpp:710-730), SSP-RK3
+// stage (:733-782) and, in stage 3, the CFL maxima of u.  In stage 1 the
+// base state u^n of v and w is the limited one (save() runs after rhs()
+// limited them in place), so it is also written back into q0.
+HTX_HD void fvh_update_cell(const FvhArgs& a, int i, int j, double dt, double& smx, double& smy) {
+    const long long fs = a.fstride;
+    long long off = (long long)j * a.pitch + i;
+    long long gxs = (long long)a.ny * (a.nx + 1), gys = (long long)(a.ny + 1) * a.nx;
+    long long bx = (long long)j * (a.nx + 1) + i, by = (long long)j * a.nx + i;
+    double unew[4];
+    for (int c = 0; c < 12; ++c) {
+        double rhs = ((a.gx[c * gxs + bx] - a.gx[c * gxs + bx + 1]) * a.dy +
+                      (a.gy[c * gys + by] - a.gy[c * gys + by + a.nx]) * a.dx) /
+                     a.area;
+        double us = a.q[off + c * fs];
+        double v;
+        if (a.mode == 0) {
+            v = rhs;
+        } else if (a.mode == 1) {
+            double base = c < 4 ? us : a.lv[(c - 4) * fs + off];
+            v = base + dt * rhs;
+            if (c >= 4) const_cast<double*>(a.q0)[off + c * fs] = base;
+        } else if (a.mode == 2) {
+            v = 0.75 * a.q0[off + c * fs] + 0.25 * (us + rhs * dt);
+        } else {
+            v = (a.q0[off + c * fs] + 2.0 * (us + dt * rhs)) / 3.0;
+        }
+        if (c < 4) unew[c] = v;
+        a.qout[off + c * fs] = v;
+    }
+    if (a.mode == 3) {
+        double wu = unew[1] / unew[0], wv = unew[2] / unew[0];
+        double p = (unew[3] - 0.5 * (unew[1] * wu + unew[2] * wv)) * a.g1;
+        if (a.quirks) {
+            smx = fmax(smx, max_speed<true>(a.gam, wu, p, unew[0]));
+            smy = fmax(smy, max_speed<true>(a.gam, wv, p, unew[0]));
+        } else {
+            smx = fmax(smx, max_speed<false>(a.gam, wu, p, unew[0]));
+            smy = fmax(smy, max_speed<false>(a.gam, wv, p, unew[0]));
+        }
+    }
+}
+
+}  // namespace htx

@@ -44,6 +44,17 @@ struct VortexArgs {
     double gx[9], gw[9];
 };
 
+// Boundary conditions are applied through the index: zero gradient clamps,
+// periodic wraps.  This is what zeroGradBc / periodicBc (FV_System2D.cpp
+// :114-152) leave in every ghost cell a stencil can reach.
+HTX_HD int map_index(int g, int n, int bc) {
+    if (bc == 1) {
+        g = g % n;
+        return g < 0 ? g + n : g;
+    }
+    return g < 0 ? 0 : (g >= n ? n - 1 : g);
+}
+
 // launchers (htx_fv2d.hip)
 hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y, hipStream_t s);
 hipError_t launch_dt(Ctl* c, double dx, double dy, hipStream_t s);

@@ -228,6 +228,65 @@ HTX_HD void weno3q(const QuadConsts& C, double u0, double u1, double u2, double 
     o2 = o2 != o2 ? 1e-15 : o2;
 }
 
+// One Gauss point of weno3q (m = 0: +xi, 1: centre, 2: -xi), evaluated with
+// exactly the operations weno3q uses for that output.
+HTX_HD double weno3q_gp(const QuadConsts& C, int m, double u0, double u1, double u2, double u3, double u4) {
+    const double c1 = 13.0 / 12.0;
+    double bg0 = u0 - 4.0 * u1 + 3.0 * u2, cg0 = u0 - 2.0 * u1 + u2;
+    double bg1 = u1 - u3, cg1 = u1 - 2.0 * u2 + u3;
+    double bg2 = 3.0 * u2 - 4.0 * u3 + u4, cg2 = u2 - 2.0 * u3 + u4;
+    double s0 = 0.25 * (bg0 * bg0) + c1 * (cg0 * cg0);
+    double s1 = 0.25 * (bg1 * bg1) + c1 * (cg1 * cg1);
+    double s2 = 0.25 * (bg2 * bg2) + c1 * (cg2 * cg2);
+    double e0 = 1e-6 + s0, e1 = 1e-6 + s1, e2 = 1e-6 + s2;
+    e0 = e0 * e0; e1 = e1 * e1; e2 = e2 * e2;
+    double w0, w1, w2, ws, p0, p1, p2, o;
+    if (m == 1) {
+#if HTX_EXACT_DIVISION
+        p0 = (-u0 + 2.0 * u1 + 23.0 * u2) / 24.0;
+        p1 = (-u1 + 26.0 * u2 - u3) / 24.0;
+        p2 = (23.0 * u2 + 2.0 * u3 - u4) / 24.0;
+        w0 = C.linw0p[0] / e0; w1 = C.linw0p[1] / e1; w2 = C.linw0p[2] / e2;
+        ws = (w0 + w1) + w2;
+        w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
+        o = C.sumw0p * ((p0 * w0 + p1 * w1) + p2 * w2);
+        w0 = C.linw0m[0] / e0; w1 = C.linw0m[1] / e1; w2 = C.linw0m[2] / e2;
+        ws = (w0 + w1) + w2;
+        w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
+        o -= C.sumw0m * ((p0 * w0 + p1 * w1) + p2 * w2);
+#else
+        double r0 = 1.0 / e0, r1 = 1.0 / e1, r2 = 1.0 / e2;
+        const double r24 = 1.0 / 24.0;
+        p0 = (-u0 + 2.0 * u1 + 23.0 * u2) * r24;
+        p1 = (-u1 + 26.0 * u2 - u3) * r24;
+        p2 = (23.0 * u2 + 2.0 * u3 - u4) * r24;
+        w0 = C.linw0p[0] * r0; w1 = C.linw0p[1] * r1; w2 = C.linw0p[2] * r2;
+        ws = (w0 + w1) + w2;
+        o = C.sumw0p * (((p0 * w0 + p1 * w1) + p2 * w2) / ws);
+        w0 = C.linw0m[0] * r0; w1 = C.linw0m[1] * r1; w2 = C.linw0m[2] * r2;
+        ws = (w0 + w1) + w2;
+        o -= C.sumw0m * (((p0 * w0 + p1 * w1) + p2 * w2) / ws);
+#endif
+    } else {
+        const double(*cc)[3] = m == 0 ? C.gp : C.gm;
+        const double* lw = m == 0 ? C.linwp : C.linwm;
+        p0 = (cc[0][0] * u0 + cc[1][0] * u1) + cc[2][0] * u2;
+        p1 = (cc[0][1] * u1 + cc[1][1] * u2) + cc[2][1] * u3;
+        p2 = (cc[0][2] * u2 + cc[1][2] * u3) + cc[2][2] * u4;
+#if HTX_EXACT_DIVISION
+        w0 = lw[0] / e0; w1 = lw[1] / e1; w2 = lw[2] / e2;
+        ws = (w0 + w1) + w2;
+        w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
+        o = (p0 * w0 + p1 * w1) + p2 * w2;
+#else
+        w0 = lw[0] * (1.0 / e0); w1 = lw[1] * (1.0 / e1); w2 = lw[2] * (1.0 / e2);
+        ws = (w0 + w1) + w2;
+        o = ((p0 * w0 + p1 * w1) + p2 * w2) / ws;
+#endif
+    }
+    return o != o ? 1e-15 : o;
+}
+
 // Per-cell quantities of calcPrimitiveVariable (FV_System2D.cpp:226-263).
 HTX_HD Prim cell_prim(double g1, double q0, double q1, double q2, double q3) {
     HTX_NO_CONTRACT

@@ -22,7 +22,7 @@ LIBHTX = os.path.join(PKG_DIR, "libhtx.so")
 LIBORACLE = os.path.join(ORACLE, "liboracle.so")
 LIBORACLE_OMP = os.path.join(ORACLE, "liboracle_omp.so")
 
-HIP_SOURCES = ["htx_fv2d.hip", "htx_api.hip"]
+HIP_SOURCES = ["htx_fv2d.hip", "htx_fvh2d.hip", "htx_api.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -70,6 +70,7 @@ def build_oracle():
 
 HOST_HARNESS = os.path.join(ROOT, "tests", "host", "libfv2d_host.so")
 HOST_HARNESS_FAST = os.path.join(ROOT, "tests", "host", "libfv2d_host_fast.so")
+HOST_HARNESS_HERMITE = os.path.join(ROOT, "tests", "host", "libfvh2d_host.so")
 
 
 def build_host_harness():
@@ -82,6 +83,8 @@ def build_host_harness():
     base = [hipcc, "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off"]
     _run(base + ["-DHTX_EXACT_DIVISION=1", "-o", HOST_HARNESS, src])
     _run(base + ["-o", HOST_HARNESS_FAST, src])
+    srch = os.path.join(ROOT, "tests", "host", "fvh2d_host.hip")
+    _run(base + ["-DHTX_EXACT_DIVISION=1", "-o", HOST_HARNESS_HERMITE, srch])
     return HOST_HARNESS
 
 

@@ -999,7 +999,18 @@ static void fvh_limit(const padded* P, double** u, double** v, double** w, doubl
 
 /* FVHermite::System2D::rhs, FVHermite_System2D.cpp:429-731.  q holds 12
  * fields: u (4), v = <dq/dx> (4), w = <dq/dy> (4). */
-void orc_fvh2d_rhs(const orc_params* p, const double* q, double* rhs) {
+static void fvh2d_rhs_impl(const orc_params* p, const double* q, double* rhs, double* qlim);
+
+void orc_fvh2d_rhs(const orc_params* p, const double* q, double* rhs) { fvh2d_rhs_impl(p, q, rhs, NULL); }
+
+/* Residual plus the state as rhs() leaves it: limitDerivatives() replaces
+ * v and w in place (FVHermite_System2D.cpp:397-426), so after the call the
+ * object's state carries the limited moments. */
+void orc_fvh2d_rhs_lim(const orc_params* p, const double* q, double* rhs, double* qlim) {
+    fvh2d_rhs_impl(p, q, rhs, qlim);
+}
+
+static void fvh2d_rhs_impl(const orc_params* p, const double* q, double* rhs, double* qlim) {
     lag_init();
     padded P;
     int nx = p->nx, ny = p->ny, k = FVH_K, s = FVH_S;
@@ -1016,6 +1027,12 @@ void orc_fvh2d_rhs(const orc_params* p, const double* q, double* rhs) {
     fvh_limit(&P, U, V, W, dx);              /* :435 */
     for (int f = 0; f < 12; ++f) fill_ghosts(&P, P.q[f], p->bc); /* :437 */
     int nxx = P.nxx, nyy = P.nyy;
+    if (qlim) {
+        size_t nn = (size_t)nx * ny;
+        for (int f = 0; f < 12; ++f)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) qlim[f * nn + (size_t)j * nx + i] = P.q[f][PIX(&P, i + k, j + k)];
+    }
     size_t nt = (size_t)nxx * nyy;
     /* face states, 6 series (u, v, w left/right) x 4 fields per direction */
     double* FX = (double*)calloc(24 * nt, sizeof(double)); /* uxl uxr vxl vxr wxl wxr */
@@ -1156,14 +1173,16 @@ void orc_fvh2d_rhs(const orc_params* p, const double* q, double* rhs) {
     pad_free(&P, 12);
 }
 
-/* FVHermite::System2D::RungeKutta3, :733-782 (u, v and w) */
+/* FVHermite::System2D::RungeKutta3, :733-782 (u, v and w).  save() runs
+ * after the first rhs() (:738-742), i.e. after limitDerivatives() replaced v
+ * and w in place, so u^n of the derivative moments is the limited state;
+ * the stage results u1, u2 (:743-764) are stored unlimited. */
 void orc_fvh2d_rk3(const orc_params* p, double* q, double dt) {
     size_t n = 12 * (size_t)p->nx * p->ny;
     double* q0 = (double*)malloc(n * sizeof(double));
     double* r = (double*)malloc(n * sizeof(double));
     double* qs = (double*)malloc(n * sizeof(double));
-    memcpy(q0, q, n * sizeof(double));
-    orc_fvh2d_rhs(p, q, r);
+    orc_fvh2d_rhs_lim(p, q, r, q0);
     orc_rk3_combine(1, q0, q0, r, dt, q, (long)n);
     memcpy(qs, q, n * sizeof(double));
     orc_fvh2d_rhs(p, q, r);

@@ -61,6 +61,7 @@ void orc_hteno(double dif, const double u[4], const double d[4], double out[4]);
 double orc_hermite_limiter(double dif, const double u[3], const double d[3]);
 void orc_dflux(double gamma, int dir, const double q[4], const double w[4], double M[16]);
 void orc_fvh2d_rhs(const orc_params* p, const double* q, double* rhs);
+void orc_fvh2d_rhs_lim(const orc_params* p, const double* q, double* rhs, double* qlim);
 void orc_fvh2d_rk3(const orc_params* p, double* q, double dt);
 int orc_fvh2d_advance(const orc_params* p, double* q, double cfl, double t0, double tfinal,
                       int max_steps, double* t_out);

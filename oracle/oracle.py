@@ -52,6 +52,7 @@ _SIG = {
     "orc_dflux": (None, [_D, _I, _DP, _DP, _DP]),
     "orc_fvh2d_rhs": (None, [ctypes.POINTER(OrcParams), _DP, _DP]),
     "orc_fvh2d_rk3": (None, [ctypes.POINTER(OrcParams), _DP, _D]),
+    "orc_fvh2d_rhs_lim": (None, [ctypes.POINTER(OrcParams), _DP, _DP, _DP]),
     "orc_fvh2d_advance": (_I, [ctypes.POINTER(OrcParams), _DP, _D, _D, _D, _I, _DP]),
     "orc_ic_vortex_hermite": (None, [ctypes.POINTER(OrcParams), _D, _D, _DP, _I, _DP]),
     "orc_teno5i": (None, [_DP, _DP]),
@@ -235,6 +236,13 @@ class Oracle:
         r = np.empty_like(q)
         self.lib.orc_fvh2d_rhs(ctypes.byref(prm), pq, r.ctypes.data_as(_DP))
         return r
+
+    def fvh2d_rhs_lim(self, prm, q):
+        q, pq = _p(q)
+        r = np.empty_like(q)
+        ql = np.empty_like(q)
+        self.lib.orc_fvh2d_rhs_lim(ctypes.byref(prm), pq, r.ctypes.data_as(_DP), ql.ctypes.data_as(_DP))
+        return r, ql
 
     def fvh2d_rk3(self, prm, q, dt):
         q = np.array(q, dtype=np.float64, copy=True, order="C")

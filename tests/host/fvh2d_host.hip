@@ -1,0 +1,77 @@
+// CPU build of the Hermite-TENO kernels' per-line / per-face / per-cell
+// bodies (htx_hermite.h): the limiter chains, face reconstructions, Gauss-point
+// fluxes and the cell update run here in loops in the order the GPU launches
+// them.  Test infrastructure, compared with the FP64 oracle.
+#include <vector>
+
+#include "../../hermite-teno_amd/csrc/htx_hermite.h"
+
+using namespace htx;
+
+extern "C" int fvhh_rhs(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc, int flux,
+                        int quirks, const double* q, double* rhs, double* qlim) {
+    FvhArgs a{};
+    size_t n = (size_t)nx * ny;
+    std::vector<double> lv(8 * n), fx(24 * (size_t)ny * (nx + 1)), fy(24 * (size_t)(ny + 1) * nx),
+        gx(12 * (size_t)ny * (nx + 1)), gy(12 * (size_t)(ny + 1) * nx);
+    a.q = q;
+    a.lv = lv.data();
+    a.lv_out = lv.data();
+    a.fx = fx.data();
+    a.fy = fy.data();
+    a.gx = gx.data();
+    a.gy = gy.data();
+    a.q0 = q;
+    a.qout = rhs;
+    a.nx = nx;
+    a.ny = ny;
+    a.pitch = nx;
+    a.fstride = (long long)n;
+    a.bc = bc;
+    a.mode = 0;
+    a.quirks = quirks;
+    a.gam = gamma;
+    a.g1 = gamma - 1.0;
+    a.dx = (xr - xl) / nx;
+    a.dy = (yt - yb) / ny;
+    a.area = a.dx * a.dy;
+    fill_quad_consts(a.qc);
+    fill_herm_consts(a.hc, a.dx);
+    int fk = flux == 0 ? 0 : (quirks ? 1 : 2);
+    for (int e = 0; e < 4; ++e) {
+        for (int j = 0; j < ny; ++j) fvh_limit_line(a, 0, j, e);
+        for (int i = 0; i < nx; ++i) fvh_limit_line(a, 1, i, e);
+    }
+    long long fxs = (long long)ny * (nx + 1), fys = (long long)(ny + 1) * nx;
+    for (int j = 0; j < ny; ++j)
+        for (int fi = 0; fi <= nx; ++fi) fvh_recon_face<false>(a, fi - 1, j, a.fx, (long long)j * (nx + 1) + fi, fxs);
+    for (int fj = 0; fj <= ny; ++fj)
+        for (int i = 0; i < nx; ++i) fvh_recon_face<true>(a, i, fj - 1, a.fy, (long long)fj * nx + i, fys);
+    double o[3][12];
+    for (int j = 0; j < ny; ++j)
+        for (int fi = 0; fi <= nx; ++fi) {
+            for (int m = 0; m < 3; ++m) {
+                if (fk == 0) fvh_flux_gp<false, 0>(a, fi - 1, j, m, a.fx, fxs, o[m]);
+                else if (fk == 1) fvh_flux_gp<false, 1>(a, fi - 1, j, m, a.fx, fxs, o[m]);
+                else fvh_flux_gp<false, 2>(a, fi - 1, j, m, a.fx, fxs, o[m]);
+            }
+            for (int c = 0; c < 12; ++c) a.gx[c * fxs + (long long)j * (nx + 1) + fi] = (o[0][c] + o[1][c]) + o[2][c];
+        }
+    for (int fj = 0; fj <= ny; ++fj)
+        for (int i = 0; i < nx; ++i) {
+            for (int m = 0; m < 3; ++m) {
+                if (fk == 0) fvh_flux_gp<true, 0>(a, i, fj - 1, m, a.fy, fys, o[m]);
+                else if (fk == 1) fvh_flux_gp<true, 1>(a, i, fj - 1, m, a.fy, fys, o[m]);
+                else fvh_flux_gp<true, 2>(a, i, fj - 1, m, a.fy, fys, o[m]);
+            }
+            for (int c = 0; c < 12; ++c) a.gy[c * fys + (long long)fj * nx + i] = (o[0][c] + o[1][c]) + o[2][c];
+        }
+    double smx = 0, smy = 0;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) fvh_update_cell(a, i, j, 0.0, smx, smy);
+    if (qlim) {
+        for (size_t c = 0; c < 4 * n; ++c) qlim[c] = q[c];
+        for (size_t c = 0; c < 8 * n; ++c) qlim[4 * n + c] = lv[c];
+    }
+    return 0;
+}
