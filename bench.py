@@ -67,6 +67,26 @@ def cpu_baseline(n, seconds=12.0):
                       f"FP64 CPU oracle (gcc -O3 -fopenmp), {el:.1f} s"}
 
 
+def time_variant(n, steps, warmup, **kw):
+    """Cell-updates/s of one solver variant on the same workload (for the
+    report's `variants` block)."""
+    import htx
+    from htx.ic import vortex_workload
+
+    w = vortex_workload(n)
+    s = htx.System2D(w["xl"], w["xr"], n, w["yb"], w["yt"], n, w["cfl"], w["tmax"], 0.0, w["gamma"], bc="periodic",
+                     **kw)
+    s.isentropicVortex(w["centre"], w["modes"])
+    s.steps_async(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.steps_async(steps)
+    s.sync()
+    el = time.perf_counter() - t0
+    s.close()
+    return n * n * steps / el
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +96,7 @@ def main():
     ap.add_argument("--flux", default="hllc")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-variants", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -151,7 +172,20 @@ def main():
                      "note": "binding roof: FP64 VALU (about 71 flop per algorithmic byte)"},
         },
         "cpu_baseline": None,
+        "variants": None,
     }
+    s.close()
+    if not args.no_variants:
+        out["variants"] = {}
+        for name, kw in [("teno5_hllc_fp32_mixed", dict(flux=args.flux, precision="fp32_mixed")),
+                         ("hteno_hllc_fp64", dict(flux=args.flux, scheme="hteno")),
+                         ("teno5_rusanov_fp64", dict(flux="rusanov"))]:
+            try:
+                log(f"[gpu] variant {name}")
+                out["variants"][name] = {"value": time_variant(n, max(3, args.steps // 3), 1, **kw),
+                                         "unit": "cell-updates/s"}
+            except Exception as e:
+                log(f"[gpu] variant {name} failed: {e}")
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)

@@ -86,7 +86,9 @@ int check_params(const htx_params* p) {
     if (p->bc != HTX_BC_ZERO_GRAD && p->bc != HTX_BC_PERIODIC) return fail(HTX_EINVAL, "unknown bc");
     if (p->precision != HTX_PREC_FP64 && p->precision != HTX_PREC_FP32_MIXED && p->precision != HTX_PREC_LEARNED)
         return fail(HTX_EINVAL, "unknown precision");
-    if (p->precision != HTX_PREC_FP64) return fail(HTX_ENOTSUP, "only FP64 precision is built in this library");
+    if (p->precision == HTX_PREC_LEARNED) return fail(HTX_ENOTSUP, "the learned precision mode is not built");
+    if (p->precision == HTX_PREC_FP32_MIXED && p->scheme != HTX_SCHEME_TENO5)
+        return fail(HTX_ENOTSUP, "FP32-mixed precision is built for the TENO5 scheme only");
     return HTX_OK;
 }
 
@@ -103,6 +105,7 @@ Fv2dArgs stage_args(htx_solver* h, const double* qin, const double* q0, double* 
     a.bc = h->p.bc;
     a.mode = mode;
     a.quirks = h->p.ref_quirks ? 1 : 0;
+    a.mixed = h->p.precision == HTX_PREC_FP32_MIXED ? 1 : 0;
     a.gam = h->p.gamma;
     a.g1 = h->p.gamma - 1.0;
     a.dx = h->dx;

@@ -24,7 +24,7 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-template <int TX, int TY, int FLUX>
+template <int TX, int TY, int FLUX, bool MIXED>
 __global__ void __launch_bounds__(256, 2) fv2d_stage_kernel(Fv2dArgs a) {
     using T = Fv2dTile<TX, TY>;
     __shared__ T s;
@@ -41,15 +41,15 @@ __global__ void __launch_bounds__(256, 2) fv2d_stage_kernel(Fv2dArgs a) {
 
     ph_load<TX, TY>(a, s, i0, j0, tid);
     __syncthreads();
-    ph_recon<TX, TY, false>(a, s, tid);
+    ph_recon<TX, TY, false, MIXED>(a, s, tid);
     __syncthreads();
-    ph_flux<TX, TY, false, FLUX>(a, s, tid);
+    ph_flux<TX, TY, false, FLUX, MIXED>(a, s, tid);
     __syncthreads();
     ph_xdiv<TX, TY>(a, s, tid, dxp);
     __syncthreads();
-    ph_recon<TX, TY, true>(a, s, tid);
+    ph_recon<TX, TY, true, MIXED>(a, s, tid);
     __syncthreads();
-    ph_flux<TX, TY, true, FLUX>(a, s, tid);
+    ph_flux<TX, TY, true, FLUX, MIXED>(a, s, tid);
     __syncthreads();
     double smx = 0.0, smy = 0.0;
     ph_epilogue<TX, TY>(a, s, i0, j0, tid, dxp, dt, smx, smy);
@@ -182,25 +182,26 @@ __global__ void __launch_bounds__(256) k_ic_vortex(VortexArgs a) {
     for (int e = 0; e < 4; ++e) a.q[off + e * a.fstride] = acc[e];
 }
 
-template <int TX, int TY>
+template <int TX, int TY, bool MIXED>
 static hipError_t launch_tile(const Fv2dArgs& a0, int flux, hipStream_t s) {
     Fv2dArgs a = a0;
     a.tiles_x = (a.nx + TX - 1) / TX;
     a.tiles_y = (a.ny + TY - 1) / TY;
     dim3 grid(a.tiles_x * a.tiles_y), block(256);
     if (flux == FLUX_RUSANOV_CHAR)
-        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_RUSANOV_CHAR>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_RUSANOV_CHAR, MIXED>), grid, block, 0, s, a);
     else if (flux == FLUX_HLLC)
-        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC, MIXED>), grid, block, 0, s, a);
     else
-        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC_TORO>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC_TORO, MIXED>), grid, block, 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y, hipStream_t s) {
     (void)tile_x;
     (void)tile_y;
-    return launch_tile<16, 16>(a, flux, s);
+    if (a.mixed) return launch_tile<16, 16, true>(a, flux, s);
+    return launch_tile<16, 16, false>(a, flux, s);
 }
 
 hipError_t launch_dt(Ctl* c, double dx, double dy, hipStream_t s) {

@@ -63,7 +63,7 @@ HTX_HD Prim prim_at(const Fv2dTile<TX, TY>& s, int id) {
 
 // characteristic TENO5 on faces (FV_System2D.cpp:372-400).  YDIR selects the
 // face family; the state is read in (rho, normal, tangential, E) order.
-template <int TX, int TY, bool YDIR>
+template <int TX, int TY, bool YDIR, bool MIXED = false>
 HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
     using T = Fv2dTile<TX, TY>;
     constexpr int NW = YDIR ? T::NYF : T::NXF;
@@ -82,7 +82,10 @@ HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
             double um, up;
-            teno5(c[f], um, up);
+            if (MIXED)
+                teno5_mixed(c[f], um, up);
+            else
+                teno5(c[f], um, up);
             s.sF[2 * f][w] = um;
             s.sF[2 * f + 1][w] = up;
         }
@@ -92,7 +95,7 @@ HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
 // transverse 3-point quadrature, back-projection with the face's own R,
 // Riemann flux at the Gauss points weighted by (5/18, 4/9, 5/18)
 // (FV_System2D.cpp:402-497); the flux is stored in (rho, x, y, E) order.
-template <int TX, int TY, bool YDIR, int FLUX>
+template <int TX, int TY, bool YDIR, int FLUX, bool MIXED = false>
 HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
     using T = Fv2dTile<TX, TY>;
     constexpr int NW = YDIR ? T::NYQ : T::NXQ;
@@ -110,8 +113,17 @@ HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
         for (int f = 0; f < 4; ++f) {
             const double* L = s.sF[2 * f];
             const double* R = s.sF[2 * f + 1];
-            weno3q(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1], gl[f][2]);
-            weno3q(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1], gr[f][2]);
+            if (MIXED) {
+                weno3q_mixed(a.qc, a.qc.lw32, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0],
+                             gl[f][1], gl[f][2]);
+                weno3q_mixed(a.qc, a.qc.lw32, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0],
+                             gr[f][1], gr[f][2]);
+            } else {
+                weno3q(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
+                       gl[f][2]);
+                weno3q(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
+                       gr[f][2]);
+            }
         }
         Eig E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
         double acc[3][4];

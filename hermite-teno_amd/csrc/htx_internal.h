@@ -29,6 +29,7 @@ struct Fv2dArgs {
     int bc;             // 0 zero-gradient, 1 periodic
     int mode;           // 0 residual only, 1..3 SSP-RK3 stage
     int quirks;         // reference maxSpeed quirk in the CFL epilogue
+    int mixed;          // 1: cut-off decisions and nonlinear weights in FP32
     int tiles_x, tiles_y;
     double gam, g1, dx, dy, area;
     QuadConsts qc;

@@ -39,6 +39,7 @@ struct QuadConsts {
     double gm[3][3];   // crjgm3L(r, k)
     double linwp[3], linwm[3], linw0p[3], linw0m[3];
     double sumw0p, sumw0m;
+    float lw32[12];  // the four linear weight sets in FP32 (+xi, centre+, centre-, -xi)
 };
 
 inline void fill_quad_consts(QuadConsts& C) {
@@ -82,6 +83,12 @@ inline void fill_quad_consts(QuadConsts& C) {
     C.linwm[0] = (2882.0 * sq15 + 1179.0) / den;
     C.linwm[1] = (9672.0 * sq15 - 6448.0) / den;
     C.linwm[2] = (3166.0 * sq15 - 5211.0) / den;
+    for (int i = 0; i < 3; ++i) {
+        C.lw32[i] = (float)C.linwp[i];
+        C.lw32[3 + i] = (float)C.linw0p[i];
+        C.lw32[6 + i] = (float)C.linw0m[i];
+        C.lw32[9 + i] = (float)C.linwm[i];
+    }
 }
 
 struct Prim {  // per-cell quantities used by the Roe average
@@ -131,6 +138,51 @@ HTX_HD double teno5_side(double v0, double v1, double v2, double v3, double v4,
     return out != out ? 1e-15 : out;  // catchnan
 }
 
+// Mixed-precision TENO5 side: the same cut-off rule evaluated in FP32 on
+// differences formed in FP64; the reconstructed value is the FP64 candidate
+// combination with the selected linear weights, exactly as in teno5_side.
+// beta5 uses its translation-invariant form in first differences
+// d_k = v_k - v_{k-1}: beta5 = d^T M d with the exact rational M of the
+// reference polynomial (Schemes_Lagrange.cpp:1030-1032), which avoids the
+// 1e5-sized cancellation of the expanded form.  The 1e-40 guard becomes
+// 1e-30 (below FP32's normal range it would flush); both only act on
+// exactly flat stencils, where every candidate coincides.
+HTX_HD double teno5_side_mixed(double v0, double v1, double v2, double v3, double v4, double p0, double p1,
+                               double p2, double d0, double d1, double d2) {
+    const float c1 = 13.0f / 12.0f;
+    float c_0 = (float)(v2 - 2.0 * v3 + v4), b_0 = (float)((3.0 * v2 - 4.0 * v3 + v4) * 0.5);
+    float c_1 = (float)(v1 - 2.0 * v2 + v3), b_1 = (float)((v1 - v3) * 0.5);
+    float c_2 = (float)(v0 - 2.0 * v1 + v2), b_2 = (float)((v0 - 4.0 * v1 + 3.0 * v2) * 0.5);
+    float s0 = b_0 * b_0 + c1 * c_0 * c_0;
+    float s1 = b_1 * b_1 + c1 * c_1 * c_1;
+    float s2 = b_2 * b_2 + c1 * c_2 * c_2;
+    float e1 = (float)(v1 - v0), e2 = (float)(v2 - v1), e3 = (float)(v3 - v2), e4 = (float)(v4 - v3);
+    const float m11 = 1727.0f / 1260.0f, m22 = 2029.0f / 168.0f, m12 = -2479.0f / 672.0f;
+    const float m13 = 5123.0f / 1680.0f, m14 = -8209.0f / 10080.0f, m23 = -12157.0f / 1120.0f;
+    float t1 = m11 * e1 + m12 * e2 + m13 * e3 + m14 * e4;
+    float t2 = m12 * e1 + m22 * e2 + m23 * e3 + m13 * e4;
+    float t3 = m13 * e1 + m23 * e2 + m22 * e3 + m12 * e4;
+    float t4 = m14 * e1 + m13 * e2 + m12 * e3 + m11 * e4;
+    float b5 = e1 * t1 + e2 * t2 + e3 * t3 + e4 * t4;
+    float tau = fabsf(b5 - (s0 + s2 + 4.0f * s1) * (1.0f / 6.0f));
+    float g0 = 1.0f + tau / (s0 + 1e-30f);
+    float g1 = 1.0f + tau / (s1 + 1e-30f);
+    float g2 = 1.0f + tau / (s2 + 1e-30f);
+    g0 = g0 * g0 * g0; g0 = g0 * g0;
+    g1 = g1 * g1 * g1; g1 = g1 * g1;
+    g2 = g2 * g2 * g2; g2 = g2 * g2;
+    float gs = (g0 + g1) + g2;
+    // g/gs < C_T  <=>  g < C_T gs, including the overflow cases: an infinite
+    // g against an infinite sum is kept (inf/inf = NaN compares false), a
+    // finite one is cut, and NaN keeps every stencil
+    float lim = 5e-4f * gs;
+    bool k0 = !(g0 < lim), k1 = !(g1 < lim), k2 = !(g2 < lim);
+    double w0 = k0 ? d0 : 0.0, w1 = k1 ? d1 : 0.0, w2 = k2 ? d2 : 0.0;
+    double ws = (w0 + w1) + w2;
+    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * (1.0 / ws);
+    return out != out ? 1e-15 : out;
+}
+
 // Schemes::Lagrange::teno5 (Schemes_Lagrange.cpp:481-511): u[0..5] =
 // u_{i-2..i+3} -> (u^-_{i+1/2}, u^+_{i+1/2}).
 HTX_HD void teno5(const double u[6], double& um, double& up, unsigned* cut = nullptr) {
@@ -147,6 +199,20 @@ HTX_HD void teno5(const double u[6], double& um, double& up, unsigned* cut = nul
     double q2 = (c * u[1] + b * u[2]) + a * u[3];
     up = teno5_side(u[1], u[2], u[3], u[4], u[5], q0, q1, q2, 0.1, 0.6, 0.3, cut ? &cp : nullptr);
     if (cut) *cut = cm | (cp << 3);
+}
+
+// TENO5 with the FP32 cut-off rule of teno5_side_mixed.
+HTX_HD void teno5_mixed(const double u[6], double& um, double& up) {
+    const double a = 1.0 / 3.0, b = 5.0 / 6.0, c = -1.0 / 6.0, d = 11.0 / 6.0, e = -7.0 / 6.0;
+    // crj columns: col0 = (d, e, a), col1 = (a, b, c), col2 = (c, b, a), col3 = (a, e, d)
+    double p0 = (a * u[2] + b * u[3]) + c * u[4];
+    double p1 = (c * u[1] + b * u[2]) + a * u[3];
+    double p2 = (a * u[0] + e * u[1]) + d * u[2];
+    um = teno5_side_mixed(u[0], u[1], u[2], u[3], u[4], p0, p1, p2, 0.3, 0.6, 0.1);
+    double q0 = (d * u[3] + e * u[4]) + a * u[5];
+    double q1 = (a * u[2] + b * u[3]) + c * u[4];
+    double q2 = (c * u[1] + b * u[2]) + a * u[3];
+    up = teno5_side_mixed(u[1], u[2], u[3], u[4], u[5], q0, q1, q2, 0.1, 0.6, 0.3);
 }
 
 // Schemes::Lagrange::weno5JS3PointsQuadrature + smoothnessMeasure3L
@@ -224,6 +290,48 @@ HTX_HD void weno3q(const QuadConsts& C, double u0, double u1, double u2, double 
     o2 = ((p0 * w0 + p1 * w1) + p2 * w2) / ws;
 #endif
     o0 = o0 != o0 ? 1e-15 : o0;  // catchnan
+    o1 = o1 != o1 ? 1e-15 : o1;
+    o2 = o2 != o2 ? 1e-15 : o2;
+}
+
+// Mixed-precision weno3q: smoothness and nonlinear weights in FP32 from
+// differences formed in FP64, candidates and their combination in FP64.  The
+// last weight of each group is 1 minus the others (in FP64), so the
+// combination stays exact for constant data.
+HTX_HD void weno3q_mixed(const QuadConsts& C, const float* lw, double u0, double u1, double u2, double u3,
+                         double u4, double& o0, double& o1, double& o2) {
+    const float c1 = 13.0f / 12.0f;
+    float bg0 = (float)(u0 - 4.0 * u1 + 3.0 * u2), cg0 = (float)(u0 - 2.0 * u1 + u2);
+    float bg1 = (float)(u1 - u3), cg1 = (float)(u1 - 2.0 * u2 + u3);
+    float bg2 = (float)(3.0 * u2 - 4.0 * u3 + u4), cg2 = (float)(u2 - 2.0 * u3 + u4);
+    float e0 = 1e-6f + (0.25f * (bg0 * bg0) + c1 * (cg0 * cg0));
+    float e1 = 1e-6f + (0.25f * (bg1 * bg1) + c1 * (cg1 * cg1));
+    float e2 = 1e-6f + (0.25f * (bg2 * bg2) + c1 * (cg2 * cg2));
+    float r0 = 1.0f / (e0 * e0), r1 = 1.0f / (e1 * e1), r2 = 1.0f / (e2 * e2);
+    // lw: 4 sets of 3 FP32 linear weights (+xi, centre+, centre-, -xi)
+    double W[4][2];
+    for (int k = 0; k < 4; ++k) {
+        float w0 = lw[3 * k] * r0, w1 = lw[3 * k + 1] * r1, w2 = lw[3 * k + 2] * r2;
+        float is = 1.0f / ((w0 + w1) + w2);
+        W[k][0] = (double)(w0 * is);
+        W[k][1] = (double)(w1 * is);
+    }
+    double p0, p1, p2;
+    p0 = (C.gp[0][0] * u0 + C.gp[1][0] * u1) + C.gp[2][0] * u2;
+    p1 = (C.gp[0][1] * u1 + C.gp[1][1] * u2) + C.gp[2][1] * u3;
+    p2 = (C.gp[0][2] * u2 + C.gp[1][2] * u3) + C.gp[2][2] * u4;
+    o0 = p2 + W[0][0] * (p0 - p2) + W[0][1] * (p1 - p2);
+    const double r24 = 1.0 / 24.0;
+    p0 = (-u0 + 2.0 * u1 + 23.0 * u2) * r24;
+    p1 = (-u1 + 26.0 * u2 - u3) * r24;
+    p2 = (23.0 * u2 + 2.0 * u3 - u4) * r24;
+    o1 = C.sumw0p * (p2 + W[1][0] * (p0 - p2) + W[1][1] * (p1 - p2)) -
+         C.sumw0m * (p2 + W[2][0] * (p0 - p2) + W[2][1] * (p1 - p2));
+    p0 = (C.gm[0][0] * u0 + C.gm[1][0] * u1) + C.gm[2][0] * u2;
+    p1 = (C.gm[0][1] * u1 + C.gm[1][1] * u2) + C.gm[2][1] * u3;
+    p2 = (C.gm[0][2] * u2 + C.gm[1][2] * u3) + C.gm[2][2] * u4;
+    o2 = p2 + W[3][0] * (p0 - p2) + W[3][1] * (p1 - p2);
+    o0 = o0 != o0 ? 1e-15 : o0;
     o1 = o1 != o1 ? 1e-15 : o1;
     o2 = o2 != o2 ? 1e-15 : o2;
 }

@@ -10,7 +10,7 @@
 
 using namespace htx;
 
-template <int TX, int TY>
+template <int TX, int TY, bool MIXED = false>
 static void run(Fv2dArgs a, int flux) {
     using T = Fv2dTile<TX, TY>;
     T* s = new T();
@@ -20,18 +20,18 @@ static void run(Fv2dArgs a, int flux) {
     for (int tile = 0; tile < a.tiles_x * a.tiles_y; ++tile) {
         int i0 = (tile % a.tiles_x) * TX, j0 = (tile / a.tiles_x) * TY;
         for (int t = 0; t < T::NT; ++t) ph_load<TX, TY>(a, *s, i0, j0, t);
-        for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, false>(a, *s, t);
+        for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, false, MIXED>(a, *s, t);
         for (int t = 0; t < T::NT; ++t) {
-            if (flux == 0) ph_flux<TX, TY, false, 0>(a, *s, t);
-            else if (flux == 1) ph_flux<TX, TY, false, 1>(a, *s, t);
-            else ph_flux<TX, TY, false, 2>(a, *s, t);
+            if (flux == 0) ph_flux<TX, TY, false, 0, MIXED>(a, *s, t);
+            else if (flux == 1) ph_flux<TX, TY, false, 1, MIXED>(a, *s, t);
+            else ph_flux<TX, TY, false, 2, MIXED>(a, *s, t);
         }
         for (int t = 0; t < T::NT; ++t) ph_xdiv<TX, TY>(a, *s, t, (double(*)[4]) & dxp[(size_t)t * T::KC * 4]);
-        for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, true>(a, *s, t);
+        for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, true, MIXED>(a, *s, t);
         for (int t = 0; t < T::NT; ++t) {
-            if (flux == 0) ph_flux<TX, TY, true, 0>(a, *s, t);
-            else if (flux == 1) ph_flux<TX, TY, true, 1>(a, *s, t);
-            else ph_flux<TX, TY, true, 2>(a, *s, t);
+            if (flux == 0) ph_flux<TX, TY, true, 0, MIXED>(a, *s, t);
+            else if (flux == 1) ph_flux<TX, TY, true, 1, MIXED>(a, *s, t);
+            else ph_flux<TX, TY, true, 2, MIXED>(a, *s, t);
         }
         for (int t = 0; t < T::NT; ++t) {
             double smx = 0, smy = 0;
@@ -41,8 +41,8 @@ static void run(Fv2dArgs a, int flux) {
     delete s;
 }
 
-extern "C" int fv2dh_rhs(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc, int flux,
-                         int quirks, const double* q, double* rhs) {
+extern "C" int fv2dh_rhs_prec(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc,
+                              int flux, int quirks, int mixed, const double* q, double* rhs) {
     Fv2dArgs a{};
     a.qin = q;
     a.q0 = q;
@@ -61,6 +61,14 @@ extern "C" int fv2dh_rhs(int nx, int ny, double xl, double xr, double yb, double
     a.area = a.dx * a.dy;
     fill_quad_consts(a.qc);
     int fk = flux == 0 ? 0 : (quirks ? 1 : 2);
-    run<16, 16>(a, fk);
+    if (mixed)
+        run<16, 16, true>(a, fk);
+    else
+        run<16, 16, false>(a, fk);
     return 0;
+}
+
+extern "C" int fv2dh_rhs(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc, int flux,
+                         int quirks, const double* q, double* rhs) {
+    return fv2dh_rhs_prec(nx, ny, xl, xr, yb, yt, gamma, bc, flux, quirks, 0, q, rhs);
 }

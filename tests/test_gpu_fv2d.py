@@ -194,3 +194,29 @@ def test_transpose_symmetry_full_size():
     rt = s.rhs()
     back = np.stack([rt[0].T, rt[2].T, rt[1].T, rt[3].T])
     assert rel_l1(back, r) < 1e-12
+
+
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+def test_fp32_mixed_close_to_fp64(orc, flux):
+    """FP32-mixed mode on the GPU: residual within 1e-8 relative L1 of the FP64
+    oracle, and ten steps of a shock problem within 1e-9."""
+    import htx
+    from htx.ic import vortex_modes
+
+    n = 64
+    s = htx.System2D(0.0, 10.0, n, 0.0, 10.0, n, 0.5, 10.0, 0.0, GAMMA, flux=flux, bc="periodic",
+                     precision="fp32_mixed")
+    prm = params(n, n, 0.0, 10.0, 0.0, 10.0, GAMMA, "periodic", flux, True)
+    q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
+    s.set_state(q)
+    assert rel_l1(s.rhs(), orc.fv2d_rhs(prm, q)) < 1e-8
+    s2 = htx.System2D(0.0, 1.0, 48, 0.0, 1.0, 48, 0.5, 0.02, 0.0, GAMMA, flux=flux, bc="zero_grad",
+                      precision="fp32_mixed")
+    q2 = quadrant_state(48, 48, seed=4)
+    s2.set_state(q2)
+    s2.timeStepping(0.02)
+    prm2 = params(48, 48, 0.0, 1.0, 0.0, 1.0, GAMMA, "zero_grad", flux, True)
+    qc, t, _ = orc.fv2d_advance(prm2, q2, 0.5, 0.0, 0.02)
+    err = rel_l1(s2.get_state(), qc)
+    print(f"fp32-mixed end-time rel L1 {err:.3e}", flush=True)
+    assert err < 1e-9

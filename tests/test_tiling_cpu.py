@@ -135,3 +135,27 @@ def test_hermite_kernels_cpu_build(orc, harness_hermite, nx, ny, bc, flux):
     rc, qlc = orc.fvh2d_rhs_lim(prm, q)
     np.testing.assert_array_equal(ql, qlc)
     assert np.abs(r - rc).sum() / np.abs(rc).sum() < 1e-14
+
+
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+def test_fp32_mixed_residual_close_to_fp64(orc, harness_fast, flux):
+    """FP32-mixed mode (cut-off decisions and nonlinear weights in FP32,
+    values, projections, fluxes in FP64): the residual stays within 1e-8
+    relative L1 of the FP64 oracle on smooth and on shock data."""
+    from htx.ic import vortex_modes
+
+    harness_fast.fv2dh_rhs_prec.argtypes = [ctypes.c_int, ctypes.c_int, D, D, D, D, D, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, DP, DP]
+    for nx, ny, dom, bc, q in [
+        (64, 64, (0.0, 10.0, 0.0, 10.0), "periodic", None),
+        (96, 80, (0.0, 1.0, 0.0, 1.0), "zero_grad", quadrant(96, 80, 9)),
+    ]:
+        prm = params(nx, ny, *dom, 1.4, bc, flux, True)
+        if q is None:
+            q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
+        q = np.ascontiguousarray(q)
+        r = np.empty_like(q)
+        harness_fast.fv2dh_rhs_prec(nx, ny, *dom, 1.4, {"zero_grad": 0, "periodic": 1}[bc],
+                                    {"rusanov": 0, "hllc": 1}[flux], 1, 1, q.ctypes.data_as(DP), r.ctypes.data_as(DP))
+        rc = orc.fv2d_rhs(prm, q)
+        assert np.abs(r - rc).sum() / np.abs(rc).sum() < 1e-8
