@@ -77,26 +77,36 @@ def test_rhs_parity_riemann(orc, flux, quirks):
 
 @pytest.mark.parametrize("flux", ["hllc", "rusanov"])
 def test_step_parity_10_steps(orc, flux):
-    """Ten SSP-RK3 steps, compared after every step (dt included)."""
+    """Ten SSP-RK3 steps.  Per step, from identical inputs (the GPU state is
+    reset to the oracle's before each step): relative L1 <= 1e-12 and the same
+    dt.  Free-running over the ten steps: <= 1e-9 (rounding differences reach
+    TENO cut-off decisions sitting at their threshold, so the trajectories
+    separate slowly)."""
     from htx.ic import vortex_modes
 
     n = 64
     s, prm = make(n, n, "periodic", flux)
+    free, _ = make(n, n, "periodic", flux)
     q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
-    s.set_state(q)
+    free.set_state(q)
     qc = q.copy()
     t = 0.0
     for k in range(10):
+        s.set_state(qc)
         dt_cpu = orc.fv2d_dt(prm, qc, 0.5, t, 10.0)
         dt_gpu = s.step()
         assert dt_gpu == pytest.approx(dt_cpu, rel=1e-13)
-        qc = orc.fv2d_rk3(prm, qc, dt_cpu)
-        t += dt_cpu
-        err = rel_l1(s.get_state(), qc)
-        print(f"step {k}: rel L1 {err:.3e}", flush=True)
+        qn = orc.fv2d_rk3(prm, qc, dt_cpu)
+        err = rel_l1(s.get_state(), qn)
+        free.step()
+        err_free = rel_l1(free.get_state(), qn)
+        print(f"step {k}: per-step rel L1 {err:.3e}, free-running {err_free:.3e}", flush=True)
         assert err < TOL_STEP
-    assert s.time == pytest.approx(t, rel=1e-13)
-    assert s.nsteps == 10
+        assert err_free < TOL_END
+        qc = qn
+        t += dt_cpu
+    assert free.time == pytest.approx(t, rel=1e-13)
+    assert free.nsteps == 10
 
 
 def test_end_time_parity_riemann(orc):
