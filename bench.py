@@ -177,7 +177,8 @@ def main():
     s.close()
     if not args.no_variants:
         out["variants"] = {}
-        for name, kw in [("teno5_hllc_fp32_mixed", dict(flux=args.flux, precision="fp32_mixed")),
+        for name, kw in [("teno5_hllc_fp64_fast", dict(flux=args.flux, precision="fp64_fast")),
+                         ("teno5_hllc_fp32_mixed", dict(flux=args.flux, precision="fp32_mixed")),
                          ("hteno_hllc_fp64", dict(flux=args.flux, scheme="hteno")),
                          ("teno5_rusanov_fp64", dict(flux="rusanov"))]:
             try:

@@ -14,6 +14,10 @@ hipError_t launch_fvh2d_stage(const FvhArgs& a, int flux, hipStream_t s);
 hipError_t launch_ic_vortex_hermite(const VortexArgs& a, hipStream_t s);
 }  // namespace htx
 
+// strict IEEE build of the same kernels (htx_strict.hip)
+hipError_t htx_strict_launch_fv2d_stage(const void* a, int flux, int tile_x, int tile_y, hipStream_t s);
+hipError_t htx_strict_launch_fvh2d_stage(const void* a, int flux, hipStream_t s);
+
 using namespace htx;
 
 namespace {
@@ -84,7 +88,8 @@ int check_params(const htx_params* p) {
     if (p->scheme != HTX_SCHEME_TENO5 && p->scheme != HTX_SCHEME_HTENO) return fail(HTX_EINVAL, "unknown scheme");
     if (p->flux != HTX_FLUX_RUSANOV_CHAR && p->flux != HTX_FLUX_HLLC) return fail(HTX_EINVAL, "unknown flux");
     if (p->bc != HTX_BC_ZERO_GRAD && p->bc != HTX_BC_PERIODIC) return fail(HTX_EINVAL, "unknown bc");
-    if (p->precision != HTX_PREC_FP64 && p->precision != HTX_PREC_FP32_MIXED && p->precision != HTX_PREC_LEARNED)
+    if (p->precision != HTX_PREC_FP64 && p->precision != HTX_PREC_FP32_MIXED && p->precision != HTX_PREC_LEARNED &&
+        p->precision != HTX_PREC_FP64_FAST)
         return fail(HTX_EINVAL, "unknown precision");
     if (p->precision == HTX_PREC_LEARNED) return fail(HTX_ENOTSUP, "the learned precision mode is not built");
     if (p->precision == HTX_PREC_FP32_MIXED && p->scheme != HTX_SCHEME_TENO5)
@@ -163,11 +168,20 @@ int launch_stage(htx_solver* h, const double* qin, const double* q0, double* qou
         }
         HIP_TRY(hipEventRecord(e0, h->stream));
     }
-    if (h->p.scheme == HTX_SCHEME_HTENO)
-        HIP_TRY(launch_fvh2d_stage(hermite_args(h, qin, q0, qout, mode), flux_kind(h), h->stream));
-    else
-        HIP_TRY(launch_fv2d_stage(stage_args(h, qin, q0, qout, mode), flux_kind(h), h->p.tile_x, h->p.tile_y,
-                                  h->stream));
+    const bool strict = h->p.precision == HTX_PREC_FP64;
+    if (h->p.scheme == HTX_SCHEME_HTENO) {
+        FvhArgs a = hermite_args(h, qin, q0, qout, mode);
+        if (strict)
+            HIP_TRY(htx_strict_launch_fvh2d_stage(&a, flux_kind(h), h->stream));
+        else
+            HIP_TRY(launch_fvh2d_stage(a, flux_kind(h), h->stream));
+    } else {
+        Fv2dArgs a = stage_args(h, qin, q0, qout, mode);
+        if (strict)
+            HIP_TRY(htx_strict_launch_fv2d_stage(&a, flux_kind(h), h->p.tile_x, h->p.tile_y, h->stream));
+        else
+            HIP_TRY(launch_fv2d_stage(a, flux_kind(h), h->p.tile_x, h->p.tile_y, h->stream));
+    }
     if (h->timing) {
         HIP_TRY(hipEventRecord(e1, h->stream));
         h->events.emplace_back(e0, e1);

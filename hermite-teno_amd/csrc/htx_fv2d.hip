@@ -17,7 +17,7 @@
 // (stages 2-3) and the stage result write.
 #include "htx_fv2d_tile.h"
 
-namespace htx {
+namespace HTX_NS {
 
 __device__ __forceinline__ double wave_max(double v) {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
@@ -235,4 +235,4 @@ hipError_t launch_ic_vortex(const VortexArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-}  // namespace htx
+}  // namespace HTX_NS

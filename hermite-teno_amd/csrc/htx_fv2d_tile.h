@@ -11,7 +11,7 @@
 #pragma once
 #include "htx_internal.h"
 
-namespace htx {
+namespace HTX_NS {
 
 template <int TX, int TY>
 struct Fv2dTile {
@@ -222,4 +222,4 @@ HTX_HD int xcd_remap(int b, int nb) {
     return start + k;
 }
 
-}  // namespace htx
+}  // namespace HTX_NS

@@ -14,7 +14,7 @@
 // build used by the tests.
 #include "htx_hermite.h"
 
-namespace htx {
+namespace HTX_NS {
 
 __device__ __forceinline__ double wave_max_fvh(double v) {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
@@ -199,4 +199,4 @@ hipError_t launch_ic_vortex_hermite(const VortexArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-}  // namespace htx
+}  // namespace HTX_NS

@@ -6,7 +6,7 @@
 #pragma once
 #include "htx_internal.h"
 
-namespace htx {
+namespace HTX_NS {
 
 // Coefficients of Hermite::setGridSize (Schemes_Hermite.cpp:31-66) for the
 // grid size dif, filled on the host with the reference's expressions.
@@ -275,9 +275,9 @@ HTX_HD void fvh_recon_face(const FvhArgs& a, int i, int j, double* F, long long 
     }
 }
 
-}  // namespace htx
+}  // namespace HTX_NS
 
-namespace htx {
+namespace HTX_NS {
 
 template <bool YDIR>
 HTX_HD void rot_to_nat(const double r[4], double& q0, double& qx, double& qy, double& q3) {
@@ -416,4 +416,4 @@ HTX_HD void fvh_update_cell(const FvhArgs& a, int i, int j, double dt, double& s
     }
 }
 
-}  // namespace htx
+}  // namespace HTX_NS

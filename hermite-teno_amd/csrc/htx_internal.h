@@ -4,7 +4,7 @@
 
 #include "htx_numerics.h"
 
-namespace htx {
+namespace HTX_NS {
 
 // Device-resident time-stepping control block.  The host never needs to read
 // it between steps: dt comes from the max wave speed that the previous step's
@@ -65,4 +65,4 @@ hipError_t launch_max_speed(const double* q, int nx, int ny, int pitch, long lon
                             int quirks, Ctl* c, hipStream_t s);
 hipError_t launch_ic_vortex(const VortexArgs& a, hipStream_t s);
 
-}  // namespace htx
+}  // namespace HTX_NS

@@ -9,6 +9,14 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// The kernels are compiled twice into libhtx.so: once as the fast FP64 /
+// mixed-precision build (namespace htx) and once as the strict IEEE build
+// (htx_strict.hip: no contraction, division by division), each in its own
+// namespace so the two builds of every inline function never meet.
+#ifndef HTX_NS
+#define HTX_NS htx
+#endif
+
 // Host+device so that the same numerics can be exercised by CPU builds.
 #define HTX_HD __host__ __device__ __forceinline__
 
@@ -29,7 +37,7 @@
 #define HTX_EXACT_DIVISION 0
 #endif
 
-namespace htx {
+namespace HTX_NS {
 
 // Runtime-computed coefficients of Schemes::Lagrange that involve sqrt(15)
 // (Schemes_Lagrange.cpp:64-107).  Filled on the host with the reference's own
@@ -622,4 +630,4 @@ HTX_HD double max_speed(double gam, double wn, double p, double rho) {
     return s > 0.0 ? s : 0.0;
 }
 
-}  // namespace htx
+}  // namespace HTX_NS

@@ -1,0 +1,21 @@
+// htx_strict.hip -- the FP64 drop-in build of the stage kernels: the same
+// sources as the fast build (htx_fv2d.hip, htx_fvh2d.hip), compiled with FMA
+// contraction off (-ffp-contract=off on this file) and with every division
+// of the reference kept as a division.  Every operation is then one IEEE
+// operation in the reference's order, so the residual and each RK stage are
+// bit-identical to an IEEE evaluation of the reference formulas, and TENO's
+// cut-off decisions never diverge from the CPU solver's.
+#define HTX_NS htx_strict
+#define HTX_EXACT_DIVISION 1
+#include "htx_fv2d.hip"
+#include "htx_fvh2d.hip"
+
+// entry points for htx_api.hip, which only sees the fast build's types (the
+// argument structs have the same layout in both builds)
+hipError_t htx_strict_launch_fv2d_stage(const void* a, int flux, int tile_x, int tile_y, hipStream_t s) {
+    return htx_strict::launch_fv2d_stage(*static_cast<const htx_strict::Fv2dArgs*>(a), flux, tile_x, tile_y, s);
+}
+
+hipError_t htx_strict_launch_fvh2d_stage(const void* a, int flux, hipStream_t s) {
+    return htx_strict::launch_fvh2d_stage(*static_cast<const htx_strict::FvhArgs*>(a), flux, s);
+}

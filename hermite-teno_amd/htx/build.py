@@ -22,7 +22,7 @@ LIBHTX = os.path.join(PKG_DIR, "libhtx.so")
 LIBORACLE = os.path.join(ORACLE, "liboracle.so")
 LIBORACLE_OMP = os.path.join(ORACLE, "liboracle_omp.so")
 
-HIP_SOURCES = ["htx_fv2d.hip", "htx_fvh2d.hip", "htx_api.hip"]
+HIP_SOURCES = ["htx_fv2d.hip", "htx_fvh2d.hip", "htx_strict.hip", "htx_api.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -51,11 +51,22 @@ def _run(cmd, cwd=None):
         raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
 
 
+# per-file flags: the strict build of the kernels turns contraction off
+PER_FILE_FLAGS = {"htx_strict.hip": ["-ffp-contract=off"]}
+
+
 def build_htx(extra=()):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    objdir = os.path.join(PKG_DIR, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        flags = [f for f in HIPCC_FLAGS if f != "-shared"] + PER_FILE_FLAGS.get(src, [])
+        _run([hipcc, *flags, *extra, "-c", "-o", obj, os.path.join(CSRC, src)])
+        objs.append(obj)
     tmp = LIBHTX + ".tmp"
-    _run([hipcc, *HIPCC_FLAGS, *extra, "-o", tmp, *srcs])
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIBHTX)
     return LIBHTX
 

@@ -43,7 +43,16 @@ enum htx_error {
 enum htx_scheme { HTX_SCHEME_TENO5 = 0, HTX_SCHEME_HTENO = 1 };
 enum htx_flux { HTX_FLUX_RUSANOV_CHAR = 0, HTX_FLUX_HLLC = 1 };
 enum htx_bc { HTX_BC_ZERO_GRAD = 0, HTX_BC_PERIODIC = 1 };
-enum htx_precision { HTX_PREC_FP64 = 0, HTX_PREC_FP32_MIXED = 1, HTX_PREC_LEARNED = 2 };
+/* HTX_PREC_FP64: the drop-in; every operation is one IEEE operation in the
+ *   reference's order (no FMA contraction, divisions kept), so each step is
+ *   bit-identical to an IEEE evaluation of the reference formulas.
+ * HTX_PREC_FP64_FAST: FP64 with FMA contraction and one reciprocal per common
+ *   denominator outside the TENO decision chain; residuals agree to rounding,
+ *   but rounding can move a TENO cut-off decision that sits at its threshold.
+ * HTX_PREC_FP32_MIXED: FP32 cut-off decisions and nonlinear weights, FP64
+ *   values, projections, fluxes and RK accumulation (TENO5 scheme).
+ * HTX_PREC_LEARNED: reserved for the learned troubled-face indicator. */
+enum htx_precision { HTX_PREC_FP64 = 0, HTX_PREC_FP32_MIXED = 1, HTX_PREC_LEARNED = 2, HTX_PREC_FP64_FAST = 3 };
 
 typedef struct htx_params {
     int struct_size;    /* sizeof(htx_params), for ABI checks */

@@ -1,9 +1,15 @@
-"""Parity of the HIP FP64 2D TENO5 path (libhtx.so) against the FP64 CPU
-oracle on identical inputs.
+"""Parity of the HIP 2D TENO5 path (libhtx.so) against the FP64 CPU oracle on
+identical inputs.
 
-Tolerances (from the project's parity statement): relative L1 <= 1e-12 per
-residual / per step, <= 1e-9 at end time; the only allowed differences come
-from FMA contraction, reduction order and reciprocal rounding.
+Precision "fp64" (the drop-in) evaluates every operation as one IEEE
+operation in the reference's order, so it is expected to match the oracle
+bit for bit; the tests hold it to the parity tolerances, relative L1
+<= 1e-12 per residual / per step and <= 1e-9 at end time, and report the
+maximum absolute difference.  "fp64_fast" (FMA contraction, reciprocals of
+common denominators) is held to the same residual tolerance; its steps may
+differ by more where rounding moves a TENO cut-off decision that sits at its
+threshold.  "fp32_mixed" is judged by accuracy: it must change the solution
+by a small fraction of the FP64 scheme's own discretization error.
 """
 import numpy as np
 import pytest
@@ -18,12 +24,13 @@ TOL_STEP = 1e-12
 TOL_END = 1e-9
 
 
-def make(nx, ny, bc="periodic", flux="hllc", quirks=True, dom=(0.0, 10.0, 0.0, 10.0), tmax=10.0, cfl=0.5):
+def make(nx, ny, bc="periodic", flux="hllc", quirks=True, dom=(0.0, 10.0, 0.0, 10.0), tmax=10.0, cfl=0.5,
+         precision="fp64"):
     import htx
 
-    print(f"[case] nx={nx} ny={ny} bc={bc} flux={flux} quirks={quirks}", flush=True)
+    print(f"[case] nx={nx} ny={ny} bc={bc} flux={flux} quirks={quirks} precision={precision}", flush=True)
     s = htx.System2D(dom[0], dom[1], nx, dom[2], dom[3], ny, cfl, tmax, 0.0, GAMMA, flux=flux, bc=bc,
-                     ref_quirks=quirks)
+                     ref_quirks=quirks, precision=precision)
     prm = params(nx, ny, dom[0], dom[1], dom[2], dom[3], GAMMA, bc, flux, quirks)
     return s, prm
 
@@ -48,40 +55,42 @@ def quadrant_state(nx, ny, seed=3):
     return q
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp64_fast"])
 @pytest.mark.parametrize("nx,ny", [(32, 32), (48, 40), (17, 23)])
 @pytest.mark.parametrize("flux", ["hllc", "rusanov"])
 @pytest.mark.parametrize("bc", ["periodic", "zero_grad"])
-def test_rhs_parity_vortex(orc, nx, ny, flux, bc):
+def test_rhs_parity_vortex(orc, nx, ny, flux, bc, precision):
     from htx.ic import vortex_modes
 
-    s, prm = make(nx, ny, bc, flux)
+    s, prm = make(nx, ny, bc, flux, precision=precision)
     q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
     s.set_state(q)
     r_gpu = s.rhs()
     r_cpu = orc.fv2d_rhs(prm, q)
+    print(f"max |diff| {np.abs(r_gpu - r_cpu).max():.3e}", flush=True)
     for e in range(4):
         assert rel_l1(r_gpu[e], r_cpu[e]) < TOL_STEP, (e, rel_l1(r_gpu[e], r_cpu[e]))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp64_fast"])
 @pytest.mark.parametrize("flux", ["hllc", "rusanov"])
 @pytest.mark.parametrize("quirks", [True, False])
-def test_rhs_parity_riemann(orc, flux, quirks):
+def test_rhs_parity_riemann(orc, flux, quirks, precision):
     nx, ny = 40, 36
-    s, prm = make(nx, ny, "zero_grad", flux, quirks, dom=(0.0, 1.0, 0.0, 1.0))
+    s, prm = make(nx, ny, "zero_grad", flux, quirks, dom=(0.0, 1.0, 0.0, 1.0), precision=precision)
     q = quadrant_state(nx, ny)
     s.set_state(q)
     r_gpu = s.rhs()
     r_cpu = orc.fv2d_rhs(prm, q)
+    print(f"max |diff| {np.abs(r_gpu - r_cpu).max():.3e}", flush=True)
     assert rel_l1(r_gpu, r_cpu) < TOL_STEP
 
 
 @pytest.mark.parametrize("flux", ["hllc", "rusanov"])
 def test_step_parity_10_steps(orc, flux):
-    """Ten SSP-RK3 steps.  Per step, from identical inputs (the GPU state is
-    reset to the oracle's before each step): relative L1 <= 1e-12 and the same
-    dt.  Free-running over the ten steps: <= 1e-9 (rounding differences reach
-    TENO cut-off decisions sitting at their threshold, so the trajectories
-    separate slowly)."""
+    """Ten SSP-RK3 steps of the FP64 drop-in.  Per step, from identical inputs
+    (the GPU state is reset to the oracle's before each step): relative L1
+    <= 1e-12 and the same dt.  Free-running over the ten steps: <= 1e-9."""
     from htx.ic import vortex_modes
 
     n = 64
@@ -97,16 +106,48 @@ def test_step_parity_10_steps(orc, flux):
         dt_gpu = s.step()
         assert dt_gpu == pytest.approx(dt_cpu, rel=1e-13)
         qn = orc.fv2d_rk3(prm, qc, dt_cpu)
-        err = rel_l1(s.get_state(), qn)
+        g = s.get_state()
+        err = rel_l1(g, qn)
         free.step()
         err_free = rel_l1(free.get_state(), qn)
-        print(f"step {k}: per-step rel L1 {err:.3e}, free-running {err_free:.3e}", flush=True)
+        print(f"step {k}: per-step rel L1 {err:.3e} (max |diff| {np.abs(g - qn).max():.3e}), "
+              f"free-running {err_free:.3e}", flush=True)
         assert err < TOL_STEP
         assert err_free < TOL_END
         qc = qn
         t += dt_cpu
     assert free.time == pytest.approx(t, rel=1e-13)
     assert free.nsteps == 10
+
+
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+def test_step_fp64_fast(orc, flux):
+    """fp64_fast: ten steps of the vortex from identical inputs stay within
+    1e-10 per step (rounding can move a TENO cut-off decision at its
+    threshold, which changes a face value by far more than rounding) and
+    within 1e-9 free-running."""
+    from htx.ic import vortex_modes
+
+    n = 64
+    s, prm = make(n, n, "periodic", flux, precision="fp64_fast")
+    free, _ = make(n, n, "periodic", flux, precision="fp64_fast")
+    q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
+    free.set_state(q)
+    qc = q.copy()
+    t = 0.0
+    for k in range(10):
+        s.set_state(qc)
+        dt_cpu = orc.fv2d_dt(prm, qc, 0.5, t, 10.0)
+        assert s.step() == pytest.approx(dt_cpu, rel=1e-13)
+        qn = orc.fv2d_rk3(prm, qc, dt_cpu)
+        err = rel_l1(s.get_state(), qn)
+        free.step()
+        err_free = rel_l1(free.get_state(), qn)
+        print(f"step {k}: per-step rel L1 {err:.3e}, free-running {err_free:.3e}", flush=True)
+        assert err < 1e-10
+        assert err_free < TOL_END
+        qc = qn
+        t += dt_cpu
 
 
 def test_end_time_parity_riemann(orc):
@@ -206,27 +247,67 @@ def test_transpose_symmetry_full_size():
     assert rel_l1(back, r) < 1e-12
 
 
-@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
-def test_fp32_mixed_close_to_fp64(orc, flux):
-    """FP32-mixed mode on the GPU: residual within 1e-8 relative L1 of the FP64
-    oracle, and ten steps of a shock problem within 1e-9."""
+def _coarsen(q):
+    """2x2 cell averages of a field-major state."""
+    return 0.25 * ((q[:, 0::2, 0::2] + q[:, 1::2, 0::2]) + (q[:, 0::2, 1::2] + q[:, 1::2, 1::2]))
+
+
+def _run(n, q0, dom, bc, flux, tend, precision):
     import htx
+
+    s = htx.System2D(dom[0], dom[1], n, dom[2], dom[3], n, 0.5, tend, 0.0, GAMMA, flux=flux, bc=bc,
+                     precision=precision)
+    if q0 is None:
+        from htx.ic import vortex_modes
+
+        s.isentropicVortex((5.0, 5.0), vortex_modes())
+    else:
+        s.set_state(q0)
+    s.timeStepping(tend)
+    out = s.get_state()
+    s.close()
+    return out
+
+
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+def test_fp32_mixed_residual(orc, flux):
+    """FP32-mixed residual of the smooth vortex: within 1e-8 relative L1 of
+    the FP64 oracle (the FP32 weights perturb face values at FP32 rounding
+    level only where a stencil is partially weighted)."""
     from htx.ic import vortex_modes
 
     n = 64
-    s = htx.System2D(0.0, 10.0, n, 0.0, 10.0, n, 0.5, 10.0, 0.0, GAMMA, flux=flux, bc="periodic",
-                     precision="fp32_mixed")
-    prm = params(n, n, 0.0, 10.0, 0.0, 10.0, GAMMA, "periodic", flux, True)
+    s, prm = make(n, n, "periodic", flux, precision="fp32_mixed")
     q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
     s.set_state(q)
-    assert rel_l1(s.rhs(), orc.fv2d_rhs(prm, q)) < 1e-8
-    s2 = htx.System2D(0.0, 1.0, 48, 0.0, 1.0, 48, 0.5, 0.02, 0.0, GAMMA, flux=flux, bc="zero_grad",
-                      precision="fp32_mixed")
-    q2 = quadrant_state(48, 48, seed=4)
-    s2.set_state(q2)
-    s2.timeStepping(0.02)
-    prm2 = params(48, 48, 0.0, 1.0, 0.0, 1.0, GAMMA, "zero_grad", flux, True)
-    qc, t, _ = orc.fv2d_advance(prm2, q2, 0.5, 0.0, 0.02)
-    err = rel_l1(s2.get_state(), qc)
-    print(f"fp32-mixed end-time rel L1 {err:.3e}", flush=True)
-    assert err < 1e-9
+    err = rel_l1(s.rhs(), orc.fv2d_rhs(prm, q))
+    print(f"fp32-mixed residual rel L1 {err:.3e}", flush=True)
+    assert err < 1e-8
+
+
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+@pytest.mark.parametrize("case", ["quadrant", "vortex"])
+def test_fp32_mixed_accuracy(flux, case):
+    """FP32-mixed against FP64 at equal resolution: the difference is at most
+    5% of the FP64 scheme's own discretization error, estimated as the
+    difference between the FP64 solution and the 2x-refined FP64 solution
+    averaged back onto the grid (the refined run starts from the coarse
+    initial state split 2x2, or from the vortex generator)."""
+    n = 48 if case == "quadrant" else 64
+    if case == "quadrant":
+        dom, bc, tend = (0.0, 1.0, 0.0, 1.0), "zero_grad", 0.02
+        q0 = quadrant_state(n, n, seed=4)
+        q0f = np.repeat(np.repeat(q0, 2, axis=1), 2, axis=2).copy()
+    else:
+        dom, bc, tend = (0.0, 10.0, 0.0, 10.0), "periodic", 0.5
+        q0 = q0f = None
+    print(f"[case] fp32_mixed accuracy {case} n={n} flux={flux} t={tend}", flush=True)
+    q64 = _run(n, q0, dom, bc, flux, tend, "fp64")
+    q32 = _run(n, q0, dom, bc, flux, tend, "fp32_mixed")
+    qf = _coarsen(_run(2 * n, q0f, dom, bc, flux, tend, "fp64"))
+    disc = rel_l1(q64, qf)
+    diff = rel_l1(q32, q64)
+    print(f"fp32-mixed vs fp64 rel L1 {diff:.3e}; fp64 discretization error estimate {disc:.3e}; "
+          f"ratio {diff / disc:.3e}", flush=True)
+    assert np.isfinite(q32).all()
+    assert diff <= 0.05 * disc
