@@ -67,6 +67,22 @@ def cpu_baseline(n, seconds=12.0):
                       f"FP64 CPU oracle (gcc -O3 -fopenmp), {el:.1f} s"}
 
 
+def copy_calibration(mib=256):
+    """One device-to-device copy of `mib` MiB (read `mib`, write `mib`), run
+    only to scale hardware byte counters (FETCH_SIZE / WRITE_SIZE) against a
+    kernel whose traffic is known exactly."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"[calib] 3 copies of {mib} MiB device to device")
+
+
 def time_variant(n, steps, warmup, **kw):
     """Cell-updates/s of one solver variant on the same workload (for the
     report's `variants` block)."""
@@ -97,6 +113,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 256 MiB device copy (for scaling byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -109,9 +128,11 @@ def main():
 
     n = args.n
     w = vortex_workload(n)
-    log(f"[gpu] creating {n}x{n} solver ({args.flux}, fp64)")
+    if args.calibrate_copy:
+        copy_calibration()
+    log(f"[gpu] creating {n}x{n} solver ({args.flux}, {args.precision})")
     s = htx.System2D(w["xl"], w["xr"], n, w["yb"], w["yt"], n, w["cfl"], w["tmax"], 0.0, w["gamma"],
-                     flux=args.flux, bc="periodic", precision="fp64")
+                     flux=args.flux, bc="periodic", precision=args.precision)
     s.isentropicVortex(w["centre"], w["modes"])
     log(f"[gpu] warm-up {args.warmup} steps")
     s.steps_async(args.warmup)
@@ -154,8 +175,9 @@ def main():
         "data": "synthetic (seeded isentropic vortex + Fourier density modes, generated on device)",
         "config": {"workload": f"2D Euler periodic isentropic vortex + 8 seeded Fourier modes, {n}x{n} on "
                                f"[0,10]^2, TENO5 char. + 3-pt transverse quadrature + {args.flux.upper()}, "
-                               f"SSP-RK3 CFL 0.5, FP64 (plain drop-in path)",
-                   "nx": n, "ny": n, "scheme": "teno5", "flux": args.flux, "precision": "fp64",
+                               f"SSP-RK3 CFL 0.5, precision {args.precision}"
+                               + (" (plain drop-in path)" if args.precision == "fp64" else ""),
+                   "nx": n, "ny": n, "scheme": "teno5", "flux": args.flux, "precision": args.precision,
                    "parallelism": "single GPU"},
         "roofline": {
             "bound": "hbm",

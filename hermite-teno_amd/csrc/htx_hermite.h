@@ -159,17 +159,29 @@ HTX_HD double herm_limiter(double dif, double u0, double u1, double u2, double d
 }
 
 // Flux Jacobian times a moment vector, Euler2D::dfluxx/dfluxy
-// (Equations_Euler2D.cpp:52-98), in (rho, normal, tangential, E) order.
-HTX_HD void dflux_mul(double gam, double g1, double q3, double rho, double un, double ut, double p,
-                      const double dq[4], double out[4]) {
+// (Equations_Euler2D.cpp:52-98), in the natural (rho, x, y, E) order: each
+// row is summed left to right over the Jacobian's columns, as the 4x4
+// product of the reference does (its zero entries drop out exactly).
+template <bool YDIR>
+HTX_HD void dflux_mul(double gam, double g1, double q3, double rho, double u, double v, double p,
+                      const double d[4], double o[4]) {
     double g3 = gam - 3.0;
-    double sq = un * un;
-    double dyn = 0.5 * g1 * (sq + ut * ut);
     double h = (q3 + p) / rho;
-    out[0] = dq[1];
-    out[1] = (((dyn - sq) * dq[0] + (-g3 * un) * dq[1]) + (-g1 * ut) * dq[2]) + g1 * dq[3];
-    out[2] = ((-un * ut) * dq[0] + ut * dq[1]) + un * dq[2];
-    out[3] = ((((dyn - h) * un) * dq[0] + (h - g1 * sq) * dq[1]) + (-g1 * un * ut) * dq[2]) + (gam * un) * dq[3];
+    if (!YDIR) {
+        double sq = u * u;
+        double dyn = 0.5 * g1 * (sq + v * v);
+        o[0] = d[1];
+        o[1] = (((dyn - sq) * d[0] + (-g3 * u) * d[1]) + (-g1 * v) * d[2]) + g1 * d[3];
+        o[2] = ((-u * v) * d[0] + v * d[1]) + u * d[2];
+        o[3] = ((((dyn - h) * u) * d[0] + (h - g1 * sq) * d[1]) + (-g1 * u * v) * d[2]) + (gam * u) * d[3];
+    } else {
+        double sq = v * v;
+        double dyn = 0.5 * g1 * (sq + u * u);
+        o[0] = d[2];
+        o[1] = ((-u * v) * d[0] + v * d[1]) + u * d[2];
+        o[2] = (((dyn - sq) * d[0] + (-g1 * u) * d[1]) + (-g3 * v) * d[2]) + g1 * d[3];
+        o[3] = ((((dyn - h) * v) * d[0] + (-g1 * u * v) * d[1]) + (h - g1 * sq) * d[2]) + (gam * v) * d[3];
+    }
 }
 
 // Hermite state layout on the device: 12 fields, u (4), v = <dq/dx> (4),
@@ -287,23 +299,30 @@ HTX_HD void rot_to_nat(const double r[4], double& q0, double& qx, double& qy, do
     q3 = r[3];
 }
 
-// Euler2D::dRusanov (Equations_Euler2D.cpp:352-360) in (rho, n, t, E) order:
-// 0.5 (Jl dql + Jr dqr - R diag(a) L (dqr - dql)).
+// Euler2D::dRusanov (Equations_Euler2D.cpp:352-360):
+// 0.5 (Jl dql + Jr dqr - R diag(a) L (dqr - dql)).  States and moments come
+// in (rho, n, t, E) order, the result goes out in the same order; the
+// products run in the natural order.
 template <bool YDIR>
 HTX_HD void drusanov(double gam, double g1, const Eig& E, const double ql[4], const double qr[4], double pl,
                      double pr, const double a[4], const double dql[4], const double dqr[4], double f[4]) {
-    double jl[4], jr[4], dd[4], c0, c1, c2, c3, r0, rn, rt, r3;
+    double dl[4], dr[4], jl[4], jr[4], c0, c1, c2, c3, r0, rn, rt, r3;
+    rot_to_nat<YDIR>(dql, dl[0], dl[1], dl[2], dl[3]);
+    rot_to_nat<YDIR>(dqr, dr[0], dr[1], dr[2], dr[3]);
+#if HTX_EXACT_DIVISION
+    double unl = ql[1] / ql[0], utl = ql[2] / ql[0], unr = qr[1] / qr[0], utr = qr[2] / qr[0];
+#else
     double il = 1.0 / ql[0], ir = 1.0 / qr[0];
-    dflux_mul(gam, g1, ql[3], ql[0], ql[1] * il, ql[2] * il, pl, dql, jl);
-    dflux_mul(gam, g1, qr[3], qr[0], qr[1] * ir, qr[2] * ir, pr, dqr, jr);
-    for (int e = 0; e < 4; ++e) dd[e] = dqr[e] - dql[e];
-    double n0, nx, ny, n3;
-    rot_to_nat<YDIR>(dd, n0, nx, ny, n3);
-    eig_left<YDIR>(E, n0, nx, ny, n3, c0, c1, c2, c3);
+    double unl = ql[1] * il, utl = ql[2] * il, unr = qr[1] * ir, utr = qr[2] * ir;
+#endif
+    dflux_mul<YDIR>(gam, g1, ql[3], ql[0], YDIR ? utl : unl, YDIR ? unl : utl, pl, dl, jl);
+    dflux_mul<YDIR>(gam, g1, qr[3], qr[0], YDIR ? utr : unr, YDIR ? unr : utr, pr, dr, jr);
+    eig_left<YDIR>(E, dr[0] - dl[0], dr[1] - dl[1], dr[2] - dl[2], dr[3] - dl[3], c0, c1, c2, c3);
     eig_right(E, a[0] * c0, a[1] * c1, a[1] * c2, a[3] * c3, r0, rn, rt, r3);
+    const int n = YDIR ? 2 : 1, t = YDIR ? 1 : 2;
     f[0] = 0.5 * ((jl[0] + jr[0]) - r0);
-    f[1] = 0.5 * ((jl[1] + jr[1]) - rn);
-    f[2] = 0.5 * ((jl[2] + jr[2]) - rt);
+    f[1] = 0.5 * ((jl[n] + jr[n]) - rn);
+    f[2] = 0.5 * ((jl[t] + jr[t]) - rt);
     f[3] = 0.5 * ((jl[3] + jr[3]) - r3);
 }
 
@@ -345,7 +364,11 @@ HTX_HD void fvh_flux_gp(const FvhArgs& a, int i, int j, int m, const double* F, 
     double fl0, fln, flt, fl3, wnl, pl, rl, fr0, frn, frt, fr3, wnr, pr, rr;
     phys_flux(a.g1, Q[0][0], Q[0][1], Q[0][2], Q[0][3], fl0, fln, flt, fl3, wnl, pl, rl);
     phys_flux(a.g1, Q[1][0], Q[1][1], Q[1][2], Q[1][3], fr0, frn, frt, fr3, wnr, pr, rr);
+#if HTX_EXACT_DIVISION
+    double cl = sqrt(fabs(a.gam * pl / rl)), cr = sqrt(fabs(a.gam * pr / rr));
+#else
     double cl = sqrt(fabs(a.gam * pl * (1.0 / rl))), cr = sqrt(fabs(a.gam * pr * (1.0 / rr)));
+#endif
     double am[4];
     am[0] = fmax(fabs(wnl - cl), fabs(wnr - cr));
     am[1] = fmax(fabs(wnl), fabs(wnr));
