@@ -31,6 +31,10 @@ BYTES_PER_CELL_UPDATE = 256.0
 # TENO5 sides per field and direction, 8 transverse quadratures per face and
 # direction, back-projection, flux at 3 Gauss points, divergence, RK3).
 FLOPS_PER_CELL_UPDATE = 18.2e3
+# HBM bytes per stage-kernel launch at 1024^2, FP64, measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE scaled against a 256 MiB device copy
+# (profiles/fv2d_stage_kernel_vortex1024_fp64_hbm_traffic.md); per cell-stage.
+MEASURED_TRAFFIC_PER_CELL_STAGE = {"fp64": 99.55e6 / (1024 * 1024)}
 
 
 def log(*a):
@@ -185,7 +189,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": (MEASURED_TRAFFIC_PER_CELL_STAGE[args.precision] * cells
+                        if args.precision in MEASURED_TRAFFIC_PER_CELL_STAGE else None),
             "kernel": "fv2d_stage_kernel",
             "avg_launch_ms": avg_launch_s * 1e3,
             "bytes_per_launch": bytes_per_launch,
