@@ -370,6 +370,64 @@ int htx_init_vortex(htx_solver* h, double cx, double cy, const double* modes, in
     return HTX_OK;
 }
 
+namespace {
+PiecewiseArgs piecewise_args(const htx_solver* h, int kind) {
+    PiecewiseArgs a{};
+    a.q = h->dA;
+    a.nx = h->nx;
+    a.ny = h->ny;
+    a.pitch = h->pitch;
+    a.nfield = h->nfield;
+    a.fstride = h->fstride;
+    a.xl = h->p.xl;
+    a.yb = h->p.yb;
+    a.dx = h->dx;
+    a.dy = h->dy;
+    a.gam = h->p.gamma;
+    a.kind = kind;
+    return a;
+}
+
+int finish_ic(htx_solver* h) {
+    h->have_state = true;
+    int rc = refresh_cfl(h);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return HTX_OK;
+}
+}  // namespace
+
+int htx_init_quadrant(htx_solver* h, double xs, double ys, const double* states) {
+    if (!h || !states) return fail(HTX_EINVAL, "null argument");
+    PiecewiseArgs a = piecewise_args(h, IC_QUADRANT);
+    a.xs = xs;
+    a.ys = ys;
+    for (int k = 0; k < 4; ++k)
+        for (int e = 0; e < 4; ++e) a.st[k][e] = states[4 * k + e];
+    HIP_TRY(launch_ic_piecewise(a, h->stream));
+    return finish_ic(h);
+}
+
+int htx_init_shock_blobs(htx_solver* h, double xs, double ys, double nx_, double ny_, const double* post,
+                         const double* ambient, const double* blobs, int nblob) {
+    if (!h || !post || !ambient || (nblob > 0 && !blobs)) return fail(HTX_EINVAL, "null argument");
+    if (nblob < 0 || nblob > IC_MAX_BLOBS) return fail(HTX_EINVAL, "nblob must be in [0, 64]");
+    PiecewiseArgs a = piecewise_args(h, IC_SHOCK_BLOBS);
+    a.xs = xs;
+    a.ys = ys;
+    a.nrm[0] = nx_;
+    a.nrm[1] = ny_;
+    for (int e = 0; e < 4; ++e) {
+        a.st[0][e] = post[e];
+        a.st[1][e] = ambient[e];
+    }
+    a.nblob = nblob;
+    for (int b = 0; b < nblob; ++b)
+        for (int e = 0; e < 4; ++e) a.blob[b][e] = blobs[4 * b + e];
+    HIP_TRY(launch_ic_piecewise(a, h->stream));
+    return finish_ic(h);
+}
+
 int htx_rhs(htx_solver* h, double* rhs, size_t ld) {
     if (!h || !rhs) return fail(HTX_EINVAL, "null argument");
     if (ld < (size_t)h->nx) return fail(HTX_EINVAL, "ld < nx");

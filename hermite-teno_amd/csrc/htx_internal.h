@@ -45,6 +45,25 @@ struct VortexArgs {
     double gx[9], gw[9];
 };
 
+enum IcKind { IC_QUADRANT = 0, IC_SHOCK_BLOBS = 1 };
+constexpr int IC_MAX_BLOBS = 64;
+
+// Piecewise initial states, point-sampled at cell centres.
+//   IC_QUADRANT:    primitive state st[k], k = (x >= xs) + 2 (y >= ys)
+//   IC_SHOCK_BLOBS: st[0] where (x - xs, y - ys) . nrm < 0 (post-shock), else
+//                   st[1] with rho + sum_b A_b exp(-|x - c_b|^2 / r_b^2),
+//                   blob rows (cx, cy, A, r)
+struct PiecewiseArgs {
+    double* q;
+    int nx, ny, pitch, nfield;
+    long long fstride;
+    double xl, yb, dx, dy, gam;
+    int kind, nblob;
+    double xs, ys, nrm[2];
+    double st[4][4];
+    double blob[IC_MAX_BLOBS][4];
+};
+
 // Boundary conditions are applied through the index: zero gradient clamps,
 // periodic wraps.  This is what zeroGradBc / periodicBc (FV_System2D.cpp
 // :114-152) leave in every ghost cell a stencil can reach.
@@ -64,5 +83,7 @@ hipError_t launch_reset_smax(Ctl* c, hipStream_t s);
 hipError_t launch_max_speed(const double* q, int nx, int ny, int pitch, long long fstride, double gam,
                             int quirks, Ctl* c, hipStream_t s);
 hipError_t launch_ic_vortex(const VortexArgs& a, hipStream_t s);
+// htx_ic.hip (fast build only)
+hipError_t launch_ic_piecewise(const PiecewiseArgs& a, hipStream_t s);
 
 }  // namespace HTX_NS

@@ -65,6 +65,8 @@ SIGNATURES = {
     "htx_set_state": (_I, [_P, _DP, _S]),
     "htx_get_state": (_I, [_P, _DP, _S]),
     "htx_init_vortex": (_I, [_P, _D, _D, _DP, _I]),
+    "htx_init_quadrant": (_I, [_P, _D, _D, _DP]),
+    "htx_init_shock_blobs": (_I, [_P, _D, _D, _D, _D, _DP, _DP, _DP, _I]),
     "htx_rhs": (_I, [_P, _DP, _S]),
     "htx_compute_dt": (_I, [_P, _DP]),
     "htx_step": (_I, [_P, _DP]),

@@ -91,6 +91,33 @@ class System2D:
         ptr = dptr(mm) if mm.size else None
         check(_lib.load().htx_init_vortex(self._h, float(centre[0]), float(centre[1]), ptr, int(m.shape[0])))
 
+    def quadrant(self, xs, ys, states):
+        """Four-quadrant Riemann data (primitive rows (rho, u, v, p) for
+        k = (x >= xs) + 2 (y >= ys)), point-sampled on the device as
+        FV::System2D::Riemann2D samples its quadrant template."""
+        st = np.ascontiguousarray(np.asarray(states, dtype=np.float64).reshape(16))
+        check(_lib.load().htx_init_quadrant(self._h, float(xs), float(ys), dptr(st)))
+
+    def shock_blobs(self, xs, ys, nrm, post, ambient, blobs):
+        """Planar shock (post-shock primitive state where (x - xs, y - ys) . nrm < 0)
+        into ``ambient`` carrying Gaussian density blobs (cx, cy, A, r)."""
+        b = np.ascontiguousarray(np.asarray(blobs, dtype=np.float64).reshape(-1))
+        po = np.ascontiguousarray(post, dtype=np.float64)
+        am = np.ascontiguousarray(ambient, dtype=np.float64)
+        check(_lib.load().htx_init_shock_blobs(self._h, float(xs), float(ys), float(nrm[0]), float(nrm[1]), dptr(po),
+                                               dptr(am), dptr(b) if b.size else None, int(b.size // 4)))
+
+    def init_workload(self, w):
+        """Initial state of a workload dict from htx.ic (vortex, quadrant or
+        shock_blobs)."""
+        kind = w.get("kind", "vortex")
+        if kind == "vortex":
+            self.isentropicVortex(w["centre"], w["modes"])
+        elif kind == "quadrant":
+            self.quadrant(w["xs"], w["ys"], w["states"])
+        else:
+            self.shock_blobs(w["xs"], w["ys"], w["nrm"], w["post"], w["ambient"], w["blobs"])
+
     # -- the hot path -----------------------------------------------------
     def rhs(self):
         """One residual evaluation L(q) (FV::System2D::rhs)."""

@@ -100,6 +100,23 @@ int htx_get_state(htx_solver* h, double* q, size_t ld);
  * nmodes <= 16. */
 int htx_init_vortex(htx_solver* h, double cx, double cy, const double* modes, int nmodes);
 
+/* Four-quadrant Riemann data, generated on the device and point-sampled at
+ * cell centres as FV::System2D::Riemann2D does (FV_System2D.cpp:101-111 with
+ * the quadrant template of Equations_Euler2D.cpp:530-617), with the split
+ * point and the four states as arguments: states holds 4 primitive rows
+ * (rho, u, v, p) for quadrants k = (x >= xs) + 2 (y >= ys).  For the Hermite
+ * scheme the derivative moments are the face differences of the point data
+ * (FVHermite_System2D.cpp:89-107). */
+int htx_init_quadrant(htx_solver* h, double xs, double ys, const double* states);
+
+/* Planar shock into a gas carrying Gaussian density blobs (no reference
+ * counterpart; same point sampling): where (x - xs, y - ys) . (nx, ny) < 0
+ * the primitive state post[4], elsewhere ambient[4] with
+ * rho += A exp(-|x - c|^2 / r^2) for each blob row (cx, cy, A, r);
+ * nblob <= 64. */
+int htx_init_shock_blobs(htx_solver* h, double xs, double ys, double nx, double ny, const double* post,
+                         const double* ambient, const double* blobs, int nblob);
+
 /* One residual evaluation L(q) of the current state, FV::System2D::rhs
  * (FV_System2D.cpp:363-573); written to host memory in the state layout. */
 int htx_rhs(htx_solver* h, double* rhs, size_t ld);
