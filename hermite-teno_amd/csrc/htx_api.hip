@@ -8,6 +8,8 @@
 
 #include "../../include/htx.h"
 #include "htx_hermite.h"
+#include "htx_learn.h"
+#include "htx_learn_api.h"
 
 namespace htx {
 hipError_t launch_fvh2d_stage(const FvhArgs& a, int flux, hipStream_t s);
@@ -71,6 +73,12 @@ struct htx_solver {
     QuadConsts qc{};
     bool have_state = false;
     bool timing = false;
+    // learned classifier (precision HTX_PREC_LEARNED): packed bf16 fragments,
+    // FP32 biases and output weights, logit threshold
+    unsigned short* dFrag = nullptr;
+    float* dCvec = nullptr;
+    float cls_threshold = 0.0f;
+    bool have_classifier = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> event_pool;
 };
@@ -313,6 +321,8 @@ void htx_destroy(htx_solver* h) {
     if (h->ctl) (void)hipFree(h->ctl);
     for (double* b : {h->dLV, h->dFX, h->dFY, h->dGX, h->dGY})
         if (b) (void)hipFree(b);
+    if (h->dFrag) (void)hipFree(h->dFrag);
+    if (h->dCvec) (void)hipFree(h->dCvec);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -534,6 +544,371 @@ int htx_timing_read(htx_solver* h, double* kernel_ms, int* nlaunch) {
     if (kernel_ms) *kernel_ms = total;
     if (nlaunch) *nlaunch = (int)h->events.size();
     h->events.clear();
+    return HTX_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// learned troubled-face classifier: training set, training, inference
+
+}  // extern "C"
+
+struct htx_trainer {
+    long long capacity = 0, n = 0;
+    float* dFeat = nullptr;
+    unsigned char* dLabel = nullptr;
+    float *dPrm = nullptr, *dM = nullptr, *dV = nullptr, *dPartial = nullptr;
+    unsigned long long* dCount = nullptr;  // [0] smooth labels, [1..4] confusion counts
+    double* dLoss = nullptr;
+    int partial_rows = 0;
+    long long adam_t = 0;
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+unsigned long long splitmix(unsigned long long& s) {
+    s += 0x9E3779B97F4A7C15ull;
+    unsigned long long z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+double splitmix_uniform(unsigned long long& s) { return (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+unsigned long long gcd_u64(unsigned long long a, unsigned long long b) {
+    while (b) {
+        unsigned long long t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+constexpr int TRAIN_CHUNKS = 4;            // chunks of 32 samples per workgroup and minibatch
+constexpr int TRAIN_SAMPLES_PER_BLOCK = 32 * TRAIN_CHUNKS;
+
+}  // namespace
+
+extern "C" {
+
+int htx_trainer_create(long long capacity, htx_trainer** out) {
+    if (!out) return fail(HTX_EINVAL, "out is null");
+    *out = nullptr;
+    if (capacity <= 0 || capacity > (1ll << 31)) return fail(HTX_EINVAL, "capacity must be in [1, 2^31]");
+    htx_trainer* t = new htx_trainer();
+    t->capacity = capacity;
+    if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete t;
+        return fail(HTX_EDEVICE, "hipStreamCreate failed");
+    }
+    if (hipMalloc(&t->dFeat, sizeof(float) * CLS_IN * (size_t)capacity) != hipSuccess ||
+        hipMalloc(&t->dLabel, (size_t)capacity) != hipSuccess ||
+        hipMalloc(&t->dPrm, sizeof(float) * CLS_NPARAM) != hipSuccess ||
+        hipMalloc(&t->dM, sizeof(float) * CLS_NPARAM) != hipSuccess ||
+        hipMalloc(&t->dV, sizeof(float) * CLS_NPARAM) != hipSuccess ||
+        hipMalloc(&t->dCount, sizeof(unsigned long long) * 8) != hipSuccess ||
+        hipMalloc(&t->dLoss, sizeof(double)) != hipSuccess) {
+        htx_trainer_destroy(t);
+        return fail(HTX_ENOMEM, "device allocation failed");
+    }
+    if (hipMemset(t->dCount, 0, sizeof(unsigned long long) * 8) != hipSuccess ||
+        hipMemset(t->dPrm, 0, sizeof(float) * CLS_NPARAM) != hipSuccess) {
+        htx_trainer_destroy(t);
+        return fail(HTX_EDEVICE, "device initialisation failed");
+    }
+    *out = t;
+    return HTX_OK;
+}
+
+void htx_trainer_destroy(htx_trainer* t) {
+    if (!t) return;
+    if (t->stream) (void)hipStreamSynchronize(t->stream);
+    for (void* b : {(void*)t->dFeat, (void*)t->dLabel, (void*)t->dPrm, (void*)t->dM, (void*)t->dV,
+                    (void*)t->dPartial, (void*)t->dCount, (void*)t->dLoss})
+        if (b) (void)hipFree(b);
+    if (t->stream) (void)hipStreamDestroy(t->stream);
+    delete t;
+}
+
+int htx_trainer_add_state(htx_trainer* t, htx_solver* h) {
+    if (!t || !h) return fail(HTX_EINVAL, "null handle");
+    if (!h->have_state) return fail(HTX_ESTATE, "the solver has no state");
+    if (h->p.scheme != HTX_SCHEME_TENO5) return fail(HTX_ENOTSUP, "training faces come from the TENO5 scheme");
+    long long nf = 2ll * h->nx * h->ny;
+    if (t->n + nf > t->capacity) return fail(HTX_ENOMEM, "training set capacity exceeded");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int dir = 0; dir < 2; ++dir) {
+        LrnExtractArgs a{};
+        a.q = h->dA;
+        a.nx = h->nx;
+        a.ny = h->ny;
+        a.pitch = h->pitch;
+        a.bc = h->p.bc;
+        a.fstride = h->fstride;
+        a.gam = h->p.gamma;
+        a.feat = t->dFeat;
+        a.label = t->dLabel;
+        a.base = t->n + (long long)dir * h->nx * h->ny;
+        a.nsmooth = t->dCount;
+        HIP_TRY(lrn_launch_extract(a, dir, t->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    t->n += nf;
+    return HTX_OK;
+}
+
+int htx_trainer_size(htx_trainer* t, long long* n, long long* n_smooth) {
+    if (!t) return fail(HTX_EINVAL, "null handle");
+    unsigned long long c = 0;
+    HIP_TRY(hipMemcpy(&c, t->dCount, sizeof c, hipMemcpyDeviceToHost));
+    if (n) *n = t->n;
+    if (n_smooth) *n_smooth = (long long)c;
+    return HTX_OK;
+}
+
+int htx_trainer_read(htx_trainer* t, long long start, long long n, float* feat, unsigned char* label) {
+    if (!t) return fail(HTX_EINVAL, "null handle");
+    if (start < 0 || n < 0 || start + n > t->n) return fail(HTX_EINVAL, "range outside the training set");
+    if (feat) HIP_TRY(hipMemcpy(feat, t->dFeat + start * CLS_IN, sizeof(float) * CLS_IN * n, hipMemcpyDeviceToHost));
+    if (label) HIP_TRY(hipMemcpy(label, t->dLabel + start, (size_t)n, hipMemcpyDeviceToHost));
+    return HTX_OK;
+}
+
+int htx_trainer_init(htx_trainer* t, unsigned long long seed) {
+    if (!t) return fail(HTX_EINVAL, "null handle");
+    std::vector<float> prm(CLS_NPARAM, 0.0f);
+    unsigned long long s = seed;
+    auto fill = [&](int off, int fan_in, int fan_out) {
+        double lim = std::sqrt(6.0 / (fan_in + fan_out));
+        for (int k = 0; k < fan_in * fan_out; ++k) prm[off + k] = (float)((2.0 * splitmix_uniform(s) - 1.0) * lim);
+    };
+    fill(CLS_W1, CLS_IN, CLS_H);
+    fill(CLS_W2, CLS_H, CLS_H);
+    fill(CLS_W3, CLS_H, 1);
+    HIP_TRY(hipMemcpy(t->dPrm, prm.data(), sizeof(float) * CLS_NPARAM, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(t->dM, 0, sizeof(float) * CLS_NPARAM));
+    HIP_TRY(hipMemset(t->dV, 0, sizeof(float) * CLS_NPARAM));
+    t->adam_t = 0;
+    return HTX_OK;
+}
+
+int htx_trainer_set_params(htx_trainer* t, const float* params) {
+    if (!t || !params) return fail(HTX_EINVAL, "null argument");
+    HIP_TRY(hipMemcpy(t->dPrm, params, sizeof(float) * CLS_NPARAM, hipMemcpyHostToDevice));
+    return HTX_OK;
+}
+
+int htx_trainer_get_params(htx_trainer* t, float* params) {
+    if (!t || !params) return fail(HTX_EINVAL, "null argument");
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    HIP_TRY(hipMemcpy(params, t->dPrm, sizeof(float) * CLS_NPARAM, hipMemcpyDeviceToHost));
+    return HTX_OK;
+}
+
+int htx_trainer_gradient(htx_trainer* t, double troubled_weight, float* grad, double* loss) {
+    if (!t || !grad) return fail(HTX_EINVAL, "null argument");
+    if (t->n < 1) return fail(HTX_ESTATE, "the training set is empty");
+    if (!(troubled_weight > 0.0)) return fail(HTX_EINVAL, "troubled_weight must be positive");
+    unsigned long long nsm = 0;
+    HIP_TRY(hipMemcpy(&nsm, t->dCount, sizeof nsm, hipMemcpyDeviceToHost));
+    double fpos = (double)nsm / (double)t->n, fneg = 1.0 - fpos;
+    if (fpos <= 0.0 || fneg <= 0.0) return fail(HTX_ESTATE, "the training set holds only one class");
+    int nblk = (int)((t->n + TRAIN_SAMPLES_PER_BLOCK - 1) / TRAIN_SAMPLES_PER_BLOCK);
+    if (nblk > t->partial_rows) {
+        if (t->dPartial) HIP_TRY(hipFree(t->dPartial));
+        t->dPartial = nullptr;
+        if (hipMalloc(&t->dPartial, sizeof(float) * (CLS_NPARAM + 1) * (size_t)nblk) != hipSuccess)
+            return fail(HTX_ENOMEM, "device allocation failed");
+        t->partial_rows = nblk;
+    }
+    float* dG = nullptr;
+    if (hipMalloc(&dG, sizeof(float) * CLS_NPARAM) != hipSuccess) return fail(HTX_ENOMEM, "device allocation failed");
+    double wpos = 0.5 / fpos, wneg = 0.5 / fneg * troubled_weight;
+    double norm = 1.0 / (fpos * wpos + fneg * wneg) / (double)t->n;
+    LrnTrainArgs a{};
+    a.feat = t->dFeat;
+    a.label = t->dLabel;
+    a.n = t->n;
+    a.prm = t->dPrm;
+    a.partial = t->dPartial;
+    a.k0 = 0;
+    a.kend = t->n;
+    a.pa = 1;
+    a.pb = 0;
+    a.wpos = (float)(wpos * norm);
+    a.wneg = (float)(wneg * norm);
+    a.chunks = TRAIN_CHUNKS;
+    LrnUpdateArgs u{};
+    u.partial = t->dPartial;
+    u.nblk = nblk;
+    u.loss = t->dLoss;
+    u.grad_out = dG;
+    hipError_t e = hipMemsetAsync(t->dLoss, 0, sizeof(double), t->stream);
+    if (e == hipSuccess) e = lrn_launch_train(a, nblk, u, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(grad, dG, sizeof(float) * CLS_NPARAM, hipMemcpyDeviceToHost, t->stream);
+    double l = 0.0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&l, t->dLoss, sizeof l, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    (void)hipFree(dG);
+    if (e != hipSuccess) return fail(HTX_EDEVICE, std::string("gradient: ") + hipGetErrorString(e));
+    if (loss) *loss = l;
+    return HTX_OK;
+}
+
+int htx_trainer_train(htx_trainer* t, int epochs, int batch, double lr, double troubled_weight,
+                      unsigned long long seed, double* epoch_loss) {
+    if (!t) return fail(HTX_EINVAL, "null handle");
+    if (t->n < 1) return fail(HTX_ESTATE, "the training set is empty");
+    if (epochs < 0 || batch < TRAIN_SAMPLES_PER_BLOCK || !(lr > 0.0) || !(troubled_weight > 0.0))
+        return fail(HTX_EINVAL, "epochs >= 0, batch >= 128, lr > 0 and troubled_weight > 0 required");
+    unsigned long long nsm = 0;
+    HIP_TRY(hipMemcpy(&nsm, t->dCount, sizeof nsm, hipMemcpyDeviceToHost));
+    double fpos = (double)nsm / (double)t->n, fneg = 1.0 - fpos;
+    if (fpos <= 0.0 || fneg <= 0.0) return fail(HTX_ESTATE, "the training set holds only one class");
+    int nblk = (batch + TRAIN_SAMPLES_PER_BLOCK - 1) / TRAIN_SAMPLES_PER_BLOCK;
+    int rows = nblk;
+    if (rows > t->partial_rows) {
+        if (t->dPartial) HIP_TRY(hipFree(t->dPartial));
+        t->dPartial = nullptr;
+        if (hipMalloc(&t->dPartial, sizeof(float) * (CLS_NPARAM + 1) * (size_t)rows) != hipSuccess)
+            return fail(HTX_ENOMEM, "device allocation failed");
+        t->partial_rows = rows;
+    }
+    long long bsz = (long long)nblk * TRAIN_SAMPLES_PER_BLOCK;
+    // balanced classes, troubled faces weighted up; weights sum to about 1 per minibatch
+    double wpos = 0.5 / fpos, wneg = 0.5 / fneg * troubled_weight;
+    double norm = 1.0 / (fpos * wpos + fneg * wneg) / (double)bsz;
+    unsigned long long rs = seed;
+    const double b1 = 0.9, b2 = 0.999;
+    for (int ep = 0; ep < epochs; ++ep) {
+        unsigned long long n = (unsigned long long)t->n, pa;
+        do {
+            pa = 1 + splitmix(rs) % n;
+        } while (n > 1 && gcd_u64(pa, n) != 1);
+        unsigned long long pb = splitmix(rs) % n;
+        HIP_TRY(hipMemsetAsync(t->dLoss, 0, sizeof(double), t->stream));
+        long long nbatch = 0;
+        for (long long k0 = 0; k0 < t->n; k0 += bsz) {
+            ++t->adam_t;
+            LrnTrainArgs a{};
+            a.feat = t->dFeat;
+            a.label = t->dLabel;
+            a.n = t->n;
+            a.prm = t->dPrm;
+            a.partial = t->dPartial;
+            a.k0 = k0;
+            a.kend = k0 + bsz < t->n ? k0 + bsz : t->n;
+            a.pa = pa;
+            a.pb = pb;
+            a.wpos = (float)(wpos * norm);
+            a.wneg = (float)(wneg * norm);
+            a.chunks = TRAIN_CHUNKS;
+            LrnUpdateArgs u{};
+            u.partial = t->dPartial;
+            u.nblk = nblk;
+            u.prm = t->dPrm;
+            u.m = t->dM;
+            u.v = t->dV;
+            u.loss = t->dLoss;
+            u.beta1 = (float)b1;
+            u.beta2 = (float)b2;
+            u.eps = 1e-8f;
+            u.lr_t = (float)(lr * std::sqrt(1.0 - std::pow(b2, (double)t->adam_t)) /
+                             (1.0 - std::pow(b1, (double)t->adam_t)));
+            HIP_TRY(lrn_launch_train(a, nblk, u, t->stream));
+            ++nbatch;
+        }
+        if (epoch_loss) {
+            double l = 0.0;
+            HIP_TRY(hipMemcpyAsync(&l, t->dLoss, sizeof l, hipMemcpyDeviceToHost, t->stream));
+            HIP_TRY(hipStreamSynchronize(t->stream));
+            epoch_loss[ep] = l / (double)nbatch;
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    return HTX_OK;
+}
+
+int htx_trainer_eval(htx_trainer* t, double threshold, long long* counts, float* logits) {
+    if (!t || !counts) return fail(HTX_EINVAL, "null argument");
+    if (t->n < 1) return fail(HTX_ESTATE, "the training set is empty");
+    float* dLog = nullptr;
+    if (logits && hipMalloc(&dLog, sizeof(float) * (size_t)t->n) != hipSuccess)
+        return fail(HTX_ENOMEM, "device allocation failed");
+    HIP_TRY(hipMemsetAsync(t->dCount + 1, 0, sizeof(unsigned long long) * 4, t->stream));
+    LrnEvalArgs a{};
+    a.feat = t->dFeat;
+    a.label = t->dLabel;
+    a.n = t->n;
+    a.prm = t->dPrm;
+    a.threshold = (float)threshold;
+    a.counts = t->dCount + 1;
+    a.logits = dLog;
+    hipError_t e = lrn_launch_eval(a, t->stream);
+    unsigned long long c[4] = {0, 0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(c, t->dCount + 1, sizeof c, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess && logits)
+        e = hipMemcpyAsync(logits, dLog, sizeof(float) * (size_t)t->n, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    if (dLog) (void)hipFree(dLog);
+    if (e != hipSuccess) return fail(HTX_EDEVICE, std::string("classifier evaluation: ") + hipGetErrorString(e));
+    for (int k = 0; k < 4; ++k) counts[k] = (long long)c[k];
+    return HTX_OK;
+}
+
+int htx_load_classifier(htx_solver* h, const float* params, double threshold) {
+    if (!h || !params) return fail(HTX_EINVAL, "null argument");
+    for (int k = 0; k < CLS_NPARAM; ++k)
+        if (!std::isfinite(params[k])) return fail(HTX_EINVAL, "classifier parameters must be finite");
+    std::vector<unsigned short> frag(CLS_NFRAG);
+    std::vector<float> cvec(CLS_NCVEC);
+    cls_pack(params, frag.data(), cvec.data());
+    if (!h->dFrag && (hipMalloc(&h->dFrag, sizeof(unsigned short) * CLS_NFRAG) != hipSuccess ||
+                      hipMalloc(&h->dCvec, sizeof(float) * CLS_NCVEC) != hipSuccess))
+        return fail(HTX_ENOMEM, "device allocation failed");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(h->dFrag, frag.data(), sizeof(unsigned short) * CLS_NFRAG, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->dCvec, cvec.data(), sizeof(float) * CLS_NCVEC, hipMemcpyHostToDevice));
+    h->cls_threshold = (float)threshold;
+    h->have_classifier = true;
+    return HTX_OK;
+}
+
+int htx_classify(htx_solver* h, int dir, float* logits, float* feat) {
+    if (!h || !logits) return fail(HTX_EINVAL, "null argument");
+    if (dir != 0 && dir != 1) return fail(HTX_EINVAL, "dir must be 0 (x faces) or 1 (y faces)");
+    if (!h->have_classifier) return fail(HTX_ESTATE, "no classifier loaded");
+    if (!h->have_state) return fail(HTX_ESTATE, "the solver has no state");
+    if (h->p.scheme != HTX_SCHEME_TENO5) return fail(HTX_ENOTSUP, "the classifier runs on the TENO5 scheme");
+    size_t nf = (size_t)h->nx * h->ny;
+    float *dL = nullptr, *dF = nullptr;
+    if (hipMalloc(&dL, sizeof(float) * nf) != hipSuccess ||
+        (feat && hipMalloc(&dF, sizeof(float) * CLS_IN * nf) != hipSuccess)) {
+        if (dL) (void)hipFree(dL);
+        return fail(HTX_ENOMEM, "device allocation failed");
+    }
+    LrnClassifyArgs a{};
+    a.q = h->dA;
+    a.nx = h->nx;
+    a.ny = h->ny;
+    a.pitch = h->pitch;
+    a.bc = h->p.bc;
+    a.fstride = h->fstride;
+    a.gam = h->p.gamma;
+    a.frag = h->dFrag;
+    a.cvec = h->dCvec;
+    a.logits = dL;
+    a.feat = dF;
+    hipError_t e = lrn_launch_classify(a, dir, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(logits, dL, sizeof(float) * nf, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && feat)
+        e = hipMemcpyAsync(feat, dF, sizeof(float) * CLS_IN * nf, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(dL);
+    if (dF) (void)hipFree(dF);
+    if (e != hipSuccess) return fail(HTX_EDEVICE, std::string("classify: ") + hipGetErrorString(e));
     return HTX_OK;
 }
 

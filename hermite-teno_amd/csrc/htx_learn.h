@@ -1,0 +1,264 @@
+// htx_learn.h -- the learned troubled-face classifier: face features, the
+// TENO cut-off label it is trained on, the MLP's parameter layout and
+// activation, and the MFMA inference of one wavefront's 64 faces.
+//
+// A face sits between cells c and c+1 along direction n (x or y); its
+// stencil is the six cells c-2..c+3 that Schemes::Lagrange::teno5 reads
+// (Schemes_Lagrange.cpp:481-511).  There is no reference counterpart to the
+// classifier itself: it predicts whether plain TENO5 would keep every
+// candidate stencil on both sides of the face for all four characteristic
+// fields (cutOff, Schemes_Lagrange.cpp:1141-1149), in which case TENO5 is the
+// optimal-weight linear scheme (Lagrange::linear, :1219-1233) and the learned
+// path uses that instead.
+#pragma once
+#include "htx_internal.h"
+
+namespace HTX_NS {
+
+constexpr int CLS_IN = 12;
+constexpr int CLS_H = 64;
+// FP32 parameters, logical layout: W1[CLS_IN][CLS_H] (row = input), b1[CLS_H],
+// W2[CLS_H][CLS_H] (row = input unit), b2[CLS_H], w3[CLS_H], b3
+constexpr int CLS_W1 = 0;
+constexpr int CLS_B1 = CLS_W1 + CLS_IN * CLS_H;
+constexpr int CLS_W2 = CLS_B1 + CLS_H;
+constexpr int CLS_B2 = CLS_W2 + CLS_H * CLS_H;
+constexpr int CLS_W3 = CLS_B2 + CLS_H;
+constexpr int CLS_B3 = CLS_W3 + CLS_H;
+constexpr int CLS_NPARAM = CLS_B3 + 1;  // 5057
+
+// GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))
+// = x * sigmoid(2 sqrt(2/pi) (x + 0.044715 x^3)).
+constexpr float GELU_A = 1.5957691216057308f;   // 2 sqrt(2/pi)
+constexpr float GELU_B = 0.07135481627466469f;  // 2 sqrt(2/pi) * 0.044715
+
+HTX_HD float gelu_f(float x) {
+    float t = x * (GELU_A + GELU_B * x * x);
+    return x / (1.0f + expf(-t));
+}
+
+// (gelu(x), gelu'(x))
+HTX_HD void gelu_fd(float x, float& g, float& dg) {
+    float t = x * (GELU_A + GELU_B * x * x);
+    float s = 1.0f / (1.0f + expf(-t));
+    g = x * s;
+    dg = s + x * s * (1.0f - s) * (GELU_A + 3.0f * GELU_B * x * x);
+}
+
+// TENO contrast of one 5-value window: with TENO's smoothness measures
+// beta_k of the three 3-cell candidates and tau = |beta5 - (beta0 + beta2 +
+// 4 beta1) / 6|, the weights (1 + tau / (beta_k + eps))^6 of the best and the
+// worst candidate differ by the ratio R = (1 + tau/(bmin+eps)) / (1 + tau/
+// (bmax+eps)) >= 1.  Returns 1 - 1/R in [0, 1): 0 on a polynomial of degree
+// <= 2 (tau = 0), near 1 across a jump.  TENO5 cuts a candidate roughly when
+// R^6 exceeds 1 / (3 C_T), i.e. for 1 - 1/R above about 0.66.  FP32, on
+// differences formed in FP64 (beta5 in the difference form of teno5_side_mixed).
+HTX_HD float teno_contrast(double v0, double v1, double v2, double v3, double v4, float eps) {
+    const float c1 = 13.0f / 12.0f;
+    float c_0 = (float)(v2 - 2.0 * v3 + v4), b_0 = (float)((3.0 * v2 - 4.0 * v3 + v4) * 0.5);
+    float c_1 = (float)(v1 - 2.0 * v2 + v3), b_1 = (float)((v1 - v3) * 0.5);
+    float c_2 = (float)(v0 - 2.0 * v1 + v2), b_2 = (float)((v0 - 4.0 * v1 + 3.0 * v2) * 0.5);
+    float s0 = b_0 * b_0 + c1 * c_0 * c_0;
+    float s1 = b_1 * b_1 + c1 * c_1 * c_1;
+    float s2 = b_2 * b_2 + c1 * c_2 * c_2;
+    float e1 = (float)(v1 - v0), e2 = (float)(v2 - v1), e3 = (float)(v3 - v2), e4 = (float)(v4 - v3);
+    const float m11 = 1727.0f / 1260.0f, m22 = 2029.0f / 168.0f, m12 = -2479.0f / 672.0f;
+    const float m13 = 5123.0f / 1680.0f, m14 = -8209.0f / 10080.0f, m23 = -12157.0f / 1120.0f;
+    float t1 = m11 * e1 + m12 * e2 + m13 * e3 + m14 * e4;
+    float t2 = m12 * e1 + m22 * e2 + m23 * e3 + m13 * e4;
+    float t3 = m13 * e1 + m23 * e2 + m22 * e3 + m12 * e4;
+    float t4 = m14 * e1 + m13 * e2 + m12 * e3 + m11 * e4;
+    float b5 = e1 * t1 + e2 * t2 + e3 * t3 + e4 * t4;
+    float tau = fabsf(b5 - (s0 + s2 + 4.0f * s1) * (1.0f / 6.0f));
+    float lo = fminf(fminf(s0, s1), s2) + eps, hi = fmaxf(fmaxf(s0, s1), s2) + eps;
+    // 1 - 1/R = 1 - lo (hi + tau) / ((lo + tau) hi)
+    float r = 1.0f - (lo * (hi + tau)) / ((lo + tau) * hi);
+    return r > 0.0f ? r : 0.0f;
+}
+
+// Contrast of a 6-cell stencil: the larger of its two 5-cell windows (the
+// windows of u^- and u^+ at the face).
+HTX_HD float stencil_contrast(const double f[6], float eps) {
+    return fmaxf(teno_contrast(f[0], f[1], f[2], f[3], f[4], eps), teno_contrast(f[1], f[2], f[3], f[4], f[5], eps));
+}
+
+// The 12 classifier inputs of a face from the primitive values of its
+// stencil cells c-2..c+3 (r = density, un / ut = velocity normal / tangential
+// to the face, p = pressure):
+//   x[0..4]  density differences r_{k+1} - r_k over (sum |differences| + 1e-6 r_face)
+//   x[5..9]  the same for the pressure
+//   x[10]    max of the TENO contrasts of r and p (eps = (1e-6 scale)^2)
+//   x[11]    max of the TENO contrasts of un and ut, velocity scale = the face's sound speed
+HTX_HD void face_features(const double r[6], const double un[6], const double ut[6], const double p[6],
+                          double gam, float x[CLS_IN]) {
+    double rf = 0.5 * (fabs(r[2]) + fabs(r[3])), pf = 0.5 * (fabs(p[2]) + fabs(p[3]));
+    double dr[5], dp[5], sr = 0.0, sp = 0.0;
+    for (int k = 0; k < 5; ++k) {
+        dr[k] = r[k + 1] - r[k];
+        dp[k] = p[k + 1] - p[k];
+        sr += fabs(dr[k]);
+        sp += fabs(dp[k]);
+    }
+    double nr = 1.0 / (sr + 1e-6 * rf + 1e-300), np = 1.0 / (sp + 1e-6 * pf + 1e-300);
+    for (int k = 0; k < 5; ++k) {
+        x[k] = (float)(dr[k] * nr);
+        x[5 + k] = (float)(dp[k] * np);
+    }
+    float er = (float)(1e-12 * rf * rf) + 1e-30f, ep = (float)(1e-12 * pf * pf) + 1e-30f;
+    float cf2 = (float)(gam * pf / (rf + 1e-300));
+    float ev = 1e-12f * cf2 + 1e-30f;
+    x[10] = fmaxf(stencil_contrast(r, er), stencil_contrast(p, ep));
+    x[11] = fmaxf(stencil_contrast(un, ev), stencil_contrast(ut, ev));
+}
+
+// Primitive (r, un, ut, p) of a conserved state (rho, m_n, m_t, E).
+HTX_HD void prim_nt(double g1, double q0, double qn, double qt, double q3, double& r, double& un, double& ut,
+                    double& p) {
+    r = q0;
+    un = qn / q0;
+    ut = qt / q0;
+    p = (q3 - 0.5 * (qn * un + qt * ut)) * g1;
+}
+
+// Training label of a face: 1 when plain FP64 TENO5 and the linear scheme give
+// the same face values (|teno - linear| <= 1e-13 max|u| + 1e-9 sum|u_{k+1} - u_k|
+// over the stencil) on both sides for all four characteristic fields, else 0.
+// That holds when TENO5 keeps every candidate, and also where it cuts one only
+// because rounding left a smoothness measure of an exactly flat stencil a few
+// ulps above zero (all candidates then coincide).  c[f][m] = characteristic
+// values of field f on the six stencil cells (FV_System2D.cpp:372-400).
+HTX_HD int teno_label(const double c[4][6]) {
+    for (int f = 0; f < 4; ++f) {
+        double tm, tp, lm, lp, mx = 0.0, tv = 0.0;
+        teno5(c[f], tm, tp);
+        linear5(c[f], lm, lp);
+        for (int m = 0; m < 6; ++m) mx = fmax(mx, fabs(c[f][m]));
+        for (int m = 0; m < 5; ++m) tv += fabs(c[f][m + 1] - c[f][m]);
+        double tol = 1e-13 * mx + 1e-9 * tv;
+        if (!(fabs(tm - lm) <= tol && fabs(tp - lp) <= tol)) return 0;
+    }
+    return 1;
+}
+
+// ---------------------------------------------------------------------------
+// MFMA inference.  The bf16 weights are packed on the host into per-lane
+// fragments of the two matrix layers, computed in transposed form
+// (Z^T = W^T X^T) so that a layer's output fragment is the next layer's B
+// operand without any exchange between lanes:
+//   layer 1, v_mfma_f32_16x16x16_bf16, per 16-unit block mb = 0..3:
+//     A lane (i = lane % 16, g = lane / 16) holds W1[4g + c][16 mb + i], c = 0..3
+//     (zero for inputs >= 12); B lane holds features 4g..4g+3 of face i;
+//     D lane holds units 16 mb + 4g + c of face i
+//   layer 2, v_mfma_f32_16x16x32_bf16, per (mb, kb = 0..1):
+//     K slot 8g + c' of block kb is hidden unit 16 (2 kb + c'/4) + 4g + c'%4,
+//     which is where layer 1's outputs of blocks 2kb and 2kb+1 sit, so
+//     A lane holds W2[that unit][16 mb + i] for c' = 0..7
+//   layer 3 (64 -> 1) on the vector ALU, summed over the four lane groups.
+// Fragment buffer: frag1 (4 x 64 x 4 bf16) then frag2 (4 x 2 x 64 x 8 bf16);
+// cvec = b1[64], b2[64], w3[64], b3 in FP32.
+constexpr int CLS_FRAG1 = 4 * 64 * 4;
+constexpr int CLS_FRAG2 = 4 * 2 * 64 * 8;
+constexpr int CLS_NFRAG = CLS_FRAG1 + CLS_FRAG2;  // bf16 values
+constexpr int CLS_NCVEC = 3 * CLS_H + 1;
+
+// logical hidden unit of layer 2's K slot (kb, g, c')
+HTX_HD int cls_kslot_unit(int kb, int g, int cp) { return 16 * (2 * kb + cp / 4) + 4 * g + (cp % 4); }
+
+// bf16 round-to-nearest-even of an FP32 value (NaN kept quiet)
+HTX_HD unsigned short f32_to_bf16(float f) {
+    unsigned int u;
+    __builtin_memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+HTX_HD float bf16_to_f32(unsigned short h) {
+    unsigned int u = (unsigned int)h << 16;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+// Host-side packing of the FP32 parameters into MFMA fragments.
+inline void cls_pack(const float* prm, unsigned short* frag, float* cvec) {
+    for (int mb = 0; mb < 4; ++mb)
+        for (int lane = 0; lane < 64; ++lane) {
+            int i = lane % 16, g = lane / 16;
+            for (int c = 0; c < 4; ++c) {
+                int k = 4 * g + c;
+                float w = k < CLS_IN ? prm[CLS_W1 + k * CLS_H + 16 * mb + i] : 0.0f;
+                frag[(mb * 64 + lane) * 4 + c] = f32_to_bf16(w);
+            }
+        }
+    unsigned short* f2 = frag + CLS_FRAG1;
+    for (int mb = 0; mb < 4; ++mb)
+        for (int kb = 0; kb < 2; ++kb)
+            for (int lane = 0; lane < 64; ++lane) {
+                int i = lane % 16, g = lane / 16;
+                for (int cp = 0; cp < 8; ++cp) {
+                    int k = cls_kslot_unit(kb, g, cp);
+                    f2[((mb * 2 + kb) * 64 + lane) * 8 + cp] = f32_to_bf16(prm[CLS_W2 + k * CLS_H + 16 * mb + i]);
+                }
+            }
+    for (int u = 0; u < CLS_H; ++u) {
+        cvec[u] = prm[CLS_B1 + u];
+        cvec[CLS_H + u] = prm[CLS_B2 + u];
+        cvec[2 * CLS_H + u] = prm[CLS_W3 + u];
+    }
+    cvec[3 * CLS_H] = prm[CLS_B3];
+}
+
+typedef short cls_s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 cls_b8 __attribute__((ext_vector_type(8)));
+typedef float cls_f4 __attribute__((ext_vector_type(4)));
+
+// Logit of the calling lane's face for the 64 faces of one wavefront.
+// xs: this wave's LDS scratch of 64 x 16 bf16 (2 KB); every lane of the
+// wave must call (faces without data pass zeros).  The caller orders the
+// scratch writes before the reads (a barrier between cls_stage_features and
+// cls_logit).
+__device__ __forceinline__ void cls_stage_features(unsigned short* xs, const float x[CLS_IN]) {
+    const int lane = threadIdx.x & 63;
+    for (int k = 0; k < 16; ++k) xs[lane * 16 + k] = k < CLS_IN ? f32_to_bf16(x[k]) : (unsigned short)0;
+}
+
+__device__ __forceinline__ float cls_logit(const unsigned short* xs, const unsigned short* frag, const float* cvec) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
+    const cls_s4* F1 = reinterpret_cast<const cls_s4*>(frag);
+    const cls_b8* F2 = reinterpret_cast<const cls_b8*>(frag + CLS_FRAG1);
+    float zmine = 0.0f;
+    for (int nb = 0; nb < 4; ++nb) {
+        cls_s4 bx = *reinterpret_cast<const cls_s4*>(xs + (16 * nb + i) * 16 + 4 * g);
+        cls_b8 b2[2];
+        for (int mb = 0; mb < 4; ++mb) {
+            cls_f4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F1[mb * 64 + lane], bx, d, 0, 0, 0);
+            for (int c = 0; c < 4; ++c) {
+                float h = gelu_f(d[c] + cvec[16 * mb + 4 * g + c]);
+                b2[mb >> 1][(mb & 1) * 4 + c] = (__bf16)h;
+            }
+        }
+        float part = 0.0f;
+        for (int mb = 0; mb < 4; ++mb) {
+            cls_f4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F2[(mb * 2 + 0) * 64 + lane], b2[0], d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F2[(mb * 2 + 1) * 64 + lane], b2[1], d, 0, 0, 0);
+            for (int c = 0; c < 4; ++c) {
+                int u = 16 * mb + 4 * g + c;
+                part += cvec[2 * CLS_H + u] * gelu_f(d[c] + cvec[CLS_H + u]);
+            }
+        }
+        part += __shfl_xor(part, 16, 64);
+        part += __shfl_xor(part, 32, 64);
+        if (nb == g) zmine = part + cvec[3 * CLS_H];
+    }
+    return zmine;
+#else
+    return 0.0f;
+#endif
+}
+
+
+}  // namespace HTX_NS

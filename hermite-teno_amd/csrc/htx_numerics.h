@@ -209,6 +209,21 @@ HTX_HD void teno5(const double u[6], double& um, double& up, unsigned* cut = nul
     if (cut) *cut = cm | (cp << 3);
 }
 
+// Schemes::Lagrange::linear (Schemes_Lagrange.cpp:1219-1233): the same three
+// candidates combined with the optimal weights (0.3, 0.6, 0.1) / mirrored.
+HTX_HD void linear5(const double u[6], double& um, double& up) {
+    HTX_NO_CONTRACT
+    const double a = 1.0 / 3.0, b = 5.0 / 6.0, c = -1.0 / 6.0, d = 11.0 / 6.0, e = -7.0 / 6.0;
+    double p0 = (a * u[2] + b * u[3]) + c * u[4];
+    double p1 = (c * u[1] + b * u[2]) + a * u[3];
+    double p2 = (a * u[0] + e * u[1]) + d * u[2];
+    um = (p0 * 0.3 + p1 * 0.6) + p2 * 0.1;
+    double q0 = (d * u[3] + e * u[4]) + a * u[5];
+    double q1 = (a * u[2] + b * u[3]) + c * u[4];
+    double q2 = (c * u[1] + b * u[2]) + a * u[3];
+    up = (q0 * 0.1 + q1 * 0.6) + q2 * 0.3;
+}
+
 // TENO5 with the FP32 cut-off rule of teno5_side_mixed.
 HTX_HD void teno5_mixed(const double u[6], double& um, double& up) {
     const double a = 1.0 / 3.0, b = 5.0 / 6.0, c = -1.0 / 6.0, d = 11.0 / 6.0, e = -7.0 / 6.0;

@@ -56,6 +56,7 @@ _I = ctypes.c_int
 _S = ctypes.c_size_t
 _DP = ctypes.POINTER(ctypes.c_double)
 _IP = ctypes.POINTER(ctypes.c_int)
+_FP = ctypes.POINTER(ctypes.c_float)
 
 # name -> (restype, argtypes); every symbol include/htx.h declares
 SIGNATURES = {
@@ -76,6 +77,19 @@ SIGNATURES = {
     "htx_sync": (_I, [_P]),
     "htx_timing_enable": (_I, [_P, _I]),
     "htx_timing_read": (_I, [_P, _DP, _IP]),
+    "htx_trainer_create": (_I, [ctypes.c_longlong, ctypes.POINTER(_P)]),
+    "htx_trainer_destroy": (None, [_P]),
+    "htx_trainer_add_state": (_I, [_P, _P]),
+    "htx_trainer_size": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
+    "htx_trainer_read": (_I, [_P, ctypes.c_longlong, ctypes.c_longlong, _FP, ctypes.POINTER(ctypes.c_ubyte)]),
+    "htx_trainer_init": (_I, [_P, ctypes.c_ulonglong]),
+    "htx_trainer_set_params": (_I, [_P, _FP]),
+    "htx_trainer_get_params": (_I, [_P, _FP]),
+    "htx_trainer_train": (_I, [_P, _I, _I, _D, _D, ctypes.c_ulonglong, _DP]),
+    "htx_trainer_eval": (_I, [_P, _D, ctypes.POINTER(ctypes.c_longlong), _FP]),
+    "htx_trainer_gradient": (_I, [_P, _D, _FP, _DP]),
+    "htx_load_classifier": (_I, [_P, _FP, _D]),
+    "htx_classify": (_I, [_P, _I, _FP, _FP]),
     "htx_last_error": (ctypes.c_char_p, []),
     "htx_abi_version": (_I, []),
 }
@@ -109,3 +123,11 @@ def check(rc):
 def dptr(a):
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_DP)
+
+
+def fptr(a):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_FP)
+
+
+CLS_NPARAM = 5057

@@ -22,7 +22,7 @@ LIBHTX = os.path.join(PKG_DIR, "libhtx.so")
 LIBORACLE = os.path.join(ORACLE, "liboracle.so")
 LIBORACLE_OMP = os.path.join(ORACLE, "liboracle_omp.so")
 
-HIP_SOURCES = ["htx_fv2d.hip", "htx_fvh2d.hip", "htx_strict.hip", "htx_ic.hip", "htx_api.hip"]
+HIP_SOURCES = ["htx_fv2d.hip", "htx_fvh2d.hip", "htx_strict.hip", "htx_ic.hip", "htx_learn.hip", "htx_api.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -52,7 +52,8 @@ def _run(cmd, cwd=None):
 
 
 # per-file flags: the strict build of the kernels turns contraction off
-PER_FILE_FLAGS = {"htx_strict.hip": ["-ffp-contract=off"], "htx_ic.hip": ["-ffp-contract=off"]}
+PER_FILE_FLAGS = {"htx_strict.hip": ["-ffp-contract=off"], "htx_ic.hip": ["-ffp-contract=off"],
+                  "htx_learn.hip": ["-ffp-contract=off"]}
 
 
 def build_htx(extra=()):

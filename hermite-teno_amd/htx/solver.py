@@ -118,6 +118,23 @@ class System2D:
         else:
             self.shock_blobs(w["xs"], w["ys"], w["nrm"], w["post"], w["ambient"], w["blobs"])
 
+    # -- learned classifier ----------------------------------------------
+    def load_classifier(self, params, threshold=0.0):
+        """Classifier parameters (htx.learn layout) and logit threshold for
+        precision "learned"."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        check(_lib.load().htx_load_classifier(self._h, _lib.fptr(p), float(threshold)))
+
+    def classify(self, direction, features=False):
+        """Logits of every x (direction 0) or y (1) face of the current state,
+        indexed [j, i] (face of cell (i, j) with its +1 neighbour), from the
+        bf16 matrix-core inference; optionally the 12 features too."""
+        lg = np.empty((self.ny, self.nx), dtype=np.float32)
+        ft = np.empty((self.ny, self.nx, 12), dtype=np.float32) if features else None
+        check(_lib.load().htx_classify(self._h, int(direction), _lib.fptr(lg),
+                                       _lib.fptr(ft) if ft is not None else None))
+        return (lg, ft) if features else lg
+
     # -- the hot path -----------------------------------------------------
     def rhs(self):
         """One residual evaluation L(q) (FV::System2D::rhs)."""

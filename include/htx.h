@@ -148,6 +148,73 @@ int htx_sync(htx_solver* h);
 int htx_timing_enable(htx_solver* h, int enable);
 int htx_timing_read(htx_solver* h, double* kernel_ms, int* nlaunch);
 
+/* ------------------------------------------------------------------------
+ * Learned troubled-face classifier (no reference counterpart).
+ *
+ * A face's label is 1 when plain FP64 TENO5 keeps every candidate stencil on
+ * both sides of the face for all four characteristic fields (cutOff,
+ * Schemes_Lagrange.cpp:1141-1149), i.e. when TENO5 reduces to the
+ * optimal-weight linear scheme there.  The classifier is an MLP
+ * 12 -> 64 -> 64 -> 1 with GELU (tanh form) whose inputs are normalised
+ * density and pressure differences across the face's 6-cell stencil and TENO
+ * smoothness contrasts of density, pressure and velocity.  Parameters are
+ * HTX_CLS_NPARAM FP32 values: W1[12][64], b1[64], W2[64][64], b2[64],
+ * w3[64], b3 (row index = input).  Training runs on the GPU in FP32; the
+ * solver's inference runs the weights in bf16 on the matrix cores.
+ * ------------------------------------------------------------------------ */
+#define HTX_CLS_NPARAM 5057
+
+typedef struct htx_trainer htx_trainer;
+
+/* A training set of up to capacity faces (<= 2^31) on the device. */
+int htx_trainer_create(long long capacity, htx_trainer** out);
+void htx_trainer_destroy(htx_trainer* t);
+
+/* Appends the features and labels of every x face and every y face of the
+ * solver's current state (2 nx ny faces; TENO5 scheme). */
+int htx_trainer_add_state(htx_trainer* t, htx_solver* h);
+
+/* Faces in the set and how many of them are labelled smooth. */
+int htx_trainer_size(htx_trainer* t, long long* n, long long* n_smooth);
+
+/* Copies faces [start, start + n): features (12 floats each) and labels;
+ * either pointer may be null. */
+int htx_trainer_read(htx_trainer* t, long long start, long long n, float* feat, unsigned char* label);
+
+/* Random initial parameters (uniform Xavier weights from seed, zero biases)
+ * and a fresh Adam state. */
+int htx_trainer_init(htx_trainer* t, unsigned long long seed);
+int htx_trainer_set_params(htx_trainer* t, const float* params);
+int htx_trainer_get_params(htx_trainer* t, float* params);
+
+/* Adam on the class-balanced logistic loss, troubled faces weighted by
+ * troubled_weight; each epoch visits the set once in a seeded order, in
+ * minibatches of about batch faces (>= 128).  epoch_loss (may be null)
+ * receives the mean minibatch loss of each epoch. */
+int htx_trainer_train(htx_trainer* t, int epochs, int batch, double lr, double troubled_weight,
+                      unsigned long long seed, double* epoch_loss);
+
+/* Gradient of the training loss over the whole set at the current
+ * parameters (one pass, no update) and the loss itself. */
+int htx_trainer_gradient(htx_trainer* t, double troubled_weight, float* grad, double* loss);
+
+/* FP32 forward of every face of the set: counts[4] = smooth faces predicted
+ * smooth, troubled faces predicted smooth, troubled predicted troubled,
+ * smooth predicted troubled, a face being predicted smooth when its logit
+ * exceeds threshold; logits (may be null) receives every logit. */
+int htx_trainer_eval(htx_trainer* t, double threshold, long long* counts, float* logits);
+
+/* Loads classifier parameters into a solver (packed to bf16); a face whose
+ * logit exceeds threshold takes the linear scheme in precision
+ * HTX_PREC_LEARNED. */
+int htx_load_classifier(htx_solver* h, const float* params, double threshold);
+
+/* Logits (and, if feat is not null, the 12 features) of every face of one
+ * family of the current state, dir 0 = x faces (cell (i, j) | (i+1, j)),
+ * dir 1 = y faces, indexed j * nx + i, computed with the solver's bf16
+ * matrix-core inference. */
+int htx_classify(htx_solver* h, int dir, float* logits, float* feat);
+
 /* Message of the last failed call on this host thread. */
 const char* htx_last_error(void);
 

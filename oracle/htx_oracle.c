@@ -662,6 +662,47 @@ void orc_fv2d_rhs(const orc_params* p, const double* q, double* rhs) {
 }
 
 /* CFL reduction: FV::System2D::timeStepping (FV_System2D.cpp:623-631) */
+/* What the cut-off decisions of FV::System2D::rhs's reconstruction
+ * (FV_System2D.cpp:372-400) do to every face of one family: dir 1 = the x
+ * face between interior cells (i, j) and (i+1, j), dir 2 = the y face
+ * between (i, j) and (i, j+1), labels indexed j * nx + i; label 1 when, for
+ * every characteristic field and both sides, teno5 (Schemes_Lagrange.cpp
+ * :481-511) gives the value of linear (:1219-1233) to within
+ * 1e-13 max|u| + 1e-9 sum|u_{k+1} - u_k| of the 6-value stencil.  (The
+ * learned classifier's training target; the reference has no such label.) */
+void orc_face_labels(const orc_params* p, const double* q, int dir, unsigned char* labels) {
+    lag_init();
+    padded P;
+    int nx = p->nx, ny = p->ny, k = FV_K;
+    pad_alloc(&P, nx, ny, k, 4);
+    load_padded(&P, q, 4, p->bc);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            int pi = i + k, pj = j + k;
+            double qa[4], qb[4], L[16], R[16], cc[4], y[4], C[4][6];
+            cell_q(&P, pi, pj, qa);
+            if (dir == 1) cell_q(&P, pi + 1, pj, qb); else cell_q(&P, pi, pj + 1, qb);
+            orc_eigen(p->gamma, dir, qa, qb, L, R);
+            for (int m = 0; m < 6; ++m) {
+                if (dir == 1) cell_q(&P, pi - 2 + m, pj, cc); else cell_q(&P, pi, pj - 2 + m, cc);
+                matvec4(L, cc, y);
+                for (int e = 0; e < 4; ++e) C[e][m] = y[e];
+            }
+            int all = 1;
+            for (int e = 0; e < 4; ++e) {
+                double tv[2], lv[2], mx = 0.0, var = 0.0;
+                teno5_impl(C[e], tv, NULL);
+                orc_linear5(C[e], lv);
+                for (int m = 0; m < 6; ++m) mx = fmax(mx, fabs(C[e][m]));
+                for (int m = 0; m < 5; ++m) var += fabs(C[e][m + 1] - C[e][m]);
+                double tol = 1e-13 * mx + 1e-9 * var;
+                if (!(fabs(tv[0] - lv[0]) <= tol && fabs(tv[1] - lv[1]) <= tol)) all = 0;
+            }
+            labels[(size_t)j * nx + i] = (unsigned char)all;
+        }
+    pad_free(&P, 4);
+}
+
 void orc_fv2d_max_speed(const orc_params* p, const double* q, double* sx, double* sy) {
     size_t n = (size_t)p->nx * p->ny;
     double mx = 0.0, my = 0.0;

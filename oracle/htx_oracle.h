@@ -50,6 +50,8 @@ void orc_gauss9(double nodes[9], double weights[9]);
 
 /* ---- 2D FV (TENO5 + 3-point transverse quadrature) ---- */
 void orc_fv2d_rhs(const orc_params* p, const double* q, double* rhs);
+/* TENO5 cut-off decisions per face (label 1 = nothing cut), dir 1 x, 2 y */
+void orc_face_labels(const orc_params* p, const double* q, int dir, unsigned char* labels);
 void orc_fv2d_max_speed(const orc_params* p, const double* q, double* sx, double* sy);
 double orc_fv2d_dt(const orc_params* p, const double* q, double cfl, double t, double tfinal);
 void orc_fv2d_rk3(const orc_params* p, double* q, double dt);

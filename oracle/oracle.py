@@ -43,6 +43,7 @@ _SIG = {
     "orc_gauss9": (None, [_DP, _DP]),
     "orc_fv2d_rhs": (None, [ctypes.POINTER(OrcParams), _DP, _DP]),
     "orc_fv2d_max_speed": (None, [ctypes.POINTER(OrcParams), _DP, _DP, _DP]),
+    "orc_face_labels": (None, [ctypes.POINTER(OrcParams), _DP, _I, ctypes.POINTER(ctypes.c_ubyte)]),
     "orc_fv2d_dt": (_D, [ctypes.POINTER(OrcParams), _DP, _D, _D, _D]),
     "orc_fv2d_rk3": (None, [ctypes.POINTER(OrcParams), _DP, _D]),
     "orc_fv2d_advance": (_I, [ctypes.POINTER(OrcParams), _DP, _D, _D, _D, _I, _DP]),
@@ -180,6 +181,16 @@ class Oracle:
         r = np.empty_like(q)
         self.lib.orc_fv2d_rhs(ctypes.byref(prm), pq, r.ctypes.data_as(_DP))
         return r
+
+    def face_labels(self, prm, q, direction):
+        """TENO5 cut-off label (1 = nothing cut) of every x (direction 1) or
+        y (direction 2) face, indexed [j, i]: the face of cell (i, j) with
+        its +1 neighbour."""
+        q, pq = _p(q)
+        out = np.empty((prm.ny, prm.nx), dtype=np.uint8)
+        self.lib.orc_face_labels(ctypes.byref(prm), pq, int(direction),
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+        return out
 
     def fv2d_max_speed(self, prm, q):
         q, pq = _p(q)

@@ -1,0 +1,171 @@
+"""numpy restatement of the learned classifier's definitions (csrc/htx_learn.h)
+used as the checker of the GPU kernels: face features (FP32 arithmetic in the
+device's operation order), the MLP forward pass in FP64 and with the bf16
+rounding of the matrix-core inference, and the training loss gradient."""
+import numpy as np
+
+F = np.float32
+CLS_IN, CLS_H = 12, 64
+GELU_A = F(1.5957691216057308)
+GELU_B = F(0.07135481627466469)
+
+
+def map_index(g, n, bc):
+    if bc == "periodic":
+        return np.mod(g, n)
+    return np.clip(g, 0, n - 1)
+
+
+def stencils(q, bc, direction):
+    """Conserved state of the six stencil cells of every face: shape
+    (6, 4, ny, nx); direction 0 = x faces (cell (i, j) | (i+1, j))."""
+    nf, ny, nx = q.shape
+    J, I = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    out = np.empty((6, 4, ny, nx))
+    for m in range(6):
+        if direction == 0:
+            gi, gj = map_index(I - 2 + m, nx, bc), J
+        else:
+            gi, gj = I, map_index(J - 2 + m, ny, bc)
+        out[m] = q[:4, gj, gi]
+    return out
+
+
+def teno_contrast(v0, v1, v2, v3, v4, eps):
+    c1 = F(13.0) / F(12.0)
+    c_0, b_0 = F32(v2 - 2.0 * v3 + v4), F32((3.0 * v2 - 4.0 * v3 + v4) * 0.5)
+    c_1, b_1 = F32(v1 - 2.0 * v2 + v3), F32((v1 - v3) * 0.5)
+    c_2, b_2 = F32(v0 - 2.0 * v1 + v2), F32((v0 - 4.0 * v1 + 3.0 * v2) * 0.5)
+    s0 = b_0 * b_0 + c1 * c_0 * c_0
+    s1 = b_1 * b_1 + c1 * c_1 * c_1
+    s2 = b_2 * b_2 + c1 * c_2 * c_2
+    e1, e2, e3, e4 = F32(v1 - v0), F32(v2 - v1), F32(v3 - v2), F32(v4 - v3)
+    m11, m22, m12 = F(1727.0) / F(1260.0), F(2029.0) / F(168.0), F(-2479.0) / F(672.0)
+    m13, m14, m23 = F(5123.0) / F(1680.0), F(-8209.0) / F(10080.0), F(-12157.0) / F(1120.0)
+    t1 = m11 * e1 + m12 * e2 + m13 * e3 + m14 * e4
+    t2 = m12 * e1 + m22 * e2 + m23 * e3 + m13 * e4
+    t3 = m13 * e1 + m23 * e2 + m22 * e3 + m12 * e4
+    t4 = m14 * e1 + m13 * e2 + m12 * e3 + m11 * e4
+    b5 = e1 * t1 + e2 * t2 + e3 * t3 + e4 * t4
+    tau = np.abs(b5 - (s0 + s2 + F(4.0) * s1) * (F(1.0) / F(6.0)))
+    lo = np.minimum(np.minimum(s0, s1), s2) + eps
+    hi = np.maximum(np.maximum(s0, s1), s2) + eps
+    r = F(1.0) - (lo * (hi + tau)) / ((lo + tau) * hi)
+    return np.where(r > 0, r, F(0.0))
+
+
+def F32(a):
+    return np.asarray(a, dtype=np.float64).astype(np.float32)
+
+
+def stencil_contrast(f, eps):
+    return np.maximum(teno_contrast(f[0], f[1], f[2], f[3], f[4], eps),
+                      teno_contrast(f[1], f[2], f[3], f[4], f[5], eps))
+
+
+def features(q, gamma, bc, direction):
+    """The 12 inputs of every face of one family, shape (ny, nx, 12)."""
+    st = stencils(q, bc, direction)
+    g1 = gamma - 1.0
+    r = st[:, 0]
+    qn, qt = (st[:, 1], st[:, 2]) if direction == 0 else (st[:, 2], st[:, 1])
+    un, ut = qn / r, qt / r
+    p = (st[:, 3] - 0.5 * (qn * un + qt * ut)) * g1
+    rf = 0.5 * (np.abs(r[2]) + np.abs(r[3]))
+    pf = 0.5 * (np.abs(p[2]) + np.abs(p[3]))
+    dr = [r[k + 1] - r[k] for k in range(5)]
+    dp = [p[k + 1] - p[k] for k in range(5)]
+    sr = np.zeros_like(rf)
+    sp = np.zeros_like(pf)
+    for k in range(5):
+        sr = sr + np.abs(dr[k])
+        sp = sp + np.abs(dp[k])
+    nr = 1.0 / (sr + 1e-6 * rf + 1e-300)
+    npp = 1.0 / (sp + 1e-6 * pf + 1e-300)
+    x = np.empty(rf.shape + (12,), dtype=np.float32)
+    for k in range(5):
+        x[..., k] = F32(dr[k] * nr)
+        x[..., 5 + k] = F32(dp[k] * npp)
+    er = F32(1e-12 * rf * rf) + F(1e-30)
+    ep = F32(1e-12 * pf * pf) + F(1e-30)
+    cf2 = F32(gamma * pf / (rf + 1e-300))
+    ev = F(1e-12) * cf2 + F(1e-30)
+    x[..., 10] = np.maximum(stencil_contrast(r, er), stencil_contrast(p, ep))
+    x[..., 11] = np.maximum(stencil_contrast(un, ev), stencil_contrast(ut, ev))
+    return x
+
+
+def unpack(params):
+    p = np.asarray(params, dtype=np.float64)
+    o = 0
+    W1 = p[o:o + 768].reshape(12, 64); o += 768
+    b1 = p[o:o + 64]; o += 64
+    W2 = p[o:o + 4096].reshape(64, 64); o += 4096
+    b2 = p[o:o + 64]; o += 64
+    w3 = p[o:o + 64]; o += 64
+    return W1, b1, W2, b2, w3, p[o]
+
+
+def gelu(x):
+    return x / (1.0 + np.exp(-(x * (float(GELU_A) + float(GELU_B) * x * x))))
+
+
+def gelu_d(x):
+    t = x * (float(GELU_A) + float(GELU_B) * x * x)
+    s = 1.0 / (1.0 + np.exp(-t))
+    return s + x * s * (1.0 - s) * (float(GELU_A) + 3.0 * float(GELU_B) * x * x)
+
+
+def forward(params, X):
+    """FP64 logits of feature rows X (n, 12)."""
+    W1, b1, W2, b2, w3, b3 = unpack(params)
+    h1 = gelu(X.astype(np.float64) @ W1 + b1)
+    h2 = gelu(h1 @ W2 + b2)
+    return h2 @ w3 + b3
+
+
+def bf16(a):
+    """Round to bf16 (nearest even) and return as float64."""
+    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward_bf16(params, X):
+    """Logits as the matrix-core inference forms them: weights, features and
+    the first hidden layer rounded to bf16, sums in high precision, biases,
+    activations and the output layer in FP32-range arithmetic."""
+    W1, b1, W2, b2, w3, b3 = unpack(params)
+    h1 = gelu(bf16(X) @ bf16(W1) + b1)
+    h2 = gelu(bf16(h1) @ bf16(W2) + b2)
+    return h2 @ w3 + b3
+
+
+def loss_grad(params, X, y, troubled_weight):
+    """Class-balanced weighted logistic loss over the whole set and its
+    gradient (FP64), with the trainer's weighting."""
+    W1, b1, W2, b2, w3, b3 = unpack(params)
+    n = len(y)
+    fpos = y.mean()
+    fneg = 1.0 - fpos
+    wpos, wneg = 0.5 / fpos, 0.5 / fneg * troubled_weight
+    norm = 1.0 / (fpos * wpos + fneg * wneg) / n
+    w = np.where(y > 0, wpos, wneg) * norm
+    X = X.astype(np.float64)
+    z1 = X @ W1 + b1
+    h1 = gelu(z1)
+    z2 = h1 @ W2 + b2
+    h2 = gelu(z2)
+    z3 = h2 @ w3 + b3
+    sg = 1.0 / (1.0 + np.exp(-z3))
+    loss = np.sum(w * (np.maximum(z3, 0) + np.log1p(np.exp(-np.abs(z3))) - y * z3))
+    g3 = w * (sg - y)
+    gw3 = h2.T @ g3
+    gb3 = g3.sum()
+    G2 = np.outer(g3, w3) * gelu_d(z2)
+    gW2 = h1.T @ G2
+    gb2 = G2.sum(0)
+    G1 = (G2 @ W2.T) * gelu_d(z1)
+    gW1 = X.T @ G1
+    gb1 = G1.sum(0)
+    return loss, np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gw3, [gb3]])

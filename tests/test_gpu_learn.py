@@ -1,0 +1,138 @@
+"""GPU checks of the learned troubled-face classifier: training-set
+extraction against the oracle's labels and the numpy features, the bf16
+matrix-core inference against a bf16-rounded numpy forward pass, the FP32
+training gradient against numpy, and that training separates the classes."""
+import numpy as np
+import pytest
+
+import learn_ref as L
+from oracle import params
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 1.4
+
+
+def solver_with(q, n, bc, dom=(0.0, 1.0, 0.0, 1.0)):
+    import htx
+
+    s = htx.System2D(dom[0], dom[1], n, dom[2], dom[3], n, 0.5, 10.0, 0.0, GAMMA, bc=bc)
+    s.set_state(q)
+    return s
+
+
+def quadrant_q(n, seed):
+    from htx.ic import SplitMix64, piecewise_state, quadrant_params
+
+    xs, ys, st = quadrant_params(SplitMix64(seed))
+    w = dict(kind="quadrant", xl=0.0, xr=1.0, yb=0.0, yt=1.0, gamma=GAMMA, xs=xs, ys=ys, states=st)
+    return piecewise_state(w, n, n)
+
+
+def vortex_q(orc, n):
+    from htx.ic import vortex_modes
+
+    prm = params(n, n, 0.0, 10.0, 0.0, 10.0, GAMMA, "periodic", "hllc", True)
+    return orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
+
+
+@pytest.mark.parametrize("case", ["vortex", "quadrant"])
+def test_extraction_matches_oracle_labels_and_features(orc, case):
+    from htx.learn import Trainer
+
+    n = 40
+    if case == "vortex":
+        q, bc, dom = vortex_q(orc, n), "periodic", (0.0, 10.0, 0.0, 10.0)
+    else:
+        q, bc, dom = quadrant_q(n, 17), "zero_grad", (0.0, 1.0, 0.0, 1.0)
+    print(f"[case] extraction {case} n={n}", flush=True)
+    s = solver_with(q, n, bc, dom)
+    t = Trainer(4 * n * n)
+    t.add_state(s)
+    nf, ns = t.size()
+    assert nf == 2 * n * n
+    feat, lab = t.read(0, nf)
+    prm = params(n, n, *dom, GAMMA, bc, "hllc", True)
+    for d in range(2):
+        lab_o = orc.face_labels(prm, q, d + 1).ravel()
+        got = lab[d * n * n:(d + 1) * n * n]
+        np.testing.assert_array_equal(got, lab_o)
+        fr = L.features(q, GAMMA, bc, d).reshape(-1, 12)
+        np.testing.assert_allclose(feat[d * n * n:(d + 1) * n * n], fr, rtol=1e-5, atol=2e-6)
+    assert ns == int(lab.sum())
+    if case == "quadrant":
+        assert 0 < ns < nf
+
+
+def test_mfma_inference_matches_bf16_reference(orc):
+    n = 64
+    q = quadrant_q(n, 23)
+    print(f"[case] MFMA classifier inference n={n}", flush=True)
+    s = solver_with(q, n, "zero_grad")
+    rng = np.random.default_rng(11)
+    prm = (rng.normal(0, 0.4, 5057)).astype(np.float32)
+    s.load_classifier(prm, 0.0)
+    for d in range(2):
+        lg, ft = s.classify(d, features=True)
+        fr = L.features(q, GAMMA, "zero_grad", d)
+        np.testing.assert_allclose(ft, fr, rtol=1e-5, atol=2e-6)
+        ref = L.forward_bf16(prm, ft.reshape(-1, 12)).reshape(n, n)
+        err = np.abs(lg - ref)
+        print(f"dir {d}: max |logit - bf16 reference| {err.max():.3e}, median {np.median(err):.3e}, "
+              f"logit scale {np.abs(ref).max():.2f}", flush=True)
+        assert np.median(err) < 1e-3 * (1 + np.abs(ref).max())
+        assert err.max() < 3e-2 * (1 + np.abs(ref).max())
+        # and the FP64 forward pass without bf16 rounding is close too
+        exact = L.forward(prm, ft.reshape(-1, 12)).reshape(n, n)
+        assert np.corrcoef(exact.ravel(), lg.ravel())[0, 1] > 0.999
+
+
+def test_training_gradient_matches_numpy(orc):
+    from htx.learn import Trainer
+
+    n = 32
+    q = quadrant_q(n, 29)
+    print(f"[case] training gradient n={n}", flush=True)
+    s = solver_with(q, n, "zero_grad")
+    t = Trainer(2 * n * n)
+    t.add_state(s)
+    rng = np.random.default_rng(13)
+    prm = (rng.normal(0, 0.3, 5057)).astype(np.float32)
+    t.set_params(prm)
+    g, loss = t.gradient(troubled_weight=3.0)
+    feat, lab = t.read(0, 2 * n * n)
+    loss_r, g_r = L.loss_grad(prm.astype(np.float64), feat, lab.astype(np.float64), 3.0)
+    print(f"loss {loss:.6e} (numpy {loss_r:.6e}); gradient rel. L2 error "
+          f"{np.linalg.norm(g - g_r) / np.linalg.norm(g_r):.3e}", flush=True)
+    assert loss == pytest.approx(loss_r, rel=1e-4)
+    assert np.linalg.norm(g - g_r) <= 1e-4 * np.linalg.norm(g_r)
+
+
+def test_training_separates_classes(orc):
+    import htx
+    from htx.learn import Trainer
+
+    t = Trainer(1 << 20)
+    n = 64
+    print(f"[case] training run, snapshots at n={n}", flush=True)
+    for seed in (31, 37, 41):
+        s = solver_with(quadrant_q(n, seed), n, "zero_grad")
+        t.add_state(s)
+        s.steps_async(8)
+        t.add_state(s)
+    sv = htx.System2D(0.0, 10.0, n, 0.0, 10.0, n, 0.5, 10.0, 0.0, GAMMA, bc="periodic")
+    from htx.ic import vortex_modes
+    sv.isentropicVortex((5.0, 5.0), vortex_modes())
+    t.add_state(sv)
+    nf, ns = t.size()
+    print(f"{nf} faces, {ns} smooth", flush=True)
+    t.init(seed=3)
+    before = t.evaluate(0.0)
+    losses = t.train(epochs=40, batch=4096, lr=3e-3, troubled_weight=2.0, seed=5)
+    after = t.evaluate(0.0)
+    acc = (after["smooth_as_smooth"] + after["troubled_as_troubled"]) / nf
+    print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}; accuracy {acc:.4f}; {after}", flush=True)
+    assert losses[-1] < 0.6 * losses[0]
+    assert acc > 0.9
+    assert after["troubled_as_smooth"] < 0.2 * (nf - ns)
+    assert before != after
