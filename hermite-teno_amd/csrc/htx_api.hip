@@ -99,9 +99,8 @@ int check_params(const htx_params* p) {
     if (p->precision != HTX_PREC_FP64 && p->precision != HTX_PREC_FP32_MIXED && p->precision != HTX_PREC_LEARNED &&
         p->precision != HTX_PREC_FP64_FAST)
         return fail(HTX_EINVAL, "unknown precision");
-    if (p->precision == HTX_PREC_LEARNED) return fail(HTX_ENOTSUP, "the learned precision mode is not built");
-    if (p->precision == HTX_PREC_FP32_MIXED && p->scheme != HTX_SCHEME_TENO5)
-        return fail(HTX_ENOTSUP, "FP32-mixed precision is built for the TENO5 scheme only");
+    if ((p->precision == HTX_PREC_FP32_MIXED || p->precision == HTX_PREC_LEARNED) && p->scheme != HTX_SCHEME_TENO5)
+        return fail(HTX_ENOTSUP, "FP32-mixed and learned precision are built for the TENO5 scheme only");
     return HTX_OK;
 }
 
@@ -118,7 +117,11 @@ Fv2dArgs stage_args(htx_solver* h, const double* qin, const double* q0, double* 
     a.bc = h->p.bc;
     a.mode = mode;
     a.quirks = h->p.ref_quirks ? 1 : 0;
-    a.mixed = h->p.precision == HTX_PREC_FP32_MIXED ? 1 : 0;
+    a.mixed = (h->p.precision == HTX_PREC_FP32_MIXED || h->p.precision == HTX_PREC_LEARNED) ? 1 : 0;
+    a.learned = h->p.precision == HTX_PREC_LEARNED ? 1 : 0;
+    a.cls_frag = h->dFrag;
+    a.cls_cvec = h->dCvec;
+    a.cls_thr = h->cls_threshold;
     a.gam = h->p.gamma;
     a.g1 = h->p.gamma - 1.0;
     a.dx = h->dx;
@@ -164,6 +167,8 @@ FvhArgs hermite_args(htx_solver* h, const double* qin, const double* q0, double*
 }
 
 int launch_stage(htx_solver* h, const double* qin, const double* q0, double* qout, int mode) {
+    if (h->p.precision == HTX_PREC_LEARNED && !h->have_classifier)
+        return fail(HTX_ESTATE, "precision 'learned' needs a classifier (htx_load_classifier)");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (h->timing) {
         if (!h->event_pool.empty()) {
@@ -873,6 +878,23 @@ int htx_load_classifier(htx_solver* h, const float* params, double threshold) {
     HIP_TRY(hipMemcpy(h->dCvec, cvec.data(), sizeof(float) * CLS_NCVEC, hipMemcpyHostToDevice));
     h->cls_threshold = (float)threshold;
     h->have_classifier = true;
+    return HTX_OK;
+}
+
+int htx_classifier_stats(htx_solver* h, long long* n_faces, long long* n_linear, int reset) {
+    if (!h) return fail(HTX_EINVAL, "null handle");
+    Ctl c{};
+    int rc = read_ctl(h, &c);
+    if (rc) return rc;
+    if (n_faces) *n_faces = (long long)c.n_faces;
+    if (n_linear) *n_linear = (long long)c.n_linear;
+    if (reset) {
+        c.n_faces = 0;
+        c.n_linear = 0;
+        HIP_TRY(hipMemcpyAsync(&h->ctl->n_faces, &c.n_faces, 2 * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                               h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
     return HTX_OK;
 }
 

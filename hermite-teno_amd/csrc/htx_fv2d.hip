@@ -24,7 +24,7 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-template <int TX, int TY, int FLUX, bool MIXED>
+template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED>
 __global__ void __launch_bounds__(256, 2) fv2d_stage_kernel(Fv2dArgs a) {
     using T = Fv2dTile<TX, TY>;
     __shared__ T s;
@@ -41,13 +41,35 @@ __global__ void __launch_bounds__(256, 2) fv2d_stage_kernel(Fv2dArgs a) {
 
     ph_load<TX, TY>(a, s, i0, j0, tid);
     __syncthreads();
-    ph_recon<TX, TY, false, MIXED>(a, s, tid);
+    if (LEARNED) {
+        unsigned nface = 0, nlin = 0;
+        ph_classify<TX, TY>(a, s, i0, j0, tid, nface, nlin);
+        for (int off = 32; off > 0; off >>= 1) {
+            nface += __shfl_xor(nface, off, 64);
+            nlin += __shfl_xor(nlin, off, 64);
+        }
+        if ((tid & 63) == 0) {
+            s.sCnt[0][tid >> 6] = nface;
+            s.sCnt[1][tid >> 6] = nlin;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long f = 0, l = 0;
+            for (int k = 0; k < T::NT / 64; ++k) {
+                f += s.sCnt[0][k];
+                l += s.sCnt[1][k];
+            }
+            atomicAdd(&a.ctl->n_faces, f);
+            atomicAdd(&a.ctl->n_linear, l);
+        }
+    }
+    ph_recon<TX, TY, false, MIXED, LEARNED>(a, s, tid);
     __syncthreads();
     ph_flux<TX, TY, false, FLUX, MIXED>(a, s, tid);
     __syncthreads();
     ph_xdiv<TX, TY>(a, s, tid, dxp);
     __syncthreads();
-    ph_recon<TX, TY, true, MIXED>(a, s, tid);
+    ph_recon<TX, TY, true, MIXED, LEARNED>(a, s, tid);
     __syncthreads();
     ph_flux<TX, TY, true, FLUX, MIXED>(a, s, tid);
     __syncthreads();
@@ -182,26 +204,30 @@ __global__ void __launch_bounds__(256) k_ic_vortex(VortexArgs a) {
     for (int e = 0; e < 4; ++e) a.q[off + e * a.fstride] = acc[e];
 }
 
-template <int TX, int TY, bool MIXED>
+template <int TX, int TY, bool MIXED, bool LEARNED>
 static hipError_t launch_tile(const Fv2dArgs& a0, int flux, hipStream_t s) {
     Fv2dArgs a = a0;
     a.tiles_x = (a.nx + TX - 1) / TX;
     a.tiles_y = (a.ny + TY - 1) / TY;
     dim3 grid(a.tiles_x * a.tiles_y), block(256);
     if (flux == FLUX_RUSANOV_CHAR)
-        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_RUSANOV_CHAR, MIXED>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_RUSANOV_CHAR, MIXED, LEARNED>), grid, block, 0, s, a);
     else if (flux == FLUX_HLLC)
-        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC, MIXED>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC, MIXED, LEARNED>), grid, block, 0, s, a);
     else
-        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC_TORO, MIXED>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC_TORO, MIXED, LEARNED>), grid, block, 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y, hipStream_t s) {
     (void)tile_x;
     (void)tile_y;
-    if (a.mixed) return launch_tile<16, 16, true>(a, flux, s);
-    return launch_tile<16, 16, false>(a, flux, s);
+    if (a.learned) {
+        if (!a.cls_frag || !a.cls_cvec) return hipErrorInvalidValue;
+        return launch_tile<16, 16, true, true>(a, flux, s);
+    }
+    if (a.mixed) return launch_tile<16, 16, true, false>(a, flux, s);
+    return launch_tile<16, 16, false, false>(a, flux, s);
 }
 
 hipError_t launch_dt(Ctl* c, double dx, double dy, hipStream_t s) {

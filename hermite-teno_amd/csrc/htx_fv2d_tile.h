@@ -9,7 +9,7 @@
 //   y faces     recon (TX+4) x (TY+1): column fi at LDS col 1+fi, face fj between
 //               LDS rows 2+fj and 3+fj; flux TX x (TY+1) at cols 3..TX+2
 #pragma once
-#include "htx_internal.h"
+#include "htx_learn.h"
 
 namespace HTX_NS {
 
@@ -30,6 +30,8 @@ struct Fv2dTile {
     double sF[8][NF];  // characteristic face values (left/right per field)
     double sFl[4][NQ]; // face fluxes
     double sRed[2][NT / 64];
+    unsigned char sFlag[2][NF];  // learned precision: 1 = linear scheme on this recon face
+    unsigned int sCnt[2][NT / 64];
 };
 
 // halo load with the boundary condition applied through the index, plus the
@@ -61,9 +63,70 @@ HTX_HD Prim prim_at(const Fv2dTile<TX, TY>& s, int id) {
     return Prim{s.sP[0][id], s.sP[1][id], s.sP[2][id], s.sP[3][id]};
 }
 
+// Learned precision: classify every recon face of the tile (x faces, then y
+// faces) with the MFMA classifier; 1 = the linear scheme is enough.  The
+// features come from the tile's conserved state and Roe inputs; the
+// transposition scratch of the matrix cores reuses the face-value buffer,
+// which is free until the reconstruction.  Counts the faces the tile owns
+// (each global face once) and how many of them were flagged.
+template <int TX, int TY>
+__device__ void ph_classify(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int i0, int j0, int tid, unsigned& nface,
+                            unsigned& nlin) {
+    using T = Fv2dTile<TX, TY>;
+    static_assert(sizeof(s.sF) >= T::NT * 16 * sizeof(unsigned short), "scratch does not fit");
+    unsigned short* xs = reinterpret_cast<unsigned short*>(&s.sF[0][0]) + (tid >> 6) * 1024;
+    constexpr int NTOT = T::NXF + T::NYF;
+    for (int base = 0; base < NTOT; base += T::NT) {
+        const int w = base + tid;
+        const bool valid = w < NTOT, ydir = w >= T::NXF;
+        const int wl = ydir ? w - T::NXF : w;
+        float x[CLS_IN];
+        for (int k = 0; k < CLS_IN; ++k) x[k] = 0.0f;
+        bool owned = false;
+        if (valid) {
+            int ca, step;
+            if (!ydir) {
+                int fi = wl % (TX + 1), fj = wl / (TX + 1);
+                ca = (1 + fj) * T::HX + (2 + fi);
+                step = 1;
+                owned = fi >= 1 && fj >= 2 && fj <= TY + 1 && i0 + fi - 1 < a.nx && j0 + fj - 2 < a.ny;
+            } else {
+                int fi = wl % (TX + 4), fj = wl / (TX + 4);
+                ca = (2 + fj) * T::HX + (1 + fi);
+                step = T::HX;
+                owned = fj >= 1 && fi >= 2 && fi <= TX + 1 && j0 + fj - 1 < a.ny && i0 + fi - 2 < a.nx;
+            }
+            double r[6], un[6], ut[6], p[6];
+            for (int m = 0; m < 6; ++m) {
+                int id = ca + (m - 2) * step;
+                double u = s.sP[1][id], v = s.sP[2][id];
+                r[m] = s.sQ[0][id];
+                p[m] = (s.sQ[3][id] - 0.5 * (s.sQ[1][id] * u + s.sQ[2][id] * v)) * a.g1;
+                un[m] = ydir ? v : u;
+                ut[m] = ydir ? u : v;
+            }
+            face_features(r, un, ut, p, a.gam, x);
+        }
+        cls_stage_features(xs, x);
+        __syncthreads();
+        float z = cls_logit(xs, a.cls_frag, a.cls_cvec);
+        if (valid) {
+            unsigned char f = z > a.cls_thr ? 1 : 0;
+            s.sFlag[ydir ? 1 : 0][wl] = f;
+            if (owned) {
+                nface += 1;
+                nlin += f;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // characteristic TENO5 on faces (FV_System2D.cpp:372-400).  YDIR selects the
-// face family; the state is read in (rho, normal, tangential, E) order.
-template <int TX, int TY, bool YDIR, bool MIXED = false>
+// face family; the state is read in (rho, normal, tangential, E) order.  With
+// LEARNED, faces the classifier flagged take Lagrange::linear
+// (Schemes_Lagrange.cpp:1219-1233) instead.
+template <int TX, int TY, bool YDIR, bool MIXED = false, bool LEARNED = false>
 HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
     using T = Fv2dTile<TX, TY>;
     constexpr int NW = YDIR ? T::NYF : T::NXF;
@@ -79,15 +142,25 @@ HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
             int id = ca + (m - 2) * STEP;
             eig_left<YDIR>(E, s.sQ[0][id], s.sQ[1][id], s.sQ[2][id], s.sQ[3][id], c[0][m], c[1][m], c[2][m], c[3][m]);
         }
+        if (LEARNED && s.sFlag[YDIR ? 1 : 0][w]) {
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            double um, up;
-            if (MIXED)
-                teno5_mixed(c[f], um, up);
-            else
-                teno5(c[f], um, up);
-            s.sF[2 * f][w] = um;
-            s.sF[2 * f + 1][w] = up;
+            for (int f = 0; f < 4; ++f) {
+                double um, up;
+                linear5(c[f], um, up);
+                s.sF[2 * f][w] = um;
+                s.sF[2 * f + 1][w] = up;
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                double um, up;
+                if (MIXED)
+                    teno5_mixed(c[f], um, up);
+                else
+                    teno5(c[f], um, up);
+                s.sF[2 * f][w] = um;
+                s.sF[2 * f + 1][w] = up;
+            }
         }
     }
 }

@@ -15,6 +15,9 @@ struct Ctl {
     unsigned long long smax_x, smax_y;  // bit patterns of non-negative doubles
     int active;
     int nsteps;
+    // learned precision: faces classified and faces given the linear scheme
+    // (each face once per stage), since the last reset
+    unsigned long long n_faces, n_linear;
 };
 
 enum FluxKind { FLUX_RUSANOV_CHAR = 0, FLUX_HLLC = 1, FLUX_HLLC_TORO = 2 };
@@ -30,9 +33,13 @@ struct Fv2dArgs {
     int mode;           // 0 residual only, 1..3 SSP-RK3 stage
     int quirks;         // reference maxSpeed quirk in the CFL epilogue
     int mixed;          // 1: cut-off decisions and nonlinear weights in FP32
+    int learned;        // 1: classifier picks linear or TENO5 per face (implies mixed)
     int tiles_x, tiles_y;
     double gam, g1, dx, dy, area;
     QuadConsts qc;
+    const unsigned short* cls_frag;  // packed bf16 classifier weights
+    const float* cls_cvec;           // FP32 biases and output weights
+    float cls_thr;                   // logit above which a face is smooth
 };
 
 struct VortexArgs {

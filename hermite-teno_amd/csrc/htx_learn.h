@@ -84,31 +84,46 @@ HTX_HD float stencil_contrast(const double f[6], float eps) {
 
 // The 12 classifier inputs of a face from the primitive values of its
 // stencil cells c-2..c+3 (r = density, un / ut = velocity normal / tangential
-// to the face, p = pressure):
-//   x[0..4]  density differences r_{k+1} - r_k over (sum |differences| + 1e-6 r_face)
-//   x[5..9]  the same for the pressure
-//   x[10]    max of the TENO contrasts of r and p (eps = (1e-6 scale)^2)
-//   x[11]    max of the TENO contrasts of un and ut, velocity scale = the face's sound speed
+// to the face, p = pressure).  Scales: rf, pf = face averages of |r|, |p|,
+// cf = sqrt(gamma pf / rf).
+//   x[0..1]   density differences r_c - r_{c-1}, r_{c+1} - r_c over
+//             (sum of the five |differences| + 1e-6 rf)
+//   x[2..3]   the same for the pressure
+//   x[4..7]   log10(max |difference| / scale + 1e-12) / 12 + 1 for r, p, un,
+//             ut (scales rf, pf, cf, cf): 0 for a flat stencil, about 1 for
+//             an O(1) jump
+//   x[8..11]  TENO contrasts of r, p, un, ut (eps = 1e-12 scale^2)
+HTX_HD float log_amp(double amp) { return log10f((float)(amp + 1e-12)) * (1.0f / 12.0f) + 1.0f; }
+
 HTX_HD void face_features(const double r[6], const double un[6], const double ut[6], const double p[6],
                           double gam, float x[CLS_IN]) {
     double rf = 0.5 * (fabs(r[2]) + fabs(r[3])), pf = 0.5 * (fabs(p[2]) + fabs(p[3]));
-    double dr[5], dp[5], sr = 0.0, sp = 0.0;
+    double cf2 = gam * pf / (rf + 1e-300), cf = sqrt(cf2);
+    double sr = 0.0, sp = 0.0, ar = 0.0, ap = 0.0, au = 0.0, at = 0.0;
     for (int k = 0; k < 5; ++k) {
-        dr[k] = r[k + 1] - r[k];
-        dp[k] = p[k + 1] - p[k];
-        sr += fabs(dr[k]);
-        sp += fabs(dp[k]);
+        double dr = fabs(r[k + 1] - r[k]), dp = fabs(p[k + 1] - p[k]);
+        sr += dr;
+        sp += dp;
+        ar = fmax(ar, dr);
+        ap = fmax(ap, dp);
+        au = fmax(au, fabs(un[k + 1] - un[k]));
+        at = fmax(at, fabs(ut[k + 1] - ut[k]));
     }
     double nr = 1.0 / (sr + 1e-6 * rf + 1e-300), np = 1.0 / (sp + 1e-6 * pf + 1e-300);
-    for (int k = 0; k < 5; ++k) {
-        x[k] = (float)(dr[k] * nr);
-        x[5 + k] = (float)(dp[k] * np);
-    }
+    x[0] = (float)((r[2] - r[1]) * nr);
+    x[1] = (float)((r[3] - r[2]) * nr);
+    x[2] = (float)((p[2] - p[1]) * np);
+    x[3] = (float)((p[3] - p[2]) * np);
+    x[4] = log_amp(ar / (rf + 1e-300));
+    x[5] = log_amp(ap / (pf + 1e-300));
+    x[6] = log_amp(au / (cf + 1e-300));
+    x[7] = log_amp(at / (cf + 1e-300));
     float er = (float)(1e-12 * rf * rf) + 1e-30f, ep = (float)(1e-12 * pf * pf) + 1e-30f;
-    float cf2 = (float)(gam * pf / (rf + 1e-300));
-    float ev = 1e-12f * cf2 + 1e-30f;
-    x[10] = fmaxf(stencil_contrast(r, er), stencil_contrast(p, ep));
-    x[11] = fmaxf(stencil_contrast(un, ev), stencil_contrast(ut, ev));
+    float ev = (float)(1e-12 * cf2) + 1e-30f;
+    x[8] = stencil_contrast(r, er);
+    x[9] = stencil_contrast(p, ep);
+    x[10] = stencil_contrast(un, ev);
+    x[11] = stencil_contrast(ut, ev);
 }
 
 // Primitive (r, un, ut, p) of a conserved state (rho, m_n, m_t, E).
@@ -121,20 +136,24 @@ HTX_HD void prim_nt(double g1, double q0, double qn, double qt, double q3, doubl
 }
 
 // Training label of a face: 1 when plain FP64 TENO5 and the linear scheme give
-// the same face values (|teno - linear| <= 1e-13 max|u| + 1e-9 sum|u_{k+1} - u_k|
-// over the stencil) on both sides for all four characteristic fields, else 0.
-// That holds when TENO5 keeps every candidate, and also where it cuts one only
-// because rounding left a smoothness measure of an exactly flat stencil a few
-// ulps above zero (all candidates then coincide).  c[f][m] = characteristic
-// values of field f on the six stencil cells (FV_System2D.cpp:372-400).
+// the same face values on both sides for all four characteristic fields, to
+// within 1e-12 M + 1e-9 sum_k |u_{k+1} - u_k| of the field's own stencil, M =
+// the largest |value| of any field on the stencil; else 0.  That holds when
+// TENO5 keeps every candidate, and also where it cuts one only because the
+// data are flat up to rounding (a smoothness measure left a few ulps above
+// zero, or rounding noise on a field that is zero), where all candidates
+// coincide to rounding.  c[f][m] = characteristic values of field f on the
+// six stencil cells (FV_System2D.cpp:372-400).
 HTX_HD int teno_label(const double c[4][6]) {
+    double mx = 0.0;
+    for (int f = 0; f < 4; ++f)
+        for (int m = 0; m < 6; ++m) mx = fmax(mx, fabs(c[f][m]));
     for (int f = 0; f < 4; ++f) {
-        double tm, tp, lm, lp, mx = 0.0, tv = 0.0;
+        double tm, tp, lm, lp, tv = 0.0;
         teno5(c[f], tm, tp);
         linear5(c[f], lm, lp);
-        for (int m = 0; m < 6; ++m) mx = fmax(mx, fabs(c[f][m]));
         for (int m = 0; m < 5; ++m) tv += fabs(c[f][m + 1] - c[f][m]);
-        double tol = 1e-13 * mx + 1e-9 * tv;
+        double tol = 1e-12 * mx + 1e-9 * tv;
         if (!(fabs(tm - lm) <= tol && fabs(tp - lp) <= tol)) return 0;
     }
     return 1;

@@ -125,6 +125,12 @@ class System2D:
         p = np.ascontiguousarray(params, dtype=np.float32)
         check(_lib.load().htx_load_classifier(self._h, _lib.fptr(p), float(threshold)))
 
+    def classifier_stats(self, reset=False):
+        """(faces classified, faces given the linear scheme) since the last reset."""
+        f, l = ctypes.c_longlong(), ctypes.c_longlong()
+        check(_lib.load().htx_classifier_stats(self._h, ctypes.byref(f), ctypes.byref(l), 1 if reset else 0))
+        return f.value, l.value
+
     def classify(self, direction, features=False):
         """Logits of every x (direction 0) or y (1) face of the current state,
         indexed [j, i] (face of cell (i, j) with its +1 neighbour), from the

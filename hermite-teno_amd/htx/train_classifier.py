@@ -1,0 +1,149 @@
+"""Train the learned troubled-face classifier on solver snapshots and write
+its parameters (htx/data/classifier_teno5.npy) and a report
+(htx/data/classifier_teno5.json).
+
+The training set is every x and y face of snapshots that the GPU solver
+itself produces (plain FP64 TENO5 + HLLC) of:
+  * the Sod shock tube along x (N = 400 cells, 16 rows, transmissive),
+  * the periodic isentropic vortex with seeded density modes on [0,10]^2,
+  * seeded four-quadrant Riemann problems on [0,1]^2 (transmissive),
+labelled by whether plain TENO5 departs from the linear scheme on the face.
+The shock-plus-blobs workload is held out and used for the report.
+
+    python -m htx.train_classifier [--n 512] [--epochs 6] [--out DIR]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _solver(w, nx, ny, precision="fp64"):
+    from .solver import System2D
+
+    return System2D(w["xl"], w["xr"], nx, w["yb"], w["yt"], ny, w["cfl"], w["tmax"], 0.0, w["gamma"], flux="hllc",
+                    bc=w["bc"], precision=precision)
+
+
+def sod_workload():
+    """Sod tube along x as a quadrant problem whose split line y = -1 lies
+    below the domain: k = (x >= 0.5) + 2 picks the left or the right state."""
+    left, right = [1.0, 0.0, 0.0, 1.0], [0.125, 0.0, 0.0, 0.1]
+    return dict(kind="quadrant", xl=0.0, xr=1.0, yb=0.0, yt=16.0 / 400.0, gamma=1.4, cfl=0.5, tmax=0.2,
+                bc="zero_grad", xs=0.5, ys=-1.0, states=[left, right, left, right])
+
+
+def snapshot_plan(n):
+    from .ic import SEED_BASE, quadrant_workload, vortex_workload
+
+    plan = [("sod", sod_workload(), 400, 16, [0.02 * k for k in range(1, 11)])]
+    plan.append(("vortex", vortex_workload(n), n, n, [0.0, 0.5, 1.0, 1.5, 2.0]))
+    for k in range(8):
+        w = quadrant_workload(n, seed=SEED_BASE + 3 + 1000 * k)
+        plan.append((f"quadrant{k}", w, n, n, [0.03, 0.08, 0.15, 0.25]))
+    return plan
+
+
+def add_snapshots(trainer, w, nx, ny, times, tag):
+    s = _solver(w, nx, ny)
+    s.init_workload(w)
+    for t in times:
+        if t > 0:
+            s.timeStepping(t)
+        trainer.add_state(s)
+        n, ns = trainer.size()
+        log(f"[train] {tag} t={t:.3f}: set {n} faces, {ns} smooth")
+    s.close()
+
+
+def heldout_set(trainer_cls, n):
+    from .ic import shock_blob_workload
+
+    w = shock_blob_workload(4096)
+    nx, ny = n * 2, n // 2
+    t = trainer_cls(2 * nx * ny * 3)
+    s = _solver(w, nx, ny)
+    s.init_workload(w)
+    for f in (0.25, 0.5, 1.0):
+        s.timeStepping(f * w["tmax"])
+        t.add_state(s)
+    s.close()
+    return t
+
+
+def rates(c):
+    troubled = c["troubled_as_smooth"] + c["troubled_as_troubled"]
+    smooth = c["smooth_as_smooth"] + c["smooth_as_troubled"]
+    total = troubled + smooth
+    return {"faces": total, "flagged_smooth": (c["smooth_as_smooth"] + c["troubled_as_smooth"]) / total,
+            "smooth_recall": c["smooth_as_smooth"] / max(smooth, 1),
+            "false_smooth_of_troubled": c["troubled_as_smooth"] / max(troubled, 1),
+            "accuracy": (c["smooth_as_smooth"] + c["troubled_as_troubled"]) / total}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--batch", type=int, default=16384)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--troubled-weight", type=float, default=2.0)
+    ap.add_argument("--false-smooth", type=float, default=0.005,
+                    help="largest share of troubled training faces the threshold may call smooth")
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--out", default=os.path.join(_HERE, "data"))
+    args = ap.parse_args()
+
+    from .learn import Trainer
+
+    t0 = time.time()
+    plan = snapshot_plan(args.n)
+    cap = sum(2 * nx * ny * len(ts) for _, _, nx, ny, ts in plan)
+    tr = Trainer(cap)
+    for tag, w, nx, ny, ts in plan:
+        add_snapshots(tr, w, nx, ny, ts, tag)
+    n, ns = tr.size()
+    log(f"[train] training set: {n} faces, {ns} smooth ({ns / n:.3f}), built in {time.time() - t0:.1f} s")
+    tr.init(seed=args.seed)
+    losses = []
+    for ep in range(args.epochs):
+        lr = args.lr * (0.5 ** (ep // max(1, args.epochs // 3)))
+        l = tr.train(1, batch=args.batch, lr=lr, troubled_weight=args.troubled_weight, seed=args.seed + ep)
+        losses.append(float(l[0]))
+        log(f"[train] epoch {ep + 1}/{args.epochs} lr {lr:.2e} loss {l[0]:.5f} ({time.time() - t0:.1f} s)")
+    # threshold: the smallest logit at which at most the requested share of
+    # troubled training faces would be called smooth
+    ev = tr.evaluate(0.0, want_logits=True)
+    _, lab = tr.read(0, n)
+    lg = ev["logits"]
+    tl = np.sort(lg[lab == 0])
+    k = int(np.floor((1.0 - args.false_smooth) * len(tl)))
+    thr = float(tl[min(k, len(tl) - 1)])
+    train_rates = rates(tr.evaluate(thr))
+    log(f"[train] threshold {thr:.4f}: {train_rates}")
+    ho = heldout_set(Trainer, args.n)
+    ho.set_params(tr.params())
+    ho_rates = rates(ho.evaluate(thr))
+    log(f"[train] held out (shock + blobs): {ho_rates}")
+    os.makedirs(args.out, exist_ok=True)
+    np.save(os.path.join(args.out, "classifier_teno5.npy"), tr.params())
+    report = {"threshold": thr, "epochs": args.epochs, "batch": args.batch, "lr": args.lr,
+              "troubled_weight": args.troubled_weight, "training_faces": n, "training_smooth": ns,
+              "epoch_loss": losses, "train": train_rates, "heldout_shock_blobs": ho_rates,
+              "snapshot_n": args.n, "seconds": time.time() - t0}
+    with open(os.path.join(args.out, "classifier_teno5.json"), "w") as f:
+        json.dump(report, f, indent=1)
+    log(f"[train] wrote {args.out}/classifier_teno5.npy ({time.time() - t0:.1f} s)")
+
+
+if __name__ == "__main__":
+    main()

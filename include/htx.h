@@ -51,7 +51,9 @@ enum htx_bc { HTX_BC_ZERO_GRAD = 0, HTX_BC_PERIODIC = 1 };
  *   but rounding can move a TENO cut-off decision that sits at its threshold.
  * HTX_PREC_FP32_MIXED: FP32 cut-off decisions and nonlinear weights, FP64
  *   values, projections, fluxes and RK accumulation (TENO5 scheme).
- * HTX_PREC_LEARNED: reserved for the learned troubled-face indicator. */
+ * HTX_PREC_LEARNED: FP32-mixed TENO5 where a learned classifier (see below)
+ *   sends the faces it finds smooth to the optimal-weight linear scheme
+ *   (Lagrange::linear) and the rest to TENO5; needs htx_load_classifier. */
 enum htx_precision { HTX_PREC_FP64 = 0, HTX_PREC_FP32_MIXED = 1, HTX_PREC_LEARNED = 2, HTX_PREC_FP64_FAST = 3 };
 
 typedef struct htx_params {
@@ -214,6 +216,11 @@ int htx_load_classifier(htx_solver* h, const float* params, double threshold);
  * dir 1 = y faces, indexed j * nx + i, computed with the solver's bf16
  * matrix-core inference. */
 int htx_classify(htx_solver* h, int dir, float* logits, float* feat);
+
+/* Learned precision: faces classified (each face once per stage) and faces
+ * that took the linear scheme, summed since the last reset; reset != 0 zeroes
+ * both after reading. */
+int htx_classifier_stats(htx_solver* h, long long* n_faces, long long* n_linear, int reset);
 
 /* Message of the last failed call on this host thread. */
 const char* htx_last_error(void);

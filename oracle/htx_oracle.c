@@ -668,8 +668,9 @@ void orc_fv2d_rhs(const orc_params* p, const double* q, double* rhs) {
  * between (i, j) and (i, j+1), labels indexed j * nx + i; label 1 when, for
  * every characteristic field and both sides, teno5 (Schemes_Lagrange.cpp
  * :481-511) gives the value of linear (:1219-1233) to within
- * 1e-13 max|u| + 1e-9 sum|u_{k+1} - u_k| of the 6-value stencil.  (The
- * learned classifier's training target; the reference has no such label.) */
+ * 1e-12 M + 1e-9 sum|u_{k+1} - u_k| of the field's 6-value stencil, M = the
+ * largest |value| of any field on the stencil.  (The learned classifier's
+ * training target; the reference has no such label.) */
 void orc_face_labels(const orc_params* p, const double* q, int dir, unsigned char* labels) {
     lag_init();
     padded P;
@@ -689,13 +690,15 @@ void orc_face_labels(const orc_params* p, const double* q, int dir, unsigned cha
                 for (int e = 0; e < 4; ++e) C[e][m] = y[e];
             }
             int all = 1;
+            double mx = 0.0;
+            for (int e = 0; e < 4; ++e)
+                for (int m = 0; m < 6; ++m) mx = fmax(mx, fabs(C[e][m]));
             for (int e = 0; e < 4; ++e) {
-                double tv[2], lv[2], mx = 0.0, var = 0.0;
+                double tv[2], lv[2], var = 0.0;
                 teno5_impl(C[e], tv, NULL);
                 orc_linear5(C[e], lv);
-                for (int m = 0; m < 6; ++m) mx = fmax(mx, fabs(C[e][m]));
                 for (int m = 0; m < 5; ++m) var += fabs(C[e][m + 1] - C[e][m]);
-                double tol = 1e-13 * mx + 1e-9 * var;
+                double tol = 1e-12 * mx + 1e-9 * var;
                 if (!(fabs(tv[0] - lv[0]) <= tol && fabs(tv[1] - lv[1]) <= tol)) all = 0;
             }
             labels[(size_t)j * nx + i] = (unsigned char)all;

@@ -63,6 +63,10 @@ def stencil_contrast(f, eps):
                       teno_contrast(f[1], f[2], f[3], f[4], f[5], eps))
 
 
+def log_amp(a):
+    return np.log10(F32(a + 1e-12)) * (F(1.0) / F(12.0)) + F(1.0)
+
+
 def features(q, gamma, bc, direction):
     """The 12 inputs of every face of one family, shape (ny, nx, 12)."""
     st = stencils(q, bc, direction)
@@ -73,25 +77,36 @@ def features(q, gamma, bc, direction):
     p = (st[:, 3] - 0.5 * (qn * un + qt * ut)) * g1
     rf = 0.5 * (np.abs(r[2]) + np.abs(r[3]))
     pf = 0.5 * (np.abs(p[2]) + np.abs(p[3]))
-    dr = [r[k + 1] - r[k] for k in range(5)]
-    dp = [p[k + 1] - p[k] for k in range(5)]
+    cf2 = gamma * pf / (rf + 1e-300)
+    cf = np.sqrt(cf2)
     sr = np.zeros_like(rf)
     sp = np.zeros_like(pf)
+    ar, ap, au, at = (np.zeros_like(rf) for _ in range(4))
     for k in range(5):
-        sr = sr + np.abs(dr[k])
-        sp = sp + np.abs(dp[k])
+        dr, dp = np.abs(r[k + 1] - r[k]), np.abs(p[k + 1] - p[k])
+        sr = sr + dr
+        sp = sp + dp
+        ar, ap = np.maximum(ar, dr), np.maximum(ap, dp)
+        au = np.maximum(au, np.abs(un[k + 1] - un[k]))
+        at = np.maximum(at, np.abs(ut[k + 1] - ut[k]))
     nr = 1.0 / (sr + 1e-6 * rf + 1e-300)
     npp = 1.0 / (sp + 1e-6 * pf + 1e-300)
     x = np.empty(rf.shape + (12,), dtype=np.float32)
-    for k in range(5):
-        x[..., k] = F32(dr[k] * nr)
-        x[..., 5 + k] = F32(dp[k] * npp)
+    x[..., 0] = F32((r[2] - r[1]) * nr)
+    x[..., 1] = F32((r[3] - r[2]) * nr)
+    x[..., 2] = F32((p[2] - p[1]) * npp)
+    x[..., 3] = F32((p[3] - p[2]) * npp)
+    x[..., 4] = log_amp(ar / (rf + 1e-300))
+    x[..., 5] = log_amp(ap / (pf + 1e-300))
+    x[..., 6] = log_amp(au / (cf + 1e-300))
+    x[..., 7] = log_amp(at / (cf + 1e-300))
     er = F32(1e-12 * rf * rf) + F(1e-30)
     ep = F32(1e-12 * pf * pf) + F(1e-30)
-    cf2 = F32(gamma * pf / (rf + 1e-300))
-    ev = F(1e-12) * cf2 + F(1e-30)
-    x[..., 10] = np.maximum(stencil_contrast(r, er), stencil_contrast(p, ep))
-    x[..., 11] = np.maximum(stencil_contrast(un, ev), stencil_contrast(ut, ev))
+    ev = F32(1e-12 * cf2) + F(1e-30)
+    x[..., 8] = stencil_contrast(r, er)
+    x[..., 9] = stencil_contrast(p, ep)
+    x[..., 10] = stencil_contrast(un, ev)
+    x[..., 11] = stencil_contrast(ut, ev)
     return x
 
 

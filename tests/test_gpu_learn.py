@@ -136,3 +136,47 @@ def test_training_separates_classes(orc):
     assert acc > 0.9
     assert after["troubled_as_smooth"] < 0.2 * (nf - ns)
     assert before != after
+
+
+def _pair(precision, n, w, threshold=None, params_=None):
+    import htx
+
+    s = htx.System2D(w["xl"], w["xr"], n, w["yb"], w["yt"], n, w["cfl"], w["tmax"], 0.0, w["gamma"], flux="hllc",
+                     bc=w["bc"], precision=precision)
+    if precision == "learned":
+        s.load_classifier(params_, threshold)
+    s.init_workload(w)
+    return s
+
+
+@pytest.mark.parametrize("case", ["quadrant", "vortex"])
+def test_learned_mode_limits(case):
+    """With a threshold no logit reaches, precision "learned" sends every face
+    to TENO5 and matches "fp32_mixed" bit for bit; with a threshold every
+    logit passes it runs the linear scheme everywhere, and the face counts
+    cover every face once per stage."""
+    from htx.ic import quadrant_workload, vortex_workload
+
+    n = 64
+    w = quadrant_workload(n) if case == "quadrant" else vortex_workload(n)
+    rng = np.random.default_rng(3)
+    prm = rng.normal(0, 0.3, 5057).astype(np.float32)
+    print(f"[case] learned limits {case} n={n}", flush=True)
+    ref = _pair("fp32_mixed", n, w)
+    ref.steps_async(4)
+    never = _pair("learned", n, w, 1e30, prm)
+    never.steps_async(4)
+    np.testing.assert_array_equal(never.get_state(), ref.get_state())
+    f, l = never.classifier_stats()
+    assert f == 2 * n * n * 3 * 4 and l == 0
+    always = _pair("learned", n, w, -1e30, prm)
+    always.steps_async(4)
+    f, l = always.classifier_stats(reset=True)
+    assert f == l == 2 * n * n * 3 * 4
+    assert always.classifier_stats() == (0, 0)
+    q = always.get_state()
+    assert np.isfinite(q).all()
+    if case == "vortex":
+        # smooth data: the linear scheme stays within discretization error of TENO
+        from conftest import rel_l1
+        assert rel_l1(q, ref.get_state()) < 1e-4

@@ -58,9 +58,10 @@ def test_labels_transpose(orc):
 
 
 def test_feature_invariances():
-    """Scaling density and pressure by constants leaves the density and
-    pressure features and the contrasts unchanged (up to FP32 rounding);
-    a uniform state gives all-zero features; the contrasts lie in [0, 1)."""
+    """Scaling density and pressure by a common constant leaves every
+    feature unchanged (up to FP32 rounding: the differences and amplitudes
+    are relative to the face's own scales, the contrasts scale-free); a
+    uniform state gives all-zero features; the contrasts lie in [0, 1)."""
     rng = np.random.default_rng(5)
     n = 16
     r = 1.0 + 0.3 * rng.random((n, n))
@@ -69,7 +70,8 @@ def test_feature_invariances():
     x0 = L.features(cons(r, u, v, p), 1.4, "periodic", 0)
     x1 = L.features(cons(4.0 * r, u, v, 4.0 * p), 1.4, "periodic", 0)
     np.testing.assert_allclose(x0, x1, rtol=2e-5, atol=2e-6)
-    assert (x0[..., 10:] >= 0).all() and (x0[..., 10:] < 1).all()
+    assert (x0[..., 8:] >= 0).all() and (x0[..., 8:] < 1).all()
+    assert (x0[..., 4:8] > 0.3).all()
     xz = L.features(cons(1 + 0 * r, 0.3 + 0 * r, 0 * r, 2 + 0 * r), 1.4, "periodic", 1)
     assert np.abs(xz).max() < 1e-6
 
