@@ -97,7 +97,10 @@ __global__ void __launch_bounds__(256, 2) fv2d_stage_kernel(Fv2dArgs a) {
 }
 
 // dt from the accumulated max wave speeds (FV_System2D.cpp:633,645); a step
-// with t >= tFinal is marked inactive and every stage kernel skips it.
+// with t >= tFinal is marked inactive and every stage kernel skips it.  The
+// speeds are consumed (zeroed for stage 3 to accumulate afresh) only by an
+// active step: an inactive step computes none, so the speeds of the last
+// state carry over to the first step of a later advance.
 __global__ void k_dt(Ctl* c, double dx, double dy) {
     double t = c->t, tf = c->tfinal;
     if (t < tf) {
@@ -107,12 +110,12 @@ __global__ void k_dt(Ctl* c, double dx, double dy) {
         dt = t + dt > tf ? tf - t : dt;
         c->dt = dt;
         c->active = 1;
+        c->smax_x = 0ull;
+        c->smax_y = 0ull;
     } else {
         c->dt = 0.0;
         c->active = 0;
     }
-    c->smax_x = 0ull;
-    c->smax_y = 0ull;
 }
 
 __global__ void k_tadvance(Ctl* c) {

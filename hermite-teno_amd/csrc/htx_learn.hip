@@ -68,8 +68,12 @@ __global__ void __launch_bounds__(256) k_extract(LrnExtractArgs a) {
         float x[CLS_IN];
         stencil_features<YDIR>(st, a.gam, x);
         long long o = a.base + id;
-        for (int k = 0; k < CLS_IN; ++k) a.feat[o * CLS_IN + k] = x[k];
-        int lab = stencil_label<YDIR>(st, a.gam - 1.0);
+        // a face of a non-finite state enters the set as troubled with zero
+        // features rather than poisoning the training with NaN
+        bool finite = true;
+        for (int k = 0; k < CLS_IN; ++k) finite = finite && (x[k] - x[k] == 0.0f);
+        for (int k = 0; k < CLS_IN; ++k) a.feat[o * CLS_IN + k] = finite ? x[k] : 0.0f;
+        int lab = finite ? stencil_label<YDIR>(st, a.gam - 1.0) : 0;
         a.label[o] = (unsigned char)lab;
         smooth = (unsigned)lab;
     }

@@ -313,3 +313,40 @@ def test_fp32_mixed_accuracy(flux, case):
         print(f"{name}: L1 error vs 4x finer FP64: fp64 {e64:.4e}, fp32_mixed {e32:.4e}, "
               f"ratio {e32 / e64:.4f}; fp32_mixed - fp64 {rel_l1(q32[sl], q64[sl]):.3e}", flush=True)
         assert e32 <= 1.10 * e64
+
+
+@pytest.mark.parametrize("scheme", ["teno5", "hteno"])
+def test_consecutive_advances_match_one_trajectory(orc, scheme):
+    """timeStepping to several end times in turn: each advance starts from the
+    wave speeds of the state it inherits (the steps skipped past the previous
+    end time must not clear them), so the trajectory is the oracle's, which
+    recomputes the speeds from the state at every step.  Sod tube along x,
+    400 x 16 cells advanced to t = 0.01, ..., 0.05 (Hermite: 200 x 8 cells to
+    t = 0.01, 0.02, 0.03)."""
+    import htx
+    from htx.ic import piecewise_state
+
+    hermite = scheme == "hteno"
+    nx, ny, nadv = (200, 8, 3) if hermite else (400, 16, 5)
+    left, right = [1.0, 0.0, 0.0, 1.0], [0.125, 0.0, 0.0, 0.1]
+    w = dict(kind="quadrant", xl=0.0, xr=1.0, yb=0.0, yt=ny / nx, gamma=GAMMA, xs=0.5, ys=-1.0,
+             states=[left, right, left, right])
+    print(f"[case] consecutive advances, Sod {nx}x{ny}, {scheme}", flush=True)
+    s = htx.System2D(0.0, 1.0, nx, 0.0, w["yt"], ny, 0.5, 1.0, 0.0, GAMMA, scheme=scheme, flux="hllc",
+                     bc="zero_grad")
+    s.init_workload(w)
+    q = piecewise_state(w, nx, ny, hermite=hermite)
+    np.testing.assert_array_equal(s.get_state(), q)
+    prm = params(nx, ny, 0.0, 1.0, 0.0, w["yt"], GAMMA, "zero_grad", "hllc", True)
+    adv = orc.fvh2d_advance if hermite else orc.fv2d_advance
+    t = 0.0
+    for k in range(1, nadv + 1):
+        tf = 0.01 * k
+        ns = s.timeStepping(tf)
+        q, t, nc = adv(prm, q, 0.5, t, tf)
+        err = rel_l1(s.get_state(), q)
+        print(f"t={tf:.2f}: {ns} steps (oracle {nc}), rel L1 {err:.3e}, min rho {s.get_state()[0].min():.4f}",
+              flush=True)
+        assert ns == nc
+        assert s.time == pytest.approx(t, rel=1e-14)
+        assert err < TOL_END
