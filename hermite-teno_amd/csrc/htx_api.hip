@@ -881,18 +881,17 @@ int htx_load_classifier(htx_solver* h, const float* params, double threshold) {
     return HTX_OK;
 }
 
-int htx_classifier_stats(htx_solver* h, long long* n_faces, long long* n_linear, int reset) {
+int htx_classifier_stats(htx_solver* h, long long* n_faces, long long* n_flagged, long long* n_linear, int reset) {
     if (!h) return fail(HTX_EINVAL, "null handle");
     Ctl c{};
     int rc = read_ctl(h, &c);
     if (rc) return rc;
     if (n_faces) *n_faces = (long long)c.n_faces;
+    if (n_flagged) *n_flagged = (long long)c.n_flagged;
     if (n_linear) *n_linear = (long long)c.n_linear;
     if (reset) {
-        c.n_faces = 0;
-        c.n_linear = 0;
-        HIP_TRY(hipMemcpyAsync(&h->ctl->n_faces, &c.n_faces, 2 * sizeof(unsigned long long), hipMemcpyHostToDevice,
-                               h->stream));
+        unsigned long long z[3] = {0, 0, 0};
+        HIP_TRY(hipMemcpyAsync(&h->ctl->n_faces, z, sizeof z, hipMemcpyHostToDevice, h->stream));
         HIP_TRY(hipStreamSynchronize(h->stream));
     }
     return HTX_OK;

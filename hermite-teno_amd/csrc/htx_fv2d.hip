@@ -41,35 +41,31 @@ __global__ void __launch_bounds__(256, 2) fv2d_stage_kernel(Fv2dArgs a) {
 
     ph_load<TX, TY>(a, s, i0, j0, tid);
     __syncthreads();
-    if (LEARNED) {
-        unsigned nface = 0, nlin = 0;
-        ph_classify<TX, TY>(a, s, i0, j0, tid, nface, nlin);
-        for (int off = 32; off > 0; off >>= 1) {
-            nface += __shfl_xor(nface, off, 64);
-            nlin += __shfl_xor(nlin, off, 64);
-        }
-        if ((tid & 63) == 0) {
-            s.sCnt[0][tid >> 6] = nface;
-            s.sCnt[1][tid >> 6] = nlin;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            unsigned long long f = 0, l = 0;
-            for (int k = 0; k < T::NT / 64; ++k) {
-                f += s.sCnt[0][k];
-                l += s.sCnt[1][k];
-            }
-            atomicAdd(&a.ctl->n_faces, f);
-            atomicAdd(&a.ctl->n_linear, l);
-        }
-    }
-    ph_recon<TX, TY, false, MIXED, LEARNED>(a, s, tid);
+    unsigned cnt[3] = {0, 0, 0};
+    if (LEARNED) ph_classify<TX, TY>(a, s, tid);
+    ph_recon<TX, TY, false, MIXED, LEARNED>(a, s, tid, i0, j0, cnt);
     __syncthreads();
     ph_flux<TX, TY, false, FLUX, MIXED>(a, s, tid);
     __syncthreads();
     ph_xdiv<TX, TY>(a, s, tid, dxp);
     __syncthreads();
-    ph_recon<TX, TY, true, MIXED, LEARNED>(a, s, tid);
+    ph_recon<TX, TY, true, MIXED, LEARNED>(a, s, tid, i0, j0, cnt);
+    if (LEARNED) {
+        for (int k = 0; k < 3; ++k) {
+            unsigned v = cnt[k];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if ((tid & 63) == 0) s.sCnt[k][tid >> 6] = v;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long c3[3] = {0, 0, 0};
+            for (int k = 0; k < 3; ++k)
+                for (int w = 0; w < T::NT / 64; ++w) c3[k] += s.sCnt[k][w];
+            atomicAdd(&a.ctl->n_faces, c3[0]);
+            atomicAdd(&a.ctl->n_flagged, c3[1]);
+            atomicAdd(&a.ctl->n_linear, c3[2]);
+        }
+    }
     __syncthreads();
     ph_flux<TX, TY, true, FLUX, MIXED>(a, s, tid);
     __syncthreads();

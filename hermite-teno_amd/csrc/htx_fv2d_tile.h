@@ -31,7 +31,7 @@ struct Fv2dTile {
     double sFl[4][NQ]; // face fluxes
     double sRed[2][NT / 64];
     unsigned char sFlag[2][NF];  // learned precision: 1 = linear scheme on this recon face
-    unsigned int sCnt[2][NT / 64];
+    unsigned int sCnt[3][NT / 64];
 };
 
 // halo load with the boundary condition applied through the index, plus the
@@ -67,11 +67,9 @@ HTX_HD Prim prim_at(const Fv2dTile<TX, TY>& s, int id) {
 // faces) with the MFMA classifier; 1 = the linear scheme is enough.  The
 // features come from the tile's conserved state and Roe inputs; the
 // transposition scratch of the matrix cores reuses the face-value buffer,
-// which is free until the reconstruction.  Counts the faces the tile owns
-// (each global face once) and how many of them were flagged.
+// which is free until the reconstruction.
 template <int TX, int TY>
-__device__ void ph_classify(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int i0, int j0, int tid, unsigned& nface,
-                            unsigned& nlin) {
+__device__ void ph_classify(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
     using T = Fv2dTile<TX, TY>;
     static_assert(sizeof(s.sF) >= T::NT * 16 * sizeof(unsigned short), "scratch does not fit");
     unsigned short* xs = reinterpret_cast<unsigned short*>(&s.sF[0][0]) + (tid >> 6) * 1024;
@@ -82,19 +80,16 @@ __device__ void ph_classify(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int i0, int 
         const int wl = ydir ? w - T::NXF : w;
         float x[CLS_IN];
         for (int k = 0; k < CLS_IN; ++k) x[k] = 0.0f;
-        bool owned = false;
         if (valid) {
             int ca, step;
             if (!ydir) {
                 int fi = wl % (TX + 1), fj = wl / (TX + 1);
                 ca = (1 + fj) * T::HX + (2 + fi);
                 step = 1;
-                owned = fi >= 1 && fj >= 2 && fj <= TY + 1 && i0 + fi - 1 < a.nx && j0 + fj - 2 < a.ny;
             } else {
                 int fi = wl % (TX + 4), fj = wl / (TX + 4);
                 ca = (2 + fj) * T::HX + (1 + fi);
                 step = T::HX;
-                owned = fj >= 1 && fi >= 2 && fi <= TX + 1 && j0 + fj - 1 < a.ny && i0 + fi - 2 < a.nx;
             }
             double r[6], un[6], ut[6], p[6];
             for (int m = 0; m < 6; ++m) {
@@ -110,24 +105,37 @@ __device__ void ph_classify(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int i0, int 
         cls_stage_features(xs, x);
         __syncthreads();
         float z = cls_logit(xs, a.cls_frag, a.cls_cvec);
-        if (valid) {
-            unsigned char f = z > a.cls_thr ? 1 : 0;
-            s.sFlag[ydir ? 1 : 0][wl] = f;
-            if (owned) {
-                nface += 1;
-                nlin += f;
-            }
-        }
+        if (valid) s.sFlag[ydir ? 1 : 0][wl] = z > a.cls_thr ? 1 : 0;
         __syncthreads();
     }
+}
+
+// FP64 positivity of the conservative state that characteristic face values
+// c[0..3] back-project to (density and pressure > 0).
+HTX_HD bool face_state_positive(const Eig& E, double g1, const double c[4]) {
+    double q0, qn, qt, q3;
+    eig_right(E, c[0], c[1], c[2], c[3], q0, qn, qt, q3);
+    return q0 > 0.0 && (q3 - 0.5 * (qn * qn + qt * qt) / q0) * g1 > 0.0;
+}
+
+// Whether recon face (fi, fj) of family YDIR is one the tile owns: every
+// global face is owned by exactly one tile (an x face by the tile of its left
+// cell, a y face by the tile of its lower cell).
+template <int TX, int TY, bool YDIR>
+HTX_HD bool recon_face_owned(int fi, int fj, int i0, int j0, int nx, int ny) {
+    if (!YDIR) return fi >= 1 && fi <= TX && fj >= 2 && fj <= TY + 1 && i0 + fi - 1 < nx && j0 + fj - 2 < ny;
+    return fj >= 1 && fj <= TY && fi >= 2 && fi <= TX + 1 && j0 + fj - 1 < ny && i0 + fi - 2 < nx;
 }
 
 // characteristic TENO5 on faces (FV_System2D.cpp:372-400).  YDIR selects the
 // face family; the state is read in (rho, normal, tangential, E) order.  With
 // LEARNED, faces the classifier flagged take Lagrange::linear
-// (Schemes_Lagrange.cpp:1219-1233) instead.
+// (Schemes_Lagrange.cpp:1219-1233) instead, provided the linear face values
+// give positive density and pressure on both sides (checked in FP64; else
+// TENO5); cnt accumulates (owned faces, flagged, linear) of this thread.
 template <int TX, int TY, bool YDIR, bool MIXED = false, bool LEARNED = false>
-HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
+HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid, int i0 = 0, int j0 = 0,
+                     unsigned* cnt = nullptr) {
     using T = Fv2dTile<TX, TY>;
     constexpr int NW = YDIR ? T::NYF : T::NXF;
     constexpr int RW = YDIR ? (TX + 4) : (TX + 1);   // faces per row of the face grid
@@ -142,15 +150,29 @@ HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
             int id = ca + (m - 2) * STEP;
             eig_left<YDIR>(E, s.sQ[0][id], s.sQ[1][id], s.sQ[2][id], s.sQ[3][id], c[0][m], c[1][m], c[2][m], c[3][m]);
         }
-        if (LEARNED && s.sFlag[YDIR ? 1 : 0][w]) {
+        bool lin = false;
+        if (LEARNED) {
+            const bool flag = s.sFlag[YDIR ? 1 : 0][w] != 0;
+            if (flag) {
+                double lm[4], lp[4];
 #pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                double um, up;
-                linear5(c[f], um, up);
-                s.sF[2 * f][w] = um;
-                s.sF[2 * f + 1][w] = up;
+                for (int f = 0; f < 4; ++f) linear5(c[f], lm[f], lp[f]);
+                lin = face_state_positive(E, a.g1, lm) && face_state_positive(E, a.g1, lp);
+                if (lin) {
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) {
+                        s.sF[2 * f][w] = lm[f];
+                        s.sF[2 * f + 1][w] = lp[f];
+                    }
+                }
             }
-        } else {
+            if (cnt && recon_face_owned<TX, TY, YDIR>(fi, fj, i0, j0, a.nx, a.ny)) {
+                cnt[0] += 1;
+                cnt[1] += flag ? 1 : 0;
+                cnt[2] += lin ? 1 : 0;
+            }
+        }
+        if (!lin) {
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
                 double um, up;

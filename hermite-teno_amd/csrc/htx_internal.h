@@ -15,9 +15,10 @@ struct Ctl {
     unsigned long long smax_x, smax_y;  // bit patterns of non-negative doubles
     int active;
     int nsteps;
-    // learned precision: faces classified and faces given the linear scheme
-    // (each face once per stage), since the last reset
-    unsigned long long n_faces, n_linear;
+    // learned precision, each face once per stage since the last reset: faces
+    // reconstructed, faces the classifier flagged smooth, flagged faces that
+    // kept the linear scheme (the rest failed the positivity check)
+    unsigned long long n_faces, n_flagged, n_linear;
 };
 
 enum FluxKind { FLUX_RUSANOV_CHAR = 0, FLUX_HLLC = 1, FLUX_HLLC_TORO = 2 };

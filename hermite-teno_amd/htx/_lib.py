@@ -90,7 +90,8 @@ SIGNATURES = {
     "htx_trainer_gradient": (_I, [_P, _D, _FP, _DP]),
     "htx_load_classifier": (_I, [_P, _FP, _D]),
     "htx_classify": (_I, [_P, _I, _FP, _FP]),
-    "htx_classifier_stats": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), _I]),
+    "htx_classifier_stats": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                                  ctypes.POINTER(ctypes.c_longlong), _I]),
     "htx_last_error": (ctypes.c_char_p, []),
     "htx_abi_version": (_I, []),
 }

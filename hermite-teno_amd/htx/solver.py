@@ -126,10 +126,12 @@ class System2D:
         check(_lib.load().htx_load_classifier(self._h, _lib.fptr(p), float(threshold)))
 
     def classifier_stats(self, reset=False):
-        """(faces classified, faces given the linear scheme) since the last reset."""
-        f, l = ctypes.c_longlong(), ctypes.c_longlong()
-        check(_lib.load().htx_classifier_stats(self._h, ctypes.byref(f), ctypes.byref(l), 1 if reset else 0))
-        return f.value, l.value
+        """(faces reconstructed, flagged smooth, took the linear scheme) since
+        the last reset, each face once per stage."""
+        f, g, l = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        check(_lib.load().htx_classifier_stats(self._h, ctypes.byref(f), ctypes.byref(g), ctypes.byref(l),
+                                               1 if reset else 0))
+        return f.value, g.value, l.value
 
     def classify(self, direction, features=False):
         """Logits of every x (direction 0) or y (1) face of the current state,

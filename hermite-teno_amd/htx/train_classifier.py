@@ -4,13 +4,16 @@ its parameters (htx/data/classifier_teno5.npy) and a report
 
 The training set is every x and y face of snapshots that the GPU solver
 itself produces (plain FP64 TENO5 + HLLC) of:
-  * the Sod shock tube along x (N = 400 cells, 16 rows, transmissive),
-  * the periodic isentropic vortex with seeded density modes on [0,10]^2,
-  * seeded four-quadrant Riemann problems on [0,1]^2 (transmissive),
+  * shock tubes (Sod, Lax, blast, Toro 123 and 5, a contact) along x and y,
+    1024 x 16 cells, transmissive,
+  * the periodic isentropic vortex with seeded density modes on [0,10]^2 at
+    three resolutions,
+  * seeded four-quadrant Riemann problems on [0,1]^2 (transmissive) at two
+    resolutions,
 labelled by whether plain TENO5 departs from the linear scheme on the face.
 The shock-plus-blobs workload is held out and used for the report.
 
-    python -m htx.train_classifier [--n 512] [--epochs 6] [--out DIR]
+    python -m htx.train_classifier [--n 512] [--epochs 30] [--out DIR]
 """
 import argparse
 import json
@@ -34,22 +37,54 @@ def _solver(w, nx, ny, precision="fp64"):
                     bc=w["bc"], precision=precision)
 
 
+# One-dimensional shock tubes (the 1D Euler shock-tube configuration and its
+# classic variants): primitive (rho, u_normal, p) left | right, split at 0.5,
+# end time.  Sod; Lax; the left half of Woodward-Colella's blast wave; Toro's
+# 123 problem (two rarefactions); Toro's test 5 (two strong shocks); a
+# stationary contact.
+SHOCK_TUBES = {
+    "sod": ((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 0.2),
+    "lax": ((0.445, 0.698, 3.528), (0.5, 0.0, 0.571), 0.14),
+    "blast": ((1.0, 0.0, 1000.0), (1.0, 0.0, 0.01), 0.012),
+    "toro123": ((1.0, -2.0, 0.4), (1.0, 2.0, 0.4), 0.15),
+    "toro5": ((5.99924, 19.5975, 460.894), (5.99242, -6.19633, 46.0950), 0.035),
+    "contact": ((1.4, 0.0, 1.0), (1.0, 0.0, 1.0), 0.2),
+}
+
+
+def tube_workload(name, along_y=False, n=1024, width=16):
+    """A shock tube as a quadrant problem whose second split line lies outside
+    the domain, so that only the split at 0.5 along the tube is seen."""
+    (rl, ul, pl), (rr, ur, pr), tend = SHOCK_TUBES[name]
+    if not along_y:
+        left, right = [rl, ul, 0.0, pl], [rr, ur, 0.0, pr]
+        return dict(kind="quadrant", xl=0.0, xr=1.0, yb=0.0, yt=width / n, gamma=1.4, cfl=0.5, tmax=tend,
+                    bc="zero_grad", xs=0.5, ys=-1.0, states=[left, right, left, right]), n, width
+    lo, hi = [rl, 0.0, ul, pl], [rr, 0.0, ur, pr]
+    return dict(kind="quadrant", xl=0.0, xr=width / n, yb=0.0, yt=1.0, gamma=1.4, cfl=0.5, tmax=tend,
+                bc="zero_grad", xs=-1.0, ys=0.5, states=[lo, lo, hi, hi]), width, n
+
+
 def sod_workload():
-    """Sod tube along x as a quadrant problem whose split line y = -1 lies
-    below the domain: k = (x >= 0.5) + 2 picks the left or the right state."""
-    left, right = [1.0, 0.0, 0.0, 1.0], [0.125, 0.0, 0.0, 0.1]
-    return dict(kind="quadrant", xl=0.0, xr=1.0, yb=0.0, yt=16.0 / 400.0, gamma=1.4, cfl=0.5, tmax=0.2,
-                bc="zero_grad", xs=0.5, ys=-1.0, states=[left, right, left, right])
+    """The Sod tube along x on 400 x 16 cells (dx = dy)."""
+    return tube_workload("sod", n=400)[0]
 
 
 def snapshot_plan(n):
     from .ic import SEED_BASE, quadrant_workload, vortex_workload
 
-    plan = [("sod", sod_workload(), 400, 16, [0.02 * k for k in range(1, 11)])]
-    plan.append(("vortex", vortex_workload(n), n, n, [0.0, 0.5, 1.0, 1.5, 2.0]))
-    for k in range(8):
-        w = quadrant_workload(n, seed=SEED_BASE + 3 + 1000 * k)
-        plan.append((f"quadrant{k}", w, n, n, [0.03, 0.08, 0.15, 0.25]))
+    plan = []
+    for name in SHOCK_TUBES:
+        for along_y in (False, True):
+            w, nx, ny = tube_workload(name, along_y)
+            plan.append((f"{name}{'-y' if along_y else '-x'}", w, nx, ny,
+                         [w["tmax"] * f for f in (0.2, 0.4, 0.6, 0.8, 1.0)]))
+    for m in (n // 4, n // 2, n):
+        plan.append((f"vortex{m}", vortex_workload(m), m, m, [0.0, 0.5, 1.0, 2.0]))
+    for m in (n // 2, n):
+        for k in range(8):
+            w = quadrant_workload(m, seed=SEED_BASE + 3 + 1000 * k + m)
+            plan.append((f"quadrant{k}-{m}", w, m, m, [0.03, 0.08, 0.15, 0.25]))
     return plan
 
 
@@ -93,10 +128,10 @@ def rates(c):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--troubled-weight", type=float, default=2.0)
+    ap.add_argument("--troubled-weight", type=float, default=4.0)
     ap.add_argument("--false-smooth", type=float, default=0.005,
                     help="largest share of troubled training faces the threshold may call smooth")
     ap.add_argument("--seed", type=int, default=7)
@@ -116,7 +151,7 @@ def main():
     tr.init(seed=args.seed)
     losses = []
     for ep in range(args.epochs):
-        lr = args.lr * (0.5 ** (ep // max(1, args.epochs // 3)))
+        lr = args.lr * 0.5 * (1.0 + np.cos(np.pi * ep / args.epochs))  # cosine decay
         l = tr.train(1, batch=args.batch, lr=lr, troubled_weight=args.troubled_weight, seed=args.seed + ep)
         losses.append(float(l[0]))
         log(f"[train] epoch {ep + 1}/{args.epochs} lr {lr:.2e} loss {l[0]:.5f} ({time.time() - t0:.1f} s)")

@@ -53,7 +53,8 @@ enum htx_bc { HTX_BC_ZERO_GRAD = 0, HTX_BC_PERIODIC = 1 };
  *   values, projections, fluxes and RK accumulation (TENO5 scheme).
  * HTX_PREC_LEARNED: FP32-mixed TENO5 where a learned classifier (see below)
  *   sends the faces it finds smooth to the optimal-weight linear scheme
- *   (Lagrange::linear) and the rest to TENO5; needs htx_load_classifier. */
+ *   (Lagrange::linear), guarded by an FP64 positivity check, and the rest
+ *   to TENO5; needs htx_load_classifier. */
 enum htx_precision { HTX_PREC_FP64 = 0, HTX_PREC_FP32_MIXED = 1, HTX_PREC_LEARNED = 2, HTX_PREC_FP64_FAST = 3 };
 
 typedef struct htx_params {
@@ -217,10 +218,12 @@ int htx_load_classifier(htx_solver* h, const float* params, double threshold);
  * matrix-core inference. */
 int htx_classify(htx_solver* h, int dir, float* logits, float* feat);
 
-/* Learned precision: faces classified (each face once per stage) and faces
- * that took the linear scheme, summed since the last reset; reset != 0 zeroes
- * both after reading. */
-int htx_classifier_stats(htx_solver* h, long long* n_faces, long long* n_linear, int reset);
+/* Learned precision, summed since the last reset (each face once per
+ * stage): faces reconstructed, faces the classifier flagged smooth, and
+ * flagged faces that took the linear scheme; a flagged face whose linear
+ * values back-project to a non-positive density or pressure (FP64 check)
+ * falls back to TENO5.  reset != 0 zeroes the counts after reading. */
+int htx_classifier_stats(htx_solver* h, long long* n_faces, long long* n_flagged, long long* n_linear, int reset);
 
 /* Message of the last failed call on this host thread. */
 const char* htx_last_error(void);
