@@ -87,6 +87,31 @@ def copy_calibration(mib=256):
     log(f"[calib] 3 copies of {mib} MiB device to device")
 
 
+def study():
+    """Shock-plus-blobs throughput at 4096^2 for each precision, and the
+    matched-accuracy study (L1 against a 4x finer FP64 solution) at reduced
+    sizes: shock plus blobs 512 x 128, the vortex at 256^2 to t = 2, the
+    quadrant problem at 256^2 to t = 0.25."""
+    from htx.ic import quadrant_workload, shock_blob_workload, vortex_workload
+    from htx.study import matched_accuracy, throughput
+
+    w4 = shock_blob_workload(4096)
+    sb = {"workload": f"planar Mach {w4['mach']:.2f} shock into 32 seeded Gaussian density blobs, 4096 x 4096 on "
+                      f"[0,4] x [0,1], TENO5 + HLLC, transmissive, 20 steps from t = 0 after 2 warm-up steps"}
+    for p in ("fp64", "fp64_fast", "fp32_mixed", "learned"):
+        log(f"[study] shock-blob throughput {p}")
+        sb[p] = throughput(w4, 4096, 4096, p, steps=20, warmup=2)
+    sb["learned_vs_fp64"] = sb["learned"]["value"] / sb["fp64"]["value"]
+    acc = {}
+    log("[study] matched accuracy: shock + blobs 512 x 128")
+    acc["shock_blobs_512x128"] = matched_accuracy(w4, 512, 128)
+    log("[study] matched accuracy: vortex 256^2")
+    acc["vortex_256"] = matched_accuracy(vortex_workload(256), 256, 256)
+    log("[study] matched accuracy: quadrant 256^2")
+    acc["quadrant_256"] = matched_accuracy(quadrant_workload(256), 256, 256)
+    return sb, acc
+
+
 def time_variant(n, steps, warmup, **kw):
     """Cell-updates/s of one solver variant on the same workload (for the
     report's `variants` block)."""
@@ -99,12 +124,20 @@ def time_variant(n, steps, warmup, **kw):
     s.isentropicVortex(w["centre"], w["modes"])
     s.steps_async(warmup)
     s.sync()
+    learned = kw.get("precision") == "learned"
+    if learned:
+        s.classifier_stats(reset=True)
     t0 = time.perf_counter()
     s.steps_async(steps)
     s.sync()
     el = time.perf_counter() - t0
+    r = {"value": n * n * steps / el, "unit": "cell-updates/s"}
+    if learned:
+        f, g, l = s.classifier_stats()
+        r["flagged_fraction"] = g / max(f, 1)
+        r["linear_fraction"] = l / max(f, 1)
     s.close()
-    return n * n * steps / el
+    return r
 
 
 def main():
@@ -120,6 +153,8 @@ def main():
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device copy (for scaling byte counters)")
+    ap.add_argument("--no-study", action="store_true",
+                    help="skip the shock-plus-blobs throughput and the matched-accuracy study")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -206,14 +241,19 @@ def main():
         out["variants"] = {}
         for name, kw in [("teno5_hllc_fp64_fast", dict(flux=args.flux, precision="fp64_fast")),
                          ("teno5_hllc_fp32_mixed", dict(flux=args.flux, precision="fp32_mixed")),
+                         ("teno5_hllc_learned", dict(flux=args.flux, precision="learned")),
                          ("hteno_hllc_fp64", dict(flux=args.flux, scheme="hteno")),
                          ("teno5_rusanov_fp64", dict(flux="rusanov"))]:
             try:
                 log(f"[gpu] variant {name}")
-                out["variants"][name] = {"value": time_variant(n, max(3, args.steps // 3), 1, **kw),
-                                         "unit": "cell-updates/s"}
+                out["variants"][name] = time_variant(n, max(3, args.steps // 3), 1, **kw)
             except Exception as e:
                 log(f"[gpu] variant {name} failed: {e}")
+    if not args.no_study:
+        try:
+            out["shock_blobs"], out["matched_accuracy"] = study()
+        except Exception as e:
+            log(f"[gpu] study failed: {e}")
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)

@@ -7,6 +7,7 @@ both sides for all four characteristic fields; the classifier (MLP
 differences and TENO smoothness contrasts of the face's 6-cell stencil.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -105,3 +106,17 @@ class Trainer:
         if want_logits:
             out["logits"] = lg
         return out
+
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+def default_classifier():
+    """(parameters, logit threshold) of the classifier shipped with the
+    package (htx/data/classifier_teno5.*, written by htx.train_classifier)."""
+    import json
+
+    p = np.load(os.path.join(DATA, "classifier_teno5.npy"), allow_pickle=False)
+    with open(os.path.join(DATA, "classifier_teno5.json")) as f:
+        thr = float(json.load(f)["threshold"])
+    return p.astype(np.float32), thr

@@ -19,7 +19,7 @@ class System2D:
 
     def __init__(self, xl, xr, nx, yb, yt, ny, cflnumber, tmax, tInit, specificHeatRatio, *,
                  scheme="teno5", flux="hllc", bc="zero_grad", precision="fp64", ref_quirks=True,
-                 tile=(0, 0)):
+                 tile=(0, 0), classifier=None):
         lib = _lib.load()
         p = _lib.HtxParams()
         check(lib.htx_params_default(ctypes.byref(p)))
@@ -44,6 +44,11 @@ class System2D:
         self.gamma = p.gamma
         self.tFinal = p.tmax
         self.exact = None
+        if precision == "learned":
+            from .learn import default_classifier
+
+            prm, thr = classifier if classifier is not None else default_classifier()
+            self.load_classifier(prm, thr)
 
     # -- lifetime ---------------------------------------------------------
     def close(self):

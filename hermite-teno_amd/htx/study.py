@@ -1,0 +1,84 @@
+"""Matched-accuracy study: solution error of each precision against a finer
+FP64 solution, and throughput at equal resolution.
+
+The error of a run at n x m cells is the relative L1 difference between its
+end state and a `factor`-times finer plain-FP64 run averaged back onto the
+n x m grid (block means); the finer run starts from the same workload
+generated at its own resolution.  Both runs stop at the workload's end time
+(or `tend`).
+"""
+import time
+
+import numpy as np
+
+from .solver import System2D
+
+
+def block_average(q, f):
+    nf, ny, nx = q.shape
+    return q.reshape(nf, ny // f, f, nx // f, f).mean(axis=(2, 4))
+
+
+def rel_l1(a, b):
+    return float(np.abs(a - b).sum() / np.abs(b).sum())
+
+
+def run(w, nx, ny, precision, tend, flux="hllc"):
+    """End state, wall seconds, steps, and for precision 'learned' the
+    classifier statistics (faces, flagged, linear)."""
+    s = System2D(w["xl"], w["xr"], nx, w["yb"], w["yt"], ny, w["cfl"], tend, 0.0, w["gamma"], flux=flux,
+                 bc=w["bc"], precision=precision)
+    s.init_workload(w)
+    t0 = time.perf_counter()
+    steps = s.timeStepping(tend)
+    s.sync()
+    el = time.perf_counter() - t0
+    stats = s.classifier_stats() if precision == "learned" else None
+    q = s.get_state()
+    s.close()
+    return q, el, steps, stats
+
+
+def matched_accuracy(w, nx, ny, factor=4, precisions=("fp64", "fp32_mixed", "learned"), tend=None, flux="hllc"):
+    tend = w["tmax"] if tend is None else tend
+    qf, elf, stf, _ = run(w, factor * nx, factor * ny, "fp64_fast", tend, flux)
+    ref = block_average(qf, factor)
+    out = {"reference": {"nx": factor * nx, "ny": factor * ny, "precision": "fp64_fast", "steps": stf,
+                         "seconds": elf}, "nx": nx, "ny": ny, "t_end": tend}
+    for p in precisions:
+        q, el, st, stats = run(w, nx, ny, p, tend, flux)
+        r = {"l1_rho": rel_l1(q[0], ref[0]), "l1_all": rel_l1(q[:4], ref[:4]), "steps": st, "seconds": el,
+             "finite": bool(np.isfinite(q).all())}
+        if stats is not None:
+            f, g, l = stats
+            r["flagged_fraction"] = g / max(f, 1)
+            r["linear_fraction"] = l / max(f, 1)
+            r["teno_fallback_fraction"] = 1.0 - l / max(f, 1)
+        out[p] = r
+    if "fp64" in out:
+        for p in precisions:
+            out[p]["l1_rho_vs_fp64"] = out[p]["l1_rho"] / out["fp64"]["l1_rho"]
+    return out
+
+
+def throughput(w, nx, ny, precision, steps=20, warmup=2, flux="hllc"):
+    """Cell-updates/s over `steps` SSP-RK3 steps from the workload's initial
+    state (plus, for 'learned', the flagged and linear face fractions)."""
+    s = System2D(w["xl"], w["xr"], nx, w["yb"], w["yt"], ny, w["cfl"], 1e30, 0.0, w["gamma"], flux=flux,
+                 bc=w["bc"], precision=precision)
+    s.init_workload(w)
+    s.steps_async(warmup)
+    s.sync()
+    if precision == "learned":
+        s.classifier_stats(reset=True)
+    t0 = time.perf_counter()
+    s.steps_async(steps)
+    s.sync()
+    el = time.perf_counter() - t0
+    r = {"value": nx * ny * steps / el, "unit": "cell-updates/s", "steps": steps}
+    if precision == "learned":
+        f, g, l = s.classifier_stats()
+        r["flagged_fraction"] = g / max(f, 1)
+        r["linear_fraction"] = l / max(f, 1)
+    s.close()
+    return r
