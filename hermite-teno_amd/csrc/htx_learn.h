@@ -71,8 +71,12 @@ HTX_HD float teno_contrast(double v0, double v1, double v2, double v3, double v4
     float b5 = e1 * t1 + e2 * t2 + e3 * t3 + e4 * t4;
     float tau = fabsf(b5 - (s0 + s2 + 4.0f * s1) * (1.0f / 6.0f));
     float lo = fminf(fminf(s0, s1), s2) + eps, hi = fmaxf(fmaxf(s0, s1), s2) + eps;
-    // 1 - 1/R = 1 - lo (hi + tau) / ((lo + tau) hi)
-    float r = 1.0f - (lo * (hi + tau)) / ((lo + tau) * hi);
+    // 1/R = (1 + tau/hi) / (1 + tau/lo), kept free of overflow: when tau/hi
+    // overflows every candidate is flat against tau (R = 1); when only tau/lo
+    // does, 1/R = 0
+    float x = tau / hi, y = tau / lo;
+    float inv = (x - x != 0.0f) ? 1.0f : (1.0f + x) / (1.0f + y);
+    float r = 1.0f - inv;
     return r > 0.0f ? r : 0.0f;
 }
 

@@ -173,18 +173,31 @@ HTX_HD double teno5_side_mixed(double v0, double v1, double v2, double v3, doubl
     float t4 = m14 * e1 + m13 * e2 + m12 * e3 + m11 * e4;
     float b5 = e1 * t1 + e2 * t2 + e3 * t3 + e4 * t4;
     float tau = fabsf(b5 - (s0 + s2 + 4.0f * s1) * (1.0f / 6.0f));
-    float g0 = 1.0f + tau / (s0 + 1e-30f);
-    float g1 = 1.0f + tau / (s1 + 1e-30f);
-    float g2 = 1.0f + tau / (s2 + 1e-30f);
-    g0 = g0 * g0 * g0; g0 = g0 * g0;
-    g1 = g1 * g1 * g1; g1 = g1 * g1;
-    g2 = g2 * g2 * g2; g2 = g2 * g2;
-    float gs = (g0 + g1) + g2;
-    // g/gs < C_T  <=>  g < C_T gs, including the overflow cases: an infinite
-    // g against an infinite sum is kept (inf/inf = NaN compares false), a
-    // finite one is cut, and NaN keeps every stencil
-    float lim = 5e-4f * gs;
-    bool k0 = !(g0 < lim), k1 = !(g1 < lim), k2 = !(g2 < lim);
+    // a_k = 1 + tau / (beta_k + eps); the cut-off compares a_k^6 / sum_j a_j^6
+    // with C_T.  a_k^6 overflows FP32 long before FP64, so the rule is
+    // evaluated on r_k = a_k / max_j a_j <= 1: a_k^6 / sum a_j^6 = r_k^6 / sum r_j^6.
+    // Candidates whose a_k is infinite (tau / eps overflows: a flat candidate
+    // next to a jump) dominate every finite one; NaN keeps every stencil.
+    float a0 = 1.0f + tau / (s0 + 1e-30f);
+    float a1 = 1.0f + tau / (s1 + 1e-30f);
+    float a2 = 1.0f + tau / (s2 + 1e-30f);
+    float am = fmaxf(fmaxf(a0, a1), a2);
+    bool k0, k1, k2;
+    if (am - am != 0.0f && am == am) {  // am is infinite
+        k0 = a0 == am;
+        k1 = a1 == am;
+        k2 = a2 == am;
+    } else {
+        float im = 1.0f / am;
+        float r0 = a0 * im, r1 = a1 * im, r2 = a2 * im;
+        r0 = r0 * r0 * r0; r0 = r0 * r0;
+        r1 = r1 * r1 * r1; r1 = r1 * r1;
+        r2 = r2 * r2 * r2; r2 = r2 * r2;
+        float lim = 5e-4f * ((r0 + r1) + r2);
+        k0 = !(r0 < lim);
+        k1 = !(r1 < lim);
+        k2 = !(r2 < lim);
+    }
     double w0 = k0 ? d0 : 0.0, w1 = k1 ? d1 : 0.0, w2 = k2 ? d2 : 0.0;
     double ws = (w0 + w1) + w2;
     double out = ((p0 * w0 + p1 * w1) + p2 * w2) * (1.0 / ws);

@@ -72,3 +72,12 @@ extern "C" int fv2dh_rhs(int nx, int ny, double xl, double xr, double yb, double
                          int quirks, const double* q, double* rhs) {
     return fv2dh_rhs_prec(nx, ny, xl, xr, yb, yt, gamma, bc, flux, quirks, 0, q, rhs);
 }
+
+// one TENO5 evaluation (u_{i-2..i+3} -> u^-, u^+) with the FP64 rule or the
+// FP32-mixed rule of the kernels
+extern "C" void fv2dh_teno5(const double* u, int mixed, double* out) {
+    if (mixed)
+        teno5_mixed(u, out[0], out[1]);
+    else
+        teno5(u, out[0], out[1]);
+}

@@ -50,7 +50,10 @@ def teno_contrast(v0, v1, v2, v3, v4, eps):
     tau = np.abs(b5 - (s0 + s2 + F(4.0) * s1) * (F(1.0) / F(6.0)))
     lo = np.minimum(np.minimum(s0, s1), s2) + eps
     hi = np.maximum(np.maximum(s0, s1), s2) + eps
-    r = F(1.0) - (lo * (hi + tau)) / ((lo + tau) * hi)
+    with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+        x, y = tau / hi, tau / lo
+        inv = np.where(np.isfinite(x), (F(1.0) + x) / (F(1.0) + y), F(1.0))
+    r = F(1.0) - inv.astype(np.float32)
     return np.where(r > 0, r, F(0.0))
 
 

@@ -158,3 +158,34 @@ def test_fp32_mixed_residual_close_to_fp64(orc, harness_fast, flux):
                                     {"rusanov": 0, "hllc": 1}[flux], 1, 1, q.ctypes.data_as(DP), r.ctypes.data_as(DP))
         rc = orc.fv2d_rhs(prm, q)
         assert np.abs(r - rc).sum() / np.abs(rc).sum() < 1e-8
+
+
+def test_fp32_mixed_cutoff_never_overflows(harness_fast):
+    """The FP32 cut-off rule on (nearly) flat stencils: the raw weights
+    (1 + tau / (beta + eps))^6 can overflow FP32 there (their sum can, even
+    when each one fits), which once cut every candidate and returned
+    catchnan's 1e-15.  The rule is evaluated on ratios to the largest
+    weight, so the face value stays the flat value."""
+    harness_fast.fv2dh_teno5.argtypes = [DP, ctypes.c_int, DP]
+    rng = np.random.default_rng(9)
+    out = np.empty(2)
+    worst = 0.0
+    for k in range(20000):
+        base = 10.0 ** rng.uniform(-3, 3)
+        u = base * (1.0 + rng.integers(-2, 3, 6) * 2.0 ** -52)
+        harness_fast.fv2dh_teno5(u.ctypes.data_as(DP), 1, out.ctypes.data_as(DP))
+        worst = max(worst, np.abs(out - base).max() / base)
+    assert worst < 1e-14
+    # stencils from a shock-plus-blobs state ahead of the shock: flat up to a
+    # ramp of relative size 1e-12 .. 1e-9, where every candidate was cut
+    for u in ([2.1450916869464454, 2.1450916869464436, 2.1450916869464436, 2.145091686946442,
+               2.1450916869431627, 2.1450916860808373],
+              [2.1450916869464542, 2.1450916869464542, 2.1450916869464454, 2.1450916869464454,
+               2.1450916869464542, 2.1450916869313623],
+              [2.1450916869464507, 2.145091686946447, 2.1450916869464454, 2.1450916869464525,
+               2.1450916869313623, 2.1450916851272535]):
+        u = np.array(u)
+        ref = np.empty(2)
+        harness_fast.fv2dh_teno5(u.ctypes.data_as(DP), 0, ref.ctypes.data_as(DP))
+        harness_fast.fv2dh_teno5(u.ctypes.data_as(DP), 1, out.ctypes.data_as(DP))
+        assert np.all(np.abs(out - ref) < 1e-9 * u[0]), (out, ref)
