@@ -77,6 +77,7 @@ struct htx_solver {
     // FP32 biases and output weights, logit threshold
     unsigned short* dFrag = nullptr;
     float* dCvec = nullptr;
+    unsigned char* dFlags = nullptr;  // per-tile recon-face flags (learned precision)
     float cls_threshold = 0.0f;
     bool have_classifier = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -122,6 +123,8 @@ Fv2dArgs stage_args(htx_solver* h, const double* qin, const double* q0, double* 
     a.cls_frag = h->dFrag;
     a.cls_cvec = h->dCvec;
     a.cls_thr = h->cls_threshold;
+    a.cls_flags = h->dFlags;
+    a.cls_fresh = mode <= 1 ? 1 : 0;
     a.gam = h->p.gamma;
     a.g1 = h->p.gamma - 1.0;
     a.dx = h->dx;
@@ -285,6 +288,10 @@ int htx_create(const htx_params* p, htx_solver** out) {
         htx_destroy(h);
         return fail(HTX_ENOMEM, "device allocation failed");
     }
+    if (p->precision == HTX_PREC_LEARNED && hipMalloc(&h->dFlags, fv2d_flag_bytes(p->nx, p->ny)) != hipSuccess) {
+        htx_destroy(h);
+        return fail(HTX_ENOMEM, "device allocation failed");
+    }
     if (p->scheme == HTX_SCHEME_HTENO) {
         size_t nxf = (size_t)(p->nx + 1) * p->ny, nyf = (size_t)p->nx * (p->ny + 1);
         if (hipMalloc(&h->dLV, sizeof(double) * 8 * (size_t)h->fstride) != hipSuccess ||
@@ -328,6 +335,7 @@ void htx_destroy(htx_solver* h) {
         if (b) (void)hipFree(b);
     if (h->dFrag) (void)hipFree(h->dFrag);
     if (h->dCvec) (void)hipFree(h->dCvec);
+    if (h->dFlags) (void)hipFree(h->dFlags);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }

@@ -42,7 +42,19 @@ __global__ void __launch_bounds__(256, 2) fv2d_stage_kernel(Fv2dArgs a) {
     ph_load<TX, TY>(a, s, i0, j0, tid);
     __syncthreads();
     unsigned cnt[3] = {0, 0, 0};
-    if (LEARNED) ph_classify<TX, TY>(a, s, tid);
+    if (LEARNED) {
+        // the classifier runs in the first stage of a step (and for a lone
+        // residual); stages 2 and 3 reuse the flags it stored for this tile
+        unsigned char* fl = a.cls_flags + (size_t)tile * sizeof(s.sFlag);
+        unsigned char* sf = &s.sFlag[0][0];
+        if (a.cls_fresh) {
+            ph_classify<TX, TY>(a, s, tid);
+            for (int k = tid; k < (int)sizeof(s.sFlag); k += T::NT) fl[k] = sf[k];
+        } else {
+            for (int k = tid; k < (int)sizeof(s.sFlag); k += T::NT) sf[k] = fl[k];
+            __syncthreads();
+        }
+    }
     ph_recon<TX, TY, false, MIXED, LEARNED>(a, s, tid, i0, j0, cnt);
     __syncthreads();
     ph_flux<TX, TY, false, FLUX, MIXED>(a, s, tid);
@@ -218,11 +230,17 @@ static hipError_t launch_tile(const Fv2dArgs& a0, int flux, hipStream_t s) {
     return hipGetLastError();
 }
 
+size_t fv2d_flag_bytes(int nx, int ny) {
+    using T = Fv2dTile<16, 16>;
+    size_t tiles = (size_t)((nx + 15) / 16) * (size_t)((ny + 15) / 16);
+    return tiles * sizeof(((T*)nullptr)->sFlag);
+}
+
 hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y, hipStream_t s) {
     (void)tile_x;
     (void)tile_y;
     if (a.learned) {
-        if (!a.cls_frag || !a.cls_cvec) return hipErrorInvalidValue;
+        if (!a.cls_frag || !a.cls_cvec || !a.cls_flags) return hipErrorInvalidValue;
         return launch_tile<16, 16, true, true>(a, flux, s);
     }
     if (a.mixed) return launch_tile<16, 16, true, false>(a, flux, s);

@@ -41,6 +41,8 @@ struct Fv2dArgs {
     const unsigned short* cls_frag;  // packed bf16 classifier weights
     const float* cls_cvec;           // FP32 biases and output weights
     float cls_thr;                   // logit above which a face is smooth
+    unsigned char* cls_flags;        // per tile: the recon-face flags of the step's first stage
+    int cls_fresh;                   // 1: classify and store the flags; 0: reuse the stored ones
 };
 
 struct VortexArgs {
@@ -85,6 +87,8 @@ HTX_HD int map_index(int g, int n, int bc) {
 
 // launchers (htx_fv2d.hip)
 hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y, hipStream_t s);
+// bytes of the learned precision's per-tile face-flag buffer for an nx x ny grid
+size_t fv2d_flag_bytes(int nx, int ny);
 hipError_t launch_dt(Ctl* c, double dx, double dy, hipStream_t s);
 hipError_t launch_tadvance(Ctl* c, hipStream_t s);
 hipError_t launch_reset_smax(Ctl* c, hipStream_t s);

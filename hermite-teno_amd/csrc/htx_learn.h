@@ -27,41 +27,47 @@ constexpr int CLS_W3 = CLS_B2 + CLS_H;
 constexpr int CLS_B3 = CLS_W3 + CLS_H;
 constexpr int CLS_NPARAM = CLS_B3 + 1;  // 5057
 
-// GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))
-// = x * sigmoid(2 sqrt(2/pi) (x + 0.044715 x^3)).
-constexpr float GELU_A = 1.5957691216057308f;   // 2 sqrt(2/pi)
-constexpr float GELU_B = 0.07135481627466469f;  // 2 sqrt(2/pi) * 0.044715
+// GELU, x * Phi(x), with the normal CDF Phi replaced by the cubic
+// 0.5 + t (A + B t^2) of t = clamp(x, -X0, X0): it has Phi's value and slope
+// at 0 (A = 1/sqrt(2 pi)) and reaches 0 and 1 with zero slope at -X0 and X0
+// (X0 = 0.75 sqrt(2 pi), B = -1/(4 X0^3)), so the activation is C1; it
+// differs from Phi by at most 0.039.  Five vector instructions and no
+// transcendental; training and inference use the same function.
+constexpr float GELU_A = 0.3989422804014327f;
+constexpr float GELU_B = -0.037625858331402064f;
+constexpr float GELU_X0 = 1.8799712059732503f;
 
 HTX_HD float gelu_f(float x) {
-    float t = x * (GELU_A + GELU_B * x * x);
-    return x / (1.0f + expf(-t));
+    float t = fminf(fmaxf(x, -GELU_X0), GELU_X0);
+    return x * (0.5f + t * (GELU_A + GELU_B * t * t));
 }
 
 // (gelu(x), gelu'(x))
 HTX_HD void gelu_fd(float x, float& g, float& dg) {
-    float t = x * (GELU_A + GELU_B * x * x);
-    float s = 1.0f / (1.0f + expf(-t));
-    g = x * s;
-    dg = s + x * s * (1.0f - s) * (GELU_A + 3.0f * GELU_B * x * x);
+    float t = fminf(fmaxf(x, -GELU_X0), GELU_X0);
+    float phi = 0.5f + t * (GELU_A + GELU_B * t * t);
+    g = x * phi;
+    bool in = x > -GELU_X0 && x < GELU_X0;
+    dg = phi + (in ? x * (GELU_A + 3.0f * GELU_B * t * t) : 0.0f);
 }
 
-// TENO contrast of one 5-value window: with TENO's smoothness measures
-// beta_k of the three 3-cell candidates and tau = |beta5 - (beta0 + beta2 +
-// 4 beta1) / 6|, the weights (1 + tau / (beta_k + eps))^6 of the best and the
-// worst candidate differ by the ratio R = (1 + tau/(bmin+eps)) / (1 + tau/
-// (bmax+eps)) >= 1.  Returns 1 - 1/R in [0, 1): 0 on a polynomial of degree
-// <= 2 (tau = 0), near 1 across a jump.  TENO5 cuts a candidate roughly when
-// R^6 exceeds 1 / (3 C_T), i.e. for 1 - 1/R above about 0.66.  FP32, on
-// differences formed in FP64 (beta5 in the difference form of teno5_side_mixed).
-HTX_HD float teno_contrast(double v0, double v1, double v2, double v3, double v4, float eps) {
+// TENO contrast of one 5-value window given its first differences
+// e_k = v_k - v_{k-1} (k = 1..4).  With TENO's smoothness measures beta_k of
+// the three 3-cell candidates and tau = |beta5 - (beta0 + beta2 + 4 beta1)/6|
+// (Schemes_Lagrange.cpp:1013-1034, beta5 in the difference form of
+// teno5_side_mixed), the weights (1 + tau/(beta_k + eps))^6 of the best and the
+// worst candidate differ by the ratio R = (1 + tau/(bmin+eps)) /
+// (1 + tau/(bmax+eps)) >= 1.  Returns 1 - 1/R in [0, 1): 0 on a polynomial of
+// degree <= 2 (tau = 0), near 1 across a jump.  TENO5 cuts a candidate roughly
+// when R^6 exceeds 1 / (3 C_T), i.e. for 1 - 1/R above about 0.66.  FP32.
+HTX_HD float teno_contrast(float e1, float e2, float e3, float e4, float eps) {
     const float c1 = 13.0f / 12.0f;
-    float c_0 = (float)(v2 - 2.0 * v3 + v4), b_0 = (float)((3.0 * v2 - 4.0 * v3 + v4) * 0.5);
-    float c_1 = (float)(v1 - 2.0 * v2 + v3), b_1 = (float)((v1 - v3) * 0.5);
-    float c_2 = (float)(v0 - 2.0 * v1 + v2), b_2 = (float)((v0 - 4.0 * v1 + 3.0 * v2) * 0.5);
+    float c_0 = e4 - e3, b_0 = (e4 - 3.0f * e3) * 0.5f;
+    float c_1 = e3 - e2, b_1 = -(e2 + e3) * 0.5f;
+    float c_2 = e2 - e1, b_2 = (3.0f * e2 - e1) * 0.5f;
     float s0 = b_0 * b_0 + c1 * c_0 * c_0;
     float s1 = b_1 * b_1 + c1 * c_1 * c_1;
     float s2 = b_2 * b_2 + c1 * c_2 * c_2;
-    float e1 = (float)(v1 - v0), e2 = (float)(v2 - v1), e3 = (float)(v3 - v2), e4 = (float)(v4 - v3);
     const float m11 = 1727.0f / 1260.0f, m22 = 2029.0f / 168.0f, m12 = -2479.0f / 672.0f;
     const float m13 = 5123.0f / 1680.0f, m14 = -8209.0f / 10080.0f, m23 = -12157.0f / 1120.0f;
     float t1 = m11 * e1 + m12 * e2 + m13 * e3 + m14 * e4;
@@ -74,21 +80,24 @@ HTX_HD float teno_contrast(double v0, double v1, double v2, double v3, double v4
     // 1/R = (1 + tau/hi) / (1 + tau/lo), kept free of overflow: when tau/hi
     // overflows every candidate is flat against tau (R = 1); when only tau/lo
     // does, 1/R = 0
-    float x = tau / hi, y = tau / lo;
-    float inv = (x - x != 0.0f) ? 1.0f : (1.0f + x) / (1.0f + y);
+    float x = tau * frcp(hi), y = tau * frcp(lo);
+    float inv = (x - x != 0.0f) ? 1.0f : (1.0f + x) * frcp(1.0f + y);
     float r = 1.0f - inv;
     return r > 0.0f ? r : 0.0f;
 }
 
-// Contrast of a 6-cell stencil: the larger of its two 5-cell windows (the
-// windows of u^- and u^+ at the face).
-HTX_HD float stencil_contrast(const double f[6], float eps) {
-    return fmaxf(teno_contrast(f[0], f[1], f[2], f[3], f[4], eps), teno_contrast(f[1], f[2], f[3], f[4], f[5], eps));
+// Contrast of a 6-cell stencil from its five differences: the larger of its
+// two 5-cell windows (the windows of u^- and u^+ at the face).
+HTX_HD float stencil_contrast(const float d[5], float eps) {
+    return fmaxf(teno_contrast(d[0], d[1], d[2], d[3], eps), teno_contrast(d[1], d[2], d[3], d[4], eps));
 }
+
+HTX_HD float log_amp(float amp) { return flog2(amp + 1e-12f) * (0.30102999566398120f / 12.0f) + 1.0f; }
 
 // The 12 classifier inputs of a face from the primitive values of its
 // stencil cells c-2..c+3 (r = density, un / ut = velocity normal / tangential
-// to the face, p = pressure).  Scales: rf, pf = face averages of |r|, |p|,
+// to the face, p = pressure).  Differences are formed in FP64, everything
+// after them in FP32.  Scales: rf, pf = face averages of |r|, |p|,
 // cf = sqrt(gamma pf / rf).
 //   x[0..1]   density differences r_c - r_{c-1}, r_{c+1} - r_c over
 //             (sum of the five |differences| + 1e-6 rf)
@@ -97,37 +106,40 @@ HTX_HD float stencil_contrast(const double f[6], float eps) {
 //             ut (scales rf, pf, cf, cf): 0 for a flat stencil, about 1 for
 //             an O(1) jump
 //   x[8..11]  TENO contrasts of r, p, un, ut (eps = 1e-12 scale^2)
-HTX_HD float log_amp(double amp) { return log10f((float)(amp + 1e-12)) * (1.0f / 12.0f) + 1.0f; }
-
 HTX_HD void face_features(const double r[6], const double un[6], const double ut[6], const double p[6],
                           double gam, float x[CLS_IN]) {
-    double rf = 0.5 * (fabs(r[2]) + fabs(r[3])), pf = 0.5 * (fabs(p[2]) + fabs(p[3]));
-    double cf2 = gam * pf / (rf + 1e-300), cf = sqrt(cf2);
-    double sr = 0.0, sp = 0.0, ar = 0.0, ap = 0.0, au = 0.0, at = 0.0;
+    float dr[5], dp[5], du[5], dv[5];
+    float sr = 0.0f, sp = 0.0f, ar = 0.0f, ap = 0.0f, au = 0.0f, at = 0.0f;
     for (int k = 0; k < 5; ++k) {
-        double dr = fabs(r[k + 1] - r[k]), dp = fabs(p[k + 1] - p[k]);
-        sr += dr;
-        sp += dp;
-        ar = fmax(ar, dr);
-        ap = fmax(ap, dp);
-        au = fmax(au, fabs(un[k + 1] - un[k]));
-        at = fmax(at, fabs(ut[k + 1] - ut[k]));
+        dr[k] = (float)(r[k + 1] - r[k]);
+        dp[k] = (float)(p[k + 1] - p[k]);
+        du[k] = (float)(un[k + 1] - un[k]);
+        dv[k] = (float)(ut[k + 1] - ut[k]);
+        sr += fabsf(dr[k]);
+        sp += fabsf(dp[k]);
+        ar = fmaxf(ar, fabsf(dr[k]));
+        ap = fmaxf(ap, fabsf(dp[k]));
+        au = fmaxf(au, fabsf(du[k]));
+        at = fmaxf(at, fabsf(dv[k]));
     }
-    double nr = 1.0 / (sr + 1e-6 * rf + 1e-300), np = 1.0 / (sp + 1e-6 * pf + 1e-300);
-    x[0] = (float)((r[2] - r[1]) * nr);
-    x[1] = (float)((r[3] - r[2]) * nr);
-    x[2] = (float)((p[2] - p[1]) * np);
-    x[3] = (float)((p[3] - p[2]) * np);
-    x[4] = log_amp(ar / (rf + 1e-300));
-    x[5] = log_amp(ap / (pf + 1e-300));
-    x[6] = log_amp(au / (cf + 1e-300));
-    x[7] = log_amp(at / (cf + 1e-300));
-    float er = (float)(1e-12 * rf * rf) + 1e-30f, ep = (float)(1e-12 * pf * pf) + 1e-30f;
-    float ev = (float)(1e-12 * cf2) + 1e-30f;
-    x[8] = stencil_contrast(r, er);
-    x[9] = stencil_contrast(p, ep);
-    x[10] = stencil_contrast(un, ev);
-    x[11] = stencil_contrast(ut, ev);
+    float rf = (float)(0.5 * (fabs(r[2]) + fabs(r[3]))), pf = (float)(0.5 * (fabs(p[2]) + fabs(p[3])));
+    float irf = frcp(rf + 1e-30f), ipf = frcp(pf + 1e-30f);
+    float cf2 = (float)gam * pf * irf;
+    float icf = frcp(sqrtf(cf2) + 1e-30f);
+    float nr = frcp(sr + 1e-6f * rf + 1e-30f), np = frcp(sp + 1e-6f * pf + 1e-30f);
+    x[0] = dr[1] * nr;
+    x[1] = dr[2] * nr;
+    x[2] = dp[1] * np;
+    x[3] = dp[2] * np;
+    x[4] = log_amp(ar * irf);
+    x[5] = log_amp(ap * ipf);
+    x[6] = log_amp(au * icf);
+    x[7] = log_amp(at * icf);
+    x[8] = stencil_contrast(dr, 1e-12f * rf * rf + 1e-30f);
+    x[9] = stencil_contrast(dp, 1e-12f * pf * pf + 1e-30f);
+    float ev = 1e-12f * cf2 + 1e-30f;
+    x[10] = stencil_contrast(du, ev);
+    x[11] = stencil_contrast(dv, ev);
 }
 
 // Primitive (r, un, ut, p) of a conserved state (rho, m_n, m_t, E).
@@ -243,6 +255,7 @@ typedef float cls_f4 __attribute__((ext_vector_type(4)));
 // cls_logit).
 __device__ __forceinline__ void cls_stage_features(unsigned short* xs, const float x[CLS_IN]) {
     const int lane = threadIdx.x & 63;
+#pragma unroll
     for (int k = 0; k < 16; ++k) xs[lane * 16 + k] = k < CLS_IN ? f32_to_bf16(x[k]) : (unsigned short)0;
 }
 
@@ -252,22 +265,27 @@ __device__ __forceinline__ float cls_logit(const unsigned short* xs, const unsig
     const cls_s4* F1 = reinterpret_cast<const cls_s4*>(frag);
     const cls_b8* F2 = reinterpret_cast<const cls_b8*>(frag + CLS_FRAG1);
     float zmine = 0.0f;
+#pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
         cls_s4 bx = *reinterpret_cast<const cls_s4*>(xs + (16 * nb + i) * 16 + 4 * g);
         cls_b8 b2[2];
+#pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
             cls_f4 d = {0.0f, 0.0f, 0.0f, 0.0f};
             d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F1[mb * 64 + lane], bx, d, 0, 0, 0);
+#pragma unroll
             for (int c = 0; c < 4; ++c) {
                 float h = gelu_f(d[c] + cvec[16 * mb + 4 * g + c]);
                 b2[mb >> 1][(mb & 1) * 4 + c] = (__bf16)h;
             }
         }
         float part = 0.0f;
+#pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
             cls_f4 d = {0.0f, 0.0f, 0.0f, 0.0f};
             d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F2[(mb * 2 + 0) * 64 + lane], b2[0], d, 0, 0, 0);
             d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F2[(mb * 2 + 1) * 64 + lane], b2[1], d, 0, 0, 0);
+#pragma unroll
             for (int c = 0; c < 4; ++c) {
                 int u = 16 * mb + 4 * g + c;
                 part += cvec[2 * CLS_H + u] * gelu_f(d[c] + cvec[CLS_H + u]);

@@ -99,6 +99,25 @@ inline void fill_quad_consts(QuadConsts& C) {
     }
 }
 
+// Fast FP32 reciprocal and base-2 logarithm on the device (one hardware
+// instruction each, about 1 ulp), used only by the FP32 decision paths;
+// exact forms on the host.
+HTX_HD float frcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
+HTX_HD float flog2(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_logf(x);
+#else
+    return log2f(x);
+#endif
+}
+
 struct Prim {  // per-cell quantities used by the Roe average
     double sr, u, v, h;
 };
@@ -157,14 +176,18 @@ HTX_HD double teno5_side(double v0, double v1, double v2, double v3, double v4,
 // exactly flat stencils, where every candidate coincides.
 HTX_HD double teno5_side_mixed(double v0, double v1, double v2, double v3, double v4, double p0, double p1,
                                double p2, double d0, double d1, double d2) {
+    // smoothness measures from the first differences e_k = v_k - v_{k-1}
+    // (formed in FP64, then FP32): v2 - 2 v3 + v4 = e4 - e3,
+    // (3 v2 - 4 v3 + v4)/2 = (e4 - 3 e3)/2, (v1 - v3)/2 = -(e2 + e3)/2,
+    // (v0 - 4 v1 + 3 v2)/2 = (3 e2 - e1)/2
     const float c1 = 13.0f / 12.0f;
-    float c_0 = (float)(v2 - 2.0 * v3 + v4), b_0 = (float)((3.0 * v2 - 4.0 * v3 + v4) * 0.5);
-    float c_1 = (float)(v1 - 2.0 * v2 + v3), b_1 = (float)((v1 - v3) * 0.5);
-    float c_2 = (float)(v0 - 2.0 * v1 + v2), b_2 = (float)((v0 - 4.0 * v1 + 3.0 * v2) * 0.5);
+    float e1 = (float)(v1 - v0), e2 = (float)(v2 - v1), e3 = (float)(v3 - v2), e4 = (float)(v4 - v3);
+    float c_0 = e4 - e3, b_0 = (e4 - 3.0f * e3) * 0.5f;
+    float c_1 = e3 - e2, b_1 = -(e2 + e3) * 0.5f;
+    float c_2 = e2 - e1, b_2 = (3.0f * e2 - e1) * 0.5f;
     float s0 = b_0 * b_0 + c1 * c_0 * c_0;
     float s1 = b_1 * b_1 + c1 * c_1 * c_1;
     float s2 = b_2 * b_2 + c1 * c_2 * c_2;
-    float e1 = (float)(v1 - v0), e2 = (float)(v2 - v1), e3 = (float)(v3 - v2), e4 = (float)(v4 - v3);
     const float m11 = 1727.0f / 1260.0f, m22 = 2029.0f / 168.0f, m12 = -2479.0f / 672.0f;
     const float m13 = 5123.0f / 1680.0f, m14 = -8209.0f / 10080.0f, m23 = -12157.0f / 1120.0f;
     float t1 = m11 * e1 + m12 * e2 + m13 * e3 + m14 * e4;
@@ -178,9 +201,9 @@ HTX_HD double teno5_side_mixed(double v0, double v1, double v2, double v3, doubl
     // evaluated on r_k = a_k / max_j a_j <= 1: a_k^6 / sum a_j^6 = r_k^6 / sum r_j^6.
     // Candidates whose a_k is infinite (tau / eps overflows: a flat candidate
     // next to a jump) dominate every finite one; NaN keeps every stencil.
-    float a0 = 1.0f + tau / (s0 + 1e-30f);
-    float a1 = 1.0f + tau / (s1 + 1e-30f);
-    float a2 = 1.0f + tau / (s2 + 1e-30f);
+    float a0 = 1.0f + tau * frcp(s0 + 1e-30f);
+    float a1 = 1.0f + tau * frcp(s1 + 1e-30f);
+    float a2 = 1.0f + tau * frcp(s2 + 1e-30f);
     float am = fmaxf(fmaxf(a0, a1), a2);
     bool k0, k1, k2;
     if (am - am != 0.0f && am == am) {  // am is infinite
@@ -188,7 +211,7 @@ HTX_HD double teno5_side_mixed(double v0, double v1, double v2, double v3, doubl
         k1 = a1 == am;
         k2 = a2 == am;
     } else {
-        float im = 1.0f / am;
+        float im = frcp(am);
         float r0 = a0 * im, r1 = a1 * im, r2 = a2 * im;
         r0 = r0 * r0 * r0; r0 = r0 * r0;
         r1 = r1 * r1 * r1; r1 = r1 * r1;
@@ -336,19 +359,24 @@ HTX_HD void weno3q(const QuadConsts& C, double u0, double u1, double u2, double 
 // combination stays exact for constant data.
 HTX_HD void weno3q_mixed(const QuadConsts& C, const float* lw, double u0, double u1, double u2, double u3,
                          double u4, double& o0, double& o1, double& o2) {
+    // smoothness measures from the first differences d_k = u_k - u_{k-1}
+    // (formed in FP64, then FP32): u0 - 4 u1 + 3 u2 = 3 d2 - d1,
+    // u0 - 2 u1 + u2 = d2 - d1, u1 - u3 = -(d2 + d3), u1 - 2 u2 + u3 = d3 - d2,
+    // 3 u2 - 4 u3 + u4 = d4 - 3 d3, u2 - 2 u3 + u4 = d4 - d3
     const float c1 = 13.0f / 12.0f;
-    float bg0 = (float)(u0 - 4.0 * u1 + 3.0 * u2), cg0 = (float)(u0 - 2.0 * u1 + u2);
-    float bg1 = (float)(u1 - u3), cg1 = (float)(u1 - 2.0 * u2 + u3);
-    float bg2 = (float)(3.0 * u2 - 4.0 * u3 + u4), cg2 = (float)(u2 - 2.0 * u3 + u4);
+    float d1 = (float)(u1 - u0), d2 = (float)(u2 - u1), d3 = (float)(u3 - u2), d4 = (float)(u4 - u3);
+    float bg0 = 3.0f * d2 - d1, cg0 = d2 - d1;
+    float bg1 = -(d2 + d3), cg1 = d3 - d2;
+    float bg2 = d4 - 3.0f * d3, cg2 = d4 - d3;
     float e0 = 1e-6f + (0.25f * (bg0 * bg0) + c1 * (cg0 * cg0));
     float e1 = 1e-6f + (0.25f * (bg1 * bg1) + c1 * (cg1 * cg1));
     float e2 = 1e-6f + (0.25f * (bg2 * bg2) + c1 * (cg2 * cg2));
-    float r0 = 1.0f / (e0 * e0), r1 = 1.0f / (e1 * e1), r2 = 1.0f / (e2 * e2);
+    float r0 = frcp(e0 * e0), r1 = frcp(e1 * e1), r2 = frcp(e2 * e2);
     // lw: 4 sets of 3 FP32 linear weights (+xi, centre+, centre-, -xi)
     double W[4][2];
     for (int k = 0; k < 4; ++k) {
         float w0 = lw[3 * k] * r0, w1 = lw[3 * k + 1] * r1, w2 = lw[3 * k + 2] * r2;
-        float is = 1.0f / ((w0 + w1) + w2);
+        float is = frcp((w0 + w1) + w2);
         W[k][0] = (double)(w0 * is);
         W[k][1] = (double)(w1 * is);
     }

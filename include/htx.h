@@ -54,7 +54,9 @@ enum htx_bc { HTX_BC_ZERO_GRAD = 0, HTX_BC_PERIODIC = 1 };
  * HTX_PREC_LEARNED: FP32-mixed TENO5 where a learned classifier (see below)
  *   sends the faces it finds smooth to the optimal-weight linear scheme
  *   (Lagrange::linear), guarded by an FP64 positivity check, and the rest
- *   to TENO5; needs htx_load_classifier. */
+ *   to TENO5; the classifier runs once per step, on the state the step
+ *   starts from, and its flags serve all three stages; needs
+ *   htx_load_classifier. */
 enum htx_precision { HTX_PREC_FP64 = 0, HTX_PREC_FP32_MIXED = 1, HTX_PREC_LEARNED = 2, HTX_PREC_FP64_FAST = 3 };
 
 typedef struct htx_params {

@@ -6,8 +6,9 @@ import numpy as np
 
 F = np.float32
 CLS_IN, CLS_H = 12, 64
-GELU_A = F(1.5957691216057308)
-GELU_B = F(0.07135481627466469)
+GELU_A = 0.3989422804014327
+GELU_B = -0.037625858331402064
+GELU_X0 = 1.8799712059732503
 
 
 def map_index(g, n, bc):
@@ -31,15 +32,15 @@ def stencils(q, bc, direction):
     return out
 
 
-def teno_contrast(v0, v1, v2, v3, v4, eps):
+def teno_contrast(e1, e2, e3, e4, eps):
+    """TENO contrast of a window from its four FP32 differences."""
     c1 = F(13.0) / F(12.0)
-    c_0, b_0 = F32(v2 - 2.0 * v3 + v4), F32((3.0 * v2 - 4.0 * v3 + v4) * 0.5)
-    c_1, b_1 = F32(v1 - 2.0 * v2 + v3), F32((v1 - v3) * 0.5)
-    c_2, b_2 = F32(v0 - 2.0 * v1 + v2), F32((v0 - 4.0 * v1 + 3.0 * v2) * 0.5)
+    c_0, b_0 = e4 - e3, (e4 - F(3.0) * e3) * F(0.5)
+    c_1, b_1 = e3 - e2, -(e2 + e3) * F(0.5)
+    c_2, b_2 = e2 - e1, (F(3.0) * e2 - e1) * F(0.5)
     s0 = b_0 * b_0 + c1 * c_0 * c_0
     s1 = b_1 * b_1 + c1 * c_1 * c_1
     s2 = b_2 * b_2 + c1 * c_2 * c_2
-    e1, e2, e3, e4 = F32(v1 - v0), F32(v2 - v1), F32(v3 - v2), F32(v4 - v3)
     m11, m22, m12 = F(1727.0) / F(1260.0), F(2029.0) / F(168.0), F(-2479.0) / F(672.0)
     m13, m14, m23 = F(5123.0) / F(1680.0), F(-8209.0) / F(10080.0), F(-12157.0) / F(1120.0)
     t1 = m11 * e1 + m12 * e2 + m13 * e3 + m14 * e4
@@ -61,13 +62,13 @@ def F32(a):
     return np.asarray(a, dtype=np.float64).astype(np.float32)
 
 
-def stencil_contrast(f, eps):
-    return np.maximum(teno_contrast(f[0], f[1], f[2], f[3], f[4], eps),
-                      teno_contrast(f[1], f[2], f[3], f[4], f[5], eps))
+def stencil_contrast(d, eps):
+    """d = the five FP32 differences of a 6-cell stencil."""
+    return np.maximum(teno_contrast(d[0], d[1], d[2], d[3], eps), teno_contrast(d[1], d[2], d[3], d[4], eps))
 
 
 def log_amp(a):
-    return np.log10(F32(a + 1e-12)) * (F(1.0) / F(12.0)) + F(1.0)
+    return np.log2(a + F(1e-12)) * F(0.30102999566398120 / 12.0) + F(1.0)
 
 
 def features(q, gamma, bc, direction):
@@ -78,38 +79,38 @@ def features(q, gamma, bc, direction):
     qn, qt = (st[:, 1], st[:, 2]) if direction == 0 else (st[:, 2], st[:, 1])
     un, ut = qn / r, qt / r
     p = (st[:, 3] - 0.5 * (qn * un + qt * ut)) * g1
-    rf = 0.5 * (np.abs(r[2]) + np.abs(r[3]))
-    pf = 0.5 * (np.abs(p[2]) + np.abs(p[3]))
-    cf2 = gamma * pf / (rf + 1e-300)
-    cf = np.sqrt(cf2)
-    sr = np.zeros_like(rf)
-    sp = np.zeros_like(pf)
-    ar, ap, au, at = (np.zeros_like(rf) for _ in range(4))
+    dr = [F32(r[k + 1] - r[k]) for k in range(5)]
+    dp = [F32(p[k + 1] - p[k]) for k in range(5)]
+    du = [F32(un[k + 1] - un[k]) for k in range(5)]
+    dv = [F32(ut[k + 1] - ut[k]) for k in range(5)]
+    sr = sp = np.zeros(r.shape[1:], dtype=np.float32)
+    ar = ap = au = at = np.zeros(r.shape[1:], dtype=np.float32)
     for k in range(5):
-        dr, dp = np.abs(r[k + 1] - r[k]), np.abs(p[k + 1] - p[k])
-        sr = sr + dr
-        sp = sp + dp
-        ar, ap = np.maximum(ar, dr), np.maximum(ap, dp)
-        au = np.maximum(au, np.abs(un[k + 1] - un[k]))
-        at = np.maximum(at, np.abs(ut[k + 1] - ut[k]))
-    nr = 1.0 / (sr + 1e-6 * rf + 1e-300)
-    npp = 1.0 / (sp + 1e-6 * pf + 1e-300)
-    x = np.empty(rf.shape + (12,), dtype=np.float32)
-    x[..., 0] = F32((r[2] - r[1]) * nr)
-    x[..., 1] = F32((r[3] - r[2]) * nr)
-    x[..., 2] = F32((p[2] - p[1]) * npp)
-    x[..., 3] = F32((p[3] - p[2]) * npp)
-    x[..., 4] = log_amp(ar / (rf + 1e-300))
-    x[..., 5] = log_amp(ap / (pf + 1e-300))
-    x[..., 6] = log_amp(au / (cf + 1e-300))
-    x[..., 7] = log_amp(at / (cf + 1e-300))
-    er = F32(1e-12 * rf * rf) + F(1e-30)
-    ep = F32(1e-12 * pf * pf) + F(1e-30)
-    ev = F32(1e-12 * cf2) + F(1e-30)
-    x[..., 8] = stencil_contrast(r, er)
-    x[..., 9] = stencil_contrast(p, ep)
-    x[..., 10] = stencil_contrast(un, ev)
-    x[..., 11] = stencil_contrast(ut, ev)
+        sr = sr + np.abs(dr[k])
+        sp = sp + np.abs(dp[k])
+        ar, ap = np.maximum(ar, np.abs(dr[k])), np.maximum(ap, np.abs(dp[k]))
+        au, at = np.maximum(au, np.abs(du[k])), np.maximum(at, np.abs(dv[k]))
+    rf = F32(0.5 * (np.abs(r[2]) + np.abs(r[3])))
+    pf = F32(0.5 * (np.abs(p[2]) + np.abs(p[3])))
+    irf, ipf = F(1.0) / (rf + F(1e-30)), F(1.0) / (pf + F(1e-30))
+    cf2 = F(gamma) * pf * irf
+    icf = F(1.0) / (np.sqrt(cf2) + F(1e-30))
+    nr = F(1.0) / (sr + F(1e-6) * rf + F(1e-30))
+    npp = F(1.0) / (sp + F(1e-6) * pf + F(1e-30))
+    x = np.empty(r.shape[1:] + (12,), dtype=np.float32)
+    x[..., 0] = dr[1] * nr
+    x[..., 1] = dr[2] * nr
+    x[..., 2] = dp[1] * npp
+    x[..., 3] = dp[2] * npp
+    x[..., 4] = log_amp(ar * irf)
+    x[..., 5] = log_amp(ap * ipf)
+    x[..., 6] = log_amp(au * icf)
+    x[..., 7] = log_amp(at * icf)
+    x[..., 8] = stencil_contrast(dr, F(1e-12) * rf * rf + F(1e-30))
+    x[..., 9] = stencil_contrast(dp, F(1e-12) * pf * pf + F(1e-30))
+    ev = F(1e-12) * cf2 + F(1e-30)
+    x[..., 10] = stencil_contrast(du, ev)
+    x[..., 11] = stencil_contrast(dv, ev)
     return x
 
 
@@ -125,13 +126,15 @@ def unpack(params):
 
 
 def gelu(x):
-    return x / (1.0 + np.exp(-(x * (float(GELU_A) + float(GELU_B) * x * x))))
+    t = np.clip(x, -GELU_X0, GELU_X0)
+    return x * (0.5 + t * (GELU_A + GELU_B * t * t))
 
 
 def gelu_d(x):
-    t = x * (float(GELU_A) + float(GELU_B) * x * x)
-    s = 1.0 / (1.0 + np.exp(-t))
-    return s + x * s * (1.0 - s) * (float(GELU_A) + 3.0 * float(GELU_B) * x * x)
+    t = np.clip(x, -GELU_X0, GELU_X0)
+    phi = 0.5 + t * (GELU_A + GELU_B * t * t)
+    inside = (x > -GELU_X0) & (x < GELU_X0)
+    return phi + np.where(inside, x * (GELU_A + 3.0 * GELU_B * t * t), 0.0)
 
 
 def forward(params, X):

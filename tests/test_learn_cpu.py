@@ -81,10 +81,30 @@ def test_contrast_of_polynomials_and_jumps():
     1 across a jump."""
     t = np.arange(5, dtype=np.float64)
     eps = np.float32(1e-30)
-    quad = L.teno_contrast(*(1.0 + 0.3 * t - 0.05 * t * t), eps)
-    assert quad < 1e-3
-    jump = L.teno_contrast(*np.where(t < 2.5, 1.0, 2.0), eps)
-    assert jump > 0.99
+    quad = 1.0 + 0.3 * t - 0.05 * t * t
+    assert L.teno_contrast(*L.F32(np.diff(quad)), eps) < 1e-3
+    jump = np.where(t < 2.5, 1.0, 2.0)
+    assert L.teno_contrast(*L.F32(np.diff(jump)), eps) > 0.99
+
+
+def test_difference_form_smoothness_measures_match_the_reference_polynomials():
+    """The contrast's beta_k in first differences equal TENO5's smoothness
+    measures written on the values (Schemes_Lagrange.cpp:1013-1026), and its
+    beta5 the 5-point quadratic form /5040 (:1029-1034): on random windows
+    the contrast agrees with one computed from the oracle's indicators."""
+    from oracle import Oracle
+
+    orc = Oracle()
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        u = rng.normal(0, 1, 6)
+        SI, b35 = orc.teno5_indicators(u)
+        s = np.array(SI[:3])
+        tau = abs(b35[0] - (s[0] + s[2] + 4 * s[1]) / 6)
+        lo, hi = s.min() + 1e-30, s.max() + 1e-30
+        ref = 1 - (1 + tau / hi) / (1 + tau / lo)
+        got = L.teno_contrast(*L.F32(np.diff(u[:5])), np.float32(1e-30))
+        assert abs(got - ref) < 2e-4 * (1 + abs(ref))
 
 
 def test_mlp_gradient_finite_differences():
