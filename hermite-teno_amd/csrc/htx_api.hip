@@ -55,6 +55,9 @@ void gauss9(double x[9], double w[9]) {
 
 }  // namespace
 
+// error reporting for the other translation units of the C ABI
+int htx_internal_fail(int code, const std::string& msg) { return fail(code, msg); }
+
 struct htx_solver {
     htx_params p{};
     int nx = 0, ny = 0, nfield = 4, pitch = 0;

@@ -1,0 +1,829 @@
+// htx_batch.hip -- many independent 2D Euler problems of uneven size
+// advanced together (include/htx.h, "Batches").
+//
+// TENO5 instances run in ONE persistent launch per RK stage: the grid is the
+// GPU's resident capacity of workgroups, each workgroup walks a contiguous
+// run of the batch's global tile list (all instances' tiles back to back), and
+// a tile finds its instance and that stage's argument block through a
+// tile -> instance table.  Hermite instances run batched versions of the five
+// Hermite launches (limiter, recon x/y, Gauss-point fluxes x/y, update), each
+// over the concatenated work items of all Hermite instances, a workgroup never
+// straddling two instances.  Every instance keeps its own control block (t,
+// dt, CFL speeds, end time), so instances of different sizes and speeds step
+// with their own dt; one batched kernel computes all dt and one advances all
+// times.  A whole step can be captured once in a hipGraph and replayed.  The
+// tile shape of the TENO5 launch comes from htx_batch_tune, which times the
+// built shapes on the batch itself.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/htx.h"
+#include "htx_fv2d_tile.h"
+#include "htx_hermite.h"
+
+int htx_internal_fail(int code, const std::string& msg);
+
+namespace htx {
+
+namespace {
+
+constexpr int HK_LIMIT = 0, HK_RECON_X = 1, HK_RECON_Y = 2, HK_FLUX_X = 3, HK_FLUX_Y = 4, HK_UPDATE = 5;
+
+struct BatchTenoArgs {
+    const Fv2dArgs* args;   // [mode - 1][instance]
+    const int* tile_inst;   // global tile -> TENO instance
+    const int* tile0;       // first global tile of each instance
+    unsigned char* flags;   // learned precision: per global tile
+    int ninst, ntiles, mode;
+};
+
+template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED>
+__global__ void __launch_bounds__(256, 2) k_batch_teno(BatchTenoArgs B) {
+    using T = Fv2dTile<TX, TY>;
+    __shared__ T s;
+    const int per = (B.ntiles + gridDim.x - 1) / gridDim.x;
+    const int g0 = blockIdx.x * per;
+    const int g1 = min(g0 + per, B.ntiles);
+    for (int g = g0; g < g1; ++g) {
+        const int b = B.tile_inst[g];
+        const Fv2dArgs& a = B.args[(B.mode - 1) * B.ninst + b];
+        if (!a.ctl->active) continue;  // uniform over the workgroup
+        const double dt = a.ctl->dt;
+        unsigned char* fl = LEARNED ? B.flags + (size_t)g * sizeof(s.sFlag) : nullptr;
+        stage_tile<TX, TY, FLUX, MIXED, LEARNED>(a, s, g - B.tile0[b], dt, fl, threadIdx.x);
+        __syncthreads();
+    }
+}
+
+struct BatchHermArgs {
+    const FvhArgs* args;    // [mode - 1][instance]
+    const int* inst[6];     // block -> Hermite instance, per launch kind
+    const int* first[6];    // first block of each instance, per launch kind
+    int ninst, mode;
+};
+
+__device__ __forceinline__ const FvhArgs& herm_args(const BatchHermArgs& B, int kind, int& lb) {
+    const int b = B.inst[kind][blockIdx.x];
+    lb = blockIdx.x - B.first[kind][b];
+    return B.args[(B.mode - 1) * B.ninst + b];
+}
+
+__global__ void __launch_bounds__(256) kb_fvh_limit(BatchHermArgs B) {
+    int lb;
+    const FvhArgs& a = herm_args(B, HK_LIMIT, lb);
+    if (!a.ctl->active) return;
+    int t = lb * 256 + threadIdx.x;
+    int nxl = 4 * a.ny, nyl = 4 * a.nx;
+    if (t < nxl) {
+        fvh_limit_line(a, 0, t % a.ny, t / a.ny);
+    } else if (t < nxl + nyl) {
+        t -= nxl;
+        fvh_limit_line(a, 1, t % a.nx, t / a.nx);
+    }
+}
+
+template <bool YDIR>
+__global__ void __launch_bounds__(256) kb_fvh_recon(BatchHermArgs B) {
+    int lb;
+    const FvhArgs& a = herm_args(B, YDIR ? HK_RECON_Y : HK_RECON_X, lb);
+    if (!a.ctl->active) return;
+    long long nf = YDIR ? (long long)a.nx * (a.ny + 1) : (long long)(a.nx + 1) * a.ny;
+    long long t = lb * 256ll + threadIdx.x;
+    if (t >= nf) return;
+    if (YDIR) {
+        int i = (int)(t % a.nx), fj = (int)(t / a.nx);
+        fvh_recon_face<true>(a, i, fj - 1, a.fy, t, nf);
+    } else {
+        int fi = (int)(t % (a.nx + 1)), j = (int)(t / (a.nx + 1));
+        fvh_recon_face<false>(a, fi - 1, j, a.fx, t, nf);
+    }
+}
+
+template <bool YDIR, int FLUX>
+__global__ void __launch_bounds__(192) kb_fvh_flux(BatchHermArgs B) {
+    __shared__ double acc[3][12][64];
+    int lb;
+    const FvhArgs& a = herm_args(B, YDIR ? HK_FLUX_Y : HK_FLUX_X, lb);
+    if (!a.ctl->active) return;
+    long long nf = YDIR ? (long long)a.nx * (a.ny + 1) : (long long)(a.nx + 1) * a.ny;
+    long long t = lb * 64ll + threadIdx.x;
+    int m = threadIdx.y;
+    if (t < nf) {
+        int i, j;
+        if (YDIR) {
+            i = (int)(t % a.nx);
+            j = (int)(t / a.nx) - 1;
+        } else {
+            i = (int)(t % (a.nx + 1)) - 1;
+            j = (int)(t / (a.nx + 1));
+        }
+        double o[12];
+        fvh_flux_gp<YDIR, FLUX>(a, i, j, m, YDIR ? a.fy : a.fx, nf, o);
+        for (int c = 0; c < 12; ++c) acc[m][c][threadIdx.x] = o[c];
+    }
+    __syncthreads();
+    if (t < nf && m == 0) {
+        double* G = YDIR ? a.gy : a.gx;
+        for (int c = 0; c < 12; ++c)
+            G[c * nf + t] = (acc[0][c][threadIdx.x] + acc[1][c][threadIdx.x]) + acc[2][c][threadIdx.x];
+    }
+}
+
+__global__ void __launch_bounds__(256) kb_fvh_update(BatchHermArgs B) {
+    __shared__ double red[2][4];
+    int lb;
+    const FvhArgs& a = herm_args(B, HK_UPDATE, lb);
+    if (!a.ctl->active) return;
+    const double dt = a.ctl->dt;
+    long long n = (long long)a.nx * a.ny;
+    long long t = lb * 256ll + threadIdx.x;
+    double smx = 0.0, smy = 0.0;
+    if (t < n) fvh_update_cell(a, (int)(t % a.nx), (int)(t / a.nx), dt, smx, smy);
+    if (a.mode == 3) {
+        smx = wave_max(smx);
+        smy = wave_max(smy);
+        if ((threadIdx.x & 63) == 0) {
+            red[0][threadIdx.x >> 6] = smx;
+            red[1][threadIdx.x >> 6] = smy;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int k = 1; k < 4; ++k) {
+                smx = fmax(smx, red[0][k]);
+                smy = fmax(smy, red[1][k]);
+            }
+            atomicMax(&a.ctl->smax_x, (unsigned long long)__double_as_longlong(smx));
+            atomicMax(&a.ctl->smax_y, (unsigned long long)__double_as_longlong(smy));
+        }
+    }
+}
+
+__global__ void k_batch_dt(Ctl* c, const double* dxdy, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) ctl_dt(c + i, dxdy[2 * i], dxdy[2 * i + 1]);
+}
+
+__global__ void k_batch_tadvance(Ctl* c, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) ctl_tadvance(c + i);
+}
+
+// ---------------------------------------------------------------------------
+// tile shapes of the TENO5 launch (htx_batch_tune picks one)
+struct Shape {
+    int tx, ty;
+};
+constexpr Shape SHAPES[] = {{16, 16}, {32, 8}, {8, 32}};
+constexpr int NSHAPES = 3;
+
+template <int TX, int TY>
+size_t flag_bytes_per_tile() {
+    return sizeof(((Fv2dTile<TX, TY>*)nullptr)->sFlag);
+}
+
+size_t shape_flag_bytes(int k) {
+    switch (k) {
+        case 0: return flag_bytes_per_tile<16, 16>();
+        case 1: return flag_bytes_per_tile<32, 8>();
+        default: return flag_bytes_per_tile<8, 32>();
+    }
+}
+
+template <int TX, int TY>
+hipError_t launch_teno_shape(const BatchTenoArgs& B, int flux, int prec, int nblk, hipStream_t s) {
+    dim3 g(nblk), b(256);
+    if (flux == FLUX_RUSANOV_CHAR) {
+        if (prec == 0) hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_RUSANOV_CHAR, false, false>), g, b, 0, s, B);
+        else if (prec == 1) hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_RUSANOV_CHAR, true, false>), g, b, 0, s, B);
+        else hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_RUSANOV_CHAR, true, true>), g, b, 0, s, B);
+    } else {
+        if (prec == 0) hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_HLLC, false, false>), g, b, 0, s, B);
+        else if (prec == 1) hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_HLLC, true, false>), g, b, 0, s, B);
+        else hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_HLLC, true, true>), g, b, 0, s, B);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_teno(int shape, const BatchTenoArgs& B, int flux, int prec, int nblk, hipStream_t s) {
+    switch (shape) {
+        case 0: return launch_teno_shape<16, 16>(B, flux, prec, nblk, s);
+        case 1: return launch_teno_shape<32, 8>(B, flux, prec, nblk, s);
+        default: return launch_teno_shape<8, 32>(B, flux, prec, nblk, s);
+    }
+}
+
+hipError_t launch_herm(const BatchHermArgs& B, const int nblocks[6], int flux, hipStream_t s) {
+    hipLaunchKernelGGL(kb_fvh_limit, dim3(nblocks[HK_LIMIT]), dim3(256), 0, s, B);
+    hipLaunchKernelGGL(kb_fvh_recon<false>, dim3(nblocks[HK_RECON_X]), dim3(256), 0, s, B);
+    hipLaunchKernelGGL(kb_fvh_recon<true>, dim3(nblocks[HK_RECON_Y]), dim3(256), 0, s, B);
+    dim3 fb(64, 3);
+    if (flux == FLUX_RUSANOV_CHAR) {
+        hipLaunchKernelGGL((kb_fvh_flux<false, FLUX_RUSANOV_CHAR>), dim3(nblocks[HK_FLUX_X]), fb, 0, s, B);
+        hipLaunchKernelGGL((kb_fvh_flux<true, FLUX_RUSANOV_CHAR>), dim3(nblocks[HK_FLUX_Y]), fb, 0, s, B);
+    } else if (flux == FLUX_HLLC) {
+        hipLaunchKernelGGL((kb_fvh_flux<false, FLUX_HLLC>), dim3(nblocks[HK_FLUX_X]), fb, 0, s, B);
+        hipLaunchKernelGGL((kb_fvh_flux<true, FLUX_HLLC>), dim3(nblocks[HK_FLUX_Y]), fb, 0, s, B);
+    } else {
+        hipLaunchKernelGGL((kb_fvh_flux<false, FLUX_HLLC_TORO>), dim3(nblocks[HK_FLUX_X]), fb, 0, s, B);
+        hipLaunchKernelGGL((kb_fvh_flux<true, FLUX_HLLC_TORO>), dim3(nblocks[HK_FLUX_Y]), fb, 0, s, B);
+    }
+    hipLaunchKernelGGL(kb_fvh_update, dim3(nblocks[HK_UPDATE]), dim3(256), 0, s, B);
+    return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace htx
+
+using namespace htx;
+
+struct htx_batch {
+    int n = 0;
+    std::vector<htx_params> p;
+    std::vector<int> teno, herm;           // instance ids per scheme
+    std::vector<int> slot;                 // instance -> index within its scheme list
+    std::vector<long long> off;            // instance -> offset of its state in each state copy
+    std::vector<double> dxdy;              // host copy
+    long long total = 0;                   // doubles per state copy
+    double* state = nullptr;               // 3 copies: A | B | C
+    Ctl* ctl = nullptr;
+    double* dDxdy = nullptr;
+    // TENO5 launch
+    int teno_flux = FLUX_HLLC, teno_prec = 1;  // prec 0 fp64_fast, 1 fp32_mixed, 2 learned
+    int shape = 0, ntiles = 0, nblk_teno = 0;
+    Fv2dArgs* dTenoArgs = nullptr;
+    int *dTileInst = nullptr, *dTile0 = nullptr;
+    unsigned char* dFlags = nullptr;
+    size_t flags_bytes = 0;
+    // Hermite launches
+    int herm_flux = FLUX_RUSANOV_CHAR;
+    FvhArgs* dHermArgs = nullptr;
+    int* dHermMap = nullptr;
+    int herm_blocks[6] = {0, 0, 0, 0, 0, 0};
+    long long herm_map_off[12] = {0};
+    double* herm_scratch = nullptr;
+    // classifier
+    unsigned short* dFrag = nullptr;
+    float* dCvec = nullptr;
+    float cls_thr = 0.0f;
+    bool have_cls = false;
+    // execution
+    hipStream_t stream = nullptr;
+    bool use_graph = false;
+    hipGraphExec_t graph = nullptr;
+    int cu = 256;
+};
+
+namespace {
+
+#define BHIP(expr)                                                                                      \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return htx_internal_fail(HTX_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int prec_code(int precision) {
+    if (precision == HTX_PREC_FP64_FAST) return 0;
+    if (precision == HTX_PREC_FP32_MIXED) return 1;
+    if (precision == HTX_PREC_LEARNED) return 2;
+    return -1;
+}
+
+int flux_code(const htx_params& p) {
+    if (p.flux == HTX_FLUX_RUSANOV_CHAR) return FLUX_RUSANOV_CHAR;
+    return p.ref_quirks ? FLUX_HLLC : FLUX_HLLC_TORO;
+}
+
+double* inst_state(htx_batch* b, int copy, int i) { return b->state + copy * b->total + b->off[i]; }
+
+// TENO5 argument blocks of the current tile shape, tile tables
+int build_teno_tables(htx_batch* b) {
+    const int nt = (int)b->teno.size();
+    if (nt == 0) return HTX_OK;
+    const Shape sh = SHAPES[b->shape];
+    std::vector<Fv2dArgs> args(3 * (size_t)nt);
+    std::vector<int> tile0(nt), tinst;
+    QuadConsts qc{};
+    fill_quad_consts(qc);
+    int g = 0;
+    for (int k = 0; k < nt; ++k) {
+        const int i = b->teno[k];
+        const htx_params& p = b->p[i];
+        const int tx = (p.nx + sh.tx - 1) / sh.tx, ty = (p.ny + sh.ty - 1) / sh.ty;
+        tile0[k] = g;
+        for (int t = 0; t < tx * ty; ++t) tinst.push_back(k);
+        g += tx * ty;
+        for (int mode = 1; mode <= 3; ++mode) {
+            Fv2dArgs a{};
+            double *A = inst_state(b, 0, i), *B = inst_state(b, 1, i), *C = inst_state(b, 2, i);
+            a.qin = mode == 1 ? A : (mode == 2 ? B : C);
+            a.q0 = A;
+            a.qout = mode == 1 ? B : (mode == 2 ? C : A);
+            a.ctl = b->ctl + i;
+            a.nx = p.nx;
+            a.ny = p.ny;
+            a.pitch = p.nx;
+            a.fstride = (long long)p.nx * p.ny;
+            a.bc = p.bc;
+            a.mode = mode;
+            a.quirks = p.ref_quirks ? 1 : 0;
+            a.mixed = b->teno_prec >= 1;
+            a.learned = b->teno_prec == 2;
+            a.tiles_x = tx;
+            a.tiles_y = ty;
+            a.gam = p.gamma;
+            a.g1 = p.gamma - 1.0;
+            a.dx = b->dxdy[2 * i];
+            a.dy = b->dxdy[2 * i + 1];
+            a.area = a.dx * a.dy;
+            a.qc = qc;
+            a.cls_frag = b->dFrag;
+            a.cls_cvec = b->dCvec;
+            a.cls_thr = b->cls_thr;
+            a.cls_flags = nullptr;  // the batched launch addresses flags by global tile
+            a.cls_fresh = mode == 1 ? 1 : 0;
+            args[(size_t)(mode - 1) * nt + k] = a;
+        }
+    }
+    b->ntiles = g;
+    if (b->dTenoArgs) (void)hipFree(b->dTenoArgs);
+    if (b->dTileInst) (void)hipFree(b->dTileInst);
+    if (b->dTile0) (void)hipFree(b->dTile0);
+    if (b->dFlags) (void)hipFree(b->dFlags);
+    b->dTenoArgs = nullptr;
+    b->dTileInst = b->dTile0 = nullptr;
+    b->dFlags = nullptr;
+    BHIP(hipMalloc(&b->dTenoArgs, sizeof(Fv2dArgs) * args.size()));
+    BHIP(hipMalloc(&b->dTileInst, sizeof(int) * tinst.size()));
+    BHIP(hipMalloc(&b->dTile0, sizeof(int) * tile0.size()));
+    BHIP(hipMemcpy(b->dTenoArgs, args.data(), sizeof(Fv2dArgs) * args.size(), hipMemcpyHostToDevice));
+    BHIP(hipMemcpy(b->dTileInst, tinst.data(), sizeof(int) * tinst.size(), hipMemcpyHostToDevice));
+    BHIP(hipMemcpy(b->dTile0, tile0.data(), sizeof(int) * tile0.size(), hipMemcpyHostToDevice));
+    if (b->teno_prec == 2) {
+        b->flags_bytes = shape_flag_bytes(b->shape) * (size_t)g;
+        BHIP(hipMalloc(&b->dFlags, b->flags_bytes));
+    }
+    // persistent grid: the resident capacity (2 workgroups per CU), at most
+    // one workgroup per tile
+    b->nblk_teno = std::min(g, 2 * b->cu);
+    return HTX_OK;
+}
+
+int build_herm_tables(htx_batch* b) {
+    const int nh = (int)b->herm.size();
+    if (nh == 0) return HTX_OK;
+    // scratch: limited moments (8 fields), face states (24 x 2), fluxes (12 x 2)
+    std::vector<long long> soff(nh);
+    long long stotal = 0;
+    for (int k = 0; k < nh; ++k) {
+        const htx_params& p = b->p[b->herm[k]];
+        long long n = (long long)p.nx * p.ny, nxf = (long long)(p.nx + 1) * p.ny, nyf = (long long)p.nx * (p.ny + 1);
+        soff[k] = stotal;
+        stotal += 8 * n + 24 * (nxf + nyf) + 12 * (nxf + nyf);
+    }
+    BHIP(hipMalloc(&b->herm_scratch, sizeof(double) * (size_t)stotal));
+    std::vector<FvhArgs> args(3 * (size_t)nh);
+    std::vector<int> maps[12];  // [kind] block -> instance, [6 + kind] instance -> first block
+    const int per_block[6] = {256, 256, 256, 64, 64, 256};
+    QuadConsts qc{};
+    fill_quad_consts(qc);
+    for (int k = 0; k < nh; ++k) {
+        const int i = b->herm[k];
+        const htx_params& p = b->p[i];
+        long long n = (long long)p.nx * p.ny, nxf = (long long)(p.nx + 1) * p.ny, nyf = (long long)p.nx * (p.ny + 1);
+        long long items[6] = {4ll * (p.nx + p.ny), nxf, nyf, nxf, nyf, n};
+        for (int kind = 0; kind < 6; ++kind) {
+            maps[6 + kind].push_back((int)maps[kind].size());
+            long long nb = (items[kind] + per_block[kind] - 1) / per_block[kind];
+            for (long long t = 0; t < nb; ++t) maps[kind].push_back(k);
+        }
+        double* S = b->herm_scratch + soff[k];
+        FvhArgs a{};
+        a.lv = S;
+        a.lv_out = S;
+        a.fx = S + 8 * n;
+        a.fy = a.fx + 24 * nxf;
+        a.gx = a.fy + 24 * nyf;
+        a.gy = a.gx + 12 * nxf;
+        a.ctl = b->ctl + i;
+        a.nx = p.nx;
+        a.ny = p.ny;
+        a.pitch = p.nx;
+        a.fstride = n;
+        a.bc = p.bc;
+        a.quirks = p.ref_quirks ? 1 : 0;
+        a.flux = p.flux;
+        a.gam = p.gamma;
+        a.g1 = p.gamma - 1.0;
+        a.dx = b->dxdy[2 * i];
+        a.dy = b->dxdy[2 * i + 1];
+        a.area = a.dx * a.dy;
+        a.qc = qc;
+        fill_herm_consts(a.hc, a.dx);
+        for (int mode = 1; mode <= 3; ++mode) {
+            FvhArgs m = a;
+            double *A = inst_state(b, 0, i), *B = inst_state(b, 1, i), *C = inst_state(b, 2, i);
+            m.q = mode == 1 ? A : (mode == 2 ? B : C);
+            m.q0 = A;
+            m.qout = mode == 1 ? B : (mode == 2 ? C : A);
+            m.mode = mode;
+            args[(size_t)(mode - 1) * nh + k] = m;
+        }
+    }
+    long long msize = 0;
+    for (int k = 0; k < 12; ++k) {
+        b->herm_map_off[k] = msize;
+        msize += (long long)maps[k].size();
+    }
+    for (int kind = 0; kind < 6; ++kind) b->herm_blocks[kind] = (int)maps[kind].size();
+    BHIP(hipMalloc(&b->dHermArgs, sizeof(FvhArgs) * args.size()));
+    BHIP(hipMemcpy(b->dHermArgs, args.data(), sizeof(FvhArgs) * args.size(), hipMemcpyHostToDevice));
+    BHIP(hipMalloc(&b->dHermMap, sizeof(int) * (size_t)msize));
+    for (int k = 0; k < 12; ++k)
+        BHIP(hipMemcpy(b->dHermMap + b->herm_map_off[k], maps[k].data(), sizeof(int) * maps[k].size(),
+                       hipMemcpyHostToDevice));
+    return HTX_OK;
+}
+
+int enqueue_batch_step(htx_batch* b, hipStream_t s) {
+    const int n = b->n;
+    hipLaunchKernelGGL(k_batch_dt, dim3((n + 255) / 256), dim3(256), 0, s, b->ctl, b->dDxdy, n);
+    for (int mode = 1; mode <= 3; ++mode) {
+        if (!b->teno.empty()) {
+            BatchTenoArgs T{};
+            T.args = b->dTenoArgs;
+            T.tile_inst = b->dTileInst;
+            T.tile0 = b->dTile0;
+            T.flags = b->dFlags;
+            T.ninst = (int)b->teno.size();
+            T.ntiles = b->ntiles;
+            T.mode = mode;
+            BHIP(launch_teno(b->shape, T, b->teno_flux, b->teno_prec, b->nblk_teno, s));
+        }
+        if (!b->herm.empty()) {
+            BatchHermArgs H{};
+            H.args = b->dHermArgs;
+            for (int k = 0; k < 6; ++k) {
+                H.inst[k] = b->dHermMap + b->herm_map_off[k];
+                H.first[k] = b->dHermMap + b->herm_map_off[6 + k];
+            }
+            H.ninst = (int)b->herm.size();
+            H.mode = mode;
+            BHIP(launch_herm(H, b->herm_blocks, b->herm_flux, s));
+        }
+    }
+    hipLaunchKernelGGL(k_batch_tadvance, dim3((n + 255) / 256), dim3(256), 0, s, b->ctl, n);
+    BHIP(hipGetLastError());
+    return HTX_OK;
+}
+
+int drop_graph(htx_batch* b) {
+    if (b->graph) {
+        BHIP(hipGraphExecDestroy(b->graph));
+        b->graph = nullptr;
+    }
+    return HTX_OK;
+}
+
+int capture_graph(htx_batch* b) {
+    int rc = drop_graph(b);
+    if (rc) return rc;
+    hipGraph_t g = nullptr;
+    BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_batch_step(b, b->stream);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return htx_internal_fail(HTX_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return htx_internal_fail(HTX_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+    return HTX_OK;
+}
+
+int refresh_instance_cfl(htx_batch* b, int i) {
+    const htx_params& p = b->p[i];
+    BHIP(launch_reset_smax(b->ctl + i, b->stream));
+    BHIP(launch_max_speed(inst_state(b, 0, i), p.nx, p.ny, p.nx, (long long)p.nx * p.ny, p.gamma,
+                          p.ref_quirks ? 1 : 0, b->ctl + i, b->stream));
+    return HTX_OK;
+}
+
+int check_index(htx_batch* b, int i) {
+    if (!b) return htx_internal_fail(HTX_EINVAL, "null batch");
+    if (i < 0 || i >= b->n) return htx_internal_fail(HTX_EINVAL, "instance index out of range");
+    return HTX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int htx_batch_create(const htx_params* ps, int n, htx_batch** out) {
+    if (!out) return htx_internal_fail(HTX_EINVAL, "out is null");
+    *out = nullptr;
+    if (!ps || n < 1) return htx_internal_fail(HTX_EINVAL, "need n >= 1 parameter blocks");
+    int teno_flux = -1, teno_prec = -1, herm_flux = -1;
+    for (int i = 0; i < n; ++i) {
+        const htx_params& p = ps[i];
+        if (p.struct_size != (int)sizeof(htx_params)) return htx_internal_fail(HTX_EINVAL, "params.struct_size mismatch");
+        if (p.dim != 2 || p.nx <= 0 || p.ny <= 0 || !(p.xr > p.xl) || !(p.yt > p.yb) || !(p.gamma > 1.0) ||
+            !(p.cfl > 0.0) || (p.bc != HTX_BC_ZERO_GRAD && p.bc != HTX_BC_PERIODIC) ||
+            (p.flux != HTX_FLUX_RUSANOV_CHAR && p.flux != HTX_FLUX_HLLC))
+            return htx_internal_fail(HTX_EINVAL, "instance " + std::to_string(i) + ": invalid parameters");
+        if (p.scheme == HTX_SCHEME_TENO5) {
+            int pc = prec_code(p.precision), fc = flux_code(p);
+            if (pc < 0)
+                return htx_internal_fail(HTX_ENOTSUP, "batched TENO5 runs fp64_fast, fp32_mixed or learned precision");
+            if (fc == FLUX_HLLC_TORO) return htx_internal_fail(HTX_ENOTSUP, "batched HLLC keeps the reference quirks");
+            if ((teno_prec >= 0 && pc != teno_prec) || (teno_flux >= 0 && fc != teno_flux))
+                return htx_internal_fail(HTX_ENOTSUP, "the TENO5 instances of a batch share flux and precision");
+            teno_prec = pc;
+            teno_flux = fc;
+        } else if (p.scheme == HTX_SCHEME_HTENO) {
+            if (p.precision != HTX_PREC_FP64_FAST)
+                return htx_internal_fail(HTX_ENOTSUP, "batched Hermite instances run fp64_fast precision");
+            int fc = flux_code(p);
+            if (herm_flux >= 0 && fc != herm_flux)
+                return htx_internal_fail(HTX_ENOTSUP, "the Hermite instances of a batch share their flux");
+            herm_flux = fc;
+        } else {
+            return htx_internal_fail(HTX_EINVAL, "unknown scheme");
+        }
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return htx_internal_fail(HTX_EDEVICE, "no HIP device");
+    htx_batch* b = new htx_batch();
+    b->n = n;
+    b->p.assign(ps, ps + n);
+    b->slot.resize(n);
+    b->off.resize(n);
+    b->dxdy.resize(2 * (size_t)n);
+    if (teno_prec >= 0) b->teno_prec = teno_prec;
+    if (teno_flux >= 0) b->teno_flux = teno_flux;
+    if (herm_flux >= 0) b->herm_flux = herm_flux;
+    long long total = 0;
+    for (int i = 0; i < n; ++i) {
+        const htx_params& p = ps[i];
+        int nf = p.scheme == HTX_SCHEME_HTENO ? 12 : 4;
+        b->off[i] = total;
+        total += (long long)nf * p.nx * p.ny;
+        total = (total + 31) / 32 * 32;  // 256-byte aligned instance starts
+        b->dxdy[2 * i] = (p.xr - p.xl) / p.nx;
+        b->dxdy[2 * i + 1] = (p.yt - p.yb) / p.ny;
+        if (p.scheme == HTX_SCHEME_TENO5) {
+            b->slot[i] = (int)b->teno.size();
+            b->teno.push_back(i);
+        } else {
+            b->slot[i] = (int)b->herm.size();
+            b->herm.push_back(i);
+        }
+    }
+    b->total = total;
+    int rc = HTX_OK;
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.multiProcessorCount > 0) b->cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&b->state, sizeof(double) * 3 * (size_t)total) != hipSuccess ||
+        hipMalloc(&b->ctl, sizeof(Ctl) * (size_t)n) != hipSuccess ||
+        hipMalloc(&b->dDxdy, sizeof(double) * 2 * (size_t)n) != hipSuccess) {
+        htx_batch_destroy(b);
+        return htx_internal_fail(HTX_ENOMEM, "device allocation failed");
+    }
+    std::vector<Ctl> c(n);
+    for (int i = 0; i < n; ++i) {
+        c[i] = Ctl{};
+        c[i].t = ps[i].tinit;
+        c[i].tfinal = ps[i].tmax;
+        c[i].cfl = ps[i].cfl;
+    }
+    if (hipMemcpy(b->ctl, c.data(), sizeof(Ctl) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->dDxdy, b->dxdy.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(b->state, 0, sizeof(double) * 3 * (size_t)total) != hipSuccess) {
+        htx_batch_destroy(b);
+        return htx_internal_fail(HTX_EDEVICE, "batch initialisation failed");
+    }
+    if ((rc = build_teno_tables(b)) || (rc = build_herm_tables(b))) {
+        htx_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return HTX_OK;
+}
+
+void htx_batch_destroy(htx_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (void* m : {(void*)b->state, (void*)b->ctl, (void*)b->dDxdy, (void*)b->dTenoArgs, (void*)b->dTileInst,
+                    (void*)b->dTile0, (void*)b->dFlags, (void*)b->dHermArgs, (void*)b->dHermMap,
+                    (void*)b->herm_scratch, (void*)b->dFrag, (void*)b->dCvec})
+        if (m) (void)hipFree(m);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int htx_batch_set_state(htx_batch* b, int i, const double* q, size_t ld) {
+    int rc = check_index(b, i);
+    if (rc) return rc;
+    const htx_params& p = b->p[i];
+    if (!q || ld < (size_t)p.nx) return htx_internal_fail(HTX_EINVAL, "null state or ld < nx");
+    int nf = p.scheme == HTX_SCHEME_HTENO ? 12 : 4;
+    BHIP(hipMemcpy2DAsync(inst_state(b, 0, i), sizeof(double) * p.nx, q, sizeof(double) * ld, sizeof(double) * p.nx,
+                          (size_t)p.ny * nf, hipMemcpyHostToDevice, b->stream));
+    if ((rc = refresh_instance_cfl(b, i))) return rc;
+    BHIP(hipStreamSynchronize(b->stream));
+    return HTX_OK;
+}
+
+int htx_batch_get_state(htx_batch* b, int i, double* q, size_t ld) {
+    int rc = check_index(b, i);
+    if (rc) return rc;
+    const htx_params& p = b->p[i];
+    if (!q || ld < (size_t)p.nx) return htx_internal_fail(HTX_EINVAL, "null state or ld < nx");
+    int nf = p.scheme == HTX_SCHEME_HTENO ? 12 : 4;
+    BHIP(hipMemcpy2DAsync(q, sizeof(double) * ld, inst_state(b, 0, i), sizeof(double) * p.nx, sizeof(double) * p.nx,
+                          (size_t)p.ny * nf, hipMemcpyDeviceToHost, b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
+    return HTX_OK;
+}
+
+int htx_batch_init_quadrant(htx_batch* b, int i, double xs, double ys, const double* states) {
+    int rc = check_index(b, i);
+    if (rc) return rc;
+    if (!states) return htx_internal_fail(HTX_EINVAL, "null states");
+    const htx_params& p = b->p[i];
+    PiecewiseArgs a{};
+    a.q = inst_state(b, 0, i);
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.pitch = p.nx;
+    a.nfield = p.scheme == HTX_SCHEME_HTENO ? 12 : 4;
+    a.fstride = (long long)p.nx * p.ny;
+    a.xl = p.xl;
+    a.yb = p.yb;
+    a.dx = b->dxdy[2 * i];
+    a.dy = b->dxdy[2 * i + 1];
+    a.gam = p.gamma;
+    a.kind = IC_QUADRANT;
+    a.xs = xs;
+    a.ys = ys;
+    for (int k = 0; k < 4; ++k)
+        for (int e = 0; e < 4; ++e) a.st[k][e] = states[4 * k + e];
+    BHIP(launch_ic_piecewise(a, b->stream));
+    if ((rc = refresh_instance_cfl(b, i))) return rc;
+    BHIP(hipStreamSynchronize(b->stream));
+    return HTX_OK;
+}
+
+int htx_batch_load_classifier(htx_batch* b, const float* params, double threshold) {
+    if (!b || !params) return htx_internal_fail(HTX_EINVAL, "null argument");
+    for (int k = 0; k < CLS_NPARAM; ++k)
+        if (!std::isfinite(params[k])) return htx_internal_fail(HTX_EINVAL, "classifier parameters must be finite");
+    std::vector<unsigned short> frag(CLS_NFRAG);
+    std::vector<float> cvec(CLS_NCVEC);
+    cls_pack(params, frag.data(), cvec.data());
+    if (!b->dFrag) {
+        BHIP(hipMalloc(&b->dFrag, sizeof(unsigned short) * CLS_NFRAG));
+        BHIP(hipMalloc(&b->dCvec, sizeof(float) * CLS_NCVEC));
+    }
+    BHIP(hipStreamSynchronize(b->stream));
+    BHIP(hipMemcpy(b->dFrag, frag.data(), sizeof(unsigned short) * CLS_NFRAG, hipMemcpyHostToDevice));
+    BHIP(hipMemcpy(b->dCvec, cvec.data(), sizeof(float) * CLS_NCVEC, hipMemcpyHostToDevice));
+    b->cls_thr = (float)threshold;
+    b->have_cls = true;
+    int rc = build_teno_tables(b);
+    if (rc) return rc;
+    return drop_graph(b);
+}
+
+int htx_batch_steps(htx_batch* b, int nsteps) {
+    if (!b) return htx_internal_fail(HTX_EINVAL, "null batch");
+    if (nsteps < 0) return htx_internal_fail(HTX_EINVAL, "nsteps < 0");
+    if (!b->teno.empty() && b->teno_prec == 2 && !b->have_cls)
+        return htx_internal_fail(HTX_ESTATE, "learned precision needs htx_batch_load_classifier");
+    int rc;
+    if (b->use_graph && !b->graph && (rc = capture_graph(b))) return rc;
+    for (int k = 0; k < nsteps; ++k) {
+        if (b->use_graph)
+            BHIP(hipGraphLaunch(b->graph, b->stream));
+        else if ((rc = enqueue_batch_step(b, b->stream)))
+            return rc;
+    }
+    return HTX_OK;
+}
+
+int htx_batch_advance(htx_batch* b, int max_steps, int* nsteps) {
+    if (!b) return htx_internal_fail(HTX_EINVAL, "null batch");
+    std::vector<Ctl> c(b->n);
+    int done = 0, chunk = 8, rc;
+    while (done < max_steps) {
+        int k = std::min(chunk, max_steps - done);
+        if ((rc = htx_batch_steps(b, k))) return rc;
+        done += k;
+        BHIP(hipMemcpyAsync(c.data(), b->ctl, sizeof(Ctl) * b->n, hipMemcpyDeviceToHost, b->stream));
+        BHIP(hipStreamSynchronize(b->stream));
+        bool all = true;
+        for (int i = 0; i < b->n; ++i) all = all && !(c[i].t < c[i].tfinal);
+        if (all) break;
+        chunk = std::min(chunk * 2, 128);
+    }
+    if (nsteps) *nsteps = done;
+    return HTX_OK;
+}
+
+int htx_batch_get_time(htx_batch* b, int i, double* t, int* nsteps) {
+    int rc = check_index(b, i);
+    if (rc) return rc;
+    Ctl c{};
+    BHIP(hipMemcpyAsync(&c, b->ctl + i, sizeof(Ctl), hipMemcpyDeviceToHost, b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
+    if (t) *t = c.t;
+    if (nsteps) *nsteps = c.nsteps;
+    return HTX_OK;
+}
+
+int htx_batch_sync(htx_batch* b) {
+    if (!b) return htx_internal_fail(HTX_EINVAL, "null batch");
+    BHIP(hipStreamSynchronize(b->stream));
+    return HTX_OK;
+}
+
+int htx_batch_use_graph(htx_batch* b, int enable) {
+    if (!b) return htx_internal_fail(HTX_EINVAL, "null batch");
+    b->use_graph = enable != 0;
+    if (!b->use_graph) return drop_graph(b);
+    return HTX_OK;
+}
+
+int htx_batch_set_tile(htx_batch* b, int tile_x, int tile_y) {
+    if (!b) return htx_internal_fail(HTX_EINVAL, "null batch");
+    int k = -1;
+    for (int s = 0; s < NSHAPES; ++s)
+        if (SHAPES[s].tx == tile_x && SHAPES[s].ty == tile_y) k = s;
+    if (k < 0) return htx_internal_fail(HTX_EINVAL, "tile shape not built (16x16, 32x8, 8x32)");
+    BHIP(hipStreamSynchronize(b->stream));
+    b->shape = k;
+    int rc = build_teno_tables(b);
+    if (rc) return rc;
+    return drop_graph(b);
+}
+
+int htx_batch_tune(htx_batch* b, int steps, int* tile_x, int* tile_y, double* ms) {
+    if (!b) return htx_internal_fail(HTX_EINVAL, "null batch");
+    if (steps < 1) return htx_internal_fail(HTX_EINVAL, "steps >= 1");
+    if (!b->teno.empty() && b->teno_prec == 2 && !b->have_cls)
+        return htx_internal_fail(HTX_ESTATE, "learned precision needs htx_batch_load_classifier");
+    // time every built shape on the batch itself, from the same state
+    double* saved = nullptr;
+    Ctl* saved_ctl = nullptr;
+    if (hipMalloc(&saved, sizeof(double) * (size_t)b->total) != hipSuccess ||
+        hipMalloc(&saved_ctl, sizeof(Ctl) * (size_t)b->n) != hipSuccess) {
+        if (saved) (void)hipFree(saved);
+        return htx_internal_fail(HTX_ENOMEM, "device allocation failed");
+    }
+    hipEvent_t e0, e1;
+    BHIP(hipEventCreate(&e0));
+    BHIP(hipEventCreate(&e1));
+    BHIP(hipMemcpyAsync(saved, b->state, sizeof(double) * (size_t)b->total, hipMemcpyDeviceToDevice, b->stream));
+    BHIP(hipMemcpyAsync(saved_ctl, b->ctl, sizeof(Ctl) * (size_t)b->n, hipMemcpyDeviceToDevice, b->stream));
+    const bool graph = b->use_graph;
+    b->use_graph = false;
+    int best = b->shape, rc = HTX_OK;
+    double best_ms = 1e300;
+    for (int k = 0; k < NSHAPES && !rc; ++k) {
+        b->shape = k;
+        if ((rc = build_teno_tables(b)) || (rc = enqueue_batch_step(b, b->stream))) break;  // warm-up
+        BHIP(hipEventRecord(e0, b->stream));
+        for (int s = 0; s < steps && !rc; ++s) rc = enqueue_batch_step(b, b->stream);
+        BHIP(hipEventRecord(e1, b->stream));
+        BHIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        BHIP(hipEventElapsedTime(&t, e0, e1));
+        if (ms) ms[k] = t / steps;
+        if (t < best_ms) {
+            best_ms = t;
+            best = k;
+        }
+        BHIP(hipMemcpyAsync(b->state, saved, sizeof(double) * (size_t)b->total, hipMemcpyDeviceToDevice, b->stream));
+        BHIP(hipMemcpyAsync(b->ctl, saved_ctl, sizeof(Ctl) * (size_t)b->n, hipMemcpyDeviceToDevice, b->stream));
+    }
+    BHIP(hipStreamSynchronize(b->stream));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(saved);
+    (void)hipFree(saved_ctl);
+    b->use_graph = graph;
+    if (rc) return rc;
+    b->shape = best;
+    if ((rc = build_teno_tables(b)) || (rc = drop_graph(b))) return rc;
+    if (tile_x) *tile_x = SHAPES[best].tx;
+    if (tile_y) *tile_y = SHAPES[best].ty;
+    return HTX_OK;
+}
+
+}  // extern "C"

@@ -19,11 +19,6 @@
 
 namespace HTX_NS {
 
-__device__ __forceinline__ double wave_max(double v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
 template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED>
 __global__ void __launch_bounds__(256, 2) fv2d_stage_kernel(Fv2dArgs a) {
     using T = Fv2dTile<TX, TY>;
@@ -33,105 +28,15 @@ __global__ void __launch_bounds__(256, 2) fv2d_stage_kernel(Fv2dArgs a) {
         if (!a.ctl->active) return;
         dt = a.ctl->dt;
     }
-    const int tid = threadIdx.x;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    const int i0 = (tile % a.tiles_x) * TX;
-    const int j0 = (tile / a.tiles_x) * TY;
-    double dxp[T::KC][4];
-
-    ph_load<TX, TY>(a, s, i0, j0, tid);
-    __syncthreads();
-    unsigned cnt[3] = {0, 0, 0};
-    if (LEARNED) {
-        // the classifier runs in the first stage of a step (and for a lone
-        // residual); stages 2 and 3 reuse the flags it stored for this tile
-        unsigned char* fl = a.cls_flags + (size_t)tile * sizeof(s.sFlag);
-        unsigned char* sf = &s.sFlag[0][0];
-        if (a.cls_fresh) {
-            ph_classify<TX, TY>(a, s, tid);
-            for (int k = tid; k < (int)sizeof(s.sFlag); k += T::NT) fl[k] = sf[k];
-        } else {
-            for (int k = tid; k < (int)sizeof(s.sFlag); k += T::NT) sf[k] = fl[k];
-            __syncthreads();
-        }
-    }
-    ph_recon<TX, TY, false, MIXED, LEARNED>(a, s, tid, i0, j0, cnt);
-    __syncthreads();
-    ph_flux<TX, TY, false, FLUX, MIXED>(a, s, tid);
-    __syncthreads();
-    ph_xdiv<TX, TY>(a, s, tid, dxp);
-    __syncthreads();
-    ph_recon<TX, TY, true, MIXED, LEARNED>(a, s, tid, i0, j0, cnt);
-    if (LEARNED) {
-        for (int k = 0; k < 3; ++k) {
-            unsigned v = cnt[k];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            if ((tid & 63) == 0) s.sCnt[k][tid >> 6] = v;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            unsigned long long c3[3] = {0, 0, 0};
-            for (int k = 0; k < 3; ++k)
-                for (int w = 0; w < T::NT / 64; ++w) c3[k] += s.sCnt[k][w];
-            atomicAdd(&a.ctl->n_faces, c3[0]);
-            atomicAdd(&a.ctl->n_flagged, c3[1]);
-            atomicAdd(&a.ctl->n_linear, c3[2]);
-        }
-    }
-    __syncthreads();
-    ph_flux<TX, TY, true, FLUX, MIXED>(a, s, tid);
-    __syncthreads();
-    double smx = 0.0, smy = 0.0;
-    ph_epilogue<TX, TY>(a, s, i0, j0, tid, dxp, dt, smx, smy);
-    if (a.mode == 3) {
-        smx = wave_max(smx);
-        smy = wave_max(smy);
-        if ((tid & 63) == 0) {
-            s.sRed[0][tid >> 6] = smx;
-            s.sRed[1][tid >> 6] = smy;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            double mx = 0.0, my = 0.0;
-#pragma unroll
-            for (int k = 0; k < T::NT / 64; ++k) {
-                mx = fmax(mx, s.sRed[0][k]);
-                my = fmax(my, s.sRed[1][k]);
-            }
-            atomicMax(&a.ctl->smax_x, (unsigned long long)__double_as_longlong(mx));
-            atomicMax(&a.ctl->smax_y, (unsigned long long)__double_as_longlong(my));
-        }
-    }
+    unsigned char* fl = LEARNED ? a.cls_flags + (size_t)tile * sizeof(s.sFlag) : nullptr;
+    stage_tile<TX, TY, FLUX, MIXED, LEARNED>(a, s, tile, dt, fl, threadIdx.x);
 }
 
-// dt from the accumulated max wave speeds (FV_System2D.cpp:633,645); a step
-// with t >= tFinal is marked inactive and every stage kernel skips it.  The
-// speeds are consumed (zeroed for stage 3 to accumulate afresh) only by an
-// active step: an inactive step computes none, so the speeds of the last
-// state carry over to the first step of a later advance.
-__global__ void k_dt(Ctl* c, double dx, double dy) {
-    double t = c->t, tf = c->tfinal;
-    if (t < tf) {
-        double sx = __longlong_as_double((long long)c->smax_x);
-        double sy = __longlong_as_double((long long)c->smax_y);
-        double dt = c->cfl * fmin(dx, dy) / fmax(sx, sy);
-        dt = t + dt > tf ? tf - t : dt;
-        c->dt = dt;
-        c->active = 1;
-        c->smax_x = 0ull;
-        c->smax_y = 0ull;
-    } else {
-        c->dt = 0.0;
-        c->active = 0;
-    }
-}
+// one-thread control kernels around a step (ctl_dt, ctl_tadvance)
+__global__ void k_dt(Ctl* c, double dx, double dy) { ctl_dt(c, dx, dy); }
 
-__global__ void k_tadvance(Ctl* c) {
-    if (c->active) {
-        c->t += c->dt;
-        c->nsteps += 1;
-    }
-}
+__global__ void k_tadvance(Ctl* c) { ctl_tadvance(c); }
 
 __global__ void k_reset_smax(Ctl* c) {
     c->smax_x = 0ull;

@@ -308,6 +308,89 @@ HTX_HD void ph_epilogue(const Fv2dArgs& a, const Fv2dTile<TX, TY>& s, int i0, in
     }
 }
 
+__device__ __forceinline__ double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// One RK stage (or residual) of tile `tile` of the instance described by a:
+// halo load, x and y reconstruction and fluxes, epilogue, and in stage 3
+// the tile's CFL maxima (one atomic per workgroup).  dt = the stage's time
+// step; fl = this tile's stored recon-face flags (learned precision).
+// Ends with every thread past its last LDS access only for mode 3; callers
+// that reuse the tile storage synchronise before the next tile.
+template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED>
+__device__ __forceinline__ void stage_tile(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tile, double dt,
+                                           unsigned char* fl, int tid) {
+    using T = Fv2dTile<TX, TY>;
+    const int i0 = (tile % a.tiles_x) * TX;
+    const int j0 = (tile / a.tiles_x) * TY;
+    double dxp[T::KC][4];
+
+    ph_load<TX, TY>(a, s, i0, j0, tid);
+    __syncthreads();
+    unsigned cnt[3] = {0, 0, 0};
+    if (LEARNED) {
+        // the classifier runs in the first stage of a step (and for a lone
+        // residual); stages 2 and 3 reuse the flags it stored for this tile
+        unsigned char* sf = &s.sFlag[0][0];
+        if (a.cls_fresh) {
+            ph_classify<TX, TY>(a, s, tid);
+            for (int k = tid; k < (int)sizeof(s.sFlag); k += T::NT) fl[k] = sf[k];
+        } else {
+            for (int k = tid; k < (int)sizeof(s.sFlag); k += T::NT) sf[k] = fl[k];
+            __syncthreads();
+        }
+    }
+    ph_recon<TX, TY, false, MIXED, LEARNED>(a, s, tid, i0, j0, cnt);
+    __syncthreads();
+    ph_flux<TX, TY, false, FLUX, MIXED>(a, s, tid);
+    __syncthreads();
+    ph_xdiv<TX, TY>(a, s, tid, dxp);
+    __syncthreads();
+    ph_recon<TX, TY, true, MIXED, LEARNED>(a, s, tid, i0, j0, cnt);
+    if (LEARNED) {
+        for (int k = 0; k < 3; ++k) {
+            unsigned v = cnt[k];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if ((tid & 63) == 0) s.sCnt[k][tid >> 6] = v;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long c3[3] = {0, 0, 0};
+            for (int k = 0; k < 3; ++k)
+                for (int w = 0; w < T::NT / 64; ++w) c3[k] += s.sCnt[k][w];
+            atomicAdd(&a.ctl->n_faces, c3[0]);
+            atomicAdd(&a.ctl->n_flagged, c3[1]);
+            atomicAdd(&a.ctl->n_linear, c3[2]);
+        }
+    }
+    __syncthreads();
+    ph_flux<TX, TY, true, FLUX, MIXED>(a, s, tid);
+    __syncthreads();
+    double smx = 0.0, smy = 0.0;
+    ph_epilogue<TX, TY>(a, s, i0, j0, tid, dxp, dt, smx, smy);
+    if (a.mode == 3) {
+        smx = wave_max(smx);
+        smy = wave_max(smy);
+        if ((tid & 63) == 0) {
+            s.sRed[0][tid >> 6] = smx;
+            s.sRed[1][tid >> 6] = smy;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double mx = 0.0, my = 0.0;
+#pragma unroll
+            for (int k = 0; k < T::NT / 64; ++k) {
+                mx = fmax(mx, s.sRed[0][k]);
+                my = fmax(my, s.sRed[1][k]);
+            }
+            atomicMax(&a.ctl->smax_x, (unsigned long long)__double_as_longlong(mx));
+            atomicMax(&a.ctl->smax_y, (unsigned long long)__double_as_longlong(my));
+        }
+    }
+}
+
 // Workgroups are dealt to the 8 XCDs round-robin; give each XCD one
 // contiguous run of tiles so neighbouring tiles share that XCD's L2.
 HTX_HD int xcd_remap(int b, int nb) {

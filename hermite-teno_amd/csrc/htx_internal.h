@@ -23,6 +23,36 @@ struct Ctl {
 
 enum FluxKind { FLUX_RUSANOV_CHAR = 0, FLUX_HLLC = 1, FLUX_HLLC_TORO = 2 };
 
+// dt of the next step from the accumulated max wave speeds
+// (FV_System2D.cpp:633,645); a step with t >= tFinal is marked inactive and
+// every stage kernel skips it.  The speeds are consumed (zeroed for stage 3
+// to accumulate afresh) only by an active step: an inactive step computes
+// none, so the speeds of the last state carry over to the first step of a
+// later advance.
+__device__ inline void ctl_dt(Ctl* c, double dx, double dy) {
+    double t = c->t, tf = c->tfinal;
+    if (t < tf) {
+        double sx = __longlong_as_double((long long)c->smax_x);
+        double sy = __longlong_as_double((long long)c->smax_y);
+        double dt = c->cfl * fmin(dx, dy) / fmax(sx, sy);
+        dt = t + dt > tf ? tf - t : dt;
+        c->dt = dt;
+        c->active = 1;
+        c->smax_x = 0ull;
+        c->smax_y = 0ull;
+    } else {
+        c->dt = 0.0;
+        c->active = 0;
+    }
+}
+
+__device__ inline void ctl_tadvance(Ctl* c) {
+    if (c->active) {
+        c->t += c->dt;
+        c->nsteps += 1;
+    }
+}
+
 struct Fv2dArgs {
     const double* qin;  // state whose residual is taken
     const double* q0;   // u^n (stages 2 and 3)

@@ -92,6 +92,19 @@ SIGNATURES = {
     "htx_classify": (_I, [_P, _I, _FP, _FP]),
     "htx_classifier_stats": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                                   ctypes.POINTER(ctypes.c_longlong), _I]),
+    "htx_batch_create": (_I, [ctypes.POINTER(HtxParams), _I, ctypes.POINTER(_P)]),
+    "htx_batch_destroy": (None, [_P]),
+    "htx_batch_set_state": (_I, [_P, _I, _DP, _S]),
+    "htx_batch_get_state": (_I, [_P, _I, _DP, _S]),
+    "htx_batch_init_quadrant": (_I, [_P, _I, _D, _D, _DP]),
+    "htx_batch_load_classifier": (_I, [_P, _FP, _D]),
+    "htx_batch_steps": (_I, [_P, _I]),
+    "htx_batch_advance": (_I, [_P, _I, _IP]),
+    "htx_batch_get_time": (_I, [_P, _I, _DP, _IP]),
+    "htx_batch_sync": (_I, [_P]),
+    "htx_batch_use_graph": (_I, [_P, _I]),
+    "htx_batch_set_tile": (_I, [_P, _I, _I]),
+    "htx_batch_tune": (_I, [_P, _I, _IP, _IP, _DP]),
     "htx_last_error": (ctypes.c_char_p, []),
     "htx_abi_version": (_I, []),
 }

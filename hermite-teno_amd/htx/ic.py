@@ -149,3 +149,26 @@ def piecewise_state(w, nx, ny, hermite=False):
         qp, qm = _cons(_piecewise_prim(w, X, Y + dy / 2.0), g1), _cons(_piecewise_prim(w, X, Y - dy / 2.0), g1)
         q += [(qp[e] - qm[e]) / dy for e in range(4)]
     return np.stack(q)
+
+
+def batch_workload(count=256, seed=SEED_BASE + 5, lo=64, hi=1024, tend=0.2):
+    """Serving-style sweep: `count` instances, each with nx, ny =
+    round(exp(U[ln lo, ln hi])) drawn independently, the domain
+    [0, nx/m] x [0, ny/m] with m = max(nx, ny) (square cells), a seeded
+    four-quadrant Riemann problem (quadrant_params) and a scheme drawn with
+    probability 1/2 each from TENO5 + HLLC and Hermite-TENO + Rusanov;
+    transmissive boundaries, gamma 1.4, CFL 0.5, t_end = tend.  Draw order per
+    instance: nx, ny, scheme, then the quadrant parameters."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        nx = int(round(math.exp(rng.uniform_in(math.log(lo), math.log(hi)))))
+        ny = int(round(math.exp(rng.uniform_in(math.log(lo), math.log(hi)))))
+        scheme = "teno5" if rng.uniform() < 0.5 else "hteno"
+        xs, ys, states = quadrant_params(rng)
+        m = max(nx, ny)
+        # the quadrant split point is drawn on the unit square; scale it into the domain
+        out.append(dict(kind="quadrant", scheme=scheme, flux="hllc" if scheme == "teno5" else "rusanov",
+                        xl=0.0, xr=nx / m, nx=nx, yb=0.0, yt=ny / m, ny=ny, cfl=0.5, tmax=tend, gamma=1.4,
+                        bc="zero_grad", xs=xs * nx / m, ys=ys * ny / m, states=states))
+    return out

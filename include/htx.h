@@ -227,6 +227,55 @@ int htx_classify(htx_solver* h, int dir, float* logits, float* feat);
  * falls back to TENO5.  reset != 0 zeroes the counts after reading. */
 int htx_classifier_stats(htx_solver* h, long long* n_faces, long long* n_flagged, long long* n_linear, int reset);
 
+/* ------------------------------------------------------------------------
+ * Batches: many independent problems of uneven size advanced together (no
+ * reference counterpart; the reference runs one System2D at a time).
+ *
+ * Every instance has its own parameter block (size, domain, end time tmax,
+ * CFL, boundary condition, scheme) and its own dt.  TENO5 instances run in
+ * one persistent launch per RK stage and must share flux and precision
+ * (fp64_fast, fp32_mixed or learned); Hermite instances run batched Hermite
+ * launches in fp64_fast and must share their flux.  A step can be captured
+ * once in a hipGraph and replayed.  Instances past their tmax are skipped.
+ * ------------------------------------------------------------------------ */
+typedef struct htx_batch htx_batch;
+
+int htx_batch_create(const htx_params* ps, int n, htx_batch** out);
+void htx_batch_destroy(htx_batch* b);
+
+/* Host state of instance i, the layout of htx_set_state / htx_get_state. */
+int htx_batch_set_state(htx_batch* b, int i, const double* q, size_t ld);
+int htx_batch_get_state(htx_batch* b, int i, double* q, size_t ld);
+
+/* Four-quadrant Riemann data for instance i, as htx_init_quadrant. */
+int htx_batch_init_quadrant(htx_batch* b, int i, double xs, double ys, const double* states);
+
+/* Classifier of the learned precision, as htx_load_classifier. */
+int htx_batch_load_classifier(htx_batch* b, const float* params, double threshold);
+
+/* Enqueues n steps of every instance (without blocking). */
+int htx_batch_steps(htx_batch* b, int n);
+
+/* Steps until every instance reaches its tmax or max_steps steps were
+ * enqueued; *nsteps receives the steps enqueued (may be null). */
+int htx_batch_advance(htx_batch* b, int max_steps, int* nsteps);
+
+int htx_batch_get_time(htx_batch* b, int i, double* t, int* nsteps);
+int htx_batch_sync(htx_batch* b);
+
+/* 1: capture a step into a hipGraph at the next htx_batch_steps and replay
+ * it; 0: launch kernel by kernel. */
+int htx_batch_use_graph(htx_batch* b, int enable);
+
+/* Tile shape of the TENO5 launch: 16x16, 32x8 or 8x32. */
+int htx_batch_set_tile(htx_batch* b, int tile_x, int tile_y);
+
+/* Autotuner: times `steps` steps of the batch with every built tile shape
+ * (from the current state, which is restored afterwards), keeps the fastest
+ * and returns it; ms (may be null) receives the per-step milliseconds of the
+ * three shapes in the order 16x16, 32x8, 8x32. */
+int htx_batch_tune(htx_batch* b, int steps, int* tile_x, int* tile_y, double* ms);
+
 /* Message of the last failed call on this host thread. */
 const char* htx_last_error(void);
 

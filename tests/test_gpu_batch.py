@@ -1,0 +1,124 @@
+"""Batches of independent problems: the persistent TENO5 launch and the
+batched Hermite launches reproduce single-instance solvers, every built tile
+shape gives the same result, graph replays equal eager launches, the
+autotuner restores the state it times from."""
+import numpy as np
+import pytest
+
+from conftest import rel_l1
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 1.4
+
+
+def instances():
+    from htx.ic import SplitMix64, quadrant_params
+
+    rng = SplitMix64(77)
+    out = []
+    for nx, ny, scheme in [(40, 24, "teno5"), (64, 64, "teno5"), (17, 33, "teno5"), (32, 32, "hteno"),
+                           (24, 40, "hteno"), (100, 20, "teno5")]:
+        xs, ys, st = quadrant_params(rng)
+        m = max(nx, ny)
+        out.append(dict(nx=nx, ny=ny, scheme=scheme, xr=nx / m, yt=ny / m, xs=xs * nx / m, ys=ys * ny / m,
+                        states=st, flux="hllc" if scheme == "teno5" else "rusanov"))
+    return out
+
+
+def make_batch(precision="fp64_fast", tmax=0.05):
+    from htx.batch import Batch, params_for
+
+    inst = instances()
+    ps = [params_for(0.0, d["xr"], d["nx"], 0.0, d["yt"], d["ny"], 0.5, tmax, GAMMA, scheme=d["scheme"],
+                     flux=d["flux"], precision=precision if d["scheme"] == "teno5" else "fp64_fast")
+          for d in inst]
+    b = Batch(ps)
+    for i, d in enumerate(inst):
+        b.quadrant(i, d["xs"], d["ys"], d["states"])
+    return b, inst
+
+
+def single(d, precision, nsteps):
+    import htx
+
+    s = htx.System2D(0.0, d["xr"], d["nx"], 0.0, d["yt"], d["ny"], 0.5, 10.0, 0.0, GAMMA, scheme=d["scheme"],
+                     flux=d["flux"], bc="zero_grad", precision=precision if d["scheme"] == "teno5" else "fp64_fast")
+    s.quadrant(d["xs"], d["ys"], d["states"])
+    s.steps_async(nsteps)
+    return s.get_state(), s.time
+
+
+def test_batch_matches_single_instances():
+    print("[case] batch of 6 (4 TENO5, 2 Hermite) vs single solvers, 5 steps", flush=True)
+    b, inst = make_batch(tmax=10.0)
+    b.steps(5)
+    for i, d in enumerate(inst):
+        q_ref, t_ref = single(d, "fp64_fast", 5)
+        q = b.get_state(i)
+        t, n = b.time(i)
+        err = rel_l1(q, q_ref)
+        print(f"instance {i} {d['scheme']} {d['nx']}x{d['ny']}: rel L1 {err:.3e}, t {t:.6g} (single {t_ref:.6g})",
+              flush=True)
+        assert n == 5
+        assert t == pytest.approx(t_ref, rel=1e-12)
+        assert err < 1e-12
+
+
+def test_batch_tile_shapes_agree():
+    """The tile shape changes where faces are computed, not what: every shape
+    gives the same state bit for bit."""
+    print("[case] batch tile shapes 16x16, 32x8, 8x32", flush=True)
+    ref = None
+    for tx, ty in [(16, 16), (32, 8), (8, 32)]:
+        b, inst = make_batch(tmax=10.0)
+        b.set_tile(tx, ty)
+        b.steps(4)
+        qs = [b.get_state(i) for i in range(len(inst))]
+        if ref is None:
+            ref = qs
+        else:
+            for q, r in zip(qs, ref):
+                np.testing.assert_array_equal(q, r)
+        b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64_fast", "learned"])
+def test_batch_graph_replays_match_eager(precision):
+    """A captured step replayed several times, with eager work and
+    synchronisation between the replays, equals eager launches after every
+    replay."""
+    print(f"[case] batch graph replay vs eager ({precision})", flush=True)
+    g, inst = make_batch(precision, tmax=10.0)
+    e, _ = make_batch(precision, tmax=10.0)
+    if precision == "learned":
+        g.load_classifier()
+        e.load_classifier()
+    g.use_graph(True)
+    for k in range(4):
+        g.steps(2)
+        e.steps(2)
+        g.sync()
+        e.sync()
+        for i in range(len(inst)):
+            np.testing.assert_array_equal(g.get_state(i), e.get_state(i))
+            assert g.time(i) == e.time(i)
+
+
+def test_batch_advance_and_tune():
+    print("[case] batch advance to per-instance end times, autotuner", flush=True)
+    b, inst = make_batch(tmax=0.02)
+    before = [b.get_state(i) for i in range(len(inst))]
+    shape, ms = b.tune(steps=2)
+    print(f"autotuner: {shape}, ms per step {ms}", flush=True)
+    assert shape in [(16, 16), (32, 8), (8, 32)]
+    assert all(m > 0 for m in ms)
+    for i in range(len(inst)):
+        np.testing.assert_array_equal(b.get_state(i), before[i])
+        assert b.time(i) == (0.0, 0)
+    n = b.advance()
+    for i in range(len(inst)):
+        t, _ = b.time(i)
+        assert t == pytest.approx(0.02, rel=1e-14)
+        assert np.isfinite(b.get_state(i)).all()
+    assert n >= 1
