@@ -112,6 +112,76 @@ def study():
     return sb, acc
 
 
+def batch_study(count=256, steps=20, t_compare=0.02):
+    """Serving-style sweep: `count` seeded quadrant problems with sides
+    log-uniform in [64, 1024], half TENO5 + HLLC in the FP32-mixed precision
+    (FP32 cut-off decisions and nonlinear weights, FP64 values, fluxes and RK
+    accumulation), half Hermite-TENO + Rusanov (FP64 fast), advanced together; aggregate
+    cell-updates/s over `steps` steps from t = 0 (every instance active;
+    graph replay, autotuned tile shape) and each instance's relative L1
+    density difference at t = t_compare against the plain FP64 drop-in solver
+    run alone (t_compare shortens the workload's t_end = 0.2 to bound the
+    bench's run time)."""
+    import numpy as np
+
+    import htx
+    from htx.batch import Batch, params_for
+    from htx.ic import batch_workload
+
+    ws = batch_workload(count)
+
+    def make(tmax):
+        ps = [params_for(w["xl"], w["xr"], w["nx"], w["yb"], w["yt"], w["ny"], w["cfl"], tmax, w["gamma"],
+                         scheme=w["scheme"], flux=w["flux"],
+                         precision="fp32_mixed" if w["scheme"] == "teno5" else "fp64_fast") for w in ws]
+        bb = Batch(ps)
+        for i, w in enumerate(ws):
+            bb.quadrant(i, w["xs"], w["ys"], w["states"])
+        return bb
+
+    b = make(ws[0]["tmax"])
+    cells = sum(w["nx"] * w["ny"] for w in ws)
+    out = {"instances": count, "cells": cells,
+           "teno5_instances": sum(w["scheme"] == "teno5" for w in ws),
+           "workload": f"{count} seeded quadrant problems, sides log-uniform in [64, 1024], TENO5+HLLC (fp32_mixed) "
+                       f"or Hermite-TENO+Rusanov (fp64_fast) with probability 1/2, t_end 0.2"}
+    log("[batch] autotuning the TENO5 tile shape")
+    shape, ms = b.tune(steps=2)
+    out["tile"] = f"{shape[0]}x{shape[1]}"
+    out["tune_ms_per_step"] = {"16x16": ms[0], "32x8": ms[1], "8x32": ms[2]}
+    for graph in (False, True):
+        b.use_graph(graph)
+        b.steps(2)
+        b.sync()
+        t0 = time.perf_counter()
+        b.steps(steps)
+        b.sync()
+        el = time.perf_counter() - t0
+        out["graph" if graph else "eager"] = {"value": cells * steps / el, "unit": "cell-updates/s",
+                                              "ms_per_step": el * 1e3 / steps}
+    b.close()
+    if t_compare:
+        log(f"[batch] advancing every instance to t = {t_compare}")
+        b = make(t_compare)
+        b.set_tile(*shape)
+        b.use_graph(True)
+        b.advance()
+        errs = {"teno5": [], "hteno": []}
+        for i, w in enumerate(ws):
+            s = htx.System2D(w["xl"], w["xr"], w["nx"], w["yb"], w["yt"], w["ny"], w["cfl"], t_compare, 0.0,
+                             w["gamma"], scheme=w["scheme"], flux=w["flux"], bc=w["bc"], precision="fp64")
+            s.quadrant(w["xs"], w["ys"], w["states"])
+            s.timeStepping(t_compare)
+            ref = s.get_state()[0]
+            s.close()
+            q = b.get_state(i)[0]
+            errs[w["scheme"]].append(float(np.abs(q - ref).sum() / np.abs(ref).sum()))
+        b.close()
+        out["l1_rho_vs_fp64"] = {"t": t_compare, **{k: {"median": float(np.median(v)), "max": float(np.max(v)),
+                                                        "count": len(v)} for k, v in errs.items() if v}}
+    return out
+
+
 def time_variant(n, steps, warmup, **kw):
     """Cell-updates/s of one solver variant on the same workload (for the
     report's `variants` block)."""
@@ -254,6 +324,10 @@ def main():
             out["shock_blobs"], out["matched_accuracy"] = study()
         except Exception as e:
             log(f"[gpu] study failed: {e}")
+        try:
+            out["batch"] = batch_study()
+        except Exception as e:
+            log(f"[gpu] batch study failed: {e}")
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)

@@ -310,7 +310,7 @@ int htx_create(const htx_params* p, htx_solver** out) {
     c.t = p->tinit;
     c.tfinal = p->tmax;
     c.cfl = p->cfl;
-    if (hipMemcpy(h->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess) {
+    if (copy_sync(h->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, h->stream) != hipSuccess) {
         htx_destroy(h);
         return fail(HTX_EDEVICE, "control block upload failed");
     }
@@ -629,8 +629,8 @@ int htx_trainer_create(long long capacity, htx_trainer** out) {
         htx_trainer_destroy(t);
         return fail(HTX_ENOMEM, "device allocation failed");
     }
-    if (hipMemset(t->dCount, 0, sizeof(unsigned long long) * 8) != hipSuccess ||
-        hipMemset(t->dPrm, 0, sizeof(float) * CLS_NPARAM) != hipSuccess) {
+    if (zero_sync(t->dCount, sizeof(unsigned long long) * 8, t->stream) != hipSuccess ||
+        zero_sync(t->dPrm, sizeof(float) * CLS_NPARAM, t->stream) != hipSuccess) {
         htx_trainer_destroy(t);
         return fail(HTX_EDEVICE, "device initialisation failed");
     }
@@ -678,7 +678,7 @@ int htx_trainer_add_state(htx_trainer* t, htx_solver* h) {
 int htx_trainer_size(htx_trainer* t, long long* n, long long* n_smooth) {
     if (!t) return fail(HTX_EINVAL, "null handle");
     unsigned long long c = 0;
-    HIP_TRY(hipMemcpy(&c, t->dCount, sizeof c, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(&c, t->dCount, sizeof c, hipMemcpyDeviceToHost, t->stream));
     if (n) *n = t->n;
     if (n_smooth) *n_smooth = (long long)c;
     return HTX_OK;
@@ -687,8 +687,9 @@ int htx_trainer_size(htx_trainer* t, long long* n, long long* n_smooth) {
 int htx_trainer_read(htx_trainer* t, long long start, long long n, float* feat, unsigned char* label) {
     if (!t) return fail(HTX_EINVAL, "null handle");
     if (start < 0 || n < 0 || start + n > t->n) return fail(HTX_EINVAL, "range outside the training set");
-    if (feat) HIP_TRY(hipMemcpy(feat, t->dFeat + start * CLS_IN, sizeof(float) * CLS_IN * n, hipMemcpyDeviceToHost));
-    if (label) HIP_TRY(hipMemcpy(label, t->dLabel + start, (size_t)n, hipMemcpyDeviceToHost));
+    if (feat)
+        HIP_TRY(copy_sync(feat, t->dFeat + start * CLS_IN, sizeof(float) * CLS_IN * n, hipMemcpyDeviceToHost, t->stream));
+    if (label) HIP_TRY(copy_sync(label, t->dLabel + start, (size_t)n, hipMemcpyDeviceToHost, t->stream));
     return HTX_OK;
 }
 
@@ -703,23 +704,22 @@ int htx_trainer_init(htx_trainer* t, unsigned long long seed) {
     fill(CLS_W1, CLS_IN, CLS_H);
     fill(CLS_W2, CLS_H, CLS_H);
     fill(CLS_W3, CLS_H, 1);
-    HIP_TRY(hipMemcpy(t->dPrm, prm.data(), sizeof(float) * CLS_NPARAM, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(t->dM, 0, sizeof(float) * CLS_NPARAM));
-    HIP_TRY(hipMemset(t->dV, 0, sizeof(float) * CLS_NPARAM));
+    HIP_TRY(copy_sync(t->dPrm, prm.data(), sizeof(float) * CLS_NPARAM, hipMemcpyHostToDevice, t->stream));
+    HIP_TRY(zero_sync(t->dM, sizeof(float) * CLS_NPARAM, t->stream));
+    HIP_TRY(zero_sync(t->dV, sizeof(float) * CLS_NPARAM, t->stream));
     t->adam_t = 0;
     return HTX_OK;
 }
 
 int htx_trainer_set_params(htx_trainer* t, const float* params) {
     if (!t || !params) return fail(HTX_EINVAL, "null argument");
-    HIP_TRY(hipMemcpy(t->dPrm, params, sizeof(float) * CLS_NPARAM, hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(t->dPrm, params, sizeof(float) * CLS_NPARAM, hipMemcpyHostToDevice, t->stream));
     return HTX_OK;
 }
 
 int htx_trainer_get_params(htx_trainer* t, float* params) {
     if (!t || !params) return fail(HTX_EINVAL, "null argument");
-    HIP_TRY(hipStreamSynchronize(t->stream));
-    HIP_TRY(hipMemcpy(params, t->dPrm, sizeof(float) * CLS_NPARAM, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(params, t->dPrm, sizeof(float) * CLS_NPARAM, hipMemcpyDeviceToHost, t->stream));
     return HTX_OK;
 }
 
@@ -728,7 +728,7 @@ int htx_trainer_gradient(htx_trainer* t, double troubled_weight, float* grad, do
     if (t->n < 1) return fail(HTX_ESTATE, "the training set is empty");
     if (!(troubled_weight > 0.0)) return fail(HTX_EINVAL, "troubled_weight must be positive");
     unsigned long long nsm = 0;
-    HIP_TRY(hipMemcpy(&nsm, t->dCount, sizeof nsm, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(&nsm, t->dCount, sizeof nsm, hipMemcpyDeviceToHost, t->stream));
     double fpos = (double)nsm / (double)t->n, fneg = 1.0 - fpos;
     if (fpos <= 0.0 || fneg <= 0.0) return fail(HTX_ESTATE, "the training set holds only one class");
     int nblk = (int)((t->n + TRAIN_SAMPLES_PER_BLOCK - 1) / TRAIN_SAMPLES_PER_BLOCK);
@@ -780,7 +780,7 @@ int htx_trainer_train(htx_trainer* t, int epochs, int batch, double lr, double t
     if (epochs < 0 || batch < TRAIN_SAMPLES_PER_BLOCK || !(lr > 0.0) || !(troubled_weight > 0.0))
         return fail(HTX_EINVAL, "epochs >= 0, batch >= 128, lr > 0 and troubled_weight > 0 required");
     unsigned long long nsm = 0;
-    HIP_TRY(hipMemcpy(&nsm, t->dCount, sizeof nsm, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(&nsm, t->dCount, sizeof nsm, hipMemcpyDeviceToHost, t->stream));
     double fpos = (double)nsm / (double)t->n, fneg = 1.0 - fpos;
     if (fpos <= 0.0 || fneg <= 0.0) return fail(HTX_ESTATE, "the training set holds only one class");
     int nblk = (batch + TRAIN_SAMPLES_PER_BLOCK - 1) / TRAIN_SAMPLES_PER_BLOCK;
@@ -885,8 +885,8 @@ int htx_load_classifier(htx_solver* h, const float* params, double threshold) {
                       hipMalloc(&h->dCvec, sizeof(float) * CLS_NCVEC) != hipSuccess))
         return fail(HTX_ENOMEM, "device allocation failed");
     HIP_TRY(hipStreamSynchronize(h->stream));
-    HIP_TRY(hipMemcpy(h->dFrag, frag.data(), sizeof(unsigned short) * CLS_NFRAG, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(h->dCvec, cvec.data(), sizeof(float) * CLS_NCVEC, hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(h->dFrag, frag.data(), sizeof(unsigned short) * CLS_NFRAG, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(copy_sync(h->dCvec, cvec.data(), sizeof(float) * CLS_NCVEC, hipMemcpyHostToDevice, h->stream));
     h->cls_threshold = (float)threshold;
     h->have_classifier = true;
     return HTX_OK;

@@ -115,6 +115,21 @@ HTX_HD int map_index(int g, int n, int bc) {
     return g < 0 ? 0 : (g >= n ? n - 1 : g);
 }
 
+// Host-side copies and fills of the library's objects run on the object's own
+// stream and are waited for.  Those streams are non-blocking: a copy issued
+// on the null stream is not ordered against their kernels, and a pageable
+// host-to-device copy may return before its data has landed, so a kernel
+// launched next on the object's stream could read the old contents.
+inline hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+
+inline hipError_t zero_sync(void* dst, size_t bytes, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(dst, 0, bytes, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+
 // launchers (htx_fv2d.hip)
 hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y, hipStream_t s);
 // bytes of the learned precision's per-tile face-flag buffer for an nx x ny grid

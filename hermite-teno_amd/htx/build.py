@@ -8,6 +8,8 @@ Everything builds in-tree with plain hipcc / gcc command lines:
                                      CPU baseline timing
   tests/host/libfv2d_host.so         CPU build of the stage kernel's tiling
                                      (test infrastructure)
+  tests/host/libbatch_host.so        CPU walk of the batched launches'
+                                     bookkeeping (test infrastructure)
 """
 import os
 import subprocess
@@ -84,6 +86,7 @@ def build_oracle():
 HOST_HARNESS = os.path.join(ROOT, "tests", "host", "libfv2d_host.so")
 HOST_HARNESS_FAST = os.path.join(ROOT, "tests", "host", "libfv2d_host_fast.so")
 HOST_HARNESS_HERMITE = os.path.join(ROOT, "tests", "host", "libfvh2d_host.so")
+HOST_HARNESS_BATCH = os.path.join(ROOT, "tests", "host", "libbatch_host.so")
 
 
 def build_host_harness():
@@ -98,6 +101,8 @@ def build_host_harness():
     _run(base + ["-o", HOST_HARNESS_FAST, src])
     srch = os.path.join(ROOT, "tests", "host", "fvh2d_host.hip")
     _run(base + ["-DHTX_EXACT_DIVISION=1", "-o", HOST_HARNESS_HERMITE, srch])
+    srcb = os.path.join(ROOT, "tests", "host", "batch_host.hip")
+    _run(base + ["-o", HOST_HARNESS_BATCH, srcb])
     return HOST_HARNESS
 
 

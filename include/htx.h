@@ -234,8 +234,9 @@ int htx_classifier_stats(htx_solver* h, long long* n_faces, long long* n_flagged
  * Every instance has its own parameter block (size, domain, end time tmax,
  * CFL, boundary condition, scheme) and its own dt.  TENO5 instances run in
  * one persistent launch per RK stage and must share flux and precision
- * (fp64_fast, fp32_mixed or learned); Hermite instances run batched Hermite
- * launches in fp64_fast and must share their flux.  A step can be captured
+ * (fp64_fast or fp32_mixed; the learned precision is not available in
+ * batches and htx_batch_create returns HTX_ENOTSUP for it); Hermite instances
+ * run batched Hermite launches in fp64_fast and must share their flux.  A step can be captured
  * once in a hipGraph and replayed.  Instances past their tmax are skipped.
  * ------------------------------------------------------------------------ */
 typedef struct htx_batch htx_batch;
@@ -250,7 +251,8 @@ int htx_batch_get_state(htx_batch* b, int i, double* q, size_t ld);
 /* Four-quadrant Riemann data for instance i, as htx_init_quadrant. */
 int htx_batch_init_quadrant(htx_batch* b, int i, double xs, double ys, const double* states);
 
-/* Classifier of the learned precision, as htx_load_classifier. */
+/* Classifier of the learned precision, as htx_load_classifier (kept for
+ * the learned batches; no batch runs that precision at present). */
 int htx_batch_load_classifier(htx_batch* b, const float* params, double threshold);
 
 /* Enqueues n steps of every instance (without blocking). */

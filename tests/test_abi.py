@@ -75,3 +75,16 @@ def test_python_mirror_raises_typed_error():
     with pytest.raises(htx.HtxError) as ei:
         htx.System2D(0.0, 1.0, 0, 0.0, 1.0, 8, 0.5, 1.0, 0.0, 1.4)
     assert ei.value.code == -1
+
+
+def test_batch_rejects_learned_precision():
+    """Batches run TENO5 in fp64_fast or fp32_mixed; the learned precision
+    is refused (HTX_ENOTSUP) before any device work."""
+    import htx
+    from htx.batch import Batch, params_for
+
+    ps = [params_for(0.0, 1.0, 32, 0.0, 1.0, 32, 0.5, 0.1, 1.4, precision="learned")]
+    with pytest.raises(htx.HtxError) as ei:
+        Batch(ps)
+    assert ei.value.code == -5
+    assert "fp64_fast or fp32_mixed" in str(ei.value)

@@ -83,7 +83,12 @@ def test_batch_tile_shapes_agree():
         b.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64_fast", "learned"])
+LEARNED_OFF = pytest.mark.skip(
+    reason="batched learned precision is switched off (htx_batch_create returns HTX_ENOTSUP): its persistent "
+           "launch faulted the GPU with an illegal address inside htx_batch_tune, cause not yet found")
+
+
+@pytest.mark.parametrize("precision", ["fp64_fast", pytest.param("learned", marks=LEARNED_OFF)])
 def test_batch_graph_replays_match_eager(precision):
     """A captured step replayed several times, with eager work and
     synchronisation between the replays, equals eager launches after every
@@ -122,3 +127,53 @@ def test_batch_advance_and_tune():
         assert t == pytest.approx(0.02, rel=1e-14)
         assert np.isfinite(b.get_state(i)).all()
     assert n >= 1
+
+
+@pytest.mark.parametrize("precision", ["fp64_fast", "fp32_mixed", pytest.param("learned", marks=LEARNED_OFF)])
+def test_batch_many_tiles_per_workgroup(precision):
+    """More TENO5 tiles than the persistent grid holds (3 x 19 x 17 tiles of
+    16 x 16 against 2 workgroups per CU), so each workgroup walks several
+    tiles; the tables are rebuilt by the autotuner in between.  Every
+    instance equals its single-instance solver."""
+    import htx
+    from htx.batch import Batch, params_for
+    from htx.ic import SplitMix64, quadrant_params
+
+    print(f"[case] batch with several tiles per workgroup ({precision}), tune, 3 steps", flush=True)
+    rng = SplitMix64(5)
+    inst = []
+    for nx, ny, scheme in [(300, 260, "teno5"), (128, 96, "hteno"), (290, 270, "teno5"), (310, 250, "teno5")]:
+        xs, ys, st = quadrant_params(rng)
+        m = max(nx, ny)
+        inst.append(dict(nx=nx, ny=ny, scheme=scheme, xr=nx / m, yt=ny / m, xs=xs * nx / m, ys=ys * ny / m,
+                         states=st, flux="hllc" if scheme == "teno5" else "rusanov"))
+    ps = [params_for(0.0, d["xr"], d["nx"], 0.0, d["yt"], d["ny"], 0.5, 10.0, GAMMA, scheme=d["scheme"],
+                     flux=d["flux"], precision=precision if d["scheme"] == "teno5" else "fp64_fast") for d in inst]
+    b = Batch(ps)
+    for i, d in enumerate(inst):
+        b.quadrant(i, d["xs"], d["ys"], d["states"])
+    if precision == "learned":
+        b.load_classifier()
+    shape, _ = b.tune(steps=1)
+    print(f"autotuner: {shape}", flush=True)
+    b.set_tile(16, 16)
+    b.steps(3)
+    for i, d in enumerate(inst):
+        s = htx.System2D(0.0, d["xr"], d["nx"], 0.0, d["yt"], d["ny"], 0.5, 10.0, 0.0, GAMMA, scheme=d["scheme"],
+                         flux=d["flux"], bc="zero_grad",
+                         precision=precision if d["scheme"] == "teno5" else "fp64_fast")
+        s.quadrant(d["xs"], d["ys"], d["states"])
+        s.steps_async(3)
+        q_ref, t_ref = s.get_state(), s.time
+        s.close()
+        q = b.get_state(i)
+        t, n = b.time(i)
+        err = rel_l1(q, q_ref)
+        print(f"instance {i} {d['scheme']} {d['nx']}x{d['ny']}: rel L1 {err:.3e}", flush=True)
+        assert n == 3
+        assert t == pytest.approx(t_ref, rel=1e-12)
+        # the batched and the single-instance kernels are separate builds of
+        # the same tile code; FP64 rounding-level differences between them
+        # can flip an FP32 cut-off decision of the mixed precision
+        assert err < (1e-10 if precision == "fp32_mixed" and d["scheme"] == "teno5" else 1e-12)
+    b.close()
