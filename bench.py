@@ -112,6 +112,21 @@ def study():
     return sb, acc
 
 
+def precision_study(n=2048):
+    """Seeded four-quadrant Riemann problem at n x n on [0,1]^2, transmissive,
+    TENO5 + HLLC to t = 0.25: L1 and Linf error per conserved variable of
+    fp32_mixed and learned against the plain FP64 GPU solution."""
+    from htx.ic import quadrant_workload
+    from htx.study import precision_report
+
+    w = quadrant_workload(n)
+    log(f"[precision] quadrant {n}^2 to t = {w['tmax']}: fp64, fp32_mixed, learned")
+    r = precision_report(w, n, n)
+    r["workload"] = (f"seeded four-quadrant Riemann problem, {n} x {n} on [0,1]^2, transmissive, TENO5 + HLLC, "
+                     f"CFL {w['cfl']}, t_end {w['tmax']}; errors against plain FP64 on the GPU")
+    return r
+
+
 def batch_study(count=256, steps=20, t_compare=0.02):
     """Serving-style sweep: `count` seeded quadrant problems with sides
     log-uniform in [64, 1024], half TENO5 + HLLC in the FP32-mixed precision
@@ -324,6 +339,10 @@ def main():
             out["shock_blobs"], out["matched_accuracy"] = study()
         except Exception as e:
             log(f"[gpu] study failed: {e}")
+        try:
+            out["precision"] = precision_study()
+        except Exception as e:
+            log(f"[gpu] precision study failed: {e}")
         try:
             out["batch"] = batch_study()
         except Exception as e:

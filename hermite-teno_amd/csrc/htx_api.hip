@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/htx.h"
+#include "htx_fv2d_tile.h"
 #include "htx_hermite.h"
 #include "htx_learn.h"
 #include "htx_learn_api.h"
@@ -847,7 +848,22 @@ int htx_trainer_train(htx_trainer* t, int epochs, int batch, double lr, double t
     return HTX_OK;
 }
 
+namespace {
+int trainer_eval(htx_trainer* t, double threshold, long long* counts, float* logits, int rounding);
+}
+
 int htx_trainer_eval(htx_trainer* t, double threshold, long long* counts, float* logits) {
+    return trainer_eval(t, threshold, counts, logits, 0);
+}
+
+int htx_trainer_eval_inference(htx_trainer* t, double threshold, long long* counts, float* logits) {
+    return trainer_eval(t, threshold, counts, logits, 1);
+}
+
+}  // extern "C"
+
+namespace {
+int trainer_eval(htx_trainer* t, double threshold, long long* counts, float* logits, int rounding) {
     if (!t || !counts) return fail(HTX_EINVAL, "null argument");
     if (t->n < 1) return fail(HTX_ESTATE, "the training set is empty");
     float* dLog = nullptr;
@@ -862,6 +878,7 @@ int htx_trainer_eval(htx_trainer* t, double threshold, long long* counts, float*
     a.threshold = (float)threshold;
     a.counts = t->dCount + 1;
     a.logits = dLog;
+    a.inference_rounding = rounding;
     hipError_t e = lrn_launch_eval(a, t->stream);
     unsigned long long c[4] = {0, 0, 0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(c, t->dCount + 1, sizeof c, hipMemcpyDeviceToHost, t->stream);
@@ -873,6 +890,9 @@ int htx_trainer_eval(htx_trainer* t, double threshold, long long* counts, float*
     for (int k = 0; k < 4; ++k) counts[k] = (long long)c[k];
     return HTX_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int htx_load_classifier(htx_solver* h, const float* params, double threshold) {
     if (!h || !params) return fail(HTX_EINVAL, "null argument");
@@ -889,6 +909,34 @@ int htx_load_classifier(htx_solver* h, const float* params, double threshold) {
     HIP_TRY(copy_sync(h->dCvec, cvec.data(), sizeof(float) * CLS_NCVEC, hipMemcpyHostToDevice, h->stream));
     h->cls_threshold = (float)threshold;
     h->have_classifier = true;
+    return HTX_OK;
+}
+
+int htx_classifier_flags(htx_solver* h, unsigned char* fx, unsigned char* fy) {
+    if (!h || !fx || !fy) return fail(HTX_EINVAL, "null argument");
+    if (h->p.precision != HTX_PREC_LEARNED || !h->dFlags)
+        return fail(HTX_ESTATE, "the solver does not run the learned precision");
+    // the stage kernel's per-tile buffer: 16 x 16 tiles, x recon faces then
+    // y recon faces of Fv2dTile<16, 16>, in the tile's face order
+    using T = Fv2dTile<16, 16>;
+    const int tx = (h->nx + 15) / 16, ty = (h->ny + 15) / 16;
+    std::vector<unsigned char> buf(fv2d_flag_bytes(h->nx, h->ny));
+    HIP_TRY(copy_sync(buf.data(), h->dFlags, buf.size(), hipMemcpyDeviceToHost, h->stream));
+    const size_t per = buf.size() / ((size_t)tx * ty);
+    for (int t = 0; t < tx * ty; ++t) {
+        const int i0 = (t % tx) * 16, j0 = (t / tx) * 16;
+        const unsigned char* f = buf.data() + (size_t)t * per;
+        for (int w = 0; w < T::NXF; ++w) {
+            int fi = w % 17, fj = w / 17;
+            if (!recon_face_owned<16, 16, false>(fi, fj, i0, j0, h->nx, h->ny)) continue;
+            fx[(size_t)(j0 + fj - 2) * h->nx + (i0 + fi - 1)] = f[w];
+        }
+        for (int w = 0; w < T::NYF; ++w) {
+            int fi = w % 20, fj = w / 20;
+            if (!recon_face_owned<16, 16, true>(fi, fj, i0, j0, h->nx, h->ny)) continue;
+            fy[(size_t)(j0 + fj - 1) * h->nx + (i0 + fi - 2)] = f[T::NF + w];
+        }
+    }
     return HTX_OK;
 }
 

@@ -111,11 +111,23 @@ __device__ void ph_classify(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
 }
 
 // FP64 positivity of the conservative state that characteristic face values
-// c[0..3] back-project to (density and pressure > 0).
-HTX_HD bool face_state_positive(const Eig& E, double g1, const double c[4]) {
+// c[0..3] back-project to: density > 0 and pressure (gamma - 1)(E - m^2 / 2
+// rho) > 0, the latter tested as rho E - m^2 / 2 > 0 (same sign for rho > 0
+// and gamma > 1, no division).
+HTX_HD bool face_state_positive(const Eig& E, const double c[4]) {
     double q0, qn, qt, q3;
     eig_right(E, c[0], c[1], c[2], c[3], q0, qn, qt, q3);
-    return q0 > 0.0 && (q3 - 0.5 * (qn * qn + qt * qt) / q0) * g1 > 0.0;
+    return q0 > 0.0 && q3 * q0 - 0.5 * (qn * qn + qt * qt) > 0.0;
+}
+
+// Whether p holds on every active lane of the wavefront (on the device) or
+// for this work item (on the host, where a phase runs item by item).
+HTX_HD bool wave_all(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __all(p ? 1 : 0) != 0;
+#else
+    return p;
+#endif
 }
 
 // Whether recon face (fi, fj) of family YDIR is one the tile owns: every
@@ -132,7 +144,8 @@ HTX_HD bool recon_face_owned(int fi, int fj, int i0, int j0, int nx, int ny) {
 // LEARNED, faces the classifier flagged take Lagrange::linear
 // (Schemes_Lagrange.cpp:1219-1233) instead, provided the linear face values
 // give positive density and pressure on both sides (checked in FP64; else
-// TENO5); cnt accumulates (owned faces, flagged, linear) of this thread.
+// TENO5); the face's flag is then overwritten with the scheme it took (1 =
+// linear), and cnt accumulates (owned faces, flagged, linear) of this thread.
 template <int TX, int TY, bool YDIR, bool MIXED = false, bool LEARNED = false>
 HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid, int i0 = 0, int j0 = 0,
                      unsigned* cnt = nullptr) {
@@ -157,7 +170,7 @@ HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid, int i0 = 0
                 double lm[4], lp[4];
 #pragma unroll
                 for (int f = 0; f < 4; ++f) linear5(c[f], lm[f], lp[f]);
-                lin = face_state_positive(E, a.g1, lm) && face_state_positive(E, a.g1, lp);
+                lin = face_state_positive(E, lm) && face_state_positive(E, lp);
                 if (lin) {
 #pragma unroll
                     for (int f = 0; f < 4; ++f) {
@@ -171,6 +184,9 @@ HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid, int i0 = 0
                 cnt[1] += flag ? 1 : 0;
                 cnt[2] += lin ? 1 : 0;
             }
+            // from here on the flag records what the face took (the flux
+            // phase of this direction reads it)
+            s.sFlag[YDIR ? 1 : 0][w] = lin ? 1 : 0;
         }
         if (!lin) {
 #pragma unroll
@@ -187,10 +203,42 @@ HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid, int i0 = 0
     }
 }
 
+// Learned path: whether flux face (fi, fj) of family YDIR may take the
+// linear transverse quadrature.  Its 5 transverse recon faces must all have
+// taken the linear scheme (normal smoothness along the window), and the
+// classifier must have flagged smooth the 2 x 2 recon faces of the other
+// family that sit on the face's two cells across the face's row (x flux)
+// or column (y flux): their 6-cell stencils cover the window's transverse
+// extent, so a discontinuity running along the face normal through the
+// window is seen there.
+template <int TX, int TY, bool YDIR>
+HTX_HD bool flux_window_smooth(const Fv2dTile<TX, TY>& s, int fi, int fj) {
+    if (!YDIR) {
+        const unsigned char* N = s.sFlag[0];
+        const int b = fj * (TX + 1) + fi;
+        bool ok = N[b] & N[b + (TX + 1)] & N[b + 2 * (TX + 1)] & N[b + 3 * (TX + 1)] & N[b + 4 * (TX + 1)];
+        const unsigned char* C = s.sFlag[1];  // y recon face (column 1 + fi, between rows 2 + fj, 3 + fj)
+        const int c = fj * (TX + 4) + 1 + fi;
+        return ok && (C[c] & C[c + 1] & C[c + (TX + 4)] & C[c + (TX + 4) + 1]);
+    }
+    const unsigned char* N = s.sFlag[1];
+    const int b = fj * (TX + 4) + fi;
+    bool ok = N[b] & N[b + 1] & N[b + 2] & N[b + 3] & N[b + 4];
+    const unsigned char* C = s.sFlag[0];  // x recon face (between columns 2 + fi, 3 + fi, row 1 + fj)
+    const int c = (1 + fj) * (TX + 1) + fi;
+    return ok && (C[c] & C[c + 1] & C[c + (TX + 1)] & C[c + (TX + 1) + 1]);
+}
+
 // transverse 3-point quadrature, back-projection with the face's own R,
 // Riemann flux at the Gauss points weighted by (5/18, 4/9, 5/18)
 // (FV_System2D.cpp:402-497); the flux is stored in (rho, x, y, E) order.
-template <int TX, int TY, bool YDIR, int FLUX, bool MIXED = false>
+// With LEARNED, when every face a wavefront handles has a smooth window
+// (flux_window_smooth), those faces take the quadrature's linear limit,
+// except a face whose back-projected Gauss-point states then include a
+// non-positive density or pressure (checked in FP64); every other face takes
+// the nonlinear quadrature.  The wavefront-wide decision keeps the two paths
+// from running one after the other in the same wavefront.
+template <int TX, int TY, bool YDIR, int FLUX, bool MIXED = false, bool LEARNED = false>
 HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
     using T = Fv2dTile<TX, TY>;
     constexpr int NW = YDIR ? T::NYQ : T::NXQ;
@@ -203,24 +251,49 @@ HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
         // first of the 5 transverse recon faces, and the face's left/lower cell
         int b = YDIR ? fj * RR + fi : fj * RR + fi;
         int ca = YDIR ? (2 + fj) * T::HX + (3 + fi) : (3 + fj) * T::HX + (2 + fi);
+        // the learned path needs the eigensystem for its positivity check;
+        // the others form it after the quadrature (fewer live registers)
+        Eig E;
+        if (LEARNED) E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
         double gl[4][3], gr[4][3];
+        bool linq = false;
+        if (LEARNED && wave_all(flux_window_smooth<TX, TY, YDIR>(s, fi, fj))) {
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            const double* L = s.sF[2 * f];
-            const double* R = s.sF[2 * f + 1];
-            if (MIXED) {
-                weno3q_mixed(a.qc, a.qc.lw32, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0],
-                             gl[f][1], gl[f][2]);
-                weno3q_mixed(a.qc, a.qc.lw32, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0],
-                             gr[f][1], gr[f][2]);
-            } else {
-                weno3q(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
-                       gl[f][2]);
-                weno3q(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
-                       gr[f][2]);
+            for (int f = 0; f < 4; ++f) {
+                const double* L = s.sF[2 * f];
+                const double* R = s.sF[2 * f + 1];
+                weno3q_linear(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
+                              gl[f][2]);
+                weno3q_linear(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
+                              gr[f][2]);
+            }
+            linq = true;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                double gm[4] = {gl[0][m], gl[1][m], gl[2][m], gl[3][m]};
+                double gp[4] = {gr[0][m], gr[1][m], gr[2][m], gr[3][m]};
+                linq = linq && face_state_positive(E, gm) && face_state_positive(E, gp);
             }
         }
-        Eig E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
+        if (!linq) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                const double* L = s.sF[2 * f];
+                const double* R = s.sF[2 * f + 1];
+                if (MIXED) {
+                    weno3q_mixed(a.qc, a.qc.lw32, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS],
+                                 gl[f][0], gl[f][1], gl[f][2]);
+                    weno3q_mixed(a.qc, a.qc.lw32, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS],
+                                 gr[f][0], gr[f][1], gr[f][2]);
+                } else {
+                    weno3q(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
+                           gl[f][2]);
+                    weno3q(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
+                           gr[f][2]);
+                }
+            }
+        }
+        if (!LEARNED) E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
         double acc[3][4];
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
@@ -339,12 +412,13 @@ __device__ __forceinline__ void stage_tile(const Fv2dArgs& a, Fv2dTile<TX, TY>& 
             for (int k = tid; k < (int)sizeof(s.sFlag); k += T::NT) fl[k] = sf[k];
         } else {
             for (int k = tid; k < (int)sizeof(s.sFlag); k += T::NT) sf[k] = fl[k];
-            __syncthreads();
         }
+        // the reconstruction overwrites flags that other threads copy above
+        __syncthreads();
     }
     ph_recon<TX, TY, false, MIXED, LEARNED>(a, s, tid, i0, j0, cnt);
     __syncthreads();
-    ph_flux<TX, TY, false, FLUX, MIXED>(a, s, tid);
+    ph_flux<TX, TY, false, FLUX, MIXED, LEARNED>(a, s, tid);
     __syncthreads();
     ph_xdiv<TX, TY>(a, s, tid, dxp);
     __syncthreads();
@@ -366,7 +440,7 @@ __device__ __forceinline__ void stage_tile(const Fv2dArgs& a, Fv2dTile<TX, TY>& 
         }
     }
     __syncthreads();
-    ph_flux<TX, TY, true, FLUX, MIXED>(a, s, tid);
+    ph_flux<TX, TY, true, FLUX, MIXED, LEARNED>(a, s, tid);
     __syncthreads();
     double smx = 0.0, smy = 0.0;
     ph_epilogue<TX, TY>(a, s, i0, j0, tid, dxp, dt, smx, smy);

@@ -61,6 +61,10 @@ HTX_HD void gelu_fd(float x, float& g, float& dg) {
 // degree <= 2 (tau = 0), near 1 across a jump.  TENO5 cuts a candidate roughly
 // when R^6 exceeds 1 / (3 C_T), i.e. for 1 - 1/R above about 0.66.  FP32.
 HTX_HD float teno_contrast(float e1, float e2, float e3, float e4, float eps) {
+    // no contraction: the features must come out the same in the training
+    // extraction and in the stage kernel's fused inference, whatever
+    // contraction the including file allows
+    HTX_NO_CONTRACT
     const float c1 = 13.0f / 12.0f;
     float c_0 = e4 - e3, b_0 = (e4 - 3.0f * e3) * 0.5f;
     float c_1 = e3 - e2, b_1 = -(e2 + e3) * 0.5f;
@@ -81,7 +85,11 @@ HTX_HD float teno_contrast(float e1, float e2, float e3, float e4, float eps) {
     // overflows every candidate is flat against tau (R = 1); when only tau/lo
     // does, 1/R = 0
     float x = tau * frcp(hi), y = tau * frcp(lo);
-    float inv = (x - x != 0.0f) ? 1.0f : (1.0f + x) * frcp(1.0f + y);
+    // x >= 0; not finite (infinite or NaN) unless x <= FLT_MAX.  (A test of
+    // x - x would be folded with the product above into one fused
+    // multiply-add, which returns the product's rounding error, under
+    // contraction.)
+    float inv = !(x <= 3.40282347e38f) ? 1.0f : (1.0f + x) * frcp(1.0f + y);
     float r = 1.0f - inv;
     return r > 0.0f ? r : 0.0f;
 }
@@ -92,7 +100,10 @@ HTX_HD float stencil_contrast(const float d[5], float eps) {
     return fmaxf(teno_contrast(d[0], d[1], d[2], d[3], eps), teno_contrast(d[1], d[2], d[3], d[4], eps));
 }
 
-HTX_HD float log_amp(float amp) { return flog2(amp + 1e-12f) * (0.30102999566398120f / 12.0f) + 1.0f; }
+HTX_HD float log_amp(float amp) {
+    HTX_NO_CONTRACT
+    return flog2(amp + 1e-12f) * (0.30102999566398120f / 12.0f) + 1.0f;
+}
 
 // The 12 classifier inputs of a face from the primitive values of its
 // stencil cells c-2..c+3 (r = density, un / ut = velocity normal / tangential
@@ -108,6 +119,7 @@ HTX_HD float log_amp(float amp) { return flog2(amp + 1e-12f) * (0.30102999566398
 //   x[8..11]  TENO contrasts of r, p, un, ut (eps = 1e-12 scale^2)
 HTX_HD void face_features(const double r[6], const double un[6], const double ut[6], const double p[6],
                           double gam, float x[CLS_IN]) {
+    HTX_NO_CONTRACT
     float dr[5], dp[5], du[5], dv[5];
     float sr = 0.0f, sp = 0.0f, ar = 0.0f, ap = 0.0f, au = 0.0f, at = 0.0f;
     for (int k = 0; k < 5; ++k) {

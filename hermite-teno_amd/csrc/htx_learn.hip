@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) k_extract(LrnExtractArgs a) {
         // a face of a non-finite state enters the set as troubled with zero
         // features rather than poisoning the training with NaN
         bool finite = true;
-        for (int k = 0; k < CLS_IN; ++k) finite = finite && (x[k] - x[k] == 0.0f);
+        for (int k = 0; k < CLS_IN; ++k) finite = finite && fabsf(x[k]) <= 3.40282347e38f;
         for (int k = 0; k < CLS_IN; ++k) a.feat[o * CLS_IN + k] = finite ? x[k] : 0.0f;
         int lab = finite ? stencil_label<YDIR>(st, a.gam - 1.0) : 0;
         a.label[o] = (unsigned char)lab;
@@ -240,14 +240,21 @@ __global__ void __launch_bounds__(256) k_train_update(LrnUpdateArgs a) {
     a.prm[p] -= a.lr_t * m / (sqrtf(v) + a.eps);
 }
 
+// bf16 rounding of an FP32 value, kept in FP32
+__device__ __forceinline__ float bf16_round(float x) { return bf16_to_f32(f32_to_bf16(x)); }
+
 // FP32 forward of every sample (one per work item): confusion counts at the
-// logit threshold, summed loss, optionally the logits
+// logit threshold, summed loss, optionally the logits.  With
+// inference_rounding the weights of the two matrix layers, the features and
+// layer 1's activations are rounded to bf16 where the solver's inference
+// rounds them (the sums stay FP32).
 __global__ void __launch_bounds__(256) k_eval(LrnEvalArgs a) {
     __shared__ float sW1[CLS_IN * CLS_H], sB1[CLS_H], sW2[CLS_H * CLS_H], sB2[CLS_H], sW3[CLS_H], sB3;
     __shared__ unsigned long long red[4][4];
     const int tid = threadIdx.x;
-    for (int p = tid; p < CLS_IN * CLS_H; p += 256) sW1[p] = a.prm[CLS_W1 + p];
-    for (int p = tid; p < CLS_H * CLS_H; p += 256) sW2[p] = a.prm[CLS_W2 + p];
+    const bool rb = a.inference_rounding != 0;
+    for (int p = tid; p < CLS_IN * CLS_H; p += 256) sW1[p] = rb ? bf16_round(a.prm[CLS_W1 + p]) : a.prm[CLS_W1 + p];
+    for (int p = tid; p < CLS_H * CLS_H; p += 256) sW2[p] = rb ? bf16_round(a.prm[CLS_W2 + p]) : a.prm[CLS_W2 + p];
     if (tid < CLS_H) {
         sB1[tid] = a.prm[CLS_B1 + tid];
         sB2[tid] = a.prm[CLS_B2 + tid];
@@ -258,11 +265,11 @@ __global__ void __launch_bounds__(256) k_eval(LrnEvalArgs a) {
     unsigned long long cnt[4] = {0, 0, 0, 0};  // smooth->smooth, rough->smooth, rough->rough, smooth->rough
     for (long long s = blockIdx.x * 256ll + tid; s < a.n; s += (long long)gridDim.x * 256) {
         float x[CLS_IN], h1[CLS_H];
-        for (int k = 0; k < CLS_IN; ++k) x[k] = a.feat[s * CLS_IN + k];
+        for (int k = 0; k < CLS_IN; ++k) x[k] = rb ? bf16_round(a.feat[s * CLS_IN + k]) : a.feat[s * CLS_IN + k];
         for (int u = 0; u < CLS_H; ++u) {
             float z = sB1[u];
             for (int k = 0; k < CLS_IN; ++k) z += x[k] * sW1[k * CLS_H + u];
-            h1[u] = gelu_f(z);
+            h1[u] = rb ? bf16_round(gelu_f(z)) : gelu_f(z);
         }
         float z3 = sB3;
         for (int u = 0; u < CLS_H; ++u) {

@@ -46,6 +46,8 @@ struct LrnEvalArgs {
     float threshold;
     unsigned long long* counts;  // smooth->smooth, troubled->smooth, troubled->troubled, smooth->troubled
     float* logits;               // optional
+    int inference_rounding;      // 1: round W1, W2, the features and layer 1's outputs to bf16 as the
+                                 // solver's matrix-core inference does
 };
 
 struct LrnClassifyArgs {

@@ -48,6 +48,10 @@ struct QuadConsts {
     double linwp[3], linwm[3], linw0p[3], linw0m[3];
     double sumw0p, sumw0m;
     float lw32[12];  // the four linear weight sets in FP32 (+xi, centre+, centre-, -xi)
+    // weno3q with every nonlinear weight equal to its linear weight, folded into
+    // one 5-point filter per Gauss point (+xi, centre, -xi): the learned path's
+    // transverse quadrature on smooth windows
+    double lq[3][5];
 };
 
 inline void fill_quad_consts(QuadConsts& C) {
@@ -97,6 +101,20 @@ inline void fill_quad_consts(QuadConsts& C) {
         C.lw32[6 + i] = (float)C.linw0m[i];
         C.lw32[9 + i] = (float)C.linwm[i];
     }
+    // candidate k at a Gauss point is sum_j c[j][k] u_{k+j}; its linear weight
+    // multiplies each tap.  The centre's split groups recombine to the plain
+    // centre weights lin0 (sumw0p linw0p - sumw0m linw0m = p - m = lin0).
+    const double cc[3][3] = {{-1.0 / 24.0, -1.0 / 24.0, 23.0 / 24.0},
+                             {2.0 / 24.0, 26.0 / 24.0, 2.0 / 24.0},
+                             {23.0 / 24.0, -1.0 / 24.0, -1.0 / 24.0}};
+    for (int m = 0; m < 3; ++m)
+        for (int i = 0; i < 5; ++i) C.lq[m][i] = 0.0;
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j) {
+            C.lq[0][k + j] += C.linwp[k] * C.gp[j][k];
+            C.lq[1][k + j] += lin0[k] * cc[j][k];
+            C.lq[2][k + j] += C.linwm[k] * C.gm[j][k];
+        }
 }
 
 // Fast FP32 reciprocal and base-2 logarithm on the device (one hardware
@@ -206,7 +224,7 @@ HTX_HD double teno5_side_mixed(double v0, double v1, double v2, double v3, doubl
     float a2 = 1.0f + tau * frcp(s2 + 1e-30f);
     float am = fmaxf(fmaxf(a0, a1), a2);
     bool k0, k1, k2;
-    if (am - am != 0.0f && am == am) {  // am is infinite
+    if (am > 3.40282347e38f) {  // am is infinite (am >= 1; NaN fails the test)
         k0 = a0 == am;
         k1 = a1 == am;
         k2 = a2 == am;
@@ -398,6 +416,16 @@ HTX_HD void weno3q_mixed(const QuadConsts& C, const float* lw, double u0, double
     o0 = o0 != o0 ? 1e-15 : o0;
     o1 = o1 != o1 ? 1e-15 : o1;
     o2 = o2 != o2 ? 1e-15 : o2;
+}
+
+// weno3q's linear limit: the 5th-order central interpolation of 5 transverse
+// values to the Gauss points (+xi, 0, -xi), exact for cell averages of a
+// quartic.  Used by the learned path where the window is smooth.
+HTX_HD void weno3q_linear(const QuadConsts& C, double u0, double u1, double u2, double u3, double u4, double& o0,
+                          double& o1, double& o2) {
+    o0 = C.lq[0][0] * u0 + C.lq[0][1] * u1 + C.lq[0][2] * u2 + C.lq[0][3] * u3 + C.lq[0][4] * u4;
+    o1 = C.lq[1][0] * u0 + C.lq[1][1] * u1 + C.lq[1][2] * u2 + C.lq[1][3] * u3 + C.lq[1][4] * u4;
+    o2 = C.lq[2][0] * u0 + C.lq[2][1] * u1 + C.lq[2][2] * u2 + C.lq[2][3] * u3 + C.lq[2][4] * u4;
 }
 
 // One Gauss point of weno3q (m = 0: +xi, 1: centre, 2: -xi), evaluated with

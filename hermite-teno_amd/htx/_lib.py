@@ -87,9 +87,11 @@ SIGNATURES = {
     "htx_trainer_get_params": (_I, [_P, _FP]),
     "htx_trainer_train": (_I, [_P, _I, _I, _D, _D, ctypes.c_ulonglong, _DP]),
     "htx_trainer_eval": (_I, [_P, _D, ctypes.POINTER(ctypes.c_longlong), _FP]),
+    "htx_trainer_eval_inference": (_I, [_P, _D, ctypes.POINTER(ctypes.c_longlong), _FP]),
     "htx_trainer_gradient": (_I, [_P, _D, _FP, _DP]),
     "htx_load_classifier": (_I, [_P, _FP, _D]),
     "htx_classify": (_I, [_P, _I, _FP, _FP]),
+    "htx_classifier_flags": (_I, [_P, ctypes.c_void_p, ctypes.c_void_p]),
     "htx_classifier_stats": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                                   ctypes.POINTER(ctypes.c_longlong), _I]),
     "htx_batch_create": (_I, [ctypes.POINTER(HtxParams), _I, ctypes.POINTER(_P)]),

@@ -94,13 +94,15 @@ class Trainer:
         check(_lib.load().htx_trainer_gradient(self._h, float(troubled_weight), fptr(g), ctypes.byref(l)))
         return g, l.value
 
-    def evaluate(self, threshold=0.0, want_logits=False):
+    def evaluate(self, threshold=0.0, want_logits=False, inference_rounding=False):
         """FP32 forward over the set: confusion counts at the logit threshold
-        (and the logits)."""
+        (and the logits); with inference_rounding, rounded to bf16 where the
+        solver's matrix-core inference rounds."""
         c = (ctypes.c_longlong * 4)()
         n, _ = self.size()
         lg = np.empty(n, dtype=np.float32) if want_logits else None
-        check(_lib.load().htx_trainer_eval(self._h, float(threshold), c, fptr(lg) if lg is not None else None))
+        fn = _lib.load().htx_trainer_eval_inference if inference_rounding else _lib.load().htx_trainer_eval
+        check(fn(self._h, float(threshold), c, fptr(lg) if lg is not None else None))
         out = {"smooth_as_smooth": c[0], "troubled_as_smooth": c[1], "troubled_as_troubled": c[2],
                "smooth_as_troubled": c[3]}
         if want_logits:

@@ -130,6 +130,15 @@ class System2D:
         p = np.ascontiguousarray(params, dtype=np.float32)
         check(_lib.load().htx_load_classifier(self._h, _lib.fptr(p), float(threshold)))
 
+    def classifier_flags(self):
+        """(x-face flags, y-face flags), each [ny, nx] uint8, 1 = smooth: what
+        the stage kernel's fused classifier decided in the last classified
+        stage."""
+        fx = np.zeros((self.ny, self.nx), dtype=np.uint8)
+        fy = np.zeros((self.ny, self.nx), dtype=np.uint8)
+        check(_lib.load().htx_classifier_flags(self._h, fx.ctypes.data, fy.ctypes.data))
+        return fx, fy
+
     def classifier_stats(self, reset=False):
         """(faces reconstructed, flagged smooth, took the linear scheme) since
         the last reset, each face once per stage."""

@@ -82,3 +82,27 @@ def throughput(w, nx, ny, precision, steps=20, warmup=2, flux="hllc"):
         r["linear_fraction"] = l / max(f, 1)
     s.close()
     return r
+
+
+def precision_report(w, nx, ny, precisions=("fp32_mixed", "learned"), tend=None, flux="hllc"):
+    """Error of each reduced precision against the plain FP64 GPU solution of
+    the same run (same grid, same end time): per conserved variable the
+    relative L1 (sum |q - q64| / sum |q64|) and the relative Linf
+    (max |q - q64| / max |q64|), plus steps and wall seconds."""
+    tend = w["tmax"] if tend is None else tend
+    q64, el64, st64, _ = run(w, nx, ny, "fp64", tend, flux)
+    names = ("rho", "rho_u", "rho_v", "E")
+    out = {"nx": nx, "ny": ny, "t_end": tend, "fp64": {"steps": st64, "seconds": el64}}
+    for p in precisions:
+        q, el, st, stats = run(w, nx, ny, p, tend, flux)
+        r = {"steps": st, "seconds": el, "finite": bool(np.isfinite(q).all())}
+        for k, nm in enumerate(names):
+            d = np.abs(q[k] - q64[k])
+            r[f"l1_{nm}"] = float(d.sum() / np.abs(q64[k]).sum())
+            r[f"linf_{nm}"] = float(d.max() / np.abs(q64[k]).max())
+        if stats is not None:
+            f, g, l = stats
+            r["flagged_fraction"] = g / max(f, 1)
+            r["linear_fraction"] = l / max(f, 1)
+        out[p] = r
+    return out

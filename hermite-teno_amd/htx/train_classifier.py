@@ -89,15 +89,27 @@ def snapshot_plan(n):
 
 
 def add_snapshots(trainer, w, nx, ny, times, tag):
+    """Adds the faces of the solver's state at each time; a run whose state
+    stops being finite contributes nothing from then on (its faces carry no
+    information, and zero features would collide with flat stencils).
+    Returns the number of snapshots skipped."""
+    import numpy as np
+
     s = _solver(w, nx, ny)
     s.init_workload(w)
-    for t in times:
+    skipped = 0
+    for k, t in enumerate(times):
         if t > 0:
             s.timeStepping(t)
+        if not np.isfinite(s.get_state()).all():
+            skipped = len(times) - k
+            log(f"[train] {tag} t={t:.3f}: state not finite, {skipped} snapshot(s) skipped")
+            break
         trainer.add_state(s)
         n, ns = trainer.size()
         log(f"[train] {tag} t={t:.3f}: set {n} faces, {ns} smooth")
     s.close()
+    return skipped
 
 
 def heldout_set(trainer_cls, n):
@@ -113,6 +125,28 @@ def heldout_set(trainer_cls, n):
         t.add_state(s)
     s.close()
     return t
+
+
+def _flat_logit(prm):
+    """Logit of a face whose 12 features are all zero (a flat stencil), with
+    the inference's bf16 rounding of the matrix-layer weights."""
+    import numpy as np
+
+    def r(a):
+        a = np.asarray(a, dtype=np.float32)
+        u = a.view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+        return u.astype(np.uint32).view(np.float32)
+
+    from .learn import unpack
+
+    W1, b1, W2, b2, w3, b3 = unpack(prm)
+    gelu = lambda z: z * (0.5 + np.clip(z, -1.8799712059732503, 1.8799712059732503) *  # noqa: E731
+                          (0.3989422804014327 - 0.037625858331402064 *
+                           np.clip(z, -1.8799712059732503, 1.8799712059732503) ** 2))
+    h1 = r(gelu(b1.astype(np.float32)))
+    h2 = gelu(h1 @ r(W2) + b2)
+    return float(h2 @ w3 + b3)
 
 
 def rates(c):
@@ -134,6 +168,8 @@ def main():
     ap.add_argument("--troubled-weight", type=float, default=4.0)
     ap.add_argument("--false-smooth", type=float, default=0.005,
                     help="largest share of troubled training faces the threshold may call smooth")
+    ap.add_argument("--threshold", type=float, default=None,
+                    help="use this logit threshold instead of the false-smooth quantile rule")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--out", default=os.path.join(_HERE, "data"))
     args = ap.parse_args()
@@ -144,8 +180,11 @@ def main():
     plan = snapshot_plan(args.n)
     cap = sum(2 * nx * ny * len(ts) for _, _, nx, ny, ts in plan)
     tr = Trainer(cap)
+    skipped = {}
     for tag, w, nx, ny, ts in plan:
-        add_snapshots(tr, w, nx, ny, ts, tag)
+        k = add_snapshots(tr, w, nx, ny, ts, tag)
+        if k:
+            skipped[tag] = k
     n, ns = tr.size()
     log(f"[train] training set: {n} faces, {ns} smooth ({ns / n:.3f}), built in {time.time() - t0:.1f} s")
     tr.init(seed=args.seed)
@@ -156,24 +195,31 @@ def main():
         losses.append(float(l[0]))
         log(f"[train] epoch {ep + 1}/{args.epochs} lr {lr:.2e} loss {l[0]:.5f} ({time.time() - t0:.1f} s)")
     # threshold: the smallest logit at which at most the requested share of
-    # troubled training faces would be called smooth
-    ev = tr.evaluate(0.0, want_logits=True)
+    # troubled training faces would be called smooth, on the logits of the
+    # solver's inference (bf16 rounding)
+    ev = tr.evaluate(0.0, want_logits=True, inference_rounding=True)
     _, lab = tr.read(0, n)
     lg = ev["logits"]
     tl = np.sort(lg[lab == 0])
     k = int(np.floor((1.0 - args.false_smooth) * len(tl)))
-    thr = float(tl[min(k, len(tl) - 1)])
-    train_rates = rates(tr.evaluate(thr))
+    thr_rule = float(tl[min(k, len(tl) - 1)])
+    thr = thr_rule if args.threshold is None else float(args.threshold)
+    train_rates = rates(tr.evaluate(thr, inference_rounding=True))
     log(f"[train] threshold {thr:.4f}: {train_rates}")
     ho = heldout_set(Trainer, args.n)
     ho.set_params(tr.params())
-    ho_rates = rates(ho.evaluate(thr))
+    ho_rates = rates(ho.evaluate(thr, inference_rounding=True))
+    # a flat stencil (all features 0) is smooth for TENO5; report where it lands
+    flat_logit = _flat_logit(tr.params())
+    log(f"[train] flat-stencil logit {flat_logit:.4f} (threshold {thr:.4f})")
     log(f"[train] held out (shock + blobs): {ho_rates}")
     os.makedirs(args.out, exist_ok=True)
     np.save(os.path.join(args.out, "classifier_teno5.npy"), tr.params())
-    report = {"threshold": thr, "epochs": args.epochs, "batch": args.batch, "lr": args.lr,
+    report = {"threshold": thr, "threshold_false_smooth_rule": thr_rule, "false_smooth": args.false_smooth,
+              "epochs": args.epochs, "batch": args.batch, "lr": args.lr,
               "troubled_weight": args.troubled_weight, "training_faces": n, "training_smooth": ns,
               "epoch_loss": losses, "train": train_rates, "heldout_shock_blobs": ho_rates,
+              "flat_stencil_logit": flat_logit, "skipped_snapshots": skipped,
               "snapshot_n": args.n, "seconds": time.time() - t0}
     with open(os.path.join(args.out, "classifier_teno5.json"), "w") as f:
         json.dump(report, f, indent=1)

@@ -160,7 +160,8 @@ int htx_timing_read(htx_solver* h, double* kernel_ms, int* nlaunch);
  * both sides of the face for all four characteristic fields (cutOff,
  * Schemes_Lagrange.cpp:1141-1149), i.e. when TENO5 reduces to the
  * optimal-weight linear scheme there.  The classifier is an MLP
- * 12 -> 64 -> 64 -> 1 with GELU (tanh form) whose inputs are normalised
+ * 12 -> 64 -> 64 -> 1 with GELU (x times a C1 cubic approximation of the
+ * normal CDF, the same in training and inference) whose inputs are normalised
  * density and pressure differences across the face's 6-cell stencil and TENO
  * smoothness contrasts of density, pressure and velocity.  Parameters are
  * HTX_CLS_NPARAM FP32 values: W1[12][64], b1[64], W2[64][64], b2[64],
@@ -209,6 +210,12 @@ int htx_trainer_gradient(htx_trainer* t, double troubled_weight, float* grad, do
  * exceeds threshold; logits (may be null) receives every logit. */
 int htx_trainer_eval(htx_trainer* t, double threshold, long long* counts, float* logits);
 
+/* The same with the rounding of the solver's inference: the two matrix
+ * layers' weights, the features and layer 1's activations rounded to bf16
+ * (sums in FP32).  Thresholds meant for htx_load_classifier are chosen on
+ * these logits. */
+int htx_trainer_eval_inference(htx_trainer* t, double threshold, long long* counts, float* logits);
+
 /* Loads classifier parameters into a solver (packed to bf16); a face whose
  * logit exceeds threshold takes the linear scheme in precision
  * HTX_PREC_LEARNED. */
@@ -219,6 +226,12 @@ int htx_load_classifier(htx_solver* h, const float* params, double threshold);
  * dir 1 = y faces, indexed j * nx + i, computed with the solver's bf16
  * matrix-core inference. */
 int htx_classify(htx_solver* h, int dir, float* logits, float* feat);
+
+/* Learned precision: the classifier flags (1 = smooth) that the stage
+ * kernel computed in the last step's first stage (or the last htx_rhs), for
+ * every x face (cell (i, j) | (i+1, j), fx[j * nx + i]) and every y face
+ * (fy, same indexing). */
+int htx_classifier_flags(htx_solver* h, unsigned char* fx, unsigned char* fy);
 
 /* Learned precision, summed since the last reset (each face once per
  * stage): faces reconstructed, faces the classifier flagged smooth, and

@@ -99,7 +99,7 @@ void teno_tile(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tile, double dt, unsi
     }
     unsigned cnt[3] = {0, 0, 0};
     for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, false, MIXED, LEARNED>(a, s, t, i0, j0, cnt);
-    for (int t = 0; t < T::NT; ++t) ph_flux<TX, TY, false, FLUX, MIXED>(a, s, t);
+    for (int t = 0; t < T::NT; ++t) ph_flux<TX, TY, false, FLUX, MIXED, LEARNED>(a, s, t);
     for (int t = 0; t < T::NT; ++t) ph_xdiv<TX, TY>(a, s, t, dx(t));
     for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, true, MIXED, LEARNED>(a, s, t, i0, j0, cnt);
     if (LEARNED) {
@@ -107,7 +107,7 @@ void teno_tile(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tile, double dt, unsi
         a.ctl->n_flagged += cnt[1];
         a.ctl->n_linear += cnt[2];
     }
-    for (int t = 0; t < T::NT; ++t) ph_flux<TX, TY, true, FLUX, MIXED>(a, s, t);
+    for (int t = 0; t < T::NT; ++t) ph_flux<TX, TY, true, FLUX, MIXED, LEARNED>(a, s, t);
     double mx = 0.0, my = 0.0;
     for (int t = 0; t < T::NT; ++t) {
         double smx = 0.0, smy = 0.0;

@@ -10,8 +10,13 @@
 
 using namespace htx;
 
-template <int TX, int TY, bool MIXED = false>
-static void run(Fv2dArgs a, int flux) {
+// LEARNED: recon faces start flagged smooth where fx / fy say so (x face
+// between cells (i, j) and (i+1, j) at fx[j nx + i], y face between (i, j)
+// and (i, j+1) at fy[j nx + i]; faces outside the grid and every face when
+// fx is null: all flagged), so the linear scheme and the linear transverse
+// quadrature run wherever flags and positivity checks allow.
+template <int TX, int TY, bool MIXED = false, bool LEARNED = false>
+static void run(Fv2dArgs a, int flux, const unsigned char* fx = nullptr, const unsigned char* fy = nullptr) {
     using T = Fv2dTile<TX, TY>;
     T* s = new T();
     a.tiles_x = (a.nx + TX - 1) / TX;
@@ -20,18 +25,32 @@ static void run(Fv2dArgs a, int flux) {
     for (int tile = 0; tile < a.tiles_x * a.tiles_y; ++tile) {
         int i0 = (tile % a.tiles_x) * TX, j0 = (tile / a.tiles_x) * TY;
         for (int t = 0; t < T::NT; ++t) ph_load<TX, TY>(a, *s, i0, j0, t);
-        for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, false, MIXED>(a, *s, t);
+        if (LEARNED) {
+            for (int k = 0; k < 2; ++k)
+                for (int w = 0; w < T::NF; ++w) s->sFlag[k][w] = 1;
+            if (fx) {
+                for (int w = 0; w < T::NXF; ++w) {
+                    int gi = i0 - 1 + w % (TX + 1), gj = j0 - 2 + w / (TX + 1);
+                    if (gi >= 0 && gi < a.nx - 1 && gj >= 0 && gj < a.ny) s->sFlag[0][w] = fx[(size_t)gj * a.nx + gi];
+                }
+                for (int w = 0; w < T::NYF; ++w) {
+                    int gi = i0 - 2 + w % (TX + 4), gj = j0 - 1 + w / (TX + 4);
+                    if (gi >= 0 && gi < a.nx && gj >= 0 && gj < a.ny - 1) s->sFlag[1][w] = fy[(size_t)gj * a.nx + gi];
+                }
+            }
+        }
+        for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, false, MIXED, LEARNED>(a, *s, t);
         for (int t = 0; t < T::NT; ++t) {
-            if (flux == 0) ph_flux<TX, TY, false, 0, MIXED>(a, *s, t);
-            else if (flux == 1) ph_flux<TX, TY, false, 1, MIXED>(a, *s, t);
-            else ph_flux<TX, TY, false, 2, MIXED>(a, *s, t);
+            if (flux == 0) ph_flux<TX, TY, false, 0, MIXED, LEARNED>(a, *s, t);
+            else if (flux == 1) ph_flux<TX, TY, false, 1, MIXED, LEARNED>(a, *s, t);
+            else ph_flux<TX, TY, false, 2, MIXED, LEARNED>(a, *s, t);
         }
         for (int t = 0; t < T::NT; ++t) ph_xdiv<TX, TY>(a, *s, t, (double(*)[4]) & dxp[(size_t)t * T::KC * 4]);
-        for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, true, MIXED>(a, *s, t);
+        for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, true, MIXED, LEARNED>(a, *s, t);
         for (int t = 0; t < T::NT; ++t) {
-            if (flux == 0) ph_flux<TX, TY, true, 0, MIXED>(a, *s, t);
-            else if (flux == 1) ph_flux<TX, TY, true, 1, MIXED>(a, *s, t);
-            else ph_flux<TX, TY, true, 2, MIXED>(a, *s, t);
+            if (flux == 0) ph_flux<TX, TY, true, 0, MIXED, LEARNED>(a, *s, t);
+            else if (flux == 1) ph_flux<TX, TY, true, 1, MIXED, LEARNED>(a, *s, t);
+            else ph_flux<TX, TY, true, 2, MIXED, LEARNED>(a, *s, t);
         }
         for (int t = 0; t < T::NT; ++t) {
             double smx = 0, smy = 0;
@@ -41,6 +60,7 @@ static void run(Fv2dArgs a, int flux) {
     delete s;
 }
 
+// mixed: 0 FP64, 1 FP32-mixed, 2 learned with every face flagged smooth
 extern "C" int fv2dh_rhs_prec(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc,
                               int flux, int quirks, int mixed, const double* q, double* rhs) {
     Fv2dArgs a{};
@@ -61,7 +81,9 @@ extern "C" int fv2dh_rhs_prec(int nx, int ny, double xl, double xr, double yb, d
     a.area = a.dx * a.dy;
     fill_quad_consts(a.qc);
     int fk = flux == 0 ? 0 : (quirks ? 1 : 2);
-    if (mixed)
+    if (mixed == 2)
+        run<16, 16, true, true>(a, fk);
+    else if (mixed)
         run<16, 16, true>(a, fk);
     else
         run<16, 16, false>(a, fk);
@@ -80,4 +102,91 @@ extern "C" void fv2dh_teno5(const double* u, int mixed, double* out) {
         teno5_mixed(u, out[0], out[1]);
     else
         teno5(u, out[0], out[1]);
+}
+
+// the transverse quadrature's linear limit (5 values -> +xi, centre, -xi)
+// with the constants the kernels receive
+extern "C" void fv2dh_linquad(const double* u, double* out) {
+    QuadConsts C{};
+    fill_quad_consts(C);
+    weno3q_linear(C, u[0], u[1], u[2], u[3], u[4], out[0], out[1], out[2]);
+}
+
+// residual of the learned path with given face flags (layout of run's fx, fy)
+extern "C" int fv2dh_rhs_flags(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc,
+                               int flux, int quirks, const double* q, double* rhs, const unsigned char* fx,
+                               const unsigned char* fy) {
+    Fv2dArgs a{};
+    a.qin = q;
+    a.q0 = q;
+    a.qout = rhs;
+    a.nx = nx;
+    a.ny = ny;
+    a.pitch = nx;
+    a.fstride = (long long)nx * ny;
+    a.bc = bc;
+    a.mode = 0;
+    a.quirks = quirks;
+    a.gam = gamma;
+    a.g1 = gamma - 1.0;
+    a.dx = (xr - xl) / nx;
+    a.dy = (yt - yb) / ny;
+    a.area = a.dx * a.dy;
+    fill_quad_consts(a.qc);
+    run<16, 16, true, true>(a, flux == 0 ? 0 : (quirks ? 1 : 2), fx, fy);
+    return 0;
+}
+
+// features of every owned recon face as the stage kernel's fused classifier
+// forms them from its tile (x faces into featx[j nx + i][12], y faces into
+// featy), for comparison with the standalone extraction
+extern "C" int fv2dh_tile_features(int nx, int ny, double gamma, int bc, const double* q, float* featx,
+                                   float* featy) {
+    using T = Fv2dTile<16, 16>;
+    Fv2dArgs a{};
+    a.qin = q;
+    a.nx = nx;
+    a.ny = ny;
+    a.pitch = nx;
+    a.fstride = (long long)nx * ny;
+    a.bc = bc;
+    a.gam = gamma;
+    a.g1 = gamma - 1.0;
+    a.tiles_x = (nx + 15) / 16;
+    a.tiles_y = (ny + 15) / 16;
+    T* s = new T();
+    for (int tile = 0; tile < a.tiles_x * a.tiles_y; ++tile) {
+        int i0 = (tile % a.tiles_x) * 16, j0 = (tile / a.tiles_x) * 16;
+        for (int t = 0; t < T::NT; ++t) ph_load<16, 16>(a, *s, i0, j0, t);
+        for (int w = 0; w < T::NXF + T::NYF; ++w) {
+            const bool ydir = w >= T::NXF;
+            const int wl = ydir ? w - T::NXF : w;
+            int ca, step, fi, fj;
+            if (!ydir) {
+                fi = wl % 17, fj = wl / 17;
+                ca = (1 + fj) * T::HX + (2 + fi);
+                step = 1;
+                if (!recon_face_owned<16, 16, false>(fi, fj, i0, j0, nx, ny)) continue;
+            } else {
+                fi = wl % 20, fj = wl / 20;
+                ca = (2 + fj) * T::HX + (1 + fi);
+                step = T::HX;
+                if (!recon_face_owned<16, 16, true>(fi, fj, i0, j0, nx, ny)) continue;
+            }
+            double r[6], un[6], ut[6], p[6];
+            for (int m = 0; m < 6; ++m) {
+                int id = ca + (m - 2) * step;
+                double u = s->sP[1][id], v = s->sP[2][id];
+                r[m] = s->sQ[0][id];
+                p[m] = (s->sQ[3][id] - 0.5 * (s->sQ[1][id] * u + s->sQ[2][id] * v)) * a.g1;
+                un[m] = ydir ? v : u;
+                ut[m] = ydir ? u : v;
+            }
+            float* out = ydir ? featy + ((size_t)(j0 + fj - 1) * nx + (i0 + fi - 2)) * 12
+                              : featx + ((size_t)(j0 + fj - 2) * nx + (i0 + fi - 1)) * 12;
+            face_features(r, un, ut, p, a.gam, out);
+        }
+    }
+    delete s;
+    return 0;
 }

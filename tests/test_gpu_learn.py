@@ -189,3 +189,67 @@ def test_learned_mode_limits(case):
         ref.steps_async(4)
         # smooth data: the linear scheme stays within discretization error of TENO
         assert rel_l1(q, ref.get_state()) < 1e-4
+
+
+def test_learned_all_smooth_matches_cpu_tiling():
+    """Every face flagged smooth (a threshold every logit passes): the GPU
+    residual equals the CPU build of the same tile phases with all flags set
+    (linear scheme, linear transverse quadrature, FP64 positivity
+    fallbacks) to the rounding of FMA contraction."""
+    import ctypes
+    import os
+
+    from conftest import ROOT, rel_l1
+    from htx.ic import vortex_workload
+
+    lib_path = os.path.join(ROOT, "tests", "host", "libfv2d_host_fast.so")
+    if not os.path.exists(lib_path):
+        pytest.skip("tests/host/libfv2d_host_fast.so not built")
+    lib = ctypes.CDLL(lib_path)
+    D, DP = ctypes.c_double, ctypes.POINTER(ctypes.c_double)
+    lib.fv2dh_rhs_prec.argtypes = [ctypes.c_int, ctypes.c_int, D, D, D, D, D, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, DP, DP]
+    n = 64
+    w = vortex_workload(n)
+    print(f"[case] learned all-smooth residual vs CPU tiling, vortex n={n}", flush=True)
+    prm = np.random.default_rng(3).normal(0, 0.3, 5057).astype(np.float32)
+    s = _pair("learned", n, w, -1e30, prm)
+    q = np.ascontiguousarray(s.get_state())
+    r_gpu = s.rhs()
+    r_cpu = np.empty_like(q)
+    lib.fv2dh_rhs_prec(n, n, w["xl"], w["xr"], w["yb"], w["yt"], w["gamma"], 1, 1, 1, 2, q.ctypes.data_as(DP),
+                       r_cpu.ctypes.data_as(DP))
+    err = rel_l1(r_gpu, r_cpu)
+    print(f"rel L1 {err:.3e}", flush=True)
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("case", ["shock_blobs", "quadrant"])
+def test_fused_classifier_flags_match_standalone(case):
+    """The flags the stage kernel's fused classifier computes for the faces
+    each tile owns equal the standalone inference's logits against the same
+    threshold (features from the tile's LDS copy of the state versus from
+    HBM, both through the same matrix-core code); only faces whose logit
+    lies within rounding of the threshold may differ."""
+    import htx
+    from htx.ic import quadrant_workload, shock_blob_workload
+
+    w, nx, ny = (shock_blob_workload(4096), 512, 128) if case == "shock_blobs" else (quadrant_workload(256), 256, 256)
+    from htx.learn import default_classifier
+
+    prm, _ = default_classifier()
+    thr = 1.0
+    print(f"[case] fused classifier flags vs standalone, {case} {nx}x{ny}", flush=True)
+    s = htx.System2D(w["xl"], w["xr"], nx, w["yb"], w["yt"], ny, w["cfl"], w["tmax"], 0.0, w["gamma"],
+                     flux="hllc", bc=w["bc"], precision="learned", classifier=(prm, thr))
+    s.init_workload(w)
+    s.rhs()
+    fx, fy = s.classifier_flags()
+    for d, f in ((0, fx), (1, fy)):
+        lg = s.classify(d)
+        ref = (lg > thr).astype(np.uint8)
+        bad = (f != ref) & (np.abs(lg - thr) > 1e-3)
+        print(f"dir {d}: flagged {f.mean():.3f} (standalone {ref.mean():.3f}), mismatches {int(bad.sum())}",
+              flush=True)
+        assert bad.sum() == 0
+    s.close()

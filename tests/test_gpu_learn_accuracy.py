@@ -1,8 +1,8 @@
 """The learned precision with the shipped classifier: it stays finite and
 positive on shock data, and its solution error against a 4x finer FP64
-solution stays close to plain FP64 TENO5's (the figures are printed; the
-bound asserted here is the safety margin, the matched-accuracy numbers are
-reported by bench.py)."""
+solution stays within 10 % of plain FP64 TENO5's, for the density and for
+all four conserved fields together (the matched-accuracy target; bench.py
+reports the same figures at larger sizes)."""
 import numpy as np
 import pytest
 
@@ -25,4 +25,5 @@ def test_learned_matched_accuracy(case):
     for p in ("fp64", "learned"):
         print(f"{p}: {r[p]}", flush=True)
     assert r["learned"]["finite"]
-    assert r["learned"]["l1_rho_vs_fp64"] < 1.5
+    assert r["learned"]["l1_rho_vs_fp64"] < 1.1
+    assert r["learned"]["l1_all"] < 1.1 * r["fp64"]["l1_all"]

@@ -135,3 +135,34 @@ def test_rounding_cut_on_flat_stencil_is_not_troubled(orc):
     u = np.full(6, 2.5 * 0.14285714285714285)
     assert list(orc.teno5_flags(u)) != [1] * 6
     assert np.allclose(orc.teno5(u), orc.linear5(u), rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("lib_name", ["libfv2d_host_fast.so", "libfv2d_host_contract.so"])
+def test_tile_features_match_extraction(lib_name):
+    """The features the stage kernel's fused classifier forms from its LDS
+    tile equal the training extraction's (numpy restatement), also when the
+    tile code is compiled with the product's contraction setting and FMA
+    available (the device-side counterpart is
+    test_gpu_learn.py::test_fused_classifier_flags_match_standalone)."""
+    import ctypes
+    import os
+
+    from conftest import ROOT
+    from htx.ic import shock_blob_workload, piecewise_state
+
+    path = os.path.join(ROOT, "tests", "host", lib_name)
+    if not os.path.exists(path):
+        pytest.skip(f"tests/host/{lib_name} not built")
+    lib = ctypes.CDLL(path)
+    FP, DP = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)
+    lib.fv2dh_tile_features.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, DP, FP, FP]
+    w = shock_blob_workload(4096)
+    nx, ny = 128, 32
+    q = np.ascontiguousarray(piecewise_state(w, nx, ny))
+    for d in (0, 1):
+        out = [np.zeros((ny, nx, 12), np.float32), np.zeros((ny, nx, 12), np.float32)]
+        lib.fv2dh_tile_features(nx, ny, w["gamma"], 0, q.ctypes.data_as(DP), out[0].ctypes.data_as(FP),
+                                out[1].ctypes.data_as(FP))
+        ref = np.asarray(L.features(q, w["gamma"], "zero_grad", d)).reshape(ny, nx, 12)
+        assert np.abs(out[d] - ref).max() < 1e-5
+        assert (ref[..., 8:] > 0.5).any()  # the data hold jumps, so contrasts near 1 must appear

@@ -189,3 +189,61 @@ def test_fp32_mixed_cutoff_never_overflows(harness_fast):
         harness_fast.fv2dh_teno5(u.ctypes.data_as(DP), 0, ref.ctypes.data_as(DP))
         harness_fast.fv2dh_teno5(u.ctypes.data_as(DP), 1, out.ctypes.data_as(DP))
         assert np.all(np.abs(out - ref) < 1e-9 * u[0]), (out, ref)
+
+
+def test_linear_quadrature_exact_for_quartics(harness_fast):
+    """The learned path's transverse quadrature on smooth windows (weno3q
+    with linear weights, folded into one 5-point filter per Gauss point):
+    cell averages of any quartic give its exact values at +xi, 0, -xi;
+    quadratics agree with the oracle's nonlinear quadrature."""
+    from test_oracle import SQ15, cell_avg_poly, poly
+
+    harness_fast.fv2dh_linquad.argtypes = [DP, DP]
+    out = np.empty(3)
+    xi = SQ15 / 10.0
+    rng = np.random.default_rng(4)
+    for deg in range(5):
+        coef = rng.uniform(-2, 2, deg + 1)
+        u = np.array([cell_avg_poly(coef, m - 2.5, m - 1.5) for m in range(5)])
+        harness_fast.fv2dh_linquad(u.ctypes.data_as(DP), out.ctypes.data_as(DP))
+        np.testing.assert_allclose(out, [poly(coef, xi), poly(coef, 0.0), poly(coef, -xi)], rtol=0, atol=5e-14)
+    # a sixth-degree monomial is not reproduced (the filter is 5th order, not more)
+    u = np.array([cell_avg_poly([0, 0, 0, 0, 0, 0, 1.0], m - 2.5, m - 1.5) for m in range(5)])
+    harness_fast.fv2dh_linquad(u.ctypes.data_as(DP), out.ctypes.data_as(DP))
+    assert abs(out[0] - xi ** 6) > 1e-3
+
+
+def _rhs_prec(lib, nx, ny, dom, bc, flux, mixed, q):
+    lib.fv2dh_rhs_prec.argtypes = [ctypes.c_int, ctypes.c_int, D, D, D, D, D, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, DP, DP]
+    q = np.ascontiguousarray(q)
+    r = np.empty_like(q)
+    lib.fv2dh_rhs_prec(nx, ny, *dom, 1.4, {"zero_grad": 0, "periodic": 1}[bc], {"rusanov": 0, "hllc": 1}[flux], 1,
+                       mixed, q.ctypes.data_as(DP), r.ctypes.data_as(DP))
+    return r
+
+
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+def test_learned_all_smooth_residual(orc, harness_fast, flux):
+    """The learned path with every face flagged smooth (linear scheme normal
+    to the face, linear transverse quadrature, positivity fallbacks): a
+    uniform state has zero residual, the periodic residual conserves every
+    field, and on the smooth vortex its distance to plain TENO5's residual
+    is discretisation error, shrinking at high order (> 16x per halving of
+    the cell size)."""
+    from htx.ic import vortex_modes
+
+    dom = (0.0, 10.0, 0.0, 10.0)
+    errs = []
+    for n in (48, 96):
+        prm = params(n, n, *dom, 1.4, "periodic", flux, True)
+        q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
+        r = _rhs_prec(harness_fast, n, n, dom, "periodic", flux, 2, q)
+        rc = orc.fv2d_rhs(prm, q)
+        errs.append(np.abs(r - rc).sum() / np.abs(rc).sum())
+        for f in range(4):
+            assert abs(r[f].sum()) <= 1e-12 * np.abs(r[f]).sum()
+    assert 0.0 < errs[1] < 1e-4 and errs[0] / errs[1] > 16.0, errs
+    n = 40
+    u = np.ones((4, n, n)) * np.array([1.2, 0.3, -0.4, 2.5])[:, None, None]
+    assert np.abs(_rhs_prec(harness_fast, n, n, dom, "periodic", flux, 2, u)).max() < 1e-13
