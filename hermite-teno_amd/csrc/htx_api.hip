@@ -7,7 +7,6 @@
 #include <vector>
 
 #include "../../include/htx.h"
-#include "htx_fv2d_tile.h"
 #include "htx_hermite.h"
 #include "htx_learn.h"
 #include "htx_learn_api.h"
@@ -81,7 +80,8 @@ struct htx_solver {
     // FP32 biases and output weights, logit threshold
     unsigned short* dFrag = nullptr;
     float* dCvec = nullptr;
-    unsigned char* dFlags = nullptr;  // per-tile recon-face flags (learned precision)
+    unsigned char* dFlagsX = nullptr;  // classifier flags of the x and y faces (learned precision)
+    unsigned char* dFlagsY = nullptr;
     float cls_threshold = 0.0f;
     bool have_classifier = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -124,11 +124,8 @@ Fv2dArgs stage_args(htx_solver* h, const double* qin, const double* q0, double* 
     a.quirks = h->p.ref_quirks ? 1 : 0;
     a.mixed = (h->p.precision == HTX_PREC_FP32_MIXED || h->p.precision == HTX_PREC_LEARNED) ? 1 : 0;
     a.learned = h->p.precision == HTX_PREC_LEARNED ? 1 : 0;
-    a.cls_frag = h->dFrag;
-    a.cls_cvec = h->dCvec;
-    a.cls_thr = h->cls_threshold;
-    a.cls_flags = h->dFlags;
-    a.cls_fresh = mode <= 1 ? 1 : 0;
+    a.face_flags_x = h->dFlagsX;
+    a.face_flags_y = h->dFlagsY;
     a.gam = h->p.gamma;
     a.g1 = h->p.gamma - 1.0;
     a.dx = h->dx;
@@ -189,6 +186,24 @@ int launch_stage(htx_solver* h, const double* qin, const double* q0, double* qou
         HIP_TRY(hipEventRecord(e0, h->stream));
     }
     const bool strict = h->p.precision == HTX_PREC_FP64;
+    if (h->p.precision == HTX_PREC_LEARNED && mode <= 1) {
+        // classify every face of u^n once per step; stages 2 and 3 reuse the flags
+        LrnFlagsArgs f{};
+        f.q = qin;
+        f.nx = h->nx;
+        f.ny = h->ny;
+        f.pitch = h->pitch;
+        f.bc = h->p.bc;
+        f.fstride = h->fstride;
+        f.gam = h->p.gamma;
+        f.frag = h->dFrag;
+        f.cvec = h->dCvec;
+        f.threshold = h->cls_threshold;
+        f.ctl = mode == 1 ? h->ctl : nullptr;
+        f.flags_x = h->dFlagsX;
+        f.flags_y = h->dFlagsY;
+        HIP_TRY(lrn_launch_flags(f, h->stream));
+    }
     if (h->p.scheme == HTX_SCHEME_HTENO) {
         FvhArgs a = hermite_args(h, qin, q0, qout, mode);
         if (strict)
@@ -292,7 +307,8 @@ int htx_create(const htx_params* p, htx_solver** out) {
         htx_destroy(h);
         return fail(HTX_ENOMEM, "device allocation failed");
     }
-    if (p->precision == HTX_PREC_LEARNED && hipMalloc(&h->dFlags, fv2d_flag_bytes(p->nx, p->ny)) != hipSuccess) {
+    if (p->precision == HTX_PREC_LEARNED && (hipMalloc(&h->dFlagsX, lrn_flag_bytes_x(p->nx, p->ny)) != hipSuccess ||
+                                             hipMalloc(&h->dFlagsY, lrn_flag_bytes_y(p->nx, p->ny)) != hipSuccess)) {
         htx_destroy(h);
         return fail(HTX_ENOMEM, "device allocation failed");
     }
@@ -339,7 +355,8 @@ void htx_destroy(htx_solver* h) {
         if (b) (void)hipFree(b);
     if (h->dFrag) (void)hipFree(h->dFrag);
     if (h->dCvec) (void)hipFree(h->dCvec);
-    if (h->dFlags) (void)hipFree(h->dFlags);
+    if (h->dFlagsX) (void)hipFree(h->dFlagsX);
+    if (h->dFlagsY) (void)hipFree(h->dFlagsY);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -914,29 +931,10 @@ int htx_load_classifier(htx_solver* h, const float* params, double threshold) {
 
 int htx_classifier_flags(htx_solver* h, unsigned char* fx, unsigned char* fy) {
     if (!h || !fx || !fy) return fail(HTX_EINVAL, "null argument");
-    if (h->p.precision != HTX_PREC_LEARNED || !h->dFlags)
+    if (h->p.precision != HTX_PREC_LEARNED || !h->dFlagsX)
         return fail(HTX_ESTATE, "the solver does not run the learned precision");
-    // the stage kernel's per-tile buffer: 16 x 16 tiles, x recon faces then
-    // y recon faces of Fv2dTile<16, 16>, in the tile's face order
-    using T = Fv2dTile<16, 16>;
-    const int tx = (h->nx + 15) / 16, ty = (h->ny + 15) / 16;
-    std::vector<unsigned char> buf(fv2d_flag_bytes(h->nx, h->ny));
-    HIP_TRY(copy_sync(buf.data(), h->dFlags, buf.size(), hipMemcpyDeviceToHost, h->stream));
-    const size_t per = buf.size() / ((size_t)tx * ty);
-    for (int t = 0; t < tx * ty; ++t) {
-        const int i0 = (t % tx) * 16, j0 = (t / tx) * 16;
-        const unsigned char* f = buf.data() + (size_t)t * per;
-        for (int w = 0; w < T::NXF; ++w) {
-            int fi = w % 17, fj = w / 17;
-            if (!recon_face_owned<16, 16, false>(fi, fj, i0, j0, h->nx, h->ny)) continue;
-            fx[(size_t)(j0 + fj - 2) * h->nx + (i0 + fi - 1)] = f[w];
-        }
-        for (int w = 0; w < T::NYF; ++w) {
-            int fi = w % 20, fj = w / 20;
-            if (!recon_face_owned<16, 16, true>(fi, fj, i0, j0, h->nx, h->ny)) continue;
-            fy[(size_t)(j0 + fj - 1) * h->nx + (i0 + fi - 2)] = f[T::NF + w];
-        }
-    }
+    HIP_TRY(copy_sync(fx, h->dFlagsX, lrn_flag_bytes_x(h->nx, h->ny), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(copy_sync(fy, h->dFlagsY, lrn_flag_bytes_y(h->nx, h->ny), hipMemcpyDeviceToHost, h->stream));
     return HTX_OK;
 }
 

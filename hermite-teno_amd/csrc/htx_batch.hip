@@ -33,11 +33,10 @@ struct BatchTenoArgs {
     const Fv2dArgs* args;   // [mode - 1][instance]
     const int* tile_inst;   // global tile -> TENO instance
     const int* tile0;       // first global tile of each instance
-    unsigned char* flags;   // learned precision: per global tile
     int ninst, ntiles, mode;
 };
 
-template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED>
+template <int TX, int TY, int FLUX, bool MIXED>
 __global__ void __launch_bounds__(256, 2) k_batch_teno(BatchTenoArgs B) {
     using T = Fv2dTile<TX, TY>;
     __shared__ T s;
@@ -49,8 +48,7 @@ __global__ void __launch_bounds__(256, 2) k_batch_teno(BatchTenoArgs B) {
         const Fv2dArgs& a = B.args[(B.mode - 1) * B.ninst + b];
         if (!a.ctl->active) continue;  // uniform over the workgroup
         const double dt = a.ctl->dt;
-        unsigned char* fl = LEARNED ? B.flags + (size_t)g * sizeof(s.sFlag) : nullptr;
-        stage_tile<TX, TY, FLUX, MIXED, LEARNED>(a, s, g - B.tile0[b], dt, fl, threadIdx.x);
+        stage_tile<TX, TY, FLUX, MIXED, false>(a, s, g - B.tile0[b], dt, threadIdx.x);
         __syncthreads();
     }
 }
@@ -177,29 +175,15 @@ constexpr Shape SHAPES[] = {{16, 16}, {32, 8}, {8, 32}};
 constexpr int NSHAPES = 3;
 
 template <int TX, int TY>
-size_t flag_bytes_per_tile() {
-    return sizeof(((Fv2dTile<TX, TY>*)nullptr)->sFlag);
-}
-
-size_t shape_flag_bytes(int k) {
-    switch (k) {
-        case 0: return flag_bytes_per_tile<16, 16>();
-        case 1: return flag_bytes_per_tile<32, 8>();
-        default: return flag_bytes_per_tile<8, 32>();
-    }
-}
-
-template <int TX, int TY>
 hipError_t launch_teno_shape(const BatchTenoArgs& B, int flux, int prec, int nblk, hipStream_t s) {
     dim3 g(nblk), b(256);
+    if (prec != 0 && prec != 1) return hipErrorInvalidValue;  // batches run fp64_fast or fp32_mixed
     if (flux == FLUX_RUSANOV_CHAR) {
-        if (prec == 0) hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_RUSANOV_CHAR, false, false>), g, b, 0, s, B);
-        else if (prec == 1) hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_RUSANOV_CHAR, true, false>), g, b, 0, s, B);
-        else hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_RUSANOV_CHAR, true, true>), g, b, 0, s, B);
+        if (prec == 0) hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_RUSANOV_CHAR, false>), g, b, 0, s, B);
+        else hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_RUSANOV_CHAR, true>), g, b, 0, s, B);
     } else {
-        if (prec == 0) hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_HLLC, false, false>), g, b, 0, s, B);
-        else if (prec == 1) hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_HLLC, true, false>), g, b, 0, s, B);
-        else hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_HLLC, true, true>), g, b, 0, s, B);
+        if (prec == 0) hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_HLLC, false>), g, b, 0, s, B);
+        else hipLaunchKernelGGL((k_batch_teno<TX, TY, FLUX_HLLC, true>), g, b, 0, s, B);
     }
     return hipGetLastError();
 }
@@ -247,12 +231,10 @@ struct htx_batch {
     Ctl* ctl = nullptr;
     double* dDxdy = nullptr;
     // TENO5 launch
-    int teno_flux = FLUX_HLLC, teno_prec = 1;  // prec 0 fp64_fast, 1 fp32_mixed, 2 learned
+    int teno_flux = FLUX_HLLC, teno_prec = 1;  // prec 0 fp64_fast, 1 fp32_mixed
     int shape = 0, ntiles = 0, nblk_teno = 0;
     Fv2dArgs* dTenoArgs = nullptr;
     int *dTileInst = nullptr, *dTile0 = nullptr;
-    unsigned char* dFlags = nullptr;
-    size_t flags_bytes = 0;
     // Hermite launches
     int herm_flux = FLUX_RUSANOV_CHAR;
     FvhArgs* dHermArgs = nullptr;
@@ -310,8 +292,7 @@ int build_teno_tables(htx_batch* b) {
     const int nt = (int)b->teno.size();
     if (nt == 0) return HTX_OK;
     BatchLayout L = layout_of(b);
-    TenoTables T = make_teno_tables(L, b->state, b->ctl, SHAPES[b->shape].tx, SHAPES[b->shape].ty, b->teno_prec,
-                                    b->dFrag, b->dCvec, b->cls_thr);
+    TenoTables T = make_teno_tables(L, b->state, b->ctl, SHAPES[b->shape].tx, SHAPES[b->shape].ty, b->teno_prec);
     const int g = T.ntiles;
     b->ntiles = g;
     // the old tables may still be read by launches in flight
@@ -319,10 +300,8 @@ int build_teno_tables(htx_batch* b) {
     if (b->dTenoArgs) (void)hipFree(b->dTenoArgs);
     if (b->dTileInst) (void)hipFree(b->dTileInst);
     if (b->dTile0) (void)hipFree(b->dTile0);
-    if (b->dFlags) (void)hipFree(b->dFlags);
     b->dTenoArgs = nullptr;
     b->dTileInst = b->dTile0 = nullptr;
-    b->dFlags = nullptr;
     BHIP(hipMalloc(&b->dTenoArgs, sizeof(Fv2dArgs) * T.args.size()));
     BHIP(hipMalloc(&b->dTileInst, sizeof(int) * T.tile_inst.size()));
     BHIP(hipMalloc(&b->dTile0, sizeof(int) * T.tile0.size()));
@@ -330,10 +309,6 @@ int build_teno_tables(htx_batch* b) {
     BHIP(copy_sync(b->dTileInst, T.tile_inst.data(), sizeof(int) * T.tile_inst.size(), hipMemcpyHostToDevice,
                    b->stream));
     BHIP(copy_sync(b->dTile0, T.tile0.data(), sizeof(int) * T.tile0.size(), hipMemcpyHostToDevice, b->stream));
-    if (b->teno_prec == 2) {
-        b->flags_bytes = shape_flag_bytes(b->shape) * (size_t)g;
-        BHIP(hipMalloc(&b->dFlags, b->flags_bytes));
-    }
     // persistent grid: the resident capacity (2 workgroups per CU), at most
     // one workgroup per tile
     b->nblk_teno = std::min(g, 2 * b->cu);
@@ -371,7 +346,6 @@ int enqueue_batch_step(htx_batch* b, hipStream_t s) {
             T.args = b->dTenoArgs;
             T.tile_inst = b->dTileInst;
             T.tile0 = b->dTile0;
-            T.flags = b->dFlags;
             T.ninst = (int)b->teno.size();
             T.ntiles = b->ntiles;
             T.mode = mode;
@@ -524,7 +498,7 @@ void htx_batch_destroy(htx_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     for (void* m : {(void*)b->state, (void*)b->ctl, (void*)b->dDxdy, (void*)b->dTenoArgs, (void*)b->dTileInst,
-                    (void*)b->dTile0, (void*)b->dFlags, (void*)b->dHermArgs, (void*)b->dHermMap,
+                    (void*)b->dTile0, (void*)b->dHermArgs, (void*)b->dHermMap,
                     (void*)b->herm_scratch, (void*)b->dFrag, (void*)b->dCvec})
         if (m) (void)hipFree(m);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -607,8 +581,6 @@ int htx_batch_load_classifier(htx_batch* b, const float* params, double threshol
 int htx_batch_steps(htx_batch* b, int nsteps) {
     if (!b) return htx_internal_fail(HTX_EINVAL, "null batch");
     if (nsteps < 0) return htx_internal_fail(HTX_EINVAL, "nsteps < 0");
-    if (!b->teno.empty() && b->teno_prec == 2 && !b->have_cls)
-        return htx_internal_fail(HTX_ESTATE, "learned precision needs htx_batch_load_classifier");
     int rc;
     if (b->use_graph && !b->graph && (rc = capture_graph(b))) return rc;
     for (int k = 0; k < nsteps; ++k) {
@@ -679,8 +651,6 @@ int htx_batch_set_tile(htx_batch* b, int tile_x, int tile_y) {
 int htx_batch_tune(htx_batch* b, int steps, int* tile_x, int* tile_y, double* ms) {
     if (!b) return htx_internal_fail(HTX_EINVAL, "null batch");
     if (steps < 1) return htx_internal_fail(HTX_EINVAL, "steps >= 1");
-    if (!b->teno.empty() && b->teno_prec == 2 && !b->have_cls)
-        return htx_internal_fail(HTX_ESTATE, "learned precision needs htx_batch_load_classifier");
     // time every built shape on the batch itself, from the same state
     double* saved = nullptr;
     Ctl* saved_ctl = nullptr;

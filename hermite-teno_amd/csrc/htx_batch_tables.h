@@ -55,8 +55,8 @@ struct TenoTables {
 };
 
 // state = base of the three state copies (A | B | C, each `total` doubles)
-inline TenoTables make_teno_tables(const BatchLayout& L, double* state, Ctl* ctl, int TX, int TY, int prec,
-                                   const unsigned short* frag, const float* cvec, float thr) {
+// prec: 0 fp64_fast, 1 fp32_mixed
+inline TenoTables make_teno_tables(const BatchLayout& L, double* state, Ctl* ctl, int TX, int TY, int prec) {
     TenoTables T;
     const int nt = (int)L.teno.size();
     T.args.resize(3 * (size_t)nt);
@@ -88,7 +88,7 @@ inline TenoTables make_teno_tables(const BatchLayout& L, double* state, Ctl* ctl
             a.mode = mode;
             a.quirks = p.ref_quirks ? 1 : 0;
             a.mixed = prec >= 1;
-            a.learned = prec == 2;
+            a.learned = 0;
             a.tiles_x = tx;
             a.tiles_y = ty;
             a.gam = p.gamma;
@@ -97,11 +97,6 @@ inline TenoTables make_teno_tables(const BatchLayout& L, double* state, Ctl* ctl
             a.dy = L.dxdy[2 * i + 1];
             a.area = a.dx * a.dy;
             a.qc = qc;
-            a.cls_frag = frag;
-            a.cls_cvec = cvec;
-            a.cls_thr = thr;
-            a.cls_flags = nullptr;  // the batched launch addresses flags by global tile
-            a.cls_fresh = mode == 1 ? 1 : 0;
             T.args[(size_t)(mode - 1) * nt + k] = a;
         }
     }

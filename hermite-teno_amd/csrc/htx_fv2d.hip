@@ -29,8 +29,7 @@ __global__ void __launch_bounds__(256, 2) fv2d_stage_kernel(Fv2dArgs a) {
         dt = a.ctl->dt;
     }
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    unsigned char* fl = LEARNED ? a.cls_flags + (size_t)tile * sizeof(s.sFlag) : nullptr;
-    stage_tile<TX, TY, FLUX, MIXED, LEARNED>(a, s, tile, dt, fl, threadIdx.x);
+    stage_tile<TX, TY, FLUX, MIXED, LEARNED>(a, s, tile, dt, threadIdx.x);
 }
 
 // one-thread control kernels around a step (ctl_dt, ctl_tadvance)
@@ -135,17 +134,11 @@ static hipError_t launch_tile(const Fv2dArgs& a0, int flux, hipStream_t s) {
     return hipGetLastError();
 }
 
-size_t fv2d_flag_bytes(int nx, int ny) {
-    using T = Fv2dTile<16, 16>;
-    size_t tiles = (size_t)((nx + 15) / 16) * (size_t)((ny + 15) / 16);
-    return tiles * sizeof(((T*)nullptr)->sFlag);
-}
-
 hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y, hipStream_t s) {
     (void)tile_x;
     (void)tile_y;
     if (a.learned) {
-        if (!a.cls_frag || !a.cls_cvec || !a.cls_flags) return hipErrorInvalidValue;
+        if (!a.face_flags_x || !a.face_flags_y) return hipErrorInvalidValue;
         return launch_tile<16, 16, true, true>(a, flux, s);
     }
     if (a.mixed) return launch_tile<16, 16, true, false>(a, flux, s);

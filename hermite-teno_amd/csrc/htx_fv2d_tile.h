@@ -63,50 +63,34 @@ HTX_HD Prim prim_at(const Fv2dTile<TX, TY>& s, int id) {
     return Prim{s.sP[0][id], s.sP[1][id], s.sP[2][id], s.sP[3][id]};
 }
 
-// Learned precision: classify every recon face of the tile (x faces, then y
-// faces) with the MFMA classifier; 1 = the linear scheme is enough.  The
-// features come from the tile's conserved state and Roe inputs; the
-// transposition scratch of the matrix cores reuses the face-value buffer,
-// which is free until the reconstruction.
+// Learned precision: the classifier flag of face g (cells g | g+1 along a
+// grid line of n cells) from a line of flags stored for g = -1 .. n; faces
+// further out follow the boundary condition (periodic: the same face of
+// the wrapped grid) or take TENO5 (zero gradient: they feed only halo faces
+// whose fluxes no cell uses).
+HTX_HD unsigned char line_flag(const unsigned char* line, int g, int n, int bc, int stride) {
+    if (g >= -1 && g <= n) return line[(long long)(g + 1) * stride];
+    if (bc == 1) {
+        g %= n;
+        g = g < 0 ? g + n : g;
+        return line[(long long)(g + 1) * stride];
+    }
+    return 0;
+}
+
+// Learned precision: the flags of the tile's recon faces, from the face-flag
+// arrays of the step (rows of x faces and columns of y faces outside the grid
+// map like the cells their stencils read).
 template <int TX, int TY>
-__device__ void ph_classify(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
+HTX_HD void ph_flags_load(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int i0, int j0, int tid) {
     using T = Fv2dTile<TX, TY>;
-    static_assert(sizeof(s.sF) >= T::NT * 16 * sizeof(unsigned short), "scratch does not fit");
-    unsigned short* xs = reinterpret_cast<unsigned short*>(&s.sF[0][0]) + (tid >> 6) * 1024;
-    constexpr int NTOT = T::NXF + T::NYF;
-    for (int base = 0; base < NTOT; base += T::NT) {
-        const int w = base + tid;
-        const bool valid = w < NTOT, ydir = w >= T::NXF;
-        const int wl = ydir ? w - T::NXF : w;
-        float x[CLS_IN];
-        for (int k = 0; k < CLS_IN; ++k) x[k] = 0.0f;
-        if (valid) {
-            int ca, step;
-            if (!ydir) {
-                int fi = wl % (TX + 1), fj = wl / (TX + 1);
-                ca = (1 + fj) * T::HX + (2 + fi);
-                step = 1;
-            } else {
-                int fi = wl % (TX + 4), fj = wl / (TX + 4);
-                ca = (2 + fj) * T::HX + (1 + fi);
-                step = T::HX;
-            }
-            double r[6], un[6], ut[6], p[6];
-            for (int m = 0; m < 6; ++m) {
-                int id = ca + (m - 2) * step;
-                double u = s.sP[1][id], v = s.sP[2][id];
-                r[m] = s.sQ[0][id];
-                p[m] = (s.sQ[3][id] - 0.5 * (s.sQ[1][id] * u + s.sQ[2][id] * v)) * a.g1;
-                un[m] = ydir ? v : u;
-                ut[m] = ydir ? u : v;
-            }
-            face_features(r, un, ut, p, a.gam, x);
-        }
-        cls_stage_features(xs, x);
-        __syncthreads();
-        float z = cls_logit(xs, a.cls_frag, a.cls_cvec);
-        if (valid) s.sFlag[ydir ? 1 : 0][wl] = z > a.cls_thr ? 1 : 0;
-        __syncthreads();
+    for (int w = tid; w < T::NXF; w += T::NT) {
+        int gi = i0 - 1 + w % (TX + 1), gj = map_index(j0 - 2 + w / (TX + 1), a.ny, a.bc);
+        s.sFlag[0][w] = line_flag(a.face_flags_x + (long long)gj * (a.nx + 2), gi, a.nx, a.bc, 1);
+    }
+    for (int w = tid; w < T::NYF; w += T::NT) {
+        int gi = map_index(i0 - 2 + w % (TX + 4), a.nx, a.bc), gj = j0 - 1 + w / (TX + 4);
+        s.sFlag[1][w] = line_flag(a.face_flags_y + gi, gj, a.ny, a.bc, a.nx);
     }
 }
 
@@ -120,15 +104,6 @@ HTX_HD bool face_state_positive(const Eig& E, const double c[4]) {
     return q0 > 0.0 && q3 * q0 - 0.5 * (qn * qn + qt * qt) > 0.0;
 }
 
-// Whether p holds on every active lane of the wavefront (on the device) or
-// for this work item (on the host, where a phase runs item by item).
-HTX_HD bool wave_all(bool p) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __all(p ? 1 : 0) != 0;
-#else
-    return p;
-#endif
-}
 
 // Whether recon face (fi, fj) of family YDIR is one the tile owns: every
 // global face is owned by exactly one tile (an x face by the tile of its left
@@ -229,6 +204,28 @@ HTX_HD bool flux_window_smooth(const Fv2dTile<TX, TY>& s, int fi, int fj) {
     return ok && (C[c] & C[c + 1] & C[c + (TX + 1)] & C[c + (TX + 1) + 1]);
 }
 
+// Whether every flux face of the wavefront that handles flux face w (work
+// item w = tid + k NT of ph_flux) has a smooth window.  On the device a
+// wavefront vote over its active lanes; on the host, where a phase runs item
+// by item, the same vote taken over the items of that wavefront.
+template <int TX, int TY, bool YDIR>
+HTX_HD bool window_smooth_wave(const Fv2dTile<TX, TY>& s, int w, int tid) {
+    using T = Fv2dTile<TX, TY>;
+    constexpr int NW = YDIR ? T::NYQ : T::NXQ;
+    constexpr int RW = YDIR ? TX : (TX + 1);
+#if defined(__HIP_DEVICE_COMPILE__)
+    (void)tid;
+    (void)NW;
+    return __all(flux_window_smooth<TX, TY, YDIR>(s, w % RW, w / RW) ? 1 : 0) != 0;
+#else
+    const int w0 = w - (tid & 63);
+    bool all = true;
+    for (int l = 0; l < 64 && w0 + l < NW; ++l)
+        all = all && flux_window_smooth<TX, TY, YDIR>(s, (w0 + l) % RW, (w0 + l) / RW);
+    return all;
+#endif
+}
+
 // transverse 3-point quadrature, back-projection with the face's own R,
 // Riemann flux at the Gauss points weighted by (5/18, 4/9, 5/18)
 // (FV_System2D.cpp:402-497); the flux is stored in (rho, x, y, E) order.
@@ -257,7 +254,7 @@ HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
         if (LEARNED) E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
         double gl[4][3], gr[4][3];
         bool linq = false;
-        if (LEARNED && wave_all(flux_window_smooth<TX, TY, YDIR>(s, fi, fj))) {
+        if (LEARNED && window_smooth_wave<TX, TY, YDIR>(s, w, tid)) {
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
                 const double* L = s.sF[2 * f];
@@ -389,33 +386,19 @@ __device__ __forceinline__ double wave_max(double v) {
 // One RK stage (or residual) of tile `tile` of the instance described by a:
 // halo load, x and y reconstruction and fluxes, epilogue, and in stage 3
 // the tile's CFL maxima (one atomic per workgroup).  dt = the stage's time
-// step; fl = this tile's stored recon-face flags (learned precision).
-// Ends with every thread past its last LDS access only for mode 3; callers
-// that reuse the tile storage synchronise before the next tile.
+// step.  Ends with every thread past its last LDS access only for mode 3;
+// callers that reuse the tile storage synchronise before the next tile.
 template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED>
-__device__ __forceinline__ void stage_tile(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tile, double dt,
-                                           unsigned char* fl, int tid) {
+__device__ __forceinline__ void stage_tile(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tile, double dt, int tid) {
     using T = Fv2dTile<TX, TY>;
     const int i0 = (tile % a.tiles_x) * TX;
     const int j0 = (tile / a.tiles_x) * TY;
     double dxp[T::KC][4];
 
     ph_load<TX, TY>(a, s, i0, j0, tid);
+    if (LEARNED) ph_flags_load<TX, TY>(a, s, i0, j0, tid);
     __syncthreads();
     unsigned cnt[3] = {0, 0, 0};
-    if (LEARNED) {
-        // the classifier runs in the first stage of a step (and for a lone
-        // residual); stages 2 and 3 reuse the flags it stored for this tile
-        unsigned char* sf = &s.sFlag[0][0];
-        if (a.cls_fresh) {
-            ph_classify<TX, TY>(a, s, tid);
-            for (int k = tid; k < (int)sizeof(s.sFlag); k += T::NT) fl[k] = sf[k];
-        } else {
-            for (int k = tid; k < (int)sizeof(s.sFlag); k += T::NT) sf[k] = fl[k];
-        }
-        // the reconstruction overwrites flags that other threads copy above
-        __syncthreads();
-    }
     ph_recon<TX, TY, false, MIXED, LEARNED>(a, s, tid, i0, j0, cnt);
     __syncthreads();
     ph_flux<TX, TY, false, FLUX, MIXED, LEARNED>(a, s, tid);

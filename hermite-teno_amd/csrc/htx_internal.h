@@ -64,15 +64,16 @@ struct Fv2dArgs {
     int mode;           // 0 residual only, 1..3 SSP-RK3 stage
     int quirks;         // reference maxSpeed quirk in the CFL epilogue
     int mixed;          // 1: cut-off decisions and nonlinear weights in FP32
-    int learned;        // 1: classifier picks linear or TENO5 per face (implies mixed)
+    int learned;        // 1: classifier flags pick linear or TENO5 per face (implies mixed)
     int tiles_x, tiles_y;
     double gam, g1, dx, dy, area;
     QuadConsts qc;
-    const unsigned short* cls_frag;  // packed bf16 classifier weights
-    const float* cls_cvec;           // FP32 biases and output weights
-    float cls_thr;                   // logit above which a face is smooth
-    unsigned char* cls_flags;        // per tile: the recon-face flags of the step's first stage
-    int cls_fresh;                   // 1: classify and store the flags; 0: reuse the stored ones
+    // learned precision: the classifier's flags (1 = smooth) of the step's
+    // state, x faces i | i+1 (i = -1 .. nx) of row j at face_flags_x[j (nx + 2)
+    // + i + 1], y faces j | j+1 (j = -1 .. ny) of column i at
+    // face_flags_y[(j + 1) nx + i]
+    const unsigned char* face_flags_x;
+    const unsigned char* face_flags_y;
 };
 
 struct VortexArgs {
@@ -132,8 +133,6 @@ inline hipError_t zero_sync(void* dst, size_t bytes, hipStream_t s) {
 
 // launchers (htx_fv2d.hip)
 hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y, hipStream_t s);
-// bytes of the learned precision's per-tile face-flag buffer for an nx x ny grid
-size_t fv2d_flag_bytes(int nx, int ny);
 hipError_t launch_dt(Ctl* c, double dx, double dy, hipStream_t s);
 hipError_t launch_tadvance(Ctl* c, hipStream_t s);
 hipError_t launch_reset_smax(Ctl* c, hipStream_t s);

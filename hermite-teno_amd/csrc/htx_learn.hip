@@ -313,6 +313,33 @@ __global__ void __launch_bounds__(256) k_classify(LrnClassifyArgs a) {
     if (id < n) a.logits[id] = z;
 }
 
+// classifier flags of every face of one family, including the faces one
+// cell past each boundary (the stage kernel's tiles reach them)
+template <bool YDIR>
+__global__ void __launch_bounds__(256) k_face_flags(LrnFlagsArgs a) {
+    __shared__ unsigned short xs[4][64 * 16];
+    if (a.ctl && !static_cast<const Ctl*>(a.ctl)->active) return;  // uniform over the grid
+    const int w = YDIR ? a.nx : a.nx + 2;
+    const long long n = YDIR ? (long long)a.nx * (a.ny + 2) : (long long)(a.nx + 2) * a.ny;
+    long long id = blockIdx.x * 256ll + threadIdx.x;
+    float x[CLS_IN];
+    for (int k = 0; k < CLS_IN; ++k) x[k] = 0.0f;
+    if (id < n) {
+        int i = (int)(id % w), j = (int)(id / w);
+        if (YDIR)
+            j -= 1;
+        else
+            i -= 1;
+        Stencil6 st;
+        load_stencil<YDIR>(a.q, a.nx, a.ny, a.pitch, a.fstride, a.bc, i, j, st);
+        stencil_features<YDIR>(st, a.gam, x);
+    }
+    cls_stage_features(xs[threadIdx.x >> 6], x);
+    __syncthreads();
+    float z = cls_logit(xs[threadIdx.x >> 6], a.frag, a.cvec);
+    if (id < n) (YDIR ? a.flags_y : a.flags_x)[id] = z > a.threshold ? 1 : 0;
+}
+
 }  // namespace
 
 }  // namespace htx_lrn
@@ -350,6 +377,16 @@ hipError_t lrn_launch_classify(const LrnClassifyArgs& a, int dir, hipStream_t s)
         hipLaunchKernelGGL(k_classify<false>, g, dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL(k_classify<true>, g, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+size_t lrn_flag_bytes_x(int nx, int ny) { return (size_t)(nx + 2) * ny; }
+size_t lrn_flag_bytes_y(int nx, int ny) { return (size_t)nx * (ny + 2); }
+
+hipError_t lrn_launch_flags(const LrnFlagsArgs& a, hipStream_t s) {
+    long long nxf = (long long)lrn_flag_bytes_x(a.nx, a.ny), nyf = (long long)lrn_flag_bytes_y(a.nx, a.ny);
+    hipLaunchKernelGGL(k_face_flags<false>, dim3((unsigned)((nxf + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_face_flags<true>, dim3((unsigned)((nyf + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -61,7 +61,29 @@ struct LrnClassifyArgs {
     float* feat;     // optional [face][12]
 };
 
+// Classifier flags of every face for the learned precision's stage kernel
+// (1 = smooth: logit above the threshold).  x faces (cell i | i+1 of row j,
+// i = -1 .. nx) at flags_x[j (nx + 2) + i + 1]; y faces (cell j | j+1 of
+// column i, j = -1 .. ny) at flags_y[(j + 1) nx + i].  Cells outside the
+// grid follow the boundary condition.  With ctl set, nothing is done when
+// the step is inactive.
+struct LrnFlagsArgs {
+    const double* q;
+    int nx, ny, pitch, bc;
+    long long fstride;
+    double gam;
+    const unsigned short* frag;
+    const float* cvec;
+    float threshold;
+    const void* ctl;            // Ctl* of the solver, or null
+    unsigned char* flags_x;
+    unsigned char* flags_y;
+};
+
 hipError_t lrn_launch_extract(const LrnExtractArgs& a, int dir, hipStream_t s);
+hipError_t lrn_launch_flags(const LrnFlagsArgs& a, hipStream_t s);
+size_t lrn_flag_bytes_x(int nx, int ny);
+size_t lrn_flag_bytes_y(int nx, int ny);
 hipError_t lrn_launch_train(const LrnTrainArgs& a, int nblk, const LrnUpdateArgs& u, hipStream_t s);
 hipError_t lrn_launch_eval(const LrnEvalArgs& a, hipStream_t s);
 hipError_t lrn_launch_classify(const LrnClassifyArgs& a, int dir, hipStream_t s);

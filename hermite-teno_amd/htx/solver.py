@@ -131,11 +131,11 @@ class System2D:
         check(_lib.load().htx_load_classifier(self._h, _lib.fptr(p), float(threshold)))
 
     def classifier_flags(self):
-        """(x-face flags, y-face flags), each [ny, nx] uint8, 1 = smooth: what
-        the stage kernel's fused classifier decided in the last classified
-        stage."""
-        fx = np.zeros((self.ny, self.nx), dtype=np.uint8)
-        fy = np.zeros((self.ny, self.nx), dtype=np.uint8)
+        """(x-face flags [ny, nx + 2], y-face flags [ny + 2, nx]), uint8, 1 =
+        smooth, of the last step's state: x face i | i+1 of row j at
+        [j, i + 1] for i = -1 .. nx, y face j | j+1 of column i at [j + 1, i]."""
+        fx = np.zeros((self.ny, self.nx + 2), dtype=np.uint8)
+        fy = np.zeros((self.ny + 2, self.nx), dtype=np.uint8)
         check(_lib.load().htx_classifier_flags(self._h, fx.ctypes.data, fy.ctypes.data))
         return fx, fy
 

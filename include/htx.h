@@ -227,10 +227,11 @@ int htx_load_classifier(htx_solver* h, const float* params, double threshold);
  * matrix-core inference. */
 int htx_classify(htx_solver* h, int dir, float* logits, float* feat);
 
-/* Learned precision: the classifier flags (1 = smooth) that the stage
- * kernel computed in the last step's first stage (or the last htx_rhs), for
- * every x face (cell (i, j) | (i+1, j), fx[j * nx + i]) and every y face
- * (fy, same indexing). */
+/* Learned precision: the classifier flags (1 = smooth) of the last step's
+ * state (or of the last htx_rhs) that the stage kernels read: x faces
+ * (i, j) | (i+1, j) for i = -1 .. nx at fx[j * (nx + 2) + i + 1] (ny rows),
+ * y faces (i, j) | (i, j+1) for j = -1 .. ny at fy[(j + 1) * nx + i]; the
+ * faces one cell past each boundary see the boundary condition. */
 int htx_classifier_flags(htx_solver* h, unsigned char* fx, unsigned char* fy);
 
 /* Learned precision, summed since the last reset (each face once per

@@ -48,66 +48,18 @@ void atomic_max_bits(unsigned long long* p, double v) {
     if (b > *p) *p = b;
 }
 
-// Recon-face flags of the learned precision for one tile: the faces and
-// features of ph_classify, the flag from a fixed rule on the features (a
-// stand-in for the classifier, which runs on the matrix cores only) that
-// marks a mix of faces smooth.
-template <int TX, int TY>
-void host_classify(const Fv2dArgs& a, Fv2dTile<TX, TY>& s) {
-    using T = Fv2dTile<TX, TY>;
-    for (int w = 0; w < T::NXF + T::NYF; ++w) {
-        const bool ydir = w >= T::NXF;
-        const int wl = ydir ? w - T::NXF : w;
-        int ca, step;
-        if (!ydir) {
-            ca = (1 + wl / (TX + 1)) * T::HX + (2 + wl % (TX + 1));
-            step = 1;
-        } else {
-            ca = (2 + wl / (TX + 4)) * T::HX + (1 + wl % (TX + 4));
-            step = T::HX;
-        }
-        double r[6], un[6], ut[6], p[6];
-        for (int m = 0; m < 6; ++m) {
-            int id = ca + (m - 2) * step;
-            double u = s.sP[1][id], v = s.sP[2][id];
-            r[m] = s.sQ[0][id];
-            p[m] = (s.sQ[3][id] - 0.5 * (s.sQ[1][id] * u + s.sQ[2][id] * v)) * a.g1;
-            un[m] = ydir ? v : u;
-            ut[m] = ydir ? u : v;
-        }
-        float x[CLS_IN];
-        face_features(r, un, ut, p, a.gam, x);
-        s.sFlag[ydir ? 1 : 0][wl] = x[4] < 0.6f ? 1 : 0;
-    }
-}
-
-template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED>
-void teno_tile(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tile, double dt, unsigned char* fl) {
+template <int TX, int TY, int FLUX, bool MIXED>
+void teno_tile(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tile, double dt) {
     using T = Fv2dTile<TX, TY>;
     const int i0 = (tile % a.tiles_x) * TX, j0 = (tile / a.tiles_x) * TY;
     std::vector<double> dxp((size_t)T::NT * T::KC * 4);
     auto dx = [&](int t) { return (double(*)[4]) & dxp[(size_t)t * T::KC * 4]; };
     for (int t = 0; t < T::NT; ++t) ph_load<TX, TY>(a, s, i0, j0, t);
-    if (LEARNED) {
-        unsigned char* sf = &s.sFlag[0][0];
-        if (a.cls_fresh) {
-            host_classify<TX, TY>(a, s);
-            for (int k = 0; k < (int)sizeof(s.sFlag); ++k) fl[k] = sf[k];
-        } else {
-            for (int k = 0; k < (int)sizeof(s.sFlag); ++k) sf[k] = fl[k];
-        }
-    }
-    unsigned cnt[3] = {0, 0, 0};
-    for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, false, MIXED, LEARNED>(a, s, t, i0, j0, cnt);
-    for (int t = 0; t < T::NT; ++t) ph_flux<TX, TY, false, FLUX, MIXED, LEARNED>(a, s, t);
+    for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, false, MIXED>(a, s, t);
+    for (int t = 0; t < T::NT; ++t) ph_flux<TX, TY, false, FLUX, MIXED>(a, s, t);
     for (int t = 0; t < T::NT; ++t) ph_xdiv<TX, TY>(a, s, t, dx(t));
-    for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, true, MIXED, LEARNED>(a, s, t, i0, j0, cnt);
-    if (LEARNED) {
-        a.ctl->n_faces += cnt[0];
-        a.ctl->n_flagged += cnt[1];
-        a.ctl->n_linear += cnt[2];
-    }
-    for (int t = 0; t < T::NT; ++t) ph_flux<TX, TY, true, FLUX, MIXED, LEARNED>(a, s, t);
+    for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, true, MIXED>(a, s, t);
+    for (int t = 0; t < T::NT; ++t) ph_flux<TX, TY, true, FLUX, MIXED>(a, s, t);
     double mx = 0.0, my = 0.0;
     for (int t = 0; t < T::NT; ++t) {
         double smx = 0.0, smy = 0.0;
@@ -121,8 +73,8 @@ void teno_tile(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tile, double dt, unsi
     }
 }
 
-template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED>
-void teno_launch(const TenoTables& TT, int ninst, int mode, int nblk, unsigned char* flags) {
+template <int TX, int TY, int FLUX, bool MIXED>
+void teno_launch(const TenoTables& TT, int ninst, int mode, int nblk) {
     using T = Fv2dTile<TX, TY>;
     T* s = new T();
     const int per = (TT.ntiles + nblk - 1) / nblk;
@@ -132,8 +84,7 @@ void teno_launch(const TenoTables& TT, int ninst, int mode, int nblk, unsigned c
             const int b = TT.tile_inst[g];
             const Fv2dArgs& a = TT.args[(size_t)(mode - 1) * ninst + b];
             if (!a.ctl->active) continue;
-            unsigned char* fl = LEARNED ? flags + (size_t)g * sizeof(s->sFlag) : nullptr;
-            teno_tile<TX, TY, FLUX, MIXED, LEARNED>(a, *s, g - TT.tile0[b], a.ctl->dt, fl);
+            teno_tile<TX, TY, FLUX, MIXED>(a, *s, g - TT.tile0[b], a.ctl->dt);
         }
     }
     delete s;
@@ -204,22 +155,15 @@ void herm_launch(const HermTables& H, int ninst, int mode) {
     }
 }
 
-template <int TX, int TY, int FLUX>
-void teno_dispatch_prec(const TenoTables& TT, int ninst, int mode, int nblk, int prec, unsigned char* flags) {
-    if (prec == 0) teno_launch<TX, TY, FLUX, false, false>(TT, ninst, mode, nblk, flags);
-    else if (prec == 1) teno_launch<TX, TY, FLUX, true, false>(TT, ninst, mode, nblk, flags);
-    else teno_launch<TX, TY, FLUX, true, true>(TT, ninst, mode, nblk, flags);
-}
-
 template <int TX, int TY>
-void teno_dispatch(const TenoTables& TT, int ninst, int mode, int nblk, int flux, int prec, unsigned char* flags) {
-    if (flux == FLUX_RUSANOV_CHAR) teno_dispatch_prec<TX, TY, FLUX_RUSANOV_CHAR>(TT, ninst, mode, nblk, prec, flags);
-    else teno_dispatch_prec<TX, TY, FLUX_HLLC>(TT, ninst, mode, nblk, prec, flags);
-}
-
-template <int TX, int TY>
-size_t tile_flag_bytes() {
-    return sizeof(((Fv2dTile<TX, TY>*)nullptr)->sFlag);
+void teno_dispatch(const TenoTables& TT, int ninst, int mode, int nblk, int flux, int prec) {
+    if (flux == FLUX_RUSANOV_CHAR) {
+        if (prec == 0) teno_launch<TX, TY, FLUX_RUSANOV_CHAR, false>(TT, ninst, mode, nblk);
+        else teno_launch<TX, TY, FLUX_RUSANOV_CHAR, true>(TT, ninst, mode, nblk);
+    } else {
+        if (prec == 0) teno_launch<TX, TY, FLUX_HLLC, false>(TT, ninst, mode, nblk);
+        else teno_launch<TX, TY, FLUX_HLLC, true>(TT, ninst, mode, nblk);
+    }
 }
 
 // max wave speeds of an instance's state (k_max_speed)
@@ -244,8 +188,8 @@ void host_max_speed(const double* q, const htx_params& p, Ctl* c) {
 
 }  // namespace
 
-// nsteps steps of a batch of n instances (TENO5: fp64_fast, fp32_mixed, or
-// learned with the stand-in flag rule of host_classify; Hermite: fp64_fast),
+// nsteps steps of a batch of n instances (TENO5: fp64_fast or fp32_mixed;
+// Hermite: fp64_fast),
 // tile shape tx x ty of the TENO5 launch, nblk persistent workgroups.  q holds
 // every instance's initial state back to back (4 or 12 fields, interior
 // cells); on return, the final states, and in t (if not null) each
@@ -257,7 +201,7 @@ extern "C" int batchh_steps(int n, const htx_params* ps, int tx, int ty, int nbl
     for (int i = 0; i < n; ++i) {
         if (ps[i].scheme == HTX_SCHEME_TENO5) {
             teno_flux = ps[i].flux == HTX_FLUX_RUSANOV_CHAR ? FLUX_RUSANOV_CHAR : FLUX_HLLC;
-            prec = ps[i].precision == HTX_PREC_FP64_FAST ? 0 : (ps[i].precision == HTX_PREC_LEARNED ? 2 : 1);
+            prec = ps[i].precision == HTX_PREC_FP64_FAST ? 0 : 1;
         } else {
             herm_flux = ps[i].flux == HTX_FLUX_RUSANOV_CHAR ? FLUX_RUSANOV_CHAR
                                                               : (ps[i].ref_quirks ? FLUX_HLLC : FLUX_HLLC_TORO);
@@ -278,20 +222,17 @@ extern "C" int batchh_steps(int n, const htx_params* ps, int tx, int ty, int nbl
         ctl[i].cfl = p.cfl;
         host_max_speed(state.data() + L.off[i], p, &ctl[i]);
     }
-    TenoTables TT = make_teno_tables(L, state.data(), ctl.data(), tx, ty, prec, nullptr, nullptr, 0.0f);
+    TenoTables TT = make_teno_tables(L, state.data(), ctl.data(), tx, ty, prec);
     HermTables H = make_herm_tables(L, state.data(), ctl.data(), scratch.data());
     const int nt = (int)L.teno.size(), nh = (int)L.herm.size();
     const int nb = std::max(1, std::min(nblk, TT.ntiles));
-    size_t fbytes = tx == 16 ? tile_flag_bytes<16, 16>() : (tx == 32 ? tile_flag_bytes<32, 8>() : tile_flag_bytes<8, 32>());
-    std::vector<unsigned char> flags(prec == 2 ? fbytes * (size_t)TT.ntiles : 0);
     for (int step = 0; step < nsteps; ++step) {
         for (int i = 0; i < n; ++i) host_dt(&ctl[i], L.dxdy[2 * i], L.dxdy[2 * i + 1]);
         for (int mode = 1; mode <= 3; ++mode) {
             if (nt) {
-                unsigned char* fl = flags.empty() ? nullptr : flags.data();
-                if (tx == 16 && ty == 16) teno_dispatch<16, 16>(TT, nt, mode, nb, teno_flux, prec, fl);
-                else if (tx == 32 && ty == 8) teno_dispatch<32, 8>(TT, nt, mode, nb, teno_flux, prec, fl);
-                else if (tx == 8 && ty == 32) teno_dispatch<8, 32>(TT, nt, mode, nb, teno_flux, prec, fl);
+                if (tx == 16 && ty == 16) teno_dispatch<16, 16>(TT, nt, mode, nb, teno_flux, prec);
+                else if (tx == 32 && ty == 8) teno_dispatch<32, 8>(TT, nt, mode, nb, teno_flux, prec);
+                else if (tx == 8 && ty == 32) teno_dispatch<8, 32>(TT, nt, mode, nb, teno_flux, prec);
                 else return -1;
             }
             if (nh) {

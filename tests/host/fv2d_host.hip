@@ -10,35 +10,25 @@
 
 using namespace htx;
 
-// LEARNED: recon faces start flagged smooth where fx / fy say so (x face
-// between cells (i, j) and (i+1, j) at fx[j nx + i], y face between (i, j)
-// and (i, j+1) at fy[j nx + i]; faces outside the grid and every face when
-// fx is null: all flagged), so the linear scheme and the linear transverse
-// quadrature run wherever flags and positivity checks allow.
+// LEARNED: the tiles load their recon-face flags with the kernel's own
+// ph_flags_load from face-flag arrays in the kernel's layout (fx: x faces
+// i = -1 .. nx of each row at [j (nx + 2) + i + 1]; fy: y faces j = -1 .. ny
+// of each column at [(j + 1) nx + i]); null arrays flag every face smooth.
 template <int TX, int TY, bool MIXED = false, bool LEARNED = false>
 static void run(Fv2dArgs a, int flux, const unsigned char* fx = nullptr, const unsigned char* fy = nullptr) {
     using T = Fv2dTile<TX, TY>;
     T* s = new T();
     a.tiles_x = (a.nx + TX - 1) / TX;
     a.tiles_y = (a.ny + TY - 1) / TY;
+    std::vector<unsigned char> ones_x((size_t)(a.nx + 2) * a.ny, 1), ones_y((size_t)a.nx * (a.ny + 2), 1);
+    a.face_flags_x = fx ? fx : ones_x.data();
+    a.face_flags_y = fy ? fy : ones_y.data();
     std::vector<double> dxp((size_t)T::NT * T::KC * 4);
     for (int tile = 0; tile < a.tiles_x * a.tiles_y; ++tile) {
         int i0 = (tile % a.tiles_x) * TX, j0 = (tile / a.tiles_x) * TY;
         for (int t = 0; t < T::NT; ++t) ph_load<TX, TY>(a, *s, i0, j0, t);
-        if (LEARNED) {
-            for (int k = 0; k < 2; ++k)
-                for (int w = 0; w < T::NF; ++w) s->sFlag[k][w] = 1;
-            if (fx) {
-                for (int w = 0; w < T::NXF; ++w) {
-                    int gi = i0 - 1 + w % (TX + 1), gj = j0 - 2 + w / (TX + 1);
-                    if (gi >= 0 && gi < a.nx - 1 && gj >= 0 && gj < a.ny) s->sFlag[0][w] = fx[(size_t)gj * a.nx + gi];
-                }
-                for (int w = 0; w < T::NYF; ++w) {
-                    int gi = i0 - 2 + w % (TX + 4), gj = j0 - 1 + w / (TX + 4);
-                    if (gi >= 0 && gi < a.nx && gj >= 0 && gj < a.ny - 1) s->sFlag[1][w] = fy[(size_t)gj * a.nx + gi];
-                }
-            }
-        }
+        if (LEARNED)
+            for (int t = 0; t < T::NT; ++t) ph_flags_load<TX, TY>(a, *s, i0, j0, t);
         for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, false, MIXED, LEARNED>(a, *s, t);
         for (int t = 0; t < T::NT; ++t) {
             if (flux == 0) ph_flux<TX, TY, false, 0, MIXED, LEARNED>(a, *s, t);
@@ -112,7 +102,7 @@ extern "C" void fv2dh_linquad(const double* u, double* out) {
     weno3q_linear(C, u[0], u[1], u[2], u[3], u[4], out[0], out[1], out[2]);
 }
 
-// residual of the learned path with given face flags (layout of run's fx, fy)
+// residual of the learned path with given face flags (the kernel's layout, see run)
 extern "C" int fv2dh_rhs_flags(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc,
                                int flux, int quirks, const double* q, double* rhs, const unsigned char* fx,
                                const unsigned char* fy) {

@@ -225,31 +225,62 @@ def test_learned_all_smooth_matches_cpu_tiling():
 
 
 @pytest.mark.parametrize("case", ["shock_blobs", "quadrant"])
-def test_fused_classifier_flags_match_standalone(case):
-    """The flags the stage kernel's fused classifier computes for the faces
-    each tile owns equal the standalone inference's logits against the same
-    threshold (features from the tile's LDS copy of the state versus from
-    HBM, both through the same matrix-core code); only faces whose logit
-    lies within rounding of the threshold may differ."""
-    import htx
-    from htx.ic import quadrant_workload, shock_blob_workload
+def test_face_flags_match_standalone_and_drive_the_stage(case):
+    """The per-step face-flag kernel's flags equal the standalone inference's
+    logits against the threshold face by face (only logits within rounding
+    of the threshold may differ), and the learned stage kernel's residual
+    equals the CPU build of the same tile phases driven by those very flags
+    (to the rounding of FMA contraction, which on shock data may move an
+    FP32 cut-off decision on a few faces)."""
+    import ctypes
+    import os
 
-    w, nx, ny = (shock_blob_workload(4096), 512, 128) if case == "shock_blobs" else (quadrant_workload(256), 256, 256)
+    import htx
+    from conftest import ROOT
+    from htx.ic import quadrant_workload, shock_blob_workload
     from htx.learn import default_classifier
 
-    prm, _ = default_classifier()
-    thr = 1.0
-    print(f"[case] fused classifier flags vs standalone, {case} {nx}x{ny}", flush=True)
+    w, nx, ny = (shock_blob_workload(4096), 512, 128) if case == "shock_blobs" else (quadrant_workload(256), 256, 256)
+    prm, thr = default_classifier()
+    print(f"[case] face flags vs standalone inference and CPU tiling, {case} {nx}x{ny}", flush=True)
     s = htx.System2D(w["xl"], w["xr"], nx, w["yb"], w["yt"], ny, w["cfl"], w["tmax"], 0.0, w["gamma"],
                      flux="hllc", bc=w["bc"], precision="learned", classifier=(prm, thr))
     s.init_workload(w)
-    s.rhs()
+    r_gpu = s.rhs()
     fx, fy = s.classifier_flags()
-    for d, f in ((0, fx), (1, fy)):
+    for d, f in ((0, fx[:, 1:nx + 1]), (1, fy[1:ny + 1, :])):
         lg = s.classify(d)
         ref = (lg > thr).astype(np.uint8)
         bad = (f != ref) & (np.abs(lg - thr) > 1e-3)
         print(f"dir {d}: flagged {f.mean():.3f} (standalone {ref.mean():.3f}), mismatches {int(bad.sum())}",
               flush=True)
         assert bad.sum() == 0
+    lib_path = os.path.join(ROOT, "tests", "host", "libfv2d_host_fast.so")
+    if not os.path.exists(lib_path):
+        pytest.skip("tests/host/libfv2d_host_fast.so not built")
+    lib = ctypes.CDLL(lib_path)
+    D, DP, UP = ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_ubyte)
+    lib.fv2dh_rhs_flags.argtypes = [ctypes.c_int, ctypes.c_int, D, D, D, D, D, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, DP, DP, UP, UP]
+    lib.fv2dh_rhs_prec.argtypes = [ctypes.c_int, ctypes.c_int, D, D, D, D, D, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, DP, DP]
+    q = np.ascontiguousarray(s.get_state())
+    r_cpu = np.empty_like(q)
+    lib.fv2dh_rhs_flags(nx, ny, w["xl"], w["xr"], w["yb"], w["yt"], w["gamma"], 0, 1, 1, q.ctypes.data_as(DP),
+                        r_cpu.ctypes.data_as(DP), fx.ctypes.data_as(UP), fy.ctypes.data_as(UP))
     s.close()
+    # the same comparison for plain FP32-mixed measures how many cells the
+    # rounding of FMA contraction alone moves (FP32 cut-off decisions on
+    # nearly flat stencils)
+    m = htx.System2D(w["xl"], w["xr"], nx, w["yb"], w["yt"], ny, w["cfl"], w["tmax"], 0.0, w["gamma"],
+                     flux="hllc", bc=w["bc"], precision="fp32_mixed")
+    m.set_state(q)
+    rm_gpu = m.rhs()
+    m.close()
+    rm_cpu = np.empty_like(q)
+    lib.fv2dh_rhs_prec(nx, ny, w["xl"], w["xr"], w["yb"], w["yt"], w["gamma"], 0, 1, 1, 1, q.ctypes.data_as(DP),
+                       rm_cpu.ctypes.data_as(DP))
+    frac = lambda a, b: float((np.abs(a - b) > 1e-9 * (np.abs(b) + 1.0)).mean())  # noqa: E731
+    dl, dm = frac(r_gpu, r_cpu), frac(rm_gpu, rm_cpu)
+    print(f"residual GPU vs CPU tiling, cells differing: learned {dl:.5f}, fp32_mixed {dm:.5f}", flush=True)
+    assert dl <= dm + 0.005

@@ -247,3 +247,83 @@ def test_learned_all_smooth_residual(orc, harness_fast, flux):
     n = 40
     u = np.ones((4, n, n)) * np.array([1.2, 0.3, -0.4, 2.5])[:, None, None]
     assert np.abs(_rhs_prec(harness_fast, n, n, dom, "periodic", flux, 2, u)).max() < 1e-13
+
+
+def _rhs_flags(lib, nx, ny, dom, bc, flux, q, fx, fy):
+    UP = ctypes.POINTER(ctypes.c_ubyte)
+    lib.fv2dh_rhs_flags.argtypes = [ctypes.c_int, ctypes.c_int, D, D, D, D, D, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, DP, DP, UP, UP]
+    q = np.ascontiguousarray(q)
+    fx, fy = np.ascontiguousarray(fx, dtype=np.uint8), np.ascontiguousarray(fy, dtype=np.uint8)
+    r = np.empty_like(q)
+    lib.fv2dh_rhs_flags(nx, ny, *dom, 1.4, {"zero_grad": 0, "periodic": 1}[bc], {"rusanov": 0, "hllc": 1}[flux], 1,
+                        q.ctypes.data_as(DP), r.ctypes.data_as(DP), fx.ctypes.data_as(UP), fy.ctypes.data_as(UP))
+    return r
+
+
+def _periodic_flag_arrays(cx, cy):
+    """Extended face-flag arrays of the stage kernel from per-face flags of a
+    periodic grid: cx[j, i] for x face i | i+1, cy[j, i] for y face j | j+1."""
+    fx = np.concatenate([cx[:, -1:], cx, cx[:, :1]], axis=1)
+    fy = np.concatenate([cy[-1:, :], cy, cy[:1, :]], axis=0)
+    return fx, fy
+
+
+@pytest.mark.parametrize("bc", ["periodic", "zero_grad"])
+def test_learned_flag_reaches_its_face(harness_fast, bc):
+    """One face flagged smooth, every other face TENO5, on rough data (a
+    uniform flow with 3 % cell-to-cell noise, where TENO5 and the linear
+    scheme differ on every face): the residual changes only in the cells
+    whose fluxes use that recon face (x face i | i+1 of row j: cells i and
+    i+1 of rows j-2 .. j+2, through the face's own flux and the transverse
+    quadratures of its neighbours; a y face: the transpose), and most in the
+    two cells of the face itself (the centre of its own transverse window),
+    wherever the face sits relative to the 16 x 16 tiles and the boundary."""
+    nx, ny, dom = 48, 40, (0.0, 1.0, 0.0, 1.0)
+    rng = np.random.default_rng(21)
+    w = np.array([1.0, 0.3, -0.2, 1.0])[:, None, None] * (1.0 + 0.03 * rng.standard_normal((4, ny, nx)))
+    q = np.array([w[0], w[0] * w[1], w[0] * w[2], w[3] / 0.4 + 0.5 * w[0] * (w[1] ** 2 + w[2] ** 2)])
+    zx, zy = np.zeros((ny, nx + 2), np.uint8), np.zeros((ny + 2, nx), np.uint8)
+    r0 = _rhs_flags(harness_fast, nx, ny, dom, bc, "hllc", q, zx, zy)
+    for d, i, j in [(0, 20, 21), (0, 15, 16), (0, 16, 31), (0, 31, 15), (0, 1, 3), (0, 46, 38), (1, 17, 15),
+                    (1, 31, 20), (1, 5, 0), (1, 40, 16), (1, 47, 38)]:
+        fx, fy = zx.copy(), zy.copy()
+        if d == 0:
+            fx[j, i + 1] = 1
+        else:
+            fy[j + 1, i] = 1
+        r = _rhs_flags(harness_fast, nx, ny, dom, bc, "hllc", q, fx, fy)
+        delta = np.abs(r - r0).max(axis=0)
+        jj, ii = np.nonzero(delta > 0.0)
+        jm, im = np.unravel_index(np.argmax(delta), delta.shape)
+        near = lambda c, n, span: {(c + k) % n for k in span}  # noqa: E731  (periodic wrap; no-op inside)
+        if d == 0:
+            assert set(ii) <= near(i, nx, (0, 1)) and set(jj) <= near(j, ny, range(-2, 3)), (d, i, j, set(ii), set(jj))
+            assert jm == j and im in near(i, nx, (0, 1)), (d, i, j, jm, im)
+        else:
+            assert set(jj) <= near(j, ny, (0, 1)) and set(ii) <= near(i, nx, range(-2, 3)), (d, i, j, set(ii), set(jj))
+            assert im == i and jm in near(j, ny, (0, 1)), (d, i, j, jm, im)
+
+
+def test_learned_flag_placement_is_shift_equivariant(orc, harness_fast):
+    """The learned path reads each recon face's flag from the face-flag arrays
+    through the tile's index mapping: on a periodic grid, shifting the state
+    and the per-face flags by an offset that is not a multiple of the tile
+    size (so every face lands in another tile, at another position, or in
+    another tile's halo) shifts the residual bit for bit."""
+    from htx.ic import vortex_modes
+
+    nx, ny, dom = 48, 40, (0.0, 10.0, 0.0, 10.0)
+    prm = params(nx, ny, *dom, 1.4, "periodic", "hllc", True)
+    q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
+    rng = np.random.default_rng(12)
+    cx = (rng.random((ny, nx)) < 0.7).astype(np.uint8)
+    cy = (rng.random((ny, nx)) < 0.7).astype(np.uint8)
+    r = _rhs_flags(harness_fast, nx, ny, dom, "periodic", "hllc", q, *_periodic_flag_arrays(cx, cy))
+    sj, si = 7, 5
+    roll = lambda a: np.roll(np.roll(a, sj, axis=-2), si, axis=-1)  # noqa: E731
+    r2 = _rhs_flags(harness_fast, nx, ny, dom, "periodic", "hllc", roll(q), *_periodic_flag_arrays(roll(cx), roll(cy)))
+    np.testing.assert_array_equal(r2, roll(r))
+    # and the flags matter: all-TENO flags give another residual
+    r0 = _rhs_flags(harness_fast, nx, ny, dom, "periodic", "hllc", q, *_periodic_flag_arrays(0 * cx, 0 * cy))
+    assert np.abs(r0 - r).max() > 0.0
