@@ -117,8 +117,8 @@ HTX_HD bool recon_face_owned(int fi, int fj, int i0, int j0, int nx, int ny) {
 // characteristic TENO5 on faces (FV_System2D.cpp:372-400).  YDIR selects the
 // face family; the state is read in (rho, normal, tangential, E) order.  With
 // LEARNED, faces the classifier flagged take Lagrange::linear
-// (Schemes_Lagrange.cpp:1219-1233) instead, provided the linear face values
-// give positive density and pressure on both sides (checked in FP64; else
+// (Schemes_Lagrange.cpp:1219-1233) instead, provided the linear face states
+// have positive density and pressure on both sides (checked in FP64; else
 // TENO5); the face's flag is then overwritten with the scheme it took (1 =
 // linear), and cnt accumulates (owned faces, flagged, linear) of this thread.
 template <int TX, int TY, bool YDIR, bool MIXED = false, bool LEARNED = false>
@@ -132,25 +132,32 @@ HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid, int i0 = 0
         int fi = w % RW, fj = w / RW;
         int ca = YDIR ? (2 + fj) * T::HX + (1 + fi) : (1 + fj) * T::HX + (2 + fi);
         Eig E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
-        double c[4][6];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            int id = ca + (m - 2) * STEP;
-            eig_left<YDIR>(E, s.sQ[0][id], s.sQ[1][id], s.sQ[2][id], s.sQ[3][id], c[0][m], c[1][m], c[2][m], c[3][m]);
-        }
         bool lin = false;
         if (LEARNED) {
             const bool flag = s.sFlag[YDIR ? 1 : 0][w] != 0;
             if (flag) {
-                double lm[4], lp[4];
+                // The linear scheme commutes with the characteristic
+                // projection: reconstruct the conserved variables and
+                // project the two face states only (instead of six cells).
+                double um[4], up[4];
 #pragma unroll
-                for (int f = 0; f < 4; ++f) linear5(c[f], lm[f], lp[f]);
-                lin = face_state_positive(E, lm) && face_state_positive(E, lp);
+                for (int e = 0; e < 4; ++e) {
+                    double u[6];
+#pragma unroll
+                    for (int m = 0; m < 6; ++m) u[m] = s.sQ[e][ca + (m - 2) * STEP];
+                    linear5(u, um[e], up[e]);
+                }
+                // density and pressure (gamma - 1)(E - m^2 / 2 rho) positive, FP64
+                lin = um[0] > 0.0 && um[3] * um[0] - 0.5 * (um[1] * um[1] + um[2] * um[2]) > 0.0 && up[0] > 0.0 &&
+                      up[3] * up[0] - 0.5 * (up[1] * up[1] + up[2] * up[2]) > 0.0;
                 if (lin) {
+                    double cm[4], cp[4];
+                    eig_left<YDIR>(E, um[0], um[1], um[2], um[3], cm[0], cm[1], cm[2], cm[3]);
+                    eig_left<YDIR>(E, up[0], up[1], up[2], up[3], cp[0], cp[1], cp[2], cp[3]);
 #pragma unroll
                     for (int f = 0; f < 4; ++f) {
-                        s.sF[2 * f][w] = lm[f];
-                        s.sF[2 * f + 1][w] = lp[f];
+                        s.sF[2 * f][w] = cm[f];
+                        s.sF[2 * f + 1][w] = cp[f];
                     }
                 }
             }
@@ -164,6 +171,13 @@ HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid, int i0 = 0
             s.sFlag[YDIR ? 1 : 0][w] = lin ? 1 : 0;
         }
         if (!lin) {
+            double c[4][6];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) {
+                int id = ca + (m - 2) * STEP;
+                eig_left<YDIR>(E, s.sQ[0][id], s.sQ[1][id], s.sQ[2][id], s.sQ[3][id], c[0][m], c[1][m], c[2][m],
+                               c[3][m]);
+            }
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
                 double um, up;

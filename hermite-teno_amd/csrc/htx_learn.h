@@ -283,25 +283,22 @@ __device__ __forceinline__ float cls_logit(const unsigned short* xs, const unsig
         cls_b8 b2[2];
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
-            cls_f4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+            // the biases start the accumulators
+            const float* b1 = cvec + 16 * mb + 4 * g;
+            cls_f4 d = {b1[0], b1[1], b1[2], b1[3]};
             d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F1[mb * 64 + lane], bx, d, 0, 0, 0);
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                float h = gelu_f(d[c] + cvec[16 * mb + 4 * g + c]);
-                b2[mb >> 1][(mb & 1) * 4 + c] = (__bf16)h;
-            }
+            for (int c = 0; c < 4; ++c) b2[mb >> 1][(mb & 1) * 4 + c] = (__bf16)gelu_f(d[c]);
         }
         float part = 0.0f;
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
-            cls_f4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+            const float* bb = cvec + CLS_H + 16 * mb + 4 * g;
+            cls_f4 d = {bb[0], bb[1], bb[2], bb[3]};
             d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F2[(mb * 2 + 0) * 64 + lane], b2[0], d, 0, 0, 0);
             d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F2[(mb * 2 + 1) * 64 + lane], b2[1], d, 0, 0, 0);
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                int u = 16 * mb + 4 * g + c;
-                part += cvec[2 * CLS_H + u] * gelu_f(d[c] + cvec[CLS_H + u]);
-            }
+            for (int c = 0; c < 4; ++c) part += cvec[2 * CLS_H + 16 * mb + 4 * g + c] * gelu_f(d[c]);
         }
         part += __shfl_xor(part, 16, 64);
         part += __shfl_xor(part, 32, 64);

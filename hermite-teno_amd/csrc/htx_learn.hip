@@ -322,22 +322,34 @@ __global__ void __launch_bounds__(256) k_face_flags(LrnFlagsArgs a) {
     const int w = YDIR ? a.nx : a.nx + 2;
     const long long n = YDIR ? (long long)a.nx * (a.ny + 2) : (long long)(a.nx + 2) * a.ny;
     long long id = blockIdx.x * 256ll + threadIdx.x;
-    float x[CLS_IN];
-    for (int k = 0; k < CLS_IN; ++k) x[k] = 0.0f;
+    unsigned char* out = YDIR ? a.flags_y : a.flags_x;
+    Stencil6 st;
+    bool flat = true;  // faces beyond the array count as flat
     if (id < n) {
         int i = (int)(id % w), j = (int)(id / w);
         if (YDIR)
             j -= 1;
         else
             i -= 1;
-        Stencil6 st;
         load_stencil<YDIR>(a.q, a.nx, a.ny, a.pitch, a.fstride, a.bc, i, j, st);
-        stencil_features<YDIR>(st, a.gam, x);
+        for (int m = 1; m < 6; ++m)
+            for (int e = 0; e < 4; ++e) flat = flat && st.q[m][e] == st.q[0][e];
     }
+    // A stencil of six equal states is smooth for TENO5 (every smoothness
+    // measure is zero, no candidate is cut): when all the workgroup's faces
+    // are such, the matrix cores are skipped (the classifier agrees there,
+    // with a logit far above any threshold).
+    if (__syncthreads_and(flat ? 1 : 0)) {
+        if (id < n) out[id] = 1;
+        return;
+    }
+    float x[CLS_IN];
+    for (int k = 0; k < CLS_IN; ++k) x[k] = 0.0f;
+    if (id < n) stencil_features<YDIR>(st, a.gam, x);
     cls_stage_features(xs[threadIdx.x >> 6], x);
     __syncthreads();
     float z = cls_logit(xs[threadIdx.x >> 6], a.frag, a.cvec);
-    if (id < n) (YDIR ? a.flags_y : a.flags_x)[id] = z > a.threshold ? 1 : 0;
+    if (id < n) out[id] = (flat || z > a.threshold) ? 1 : 0;
 }
 
 }  // namespace

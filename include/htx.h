@@ -218,7 +218,8 @@ int htx_trainer_eval_inference(htx_trainer* t, double threshold, long long* coun
 
 /* Loads classifier parameters into a solver (packed to bf16); a face whose
  * logit exceeds threshold takes the linear scheme in precision
- * HTX_PREC_LEARNED. */
+ * HTX_PREC_LEARNED, and so does a face whose six stencil states are equal
+ * (TENO5 cuts no candidate there), whatever the threshold. */
 int htx_load_classifier(htx_solver* h, const float* params, double threshold);
 
 /* Logits (and, if feat is not null, the 12 features) of every face of one
