@@ -215,6 +215,13 @@ HTX_HD long long fvh_off(const FvhArgs& a, int i, int j) {
 // dir 0: row j, field e, v replaced with i ascending (Gauss-Seidel: the new
 // v_{i-1} is used, ghost neighbours keep their pre-limiting values); dir 1:
 // column i, w replaced with j ascending.
+//
+// The chain's loads do not depend on it: u_{k+1} and the old v_{k+1} are
+// fetched LIMIT_AHEAD steps early into a ring of registers and u_{k-1}, u_k
+// slide along, so memory latency overlaps the recurrence (the arithmetic of
+// every step is unchanged).
+constexpr int LIMIT_AHEAD = 8;
+
 HTX_HD void fvh_limit_line(const FvhArgs& a, int dir, int line, int e) {
     HTX_NO_CONTRACT
     const long long fs = a.fstride;
@@ -224,10 +231,29 @@ HTX_HD void fvh_limit_line(const FvhArgs& a, int dir, int line, int e) {
     int n = dir == 0 ? a.nx : a.ny;
     auto at = [&](int k) { return dir == 0 ? fvh_off(a, k, line) : fvh_off(a, line, k); };
     double prev = D[at(-1)];  // ghost: pre-limiting value
-    for (int k = 0; k < n; ++k) {
-        double v = herm_limiter(a.hc.dif, U[at(k - 1)], U[at(k)], U[at(k + 1)], prev, D[at(k + 1)]);
-        O[at(k)] = v;
-        prev = v;
+    double um = U[at(-1)], uc = U[at(0)];
+    double ru[LIMIT_AHEAD], rd[LIMIT_AHEAD];  // u_{k+1}, old v_{k+1} of steps k .. k + LIMIT_AHEAD - 1
+#pragma unroll
+    for (int p = 0; p < LIMIT_AHEAD; ++p) {
+        ru[p] = U[at(p + 1)];
+        rd[p] = D[at(p + 1)];
+    }
+    for (int k0 = 0; k0 < n; k0 += LIMIT_AHEAD) {
+#pragma unroll
+        for (int p = 0; p < LIMIT_AHEAD; ++p) {
+            const int k = k0 + p;
+            if (k < n) {
+                const double up = ru[p], dp = rd[p];
+                // indices past the line map through the boundary condition
+                ru[p] = U[at(k + LIMIT_AHEAD + 1)];
+                rd[p] = D[at(k + LIMIT_AHEAD + 1)];
+                double v = herm_limiter(a.hc.dif, um, uc, up, prev, dp);
+                O[at(k)] = v;
+                prev = v;
+                um = uc;
+                uc = up;
+            }
+        }
     }
 }
 
