@@ -134,15 +134,28 @@ static hipError_t launch_tile(const Fv2dArgs& a0, int flux, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y, hipStream_t s) {
-    (void)tile_x;
-    (void)tile_y;
+template <int TX, int TY>
+static hipError_t launch_prec(const Fv2dArgs& a, int flux, hipStream_t s) {
     if (a.learned) {
         if (!a.face_flags_x || !a.face_flags_y) return hipErrorInvalidValue;
-        return launch_tile<16, 16, true, true>(a, flux, s);
+        return launch_tile<TX, TY, true, true>(a, flux, s);
     }
-    if (a.mixed) return launch_tile<16, 16, true, false>(a, flux, s);
-    return launch_tile<16, 16, false, false>(a, flux, s);
+    if (a.mixed) return launch_tile<TX, TY, true, false>(a, flux, s);
+    return launch_tile<TX, TY, false, false>(a, flux, s);
+}
+
+// Tile shapes: 16 x 16 (default), and in the fast build 16 x 8 and 32 x 8
+// (smaller LDS footprints, more workgroups per CU for kernels with few
+// registers); any other request takes the default.
+hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y, hipStream_t s) {
+#if !HTX_EXACT_DIVISION
+    if (tile_x == 16 && tile_y == 8) return launch_prec<16, 8>(a, flux, s);
+    if (tile_x == 32 && tile_y == 8) return launch_prec<32, 8>(a, flux, s);
+#else
+    (void)tile_x;
+    (void)tile_y;
+#endif
+    return launch_prec<16, 16>(a, flux, s);
 }
 
 hipError_t launch_dt(Ctl* c, double dx, double dy, hipStream_t s) {

@@ -279,12 +279,15 @@ HTX_HD void fvh_recon_face(const FvhArgs& a, int i, int j, double* F, long long 
     // hteno on (L u, L d_normal) over cells -1..2, teno5 on L d_tangential over -2..3
     double A[4][4], B[4][4], C[4][6];
     const int nc = YDIR ? 4 : 0;  // normal-derivative block: v (x) or w (y) -> limited comp offset
+    // the loops are unrolled so that A, B, C stay in registers
+#pragma unroll
     for (int m = 0; m < 6; ++m) {
         int k = m - 2;
         int ci = YDIR ? i : i + k, cj = YDIR ? j + k : j;
         double q[4];
         cellq(k, q);
         double dn[4], dt[4];
+#pragma unroll
         for (int e = 0; e < 4; ++e) {
             dn[e] = fvh_lim(a, nc + e, ci, cj);        // normal derivative moment
             dt[e] = fvh_lim(a, (4 - nc) + e, ci, cj);  // tangential derivative moment
@@ -292,13 +295,17 @@ HTX_HD void fvh_recon_face(const FvhArgs& a, int i, int j, double* F, long long 
         double c[4];
         if (m >= 1 && m <= 4) {
             eig_left<YDIR>(E, q[0], q[1], q[2], q[3], c[0], c[1], c[2], c[3]);
+#pragma unroll
             for (int e = 0; e < 4; ++e) A[e][m - 1] = c[e];
             eig_left<YDIR>(E, dn[0], dn[1], dn[2], dn[3], c[0], c[1], c[2], c[3]);
+#pragma unroll
             for (int e = 0; e < 4; ++e) B[e][m - 1] = c[e];
         }
         eig_left<YDIR>(E, dt[0], dt[1], dt[2], dt[3], c[0], c[1], c[2], c[3]);
+#pragma unroll
         for (int e = 0; e < 4; ++e) C[e][m] = c[e];
     }
+#pragma unroll
     for (int e = 0; e < 4; ++e) {
         double o[4], um, up;
         hteno(a.hc, A[e], B[e], o);
@@ -361,6 +368,7 @@ HTX_HD void drusanov(double gam, double g1, const Eig& E, const double ql[4], co
 template <bool YDIR, int FLUX>
 HTX_HD void fvh_flux_gp(const FvhArgs& a, int i, int j, int m, const double* F, long long fstr, double out[12]) {
     long long fid[5];
+#pragma unroll
     for (int k = 0; k < 5; ++k) {
         if (YDIR)
             fid[k] = (long long)(j + 1) * a.nx + map_index(i - 2 + k, a.nx, a.bc);
@@ -371,15 +379,18 @@ HTX_HD void fvh_flux_gp(const FvhArgs& a, int i, int j, int m, const double* F, 
     {
         const long long fs = a.fstride;
         long long oa = fvh_off(a, i, j), ob = YDIR ? fvh_off(a, i, j + 1) : fvh_off(a, i + 1, j);
+#pragma unroll
         for (int e = 0; e < 4; ++e) {
             qa[e] = a.q[oa + e * fs];
             qb[e] = a.q[ob + e * fs];
         }
     }
     Eig E = roe_eig(a.g1, YDIR, cell_prim(a.g1, qa[0], qa[1], qa[2], qa[3]), cell_prim(a.g1, qb[0], qb[1], qb[2], qb[3]));
-    double Q[6][4];
+    double Q[6][4];  // unrolled loops keep Q in registers
+#pragma unroll
     for (int sr = 0; sr < 6; ++sr) {
         double c[4];
+#pragma unroll
         for (int e = 0; e < 4; ++e) {
             const double* S = F + (long long)(sr * 4 + e) * fstr;
             c[e] = weno3q_gp(a.qc, m, S[fid[0]], S[fid[1]], S[fid[2]], S[fid[3]], S[fid[4]]);
@@ -412,14 +423,15 @@ HTX_HD void fvh_flux_gp(const FvhArgs& a, int i, int j, int m, const double* F, 
     drusanov<YDIR>(a.gam, a.g1, E, Q[0], Q[1], pl, pr, am, Q[sv], Q[sv + 1], fv);
     drusanov<YDIR>(a.gam, a.g1, E, Q[0], Q[1], pl, pr, am, Q[sw], Q[sw + 1], fw);
     const double gw = (m == 1) ? 8.0 / 9.0 / 2.0 : 5.0 / 9.0 / 2.0;
-    const double* fs3[3] = {fu, fv, fw};
-    for (int g = 0; g < 3; ++g) {
-        const double* f = fs3[g];
+    auto put = [&](int g, const double f[4]) {
         out[4 * g + 0] = gw * f[0];
         out[4 * g + 1] = gw * (YDIR ? f[2] : f[1]);
         out[4 * g + 2] = gw * (YDIR ? f[1] : f[2]);
         out[4 * g + 3] = gw * f[3];
-    }
+    };
+    put(0, fu);
+    put(1, fv);
+    put(2, fw);
 }
 
 // divergence of the 12 fluxes (FVHermite_System2D.cpp:710-730), SSP-RK3

@@ -72,7 +72,8 @@ typedef struct htx_params {
     int bc;             /* htx_bc: FV_System2D.cpp:365-366 */
     int precision;      /* htx_precision */
     int ref_quirks;     /* 1: reproduce the reference's HLLC and maxSpeed quirks */
-    int tile_x, tile_y; /* 0: default tile shape */
+    int tile_x, tile_y; /* TENO5 stage tile: 0 = 16 x 16; the FP64_FAST, FP32_MIXED and
+                           LEARNED precisions also build 16 x 8 and 32 x 8 */
 } htx_params;
 
 typedef struct htx_solver htx_solver;
