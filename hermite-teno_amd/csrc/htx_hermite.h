@@ -211,19 +211,138 @@ HTX_HD long long fvh_off(const FvhArgs& a, int i, int j) {
     return (long long)map_index(j, a.ny, a.bc) * a.pitch + map_index(i, a.nx, a.bc);
 }
 
+// herm_limiter split at its Gauss-Seidel dependence.  Along a limiter chain
+// only d0, the limited left neighbour, comes from the previous step; every
+// subexpression of u0, u1, u2 and d2 alone is computed ahead of the chain
+// (herm_limiter_pre, 15 values) and the chain step (herm_limiter_step) does
+// the rest.  In the strict build the two halves perform exactly the
+// operations of herm_limiter in its order, so the result is bit-identical to
+// it; the fast build reassociates (p0, the smoothness sum, the weighted
+// sum over the candidates, normalised once at the end), turns divisions by
+// chain-independent or constant denominators into products and the two
+// left on the chain into refined hardware reciprocals.
+constexpr int LIM_NV = 15;
+enum LimPreSlot {
+    LP_X = 0,   // 0.75 (u2 - u0) / dif          (fast: minus 0.25 d2)
+    LP_D2,      // d2
+    LP_P,       // u part of the first smoothness factor  (fast: minus LP_Q)
+    LP_Q,       // 494958 d2 dif
+    LP_T2,      // remaining smoothness terms            (fast: their sum)
+    LP_T3,
+    LP_T4,
+    LP_T5,
+    LP_SM1,     // (u0 - u1)^2
+    LP_SM2,     // (u1 - u2)^2
+    LP_S1,      // sm1 + 1e-6                 (fast: K = l1 / (sm1 + 1e-6) + l2 / (sm2 + 1e-6))
+    LP_S2,      // sm2 + 1e-6                 (fast: M = l1 p1 / (sm1 + 1e-6) + l2 p2 / (sm2 + 1e-6))
+    LP_P1,      // (u1 - u0) / dif
+    LP_P2,      // (u2 - u1) / dif
+    LP_C        // (l1 / l0) p1 + (l2 / l0) p2  (fast: l1 p1 + l2 p2)
+};
+
+// Slots the chain step reads (the fast build folds the others away): their
+// number LIM_NS and each slot's position among them (-1: not read), the
+// layout of the values handed from the precompute to the chain.
+#if HTX_EXACT_DIVISION
+constexpr int LIM_NS = LIM_NV;
+HTX_HD constexpr int lim_slot_pos(int v) { return v; }
+#else
+constexpr int LIM_NS = 8;
+HTX_HD constexpr int lim_slot_pos(int v) {
+    return v == LP_X ? 0 : v == LP_P ? 1 : v == LP_T2 ? 2 : v == LP_SM1 ? 3 : v == LP_SM2 ? 4 : v == LP_S1 ? 5
+         : v == LP_S2 ? 6 : v == LP_C ? 7 : -1;
+}
+#endif
+
+HTX_HD void herm_limiter_pre(double dif, double u0, double u1, double u2, double d2, double o[LIM_NV]) {
+#if HTX_EXACT_DIVISION
+    HTX_NO_CONTRACT
+    const double l0 = 1.0 - 2E-5, l1 = 1E-5, l2 = 1E-5;
+    const double D2 = d2 * dif;
+    const double p1 = (u1 - u0) / dif, p2 = (u2 - u1) / dif;
+    const double sm1 = (u0 - u1) * (u0 - u1), sm2 = (u1 - u2) * (u1 - u2);
+    o[LP_X] = 0.75 * (u2 - u0) / dif;
+    o[LP_D2] = d2;
+    o[LP_P] = 1029270.0 * u2 - 2129184.0 * u1 + 1099914.0 * u0;
+    o[LP_Q] = 494958.0 * D2;
+    o[LP_T2] = (-1029270.0 * u0 + 2129184.0 * u1 - 1099914.0 * u2 + 281121.0 * D2) * D2;
+    o[LP_T3] = (1099445.0 * u2 - 4317056.0 * u1 + 2118166.0 * u0) * u2;
+    o[LP_T4] = 4317056.0 * (u1 - u0) * u1;
+    o[LP_T5] = 1099445.0 * u0 * u0;
+    o[LP_SM1] = sm1;
+    o[LP_SM2] = sm2;
+    o[LP_S1] = sm1 + 1E-6;
+    o[LP_S2] = sm2 + 1E-6;
+    o[LP_P1] = p1;
+    o[LP_P2] = p2;
+    o[LP_C] = (l1 / l0) * p1 + (l2 / l0) * p2;
+#else
+    // w_k = l_k (1 + tau R_k) with R_k = 1 / (sm_k + 1e-6) and l0 + l1 + l2 = 1:
+    //   sum_k w_k = 1 + tau (l0 R0 + K),  K = l1 R1 + l2 R2
+    //   w0 (p0 / l0 - C) + w1 p1 + w2 p2 = (1 + tau R0)(p0 - C') + C' + tau M,
+    //   C' = l1 p1 + l2 p2,  M = l1 R1 p1 + l2 R2 p2
+    // (only the slots lim_slot_pos places are set).  Contraction is off and
+    // the fused operations are written out, so that every kernel built from
+    // these bodies (single problem, batch) rounds the same way.
+    HTX_NO_CONTRACT
+    const double l1 = 1E-5, l2 = 1E-5;
+    const double idif = 1.0 / dif;  // uniform: hoisted out of the callers' loops
+    const double D2 = d2 * dif;
+    const double p1 = (u1 - u0) * idif, p2 = (u2 - u1) * idif;
+    const double sm1 = (u0 - u1) * (u0 - u1), sm2 = (u1 - u2) * (u1 - u2);
+    const double r1 = drcp(sm1 + 1E-6), r2 = drcp(sm2 + 1E-6);
+    o[LP_X] = 0.75 * (u2 - u0) * idif - 0.25 * d2;
+    o[LP_P] = ((1029270.0 * u2 - 2129184.0 * u1) + 1099914.0 * u0) - 494958.0 * D2;
+    o[LP_T2] = (((-1029270.0 * u0 + 2129184.0 * u1) - 1099914.0 * u2) + 281121.0 * D2) * D2 +
+               ((1099445.0 * u2 - 4317056.0 * u1) + 2118166.0 * u0) * u2 + 4317056.0 * (u1 - u0) * u1 +
+               1099445.0 * u0 * u0;
+    o[LP_SM1] = sm1;
+    o[LP_SM2] = sm2;
+    o[LP_S1] = l1 * r1 + l2 * r2;
+    o[LP_S2] = l1 * r1 * p1 + l2 * r2 * p2;
+    o[LP_C] = l1 * p1 + l2 * p2;
+#endif
+}
+
+// one chain step: the limited moment from the precomputed values and d0
+HTX_HD double herm_limiter_step(double dif, const double o[LIM_NV], double d0) {
+#if HTX_EXACT_DIVISION
+    HTX_NO_CONTRACT
+    const double l0 = 1.0 - 2E-5, l1 = 1E-5, l2 = 1E-5;
+    const double D0 = d0 * dif;
+    double p0 = o[LP_X] - 0.25 * (d0 + o[LP_D2]);
+    const double e1 = (o[LP_P] + 281121.0 * D0) - o[LP_Q];
+    double sm0 = fabs((((e1 * D0 + o[LP_T2]) + o[LP_T3]) + o[LP_T4]) + o[LP_T5]) / 6720.0;
+    double a = fabs(sm0 - o[LP_SM1]) + fabs(sm0 - o[LP_SM2]);
+    double tau = a * a / 4.0;
+    double w0 = l0 * (1.0 + tau / (sm0 + 1E-6));
+    double w1 = l1 * (1.0 + tau / o[LP_S1]);
+    double w2 = l2 * (1.0 + tau / o[LP_S2]);
+    double ws = (w0 + w1) + w2;
+    w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
+    return w0 * (p0 / l0 - o[LP_C]) + (w1 * o[LP_P1] + w2 * o[LP_P2]);
+#else
+    HTX_NO_CONTRACT
+    const double l0 = 1.0 - 2E-5;
+    const double D0 = d0 * dif;
+    const double p0 = fma(-0.25, d0, o[LP_X]);
+    const double sm0 = fabs(fma(fma(281121.0, D0, o[LP_P]), D0, o[LP_T2])) * (1.0 / 6720.0);
+    const double a = fabs(sm0 - o[LP_SM1]) + fabs(sm0 - o[LP_SM2]);
+    const double tau = a * a * 0.25;
+    const double r0 = drcp(sm0 + 1E-6);
+    const double ws = fma(tau, fma(l0, r0, o[LP_S1]), 1.0);
+    const double num = fma(fma(tau, r0, 1.0), p0 - o[LP_C], fma(tau, o[LP_S2], o[LP_C]));
+    return num * drcp(ws);
+#endif
+}
+
 // limitDerivatives, FVHermite_System2D.cpp:393-427: one sequential chain.
 // dir 0: row j, field e, v replaced with i ascending (Gauss-Seidel: the new
 // v_{i-1} is used, ghost neighbours keep their pre-limiting values); dir 1:
-// column i, w replaced with j ascending.
-//
-// The chain's loads do not depend on it: u_{k+1} and the old v_{k+1} are
-// fetched LIMIT_AHEAD steps early into a ring of registers and u_{k-1}, u_k
-// slide along, so memory latency overlaps the recurrence (the arithmetic of
-// every step is unchanged).
-constexpr int LIMIT_AHEAD = 8;
-
+// column i, w replaced with j ascending.  This is the CPU form (one chain at
+// a time, the test harnesses and the batched launch); the single-problem GPU
+// kernel computes the same pre/step halves on two wavefronts.
 HTX_HD void fvh_limit_line(const FvhArgs& a, int dir, int line, int e) {
-    HTX_NO_CONTRACT
     const long long fs = a.fstride;
     const double* U = a.q + e * fs;
     const double* D = a.q + (dir == 0 ? 4 + e : 8 + e) * fs;
@@ -232,28 +351,16 @@ HTX_HD void fvh_limit_line(const FvhArgs& a, int dir, int line, int e) {
     auto at = [&](int k) { return dir == 0 ? fvh_off(a, k, line) : fvh_off(a, line, k); };
     double prev = D[at(-1)];  // ghost: pre-limiting value
     double um = U[at(-1)], uc = U[at(0)];
-    double ru[LIMIT_AHEAD], rd[LIMIT_AHEAD];  // u_{k+1}, old v_{k+1} of steps k .. k + LIMIT_AHEAD - 1
-#pragma unroll
-    for (int p = 0; p < LIMIT_AHEAD; ++p) {
-        ru[p] = U[at(p + 1)];
-        rd[p] = D[at(p + 1)];
-    }
-    for (int k0 = 0; k0 < n; k0 += LIMIT_AHEAD) {
-#pragma unroll
-        for (int p = 0; p < LIMIT_AHEAD; ++p) {
-            const int k = k0 + p;
-            if (k < n) {
-                const double up = ru[p], dp = rd[p];
-                // indices past the line map through the boundary condition
-                ru[p] = U[at(k + LIMIT_AHEAD + 1)];
-                rd[p] = D[at(k + LIMIT_AHEAD + 1)];
-                double v = herm_limiter(a.hc.dif, um, uc, up, prev, dp);
-                O[at(k)] = v;
-                prev = v;
-                um = uc;
-                uc = up;
-            }
-        }
+    for (int k = 0; k < n; ++k) {
+        // indices past the line map through the boundary condition
+        const double up = U[at(k + 1)];
+        double o[LIM_NV] = {};
+        herm_limiter_pre(a.hc.dif, um, uc, up, D[at(k + 1)], o);
+        double v = herm_limiter_step(a.hc.dif, o, prev);
+        O[at(k)] = v;
+        prev = v;
+        um = uc;
+        uc = up;
     }
 }
 

@@ -86,6 +86,7 @@ def build_oracle():
 HOST_HARNESS = os.path.join(ROOT, "tests", "host", "libfv2d_host.so")
 HOST_HARNESS_FAST = os.path.join(ROOT, "tests", "host", "libfv2d_host_fast.so")
 HOST_HARNESS_HERMITE = os.path.join(ROOT, "tests", "host", "libfvh2d_host.so")
+HOST_HARNESS_HERMITE_FAST = os.path.join(ROOT, "tests", "host", "libfvh2d_host_fast.so")
 HOST_HARNESS_BATCH = os.path.join(ROOT, "tests", "host", "libbatch_host.so")
 HOST_HARNESS_CONTRACT = os.path.join(ROOT, "tests", "host", "libfv2d_host_contract.so")
 
@@ -102,6 +103,7 @@ def build_host_harness():
     _run(base + ["-o", HOST_HARNESS_FAST, src])
     srch = os.path.join(ROOT, "tests", "host", "fvh2d_host.hip")
     _run(base + ["-DHTX_EXACT_DIVISION=1", "-o", HOST_HARNESS_HERMITE, srch])
+    _run(base + ["-o", HOST_HARNESS_HERMITE_FAST, srch])
     # the product's contraction setting, with hardware FMA on the host: the
     # learned path's features must not depend on it
     _run([hipcc, "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=fast-honor-pragmas", "-mfma",

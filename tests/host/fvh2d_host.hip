@@ -75,3 +75,16 @@ extern "C" int fvhh_rhs(int nx, int ny, double xl, double xr, double yb, double 
     }
     return 0;
 }
+
+// The limiter in one piece (herm_limiter) and split at its chain dependence
+// (herm_limiter_pre + herm_limiter_step), over n input rows
+// (u0, u1, u2, d0, d2): the two results of each row.
+extern "C" void fvhh_limiter_split(int n, double dif, const double* in, double* whole, double* split) {
+    for (int r = 0; r < n; ++r) {
+        const double* x = in + 5 * r;
+        whole[r] = herm_limiter(dif, x[0], x[1], x[2], x[3], x[4]);
+        double o[LIM_NV] = {};
+        herm_limiter_pre(dif, x[0], x[1], x[2], x[4], o);
+        split[r] = herm_limiter_step(dif, o, x[3]);
+    }
+}

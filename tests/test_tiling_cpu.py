@@ -136,6 +136,58 @@ def test_hermite_kernels_cpu_build(orc, harness_hermite, nx, ny, bc, flux):
     np.testing.assert_array_equal(r, rc)
 
 
+def test_hermite_limiter_split_is_bit_identical(harness_hermite):
+    """The limiter split at its chain dependence (the chain-independent
+    subexpressions computed ahead, the rest per step) performs the same
+    operations as the limiter in one piece: bit-identical results on random
+    data of every scale, on flat and linear data, at jumps and at zero."""
+    lib = harness_hermite
+    lib.fvhh_limiter_split.argtypes = [ctypes.c_int, D, DP, DP, DP]
+    rng = np.random.default_rng(21)
+    rows = []
+    for scale in [1e-8, 1e-3, 1.0, 1e3, 1e8]:
+        rows.append(rng.normal(size=(4000, 5)) * scale)
+    flat = np.ones((50, 5)) * rng.uniform(0.1, 10, size=(50, 1))
+    flat[:, 3:] = 0.0
+    lin = np.cumsum(np.ones((50, 5)), axis=1) * rng.uniform(-2, 2, size=(50, 1))
+    jump = np.zeros((50, 5))
+    jump[:, 2] = rng.uniform(0.5, 5, 50)
+    rows += [flat, lin, jump, np.zeros((1, 5))]
+    x = np.ascontiguousarray(np.concatenate(rows))
+    for dif in [0.01, 0.37, 1.0]:
+        whole = np.empty(len(x))
+        split = np.empty(len(x))
+        lib.fvhh_limiter_split(len(x), dif, x.ctypes.data_as(DP), whole.ctypes.data_as(DP), split.ctypes.data_as(DP))
+        np.testing.assert_array_equal(whole.view(np.int64), split.view(np.int64))
+
+
+LIB_HERM_FAST = os.path.join(ROOT, "tests", "host", "libfvh2d_host_fast.so")
+
+
+@pytest.mark.parametrize("bc,flux", [("periodic", "rusanov"), ("zero_grad", "hllc")])
+def test_hermite_kernels_cpu_build_fast(orc, bc, flux):
+    """The fast build's Hermite bodies (reassociated limiter step, shared
+    reciprocals) stay within 1e-12 relative L1 of the oracle, limited
+    moments and residual, on the vortex."""
+    if not os.path.exists(LIB_HERM_FAST):
+        pytest.skip("tests/host/libfvh2d_host_fast.so not built (run __graft_entry__.build())")
+    from htx.ic import vortex_modes
+
+    lib = ctypes.CDLL(LIB_HERM_FAST)
+    lib.fvhh_rhs.argtypes = [ctypes.c_int, ctypes.c_int, D, D, D, D, D, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                             DP, DP, DP]
+    nx, ny, dom = 24, 20, (0.0, 10.0, 0.0, 10.0)
+    prm = params(nx, ny, *dom, 1.4, bc, flux, True)
+    q = orc.ic_vortex_hermite(prm, (5.0, 5.0), vortex_modes())
+    r = np.empty_like(q)
+    ql = np.empty_like(q)
+    lib.fvhh_rhs(nx, ny, *dom, 1.4, {"zero_grad": 0, "periodic": 1}[bc], {"rusanov": 0, "hllc": 1}[flux], 1,
+                 q.ctypes.data_as(DP), r.ctypes.data_as(DP), ql.ctypes.data_as(DP))
+    rc, qlc = orc.fvh2d_rhs_lim(prm, q)
+    assert np.abs(ql - qlc).sum() / np.abs(qlc).sum() < 1e-12
+    assert np.abs(r - rc).sum() / np.abs(rc).sum() < 1e-12
+
+
 @pytest.mark.parametrize("flux", ["hllc", "rusanov"])
 def test_fp32_mixed_residual_close_to_fp64(orc, harness_fast, flux):
     """FP32-mixed mode (cut-off decisions and nonlinear weights in FP32,
