@@ -99,32 +99,12 @@ __global__ void __launch_bounds__(256) kb_fvh_recon(BatchHermArgs B) {
 
 template <bool YDIR, int FLUX>
 __global__ void __launch_bounds__(192) kb_fvh_flux(BatchHermArgs B) {
-    __shared__ double acc[3][12][64];
+    __shared__ double lds[24 * 3 * 64];
     int lb;
     const FvhArgs& a = herm_args(B, YDIR ? HK_FLUX_Y : HK_FLUX_X, lb);
     if (!a.ctl->active) return;
     long long nf = YDIR ? (long long)a.nx * (a.ny + 1) : (long long)(a.nx + 1) * a.ny;
-    long long t = lb * 64ll + threadIdx.x;
-    int m = threadIdx.y;
-    if (t < nf) {
-        int i, j;
-        if (YDIR) {
-            i = (int)(t % a.nx);
-            j = (int)(t / a.nx) - 1;
-        } else {
-            i = (int)(t % (a.nx + 1)) - 1;
-            j = (int)(t / (a.nx + 1));
-        }
-        double o[12];
-        fvh_flux_gp<YDIR, FLUX>(a, i, j, m, YDIR ? a.fy : a.fx, nf, o);
-        for (int c = 0; c < 12; ++c) acc[m][c][threadIdx.x] = o[c];
-    }
-    __syncthreads();
-    if (t < nf && m == 0) {
-        double* G = YDIR ? a.gy : a.gx;
-        for (int c = 0; c < 12; ++c)
-            G[c * nf + t] = (acc[0][c][threadIdx.x] + acc[1][c][threadIdx.x]) + acc[2][c][threadIdx.x];
-    }
+    fvh_flux_block<YDIR, FLUX>(a, lb * 64ll, nf, lds);
 }
 
 __global__ void __launch_bounds__(256) kb_fvh_update(BatchHermArgs B) {

@@ -5,10 +5,12 @@
 //   k_fvh_recon_x/y one work item per face: Roe eigensystem, characteristic
 //                   HTENO on (u, normal moment) and TENO5 on the tangential
 //                   moment -> 24 face values
-//   k_fvh_flux_x/y  one work item per (face, Gauss point): transverse 3-point
-//                   quadrature of the 24 series, back-projection, HLLC or
-//                   characteristic Rusanov for u and dRusanov for the moments;
-//                   the three Gauss points are summed in order through LDS
+//   k_fvh_flux_x/y  64 faces x 3 Gauss points per workgroup: transverse
+//                   3-point quadrature of the 24 series (each series' three
+//                   points on one work item), back-projection, HLLC or
+//                   characteristic Rusanov for u and dRusanov for the moments
+//                   per (face, Gauss point); the points are summed in order
+//                   through LDS
 //   k_fvh_update    divergence of the 12 fluxes, SSP-RK3 stage, CFL maxima
 // The work-item bodies live in htx_hermite.h and are shared with the CPU
 // build used by the tests.
@@ -158,29 +160,9 @@ __global__ void __launch_bounds__(256) k_fvh_recon_y(FvhArgs a) {
 // block (64 faces) x (3 Gauss points)
 template <bool YDIR, int FLUX>
 __global__ void __launch_bounds__(192) k_fvh_flux(FvhArgs a) {
-    __shared__ double acc[3][12][64];
+    __shared__ double lds[24 * 3 * 64];
     long long nf = YDIR ? (long long)a.nx * (a.ny + 1) : (long long)(a.nx + 1) * a.ny;
-    long long t = blockIdx.x * 64ll + threadIdx.x;
-    int m = threadIdx.y;
-    if (t < nf) {
-        int i, j;
-        if (YDIR) {
-            i = (int)(t % a.nx);
-            j = (int)(t / a.nx) - 1;
-        } else {
-            i = (int)(t % (a.nx + 1)) - 1;
-            j = (int)(t / (a.nx + 1));
-        }
-        double o[12];
-        fvh_flux_gp<YDIR, FLUX>(a, i, j, m, YDIR ? a.fy : a.fx, nf, o);
-        for (int c = 0; c < 12; ++c) acc[m][c][threadIdx.x] = o[c];
-    }
-    __syncthreads();
-    if (t < nf && m == 0) {
-        double* G = YDIR ? a.gy : a.gx;
-        for (int c = 0; c < 12; ++c)
-            G[c * nf + t] = (acc[0][c][threadIdx.x] + acc[1][c][threadIdx.x]) + acc[2][c][threadIdx.x];
-    }
+    fvh_flux_block<YDIR, FLUX>(a, blockIdx.x * 64ll, nf, lds);
 }
 
 __global__ void __launch_bounds__(256) k_fvh_update(FvhArgs a) {

@@ -466,15 +466,17 @@ HTX_HD void drusanov(double gam, double g1, const Eig& E, const double ql[4], co
     f[3] = 0.5 * ((jl[3] + jr[3]) - r3);
 }
 
-// Transverse quadrature, back-projection and the u, v, w fluxes of one face
-// at one Gauss point m (FVHermite_System2D.cpp:497-640).  The face lies
-// between cell (i, j) and its +1 neighbour in direction YDIR; the 5
-// transverse face-state rows are read at boundary-mapped positions.  Returns
-// the 12 weighted fluxes (u, v, w; each in (rho, x, y, E) order); the caller
-// sums the three Gauss points in order.
-template <bool YDIR, int FLUX>
-HTX_HD void fvh_flux_gp(const FvhArgs& a, int i, int j, int m, const double* F, long long fstr, double out[12]) {
-    long long fid[5];
+// Transverse quadrature, back-projection and the u, v, w fluxes of a face
+// (FVHermite_System2D.cpp:497-640).  The face lies between cell (i, j) and its
+// +1 neighbour in direction YDIR.  In two parts:
+//   fvh_series_gp3  the three Gauss-point values of one of the 24 face-state
+//                   series (6 series x 4 characteristic fields), from its 5
+//                   transverse rows read at boundary-mapped positions fid
+//   fvh_flux_at     from the 24 series values at Gauss point m: back-projection
+//                   and the 12 fluxes (u, v, w; each in (rho, x, y, E) order),
+//                   weighted; the caller sums the three Gauss points in order.
+template <bool YDIR>
+HTX_HD void fvh_flux_fids(const FvhArgs& a, int i, int j, long long fid[5]) {
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         if (YDIR)
@@ -482,6 +484,16 @@ HTX_HD void fvh_flux_gp(const FvhArgs& a, int i, int j, int m, const double* F, 
         else
             fid[k] = (long long)map_index(j - 2 + k, a.ny, a.bc) * (a.nx + 1) + (i + 1);
     }
+}
+
+HTX_HD void fvh_series_gp3(const QuadConsts& qc, const double* F, long long fstr, int series, const long long fid[5],
+                           double g[3]) {
+    const double* S = F + (long long)series * fstr;
+    weno3q_gp3(qc, S[fid[0]], S[fid[1]], S[fid[2]], S[fid[3]], S[fid[4]], g);
+}
+
+template <bool YDIR, int FLUX>
+HTX_HD void fvh_flux_at(const FvhArgs& a, int i, int j, int m, const double c[6][4], double out[12]) {
     double qa[4], qb[4];
     {
         const long long fs = a.fstride;
@@ -495,15 +507,7 @@ HTX_HD void fvh_flux_gp(const FvhArgs& a, int i, int j, int m, const double* F, 
     Eig E = roe_eig(a.g1, YDIR, cell_prim(a.g1, qa[0], qa[1], qa[2], qa[3]), cell_prim(a.g1, qb[0], qb[1], qb[2], qb[3]));
     double Q[6][4];  // unrolled loops keep Q in registers
 #pragma unroll
-    for (int sr = 0; sr < 6; ++sr) {
-        double c[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const double* S = F + (long long)(sr * 4 + e) * fstr;
-            c[e] = weno3q_gp(a.qc, m, S[fid[0]], S[fid[1]], S[fid[2]], S[fid[3]], S[fid[4]]);
-        }
-        eig_right(E, c[0], c[1], c[2], c[3], Q[sr][0], Q[sr][1], Q[sr][2], Q[sr][3]);
-    }
+    for (int sr = 0; sr < 6; ++sr) eig_right(E, c[sr][0], c[sr][1], c[sr][2], c[sr][3], Q[sr][0], Q[sr][1], Q[sr][2], Q[sr][3]);
     // primitive states at the Gauss point and maxLocalWaveSpeed (Equations_Euler2D.cpp:221-246)
     double fl0, fln, flt, fl3, wnl, pl, rl, fr0, frn, frt, fr3, wnr, pr, rr;
     phys_flux(a.g1, Q[0][0], Q[0][1], Q[0][2], Q[0][3], fl0, fln, flt, fl3, wnl, pl, rl);
@@ -539,6 +543,73 @@ HTX_HD void fvh_flux_gp(const FvhArgs& a, int i, int j, int m, const double* F, 
     put(0, fu);
     put(1, fv);
     put(2, fw);
+}
+
+// All of one face on one work item (the CPU form): out[m] = weighted fluxes
+// at Gauss point m.
+template <bool YDIR, int FLUX>
+HTX_HD void fvh_flux_face(const FvhArgs& a, int i, int j, const double* F, long long fstr, double out[3][12]) {
+    long long fid[5];
+    fvh_flux_fids<YDIR>(a, i, j, fid);
+    double c[3][6][4];
+    for (int s = 0; s < 24; ++s) {
+        double g[3];
+        fvh_series_gp3(a.qc, F, fstr, s, fid, g);
+        for (int m = 0; m < 3; ++m) c[m][s / 4][s % 4] = g[m];
+    }
+    for (int m = 0; m < 3; ++m) fvh_flux_at<YDIR, FLUX>(a, i, j, m, c[m], out[m]);
+}
+
+// One workgroup of 64 faces x 3 Gauss points (threadIdx.x face t0 + x,
+// threadIdx.y = m), faces t < nf of the x (YDIR false) or y family.  The
+// thread of Gauss point m forms the three Gauss-point values of series
+// 8m .. 8m+7 (smoothness shared by the three points) into LDS; after a
+// barrier each thread takes its point's 24 values, computes the fluxes, and
+// the three points are summed in order through the same LDS.
+//   lds: 24 x 3 x 64 doubles (series values), then 3 x 12 x 64 (fluxes)
+template <bool YDIR, int FLUX>
+__device__ void fvh_flux_block(const FvhArgs& a, long long t0, long long nf, double* lds) {
+    const int x = threadIdx.x, m = threadIdx.y;
+    const long long t = t0 + x;
+    const bool valid = t < nf;
+    int i = 0, j = 0;
+    if (YDIR) {
+        i = (int)(t % a.nx);
+        j = (int)(t / a.nx) - 1;
+    } else {
+        i = (int)(t % (a.nx + 1)) - 1;
+        j = (int)(t / (a.nx + 1));
+    }
+    const double* F = YDIR ? a.fy : a.fx;
+    if (valid) {
+        long long fid[5];
+        fvh_flux_fids<YDIR>(a, i, j, fid);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int s = 8 * m + q;
+            double g[3];
+            fvh_series_gp3(a.qc, F, nf, s, fid, g);
+#pragma unroll
+            for (int p = 0; p < 3; ++p) lds[(s * 3 + p) * 64 + x] = g[p];
+        }
+    }
+    __syncthreads();
+    double c[6][4];
+#pragma unroll
+    for (int s = 0; s < 24; ++s) c[s / 4][s % 4] = valid ? lds[(s * 3 + m) * 64 + x] : 0.0;
+    __syncthreads();
+    if (valid) {
+        double o[12];
+        fvh_flux_at<YDIR, FLUX>(a, i, j, m, c, o);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) lds[(m * 12 + k) * 64 + x] = o[k];
+    }
+    __syncthreads();
+    if (valid && m == 0) {
+        double* G = YDIR ? a.gy : a.gx;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) G[k * nf + t] = (lds[k * 64 + x] + lds[(12 + k) * 64 + x]) + lds[(24 + k) * 64 + x];
+    }
 }
 
 // divergence of the 12 fluxes (FVHermite_System2D.cpp:710-730), SSP-RK3

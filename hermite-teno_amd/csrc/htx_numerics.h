@@ -442,9 +442,11 @@ HTX_HD void weno3q_linear(const QuadConsts& C, double u0, double u1, double u2, 
     o2 = C.lq[2][0] * u0 + C.lq[2][1] * u1 + C.lq[2][2] * u2 + C.lq[2][3] * u3 + C.lq[2][4] * u4;
 }
 
-// One Gauss point of weno3q (m = 0: +xi, 1: centre, 2: -xi), evaluated with
-// exactly the operations weno3q uses for that output.
-HTX_HD double weno3q_gp(const QuadConsts& C, int m, double u0, double u1, double u2, double u3, double u4) {
+// The three Gauss points of weno3q (o[0]: +xi, o[1]: centre, o[2]: -xi),
+// each evaluated with exactly the operations weno3q uses for that output;
+// the smoothness indicators (and in the fast build their reciprocals) are
+// formed once for the three.
+HTX_HD void weno3q_gp3(const QuadConsts& C, double u0, double u1, double u2, double u3, double u4, double o[3]) {
     const double c1 = 13.0 / 12.0;
     double bg0 = u0 - 4.0 * u1 + 3.0 * u2, cg0 = u0 - 2.0 * u1 + u2;
     double bg1 = u1 - u3, cg1 = u1 - 2.0 * u2 + u3;
@@ -454,8 +456,12 @@ HTX_HD double weno3q_gp(const QuadConsts& C, int m, double u0, double u1, double
     double s2 = 0.25 * (bg2 * bg2) + c1 * (cg2 * cg2);
     double e0 = 1e-6 + s0, e1 = 1e-6 + s1, e2 = 1e-6 + s2;
     e0 = e0 * e0; e1 = e1 * e1; e2 = e2 * e2;
-    double w0, w1, w2, ws, p0, p1, p2, o;
-    if (m == 1) {
+#if !HTX_EXACT_DIVISION
+    const double r0 = 1.0 / e0, r1 = 1.0 / e1, r2 = 1.0 / e2;
+#endif
+    double w0, w1, w2, ws, p0, p1, p2, v;
+    // centre
+    {
 #if HTX_EXACT_DIVISION
         p0 = (-u0 + 2.0 * u1 + 23.0 * u2) / 24.0;
         p1 = (-u1 + 26.0 * u2 - u3) / 24.0;
@@ -463,25 +469,27 @@ HTX_HD double weno3q_gp(const QuadConsts& C, int m, double u0, double u1, double
         w0 = C.linw0p[0] / e0; w1 = C.linw0p[1] / e1; w2 = C.linw0p[2] / e2;
         ws = (w0 + w1) + w2;
         w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
-        o = C.sumw0p * ((p0 * w0 + p1 * w1) + p2 * w2);
+        v = C.sumw0p * ((p0 * w0 + p1 * w1) + p2 * w2);
         w0 = C.linw0m[0] / e0; w1 = C.linw0m[1] / e1; w2 = C.linw0m[2] / e2;
         ws = (w0 + w1) + w2;
         w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
-        o -= C.sumw0m * ((p0 * w0 + p1 * w1) + p2 * w2);
+        v -= C.sumw0m * ((p0 * w0 + p1 * w1) + p2 * w2);
 #else
-        double r0 = 1.0 / e0, r1 = 1.0 / e1, r2 = 1.0 / e2;
         const double r24 = 1.0 / 24.0;
         p0 = (-u0 + 2.0 * u1 + 23.0 * u2) * r24;
         p1 = (-u1 + 26.0 * u2 - u3) * r24;
         p2 = (23.0 * u2 + 2.0 * u3 - u4) * r24;
         w0 = C.linw0p[0] * r0; w1 = C.linw0p[1] * r1; w2 = C.linw0p[2] * r2;
         ws = (w0 + w1) + w2;
-        o = C.sumw0p * (((p0 * w0 + p1 * w1) + p2 * w2) / ws);
+        v = C.sumw0p * (((p0 * w0 + p1 * w1) + p2 * w2) / ws);
         w0 = C.linw0m[0] * r0; w1 = C.linw0m[1] * r1; w2 = C.linw0m[2] * r2;
         ws = (w0 + w1) + w2;
-        o -= C.sumw0m * (((p0 * w0 + p1 * w1) + p2 * w2) / ws);
+        v -= C.sumw0m * (((p0 * w0 + p1 * w1) + p2 * w2) / ws);
 #endif
-    } else {
+        o[1] = v != v ? 1e-15 : v;
+    }
+#pragma unroll
+    for (int m = 0; m <= 2; m += 2) {
         const double(*cc)[3] = m == 0 ? C.gp : C.gm;
         const double* lw = m == 0 ? C.linwp : C.linwm;
         p0 = (cc[0][0] * u0 + cc[1][0] * u1) + cc[2][0] * u2;
@@ -491,14 +499,14 @@ HTX_HD double weno3q_gp(const QuadConsts& C, int m, double u0, double u1, double
         w0 = lw[0] / e0; w1 = lw[1] / e1; w2 = lw[2] / e2;
         ws = (w0 + w1) + w2;
         w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
-        o = (p0 * w0 + p1 * w1) + p2 * w2;
+        v = (p0 * w0 + p1 * w1) + p2 * w2;
 #else
-        w0 = lw[0] * (1.0 / e0); w1 = lw[1] * (1.0 / e1); w2 = lw[2] * (1.0 / e2);
+        w0 = lw[0] * r0; w1 = lw[1] * r1; w2 = lw[2] * r2;
         ws = (w0 + w1) + w2;
-        o = ((p0 * w0 + p1 * w1) + p2 * w2) / ws;
+        v = ((p0 * w0 + p1 * w1) + p2 * w2) / ws;
 #endif
+        o[m] = v != v ? 1e-15 : v;
     }
-    return o != o ? 1e-15 : o;
 }
 
 // Per-cell quantities of calcPrimitiveVariable (FV_System2D.cpp:226-263).

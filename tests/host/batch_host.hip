@@ -128,10 +128,8 @@ void herm_launch(const HermTables& H, int ninst, int mode) {
                 long long t = lb * 64ll + tid;
                 if (t >= nf) continue;
                 double o[3][12];
-                for (int m = 0; m < 3; ++m) {
-                    if (y) fvh_flux_gp<true, FLUX>(a, (int)(t % a.nx), (int)(t / a.nx) - 1, m, a.fy, nf, o[m]);
-                    else fvh_flux_gp<false, FLUX>(a, (int)(t % (a.nx + 1)) - 1, (int)(t / (a.nx + 1)), m, a.fx, nf, o[m]);
-                }
+                if (y) fvh_flux_face<true, FLUX>(a, (int)(t % a.nx), (int)(t / a.nx) - 1, a.fy, nf, o);
+                else fvh_flux_face<false, FLUX>(a, (int)(t % (a.nx + 1)) - 1, (int)(t / (a.nx + 1)), a.fx, nf, o);
                 double* G = y ? a.gy : a.gx;
                 for (int c = 0; c < 12; ++c) G[c * nf + t] = (o[0][c] + o[1][c]) + o[2][c];
             }

@@ -50,20 +50,16 @@ extern "C" int fvhh_rhs(int nx, int ny, double xl, double xr, double yb, double 
     double o[3][12];
     for (int j = 0; j < ny; ++j)
         for (int fi = 0; fi <= nx; ++fi) {
-            for (int m = 0; m < 3; ++m) {
-                if (fk == 0) fvh_flux_gp<false, 0>(a, fi - 1, j, m, a.fx, fxs, o[m]);
-                else if (fk == 1) fvh_flux_gp<false, 1>(a, fi - 1, j, m, a.fx, fxs, o[m]);
-                else fvh_flux_gp<false, 2>(a, fi - 1, j, m, a.fx, fxs, o[m]);
-            }
+            if (fk == 0) fvh_flux_face<false, 0>(a, fi - 1, j, a.fx, fxs, o);
+            else if (fk == 1) fvh_flux_face<false, 1>(a, fi - 1, j, a.fx, fxs, o);
+            else fvh_flux_face<false, 2>(a, fi - 1, j, a.fx, fxs, o);
             for (int c = 0; c < 12; ++c) a.gx[c * fxs + (long long)j * (nx + 1) + fi] = (o[0][c] + o[1][c]) + o[2][c];
         }
     for (int fj = 0; fj <= ny; ++fj)
         for (int i = 0; i < nx; ++i) {
-            for (int m = 0; m < 3; ++m) {
-                if (fk == 0) fvh_flux_gp<true, 0>(a, i, fj - 1, m, a.fy, fys, o[m]);
-                else if (fk == 1) fvh_flux_gp<true, 1>(a, i, fj - 1, m, a.fy, fys, o[m]);
-                else fvh_flux_gp<true, 2>(a, i, fj - 1, m, a.fy, fys, o[m]);
-            }
+            if (fk == 0) fvh_flux_face<true, 0>(a, i, fj - 1, a.fy, fys, o);
+            else if (fk == 1) fvh_flux_face<true, 1>(a, i, fj - 1, a.fy, fys, o);
+            else fvh_flux_face<true, 2>(a, i, fj - 1, a.fy, fys, o);
             for (int c = 0; c < 12; ++c) a.gy[c * fys + (long long)fj * nx + i] = (o[0][c] + o[1][c]) + o[2][c];
         }
     double smx = 0, smy = 0;
