@@ -328,6 +328,7 @@ def main():
                          ("teno5_hllc_fp32_mixed", dict(flux=args.flux, precision="fp32_mixed")),
                          ("teno5_hllc_learned", dict(flux=args.flux, precision="learned")),
                          ("hteno_hllc_fp64", dict(flux=args.flux, scheme="hteno")),
+                         ("hteno_hllc_fp64_fast", dict(flux=args.flux, scheme="hteno", precision="fp64_fast")),
                          ("teno5_rusanov_fp64", dict(flux="rusanov"))]:
             try:
                 log(f"[gpu] variant {name}")
