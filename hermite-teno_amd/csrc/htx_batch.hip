@@ -66,18 +66,11 @@ __device__ __forceinline__ const FvhArgs& herm_args(const BatchHermArgs& B, int 
     return B.args[(B.mode - 1) * B.ninst + b];
 }
 
-__global__ void __launch_bounds__(256) kb_fvh_limit(BatchHermArgs B) {
+__global__ void __launch_bounds__(LIM_THREADS) kb_fvh_limit(BatchHermArgs B) {
     int lb;
     const FvhArgs& a = herm_args(B, HK_LIMIT, lb);
     if (!a.ctl->active) return;
-    int t = lb * 256 + threadIdx.x;
-    int nxl = 4 * a.ny, nyl = 4 * a.nx;
-    if (t < nxl) {
-        fvh_limit_line(a, 0, t % a.ny, t / a.ny);
-    } else if (t < nxl + nyl) {
-        t -= nxl;
-        fvh_limit_line(a, 1, t % a.nx, t / a.nx);
-    }
+    fvh_limit_block(a, lb, (4 * a.ny + 63) / 64);
 }
 
 template <bool YDIR>
@@ -177,7 +170,7 @@ hipError_t launch_teno(int shape, const BatchTenoArgs& B, int flux, int prec, in
 }
 
 hipError_t launch_herm(const BatchHermArgs& B, const int nblocks[6], int flux, hipStream_t s) {
-    hipLaunchKernelGGL(kb_fvh_limit, dim3(nblocks[HK_LIMIT]), dim3(256), 0, s, B);
+    hipLaunchKernelGGL(kb_fvh_limit, dim3(nblocks[HK_LIMIT]), dim3(LIM_THREADS), 0, s, B);
     hipLaunchKernelGGL(kb_fvh_recon<false>, dim3(nblocks[HK_RECON_X]), dim3(256), 0, s, B);
     hipLaunchKernelGGL(kb_fvh_recon<true>, dim3(nblocks[HK_RECON_Y]), dim3(256), 0, s, B);
     dim3 fb(64, 3);

@@ -14,8 +14,9 @@ namespace HTX_NS {
 
 // kinds of batched Hermite launches
 constexpr int HK_LIMIT = 0, HK_RECON_X = 1, HK_RECON_Y = 2, HK_FLUX_X = 3, HK_FLUX_Y = 4, HK_UPDATE = 5;
-// work items per workgroup of each kind
-constexpr int HK_PER_BLOCK[6] = {256, 256, 256, 64, 64, 256};
+// work items per workgroup of each kind (limiter: 64 chains, row and column
+// chains in separate workgroups, lim_blocks)
+constexpr int HK_PER_BLOCK[6] = {64, 256, 256, 64, 64, 256};
 
 struct BatchLayout {
     std::vector<htx_params> p;
@@ -136,7 +137,8 @@ inline HermTables make_herm_tables(const BatchLayout& L, double* state, Ctl* ctl
         long long items[6] = {4ll * (p.nx + p.ny), nxf, nyf, nxf, nyf, n};
         for (int kind = 0; kind < 6; ++kind) {
             H.maps[6 + kind].push_back((int)H.maps[kind].size());
-            long long nb = (items[kind] + HK_PER_BLOCK[kind] - 1) / HK_PER_BLOCK[kind];
+            long long nb = kind == HK_LIMIT ? lim_blocks(p.nx, p.ny)
+                                            : (items[kind] + HK_PER_BLOCK[kind] - 1) / HK_PER_BLOCK[kind];
             for (long long t = 0; t < nb; ++t) H.maps[kind].push_back(k);
         }
         double* S = scratch + so;

@@ -364,6 +364,128 @@ HTX_HD void fvh_limit_line(const FvhArgs& a, int dir, int line, int e) {
     }
 }
 
+// The limiter: 64 chains per workgroup, one consumer and LIM_NP producer
+// wavefronts.  A chain step's cost is the latency of its dependent
+// arithmetic, and one wavefront per SIMD runs it, so everything that does not
+// depend on the chain moves off the consumer: the producers load u and the
+// old moments, compute herm_limiter_pre for the next chunk of LIM_B steps of
+// the 64 chains into LDS (producer p takes the steps s = p mod LIM_NP) and
+// store the chunk the consumer finished before, while the consumer runs
+// herm_limiter_step over the current chunk.  The consumer touches LDS only (a
+// global store would make a later step wait for the store's completion
+// before reusing its registers).  Two LDS buffers alternate, one barrier per
+// chunk, and the producers' loads run one chunk ahead of their arithmetic.
+//   LDS: pre[buffer][slot][step][chain], 2 x LIM_NS x LIM_B x 64 doubles
+//        (15 slots strict, 8 fast), res[buffer][step][chain], 2 x LIM_B x 64
+//   blocks: blk < g0 = ceil(4 ny / 64) take the row chains (v), the next
+//           ceil(4 nx / 64) the column chains (w); LIM_THREADS threads each
+constexpr int LIM_B = HTX_EXACT_DIVISION ? 8 : 16, LIM_NP = 8, LIM_SP = LIM_B / LIM_NP;
+constexpr int LIM_THREADS = 64 * (1 + LIM_NP);
+static_assert(LIM_B % LIM_NP == 0, "every producer takes the same number of steps");
+
+// workgroups of one problem's limiter launch
+HTX_HD int lim_blocks(int nx, int ny) { return (4 * ny + 63) / 64 + (4 * nx + 63) / 64; }
+
+__device__ inline void fvh_limit_block(const FvhArgs& a, int blk, int g0) {
+    __shared__ double pre[2][LIM_NS][LIM_B][64];
+    __shared__ double res[2][LIM_B][64];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int pw = wave - 1;  // producer index
+    const int c = threadIdx.x & 63;
+    const int dir = blk < g0 ? 0 : 1;
+    const int grp = dir == 0 ? blk : blk - g0;
+    const int nl = dir == 0 ? a.ny : a.nx;  // lines per field
+    const int n = dir == 0 ? a.nx : a.ny;   // chain length
+    const int t = grp * 64 + c;
+    const bool live = t < 4 * nl;
+    const int tt = live ? t : 4 * nl - 1;  // idle lanes shadow the last chain and store nothing
+    const int line = tt % nl, e = tt / nl;
+    const long long fs = a.fstride;
+    const double* U = a.q + e * fs;
+    const double* D = a.q + (dir == 0 ? 4 + e : 8 + e) * fs;
+    double* O = a.lv_out + (dir == 0 ? e : 4 + e) * fs;
+    const long long sk = dir == 0 ? 1 : a.pitch;
+    const long long base = dir == 0 ? (long long)line * a.pitch : line;
+    // k is uniform over the workgroup; indices off the line map through the
+    // boundary condition
+    auto off = [&](int k) { return base + (long long)((k >= 0 && k < n) ? k : map_index(k, n, a.bc)) * sk; };
+    const double dif = a.hc.dif;
+    const int nch = (n + LIM_B - 1) / LIM_B;
+
+    // producer: u_{k-1}, u_k, u_{k+1} and the old moment_{k+1} of its steps
+    // k = chunk LIM_B + pw + LIM_NP j of the chunk it computes next
+    double r0[LIM_SP], r1[LIM_SP], r2[LIM_SP], rd[LIM_SP];
+    auto fetch = [&](int chunk) {
+#pragma unroll
+        for (int j = 0; j < LIM_SP; ++j) {
+            const int k = chunk * LIM_B + pw + LIM_NP * j;
+            r0[j] = U[off(k - 1)];
+            r1[j] = U[off(k)];
+            r2[j] = U[off(k + 1)];
+            rd[j] = D[off(k + 1)];
+        }
+    };
+    auto produce = [&](int chunk) {
+        const int buf = chunk & 1;
+#pragma unroll
+        for (int j = 0; j < LIM_SP; ++j) {
+            double o[LIM_NV] = {};
+            herm_limiter_pre(dif, r0[j], r1[j], r2[j], rd[j], o);
+#pragma unroll
+            for (int v = 0; v < LIM_NV; ++v)
+                if (lim_slot_pos(v) >= 0) pre[buf][lim_slot_pos(v)][pw + LIM_NP * j][c] = o[v];
+        }
+    };
+    auto store = [&](int chunk) {  // the consumer's results of a finished chunk
+        const int buf = chunk & 1;
+#pragma unroll
+        for (int j = 0; j < LIM_SP; ++j) {
+            const int s = pw + LIM_NP * j, k = chunk * LIM_B + s;
+            if (k < n && live) O[off(k)] = res[buf][s][c];
+        }
+    };
+    double prev = 0.0;
+    if (wave > 0) {
+        fetch(0);
+        produce(0);
+        if (nch > 1) fetch(1);
+    } else {
+        prev = D[off(-1)];  // ghost: pre-limiting value
+    }
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+        if (wave == 0) {
+            const int buf = ch & 1;
+            // the values of step s + 1 are read while step s computes
+            double o[LIM_NV], on[LIM_NV];
+#pragma unroll
+            for (int v = 0; v < LIM_NV; ++v) o[v] = lim_slot_pos(v) >= 0 ? pre[buf][lim_slot_pos(v)][0][c] : 0.0;
+#pragma unroll
+            for (int s = 0; s < LIM_B; ++s) {
+                if (s + 1 < LIM_B) {
+#pragma unroll
+                    for (int v = 0; v < LIM_NV; ++v)
+                        on[v] = lim_slot_pos(v) >= 0 ? pre[buf][lim_slot_pos(v)][s + 1][c] : 0.0;
+                }
+                if (ch * LIM_B + s < n) {
+                    prev = herm_limiter_step(dif, o, prev);
+                    res[buf][s][c] = prev;
+                }
+#pragma unroll
+                for (int v = 0; v < LIM_NV; ++v) o[v] = on[v];
+            }
+        } else {
+            if (ch + 1 < nch) {
+                produce(ch + 1);
+                if (ch + 2 < nch) fetch(ch + 2);
+            }
+            if (ch > 0) store(ch - 1);
+        }
+        __syncthreads();
+    }
+    if (wave > 0) store(nch - 1);
+}
+
 // limited moment of the state at (i, j): the limited interior value at the
 // boundary-mapped cell (the reference re-applies the boundary condition
 // after limiting, FVHermite_System2D.cpp:437)

@@ -101,10 +101,12 @@ void herm_launch(const HermTables& H, int ninst, int mode) {
     for (int blk = 0; blk < (int)H.maps[HK_LIMIT].size(); ++blk) {
         const FvhArgs& a = args(HK_LIMIT, blk, lb);
         if (!a.ctl->active) continue;
-        for (int tid = 0; tid < 256; ++tid) {
-            int t = lb * 256 + tid, nxl = 4 * a.ny, nyl = 4 * a.nx;
-            if (t < nxl) fvh_limit_line(a, 0, t % a.ny, t / a.ny);
-            else if (t < nxl + nyl) fvh_limit_line(a, 1, (t - nxl) % a.nx, (t - nxl) / a.nx);
+        // fvh_limit_block: block lb < g0 takes row chains, the rest column chains
+        const int g0 = (4 * a.ny + 63) / 64, dir = lb < g0 ? 0 : 1, grp = dir == 0 ? lb : lb - g0;
+        const int nl = dir == 0 ? a.ny : a.nx;
+        for (int c = 0; c < 64; ++c) {
+            int t = grp * 64 + c;
+            if (t < 4 * nl) fvh_limit_line(a, dir, t % nl, t / nl);
         }
     }
     for (int y = 0; y < 2; ++y)
