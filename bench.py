@@ -129,9 +129,9 @@ def precision_study(n=2048):
 
 def batch_study(count=256, steps=20, t_compare=0.02):
     """Serving-style sweep: `count` seeded quadrant problems with sides
-    log-uniform in [64, 1024], half TENO5 + HLLC in the FP32-mixed precision
-    (FP32 cut-off decisions and nonlinear weights, FP64 values, fluxes and RK
-    accumulation), half Hermite-TENO + Rusanov (FP64 fast), advanced together; aggregate
+    log-uniform in [64, 1024], half TENO5 + HLLC, half Hermite-TENO + Rusanov,
+    all in the FP32-mixed precision (FP32 cut-off decisions and nonlinear
+    weights, FP64 values, fluxes and RK accumulation), advanced together; aggregate
     cell-updates/s over `steps` steps from t = 0 (every instance active;
     graph replay, autotuned tile shape) and each instance's relative L1
     density difference at t = t_compare against the plain FP64 drop-in solver
@@ -147,8 +147,7 @@ def batch_study(count=256, steps=20, t_compare=0.02):
 
     def make(tmax):
         ps = [params_for(w["xl"], w["xr"], w["nx"], w["yb"], w["yt"], w["ny"], w["cfl"], tmax, w["gamma"],
-                         scheme=w["scheme"], flux=w["flux"],
-                         precision="fp32_mixed" if w["scheme"] == "teno5" else "fp64_fast") for w in ws]
+                         scheme=w["scheme"], flux=w["flux"], precision="fp32_mixed") for w in ws]
         bb = Batch(ps)
         for i, w in enumerate(ws):
             bb.quadrant(i, w["xs"], w["ys"], w["states"])
@@ -158,8 +157,8 @@ def batch_study(count=256, steps=20, t_compare=0.02):
     cells = sum(w["nx"] * w["ny"] for w in ws)
     out = {"instances": count, "cells": cells,
            "teno5_instances": sum(w["scheme"] == "teno5" for w in ws),
-           "workload": f"{count} seeded quadrant problems, sides log-uniform in [64, 1024], TENO5+HLLC (fp32_mixed) "
-                       f"or Hermite-TENO+Rusanov (fp64_fast) with probability 1/2, t_end 0.2"}
+           "workload": f"{count} seeded quadrant problems, sides log-uniform in [64, 1024], TENO5+HLLC "
+                       f"or Hermite-TENO+Rusanov with probability 1/2, fp32_mixed, t_end 0.2"}
     log("[batch] autotuning the TENO5 tile shape")
     shape, ms = b.tune(steps=2)
     out["tile"] = f"{shape[0]}x{shape[1]}"
@@ -329,6 +328,7 @@ def main():
                          ("teno5_hllc_learned", dict(flux=args.flux, precision="learned")),
                          ("hteno_hllc_fp64", dict(flux=args.flux, scheme="hteno")),
                          ("hteno_hllc_fp64_fast", dict(flux=args.flux, scheme="hteno", precision="fp64_fast")),
+                         ("hteno_hllc_fp32_mixed", dict(flux=args.flux, scheme="hteno", precision="fp32_mixed")),
                          ("teno5_rusanov_fp64", dict(flux="rusanov"))]:
             try:
                 log(f"[gpu] variant {name}")

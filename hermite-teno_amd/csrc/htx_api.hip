@@ -104,8 +104,8 @@ int check_params(const htx_params* p) {
     if (p->precision != HTX_PREC_FP64 && p->precision != HTX_PREC_FP32_MIXED && p->precision != HTX_PREC_LEARNED &&
         p->precision != HTX_PREC_FP64_FAST)
         return fail(HTX_EINVAL, "unknown precision");
-    if ((p->precision == HTX_PREC_FP32_MIXED || p->precision == HTX_PREC_LEARNED) && p->scheme != HTX_SCHEME_TENO5)
-        return fail(HTX_ENOTSUP, "FP32-mixed and learned precision are built for the TENO5 scheme only");
+    if (p->precision == HTX_PREC_LEARNED && p->scheme != HTX_SCHEME_TENO5)
+        return fail(HTX_ENOTSUP, "the learned precision is built for the TENO5 scheme only");
     return HTX_OK;
 }
 
@@ -160,6 +160,7 @@ FvhArgs hermite_args(htx_solver* h, const double* qin, const double* q0, double*
     a.mode = mode;
     a.quirks = h->p.ref_quirks ? 1 : 0;
     a.flux = h->p.flux;
+    a.mixed = h->p.precision == HTX_PREC_FP32_MIXED;
     a.gam = h->p.gamma;
     a.g1 = h->p.gamma - 1.0;
     a.dx = h->dx;

@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(LIM_THREADS) kb_fvh_limit(BatchHermArgs B) {
     fvh_limit_block(a, lb, (4 * a.ny + 63) / 64);
 }
 
-template <bool YDIR>
+template <bool YDIR, bool MIXED>
 __global__ void __launch_bounds__(256) kb_fvh_recon(BatchHermArgs B) {
     int lb;
     const FvhArgs& a = herm_args(B, YDIR ? HK_RECON_Y : HK_RECON_X, lb);
@@ -83,21 +83,21 @@ __global__ void __launch_bounds__(256) kb_fvh_recon(BatchHermArgs B) {
     if (t >= nf) return;
     if (YDIR) {
         int i = (int)(t % a.nx), fj = (int)(t / a.nx);
-        fvh_recon_face<true>(a, i, fj - 1, a.fy, t, nf);
+        fvh_recon_face<true, MIXED>(a, i, fj - 1, a.fy, t, nf);
     } else {
         int fi = (int)(t % (a.nx + 1)), j = (int)(t / (a.nx + 1));
-        fvh_recon_face<false>(a, fi - 1, j, a.fx, t, nf);
+        fvh_recon_face<false, MIXED>(a, fi - 1, j, a.fx, t, nf);
     }
 }
 
-template <bool YDIR, int FLUX>
+template <bool YDIR, int FLUX, bool MIXED>
 __global__ void __launch_bounds__(192) kb_fvh_flux(BatchHermArgs B) {
     __shared__ double lds[24 * 3 * 64];
     int lb;
     const FvhArgs& a = herm_args(B, YDIR ? HK_FLUX_Y : HK_FLUX_X, lb);
     if (!a.ctl->active) return;
     long long nf = YDIR ? (long long)a.nx * (a.ny + 1) : (long long)(a.nx + 1) * a.ny;
-    fvh_flux_block<YDIR, FLUX>(a, lb * 64ll, nf, lds);
+    fvh_flux_block<YDIR, FLUX, MIXED>(a, lb * 64ll, nf, lds);
 }
 
 __global__ void __launch_bounds__(256) kb_fvh_update(BatchHermArgs B) {
@@ -169,21 +169,30 @@ hipError_t launch_teno(int shape, const BatchTenoArgs& B, int flux, int prec, in
     }
 }
 
-hipError_t launch_herm(const BatchHermArgs& B, const int nblocks[6], int flux, hipStream_t s) {
-    hipLaunchKernelGGL(kb_fvh_limit, dim3(nblocks[HK_LIMIT]), dim3(LIM_THREADS), 0, s, B);
-    hipLaunchKernelGGL(kb_fvh_recon<false>, dim3(nblocks[HK_RECON_X]), dim3(256), 0, s, B);
-    hipLaunchKernelGGL(kb_fvh_recon<true>, dim3(nblocks[HK_RECON_Y]), dim3(256), 0, s, B);
-    dim3 fb(64, 3);
+template <bool MIXED>
+void launch_herm_faces(const BatchHermArgs& B, const int nblocks[6], int flux, hipStream_t s) {
+    hipLaunchKernelGGL((kb_fvh_recon<false, MIXED>), dim3(nblocks[HK_RECON_X]), dim3(256), 0, s, B);
+    hipLaunchKernelGGL((kb_fvh_recon<true, MIXED>), dim3(nblocks[HK_RECON_Y]), dim3(256), 0, s, B);
+    dim3 fb(64, 3), gx(nblocks[HK_FLUX_X]), gy(nblocks[HK_FLUX_Y]);
     if (flux == FLUX_RUSANOV_CHAR) {
-        hipLaunchKernelGGL((kb_fvh_flux<false, FLUX_RUSANOV_CHAR>), dim3(nblocks[HK_FLUX_X]), fb, 0, s, B);
-        hipLaunchKernelGGL((kb_fvh_flux<true, FLUX_RUSANOV_CHAR>), dim3(nblocks[HK_FLUX_Y]), fb, 0, s, B);
+        hipLaunchKernelGGL((kb_fvh_flux<false, FLUX_RUSANOV_CHAR, MIXED>), gx, fb, 0, s, B);
+        hipLaunchKernelGGL((kb_fvh_flux<true, FLUX_RUSANOV_CHAR, MIXED>), gy, fb, 0, s, B);
     } else if (flux == FLUX_HLLC) {
-        hipLaunchKernelGGL((kb_fvh_flux<false, FLUX_HLLC>), dim3(nblocks[HK_FLUX_X]), fb, 0, s, B);
-        hipLaunchKernelGGL((kb_fvh_flux<true, FLUX_HLLC>), dim3(nblocks[HK_FLUX_Y]), fb, 0, s, B);
+        hipLaunchKernelGGL((kb_fvh_flux<false, FLUX_HLLC, MIXED>), gx, fb, 0, s, B);
+        hipLaunchKernelGGL((kb_fvh_flux<true, FLUX_HLLC, MIXED>), gy, fb, 0, s, B);
     } else {
-        hipLaunchKernelGGL((kb_fvh_flux<false, FLUX_HLLC_TORO>), dim3(nblocks[HK_FLUX_X]), fb, 0, s, B);
-        hipLaunchKernelGGL((kb_fvh_flux<true, FLUX_HLLC_TORO>), dim3(nblocks[HK_FLUX_Y]), fb, 0, s, B);
+        hipLaunchKernelGGL((kb_fvh_flux<false, FLUX_HLLC_TORO, MIXED>), gx, fb, 0, s, B);
+        hipLaunchKernelGGL((kb_fvh_flux<true, FLUX_HLLC_TORO, MIXED>), gy, fb, 0, s, B);
     }
+}
+
+// mixed: the Hermite instances' shared precision is fp32_mixed
+hipError_t launch_herm(const BatchHermArgs& B, const int nblocks[6], int flux, bool mixed, hipStream_t s) {
+    hipLaunchKernelGGL(kb_fvh_limit, dim3(nblocks[HK_LIMIT]), dim3(LIM_THREADS), 0, s, B);
+    if (mixed)
+        launch_herm_faces<true>(B, nblocks, flux, s);
+    else
+        launch_herm_faces<false>(B, nblocks, flux, s);
     hipLaunchKernelGGL(kb_fvh_update, dim3(nblocks[HK_UPDATE]), dim3(256), 0, s, B);
     return hipGetLastError();
 }
@@ -210,6 +219,7 @@ struct htx_batch {
     int *dTileInst = nullptr, *dTile0 = nullptr;
     // Hermite launches
     int herm_flux = FLUX_RUSANOV_CHAR;
+    bool herm_mixed = false;                // Hermite instances in fp32_mixed
     FvhArgs* dHermArgs = nullptr;
     int* dHermMap = nullptr;
     int herm_blocks[6] = {0, 0, 0, 0, 0, 0};
@@ -333,7 +343,7 @@ int enqueue_batch_step(htx_batch* b, hipStream_t s) {
             }
             H.ninst = (int)b->herm.size();
             H.mode = mode;
-            BHIP(launch_herm(H, b->herm_blocks, b->herm_flux, s));
+            BHIP(launch_herm(H, b->herm_blocks, b->herm_flux, b->herm_mixed, s));
         }
     }
     hipLaunchKernelGGL(k_batch_tadvance, dim3((n + 255) / 256), dim3(256), 0, s, b->ctl, n);
@@ -389,7 +399,7 @@ int htx_batch_create(const htx_params* ps, int n, htx_batch** out) {
     if (!out) return htx_internal_fail(HTX_EINVAL, "out is null");
     *out = nullptr;
     if (!ps || n < 1) return htx_internal_fail(HTX_EINVAL, "need n >= 1 parameter blocks");
-    int teno_flux = -1, teno_prec = -1, herm_flux = -1;
+    int teno_flux = -1, teno_prec = -1, herm_flux = -1, herm_mixed = -1;
     for (int i = 0; i < n; ++i) {
         const htx_params& p = ps[i];
         if (p.struct_size != (int)sizeof(htx_params)) return htx_internal_fail(HTX_EINVAL, "params.struct_size mismatch");
@@ -410,12 +420,13 @@ int htx_batch_create(const htx_params* ps, int n, htx_batch** out) {
             teno_prec = pc;
             teno_flux = fc;
         } else if (p.scheme == HTX_SCHEME_HTENO) {
-            if (p.precision != HTX_PREC_FP64_FAST)
-                return htx_internal_fail(HTX_ENOTSUP, "batched Hermite instances run fp64_fast precision");
-            int fc = flux_code(p);
-            if (herm_flux >= 0 && fc != herm_flux)
-                return htx_internal_fail(HTX_ENOTSUP, "the Hermite instances of a batch share their flux");
+            if (p.precision != HTX_PREC_FP64_FAST && p.precision != HTX_PREC_FP32_MIXED)
+                return htx_internal_fail(HTX_ENOTSUP, "batched Hermite instances run fp64_fast or fp32_mixed precision");
+            int fc = flux_code(p), pm = p.precision == HTX_PREC_FP32_MIXED;
+            if ((herm_flux >= 0 && fc != herm_flux) || (herm_mixed >= 0 && pm != herm_mixed))
+                return htx_internal_fail(HTX_ENOTSUP, "the Hermite instances of a batch share flux and precision");
             herm_flux = fc;
+            herm_mixed = pm;
         } else {
             return htx_internal_fail(HTX_EINVAL, "unknown scheme");
         }
@@ -428,6 +439,7 @@ int htx_batch_create(const htx_params* ps, int n, htx_batch** out) {
     if (teno_prec >= 0) b->teno_prec = teno_prec;
     if (teno_flux >= 0) b->teno_flux = teno_flux;
     if (herm_flux >= 0) b->herm_flux = herm_flux;
+    b->herm_mixed = herm_mixed == 1;
     BatchLayout L = make_layout(ps, n);
     b->off = L.off;
     b->dxdy = L.dxdy;

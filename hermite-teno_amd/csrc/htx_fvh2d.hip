@@ -26,28 +26,30 @@ __device__ __forceinline__ double wave_max_fvh(double v) {
 // The limiter: fvh_limit_block (htx_hermite.h), 64 chains per workgroup.
 __global__ void __launch_bounds__(LIM_THREADS) k_fvh_limit(FvhArgs a, int g0) { fvh_limit_block(a, blockIdx.x, g0); }
 
+template <bool MIXED>
 __global__ void __launch_bounds__(256) k_fvh_recon_x(FvhArgs a) {
     long long nf = (long long)(a.nx + 1) * a.ny;
     long long t = blockIdx.x * 256ll + threadIdx.x;
     if (t >= nf) return;
     int fi = (int)(t % (a.nx + 1)), j = (int)(t / (a.nx + 1));
-    fvh_recon_face<false>(a, fi - 1, j, a.fx, t, nf);
+    fvh_recon_face<false, MIXED>(a, fi - 1, j, a.fx, t, nf);
 }
 
+template <bool MIXED>
 __global__ void __launch_bounds__(256) k_fvh_recon_y(FvhArgs a) {
     long long nf = (long long)a.nx * (a.ny + 1);
     long long t = blockIdx.x * 256ll + threadIdx.x;
     if (t >= nf) return;
     int i = (int)(t % a.nx), fj = (int)(t / a.nx);
-    fvh_recon_face<true>(a, i, fj - 1, a.fy, t, nf);
+    fvh_recon_face<true, MIXED>(a, i, fj - 1, a.fy, t, nf);
 }
 
 // block (64 faces) x (3 Gauss points)
-template <bool YDIR, int FLUX>
+template <bool YDIR, int FLUX, bool MIXED>
 __global__ void __launch_bounds__(192) k_fvh_flux(FvhArgs a) {
     __shared__ double lds[24 * 3 * 64];
     long long nf = YDIR ? (long long)a.nx * (a.ny + 1) : (long long)(a.nx + 1) * a.ny;
-    fvh_flux_block<YDIR, FLUX>(a, blockIdx.x * 64ll, nf, lds);
+    fvh_flux_block<YDIR, FLUX, MIXED>(a, blockIdx.x * 64ll, nf, lds);
 }
 
 __global__ void __launch_bounds__(256) k_fvh_update(FvhArgs a) {
@@ -145,22 +147,35 @@ __global__ void __launch_bounds__(256) k_ic_vortex_hermite(VortexArgs a) {
     for (int k = 0; k < 12; ++k) a.q[off + k * a.fstride] = acc[k];
 }
 
+template <bool MIXED>
+static void launch_recon_flux(const FvhArgs& a, int flux, hipStream_t s) {
+    long long nxf = (long long)(a.nx + 1) * a.ny, nyf = (long long)a.nx * (a.ny + 1);
+    hipLaunchKernelGGL(k_fvh_recon_x<MIXED>, dim3((unsigned)((nxf + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_fvh_recon_y<MIXED>, dim3((unsigned)((nyf + 255) / 256)), dim3(256), 0, s, a);
+    dim3 fb(64, 3), gx((unsigned)((nxf + 63) / 64)), gy((unsigned)((nyf + 63) / 64));
+    if (flux == FLUX_RUSANOV_CHAR) {
+        hipLaunchKernelGGL((k_fvh_flux<false, FLUX_RUSANOV_CHAR, MIXED>), gx, fb, 0, s, a);
+        hipLaunchKernelGGL((k_fvh_flux<true, FLUX_RUSANOV_CHAR, MIXED>), gy, fb, 0, s, a);
+    } else if (flux == FLUX_HLLC) {
+        hipLaunchKernelGGL((k_fvh_flux<false, FLUX_HLLC, MIXED>), gx, fb, 0, s, a);
+        hipLaunchKernelGGL((k_fvh_flux<true, FLUX_HLLC, MIXED>), gy, fb, 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_fvh_flux<false, FLUX_HLLC_TORO, MIXED>), gx, fb, 0, s, a);
+        hipLaunchKernelGGL((k_fvh_flux<true, FLUX_HLLC_TORO, MIXED>), gy, fb, 0, s, a);
+    }
+}
+
 hipError_t launch_fvh2d_stage(const FvhArgs& a, int flux, hipStream_t s) {
     hipLaunchKernelGGL(k_fvh_limit, dim3(lim_blocks(a.nx, a.ny)), dim3(LIM_THREADS), 0, s, a, (4 * a.ny + 63) / 64);
-    long long nxf = (long long)(a.nx + 1) * a.ny, nyf = (long long)a.nx * (a.ny + 1);
-    hipLaunchKernelGGL(k_fvh_recon_x, dim3((unsigned)((nxf + 255) / 256)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_fvh_recon_y, dim3((unsigned)((nyf + 255) / 256)), dim3(256), 0, s, a);
-    dim3 fb(64, 3);
-    if (flux == FLUX_RUSANOV_CHAR) {
-        hipLaunchKernelGGL((k_fvh_flux<false, FLUX_RUSANOV_CHAR>), dim3((unsigned)((nxf + 63) / 64)), fb, 0, s, a);
-        hipLaunchKernelGGL((k_fvh_flux<true, FLUX_RUSANOV_CHAR>), dim3((unsigned)((nyf + 63) / 64)), fb, 0, s, a);
-    } else if (flux == FLUX_HLLC) {
-        hipLaunchKernelGGL((k_fvh_flux<false, FLUX_HLLC>), dim3((unsigned)((nxf + 63) / 64)), fb, 0, s, a);
-        hipLaunchKernelGGL((k_fvh_flux<true, FLUX_HLLC>), dim3((unsigned)((nyf + 63) / 64)), fb, 0, s, a);
-    } else {
-        hipLaunchKernelGGL((k_fvh_flux<false, FLUX_HLLC_TORO>), dim3((unsigned)((nxf + 63) / 64)), fb, 0, s, a);
-        hipLaunchKernelGGL((k_fvh_flux<true, FLUX_HLLC_TORO>), dim3((unsigned)((nyf + 63) / 64)), fb, 0, s, a);
-    }
+#if HTX_EXACT_DIVISION
+    if (a.mixed) return hipErrorInvalidValue;  // the strict build is FP64 only
+    launch_recon_flux<false>(a, flux, s);
+#else
+    if (a.mixed)
+        launch_recon_flux<true>(a, flux, s);
+    else
+        launch_recon_flux<false>(a, flux, s);
+#endif
     long long n = (long long)a.nx * a.ny;
     hipLaunchKernelGGL(k_fvh_update, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();

@@ -78,6 +78,41 @@ HTX_HD double hteno_side(double brk, double s0, double s1, double s2, double p0,
     return out != out ? 1e-15 : out;
 }
 
+// Mixed-precision HTENO side: the indicators brk, s_k come in FP64 (formed
+// exactly as for hteno_side); tau, a_k = 1 + tau / (s_k + eps) and the cut-off
+// (C_T = 1e-3 on a_k^6 / sum_j a_j^6) are evaluated in FP32 on the ratios
+// r_k = a_k / max_j a_j as in teno5_side_mixed; the value is the FP64
+// combination of the selected linear weights.  The 1e-40 guard becomes 1e-30
+// (FP32's normal range); both act only on exactly flat stencils.
+HTX_HD double hteno_side_mixed(double brk, double s0, double s1, double s2, double p0, double p1, double p2,
+                               double d0, double d1, double d2) {
+    const float tau = (float)fabs(brk - 0.25 * (s0 + s1 + 2.0 * s2));
+    const float a0 = 1.0f + tau * frcp((float)s0 + 1e-30f);
+    const float a1 = 1.0f + tau * frcp((float)s1 + 1e-30f);
+    const float a2 = 1.0f + tau * frcp((float)s2 + 1e-30f);
+    const float am = fmaxf(fmaxf(a0, a1), a2);
+    bool k0, k1, k2;
+    if (am > 3.40282347e38f) {  // am is infinite (am >= 1; NaN fails the test)
+        k0 = a0 == am;
+        k1 = a1 == am;
+        k2 = a2 == am;
+    } else {
+        const float im = frcp(am);
+        float r0 = a0 * im, r1 = a1 * im, r2 = a2 * im;
+        r0 = r0 * r0 * r0; r0 = r0 * r0;
+        r1 = r1 * r1 * r1; r1 = r1 * r1;
+        r2 = r2 * r2 * r2; r2 = r2 * r2;
+        const float lim = 1e-3f * ((r0 + r1) + r2);
+        k0 = !(r0 < lim);
+        k1 = !(r1 < lim);
+        k2 = !(r2 < lim);
+    }
+    double w0 = k0 ? d0 : 0.0, w1 = k1 ? d1 : 0.0, w2 = k2 ? d2 : 0.0;
+    double ws = (w0 + w1) + w2;
+    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * (1.0 / ws);
+    return out != out ? 1e-15 : out;
+}
+
 // Hermite::smoothnessMeasure (:515-526) and smoothnessMeasureDerivative (:528-541)
 HTX_HD void herm_si(double u0, double u1, double u2, double v0, double v2, double& s0, double& s1, double& s2) {
     HTX_NO_CONTRACT
@@ -104,8 +139,14 @@ HTX_HD void herm_sid(double u0, double u1, double u2, double d0, double d1, doub
 
 // Hermite::hteno (Schemes_Hermite.cpp:295-379): cell averages u_{i-1..i+2} and
 // derivative moments d -> (u^-, u^+, d^-, d^+) at x_{i+1/2}.
+template <bool MIXED = false>
 HTX_HD void hteno(const HermConsts& H, const double u[4], const double d[4], double out[4]) {
     HTX_NO_CONTRACT
+    auto side = [](double brk, double s0, double s1, double s2, double p0, double p1, double p2, double d0, double d1,
+                   double d2) {
+        return MIXED ? hteno_side_mixed(brk, s0, s1, s2, p0, p1, p2, d0, d1, d2)
+                     : hteno_side(brk, s0, s1, s2, p0, p1, p2, d0, d1, d2);
+    };
     const double dif = H.dif;
     double ud0 = d[0] * dif, ud1 = d[1] * dif, ud2 = d[2] * dif, ud3 = d[3] * dif;
     double s0, s1, s2, p0, p1, p2;
@@ -114,26 +155,26 @@ HTX_HD void hteno(const HermConsts& H, const double u[4], const double d[4], dou
     p1 = H.crj[0][1] * u[1] + H.crj[1][1] * u[2] + H.crj[2][1] * d[2];
     p2 = H.crj[0][2] * u[0] + H.crj[1][2] * u[1] + H.crj[2][2] * u[2];
     herm_si(u[0], u[1], u[2], ud0, ud2, s0, s1, s2);
-    out[0] = hteno_side(herm_brk(u[0], u[1], u[2], ud0, ud2), s0, s1, s2, p0, p1, p2, H.dkm[0], H.dkm[1], H.dkm[2]);
+    out[0] = side(herm_brk(u[0], u[1], u[2], ud0, ud2), s0, s1, s2, p0, p1, p2, H.dkm[0], H.dkm[1], H.dkm[2]);
     // d^-
     p0 = H.crjd[0][0] * u[0] + H.crjd[1][0] * u[1] + H.crjd[2][0] * d[0] + H.crjd[3][0] * d[1];
     p1 = H.crjd[0][1] * u[1] + H.crjd[1][1] * u[2] + H.crjd[2][1] * d[1] + H.crjd[3][1] * d[2];
     p2 = H.crjd[0][2] * u[0] + H.crjd[1][2] * u[1] + H.crjd[2][2] * u[2] + H.crjd[3][2] * d[1];
     herm_sid(u[0], u[1], u[2], ud0, ud1, ud2, s0, s1, s2);
-    out[2] = hteno_side(herm_brkd(u[0], u[1], u[2], ud0, ud1, ud2), s0, s1, s2, p0, p1, p2, H.dkmd[0], H.dkmd[1],
+    out[2] = side(herm_brkd(u[0], u[1], u[2], ud0, ud1, ud2), s0, s1, s2, p0, p1, p2, H.dkmd[0], H.dkmd[1],
                         H.dkmd[2]);
     // u^+
     p0 = H.crj[1][1] * u[1] + H.crj[0][1] * u[2] - H.crj[2][1] * d[1];
     p1 = H.crj[1][0] * u[2] + H.crj[0][0] * u[3] - H.crj[2][0] * d[3];
     p2 = H.crj[2][2] * u[1] + H.crj[1][2] * u[2] + H.crj[0][2] * u[3];
     herm_si(u[1], u[2], u[3], ud1, ud3, s0, s1, s2);
-    out[1] = hteno_side(herm_brk(u[1], u[2], u[3], ud1, ud3), s0, s1, s2, p0, p1, p2, H.dkp[0], H.dkp[1], H.dkp[2]);
+    out[1] = side(herm_brk(u[1], u[2], u[3], ud1, ud3), s0, s1, s2, p0, p1, p2, H.dkp[0], H.dkp[1], H.dkp[2]);
     // d^+
     p0 = H.crjd[0][1] * u[1] + H.crjd[1][1] * u[2] + H.crjd[3][1] * d[1] + H.crjd[2][1] * d[2];
     p1 = H.crjd[0][0] * u[2] + H.crjd[1][0] * u[3] + H.crjd[3][0] * d[2] + H.crjd[2][0] * d[3];
     p2 = -(H.crjd[2][2] * u[1] + H.crjd[1][2] * u[2] + H.crjd[0][2] * u[3]) + H.crjd[3][2] * d[2];
     herm_sid(u[1], u[2], u[3], ud1, ud2, ud3, s0, s1, s2);
-    out[3] = hteno_side(herm_brkd(u[1], u[2], u[3], ud1, ud2, ud3), s0, s1, s2, p0, p1, p2, H.dkpd[0], H.dkpd[1],
+    out[3] = side(herm_brkd(u[1], u[2], u[3], ud1, ud2, ud3), s0, s1, s2, p0, p1, p2, H.dkpd[0], H.dkpd[1],
                         H.dkpd[2]);
 }
 
@@ -202,6 +243,7 @@ struct FvhArgs {
     int nx, ny, pitch;
     long long fstride;   // per field of the state
     int bc, mode, quirks, flux;
+    int mixed;           // 1: FP32 cut-off rules and transverse weights (fast build only)
     double gam, g1, dx, dy, area;
     QuadConsts qc;
     HermConsts hc;
@@ -493,8 +535,10 @@ HTX_HD double fvh_lim(const FvhArgs& a, int comp, int i, int j) { return a.lv[co
 
 // x or y face reconstruction (FVHermite_System2D.cpp:443-495) for the face
 // between cell (i, j) and its +1 neighbour in direction YDIR; writes the 24
-// characteristic face values at face index fid of the face buffer.
-template <bool YDIR>
+// characteristic face values at face index fid of the face buffer.  MIXED:
+// the HTENO and TENO5 cut-off rules in FP32 (hteno_side_mixed,
+// teno5_side_mixed), every value and projection in FP64.
+template <bool YDIR, bool MIXED = false>
 HTX_HD void fvh_recon_face(const FvhArgs& a, int i, int j, double* F, long long fid, long long fstr) {
     const long long fs = a.fstride;
     auto cellq = [&](int k, double q[4]) {  // k: offset along the normal
@@ -537,8 +581,11 @@ HTX_HD void fvh_recon_face(const FvhArgs& a, int i, int j, double* F, long long 
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         double o[4], um, up;
-        hteno(a.hc, A[e], B[e], o);
-        teno5(C[e], um, up);
+        hteno<MIXED>(a.hc, A[e], B[e], o);
+        if (MIXED)
+            teno5_mixed(C[e], um, up);
+        else
+            teno5(C[e], um, up);
         // series: 0 u^-, 1 u^+, 2 vn^-, 3 vn^+, 4 vt^-, 5 vt^+ (vn: normal moment)
         F[(0 * 4 + e) * fstr + fid] = o[0];
         F[(1 * 4 + e) * fstr + fid] = o[1];
@@ -608,10 +655,14 @@ HTX_HD void fvh_flux_fids(const FvhArgs& a, int i, int j, long long fid[5]) {
     }
 }
 
+template <bool MIXED = false>
 HTX_HD void fvh_series_gp3(const QuadConsts& qc, const double* F, long long fstr, int series, const long long fid[5],
                            double g[3]) {
     const double* S = F + (long long)series * fstr;
-    weno3q_gp3(qc, S[fid[0]], S[fid[1]], S[fid[2]], S[fid[3]], S[fid[4]], g);
+    if (MIXED)  // nonlinear weights in FP32, candidate values in FP64
+        weno3q_mixed(qc, qc.lw32, S[fid[0]], S[fid[1]], S[fid[2]], S[fid[3]], S[fid[4]], g[0], g[1], g[2]);
+    else
+        weno3q_gp3(qc, S[fid[0]], S[fid[1]], S[fid[2]], S[fid[3]], S[fid[4]], g);
 }
 
 template <bool YDIR, int FLUX>
@@ -669,14 +720,14 @@ HTX_HD void fvh_flux_at(const FvhArgs& a, int i, int j, int m, const double c[6]
 
 // All of one face on one work item (the CPU form): out[m] = weighted fluxes
 // at Gauss point m.
-template <bool YDIR, int FLUX>
+template <bool YDIR, int FLUX, bool MIXED = false>
 HTX_HD void fvh_flux_face(const FvhArgs& a, int i, int j, const double* F, long long fstr, double out[3][12]) {
     long long fid[5];
     fvh_flux_fids<YDIR>(a, i, j, fid);
     double c[3][6][4];
     for (int s = 0; s < 24; ++s) {
         double g[3];
-        fvh_series_gp3(a.qc, F, fstr, s, fid, g);
+        fvh_series_gp3<MIXED>(a.qc, F, fstr, s, fid, g);
         for (int m = 0; m < 3; ++m) c[m][s / 4][s % 4] = g[m];
     }
     for (int m = 0; m < 3; ++m) fvh_flux_at<YDIR, FLUX>(a, i, j, m, c[m], out[m]);
@@ -689,7 +740,7 @@ HTX_HD void fvh_flux_face(const FvhArgs& a, int i, int j, const double* F, long 
 // barrier each thread takes its point's 24 values, computes the fluxes, and
 // the three points are summed in order through the same LDS.
 //   lds: 24 x 3 x 64 doubles (series values), then 3 x 12 x 64 (fluxes)
-template <bool YDIR, int FLUX>
+template <bool YDIR, int FLUX, bool MIXED = false>
 __device__ void fvh_flux_block(const FvhArgs& a, long long t0, long long nf, double* lds) {
     const int x = threadIdx.x, m = threadIdx.y;
     const long long t = t0 + x;
@@ -710,7 +761,7 @@ __device__ void fvh_flux_block(const FvhArgs& a, long long t0, long long nf, dou
         for (int q = 0; q < 8; ++q) {
             const int s = 8 * m + q;
             double g[3];
-            fvh_series_gp3(a.qc, F, nf, s, fid, g);
+            fvh_series_gp3<MIXED>(a.qc, F, nf, s, fid, g);
 #pragma unroll
             for (int p = 0; p < 3; ++p) lds[(s * 3 + p) * 64 + x] = g[p];
         }

@@ -50,7 +50,9 @@ enum htx_bc { HTX_BC_ZERO_GRAD = 0, HTX_BC_PERIODIC = 1 };
  *   denominator outside the TENO decision chain; residuals agree to rounding,
  *   but rounding can move a TENO cut-off decision that sits at its threshold.
  * HTX_PREC_FP32_MIXED: FP32 cut-off decisions and nonlinear weights, FP64
- *   values, projections, fluxes and RK accumulation (TENO5 scheme).
+ *   values, projections, fluxes and RK accumulation (TENO5: the TENO5 cut-off
+ *   and the transverse WENO weights; Hermite-TENO: the HTENO and TENO5
+ *   cut-offs and the transverse WENO weights, the derivative limiter FP64).
  * HTX_PREC_LEARNED: FP32-mixed TENO5 where a learned classifier (see below)
  *   sends the faces it finds smooth to the optimal-weight linear scheme
  *   (Lagrange::linear), guarded by an FP64 positivity check, and the rest
@@ -252,7 +254,8 @@ int htx_classifier_stats(htx_solver* h, long long* n_faces, long long* n_flagged
  * one persistent launch per RK stage and must share flux and precision
  * (fp64_fast or fp32_mixed; the learned precision is not available in
  * batches and htx_batch_create returns HTX_ENOTSUP for it); Hermite instances
- * run batched Hermite launches in fp64_fast and must share their flux.  A step can be captured
+ * run batched Hermite launches and must share flux and precision (fp64_fast
+ * or fp32_mixed).  A step can be captured
  * once in a hipGraph and replayed.  Instances past their tmax are skipped.
  * ------------------------------------------------------------------------ */
 typedef struct htx_batch htx_batch;

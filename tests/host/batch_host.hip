@@ -117,8 +117,12 @@ void herm_launch(const HermTables& H, int ninst, int mode) {
             for (int tid = 0; tid < 256; ++tid) {
                 long long t = lb * 256ll + tid;
                 if (t >= nf) continue;
-                if (y) fvh_recon_face<true>(a, (int)(t % a.nx), (int)(t / a.nx) - 1, a.fy, t, nf);
-                else fvh_recon_face<false>(a, (int)(t % (a.nx + 1)) - 1, (int)(t / (a.nx + 1)), a.fx, t, nf);
+                const int i = y ? (int)(t % a.nx) : (int)(t % (a.nx + 1)) - 1;
+                const int j = y ? (int)(t / a.nx) - 1 : (int)(t / (a.nx + 1));
+                if (y && a.mixed) fvh_recon_face<true, true>(a, i, j, a.fy, t, nf);
+                else if (y) fvh_recon_face<true, false>(a, i, j, a.fy, t, nf);
+                else if (a.mixed) fvh_recon_face<false, true>(a, i, j, a.fx, t, nf);
+                else fvh_recon_face<false, false>(a, i, j, a.fx, t, nf);
             }
         }
     for (int y = 0; y < 2; ++y)
@@ -130,8 +134,12 @@ void herm_launch(const HermTables& H, int ninst, int mode) {
                 long long t = lb * 64ll + tid;
                 if (t >= nf) continue;
                 double o[3][12];
-                if (y) fvh_flux_face<true, FLUX>(a, (int)(t % a.nx), (int)(t / a.nx) - 1, a.fy, nf, o);
-                else fvh_flux_face<false, FLUX>(a, (int)(t % (a.nx + 1)) - 1, (int)(t / (a.nx + 1)), a.fx, nf, o);
+                const int i = y ? (int)(t % a.nx) : (int)(t % (a.nx + 1)) - 1;
+                const int j = y ? (int)(t / a.nx) - 1 : (int)(t / (a.nx + 1));
+                if (y && a.mixed) fvh_flux_face<true, FLUX, true>(a, i, j, a.fy, nf, o);
+                else if (y) fvh_flux_face<true, FLUX, false>(a, i, j, a.fy, nf, o);
+                else if (a.mixed) fvh_flux_face<false, FLUX, true>(a, i, j, a.fx, nf, o);
+                else fvh_flux_face<false, FLUX, false>(a, i, j, a.fx, nf, o);
                 double* G = y ? a.gy : a.gx;
                 for (int c = 0; c < 12; ++c) G[c * nf + t] = (o[0][c] + o[1][c]) + o[2][c];
             }

@@ -8,8 +8,9 @@
 
 using namespace htx;
 
-extern "C" int fvhh_rhs(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc, int flux,
-                        int quirks, const double* q, double* rhs, double* qlim) {
+template <bool MIXED>
+static int rhs_impl(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc, int flux,
+                    int quirks, const double* q, double* rhs, double* qlim) {
     FvhArgs a{};
     size_t n = (size_t)nx * ny;
     std::vector<double> lv(8 * n), fx(24 * (size_t)ny * (nx + 1)), fy(24 * (size_t)(ny + 1) * nx),
@@ -30,6 +31,7 @@ extern "C" int fvhh_rhs(int nx, int ny, double xl, double xr, double yb, double 
     a.bc = bc;
     a.mode = 0;
     a.quirks = quirks;
+    a.mixed = MIXED;
     a.gam = gamma;
     a.g1 = gamma - 1.0;
     a.dx = (xr - xl) / nx;
@@ -44,22 +46,23 @@ extern "C" int fvhh_rhs(int nx, int ny, double xl, double xr, double yb, double 
     }
     long long fxs = (long long)ny * (nx + 1), fys = (long long)(ny + 1) * nx;
     for (int j = 0; j < ny; ++j)
-        for (int fi = 0; fi <= nx; ++fi) fvh_recon_face<false>(a, fi - 1, j, a.fx, (long long)j * (nx + 1) + fi, fxs);
+        for (int fi = 0; fi <= nx; ++fi)
+            fvh_recon_face<false, MIXED>(a, fi - 1, j, a.fx, (long long)j * (nx + 1) + fi, fxs);
     for (int fj = 0; fj <= ny; ++fj)
-        for (int i = 0; i < nx; ++i) fvh_recon_face<true>(a, i, fj - 1, a.fy, (long long)fj * nx + i, fys);
+        for (int i = 0; i < nx; ++i) fvh_recon_face<true, MIXED>(a, i, fj - 1, a.fy, (long long)fj * nx + i, fys);
     double o[3][12];
     for (int j = 0; j < ny; ++j)
         for (int fi = 0; fi <= nx; ++fi) {
-            if (fk == 0) fvh_flux_face<false, 0>(a, fi - 1, j, a.fx, fxs, o);
-            else if (fk == 1) fvh_flux_face<false, 1>(a, fi - 1, j, a.fx, fxs, o);
-            else fvh_flux_face<false, 2>(a, fi - 1, j, a.fx, fxs, o);
+            if (fk == 0) fvh_flux_face<false, 0, MIXED>(a, fi - 1, j, a.fx, fxs, o);
+            else if (fk == 1) fvh_flux_face<false, 1, MIXED>(a, fi - 1, j, a.fx, fxs, o);
+            else fvh_flux_face<false, 2, MIXED>(a, fi - 1, j, a.fx, fxs, o);
             for (int c = 0; c < 12; ++c) a.gx[c * fxs + (long long)j * (nx + 1) + fi] = (o[0][c] + o[1][c]) + o[2][c];
         }
     for (int fj = 0; fj <= ny; ++fj)
         for (int i = 0; i < nx; ++i) {
-            if (fk == 0) fvh_flux_face<true, 0>(a, i, fj - 1, a.fy, fys, o);
-            else if (fk == 1) fvh_flux_face<true, 1>(a, i, fj - 1, a.fy, fys, o);
-            else fvh_flux_face<true, 2>(a, i, fj - 1, a.fy, fys, o);
+            if (fk == 0) fvh_flux_face<true, 0, MIXED>(a, i, fj - 1, a.fy, fys, o);
+            else if (fk == 1) fvh_flux_face<true, 1, MIXED>(a, i, fj - 1, a.fy, fys, o);
+            else fvh_flux_face<true, 2, MIXED>(a, i, fj - 1, a.fy, fys, o);
             for (int c = 0; c < 12; ++c) a.gy[c * fys + (long long)fj * nx + i] = (o[0][c] + o[1][c]) + o[2][c];
         }
     double smx = 0, smy = 0;
@@ -70,6 +73,18 @@ extern "C" int fvhh_rhs(int nx, int ny, double xl, double xr, double yb, double 
         for (size_t c = 0; c < 8 * n; ++c) qlim[4 * n + c] = lv[c];
     }
     return 0;
+}
+
+extern "C" int fvhh_rhs(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc, int flux,
+                        int quirks, const double* q, double* rhs, double* qlim) {
+    return rhs_impl<false>(nx, ny, xl, xr, yb, yt, gamma, bc, flux, quirks, q, rhs, qlim);
+}
+
+// the FP32-mixed precision's bodies (HTENO / TENO5 cut-offs and transverse
+// weights in FP32)
+extern "C" int fvhh_rhs_mixed(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc,
+                              int flux, int quirks, const double* q, double* rhs, double* qlim) {
+    return rhs_impl<true>(nx, ny, xl, xr, yb, yt, gamma, bc, flux, quirks, q, rhs, qlim);
 }
 
 // The limiter in one piece (herm_limiter) and split at its chain dependence

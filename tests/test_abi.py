@@ -88,3 +88,28 @@ def test_batch_rejects_learned_precision():
         Batch(ps)
     assert ei.value.code == -5
     assert "fp64_fast or fp32_mixed" in str(ei.value)
+
+
+def test_hermite_rejects_learned_precision():
+    """Hermite-TENO is built in fp64, fp64_fast and fp32_mixed; the learned
+    precision (a TENO5 classifier) is refused before any device work, for a
+    solver and for batched Hermite instances, as are Hermite instances of
+    one batch that mix precisions."""
+    import htx
+    from htx.batch import Batch, params_for
+
+    with pytest.raises(htx.HtxError) as ei:
+        htx.System2D(0.0, 1.0, 16, 0.0, 1.0, 16, 0.5, 1.0, 0.0, 1.4, scheme="hteno", precision="learned")
+    assert ei.value.code == -5
+    assert "TENO5 scheme only" in str(ei.value)
+    ps = [params_for(0.0, 1.0, 32, 0.0, 1.0, 32, 0.5, 0.1, 1.4, scheme="hteno", flux="rusanov",
+                     precision="learned")]
+    with pytest.raises(htx.HtxError) as ei:
+        Batch(ps)
+    assert ei.value.code == -5
+    ps = [params_for(0.0, 1.0, 32, 0.0, 1.0, 32, 0.5, 0.1, 1.4, scheme="hteno", flux="rusanov", precision=p)
+          for p in ("fp64_fast", "fp32_mixed")]
+    with pytest.raises(htx.HtxError) as ei:
+        Batch(ps)
+    assert ei.value.code == -5
+    assert "share flux and precision" in str(ei.value)

@@ -49,13 +49,13 @@ def instances():
     return out
 
 
-def run(lib, ws, precision, tx, ty, nblk, nsteps, tmax=10.0):
+def run(lib, ws, precision, tx, ty, nblk, nsteps, tmax=10.0, herm_precision="fp64_fast"):
     from htx import _lib
     from htx.batch import params_for
     from htx.ic import piecewise_state
 
     ps = [params_for(w["xl"], w["xr"], w["nx"], w["yb"], w["yt"], w["ny"], 0.5, tmax, GAMMA, scheme=w["scheme"],
-                     flux=w["flux"], precision=precision if w["scheme"] == "teno5" else "fp64_fast") for w in ws]
+                     flux=w["flux"], precision=precision if w["scheme"] == "teno5" else herm_precision) for w in ws]
     arr = (_lib.HtxParams * len(ps))(*ps)
     qs = [piecewise_state(w, w["nx"], w["ny"], hermite=w["scheme"] == "hteno") for w in ws]
     q = np.ascontiguousarray(np.concatenate([a.reshape(-1) for a in qs]))
@@ -79,10 +79,11 @@ def test_batch_walk_matches_instances_alone(harness, tile):
 
 
 def test_batch_walk_fp32_mixed_matches_instances_alone(harness):
+    """Both schemes in the FP32-mixed precision."""
     ws = instances()
-    qb, tb, _ = run(harness, ws, "fp32_mixed", 16, 16, nblk=4, nsteps=2)
+    qb, tb, _ = run(harness, ws, "fp32_mixed", 16, 16, nblk=4, nsteps=2, herm_precision="fp32_mixed")
     for i, w in enumerate(ws):
-        qa, ta, _ = run(harness, [w], "fp32_mixed", 16, 16, nblk=2, nsteps=2)
+        qa, ta, _ = run(harness, [w], "fp32_mixed", 16, 16, nblk=2, nsteps=2, herm_precision="fp32_mixed")
         np.testing.assert_array_equal(qb[i], qa[0])
         assert tb[i] == ta[0]
 

@@ -26,12 +26,12 @@ def instances():
     return out
 
 
-def make_batch(precision="fp64_fast", tmax=0.05):
+def make_batch(precision="fp64_fast", tmax=0.05, herm_precision="fp64_fast"):
     from htx.batch import Batch, params_for
 
     inst = instances()
     ps = [params_for(0.0, d["xr"], d["nx"], 0.0, d["yt"], d["ny"], 0.5, tmax, GAMMA, scheme=d["scheme"],
-                     flux=d["flux"], precision=precision if d["scheme"] == "teno5" else "fp64_fast")
+                     flux=d["flux"], precision=precision if d["scheme"] == "teno5" else herm_precision)
           for d in inst]
     b = Batch(ps)
     for i, d in enumerate(inst):
@@ -39,11 +39,11 @@ def make_batch(precision="fp64_fast", tmax=0.05):
     return b, inst
 
 
-def single(d, precision, nsteps):
+def single(d, precision, nsteps, herm_precision="fp64_fast"):
     import htx
 
     s = htx.System2D(0.0, d["xr"], d["nx"], 0.0, d["yt"], d["ny"], 0.5, 10.0, 0.0, GAMMA, scheme=d["scheme"],
-                     flux=d["flux"], bc="zero_grad", precision=precision if d["scheme"] == "teno5" else "fp64_fast")
+                     flux=d["flux"], bc="zero_grad", precision=precision if d["scheme"] == "teno5" else herm_precision)
     s.quadrant(d["xs"], d["ys"], d["states"])
     s.steps_async(nsteps)
     return s.get_state(), s.time
@@ -63,6 +63,25 @@ def test_batch_matches_single_instances():
         assert n == 5
         assert t == pytest.approx(t_ref, rel=1e-12)
         assert err < 1e-12
+
+
+def test_batch_fp32_mixed_matches_single_instances():
+    """Both schemes in the FP32-mixed precision: the batched launches run the
+    single-instance kernels' bodies, so each instance matches its solver run
+    alone (5 steps)."""
+    print("[case] batch of 6 in fp32_mixed vs single solvers, 5 steps", flush=True)
+    b, inst = make_batch("fp32_mixed", tmax=10.0, herm_precision="fp32_mixed")
+    b.steps(5)
+    for i, d in enumerate(inst):
+        q_ref, t_ref = single(d, "fp32_mixed", 5, herm_precision="fp32_mixed")
+        q = b.get_state(i)
+        t, n = b.time(i)
+        err = rel_l1(q, q_ref)
+        print(f"instance {i} {d['scheme']} {d['nx']}x{d['ny']}: rel L1 {err:.3e}", flush=True)
+        assert n == 5
+        assert t == pytest.approx(t_ref, rel=1e-12)
+        assert err < 1e-12
+    b.close()
 
 
 def test_batch_tile_shapes_agree():
