@@ -411,14 +411,15 @@ HTX_HD void fvh_limit_line(const FvhArgs& a, int dir, int line, int e) {
 // arithmetic, and one wavefront per SIMD runs it, so everything that does not
 // depend on the chain moves off the consumer: the producers load u and the
 // old moments, compute herm_limiter_pre for the next chunk of LIM_B steps of
-// the 64 chains into LDS (producer p takes the steps s = p mod LIM_NP) and
-// store the chunk the consumer finished before, while the consumer runs
+// the 64 chains into LDS (LIM_B x 64 items shared among the producer lanes
+// in the order that keeps their loads and stores coalesced) and store the
+// chunk the consumer finished before, while the consumer runs
 // herm_limiter_step over the current chunk.  The consumer touches LDS only (a
 // global store would make a later step wait for the store's completion
 // before reusing its registers).  Two LDS buffers alternate, one barrier per
 // chunk, and the producers' loads run one chunk ahead of their arithmetic.
-//   LDS: pre[buffer][slot][step][chain], 2 x LIM_NS x LIM_B x 64 doubles
-//        (15 slots strict, 8 fast), res[buffer][step][chain], 2 x LIM_B x 64
+//   LDS: pre[buffer][slot][step][chain], 2 x LIM_NS x LIM_B x 65 doubles
+//        (15 slots strict, 8 fast), res[buffer][step][chain], 2 x LIM_B x 65
 //   blocks: blk < g0 = ceil(4 ny / 64) take the row chains (v), the next
 //           ceil(4 nx / 64) the column chains (w); LIM_THREADS threads each
 constexpr int LIM_B = HTX_EXACT_DIVISION ? 8 : 16, LIM_NP = 8, LIM_SP = LIM_B / LIM_NP;
@@ -429,42 +430,76 @@ static_assert(LIM_B % LIM_NP == 0, "every producer takes the same number of step
 HTX_HD int lim_blocks(int nx, int ny) { return (4 * ny + 63) / 64 + (4 * nx + 63) / 64; }
 
 __device__ inline void fvh_limit_block(const FvhArgs& a, int blk, int g0) {
-    __shared__ double pre[2][LIM_NS][LIM_B][64];
-    __shared__ double res[2][LIM_B][64];
+    // [.][65]: the padding spreads a producer's writes of consecutive steps
+    // of one chain over the LDS banks
+    __shared__ double pre[2][LIM_NS][LIM_B][65];
+    __shared__ double res[2][LIM_B][65];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int pw = wave - 1;  // producer index
-    const int c = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63;
     const int dir = blk < g0 ? 0 : 1;
     const int grp = dir == 0 ? blk : blk - g0;
     const int nl = dir == 0 ? a.ny : a.nx;  // lines per field
     const int n = dir == 0 ? a.nx : a.ny;   // chain length
-    const int t = grp * 64 + c;
-    const bool live = t < 4 * nl;
-    const int tt = live ? t : 4 * nl - 1;  // idle lanes shadow the last chain and store nothing
-    const int line = tt % nl, e = tt / nl;
     const long long fs = a.fstride;
-    const double* U = a.q + e * fs;
-    const double* D = a.q + (dir == 0 ? 4 + e : 8 + e) * fs;
-    double* O = a.lv_out + (dir == 0 ? e : 4 + e) * fs;
     const long long sk = dir == 0 ? 1 : a.pitch;
-    const long long base = dir == 0 ? (long long)line * a.pitch : line;
-    // k is uniform over the workgroup; indices off the line map through the
-    // boundary condition
-    auto off = [&](int k) { return base + (long long)((k >= 0 && k < n) ? k : map_index(k, n, a.bc)) * sk; };
+    // chain c of the workgroup: its U, old-moment and output fields and the
+    // offset of its first cell; idle chains (past 4 nl) shadow the last one
+    // and store nothing
+    struct Chain {
+        const double *U, *D;
+        double* O;
+        long long base;
+        bool live;
+    };
+    auto chain = [&](int c) {
+        const int t = grp * 64 + c;
+        const bool live = t < 4 * nl;
+        const int tt = live ? t : 4 * nl - 1;
+        const int line = tt % nl, e = tt / nl;
+        Chain ch;
+        ch.U = a.q + e * fs;
+        ch.D = a.q + (dir == 0 ? 4 + e : 8 + e) * fs;
+        ch.O = a.lv_out + (dir == 0 ? e : 4 + e) * fs;
+        ch.base = dir == 0 ? (long long)line * a.pitch : line;
+        ch.live = live;
+        return ch;
+    };
+    // step index -> cell along the chain; off the line through the boundary
+    // condition (one wrap or a clamp covers k in [-1, n + LIM_B] when
+    // n >= LIM_B + 2)
+    auto kmap = [&](int k) {
+        if (n >= LIM_B + 2) return a.bc == 1 ? (k < 0 ? k + n : (k >= n ? k - n : k)) : min(max(k, 0), n - 1);
+        return map_index(k, n, a.bc);
+    };
     const double dif = a.hc.dif;
     const int nch = (n + LIM_B - 1) / LIM_B;
 
-    // producer: u_{k-1}, u_k, u_{k+1} and the old moment_{k+1} of its steps
-    // k = chunk LIM_B + pw + LIM_NP j of the chunk it computes next
+    // Producer items: LIM_SP (chain, step) pairs per lane and chunk.  Row
+    // chains (dir 0) are contiguous along the chain, so consecutive lanes
+    // take consecutive steps of one chain (coalesced loads and stores);
+    // column chains are contiguous across chains, so consecutive lanes take
+    // consecutive chains at one step.
+    const int q = (wave > 0 ? wave - 1 : 0) * 64 + lane;
+    int ic[LIM_SP], is[LIM_SP];
+    Chain pc[LIM_SP];
+#pragma unroll
+    for (int j = 0; j < LIM_SP; ++j) {
+        const int idx = q + 64 * LIM_NP * j;
+        ic[j] = dir == 0 ? idx / LIM_B : idx % 64;
+        is[j] = dir == 0 ? idx % LIM_B : idx / 64;
+        pc[j] = chain(ic[j]);
+    }
+    // u_{k-1}, u_k, u_{k+1} and the old moment_{k+1} of the items of the
+    // chunk the producer computes next
     double r0[LIM_SP], r1[LIM_SP], r2[LIM_SP], rd[LIM_SP];
     auto fetch = [&](int chunk) {
 #pragma unroll
         for (int j = 0; j < LIM_SP; ++j) {
-            const int k = chunk * LIM_B + pw + LIM_NP * j;
-            r0[j] = U[off(k - 1)];
-            r1[j] = U[off(k)];
-            r2[j] = U[off(k + 1)];
-            rd[j] = D[off(k + 1)];
+            const int k = chunk * LIM_B + is[j];
+            r0[j] = pc[j].U[pc[j].base + kmap(k - 1) * sk];
+            r1[j] = pc[j].U[pc[j].base + kmap(k) * sk];
+            r2[j] = pc[j].U[pc[j].base + kmap(k + 1) * sk];
+            rd[j] = pc[j].D[pc[j].base + kmap(k + 1) * sk];
         }
     };
     auto produce = [&](int chunk) {
@@ -475,15 +510,15 @@ __device__ inline void fvh_limit_block(const FvhArgs& a, int blk, int g0) {
             herm_limiter_pre(dif, r0[j], r1[j], r2[j], rd[j], o);
 #pragma unroll
             for (int v = 0; v < LIM_NV; ++v)
-                if (lim_slot_pos(v) >= 0) pre[buf][lim_slot_pos(v)][pw + LIM_NP * j][c] = o[v];
+                if (lim_slot_pos(v) >= 0) pre[buf][lim_slot_pos(v)][is[j]][ic[j]] = o[v];
         }
     };
     auto store = [&](int chunk) {  // the consumer's results of a finished chunk
         const int buf = chunk & 1;
 #pragma unroll
         for (int j = 0; j < LIM_SP; ++j) {
-            const int s = pw + LIM_NP * j, k = chunk * LIM_B + s;
-            if (k < n && live) O[off(k)] = res[buf][s][c];
+            const int k = chunk * LIM_B + is[j];
+            if (k < n && pc[j].live) pc[j].O[pc[j].base + (long long)k * sk] = res[buf][is[j]][ic[j]];
         }
     };
     double prev = 0.0;
@@ -492,7 +527,8 @@ __device__ inline void fvh_limit_block(const FvhArgs& a, int blk, int g0) {
         produce(0);
         if (nch > 1) fetch(1);
     } else {
-        prev = D[off(-1)];  // ghost: pre-limiting value
+        const Chain me = chain(lane);
+        prev = me.D[me.base + kmap(-1) * sk];  // ghost: pre-limiting value
     }
     __syncthreads();
     for (int ch = 0; ch < nch; ++ch) {
@@ -501,17 +537,17 @@ __device__ inline void fvh_limit_block(const FvhArgs& a, int blk, int g0) {
             // the values of step s + 1 are read while step s computes
             double o[LIM_NV], on[LIM_NV];
 #pragma unroll
-            for (int v = 0; v < LIM_NV; ++v) o[v] = lim_slot_pos(v) >= 0 ? pre[buf][lim_slot_pos(v)][0][c] : 0.0;
+            for (int v = 0; v < LIM_NV; ++v) o[v] = lim_slot_pos(v) >= 0 ? pre[buf][lim_slot_pos(v)][0][lane] : 0.0;
 #pragma unroll
             for (int s = 0; s < LIM_B; ++s) {
                 if (s + 1 < LIM_B) {
 #pragma unroll
                     for (int v = 0; v < LIM_NV; ++v)
-                        on[v] = lim_slot_pos(v) >= 0 ? pre[buf][lim_slot_pos(v)][s + 1][c] : 0.0;
+                        on[v] = lim_slot_pos(v) >= 0 ? pre[buf][lim_slot_pos(v)][s + 1][lane] : 0.0;
                 }
                 if (ch * LIM_B + s < n) {
                     prev = herm_limiter_step(dif, o, prev);
-                    res[buf][s][c] = prev;
+                    res[buf][s][lane] = prev;
                 }
 #pragma unroll
                 for (int v = 0; v < LIM_NV; ++v) o[v] = on[v];
