@@ -1,11 +1,14 @@
 // main.cpp-style driver: the reference's driver (main.cpp:1-176) selects one
-// case by commenting code; this one takes the case name as argv[1].  It
+// case by commenting code; this one takes the case name as argv[1]
+// (vortex, entropy, riemann on FV::System2D; hvortex, hentropy, hriemann on
+// FVHermite::System2D).  It
 // compiles against include/ and links libhtx.so; the class names and calls
 // are the reference's.
 #include <cstdio>
 #include <cstring>
 #include <string>
 
+#include "FVHermite_System2D.h"
 #include "FV_System2D.h"
 
 int main(int argc, char** argv) {
@@ -31,6 +34,27 @@ int main(int argc, char** argv) {
             htx::Options o;
             o.flux = HTX_FLUX_HLLC;
             FV::System2D systemObj(0.0, 1.0, n, 0.0, 1.0, n, 0.4, 0.02, 0.0, 1.4, o);
+            systemObj.Riemann2D();
+            systemObj.timeStepping();
+            systemObj.printResult();
+            std::printf("steps %d t %.17g\n", systemObj.steps(), systemObj.time());
+        } else if (which == "hvortex") {
+            // reference main.cpp:142-145 form (commented case), vortex at the origin
+            FVHermite::System2D systemObj(-5.0, 5.0, n, -5.0, 5.0, n, 0.4, 0.05, 0.0, 1.4);
+            systemObj.isentropicVortex();
+            systemObj.timeStepping();
+            systemObj.printResult();
+            std::printf("steps %d t %.17g\n", systemObj.steps(), systemObj.time());
+        } else if (which == "hentropy") {
+            // reference main.cpp:102-105 (commented case)
+            FVHermite::System2D systemObj(-1.0, 1.0, n, -1.0, 1.0, n, 0.4, 0.05, 0.0, 1.4);
+            systemObj.entropyWave();
+            systemObj.timeStepping();
+            systemObj.printResult();
+            std::printf("steps %d t %.17g\n", systemObj.steps(), systemObj.time());
+        } else if (which == "hriemann") {
+            // reference main.cpp:122-125 (commented case), configuration 5
+            FVHermite::System2D systemObj(0.0, 1.0, n, 0.0, 1.0, n, 0.4, 0.02, 0.0, 1.4);
             systemObj.Riemann2D();
             systemObj.timeStepping();
             systemObj.printResult();

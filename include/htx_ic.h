@@ -43,6 +43,15 @@ struct Euler2DIC {
         w[3] = p;
     }
 
+    // dconservativeOfEntropyWave, :442-452: the x (equally y) derivative of
+    // the conservative entropy-wave state (density depends on x + y only)
+    void d_entropy_wave(double u, double v, double amp, double x, double y, double c[4]) const {
+        c[0] = amp * std::cos(pi * (x + y)) * pi;
+        c[1] = u * c[0];
+        c[2] = v * c[0];
+        c[3] = (0.5 * (u * u + v * v)) * c[0];
+    }
+
     // conservativeOfIsentropicVortex (primitive part), :656-666, b = 5, at the origin
     void vortex(double x, double y, double w[4]) const {
         double r = std::pow(x, 2) + std::pow(y, 2), b = 5.0;
@@ -92,6 +101,27 @@ struct Euler2DIC {
                         double w[4], c[4];
                         prim(x + dx * gx[a] / 2.0, y + dy * gx[b] / 2.0, w);
                         conservative(w, c);
+                        for (int e = 0; e < 4; ++e) acc[e] += gw[a] * gw[b] * c[e] / 4.0;
+                    }
+                for (int e = 0; e < 4; ++e) q[e * n + (size_t)j * nx + i] = acc[e];
+            }
+    }
+
+    // The same 9x9 Gauss cell averages of a conservative-valued cons(x, y, c)
+    // (derivative moments, Euler2D::entropyWavesdx/dy :473-511).
+    template <class F>
+    void average_conservative(int nx, int ny, double dx, double dy, double xl, double yb, std::vector<double>& q,
+                              F cons) const {
+        size_t n = (size_t)nx * ny;
+        q.assign(4 * n, 0.0);
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                double x = ((double)i + 0.5) * dx + xl, y = ((double)j + 0.5) * dy + yb;
+                double acc[4] = {0, 0, 0, 0};
+                for (int a = 0; a < 9; ++a)
+                    for (int b = 0; b < 9; ++b) {
+                        double c[4];
+                        cons(x + dx * gx[a] / 2.0, y + dy * gx[b] / 2.0, c);
                         for (int e = 0; e < 4; ++e) acc[e] += gw[a] * gw[b] * c[e] / 4.0;
                     }
                 for (int e = 0; e < 4; ++e) q[e * n + (size_t)j * nx + i] = acc[e];

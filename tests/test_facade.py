@@ -31,6 +31,18 @@ def test_facade_signatures_match_reference():
         assert m in txt
 
 
+def test_hermite_facade_signatures_match_reference():
+    """Constructor argument list and public methods of FVHermite::System2D
+    (FVHermite_System2D.h:16-24)."""
+    txt = open(os.path.join(ROOT, "include", "FVHermite_System2D.h")).read()
+    assert "namespace FVHermite" in txt
+    assert ("System2D(const double& xl, const double& xr, const int& nx, const double& yb, const double& yt, "
+            "const int& ny,") in " ".join(txt.split())
+    for m in ["void entropyWave()", "void isentropicVortex()", "void Riemann2D()", "void timeStepping()",
+              "void printResult()"]:
+        assert m in txt
+
+
 def test_host_ic_matches_oracle_vortex(orc, tmp_path):
     """htx_ic.h (the facade's host initial conditions) equals the oracle's
     vortex cell averages bit for bit."""
@@ -59,7 +71,7 @@ int main() {
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["vortex", "entropy", "riemann"])
+@pytest.mark.parametrize("case", ["vortex", "entropy", "riemann", "hvortex", "hentropy", "hriemann"])
 def test_driver_runs_reference_cases(case):
     if not os.path.exists(EXE):
         pytest.skip("driver not built")
@@ -67,6 +79,39 @@ def test_driver_runs_reference_cases(case):
     r = subprocess.run([EXE, case, "32"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "steps" in r.stdout
-    if case != "riemann":
+    if case not in ("riemann", "hriemann"):
         err = float(r.stdout.split("Numerical error:")[1].split()[0])
         assert np.isfinite(err) and err < 0.1
+
+
+def test_host_entropy_wave_moments_closed_form(tmp_path):
+    """The Hermite facade's entropy-wave derivative moments (htx_ic.h
+    d_entropy_wave averaged by average_conservative) against the closed-form
+    cell average of d/dx (1 + A sin(pi (x + y))) = A pi cos(pi (x + y)):
+    (4 A / (pi dx dy)) sin(pi dx / 2) sin(pi dy / 2) cos(pi (x0 + y0)); the
+    momentum and energy moments are (u, v, (u^2 + v^2)/2) times it."""
+    prog = tmp_path / "dw.cpp"
+    prog.write_text(r'''
+#include "htx_ic.h"
+#include <cstdio>
+int main() {
+    htx::ic::Euler2DIC ic(1.4);
+    std::vector<double> q;
+    ic.average_conservative(10, 8, 0.2, 0.25, -1.0, -1.0, q, [&](double x, double y, double c[4]) {
+        ic.d_entropy_wave(1.0, 1.0, 0.5, x, y, c);
+    });
+    fwrite(q.data(), sizeof(double), q.size(), stdout);
+}
+''')
+    exe = tmp_path / "dw"
+    subprocess.run(["g++", "-std=c++11", "-O2", "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)],
+                   check=True)
+    q = np.frombuffer(subprocess.run([str(exe)], capture_output=True, check=True).stdout,
+                      dtype=np.float64).reshape(4, 8, 10)
+    dx, dy, a = 0.2, 0.25, 0.5
+    x0 = -1.0 + (np.arange(10) + 0.5) * dx
+    y0 = -1.0 + (np.arange(8) + 0.5) * dy
+    X, Y = np.meshgrid(x0, y0)
+    ref = 4 * a / (np.pi * dx * dy) * np.sin(np.pi * dx / 2) * np.sin(np.pi * dy / 2) * np.cos(np.pi * (X + Y))
+    for e, f in enumerate([1.0, 1.0, 1.0, 1.0]):
+        np.testing.assert_allclose(q[e], f * ref, rtol=0, atol=1e-12)
