@@ -554,6 +554,116 @@ int htx_get_time(htx_solver* h, double* t, int* nsteps) {
     return HTX_OK;
 }
 
+// Snapshot file, little-endian:
+//   char magic[8] "HTXSNAP1"; int32 version (1);
+//   int32 nfield, nx, ny, scheme, flux, bc, precision, ref_quirks;
+//   double xl, xr, yb, yt, cfl, tmax, tinit, gamma, t; int64 nsteps;
+//   double q[nfield][ny][nx] (field-major, interior cells);
+//   uint64 checksum of q (snap_checksum).
+namespace {
+constexpr char SNAP_MAGIC[8] = {'H', 'T', 'X', 'S', 'N', 'A', 'P', '1'};
+constexpr int SNAP_VERSION = 1;
+
+// checksum of the state words w_i: sum_i w_i (2 i + 1) mod 2^64 (an odd
+// weight per position: any single changed word or bit changes it)
+unsigned long long snap_checksum(const double* q, size_t n) {
+    unsigned long long s = 0;
+    for (size_t i = 0; i < n; ++i) {
+        unsigned long long w;
+        std::memcpy(&w, q + i, sizeof(w));
+        s += w * (2ull * i + 1ull);
+    }
+    return s;
+}
+
+struct SnapHeader {
+    int32_t nfield, nx, ny, scheme, flux, bc, precision, ref_quirks;
+    double xl, xr, yb, yt, cfl, tmax, tinit, gamma, t;
+    int64_t nsteps;
+};
+
+struct FileCloser {
+    FILE* f;
+    ~FileCloser() {
+        if (f) std::fclose(f);
+    }
+};
+}  // namespace
+
+int htx_save(htx_solver* h, const char* path) {
+    if (!h || !path) return fail(HTX_EINVAL, "null argument");
+    if (!h->have_state) return fail(HTX_ESTATE, "no state set");
+    std::vector<double> q((size_t)h->nfield * h->nx * h->ny);
+    int rc = htx_get_state(h, q.data(), (size_t)h->nx);
+    if (rc) return rc;
+    double t = 0.0;
+    int nsteps = 0;
+    if ((rc = htx_get_time(h, &t, &nsteps))) return rc;
+    SnapHeader hd{};
+    hd.nfield = h->nfield;
+    hd.nx = h->nx;
+    hd.ny = h->ny;
+    hd.scheme = h->p.scheme;
+    hd.flux = h->p.flux;
+    hd.bc = h->p.bc;
+    hd.precision = h->p.precision;
+    hd.ref_quirks = h->p.ref_quirks;
+    hd.xl = h->p.xl;
+    hd.xr = h->p.xr;
+    hd.yb = h->p.yb;
+    hd.yt = h->p.yt;
+    hd.cfl = h->p.cfl;
+    hd.tmax = h->p.tmax;
+    hd.tinit = h->p.tinit;
+    hd.gamma = h->p.gamma;
+    hd.t = t;
+    hd.nsteps = nsteps;
+    const int32_t ver = SNAP_VERSION;
+    const unsigned long long hsh = snap_checksum(q.data(), q.size());
+    FileCloser fc{std::fopen(path, "wb")};
+    if (!fc.f) return fail(HTX_EIO, std::string("cannot open ") + path + " for writing");
+    bool ok = std::fwrite(SNAP_MAGIC, 1, 8, fc.f) == 8 && std::fwrite(&ver, sizeof(ver), 1, fc.f) == 1 &&
+              std::fwrite(&hd.nfield, sizeof(int32_t), 8, fc.f) == 8 && std::fwrite(&hd.xl, sizeof(double), 9, fc.f) == 9 &&
+              std::fwrite(&hd.nsteps, sizeof(int64_t), 1, fc.f) == 1 &&
+              std::fwrite(q.data(), sizeof(double), q.size(), fc.f) == q.size() &&
+              std::fwrite(&hsh, sizeof(hsh), 1, fc.f) == 1;
+    ok = (std::fclose(fc.f) == 0) && ok;
+    fc.f = nullptr;
+    if (!ok) return fail(HTX_EIO, std::string("write to ") + path + " failed");
+    return HTX_OK;
+}
+
+int htx_load(htx_solver* h, const char* path) {
+    if (!h || !path) return fail(HTX_EINVAL, "null argument");
+    FileCloser fc{std::fopen(path, "rb")};
+    if (!fc.f) return fail(HTX_EIO, std::string("cannot open ") + path);
+    char magic[8];
+    int32_t ver = 0;
+    SnapHeader hd{};
+    if (std::fread(magic, 1, 8, fc.f) != 8 || std::memcmp(magic, SNAP_MAGIC, 8) != 0)
+        return fail(HTX_EIO, std::string(path) + " is not a snapshot");
+    if (std::fread(&ver, sizeof(ver), 1, fc.f) != 1 || ver != SNAP_VERSION)
+        return fail(HTX_EIO, "unsupported snapshot version");
+    if (std::fread(&hd.nfield, sizeof(int32_t), 8, fc.f) != 8 || std::fread(&hd.xl, sizeof(double), 9, fc.f) != 9 ||
+        std::fread(&hd.nsteps, sizeof(int64_t), 1, fc.f) != 1)
+        return fail(HTX_EIO, "truncated snapshot header");
+    if (hd.nfield != h->nfield || hd.nx != h->nx || hd.ny != h->ny || hd.scheme != h->p.scheme)
+        return fail(HTX_EINVAL, "snapshot grid or scheme differs from the solver's");
+    std::vector<double> q((size_t)h->nfield * h->nx * h->ny);
+    unsigned long long hsh = 0;
+    if (std::fread(q.data(), sizeof(double), q.size(), fc.f) != q.size() || std::fread(&hsh, sizeof(hsh), 1, fc.f) != 1)
+        return fail(HTX_EIO, "truncated snapshot");
+    if (hsh != snap_checksum(q.data(), q.size())) return fail(HTX_EIO, "snapshot checksum mismatch");
+    int rc = htx_set_state(h, q.data(), (size_t)h->nx);
+    if (rc) return rc;
+    Ctl c;
+    if ((rc = read_ctl(h, &c))) return rc;
+    c.t = hd.t;
+    c.nsteps = (int)hd.nsteps;
+    HIP_TRY(copy_sync(h->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
+    return HTX_OK;
+}
+
 int htx_sync(htx_solver* h) {
     if (!h) return fail(HTX_EINVAL, "null handle");
     HIP_TRY(hipStreamSynchronize(h->stream));

@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhtx.so")
 
 HTX_OK = 0
-ERRORS = {-1: "HTX_EINVAL", -2: "HTX_EDEVICE", -3: "HTX_ENOMEM", -4: "HTX_ESTATE", -5: "HTX_ENOTSUP"}
+ERRORS = {-1: "HTX_EINVAL", -2: "HTX_EDEVICE", -3: "HTX_ENOMEM", -4: "HTX_ESTATE", -5: "HTX_ENOTSUP", -6: "HTX_EIO"}
 
 SCHEMES = {"teno5": 0, "hteno": 1}
 FLUXES = {"rusanov": 0, "rusanov_char": 0, "hllc": 1}
@@ -75,6 +75,8 @@ SIGNATURES = {
     "htx_advance": (_I, [_P, _D, _IP]),
     "htx_get_time": (_I, [_P, _DP, _IP]),
     "htx_sync": (_I, [_P]),
+    "htx_save": (_I, [_P, ctypes.c_char_p]),
+    "htx_load": (_I, [_P, ctypes.c_char_p]),
     "htx_timing_enable": (_I, [_P, _I]),
     "htx_timing_read": (_I, [_P, _DP, _IP]),
     "htx_trainer_create": (_I, [ctypes.c_longlong, ctypes.POINTER(_P)]),

@@ -7,6 +7,7 @@ Method choices that the reference makes by commenting code in or out
 (flux, boundary condition, scheme) are constructor keywords here.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -202,6 +203,16 @@ class System2D:
         n = ctypes.c_int()
         check(_lib.load().htx_get_time(self._h, ctypes.byref(t), ctypes.byref(n)))
         return n.value
+
+    def save(self, path):
+        """Snapshot of the state, time, step count and configuration
+        (htx_save; htx.snapshot.read reads it without a GPU)."""
+        check(_lib.load().htx_save(self._h, os.fsencode(path)))
+
+    def load(self, path):
+        """Restore state, time and step count from a snapshot of a solver with
+        the same grid and scheme (htx_load)."""
+        check(_lib.load().htx_load(self._h, os.fsencode(path)))
 
     def timing_enable(self, on=True):
         check(_lib.load().htx_timing_enable(self._h, 1 if on else 0))

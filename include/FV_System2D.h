@@ -58,6 +58,10 @@ public:
         check(htx_get_state(h_, q.data(), (size_t)nx_));
         return q;
     }
+    // Snapshot of the state, time, step count and configuration (htx_save),
+    // and its restoration into a solver of the same grid and scheme.
+    void saveSnapshot(const std::string& path) { check(htx_save(h_, path.c_str())); }
+    void loadSnapshot(const std::string& path) { check(htx_load(h_, path.c_str())); }
 
 protected:
     SolverBase(double xl, double xr, int nx, double yb, double yt, int ny, double cfl, double tmax, double tinit,

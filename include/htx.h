@@ -37,7 +37,8 @@ enum htx_error {
     HTX_EDEVICE = -2,  /* HIP runtime error, or no device */
     HTX_ENOMEM = -3,   /* device allocation failed */
     HTX_ESTATE = -4,   /* call out of order (no state set) */
-    HTX_ENOTSUP = -5   /* combination not built yet */
+    HTX_ENOTSUP = -5,  /* combination not built yet */
+    HTX_EIO = -6       /* snapshot file missing, unreadable, truncated or corrupt */
 };
 
 enum htx_scheme { HTX_SCHEME_TENO5 = 0, HTX_SCHEME_HTENO = 1 };
@@ -148,6 +149,23 @@ int htx_get_time(htx_solver* h, double* t, int* nsteps);
 
 /* Waits for all work enqueued on the handle's stream. */
 int htx_sync(htx_solver* h);
+
+/* Snapshots (no reference counterpart; the reference only writes the final
+ * ASCII .dat, FV_System2D.cpp:668-715): the state, simulation time, step
+ * count and configuration in one binary file, little-endian:
+ *   char magic[8] "HTXSNAP1"; int32 version (1);
+ *   int32 nfield, nx, ny, scheme, flux, bc, precision, ref_quirks;
+ *   double xl, xr, yb, yt, cfl, tmax, tinit, gamma, t; int64 nsteps;
+ *   double q[nfield][ny][nx] (field-major interior cells, as htx_get_state);
+ *   uint64 checksum: sum over the words w_i of q (as uint64) of
+ *   w_i (2 i + 1), modulo 2^64.
+ * htx_load needs a solver of the same grid and scheme; it restores the
+ * state, t and the step count (the solver keeps its own flux, boundary
+ * condition, precision, CFL number and end time), so that continuing from
+ * a snapshot reproduces the uninterrupted run.  HTX_EIO for a missing,
+ * truncated or corrupt file. */
+int htx_save(htx_solver* h, const char* path);
+int htx_load(htx_solver* h, const char* path);
 
 /* Stage-kernel timing with HIP events on the handle's stream.  While
  * enabled, every stage-kernel launch is bracketed by an event pair;
