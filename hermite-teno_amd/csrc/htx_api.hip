@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/htx.h"
+#include "htx_fv1d.h"
 #include "htx_hermite.h"
 #include "htx_learn.h"
 #include "htx_learn_api.h"
@@ -19,6 +20,8 @@ hipError_t launch_ic_vortex_hermite(const VortexArgs& a, hipStream_t s);
 // strict IEEE build of the same kernels (htx_strict.hip)
 hipError_t htx_strict_launch_fv2d_stage(const void* a, int flux, int tile_x, int tile_y, hipStream_t s);
 hipError_t htx_strict_launch_fvh2d_stage(const void* a, int flux, hipStream_t s);
+hipError_t htx_strict_launch_fv1d_stage(const void* a, hipStream_t s);
+hipError_t htx_strict_launch_fv1d_max_speed(const double* q, int nx, double gam, void* ctl, hipStream_t s);
 
 using namespace htx;
 
@@ -93,12 +96,22 @@ namespace {
 int check_params(const htx_params* p) {
     if (!p) return fail(HTX_EINVAL, "params is null");
     if (p->struct_size != (int)sizeof(htx_params)) return fail(HTX_EINVAL, "params.struct_size mismatch");
-    if (p->dim != 2) return fail(HTX_ENOTSUP, "only dim = 2 is built");
-    if (p->nx <= 0 || p->ny <= 0) return fail(HTX_EINVAL, "grid size must be positive");
-    if (!(p->xr > p->xl) || !(p->yt > p->yb)) return fail(HTX_EINVAL, "domain must have positive extent");
+    if (p->dim != 1 && p->dim != 2) return fail(HTX_EINVAL, "dim must be 1 or 2");
+    if (p->nx <= 0 || (p->dim == 2 && p->ny <= 0)) return fail(HTX_EINVAL, "grid size must be positive");
+    if (!(p->xr > p->xl) || (p->dim == 2 && !(p->yt > p->yb)))
+        return fail(HTX_EINVAL, "domain must have positive extent");
+    if (p->dim == 1) {
+        // FV::System1D: TENO5 or the shipped TENO5i reconstruction, FP64
+        if (p->scheme != HTX_SCHEME_TENO5 && p->scheme != HTX_SCHEME_TENO5I)
+            return fail(HTX_EINVAL, "1D scheme must be teno5 or teno5i");
+        if (p->precision != HTX_PREC_FP64) return fail(HTX_ENOTSUP, "the 1D path is built in fp64 precision");
+    } else if (p->scheme == HTX_SCHEME_TENO5I) {
+        return fail(HTX_ENOTSUP, "teno5i is the 1D reconstruction");
+    }
     if (!(p->gamma > 1.0)) return fail(HTX_EINVAL, "gamma must exceed 1");
     if (!(p->cfl > 0.0)) return fail(HTX_EINVAL, "cfl must be positive");
-    if (p->scheme != HTX_SCHEME_TENO5 && p->scheme != HTX_SCHEME_HTENO) return fail(HTX_EINVAL, "unknown scheme");
+    if (p->scheme != HTX_SCHEME_TENO5 && p->scheme != HTX_SCHEME_HTENO && p->scheme != HTX_SCHEME_TENO5I)
+        return fail(HTX_EINVAL, "unknown scheme");
     if (p->flux != HTX_FLUX_RUSANOV_CHAR && p->flux != HTX_FLUX_HLLC) return fail(HTX_EINVAL, "unknown flux");
     if (p->bc != HTX_BC_ZERO_GRAD && p->bc != HTX_BC_PERIODIC) return fail(HTX_EINVAL, "unknown bc");
     if (p->precision != HTX_PREC_FP64 && p->precision != HTX_PREC_FP32_MIXED && p->precision != HTX_PREC_LEARNED &&
@@ -205,7 +218,22 @@ int launch_stage(htx_solver* h, const double* qin, const double* q0, double* qou
         f.flags_y = h->dFlagsY;
         HIP_TRY(lrn_launch_flags(f, h->stream));
     }
-    if (h->p.scheme == HTX_SCHEME_HTENO) {
+    if (h->p.dim == 1) {
+        Fv1dArgs a{};
+        a.qin = qin;
+        a.q0 = q0;
+        a.qout = qout;
+        a.ctl = h->ctl;
+        a.nx = h->nx;
+        a.bc = h->p.bc;
+        a.mode = mode;
+        a.recon = h->p.scheme == HTX_SCHEME_TENO5I ? 1 : 0;
+        a.flux = flux_kind(h);
+        a.gam = h->p.gamma;
+        a.g1 = h->p.gamma - 1.0;
+        a.dx = h->dx;
+        HIP_TRY(htx_strict_launch_fv1d_stage(&a, h->stream));
+    } else if (h->p.scheme == HTX_SCHEME_HTENO) {
         FvhArgs a = hermite_args(h, qin, q0, qout, mode);
         if (strict)
             HIP_TRY(htx_strict_launch_fvh2d_stage(&a, flux_kind(h), h->stream));
@@ -243,6 +271,10 @@ int read_ctl(htx_solver* h, Ctl* c) {
 
 int refresh_cfl(htx_solver* h) {
     HIP_TRY(launch_reset_smax(h->ctl, h->stream));
+    if (h->p.dim == 1) {
+        HIP_TRY(htx_strict_launch_fv1d_max_speed(h->dA, h->nx, h->p.gamma, h->ctl, h->stream));
+        return HTX_OK;
+    }
     HIP_TRY(launch_max_speed(h->dA, h->nx, h->ny, h->pitch, h->fstride, h->p.gamma, h->p.ref_quirks ? 1 : 0,
                              h->ctl, h->stream));
     return HTX_OK;
@@ -289,12 +321,14 @@ int htx_create(const htx_params* p, htx_solver** out) {
     htx_solver* h = new htx_solver();
     h->p = *p;
     h->nx = p->nx;
-    h->ny = p->ny;
-    h->nfield = p->scheme == HTX_SCHEME_HTENO ? 12 : 4;
+    h->ny = p->dim == 1 ? 1 : p->ny;
+    h->nfield = p->dim == 1 ? 3 : (p->scheme == HTX_SCHEME_HTENO ? 12 : 4);
     h->pitch = p->nx;
-    h->fstride = (long long)h->pitch * p->ny;
+    h->fstride = (long long)h->pitch * h->ny;
     h->dx = (p->xr - p->xl) / p->nx;
-    h->dy = (p->yt - p->yb) / p->ny;
+    // 1D: dt = cfl dx / max speed (FV_System1D.cpp:277) through the 2D rule
+    // cfl min(dx, dy) / max speed with dy = dx
+    h->dy = p->dim == 1 ? h->dx : (p->yt - p->yb) / p->ny;
     fill_quad_consts(h->qc);
     fill_herm_consts(h->hc, h->dx);  // Hermite::setGridSize(dx), FVHermite_System2D.cpp:28
     size_t bytes = sizeof(double) * (size_t)h->fstride * h->nfield;
@@ -386,6 +420,7 @@ int htx_get_state(htx_solver* h, double* q, size_t ld) {
 
 int htx_init_vortex(htx_solver* h, double cx, double cy, const double* modes, int nmodes) {
     if (!h) return fail(HTX_EINVAL, "null handle");
+    if (h->p.dim != 2) return fail(HTX_ENOTSUP, "the vortex is a 2D initial condition");
     if (nmodes < 0 || nmodes > 16 || (nmodes > 0 && !modes)) return fail(HTX_EINVAL, "nmodes must be 0..16");
     VortexArgs a{};
     a.q = h->dA;
@@ -444,6 +479,7 @@ int finish_ic(htx_solver* h) {
 
 int htx_init_quadrant(htx_solver* h, double xs, double ys, const double* states) {
     if (!h || !states) return fail(HTX_EINVAL, "null argument");
+    if (h->p.dim != 2) return fail(HTX_ENOTSUP, "quadrant data is a 2D initial condition");
     PiecewiseArgs a = piecewise_args(h, IC_QUADRANT);
     a.xs = xs;
     a.ys = ys;
@@ -456,6 +492,7 @@ int htx_init_quadrant(htx_solver* h, double xs, double ys, const double* states)
 int htx_init_shock_blobs(htx_solver* h, double xs, double ys, double nx_, double ny_, const double* post,
                          const double* ambient, const double* blobs, int nblob) {
     if (!h || !post || !ambient || (nblob > 0 && !blobs)) return fail(HTX_EINVAL, "null argument");
+    if (h->p.dim != 2) return fail(HTX_ENOTSUP, "the shock and blobs are a 2D initial condition");
     if (nblob < 0 || nblob > IC_MAX_BLOBS) return fail(HTX_EINVAL, "nblob must be in [0, 64]");
     PiecewiseArgs a = piecewise_args(h, IC_SHOCK_BLOBS);
     a.xs = xs;
@@ -780,7 +817,8 @@ void htx_trainer_destroy(htx_trainer* t) {
 int htx_trainer_add_state(htx_trainer* t, htx_solver* h) {
     if (!t || !h) return fail(HTX_EINVAL, "null handle");
     if (!h->have_state) return fail(HTX_ESTATE, "the solver has no state");
-    if (h->p.scheme != HTX_SCHEME_TENO5) return fail(HTX_ENOTSUP, "training faces come from the TENO5 scheme");
+    if (h->p.scheme != HTX_SCHEME_TENO5 || h->p.dim != 2)
+        return fail(HTX_ENOTSUP, "training faces come from the 2D TENO5 scheme");
     long long nf = 2ll * h->nx * h->ny;
     if (t->n + nf > t->capacity) return fail(HTX_ENOMEM, "training set capacity exceeded");
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1070,7 +1108,8 @@ int htx_classify(htx_solver* h, int dir, float* logits, float* feat) {
     if (dir != 0 && dir != 1) return fail(HTX_EINVAL, "dir must be 0 (x faces) or 1 (y faces)");
     if (!h->have_classifier) return fail(HTX_ESTATE, "no classifier loaded");
     if (!h->have_state) return fail(HTX_ESTATE, "the solver has no state");
-    if (h->p.scheme != HTX_SCHEME_TENO5) return fail(HTX_ENOTSUP, "the classifier runs on the TENO5 scheme");
+    if (h->p.scheme != HTX_SCHEME_TENO5 || h->p.dim != 2)
+        return fail(HTX_ENOTSUP, "the classifier runs on the 2D TENO5 scheme");
     size_t nf = (size_t)h->nx * h->ny;
     float *dL = nullptr, *dF = nullptr;
     if (hipMalloc(&dL, sizeof(float) * nf) != hipSuccess ||

@@ -1,0 +1,234 @@
+// htx_fv1d.h -- the 1D Euler path (FV::System1D, FV_System1D.cpp:121-292;
+// Equations::Euler1D, Equations_Euler1D.cpp): per-face bodies shared by the
+// GPU kernel and its CPU build.  Every operation is one IEEE operation in the
+// reference's order (the strict build includes this file).
+#pragma once
+#include "htx_internal.h"
+
+namespace HTX_NS {
+
+struct Fv1dArgs {
+    const double* qin;  // state whose residual is taken (3 fields x nx)
+    const double* q0;   // u^n (stages 2 and 3)
+    double* qout;       // residual (mode 0) or the stage result
+    Ctl* ctl;
+    int nx, bc, mode;   // bc: 0 zero-gradient, 1 periodic
+    int recon;          // 0 teno5, 1 teno5i (the reference's shipped 1D scheme)
+    int flux;           // FLUX_RUSANOV_CHAR, FLUX_HLLC (quirks), FLUX_HLLC_TORO
+    double gam, g1, dx;
+};
+
+// Euler1D::primitive / flux (Equations_Euler1D.cpp:18-37)
+HTX_HD void e1_prim(double g1, const double q[3], double w[3]) {
+    HTX_NO_CONTRACT
+    w[0] = q[0];
+    w[1] = q[1] / q[0];
+    w[2] = (q[2] - 0.5 * q[1] * w[1]) * g1;
+}
+
+HTX_HD void e1_phys_flux(const double q[3], const double w[3], double f[3]) {
+    HTX_NO_CONTRACT
+    f[0] = q[1];
+    f[1] = q[1] * w[1] + w[2];
+    f[2] = w[1] * (q[2] + w[2]);
+}
+
+// 3x3 product in the row-by-row, left-to-right order of the reference's
+// fixed-size products
+HTX_HD void e1_mv(const double M[9], const double x[3], double y[3]) {
+    HTX_NO_CONTRACT
+#pragma unroll
+    for (int r = 0; r < 3; ++r) y[r] = (M[3 * r] * x[0] + M[3 * r + 1] * x[1]) + M[3 * r + 2] * x[2];
+}
+
+// FV::System1D::eigen1DEuler (FV_System1D.cpp:121-156): Roe average of
+// cells a, b and the left / right eigenvector matrices (row-major)
+HTX_HD void e1_eigen(double g1, const double qa[3], const double qb[3], double L[9], double R[9]) {
+    HTX_NO_CONTRACT
+    double wa[3], wb[3];
+    e1_prim(g1, qa, wa);
+    e1_prim(g1, qb, wb);
+    const double sa = sqrt(fabs(wa[0])), sb = sqrt(fabs(wb[0]));
+    const double ha = (qa[2] + wa[2]) / wa[0], hb = (qb[2] + wb[2]) / wb[0];
+    const double t0 = sa / (sa + sb), t1 = 1.0 - t0;
+    const double vxm = t0 * wa[1] + t1 * wb[1];
+    const double hm = t0 * ha + t1 * hb;
+    const double qm = 0.5 * vxm * vxm;
+    const double cm = sqrt(fabs(g1 * (hm - qm)));
+    const double tc = vxm * cm;
+    R[0] = 1.0; R[1] = 1.0; R[2] = 1.0;
+    R[3] = vxm - cm; R[4] = vxm; R[5] = vxm + cm;
+    R[6] = hm - tc; R[7] = qm; R[8] = hm + tc;
+    const double rcm = 1.0 / cm, b1 = g1 * rcm * rcm, b2 = qm * b1, tr = vxm * rcm, tb = b1 * vxm, t2 = 0.5 * b1;
+    L[0] = 0.5 * (b2 + tr); L[1] = -0.5 * (tb + rcm); L[2] = t2;
+    L[3] = 1.0 - b2; L[4] = tb; L[5] = -b1;
+    L[6] = 0.5 * (b2 - tr); L[7] = -0.5 * (tb - rcm); L[8] = t2;
+}
+
+// Euler1D::HLLC (Equations_Euler1D.cpp:156-210); QUIRKS: the sound speeds
+// without gamma (:174-175) and p_L in the right star energy (:204)
+template <bool QUIRKS>
+HTX_HD void e1_hllc(double gam, double g1, const double ql[3], const double qr[3], double f[3]) {
+    HTX_NO_CONTRACT
+    double wl[3], wr[3], fl[3], fr[3], qs[3];
+    e1_prim(g1, ql, wl);
+    e1_prim(g1, qr, wr);
+    const double hl = (ql[2] + wl[2]) / wl[0], hr = (qr[2] + wr[2]) / wr[0];
+    const double rr = sqrt(fabs(wr[0] / wl[0]));
+    const double rrr = 1.0 / (1.0 + rr);
+    const double vm = (wl[1] + wr[1] * rr) * rrr;
+    const double qm = 0.5 * vm * vm;
+    const double hm = (hl + hr * rr) * rrr;
+    const double cm = sqrt(fabs(g1 * (hm - qm)));
+    const double cl = QUIRKS ? sqrt(fabs(wl[2] / wl[0])) : sqrt(fabs(gam * wl[2] / wl[0]));
+    const double cr = QUIRKS ? sqrt(fabs(wr[2] / wr[0])) : sqrt(fabs(gam * wr[2] / wr[0]));
+    const double sl = fmin(wl[1] - cl, vm - cm);
+    const double sr = fmax(wr[1] + cr, vm + cm);
+    const double sm = (qr[1] * (sr - wr[1]) - ql[1] * (sl - wl[1]) + wl[2] - wr[2]) /
+                      (wr[0] * (sr - wr[1]) - wl[0] * (sl - wl[1]));
+    const double ps = wl[2] + wl[0] * (wl[1] - sl) * (wl[1] - sm);
+    e1_phys_flux(ql, wl, fl);
+    e1_phys_flux(qr, wr, fr);
+    if (sl > 0.0) {
+        for (int k = 0; k < 3; ++k) f[k] = fl[k];
+    } else if (sl <= 0.0 && sm > 0.0) {
+        const double nom = sl - wl[1], den = sl - sm;
+        qs[0] = wl[0] * nom / den;
+        qs[1] = (nom * ql[1] + ps - wl[2]) / den;
+        qs[2] = (nom * ql[2] - wl[2] * wl[1] + ps * sm) / den;
+        for (int k = 0; k < 3; ++k) f[k] = fl[k] + sl * (qs[k] - ql[k]);
+    } else if (sm <= 0.0 && sr >= 0.0) {
+        const double nom = sr - wr[1], den = sr - sm;
+        qs[0] = wr[0] * nom / den;
+        qs[1] = (nom * qr[1] + ps - wr[2]) / den;
+        qs[2] = (nom * qr[2] - (QUIRKS ? wl[2] : wr[2]) * wr[1] + ps * sm) / den;
+        for (int k = 0; k < 3; ++k) f[k] = fr[k] + sr * (qs[k] - qr[k]);
+    } else {
+        for (int k = 0; k < 3; ++k) f[k] = fr[k];
+    }
+}
+
+// Euler1D::Rusanov with maxLocalWaveSpeed (FV_System1D.cpp:218-222):
+// 0.5 (f_L + f_R - R diag(a) L (q_R - q_L)), a per characteristic field
+HTX_HD void e1_rusanov_char(double gam, double g1, const double L[9], const double R[9], const double ql[3],
+                            const double qr[3], double f[3]) {
+    HTX_NO_CONTRACT
+    double wl[3], wr[3], fl[3], fr[3], a[3], dd[3], ld[3], rd[3];
+    e1_prim(g1, ql, wl);
+    e1_prim(g1, qr, wr);
+    const double cl = sqrt(fabs(gam * wl[2] / wl[0])), cr = sqrt(fabs(gam * wr[2] / wr[0]));
+    a[0] = fmax(fabs(wl[1] - cl), fabs(wr[1] - cr));
+    a[1] = fmax(fabs(wl[1]), fabs(wr[1]));
+    a[2] = fmax(fabs(wl[1] + cl), fabs(wr[1] + cr));
+    e1_phys_flux(ql, wl, fl);
+    e1_phys_flux(qr, wr, fr);
+    for (int e = 0; e < 3; ++e) dd[e] = qr[e] - ql[e];
+    e1_mv(L, dd, ld);
+    for (int e = 0; e < 3; ++e) ld[e] = a[e] * ld[e];
+    e1_mv(R, ld, rd);
+    for (int e = 0; e < 3; ++e) f[e] = 0.5 * ((fl[e] + fr[e]) - rd[e]);
+}
+
+// Lagrange::teno5i (Schemes_Lagrange.cpp:1235-1263): teno5 with tau the
+// largest smoothness measure of the three candidates
+HTX_HD double teno5i_side(double v0, double v1, double v2, double v3, double v4, double p0, double p1, double p2,
+                          double d0, double d1, double d2) {
+    HTX_NO_CONTRACT
+    const double c1 = 13.0 / 12.0;
+    double c_0 = v2 - 2.0 * v3 + v4, b_0 = (3.0 * v2 - 4.0 * v3 + v4) / 2.0;
+    double c_1 = v1 - 2.0 * v2 + v3, b_1 = (v1 - v3) / 2.0;
+    double c_2 = v0 - 2.0 * v1 + v2, b_2 = (v0 - 4.0 * v1 + 3.0 * v2) / 2.0;
+    double s0 = b_0 * b_0 + c1 * c_0 * c_0;
+    double s1 = b_1 * b_1 + c1 * c_1 * c_1;
+    double s2 = b_2 * b_2 + c1 * c_2 * c_2;
+    double tau = s0;
+    if (s1 > tau) tau = s1;
+    if (s2 > tau) tau = s2;
+    double g0 = 1.0 + tau / (s0 + 1e-40);
+    double g1 = 1.0 + tau / (s1 + 1e-40);
+    double g2 = 1.0 + tau / (s2 + 1e-40);
+    g0 = g0 * g0 * g0; g0 = g0 * g0;
+    g1 = g1 * g1 * g1; g1 = g1 * g1;
+    g2 = g2 * g2 * g2; g2 = g2 * g2;
+    double gs = (g0 + g1) + g2;
+    bool k0 = !(g0 / gs < 5e-4), k1 = !(g1 / gs < 5e-4), k2 = !(g2 / gs < 5e-4);
+    double w0 = k0 ? d0 : 0.0, w1 = k1 ? d1 : 0.0, w2 = k2 ? d2 : 0.0;
+    double ws = (w0 + w1) + w2;
+    w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
+    double out = (p0 * w0 + p1 * w1) + p2 * w2;
+    return out != out ? 1e-15 : out;
+}
+
+HTX_HD void teno5i(const double u[6], double& um, double& up) {
+    HTX_NO_CONTRACT
+    const double a = 1.0 / 3.0, b = 5.0 / 6.0, c = -1.0 / 6.0, d = 11.0 / 6.0, e = -7.0 / 6.0;
+    double p0 = (a * u[2] + b * u[3]) + c * u[4];
+    double p1 = (c * u[1] + b * u[2]) + a * u[3];
+    double p2 = (a * u[0] + e * u[1]) + d * u[2];
+    um = teno5i_side(u[0], u[1], u[2], u[3], u[4], p0, p1, p2, 0.3, 0.6, 0.1);
+    double q0 = (d * u[3] + e * u[4]) + a * u[5];
+    double q1 = (a * u[2] + b * u[3]) + c * u[4];
+    double q2 = (c * u[1] + b * u[2]) + a * u[3];
+    up = teno5i_side(u[1], u[2], u[3], u[4], u[5], q0, q1, q2, 0.1, 0.6, 0.3);
+}
+
+// Flux through the face between cells c and c + 1 (c = -1 .. nx - 1):
+// characteristic reconstruction (FV_System1D.cpp:194-208), back to
+// conservative variables (FV_System1D.h:48-51), and the flux (:210-223).
+HTX_HD void fv1d_face_flux(const Fv1dArgs& a, int c, double F[3]) {
+    HTX_NO_CONTRACT
+    double qa[3], qb[3], L[9], R[9];
+    const int ia = map_index(c, a.nx, a.bc), ib = map_index(c + 1, a.nx, a.bc);
+    for (int e = 0; e < 3; ++e) {
+        qa[e] = a.qin[e * a.nx + ia];
+        qb[e] = a.qin[e * a.nx + ib];
+    }
+    e1_eigen(a.g1, qa, qb, L, R);
+    double C[3][6];
+    for (int m = 0; m < 6; ++m) {
+        const int im = map_index(c - 2 + m, a.nx, a.bc);
+        double cc[3], y[3];
+        for (int e = 0; e < 3; ++e) cc[e] = a.qin[e * a.nx + im];
+        e1_mv(L, cc, y);
+        for (int e = 0; e < 3; ++e) C[e][m] = y[e];
+    }
+    double ul[3], ur[3];
+    for (int e = 0; e < 3; ++e) {
+        if (a.recon == 1)
+            teno5i(C[e], ul[e], ur[e]);
+        else
+            teno5(C[e], ul[e], ur[e]);
+    }
+    double ql[3], qr[3];
+    e1_mv(R, ul, ql);
+    e1_mv(R, ur, qr);
+    if (a.flux == FLUX_HLLC)
+        e1_hllc<true>(a.gam, a.g1, ql, qr, F);
+    else if (a.flux == FLUX_HLLC_TORO)
+        e1_hllc<false>(a.gam, a.g1, ql, qr, F);
+    else
+        e1_rusanov_char(a.gam, a.g1, L, R, ql, qr, F);
+}
+
+// residual (FV_System1D.cpp:225-227) and SSP-RK3 stage (:230-262 with
+// Schemes_SSPRK.cpp:3-15) of cell c from the fluxes of its faces
+HTX_HD double fv1d_stage_value(const Fv1dArgs& a, int c, int e, double fm, double fp, double dt) {
+    HTX_NO_CONTRACT
+    const double rhs = (fm - fp) / a.dx;
+    const double us = a.qin[e * a.nx + c];
+    if (a.mode == 0) return rhs;
+    if (a.mode == 1) return us + dt * rhs;
+    if (a.mode == 2) return 0.75 * a.q0[e * a.nx + c] + 0.25 * (us + rhs * dt);
+    return (a.q0[e * a.nx + c] + 2.0 * (us + dt * rhs)) / 3.0;
+}
+
+// Euler1D::maxSpeed (Equations_Euler1D.cpp:91-94) of a conservative state
+HTX_HD double fv1d_max_speed(double gam, double g1, const double q[3]) {
+    HTX_NO_CONTRACT
+    double w[3];
+    e1_prim(g1, q, w);
+    const double c = sqrt(fabs(gam * w[2] / w[0]));
+    return fmax(fabs(w[1] - c), fabs(w[1] + c));
+}
+
+}  // namespace HTX_NS

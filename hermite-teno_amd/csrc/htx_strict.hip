@@ -9,6 +9,7 @@
 #define HTX_EXACT_DIVISION 1
 #include "htx_fv2d.hip"
 #include "htx_fvh2d.hip"
+#include "htx_fv1d.hip"
 
 // entry points for htx_api.hip, which only sees the fast build's types (the
 // argument structs have the same layout in both builds)
@@ -18,4 +19,14 @@ hipError_t htx_strict_launch_fv2d_stage(const void* a, int flux, int tile_x, int
 
 hipError_t htx_strict_launch_fvh2d_stage(const void* a, int flux, hipStream_t s) {
     return htx_strict::launch_fvh2d_stage(*static_cast<const htx_strict::FvhArgs*>(a), flux, s);
+}
+
+// the 1D path (built here only); a points to an Fv1dArgs, whose layout does
+// not depend on the build
+hipError_t htx_strict_launch_fv1d_stage(const void* a, hipStream_t s) {
+    return htx_strict::launch_fv1d_stage(*static_cast<const htx_strict::Fv1dArgs*>(a), s);
+}
+
+hipError_t htx_strict_launch_fv1d_max_speed(const double* q, int nx, double gam, void* ctl, hipStream_t s) {
+    return htx_strict::launch_fv1d_max_speed(q, nx, gam, static_cast<htx_strict::Ctl*>(ctl), s);
 }

@@ -1,7 +1,7 @@
 // main.cpp-style driver: the reference's driver (main.cpp:1-176) selects one
 // case by commenting code; this one takes the case name as argv[1]
 // (vortex, entropy, riemann on FV::System2D; hvortex, hentropy, hriemann on
-// FVHermite::System2D).  It
+// FVHermite::System2D; sod, entropy1d on FV::System1D).  It
 // compiles against include/ and links libhtx.so; the class names and calls
 // are the reference's.
 #include <cstdio>
@@ -9,6 +9,7 @@
 #include <string>
 
 #include "FVHermite_System2D.h"
+#include "FV_System1D.h"
 #include "FV_System2D.h"
 
 int main(int argc, char** argv) {
@@ -56,6 +57,23 @@ int main(int argc, char** argv) {
             // reference main.cpp:122-125 (commented case), configuration 5
             FVHermite::System2D systemObj(0.0, 1.0, n, 0.0, 1.0, n, 0.4, 0.02, 0.0, 1.4);
             systemObj.Riemann2D();
+            systemObj.timeStepping();
+            systemObj.printResult();
+            std::printf("steps %d t %.17g\n", systemObj.steps(), systemObj.time());
+        } else if (which == "sod") {
+            // reference main.cpp:55-58 form (commented case) as the 1D
+            // configuration: Sod, TENO5 + HLLC, CFL 0.5 to t = 0.2
+            htx::Options1D o;
+            o.scheme = HTX_SCHEME_TENO5;
+            o.flux = HTX_FLUX_HLLC;
+            FV::System1D systemObj(0.0, 1.0, n, 0.5, 0.2, 0.0, 1.4, o);
+            systemObj.sodShock();
+            systemObj.timeStepping();
+            systemObj.printResult();
+            std::printf("steps %d t %.17g\n", systemObj.steps(), systemObj.time());
+        } else if (which == "entropy1d") {
+            FV::System1D systemObj(-1.0, 1.0, n, 0.4, 0.5, 0.0, 1.4);
+            systemObj.entropyWave();
             systemObj.timeStepping();
             systemObj.printResult();
             std::printf("steps %d t %.17g\n", systemObj.steps(), systemObj.time());

@@ -5,6 +5,7 @@ package is the Python-side mirror of the reference's solver interface.
 """
 from ._lib import HtxError, load  # noqa: F401
 from .solver import System2D  # noqa: F401
+from .solver1d import System1D  # noqa: F401
 from . import ic  # noqa: F401
 
-__all__ = ["System2D", "HtxError", "load", "ic"]
+__all__ = ["System2D", "System1D", "HtxError", "load", "ic"]

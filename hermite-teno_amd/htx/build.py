@@ -87,6 +87,7 @@ HOST_HARNESS = os.path.join(ROOT, "tests", "host", "libfv2d_host.so")
 HOST_HARNESS_FAST = os.path.join(ROOT, "tests", "host", "libfv2d_host_fast.so")
 HOST_HARNESS_HERMITE = os.path.join(ROOT, "tests", "host", "libfvh2d_host.so")
 HOST_HARNESS_HERMITE_FAST = os.path.join(ROOT, "tests", "host", "libfvh2d_host_fast.so")
+HOST_HARNESS_1D = os.path.join(ROOT, "tests", "host", "libfv1d_host.so")
 HOST_HARNESS_BATCH = os.path.join(ROOT, "tests", "host", "libbatch_host.so")
 HOST_HARNESS_CONTRACT = os.path.join(ROOT, "tests", "host", "libfv2d_host_contract.so")
 
@@ -104,6 +105,7 @@ def build_host_harness():
     srch = os.path.join(ROOT, "tests", "host", "fvh2d_host.hip")
     _run(base + ["-DHTX_EXACT_DIVISION=1", "-o", HOST_HARNESS_HERMITE, srch])
     _run(base + ["-o", HOST_HARNESS_HERMITE_FAST, srch])
+    _run(base + ["-DHTX_EXACT_DIVISION=1", "-o", HOST_HARNESS_1D, os.path.join(ROOT, "tests", "host", "fv1d_host.hip")])
     # the product's contraction setting, with hardware FMA on the host: the
     # learned path's features must not depend on it
     _run([hipcc, "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=fast-honor-pragmas", "-mfma",

@@ -41,7 +41,9 @@ enum htx_error {
     HTX_EIO = -6       /* snapshot file missing, unreadable, truncated or corrupt */
 };
 
-enum htx_scheme { HTX_SCHEME_TENO5 = 0, HTX_SCHEME_HTENO = 1 };
+/* HTX_SCHEME_TENO5I: Lagrange::teno5i (Schemes_Lagrange.cpp:1235-1263), the
+ * reconstruction FV::System1D ships with; 1D only. */
+enum htx_scheme { HTX_SCHEME_TENO5 = 0, HTX_SCHEME_HTENO = 1, HTX_SCHEME_TENO5I = 2 };
 enum htx_flux { HTX_FLUX_RUSANOV_CHAR = 0, HTX_FLUX_HLLC = 1 };
 enum htx_bc { HTX_BC_ZERO_GRAD = 0, HTX_BC_PERIODIC = 1 };
 /* HTX_PREC_FP64: the drop-in; every operation is one IEEE operation in the
@@ -64,7 +66,8 @@ enum htx_precision { HTX_PREC_FP64 = 0, HTX_PREC_FP32_MIXED = 1, HTX_PREC_LEARNE
 
 typedef struct htx_params {
     int struct_size;    /* sizeof(htx_params), for ABI checks */
-    int dim;            /* 2 */
+    int dim;            /* 2, or 1 for FV::System1D (nx cells, 3 fields rho, rho u, E;
+                           yb, yt, ny ignored; scheme TENO5 or TENO5I; FP64 only) */
     double xl, xr;      /* ctor (xl, xr, nx, ...), FV_System2D.h:12-15 */
     int nx;
     double yb, yt;
@@ -85,9 +88,14 @@ typedef struct htx_solver htx_solver;
 int htx_params_default(htx_params* p);
 
 /* FV::System2D::System2D / FVHermite::System2D::System2D (FV_System2D.cpp:3-47,
- * FVHermite_System2D.cpp:3-50): validates the parameters (a zero grid size is
- * HTX_EINVAL where the reference calls exit(1), Schemes_Hermite.cpp:16-25),
- * allocates the device state and sets t = tinit. */
+ * FVHermite_System2D.cpp:3-50), and FV::System1D::System1D
+ * (FV_System1D.cpp:3-33) for dim = 1: validates the parameters (a zero grid
+ * size is HTX_EINVAL where the reference calls exit(1),
+ * Schemes_Hermite.cpp:16-25), allocates the device state and sets t = tinit.
+ * The 1D state is 3 rows of nx (ld >= nx) in htx_set_state / htx_get_state /
+ * htx_rhs; its stages are FV::System1D::rhs + RungeKutta3
+ * (FV_System1D.cpp:158-262) and its dt is cfl dx / max |u -+ c|
+ * (:270-278). */
 int htx_create(const htx_params* p, htx_solver** out);
 
 /* Releases every device resource of the handle.  Null is ignored. */

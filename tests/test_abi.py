@@ -113,3 +113,29 @@ def test_hermite_rejects_learned_precision():
         Batch(ps)
     assert ei.value.code == -5
     assert "share flux and precision" in str(ei.value)
+
+
+def test_one_dimensional_parameters():
+    """dim = 1 (FV::System1D): FP64 only, TENO5 or TENO5i; TENO5i is refused
+    in 2D; dim other than 1 and 2 is a bad argument.  All checked before
+    any device work."""
+    import htx
+    from htx import _lib
+
+    lib = _lib.load()
+    import ctypes
+
+    def create(**kw):
+        p = _lib.HtxParams()
+        assert lib.htx_params_default(ctypes.byref(p)) == 0
+        for k, v in kw.items():
+            setattr(p, k, v)
+        h = ctypes.c_void_p()
+        return lib.htx_create(ctypes.byref(p), ctypes.byref(h))
+
+    assert create(dim=1, precision=_lib.PRECISIONS["fp32_mixed"]) == -5
+    assert create(dim=1, scheme=_lib.SCHEMES["hteno"]) == -1
+    assert create(dim=2, scheme=_lib.SCHEMES["teno5i"]) == -5
+    assert create(dim=3) == -1
+    with pytest.raises(htx.HtxError):
+        htx.System1D(0.0, 1.0, 0, 0.5, 0.2, 0.0, 1.4)

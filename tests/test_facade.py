@@ -43,6 +43,16 @@ def test_hermite_facade_signatures_match_reference():
         assert m in txt
 
 
+def test_system1d_facade_signatures_match_reference():
+    """Constructor argument list and public methods of FV::System1D
+    (FV_System1D.h:12-26)."""
+    txt = " ".join(open(os.path.join(ROOT, "include", "FV_System1D.h")).read().split())
+    assert ("System1D(const double& xl, const double& xr, const int& nx, const double& cflnumber, "
+            "const double& tmax, const double& tInit, const double& specificHeatRatio") in txt
+    for m in ["void entropyWave()", "void blastWave()", "void timeStepping()", "void printResult()"]:
+        assert m in txt
+
+
 def test_host_ic_matches_oracle_vortex(orc, tmp_path):
     """htx_ic.h (the facade's host initial conditions) equals the oracle's
     vortex cell averages bit for bit."""
@@ -71,7 +81,8 @@ int main() {
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["vortex", "entropy", "riemann", "hvortex", "hentropy", "hriemann"])
+@pytest.mark.parametrize("case", ["vortex", "entropy", "riemann", "hvortex", "hentropy", "hriemann", "sod",
+                                  "entropy1d"])
 def test_driver_runs_reference_cases(case):
     if not os.path.exists(EXE):
         pytest.skip("driver not built")
@@ -79,7 +90,7 @@ def test_driver_runs_reference_cases(case):
     r = subprocess.run([EXE, case, "32"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "steps" in r.stdout
-    if case not in ("riemann", "hriemann"):
+    if case not in ("riemann", "hriemann", "sod"):
         err = float(r.stdout.split("Numerical error:")[1].split()[0])
         assert np.isfinite(err) and err < 0.1
 
