@@ -18,6 +18,11 @@
  * Toro HLLC on exact Riemann data with quirks off, conservation with periodic
  * boundaries, free-stream preservation, mirror and x<->y transpose symmetry,
  * SSP-RK3 amplification polynomial) and brute-force CFL on tiny grids.
+ *
+ * Parity unpinned: e1_hllc's right-star energy quirk (p_L in E*_R,
+ * Equations_Euler1D.cpp:204) -- no check reaches that branch; and the
+ * composed trajectories as the reference compiles them (Eigen's evaluation
+ * order), which only a build of the reference could pin.
  */
 #include "htx_oracle.h"
 
