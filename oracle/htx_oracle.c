@@ -19,10 +19,9 @@
  * boundaries, free-stream preservation, mirror and x<->y transpose symmetry,
  * SSP-RK3 amplification polynomial) and brute-force CFL on tiny grids.
  *
- * Parity unpinned: e1_hllc's right-star energy quirk (p_L in E*_R,
- * Equations_Euler1D.cpp:204) -- no check reaches that branch; and the
- * composed trajectories as the reference compiles them (Eigen's evaluation
- * order), which only a build of the reference could pin.
+ * Parity unpinned: the composed trajectories as the reference compiles them
+ * (Eigen's evaluation order), which only a build of the reference could
+ * pin.
  */
 #include "htx_oracle.h"
 
@@ -1404,6 +1403,10 @@ static void e1_hllc(double gamma, int quirks, const double ql[3], const double q
     } else {
         memcpy(f, fr, sizeof fr);
     }
+}
+
+void orc_e1_hllc(double gamma, int quirks, const double ql[3], const double qr[3], double f[3]) {
+    e1_hllc(gamma, quirks, ql, qr, f);
 }
 
 /* Lagrange::teno5i, Schemes_Lagrange.cpp:1235-1263: tau = max(beta) */

@@ -57,6 +57,7 @@ _SIG = {
     "orc_fvh2d_advance": (_I, [ctypes.POINTER(OrcParams), _DP, _D, _D, _D, _I, _DP]),
     "orc_ic_vortex_hermite": (None, [ctypes.POINTER(OrcParams), _D, _D, _DP, _I, _DP]),
     "orc_teno5i": (None, [_DP, _DP]),
+    "orc_e1_hllc": (None, [_D, _I, _DP, _DP, _DP]),
     "orc_fv1d_rhs": (None, [_I, _D, _D, _D, _I, _I, _I, _DP, _DP]),
     "orc_fv1d_advance": (_I, [_I, _D, _D, _D, _I, _I, _I, _DP, _D, _D, _D, _I, _DP]),
     "orc_ic_sod": (None, [_I, _D, _D, _D, _DP]),
@@ -282,6 +283,13 @@ class Oracle:
         out = np.empty(2)
         self.lib.orc_teno5i(pu, out.ctypes.data_as(_DP))
         return out
+
+    def e1_hllc(self, gamma, ql, qr, quirks=True):
+        ql, pl = _p(ql)
+        qr, pr = _p(qr)
+        f = np.empty(3)
+        self.lib.orc_e1_hllc(gamma, 1 if quirks else 0, pl, pr, f.ctypes.data_as(_DP))
+        return f
 
     def fv1d_rhs(self, nx, xl, xr, gamma, q, recon="teno5", flux="hllc", quirks=True):
         q, pq = _p(q)

@@ -307,3 +307,31 @@ def test_teno5i_tau_is_max_beta(orc):
     np.testing.assert_allclose(orc.teno5i(u), orc.linear5(u), atol=1e-15)
     jump = [1.0, 1.0, 1.0, 1.0, 0.0, 0.0]
     assert orc.teno5i(jump)[0] == pytest.approx(1.0, abs=1e-14)
+
+
+def test_hllc_1d_quirk_energy_term(orc):
+    """1D HLLC (Equations_Euler1D.cpp:156-210) in the right star region with
+    Roe-dominated wave speeds (S_L = u~ - c~, S_R = u~ + c~ in both modes, so
+    the c-without-gamma quirk of :174-175 does not act): the modes agree in
+    mass and momentum and differ in energy flux by exactly
+    S_R u_R (p_R - p_L) / (S_R - S*), the p_L substitution of :204.  S_R is
+    the Roe speed and S* is recovered from the mass flux."""
+    wl, wr = [1.0, 1.0, 1.0], [1.0, -1.0, 1.3]
+
+    def cons(w):
+        return np.array([w[0], w[0] * w[1], w[2] / (GAMMA - 1) + 0.5 * w[0] * w[1] * w[1]])
+
+    ql, qr = cons(wl), cons(wr)
+    a, b = orc.e1_hllc(GAMMA, ql, qr, True), orc.e1_hllc(GAMMA, ql, qr, False)
+    np.testing.assert_array_equal(a[:2], b[:2])
+    rr = math.sqrt(wr[0] / wl[0])
+    vm = (wl[1] + wr[1] * rr) / (1 + rr)
+    hl, hr = (ql[2] + wl[2]) / wl[0], (qr[2] + wr[2]) / wr[0]
+    hm = (hl + hr * rr) / (1 + rr)
+    cm = math.sqrt((GAMMA - 1) * (hm - 0.5 * vm * vm))
+    sr = vm + cm
+    assert sr > wr[1] + math.sqrt(GAMMA * wr[2] / wr[0]) and vm - cm < wl[1] - math.sqrt(GAMMA * wl[2] / wl[0])
+    k = (b[0] - wr[0] * wr[1]) / sr + wr[0]
+    sm = sr - wr[0] * (sr - wr[1]) / k
+    assert sm < 0.0
+    assert a[2] - b[2] == pytest.approx(sr * wr[1] * (wr[2] - wl[2]) / (sr - sm), rel=1e-12)
