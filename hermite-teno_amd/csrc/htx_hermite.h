@@ -73,7 +73,7 @@ HTX_HD double hteno_side(double brk, double s0, double s1, double s2, double p0,
     w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
     double out = (p0 * w0 + p1 * w1) + p2 * w2;
 #else
-    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * (1.0 / ws);
+    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * drcp(ws);
 #endif
     return out != out ? 1e-15 : out;
 }
@@ -109,7 +109,7 @@ HTX_HD double hteno_side_mixed(double brk, double s0, double s1, double s2, doub
     }
     double w0 = k0 ? d0 : 0.0, w1 = k1 ? d1 : 0.0, w2 = k2 ? d2 : 0.0;
     double ws = (w0 + w1) + w2;
-    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * (1.0 / ws);
+    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * drcp(ws);
     return out != out ? 1e-15 : out;
 }
 
@@ -657,7 +657,7 @@ HTX_HD void drusanov(double gam, double g1, const Eig& E, const double ql[4], co
 #if HTX_EXACT_DIVISION
     double unl = ql[1] / ql[0], utl = ql[2] / ql[0], unr = qr[1] / qr[0], utr = qr[2] / qr[0];
 #else
-    double il = 1.0 / ql[0], ir = 1.0 / qr[0];
+    double il = drcp(ql[0]), ir = drcp(qr[0]);
     double unl = ql[1] * il, utl = ql[2] * il, unr = qr[1] * ir, utr = qr[2] * ir;
 #endif
     dflux_mul<YDIR>(gam, g1, ql[3], ql[0], YDIR ? utl : unl, YDIR ? unl : utl, pl, dl, jl);
@@ -724,7 +724,7 @@ HTX_HD void fvh_flux_at(const FvhArgs& a, int i, int j, int m, const double c[6]
 #if HTX_EXACT_DIVISION
     double cl = sqrt(fabs(a.gam * pl / rl)), cr = sqrt(fabs(a.gam * pr / rr));
 #else
-    double cl = sqrt(fabs(a.gam * pl * (1.0 / rl))), cr = sqrt(fabs(a.gam * pr * (1.0 / rr)));
+    double cl = sqrt(fabs(a.gam * pl * drcp(rl))), cr = sqrt(fabs(a.gam * pr * drcp(rr)));
 #endif
     double am[4];
     am[0] = fmax(fabs(wnl - cl), fabs(wnr - cr));

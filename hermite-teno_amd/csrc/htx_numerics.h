@@ -142,6 +142,16 @@ HTX_HD double drcp(double x) {
 #endif
 }
 
+// 1 / x where the reference divides: the correctly rounded quotient in the
+// strict build, drcp in the fast one
+HTX_HD double inv(double x) {
+#if HTX_EXACT_DIVISION
+    return 1.0 / x;
+#else
+    return drcp(x);
+#endif
+}
+
 HTX_HD float flog2(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_amdgcn_logf(x);
@@ -191,7 +201,7 @@ HTX_HD double teno5_side(double v0, double v1, double v2, double v3, double v4,
     w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
     double out = (p0 * w0 + p1 * w1) + p2 * w2;
 #else
-    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * (1.0 / ws);  // all cut: 0 * inf = NaN -> catchnan
+    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * drcp(ws);  // all cut: drcp(0) is NaN -> catchnan
 #endif
     if (cut) *cut = (k0 ? 0u : 1u) | (k1 ? 0u : 2u) | (k2 ? 0u : 4u);
     return out != out ? 1e-15 : out;  // catchnan
@@ -255,7 +265,7 @@ HTX_HD double teno5_side_mixed(double v0, double v1, double v2, double v3, doubl
     }
     double w0 = k0 ? d0 : 0.0, w1 = k1 ? d1 : 0.0, w2 = k2 ? d2 : 0.0;
     double ws = (w0 + w1) + w2;
-    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * (1.0 / ws);
+    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * drcp(ws);
     return out != out ? 1e-15 : out;
 }
 
@@ -353,7 +363,7 @@ HTX_HD void weno3q(const QuadConsts& C, double u0, double u1, double u2, double 
 #else
     // 3 reciprocals of (eps + beta_k)^2 shared by the four weight sets, one
     // reciprocal per normalisation
-    double r0 = 1.0 / e0, r1 = 1.0 / e1, r2 = 1.0 / e2;
+    double r0 = drcp(e0), r1 = drcp(e1), r2 = drcp(e2);
     // +xi
     p0 = (C.gp[0][0] * u0 + C.gp[1][0] * u1) + C.gp[2][0] * u2;
     p1 = (C.gp[0][1] * u1 + C.gp[1][1] * u2) + C.gp[2][1] * u3;
@@ -457,7 +467,7 @@ HTX_HD void weno3q_gp3(const QuadConsts& C, double u0, double u1, double u2, dou
     double e0 = 1e-6 + s0, e1 = 1e-6 + s1, e2 = 1e-6 + s2;
     e0 = e0 * e0; e1 = e1 * e1; e2 = e2 * e2;
 #if !HTX_EXACT_DIVISION
-    const double r0 = 1.0 / e0, r1 = 1.0 / e1, r2 = 1.0 / e2;
+    const double r0 = drcp(e0), r1 = drcp(e1), r2 = drcp(e2);
 #endif
     double w0, w1, w2, ws, p0, p1, p2, v;
     // centre
@@ -539,7 +549,7 @@ HTX_HD Eig roe_eig(double g1, bool ydir, const Prim& a, const Prim& b) {
     E.hm = t0 * a.h + t1 * b.h;
     E.qm = 0.5 * (vxm * vxm + vym * vym);
     E.cm = sqrt(fabs(g1 * (E.hm - E.qm)));
-    E.rcm = 1.0 / E.cm;
+    E.rcm = inv(E.cm);
     E.b1 = g1 * E.rcm * E.rcm;
     E.b2 = E.qm * E.b1;
     E.t2 = 0.5 * E.b1;
@@ -593,7 +603,7 @@ HTX_HD void phys_flux(double g1, double q0, double qn, double qt, double q3,
     wn = qn / q0;
     double wt = qt / q0;
 #else
-    double iq = 1.0 / q0;
+    double iq = drcp(q0);
     wn = qn * iq;
     double wt = qt * iq;
 #endif
@@ -617,8 +627,8 @@ HTX_HD void rusanov_char(double gam, double g1, const Eig& E, const double ql[4]
     double cl = sqrt(fabs(gam * pl / rl));
     double cr = sqrt(fabs(gam * pr / rr));
 #else
-    double cl = sqrt(fabs(gam * pl * (1.0 / rl)));
-    double cr = sqrt(fabs(gam * pr * (1.0 / rr)));
+    double cl = sqrt(fabs(gam * pl * drcp(rl)));
+    double cr = sqrt(fabs(gam * pr * drcp(rr)));
 #endif
     double a0 = fmax(fabs(wnl - cl), fabs(wnr - cr));
     double a1 = fmax(fabs(wnl), fabs(wnr));
@@ -654,7 +664,7 @@ HTX_HD void hllc(double gam, double g1, const double ql[4], const double qr[4],
     double cl = QUIRKS ? sqrt(fabs(pl / wl0)) : sqrt(fabs(gam * pl / wl0));
     double cr = QUIRKS ? sqrt(fabs(pr / wr0)) : sqrt(fabs(gam * pr / wr0));
 #else
-    double il = 1.0 / ql[0], ir = 1.0 / qr[0];
+    double il = drcp(ql[0]), ir = drcp(qr[0]);
     double wl0 = ql[0], wln = ql[1] * il, wlt = ql[2] * il;
     double wr0 = qr[0], wrn = qr[1] * ir, wrt = qr[2] * ir;
     double pl = (ql[3] - 0.5 * (ql[1] * wln + ql[2] * wlt)) * g1;
@@ -665,7 +675,7 @@ HTX_HD void hllc(double gam, double g1, const double ql[4], const double qr[4],
     double cl = QUIRKS ? sqrt(fabs(pl * il)) : sqrt(fabs(gam * pl * il));
     double cr = QUIRKS ? sqrt(fabs(pr * ir)) : sqrt(fabs(gam * pr * ir));
 #endif
-    double rrr = 1.0 / (1.0 + rho_ratio);
+    double rrr = inv(1.0 + rho_ratio);
     double vn = (wln + wrn * rho_ratio) * rrr;
     double vt = (wlt + wrt * rho_ratio) * rrr;
     double qm = 0.5 * (vn * vn + vt * vt);
@@ -689,7 +699,7 @@ HTX_HD void hllc(double gam, double g1, const double ql[4], const double qr[4],
         double st = nom * ql[2] / den;
         double s3 = (nom * ql[3] - pl * wln + ps * sm) / den;
 #else
-        double id = 1.0 / den;
+        double id = drcp(den);
         double s0 = wl0 * nom * id;
         double sn = (nom * ql[1] + ps - pl) * id;
         double st = nom * ql[2] * id;
@@ -708,7 +718,7 @@ HTX_HD void hllc(double gam, double g1, const double ql[4], const double qr[4],
         double st = nom * qr[2] / den;
         double s3 = (nom * qr[3] - pe * wrn + ps * sm) / den;
 #else
-        double id = 1.0 / den;
+        double id = drcp(den);
         double s0 = wr0 * nom * id;
         double sn = (nom * qr[1] + ps - pr) * id;
         double st = nom * qr[2] * id;
