@@ -364,7 +364,11 @@ HTX_HD void ph_epilogue(const Fv2dArgs& a, const Fv2dTile<TX, TY>& s, int i0, in
         double unew[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
+#if HTX_EXACT_DIVISION
             double rhs = (dxp[k][e] + (s.sFl[e][r * TX + c] - s.sFl[e][(r + 1) * TX + c]) * a.dx) / a.area;
+#else
+            double rhs = (dxp[k][e] + (s.sFl[e][r * TX + c] - s.sFl[e][(r + 1) * TX + c]) * a.dx) * drcp(a.area);
+#endif
             double us = s.sQ[e][lc];
             double v;
             if (a.mode == 0)
@@ -379,7 +383,12 @@ HTX_HD void ph_epilogue(const Fv2dArgs& a, const Fv2dTile<TX, TY>& s, int i0, in
             a.qout[off + e * fs] = v;
         }
         if (a.mode == 3) {
+#if HTX_EXACT_DIVISION
             double wu = unew[1] / unew[0], wv = unew[2] / unew[0];
+#else
+            const double ir = drcp(unew[0]);
+            double wu = unew[1] * ir, wv = unew[2] * ir;
+#endif
             double p = (unew[3] - 0.5 * (unew[1] * wu + unew[2] * wv)) * a.g1;
             if (a.quirks) {
                 smx = fmax(smx, max_speed<true>(a.gam, wu, p, unew[0]));

@@ -523,6 +523,8 @@ HTX_HD void weno3q_gp3(const QuadConsts& C, double u0, double u1, double u2, dou
 HTX_HD Prim cell_prim(double g1, double q0, double q1, double q2, double q3) {
     HTX_NO_CONTRACT
     Prim P;
+    // exact in both builds: these feed the eigenvectors, and through them the
+    // TENO cut-off decisions
     double u = q1 / q0, v = q2 / q0;
     double p = (q3 - 0.5 * (q1 * u + q2 * v)) * g1;
     P.sr = sqrt(fabs(q0));
@@ -549,7 +551,7 @@ HTX_HD Eig roe_eig(double g1, bool ydir, const Prim& a, const Prim& b) {
     E.hm = t0 * a.h + t1 * b.h;
     E.qm = 0.5 * (vxm * vxm + vym * vym);
     E.cm = sqrt(fabs(g1 * (E.hm - E.qm)));
-    E.rcm = inv(E.cm);
+    E.rcm = 1.0 / E.cm;  // exact in both builds: the eigenvectors feed the TENO decisions
     E.b1 = g1 * E.rcm * E.rcm;
     E.b2 = E.qm * E.b1;
     E.t2 = 0.5 * E.b1;
@@ -741,7 +743,11 @@ HTX_HD void hllc(double gam, double g1, const double ql[4], const double qr[4],
 // reference's fmax(0, .) accumulation does with them.
 template <bool QUIRKS>
 HTX_HD double max_speed(double gam, double wn, double p, double rho) {
+#if HTX_EXACT_DIVISION
     double c = sqrt(fabs(gam * p / rho));
+#else
+    double c = sqrt(fabs(gam * p * drcp(rho)));
+#endif
     double s = QUIRKS ? fmax(wn - c, fabs(wn + c)) : fmax(fabs(wn - c), fabs(wn + c));
     return s > 0.0 ? s : 0.0;
 }
