@@ -43,6 +43,13 @@ HTX_HD float gelu_f(float x) {
 }
 
 // (gelu(x), gelu'(x))
+// gelu_f with the polynomial in fused multiply-adds, for the matrix-core
+// inference (differs from gelu_f in the last bit at most)
+__device__ __forceinline__ float gelu_fma(float x) {
+    const float t = fminf(fmaxf(x, -GELU_X0), GELU_X0);
+    return x * fmaf(t, fmaf(GELU_B, t * t, GELU_A), 0.5f);
+}
+
 HTX_HD void gelu_fd(float x, float& g, float& dg) {
     float t = fminf(fmaxf(x, -GELU_X0), GELU_X0);
     float phi = 0.5f + t * (GELU_A + GELU_B * t * t);
@@ -288,7 +295,7 @@ __device__ __forceinline__ float cls_logit(const unsigned short* xs, const unsig
             cls_f4 d = {b1[0], b1[1], b1[2], b1[3]};
             d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F1[mb * 64 + lane], bx, d, 0, 0, 0);
 #pragma unroll
-            for (int c = 0; c < 4; ++c) b2[mb >> 1][(mb & 1) * 4 + c] = (__bf16)gelu_f(d[c]);
+            for (int c = 0; c < 4; ++c) b2[mb >> 1][(mb & 1) * 4 + c] = (__bf16)gelu_fma(d[c]);
         }
         float part = 0.0f;
 #pragma unroll
@@ -298,7 +305,7 @@ __device__ __forceinline__ float cls_logit(const unsigned short* xs, const unsig
             d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F2[(mb * 2 + 0) * 64 + lane], b2[0], d, 0, 0, 0);
             d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F2[(mb * 2 + 1) * 64 + lane], b2[1], d, 0, 0, 0);
 #pragma unroll
-            for (int c = 0; c < 4; ++c) part += cvec[2 * CLS_H + 16 * mb + 4 * g + c] * gelu_f(d[c]);
+            for (int c = 0; c < 4; ++c) part = fmaf(cvec[2 * CLS_H + 16 * mb + 4 * g + c], gelu_fma(d[c]), part);
         }
         part += __shfl_xor(part, 16, 64);
         part += __shfl_xor(part, 32, 64);

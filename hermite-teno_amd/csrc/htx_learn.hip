@@ -313,43 +313,95 @@ __global__ void __launch_bounds__(256) k_classify(LrnClassifyArgs a) {
     if (id < n) a.logits[id] = z;
 }
 
+// Work block of the face-flag kernel: FA faces along the stencil direction
+// by FB grid lines across it, with the primitive variables of the
+// (FA + 5) x FB cells their stencils read computed once into LDS.  In both
+// families 32 lanes run along x, the contiguous direction of the state.
+template <bool YDIR>
+struct FlagBlock {
+    static constexpr int FA = YDIR ? 8 : 32;
+    static constexpr int FB = YDIR ? 32 : 8;
+    static constexpr int NA = FA + 5;  // cells along: a0 - 2 .. a0 + FA + 2
+    static constexpr int NCELL = NA * FB;
+    // LDS slot of cell ca (along) of line cb; x-contiguous in both families
+    static HTX_HD int slot(int ca, int cb) { return YDIR ? ca * FB + cb : cb * NA + ca; }
+};
+static_assert(FlagBlock<false>::FA * FlagBlock<false>::FB == 256 && FlagBlock<true>::FA * FlagBlock<true>::FB == 256,
+              "one face per thread of a 256-thread workgroup");
+constexpr int FLAG_NCELL_MAX = FlagBlock<false>::NCELL > FlagBlock<true>::NCELL ? FlagBlock<false>::NCELL
+                                                                                : FlagBlock<true>::NCELL;
+
 // classifier flags of every face of one family, including the faces one
-// cell past each boundary (the stage kernel's tiles reach them)
+// cell past each boundary (the stage kernel's tiles reach them).  Face a of
+// line b lies between cells a and a + 1 along the family's direction,
+// a = -1 .. n_along; its flag goes to flags_x[b][a + 1] (x faces, a row of
+// nx + 2) or flags_y[a + 1][b] (y faces, a row of nx).  The features are
+// those of stencil_features on the same six cells, bit for bit.
 template <bool YDIR>
 __global__ void __launch_bounds__(256) k_face_flags(LrnFlagsArgs a) {
+    using FB_ = FlagBlock<YDIR>;
     __shared__ unsigned short xs[4][64 * 16];
+    __shared__ double cell[7][FLAG_NCELL_MAX];  // q0..q3, un, ut, p (r = q0)
     if (a.ctl && !static_cast<const Ctl*>(a.ctl)->active) return;  // uniform over the grid
-    const int w = YDIR ? a.nx : a.nx + 2;
-    const long long n = YDIR ? (long long)a.nx * (a.ny + 2) : (long long)(a.nx + 2) * a.ny;
-    long long id = blockIdx.x * 256ll + threadIdx.x;
-    unsigned char* out = YDIR ? a.flags_y : a.flags_x;
-    Stencil6 st;
-    bool flat = true;  // faces beyond the array count as flat
-    if (id < n) {
-        int i = (int)(id % w), j = (int)(id / w);
-        if (YDIR)
-            j -= 1;
-        else
-            i -= 1;
-        load_stencil<YDIR>(a.q, a.nx, a.ny, a.pitch, a.fstride, a.bc, i, j, st);
-        for (int m = 1; m < 6; ++m)
-            for (int e = 0; e < 4; ++e) flat = flat && st.q[m][e] == st.q[0][e];
+    const int tid = threadIdx.x;
+    const int n_along = YDIR ? a.ny : a.nx, n_across = YDIR ? a.nx : a.ny;
+    const int a0 = -1 + FB_::FA * (int)blockIdx.x, b0 = FB_::FB * (int)blockIdx.y;
+    const double g1 = a.gam - 1.0;
+    for (int w = tid; w < FB_::NCELL; w += 256) {
+        const int ca = YDIR ? w / FB_::FB : w % FB_::NA;
+        const int cb = YDIR ? w % FB_::FB : w / FB_::NA;
+        const int gb = b0 + cb;
+        if (gb >= n_across) continue;  // read only by faces that are not written
+        const int ga = map_index(a0 - 2 + ca, n_along, a.bc);
+        const int gi = YDIR ? gb : ga, gj = YDIR ? ga : gb;
+        const long long off = (long long)gj * a.pitch + gi;
+        double q[4];
+        for (int e = 0; e < 4; ++e) q[e] = a.q[off + e * a.fstride];
+        double r, un, ut, p;
+        prim_nt(g1, q[0], YDIR ? q[2] : q[1], YDIR ? q[1] : q[2], q[3], r, un, ut, p);
+        for (int e = 0; e < 4; ++e) cell[e][w] = q[e];
+        cell[4][w] = un;
+        cell[5][w] = ut;
+        cell[6][w] = p;
     }
+    __syncthreads();
+    const int fa = YDIR ? tid / FB_::FB : tid % FB_::FA;
+    const int fb = YDIR ? tid % FB_::FB : tid / FB_::FA;
+    const int fal = a0 + fa, fbl = b0 + fb;
+    const bool valid = fal <= n_along && fbl < n_across;
+    bool flat = true;  // faces beyond the array count as flat
+    int c[6];
+    if (valid) {
+        for (int m = 0; m < 6; ++m) c[m] = FB_::slot(fa + m, fb);
+        for (int m = 1; m < 6; ++m)
+            for (int e = 0; e < 4; ++e) flat = flat && cell[e][c[m]] == cell[e][c[0]];
+    }
+    unsigned char* out = YDIR ? a.flags_y : a.flags_x;
+    const long long o = YDIR ? (long long)(fal + 1) * a.nx + fbl : (long long)fbl * (a.nx + 2) + (fal + 1);
     // A stencil of six equal states is smooth for TENO5 (every smoothness
     // measure is zero, no candidate is cut): when all the workgroup's faces
     // are such, the matrix cores are skipped (the classifier agrees there,
     // with a logit far above any threshold).
     if (__syncthreads_and(flat ? 1 : 0)) {
-        if (id < n) out[id] = 1;
+        if (valid) out[o] = 1;
         return;
     }
     float x[CLS_IN];
     for (int k = 0; k < CLS_IN; ++k) x[k] = 0.0f;
-    if (id < n) stencil_features<YDIR>(st, a.gam, x);
-    cls_stage_features(xs[threadIdx.x >> 6], x);
+    if (valid) {
+        double r[6], un[6], ut[6], p[6];
+        for (int m = 0; m < 6; ++m) {
+            r[m] = cell[0][c[m]];
+            un[m] = cell[4][c[m]];
+            ut[m] = cell[5][c[m]];
+            p[m] = cell[6][c[m]];
+        }
+        face_features(r, un, ut, p, a.gam, x);
+    }
+    cls_stage_features(xs[tid >> 6], x);
     __syncthreads();
-    float z = cls_logit(xs[threadIdx.x >> 6], a.frag, a.cvec);
-    if (id < n) out[id] = (flat || z > a.threshold) ? 1 : 0;
+    float z = cls_logit(xs[tid >> 6], a.frag, a.cvec);
+    if (valid) out[o] = (flat || z > a.threshold) ? 1 : 0;
 }
 
 }  // namespace
@@ -396,9 +448,13 @@ size_t lrn_flag_bytes_x(int nx, int ny) { return (size_t)(nx + 2) * ny; }
 size_t lrn_flag_bytes_y(int nx, int ny) { return (size_t)nx * (ny + 2); }
 
 hipError_t lrn_launch_flags(const LrnFlagsArgs& a, hipStream_t s) {
-    long long nxf = (long long)lrn_flag_bytes_x(a.nx, a.ny), nyf = (long long)lrn_flag_bytes_y(a.nx, a.ny);
-    hipLaunchKernelGGL(k_face_flags<false>, dim3((unsigned)((nxf + 255) / 256)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_face_flags<true>, dim3((unsigned)((nyf + 255) / 256)), dim3(256), 0, s, a);
+    using BX = FlagBlock<false>;
+    using BY = FlagBlock<true>;
+    // faces a = -1 .. n along each family's direction: n + 2 of them
+    dim3 gx((unsigned)((a.nx + 2 + BX::FA - 1) / BX::FA), (unsigned)((a.ny + BX::FB - 1) / BX::FB));
+    dim3 gy((unsigned)((a.ny + 2 + BY::FA - 1) / BY::FA), (unsigned)((a.nx + BY::FB - 1) / BY::FB));
+    hipLaunchKernelGGL(k_face_flags<false>, gx, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_face_flags<true>, gy, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -17,12 +17,17 @@ def map_index(g, n, bc):
     return np.clip(g, 0, n - 1)
 
 
-def stencils(q, bc, direction):
+def stencils(q, bc, direction, flag_faces=False):
     """Conserved state of the six stencil cells of every face: shape
-    (6, 4, ny, nx); direction 0 = x faces (cell (i, j) | (i+1, j))."""
+    (6, 4, ny, nx); direction 0 = x faces (cell (i, j) | (i+1, j)).  With
+    flag_faces, the faces of the per-step flag arrays instead: also the face
+    one cell past each boundary, (6, 4, ny, nx + 2) for x faces i = -1 .. nx
+    and (6, 4, ny + 2, nx) for y faces j = -1 .. ny."""
     nf, ny, nx = q.shape
-    J, I = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
-    out = np.empty((6, 4, ny, nx))
+    a_i = np.arange(-1, nx + 1) if flag_faces and direction == 0 else np.arange(nx)
+    a_j = np.arange(-1, ny + 1) if flag_faces and direction == 1 else np.arange(ny)
+    J, I = np.meshgrid(a_j, a_i, indexing="ij")
+    out = np.empty((6, 4) + J.shape)
     for m in range(6):
         if direction == 0:
             gi, gj = map_index(I - 2 + m, nx, bc), J
@@ -71,9 +76,15 @@ def log_amp(a):
     return np.log2(a + F(1e-12)) * F(0.30102999566398120 / 12.0) + F(1.0)
 
 
-def features(q, gamma, bc, direction):
-    """The 12 inputs of every face of one family, shape (ny, nx, 12)."""
-    st = stencils(q, bc, direction)
+def features(q, gamma, bc, direction, flag_faces=False):
+    """The 12 inputs of every face of one family, shape (ny, nx, 12) (the
+    face grid of stencils() with flag_faces)."""
+    st = stencils(q, bc, direction, flag_faces)
+    return stencil_features(st, gamma, direction)
+
+
+def stencil_features(st, gamma, direction):
+    """The 12 inputs of the faces of stencils st (6, 4, ...)."""
     g1 = gamma - 1.0
     r = st[:, 0]
     qn, qt = (st[:, 1], st[:, 2]) if direction == 0 else (st[:, 2], st[:, 1])

@@ -292,3 +292,48 @@ def test_face_flags_match_standalone_and_drive_the_stage(case):
     dl, dm = frac(r_gpu, r_cpu), frac(rm_gpu, rm_cpu)
     print(f"residual GPU vs CPU tiling, cells differing: learned {dl:.5f}, fp32_mixed {dm:.5f}", flush=True)
     assert dl <= dm + 0.005
+
+
+@pytest.mark.parametrize("bc", ["zero_grad", "periodic"])
+def test_face_flags_every_face_vs_numpy(bc):
+    """Every flag of the per-step flag kernel, the faces one cell past each
+    boundary and the partial work blocks of a 100 x 72 grid included, equals
+    the numpy features run through the bf16-rounded forward pass against the
+    threshold (1 on a stencil of six equal states), except for logits within
+    the inference's rounding band (twice the largest difference between the
+    standalone matrix-core logits and that forward pass on the interior
+    faces); with periodic boundaries each face past the boundary carries its
+    wrapped image's flag."""
+    import htx
+    from htx.ic import SplitMix64, piecewise_state, quadrant_params
+    from htx.learn import default_classifier
+
+    nx, ny = 100, 72
+    xs, ys, st = quadrant_params(SplitMix64(31))
+    w = dict(kind="quadrant", xl=0.0, xr=1.0, yb=0.0, yt=1.0, gamma=GAMMA, xs=xs, ys=ys, states=st)
+    q = np.ascontiguousarray(piecewise_state(w, nx, ny))
+    prm, thr = default_classifier()
+    print(f"[case] face flags of every face vs numpy, quadrant {nx}x{ny}, {bc}", flush=True)
+    s = htx.System2D(0.0, 1.0, nx, 0.0, 1.0, ny, 0.5, 10.0, 0.0, GAMMA, bc=bc, precision="learned",
+                     classifier=(prm, thr))
+    s.set_state(q)
+    s.rhs()
+    fx, fy = s.classifier_flags()
+    for d, f in ((0, fx), (1, fy)):
+        stn = L.stencils(q, bc, d, flag_faces=True)
+        ref = L.forward_bf16(prm, L.stencil_features(stn, GAMMA, d).reshape(-1, 12)).reshape(f.shape)
+        flat = np.all(stn == stn[0:1], axis=(0, 1))
+        lg = s.classify(d)
+        inner = ref[:, 1:nx + 1] if d == 0 else ref[1:ny + 1, :]
+        band = 2.0 * float(np.abs(lg - inner).max()) + 1e-6
+        expect = (flat | (ref > thr)).astype(np.uint8)
+        decided = flat | (np.abs(ref - thr) > band)
+        bad = (f != expect) & decided
+        print(f"dir {d}: band {band:.2e}, decided {decided.mean():.4f}, flagged {f.mean():.3f}, "
+              f"mismatches {int(bad.sum())}", flush=True)
+        assert bad.sum() == 0
+        assert decided.mean() > 0.9
+    s.close()
+    if bc == "periodic":
+        assert np.array_equal(fx[:, 0], fx[:, nx]) and np.array_equal(fx[:, nx + 1], fx[:, 1])
+        assert np.array_equal(fy[0, :], fy[ny, :]) and np.array_equal(fy[ny + 1, :], fy[1, :])
