@@ -53,6 +53,7 @@ def cpu_baseline(n, seconds=12.0):
     orc = Oracle(omp=True)
     prm = params(n, n, w["xl"], w["xr"], w["yb"], w["yt"], w["gamma"], "periodic", "hllc", True)
     q = orc.ic_vortex(prm, w["centre"], w["modes"])
+    q_init = q.copy()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     log(f"[cpu] oracle OpenMP build, {threads} threads, {n}x{n}: warm-up step")
     q, t, _ = orc.fv2d_advance(prm, q, w["cfl"], 0.0, w["tmax"], 1)
@@ -66,9 +67,32 @@ def cpu_baseline(n, seconds=12.0):
         if el > seconds or steps >= 20:
             break
     rate = n * n * steps / el
-    return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+    base = {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{steps} SSP-RK3 steps of the {n}x{n} vortex workload after 1 warm-up step, "
                       f"FP64 CPU oracle (gcc -O3 -fopenmp), {el:.1f} s"}
+    return base, parity_vs_cpu(w, n, q_init, q, t, steps + 1)
+
+
+def parity_vs_cpu(w, n, q_init, q_cpu, t_cpu, steps):
+    """The FP64 drop-in GPU path on the CPU sample's own initial state and
+    step count: relative L1 of the state (all fields) and the time reached."""
+    import numpy as np
+    import htx
+
+    s = htx.System2D(w["xl"], w["xr"], n, w["yb"], w["yt"], n, w["cfl"], w["tmax"], 0.0, w["gamma"],
+                     flux="hllc", bc="periodic", precision="fp64")
+    s.set_state(q_init)
+    s.steps_async(steps)
+    s.sync()
+    q_gpu, t_gpu, k = s.get_state(), s.time, s.nsteps
+    s.close()
+    d = np.abs(q_gpu - q_cpu)
+    rel = float(d.sum() / np.abs(q_cpu).sum())
+    log(f"[parity] {steps} steps at {n}x{n}: relative L1 {rel:.3e}, max |dq| {d.max():.3e}, "
+        f"t {t_gpu!r} vs {t_cpu!r}")
+    return {"workload": f"{n}x{n} vortex, the CPU sample's initial state, {steps} SSP-RK3 steps",
+            "steps": int(k), "rel_l1": rel, "max_abs_diff": float(d.max()), "bit_identical": bool(d.max() == 0.0),
+            "t_gpu": t_gpu, "t_cpu": t_cpu, "tolerance": {"per_step": 1e-12, "end_time": 1e-9}}
 
 
 def copy_calibration(mib=256):
@@ -350,7 +374,7 @@ def main():
             log(f"[gpu] batch study failed: {e}")
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
+            out["cpu_baseline"], out["parity_vs_cpu"] = cpu_baseline(n, args.cpu_seconds)
         except Exception as e:  # the baseline is reported, never required for the GPU figure
             log(f"[cpu] baseline failed: {e}")
     print(json.dumps(out), flush=True)
