@@ -22,6 +22,8 @@ hipError_t htx_strict_launch_fv2d_stage(const void* a, int flux, int tile_x, int
 hipError_t htx_strict_launch_fvh2d_stage(const void* a, int flux, hipStream_t s);
 hipError_t htx_strict_launch_fv1d_stage(const void* a, hipStream_t s);
 hipError_t htx_strict_launch_fv1d_max_speed(const double* q, int nx, double gam, void* ctl, hipStream_t s);
+hipError_t htx_strict_launch_fv1d_kxrcf(const double* q, int nx, int bc, double radius, unsigned char* flags,
+                                        hipStream_t s);
 
 using namespace htx;
 
@@ -85,6 +87,7 @@ struct htx_solver {
     float* dCvec = nullptr;
     unsigned char* dFlagsX = nullptr;  // classifier flags of the x and y faces (learned precision)
     unsigned char* dFlagsY = nullptr;
+    unsigned char* dTroubled = nullptr;  // 1D KXRCF cell flags (allocated on first use)
     float cls_threshold = 0.0f;
     bool have_classifier = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -392,6 +395,7 @@ void htx_destroy(htx_solver* h) {
     if (h->dCvec) (void)hipFree(h->dCvec);
     if (h->dFlagsX) (void)hipFree(h->dFlagsX);
     if (h->dFlagsY) (void)hipFree(h->dFlagsY);
+    if (h->dTroubled) (void)hipFree(h->dTroubled);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -1075,6 +1079,18 @@ int htx_load_classifier(htx_solver* h, const float* params, double threshold) {
     HIP_TRY(copy_sync(h->dCvec, cvec.data(), sizeof(float) * CLS_NCVEC, hipMemcpyHostToDevice, h->stream));
     h->cls_threshold = (float)threshold;
     h->have_classifier = true;
+    return HTX_OK;
+}
+
+int htx_kxrcf_flags(htx_solver* h, unsigned char* flags) {
+    if (!h || !flags) return fail(HTX_EINVAL, "null argument");
+    if (h->p.dim != 1) return fail(HTX_ENOTSUP, "the KXRCF detector belongs to the 1D solver");
+    if (!h->have_state) return fail(HTX_ESTATE, "the solver has no state");
+    if (!h->dTroubled) HIP_TRY(hipMalloc(&h->dTroubled, (size_t)h->nx));
+    // circleRadMod of FV_System1D.cpp:19, with the host's pow as there
+    const double radius = std::pow(h->dx / 2.0, 1.5);
+    HIP_TRY(htx_strict_launch_fv1d_kxrcf(h->dA, h->nx, h->p.bc, radius, h->dTroubled, h->stream));
+    HIP_TRY(copy_sync(flags, h->dTroubled, (size_t)h->nx, hipMemcpyDeviceToHost, h->stream));
     return HTX_OK;
 }
 

@@ -231,4 +231,29 @@ HTX_HD double fv1d_max_speed(double gam, double g1, const double q[3]) {
     return fmax(fabs(w[1] - c), fabs(w[1] + c));
 }
 
+// KXRCF troubled-cell flag of cell c (FV::System1D::rhs, FV_System1D.cpp:
+// 166-192): the density at the face upwind of the momentum's sign,
+// reconstructed from the 3-cell stencils of the two cells sharing that face;
+// flagged when |h1 - h2| / (radius |rho_c|) > 1, radius = (dx/2)^1.5 (:19).
+// rho is the density row of the state, ghosts by the boundary condition.
+// The reference visits cells 1 .. nx-2 only; the first and last cells are
+// reported unflagged.
+HTX_HD double fv1d_rho_at(const double* rho, int g, int nx, int bc) { return rho[map_index(g, nx, bc)]; }
+
+HTX_HD bool fv1d_kxrcf(const double* rho, double mom, int c, int nx, int bc, double radius) {
+    HTX_NO_CONTRACT
+    if (c < 1 || c > nx - 2) return false;
+    const double rm2 = fv1d_rho_at(rho, c - 2, nx, bc), rm1 = rho[c - 1], r0 = rho[c], rp1 = rho[c + 1];
+    const double rp2 = fv1d_rho_at(rho, c + 2, nx, bc);
+    double h1, h2;
+    if (mom < 0.0) {
+        h1 = (2.0 * rm1 + 5.0 * r0 - rp1) / 6.0;
+        h2 = (-rm2 + 5.0 * rm1 + 2.0 * r0) / 6.0;
+    } else {
+        h2 = (2.0 * r0 + 5.0 * rp1 - rp2) / 6.0;
+        h1 = (-rm1 + 5.0 * r0 + 2.0 * rp1) / 6.0;
+    }
+    return fabs(h1 - h2) / (radius * fabs(r0)) > 1.0;
+}
+
 }  // namespace HTX_NS

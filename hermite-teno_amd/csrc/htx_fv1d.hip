@@ -68,8 +68,20 @@ __global__ void __launch_bounds__(FV1_T) k_fv1d_max_speed(const double* q, int n
     block_max_to_ctl(sm, ctl);
 }
 
+// KXRCF flags of every cell of a state (q = 3 fields x nx)
+__global__ void __launch_bounds__(FV1_T) k_fv1d_kxrcf(const double* q, int nx, int bc, double radius,
+                                                      unsigned char* flags) {
+    const int c = blockIdx.x * FV1_T + threadIdx.x;
+    if (c < nx) flags[c] = fv1d_kxrcf(q, q[nx + c], c, nx, bc, radius) ? 1 : 0;
+}
+
 hipError_t launch_fv1d_stage(const Fv1dArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_fv1d_stage, dim3((a.nx + FV1_T - 1) / FV1_T), dim3(FV1_T), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fv1d_kxrcf(const double* q, int nx, int bc, double radius, unsigned char* flags, hipStream_t s) {
+    hipLaunchKernelGGL(k_fv1d_kxrcf, dim3((nx + FV1_T - 1) / FV1_T), dim3(FV1_T), 0, s, q, nx, bc, radius, flags);
     return hipGetLastError();
 }
 

@@ -30,3 +30,8 @@ hipError_t htx_strict_launch_fv1d_stage(const void* a, hipStream_t s) {
 hipError_t htx_strict_launch_fv1d_max_speed(const double* q, int nx, double gam, void* ctl, hipStream_t s) {
     return htx_strict::launch_fv1d_max_speed(q, nx, gam, static_cast<htx_strict::Ctl*>(ctl), s);
 }
+
+hipError_t htx_strict_launch_fv1d_kxrcf(const double* q, int nx, int bc, double radius, unsigned char* flags,
+                                        hipStream_t s) {
+    return htx_strict::launch_fv1d_kxrcf(q, nx, bc, radius, flags, s);
+}

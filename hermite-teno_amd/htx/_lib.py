@@ -94,6 +94,7 @@ SIGNATURES = {
     "htx_load_classifier": (_I, [_P, _FP, _D]),
     "htx_classify": (_I, [_P, _I, _FP, _FP]),
     "htx_classifier_flags": (_I, [_P, ctypes.c_void_p, ctypes.c_void_p]),
+    "htx_kxrcf_flags": (_I, [_P, ctypes.c_void_p]),
     "htx_classifier_stats": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                                   ctypes.POINTER(ctypes.c_longlong), _I]),
     "htx_batch_create": (_I, [ctypes.POINTER(HtxParams), _I, ctypes.POINTER(_P)]),

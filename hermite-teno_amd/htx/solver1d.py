@@ -114,6 +114,14 @@ class System1D:
         self.set_state(self._conservative(np.where(left, 1.0, 0.125), np.zeros_like(x), np.where(left, 1.0, 0.1)))
         self.exact = None
 
+    def troubledCells(self):
+        """KXRCF troubled-cell flags (uint8, nx; 1 = discontinuous) of the
+        current state, as FV::System1D::rhs computes them
+        (FV_System1D.cpp:166-192); cells 0 and nx-1 are never flagged."""
+        f = np.zeros(self.nx, dtype=np.uint8)
+        check(_lib.load().htx_kxrcf_flags(self._h, f.ctypes.data))
+        return f
+
     # -- time stepping ----------------------------------------------------
     def rhs(self):
         r = np.empty((3, self.nx), dtype=np.float64)

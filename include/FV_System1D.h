@@ -5,7 +5,8 @@
 // The public interface is the reference's:
 //     System1D(xl, xr, nx, cflnumber, tmax, tInit, specificHeatRatio)
 //     void entropyWave(); void blastWave(); void timeStepping(); void printResult();
-// plus sodShock() (Euler1D::sodShock, the 1D configuration's initial state).
+// plus sodShock() (Euler1D::sodShock, the 1D configuration's initial state)
+// and troubledCells() (the KXRCF flags of FV::System1D::rhs).
 // The residual (characteristic TENO5i or TENO5, characteristic Rusanov or
 // HLLC), the SSP-RK3 stages and the CFL reduction run on the GPU.  The
 // optional trailing htx::Options carries the reference's code-toggled
@@ -118,6 +119,14 @@ public:
         }
         std::cout << "Numerical error:" << err / nx_ << std::endl;
         std::cout << "L inf: " << merr << std::endl;
+    }
+
+    // The per-cell discontinue flags FV::System1D::rhs sets with its KXRCF
+    // detector (FV_System1D.cpp:166-192), for the current state.
+    std::vector<unsigned char> troubledCells() const {
+        std::vector<unsigned char> f((size_t)nx_);
+        check(htx_kxrcf_flags(h_, f.data()));
+        return f;
     }
 
     std::vector<double> state() const {

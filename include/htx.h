@@ -257,6 +257,17 @@ int htx_load_classifier(htx_solver* h, const float* params, double threshold);
  * matrix-core inference. */
 int htx_classify(htx_solver* h, int dir, float* logits, float* feat);
 
+/* 1D only: the KXRCF troubled-cell flags (1 = discontinuous) that
+ * FV::System1D::rhs computes for its stage input (FV_System1D.cpp:166-192;
+ * radius circleRadMod = (dx/2)^1.5, :19), here for the current state: the
+ * density at the face upwind of the momentum's sign, reconstructed from the
+ * 3-cell stencils of the two cells sharing it, jumps by more than radius
+ * |rho|.  flags holds nx bytes; cells 0 and nx-1, which the reference never
+ * visits, are 0.  The reference's 1D scheme ignores the flags (both of its
+ * branches take teno5i); they are reported as a classical detector's flag
+ * rate. */
+int htx_kxrcf_flags(htx_solver* h, unsigned char* flags);
+
 /* Learned precision: the classifier flags (1 = smooth) of the last step's
  * state (or of the last htx_rhs) that the stage kernels read: x faces
  * (i, j) | (i+1, j) for i = -1 .. nx at fx[j * (nx + 2) + i + 1] (ny rows),

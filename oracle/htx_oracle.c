@@ -1522,6 +1522,43 @@ void orc_fv1d_rhs(int nx, double xl, double xr, double gamma, int recon, int flu
     free(fx);
 }
 
+/* The KXRCF troubled-cell flags of FV::System1D::rhs (FV_System1D.cpp:166-192,
+ * radius circleRadMod = (dx/2)^1.5 from :19): for each cell i = k+1 .. nxx-k-2
+ * of the zero-gradient ghost-padded density, the value at the face upwind
+ * of the momentum's sign reconstructed from the stencils of the two cells
+ * sharing it; flagged when |h1 - h2| / (radius |rho_i|) > 1.  The reference
+ * never visits the first and last interior cells: they come out 0. */
+void orc_fv1d_kxrcf(int nx, double xl, double xr, const double* q, unsigned char* flags) {
+    int k = FV1_K, nxx = nx + 2 * k;
+    double dx = (xr - xl) / nx, radius = pow(dx / 2.0, 1.5);
+    double* r = (double*)calloc((size_t)nxx, sizeof(double));
+    double* m = (double*)calloc((size_t)nxx, sizeof(double));
+    for (int i = 0; i < nx; ++i) {
+        r[k + i] = q[i];
+        m[k + i] = q[nx + i];
+    }
+    for (int g = 0; g < k; ++g) { /* zeroGradBc, FV_System1D.cpp:74-80 */
+        r[k - 1 - g] = r[k];
+        r[nxx - 1 - g] = r[nxx - k - 1];
+        m[k - 1 - g] = m[k];
+        m[nxx - 1 - g] = m[nxx - k - 1];
+    }
+    for (int i = 0; i < nx; ++i) flags[i] = 0;
+    for (int i = k + 1; i < nxx - k - 1; ++i) {
+        double h1, h2;
+        if (m[i] < 0.0) {
+            h1 = (2.0 * r[i - 1] + 5.0 * r[i] - r[i + 1]) / 6.0;
+            h2 = (-r[i - 2] + 5.0 * r[i - 1] + 2.0 * r[i]) / 6.0;
+        } else {
+            h2 = (2.0 * r[i] + 5.0 * r[i + 1] - r[i + 2]) / 6.0;
+            h1 = (-r[i - 1] + 5.0 * r[i] + 2.0 * r[i + 1]) / 6.0;
+        }
+        flags[i - k] = fabs(h1 - h2) / (radius * fabs(r[i])) > 1.0;
+    }
+    free(r);
+    free(m);
+}
+
 /* FV::System1D::timeStepping + RungeKutta3, FV_System1D.cpp:243-292 */
 int orc_fv1d_advance(int nx, double xl, double xr, double gamma, int recon, int flux, int quirks, double* q,
                      double cfl, double t0, double tfinal, int max_steps, double* t_out) {

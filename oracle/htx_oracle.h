@@ -78,6 +78,7 @@ void orc_fv1d_rhs(int nx, double xl, double xr, double gamma, int recon, int flu
 int orc_fv1d_advance(int nx, double xl, double xr, double gamma, int recon, int flux, int quirks,
                      double* q, double cfl, double t0, double tfinal, int max_steps, double* t_out);
 void orc_ic_sod(int nx, double xl, double xr, double gamma, double* q);
+void orc_fv1d_kxrcf(int nx, double xl, double xr, const double* q, unsigned char* flags);
 
 /* ---- test hooks: smoothness indicators ---- */
 void orc_teno5_indicators(const double u[6], double SI[6], double b35[2]);

@@ -61,6 +61,7 @@ _SIG = {
     "orc_fv1d_rhs": (None, [_I, _D, _D, _D, _I, _I, _I, _DP, _DP]),
     "orc_fv1d_advance": (_I, [_I, _D, _D, _D, _I, _I, _I, _DP, _D, _D, _D, _I, _DP]),
     "orc_ic_sod": (None, [_I, _D, _D, _D, _DP]),
+    "orc_fv1d_kxrcf": (None, [_I, _D, _D, _DP, ctypes.POINTER(ctypes.c_ubyte)]),
     "orc_teno5_indicators": (None, [_DP, _DP, _DP]),
     "orc_hteno_indicators": (None, [_DP, _DP, _DP, _DP, _DP]),
 }
@@ -306,6 +307,13 @@ class Oracle:
                                       {"rusanov": 0, "hllc": 1}[flux], 1 if quirks else 0, q.ctypes.data_as(_DP),
                                       cfl, t0, tfinal, max_steps, ctypes.byref(t))
         return q, t.value, n
+
+    def fv1d_kxrcf(self, nx, xl, xr, q):
+        """KXRCF troubled-cell flags (uint8, nx) of a 1D state (3, nx)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        f = np.zeros(nx, dtype=np.uint8)
+        self.lib.orc_fv1d_kxrcf(nx, xl, xr, q.ctypes.data_as(_DP), f.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+        return f
 
     def ic_sod(self, nx, xl=0.0, xr=1.0, gamma=1.4):
         q = np.empty((3, nx))

@@ -85,3 +85,41 @@ def test_sod_configuration_end_to_end(orc):
     assert n == nr and s.time == tr
     assert err < 1e-12
     s.close()
+
+
+def test_kxrcf_flags_match_oracle(orc):
+    """The GPU KXRCF detector equals the oracle's flags bit for bit on the
+    Sod initial state (cells 198-200), on the states of a Sod run (each of
+    40 steps) and on a seeded rough state with momenta of both signs; the
+    2D solver refuses the call."""
+    import htx
+    from htx._lib import HtxError
+
+    nx = 400
+    print("[case] KXRCF flags, Sod N=400 and a seeded state", flush=True)
+    s = htx.System1D(0.0, 1.0, nx, 0.5, 0.2, 0.0, GAMMA, scheme="teno5", flux="hllc")
+    s.sodShock()
+    f = s.troubledCells()
+    assert list(np.nonzero(f)[0]) == [198, 199, 200]
+    rates = []
+    for _ in range(40):
+        s.step()
+        q = s.get_state()
+        f = s.troubledCells()
+        np.testing.assert_array_equal(f, orc.fv1d_kxrcf(nx, 0.0, 1.0, q))
+        rates.append(f.mean())
+    print(f"flag rate over the run: {min(rates):.4f} .. {max(rates):.4f}", flush=True)
+    rng = np.random.default_rng(5)
+    q = np.empty((3, nx))
+    q[0] = 1.0 + 0.5 * rng.random(nx) * (rng.random(nx) < 0.1)
+    q[1] = rng.normal(0.0, 0.5, nx)
+    q[2] = 2.5 + 0.5 * q[1] ** 2 / q[0]
+    s.set_state(q)
+    f = s.troubledCells()
+    np.testing.assert_array_equal(f, orc.fv1d_kxrcf(nx, 0.0, 1.0, q))
+    assert 0 < f.sum() < nx
+    s.close()
+    t = htx.System2D(0.0, 1.0, 16, 0.0, 1.0, 16, 0.5, 1.0, 0.0, GAMMA)
+    with pytest.raises(HtxError):
+        htx._lib.check(htx._lib.load().htx_kxrcf_flags(t._h, np.zeros(16, np.uint8).ctypes.data))
+    t.close()
