@@ -104,17 +104,17 @@ int check_params(const htx_params* p) {
     if (!(p->xr > p->xl) || (p->dim == 2 && !(p->yt > p->yb)))
         return fail(HTX_EINVAL, "domain must have positive extent");
     if (p->dim == 1) {
-        // FV::System1D: TENO5 or the shipped TENO5i reconstruction, FP64
-        if (p->scheme != HTX_SCHEME_TENO5 && p->scheme != HTX_SCHEME_TENO5I)
-            return fail(HTX_EINVAL, "1D scheme must be teno5 or teno5i");
+        // FV::System1D: TENO5, the shipped TENO5i, WENO5-JS or WENO5-Z, FP64
+        if (p->scheme != HTX_SCHEME_TENO5 && p->scheme != HTX_SCHEME_TENO5I && p->scheme != HTX_SCHEME_WENO5JS &&
+            p->scheme != HTX_SCHEME_WENO5Z)
+            return fail(HTX_EINVAL, "1D scheme must be teno5, teno5i, weno5js or weno5z");
         if (p->precision != HTX_PREC_FP64) return fail(HTX_ENOTSUP, "the 1D path is built in fp64 precision");
-    } else if (p->scheme == HTX_SCHEME_TENO5I) {
-        return fail(HTX_ENOTSUP, "teno5i is the 1D reconstruction");
+    } else if (p->scheme == HTX_SCHEME_TENO5I || p->scheme == HTX_SCHEME_WENO5JS || p->scheme == HTX_SCHEME_WENO5Z) {
+        return fail(HTX_ENOTSUP, "teno5i, weno5js and weno5z are 1D reconstructions");
     }
     if (!(p->gamma > 1.0)) return fail(HTX_EINVAL, "gamma must exceed 1");
     if (!(p->cfl > 0.0)) return fail(HTX_EINVAL, "cfl must be positive");
-    if (p->scheme != HTX_SCHEME_TENO5 && p->scheme != HTX_SCHEME_HTENO && p->scheme != HTX_SCHEME_TENO5I)
-        return fail(HTX_EINVAL, "unknown scheme");
+    if (p->scheme < HTX_SCHEME_TENO5 || p->scheme > HTX_SCHEME_WENO5Z) return fail(HTX_EINVAL, "unknown scheme");
     if (p->flux != HTX_FLUX_RUSANOV_CHAR && p->flux != HTX_FLUX_HLLC) return fail(HTX_EINVAL, "unknown flux");
     if (p->bc != HTX_BC_ZERO_GRAD && p->bc != HTX_BC_PERIODIC) return fail(HTX_EINVAL, "unknown bc");
     if (p->precision != HTX_PREC_FP64 && p->precision != HTX_PREC_FP32_MIXED && p->precision != HTX_PREC_LEARNED &&
@@ -230,7 +230,10 @@ int launch_stage(htx_solver* h, const double* qin, const double* q0, double* qou
         a.nx = h->nx;
         a.bc = h->p.bc;
         a.mode = mode;
-        a.recon = h->p.scheme == HTX_SCHEME_TENO5I ? 1 : 0;
+        a.recon = h->p.scheme == HTX_SCHEME_TENO5I    ? 1
+                  : h->p.scheme == HTX_SCHEME_WENO5JS ? 2
+                  : h->p.scheme == HTX_SCHEME_WENO5Z  ? 3
+                                                      : 0;
         a.flux = flux_kind(h);
         a.gam = h->p.gamma;
         a.g1 = h->p.gamma - 1.0;

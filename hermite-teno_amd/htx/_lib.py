@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libhtx.so")
 HTX_OK = 0
 ERRORS = {-1: "HTX_EINVAL", -2: "HTX_EDEVICE", -3: "HTX_ENOMEM", -4: "HTX_ESTATE", -5: "HTX_ENOTSUP", -6: "HTX_EIO"}
 
-SCHEMES = {"teno5": 0, "hteno": 1, "teno5i": 2}
+SCHEMES = {"teno5": 0, "hteno": 1, "teno5i": 2, "weno5js": 3, "weno5z": 4}
 FLUXES = {"rusanov": 0, "rusanov_char": 0, "hllc": 1}
 BCS = {"zero_grad": 0, "zerograd": 0, "periodic": 1}
 PRECISIONS = {"fp64": 0, "fp32_mixed": 1, "learned": 2, "fp64_fast": 3}

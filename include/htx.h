@@ -42,8 +42,16 @@ enum htx_error {
 };
 
 /* HTX_SCHEME_TENO5I: Lagrange::teno5i (Schemes_Lagrange.cpp:1235-1263), the
- * reconstruction FV::System1D ships with; 1D only. */
-enum htx_scheme { HTX_SCHEME_TENO5 = 0, HTX_SCHEME_HTENO = 1, HTX_SCHEME_TENO5I = 2 };
+ * reconstruction FV::System1D ships with; 1D only.  HTX_SCHEME_WENO5JS /
+ * HTX_SCHEME_WENO5Z: Lagrange::weno5JS / weno5Z (Schemes_Lagrange.cpp:358-408),
+ * the reference's alternatives for the 1D solver's reconstruction; 1D only. */
+enum htx_scheme {
+    HTX_SCHEME_TENO5 = 0,
+    HTX_SCHEME_HTENO = 1,
+    HTX_SCHEME_TENO5I = 2,
+    HTX_SCHEME_WENO5JS = 3,
+    HTX_SCHEME_WENO5Z = 4
+};
 enum htx_flux { HTX_FLUX_RUSANOV_CHAR = 0, HTX_FLUX_HLLC = 1 };
 enum htx_bc { HTX_BC_ZERO_GRAD = 0, HTX_BC_PERIODIC = 1 };
 /* HTX_PREC_FP64: the drop-in; every operation is one IEEE operation in the

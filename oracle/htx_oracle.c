@@ -195,6 +195,47 @@ void orc_teno5_flags(const double u[6], int flags[6]) {
     teno5_impl(u, out, flags);
 }
 
+/* One side of weno5JS (Schemes_Lagrange.cpp:358-381): weights
+ * d_k / (eps + SI_k)^2 normalised; or of weno5Z (:383-408): weights
+ * d_k (1 + tau / (SI_k + epsz)), tau = |SI_2 - SI_0|, normalised. */
+static double weno5_side(const double* v, const double p[3], const double d[3], int z) {
+    double SI[3], w[3];
+    smoothness5(v, SI);
+    if (z) {
+        double tau = fabs(SI[2] - SI[0]);
+        for (int k = 0; k < 3; ++k) w[k] = d[k] * (1.0 + tau / (SI[k] + LC.epsz));
+    } else {
+        for (int k = 0; k < 3; ++k) w[k] = d[k] / pow(LC.eps + SI[k], 2);
+    }
+    double s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    return catchnan(dot3(p, w));
+}
+
+/* Schemes::Lagrange::weno5JS / weno5Z: the candidates of teno5_impl. */
+static void weno5_impl(const double u[6], double out[2], int z) {
+    double p[3], c[3];
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][1];
+    p[0] = dot3(c, u + 2);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][2];
+    p[1] = dot3(c, u + 1);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][3];
+    p[2] = dot3(c, u + 0);
+    out[0] = weno5_side(u + 0, p, LC.dkm, z);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][0];
+    p[0] = dot3(c, u + 3);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][1];
+    p[1] = dot3(c, u + 2);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][2];
+    p[2] = dot3(c, u + 1);
+    out[1] = weno5_side(u + 1, p, LC.dkp, z);
+}
+
+void orc_weno5(const double u[6], int z, double out[2]) {
+    lag_init();
+    weno5_impl(u, out, z);
+}
+
 /* Schemes::Lagrange::linear, Schemes_Lagrange.cpp:1219-1233 */
 void orc_linear5(const double u[6], double out[2]) {
     double p[3], c[3];
@@ -1451,7 +1492,8 @@ void orc_teno5i(const double u[6], double out[2]) {
 
 /* FV::System1D::rhs, FV_System1D.cpp:158-241 (zero-gradient BC :74-80,
  * eigen1DEuler :121-156; the KXRCF flags :166-192 feed nothing).
- * q = 3 fields x nx.  recon: 0 teno5, 1 teno5i (shipped); flux: ORC_FLUX_*. */
+ * q = 3 fields x nx.  recon: 0 teno5, 1 teno5i (shipped), 2 weno5JS, 3 weno5Z;
+ * flux: ORC_FLUX_*. */
 void orc_fv1d_rhs(int nx, double xl, double xr, double gamma, int recon, int flux, int quirks, const double* q,
                   double* rhs) {
     lag_init();
@@ -1489,7 +1531,12 @@ void orc_fv1d_rhs(int nx, double xl, double xr, double gamma, int recon, int flu
             for (int e = 0; e < 3; ++e) C[e][m] = y[e];
         }
         for (int e = 0; e < 3; ++e) {
-            if (recon == 1) teno5i_impl(C[e], o); else teno5_impl(C[e], o, NULL);
+            if (recon == 1)
+                teno5i_impl(C[e], o);
+            else if (recon == 2 || recon == 3)
+                weno5_impl(C[e], o, recon == 3);
+            else
+                teno5_impl(C[e], o, NULL);
             ul[e] = o[0];
             ur[e] = o[1];
         }

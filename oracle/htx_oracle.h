@@ -72,6 +72,7 @@ void orc_ic_vortex_hermite(const orc_params* p, double cx, double cy, const doub
 
 /* ---- 1D FV (FV::System1D): 3 fields ---- */
 void orc_teno5i(const double u[6], double out[2]);
+void orc_weno5(const double u[6], int z, double out[2]); /* z: 0 weno5JS, 1 weno5Z */
 void orc_e1_hllc(double gamma, int quirks, const double ql[3], const double qr[3], double f[3]);
 void orc_fv1d_rhs(int nx, double xl, double xr, double gamma, int recon, int flux, int quirks,
                   const double* q, double* rhs);

@@ -62,6 +62,7 @@ _SIG = {
     "orc_fv1d_advance": (_I, [_I, _D, _D, _D, _I, _I, _I, _DP, _D, _D, _D, _I, _DP]),
     "orc_ic_sod": (None, [_I, _D, _D, _D, _DP]),
     "orc_fv1d_kxrcf": (None, [_I, _D, _D, _DP, ctypes.POINTER(ctypes.c_ubyte)]),
+    "orc_weno5": (None, [_DP, _I, _DP]),
     "orc_teno5_indicators": (None, [_DP, _DP, _DP]),
     "orc_hteno_indicators": (None, [_DP, _DP, _DP, _DP, _DP]),
 }
@@ -295,7 +296,7 @@ class Oracle:
     def fv1d_rhs(self, nx, xl, xr, gamma, q, recon="teno5", flux="hllc", quirks=True):
         q, pq = _p(q)
         r = np.empty_like(q)
-        self.lib.orc_fv1d_rhs(nx, xl, xr, gamma, {"teno5": 0, "teno5i": 1}[recon], {"rusanov": 0, "hllc": 1}[flux],
+        self.lib.orc_fv1d_rhs(nx, xl, xr, gamma, {"teno5": 0, "teno5i": 1, "weno5js": 2, "weno5z": 3}[recon], {"rusanov": 0, "hllc": 1}[flux],
                               1 if quirks else 0, pq, r.ctypes.data_as(_DP))
         return r
 
@@ -303,10 +304,18 @@ class Oracle:
                      max_steps=0):
         q = np.array(q, dtype=np.float64, copy=True, order="C")
         t = _D()
-        n = self.lib.orc_fv1d_advance(nx, xl, xr, gamma, {"teno5": 0, "teno5i": 1}[recon],
+        n = self.lib.orc_fv1d_advance(nx, xl, xr, gamma, {"teno5": 0, "teno5i": 1, "weno5js": 2, "weno5z": 3}[recon],
                                       {"rusanov": 0, "hllc": 1}[flux], 1 if quirks else 0, q.ctypes.data_as(_DP),
                                       cfl, t0, tfinal, max_steps, ctypes.byref(t))
         return q, t.value, n
+
+    def weno5(self, u, z=False):
+        """weno5JS (z False) or weno5Z of a 6-value window: (u-, u+) at the
+        window's centre face."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.empty(2)
+        self.lib.orc_weno5(u.ctypes.data_as(_DP), 1 if z else 0, out.ctypes.data_as(_DP))
+        return out
 
     def fv1d_kxrcf(self, nx, xl, xr, q):
         """KXRCF troubled-cell flags (uint8, nx) of a 1D state (3, nx)."""

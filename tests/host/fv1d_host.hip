@@ -7,7 +7,7 @@
 
 using namespace htx;
 
-// recon: 0 teno5, 1 teno5i; flux: 0 Rusanov, 1 HLLC; mode 0 residual,
+// recon: 0 teno5, 1 teno5i, 2 weno5JS, 3 weno5Z; flux: 0 Rusanov, 1 HLLC; mode 0 residual,
 // 1..3 stage with dt (q0 = u^n)
 extern "C" void fv1dh_stage(int nx, double xl, double xr, double gamma, int recon, int flux, int quirks, int mode,
                             double dt, const double* q, const double* q0, double* out) {

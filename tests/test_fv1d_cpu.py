@@ -31,7 +31,7 @@ def _stage(lib, q, recon, flux, mode=0, dt=0.0, q0=None, quirks=1):
     q = np.ascontiguousarray(q)
     q0 = np.ascontiguousarray(q if q0 is None else q0)
     out = np.empty_like(q)
-    lib.fv1dh_stage(nx, 0.0, 1.0, GAMMA, {"teno5": 0, "teno5i": 1}[recon], {"rusanov": 0, "hllc": 1}[flux], quirks,
+    lib.fv1dh_stage(nx, 0.0, 1.0, GAMMA, {"teno5": 0, "teno5i": 1, "weno5js": 2, "weno5z": 3}[recon], {"rusanov": 0, "hllc": 1}[flux], quirks,
                     mode, dt, q.ctypes.data_as(DP), q0.ctypes.data_as(DP), out.ctypes.data_as(DP))
     return out
 
@@ -51,7 +51,7 @@ def _states():
 
 
 @pytest.mark.parametrize("recon,flux", [("teno5i", "rusanov"), ("teno5", "hllc"), ("teno5", "rusanov"),
-                                        ("teno5i", "hllc")])
+                                        ("teno5i", "hllc"), ("weno5js", "hllc"), ("weno5z", "rusanov")])
 def test_residual_bitwise(orc, harness, recon, flux):
     for name, q in _states().items():
         r = _stage(harness, q, recon, flux)

@@ -28,7 +28,7 @@ def _smooth_state(nx, seed=5):
     return np.stack([rho, rho * u, p / (GAMMA - 1.0) + 0.5 * (rho * u) * u])
 
 
-@pytest.mark.parametrize("recon", ["teno5i", "teno5"])
+@pytest.mark.parametrize("recon", ["teno5i", "teno5", "weno5js", "weno5z"])
 @pytest.mark.parametrize("flux", ["rusanov", "hllc"])
 @pytest.mark.parametrize("case", ["sod", "smooth", "ragged"])
 def test_rhs_parity(orc, recon, flux, case):
@@ -45,7 +45,8 @@ def test_rhs_parity(orc, recon, flux, case):
     s.close()
 
 
-@pytest.mark.parametrize("recon,flux", [("teno5i", "rusanov"), ("teno5", "hllc")])
+@pytest.mark.parametrize("recon,flux", [("teno5i", "rusanov"), ("teno5", "hllc"), ("weno5js", "hllc"),
+                                        ("weno5z", "rusanov")])
 def test_steps_parity(orc, recon, flux):
     """Ten SSP-RK3 steps of the Sod problem: dt and state of every step
     equal the oracle's."""
