@@ -288,15 +288,14 @@ HTX_HD double fv1d_max_speed(double gam, double g1, const double q[3]) {
 // reconstructed from the 3-cell stencils of the two cells sharing that face;
 // flagged when |h1 - h2| / (radius |rho_c|) > 1, radius = (dx/2)^1.5 (:19).
 // rho is the density row of the state, ghosts by the boundary condition.
-// The reference visits cells 1 .. nx-2 only; the first and last cells are
-// reported unflagged.
+// The reference's loop (i = s+1 .. nxx-s-2, s = 2) covers every cell; the
+// first and last cells read the ghost values.
 HTX_HD double fv1d_rho_at(const double* rho, int g, int nx, int bc) { return rho[map_index(g, nx, bc)]; }
 
 HTX_HD bool fv1d_kxrcf(const double* rho, double mom, int c, int nx, int bc, double radius) {
     HTX_NO_CONTRACT
-    if (c < 1 || c > nx - 2) return false;
-    const double rm2 = fv1d_rho_at(rho, c - 2, nx, bc), rm1 = rho[c - 1], r0 = rho[c], rp1 = rho[c + 1];
-    const double rp2 = fv1d_rho_at(rho, c + 2, nx, bc);
+    const double rm2 = fv1d_rho_at(rho, c - 2, nx, bc), rm1 = fv1d_rho_at(rho, c - 1, nx, bc), r0 = rho[c];
+    const double rp1 = fv1d_rho_at(rho, c + 1, nx, bc), rp2 = fv1d_rho_at(rho, c + 2, nx, bc);
     double h1, h2;
     if (mom < 0.0) {
         h1 = (2.0 * rm1 + 5.0 * r0 - rp1) / 6.0;

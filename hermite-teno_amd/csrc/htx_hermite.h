@@ -34,23 +34,23 @@ inline void fill_herm_consts(HermConsts& H, double dif) {
 // cell averages and scaled derivative moments.
 HTX_HD double herm_brk(double u0, double u1, double u2, double d0, double d2) {
     HTX_NO_CONTRACT
-    return fabs((1029270.0 * u2 - 2129184.0 * u1 + 1099914.0 * u0 + 281121.0 * d0 - 494958.0 * d2) * d0 +
+    const double v = fabs((1029270.0 * u2 - 2129184.0 * u1 + 1099914.0 * u0 + 281121.0 * d0 - 494958.0 * d2) * d0 +
                 (-1029270.0 * u0 + 2129184.0 * u1 - 1099914.0 * u2 + 281121.0 * d2) * d2 +
                 (1099445.0 * u2 - 4317056.0 * u1 + 2118166.0 * u0) * u2 + 4317056.0 * (u1 - u0) * u1 +
-                1099445.0 * u0 * u0) /
-           6720.0;
+                1099445.0 * u0 * u0);
+    return quot_const(v, 6720.0, 1.0 / 6720.0);
 }
 
 // Hermite::fullPointsSId (:605-612)
 HTX_HD double herm_brkd(double u0, double u1, double u2, double d0, double d1, double d2) {
     HTX_NO_CONTRACT
-    return fabs((3156862.0 * d2 + 13189240.0 * d1 + 1710479.0 * d0 - 9617754.0 * u2 - 531552.0 * u1 +
+    const double v = fabs((3156862.0 * d2 + 13189240.0 * d1 + 1710479.0 * d0 - 9617754.0 * u2 - 531552.0 * u1 +
                  10149306.0 * u0) * d0 +
                 (13189240.0 * d2 + 26511200.0 * d1 - 39700440.0 * u2 + 39700440.0 * u0) * d1 +
                 (1710479.0 * d2 - 10149306.0 * u2 + 531552.0 * u1 + 9617754.0 * u0) * d2 +
                 (15136011.0 * u2 - 1076544.0 * u1 - 29195478.0 * u0) * u2 + 1076544.0 * (u1 - u0) * u1 +
-                15136011.0 * u0 * u0) /
-           1680.0;
+                15136011.0 * u0 * u0);
+    return quot_const(v, 1680.0, 1.0 / 1680.0);
 }
 
 // one HTENO side (Schemes_Hermite.cpp:311-320 pattern): tau from the full
@@ -59,23 +59,7 @@ HTX_HD double hteno_side(double brk, double s0, double s1, double s2, double p0,
                          double d1, double d2) {
     HTX_NO_CONTRACT
     double tau = fabs(brk - 0.25 * (s0 + s1 + 2.0 * s2));
-    double g0 = 1.0 + tau / (s0 + 1e-40);
-    double g1 = 1.0 + tau / (s1 + 1e-40);
-    double g2 = 1.0 + tau / (s2 + 1e-40);
-    g0 = g0 * g0 * g0; g0 = g0 * g0;
-    g1 = g1 * g1 * g1; g1 = g1 * g1;
-    g2 = g2 * g2 * g2; g2 = g2 * g2;
-    double gs = (g0 + g1) + g2;
-    bool k0 = !(g0 / gs < 1e-3), k1 = !(g1 / gs < 1e-3), k2 = !(g2 / gs < 1e-3);
-    double w0 = k0 ? d0 : 0.0, w1 = k1 ? d1 : 0.0, w2 = k2 ? d2 : 0.0;
-    double ws = (w0 + w1) + w2;
-#if HTX_EXACT_DIVISION
-    w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
-    double out = (p0 * w0 + p1 * w1) + p2 * w2;
-#else
-    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * drcp(ws);
-#endif
-    return out != out ? 1e-15 : out;
+    return teno_combine(teno_keep(tau, s0, s1, s2, 1e-3), p0, p1, p2, d0, d1, d2);
 }
 
 // Mixed-precision HTENO side: the indicators brk, s_k come in FP64 (formed

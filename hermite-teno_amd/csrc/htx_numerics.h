@@ -142,6 +142,133 @@ HTX_HD double drcp(double x) {
 #endif
 }
 
+// Correctly rounded FP64 quotients x / b that share the divisor's reciprocal
+// (strict build).  gfx950 divides as follows: v_div_scale pre-scales both
+// operands, y = v_rcp_f64(b) is refined by two Newton steps
+// (e = 1 - b y, y = y + y e, twice), q0 = x y, r = x - b q0 (exact with an
+// FMA), q = q0 + r y (v_div_fmas), and v_div_fixup handles special values.
+// For a positive divisor b in [2^-300, 2^300] and a numerator x = +-0 or
+// 2^-400 <= |x| <= 2^400 the scaling and the fix-up leave every value
+// unchanged, so the same operations without them give the same bits, and
+// the reciprocal steps are shared by all quotients of one divisor: three
+// instructions per quotient instead of eleven.  The residual is formed
+// negated (b q0 - x, exact as well) so that x = -0 gives -0 as the division
+// does.  Callers guarantee the ranges (or test them and divide otherwise).
+// On the host every quotient is the plain division, the same correctly
+// rounded value.
+struct DivBy {
+    double b, y;
+};
+
+HTX_HD DivBy div_by(double b) {
+    DivBy d;
+    d.b = b;
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-b, y, 1.0);
+    d.y = fma(y, e, y);
+#else
+    d.y = 1.0 / b;
+#endif
+    return d;
+}
+
+HTX_HD double quot(const DivBy& d, double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double q0 = x * d.y;
+    const double nr = fma(d.b, q0, -x);
+    return fma(-nr, d.y, q0);
+#else
+    return x / d.b;
+#endif
+}
+
+// x / c for a constant divisor c given with yc = 1 / c correctly rounded
+// (a compile-time constant): with y the correctly rounded reciprocal, one
+// correction of q0 = x y by the exact residual gives the correctly rounded
+// quotient (Markstein's theorem), for numerators far from the exponent
+// limits; others (and zero, whose sign the correction may drop) are divided.
+HTX_HD double quot_const(double x, double c, double yc) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double ax = fabs(x);
+    if (!(ax >= 0x1p-400 && ax <= 0x1p400)) return x / c;
+    const double q0 = x * yc;
+    const double nr = fma(c, q0, -x);
+    return fma(-nr, yc, q0);
+#else
+    (void)yc;
+    return x / c;
+#endif
+}
+
+// TENO cut-off decisions of one side (cutOff, Schemes_Lagrange.cpp:1141-1149;
+// Schemes_Hermite.cpp:614-628): candidate k is kept unless g_k / sum g <
+// ct, g_k = (1 + tau / (s_k + 1e-40))^6.  The sixth power is formed as a
+// cube squared; in the strict build, where a ratio falls within 1e-13 of ct
+// (there the power's last-bit rounding could decide), the powers are formed
+// again with pow(., 6), as the reference's Eigen .pow(6) does.  NaN ratios
+// (inf / inf) keep the candidate.  Returns the kept mask (bit k).
+HTX_HD unsigned teno_keep(double tau, double s0, double s1, double s2, double ct) {
+    HTX_NO_CONTRACT
+    double a0 = 1.0 + tau / (s0 + 1e-40);
+    double a1 = 1.0 + tau / (s1 + 1e-40);
+    double a2 = 1.0 + tau / (s2 + 1e-40);
+    double g0 = a0 * a0 * a0; g0 = g0 * g0;
+    double g1 = a1 * a1 * a1; g1 = g1 * g1;
+    double g2 = a2 * a2 * a2; g2 = g2 * g2;
+    double gs = (g0 + g1) + g2;
+    // g_k >= 1, so with gs <= 2^300 every quotient is in the shared-reciprocal
+    // range (the correctly rounded quotient, in both builds)
+    double r0, r1, r2;
+    {
+        const DivBy D = div_by(gs);
+        r0 = quot(D, g0);
+        r1 = quot(D, g1);
+        r2 = quot(D, g2);
+    }
+    if (!(gs <= 0x1p300)) {
+        r0 = g0 / gs;
+        r1 = g1 / gs;
+        r2 = g2 / gs;
+    }
+#if HTX_EXACT_DIVISION
+    const double band = ct * 1e-13;
+    if (fabs(r0 - ct) <= band || fabs(r1 - ct) <= band || fabs(r2 - ct) <= band) {
+        g0 = pow(a0, 6.0);
+        g1 = pow(a1, 6.0);
+        g2 = pow(a2, 6.0);
+        gs = (g0 + g1) + g2;
+        r0 = g0 / gs;
+        r1 = g1 / gs;
+        r2 = g2 / gs;
+    }
+#endif
+    return (r0 < ct ? 0u : 1u) | (r1 < ct ? 0u : 2u) | (r2 < ct ? 0u : 4u);
+}
+
+// p0 w0 + p1 w1 + p2 w2 with the kept linear weights normalised,
+// w_k = d_k / ((d'_0 + d'_1) + d'_2), d'_k = d_k if kept else 0, as
+// Schemes_Lagrange.cpp:496-498 evaluates it, then catchnan.  At least one
+// candidate is always kept, so the divisor lies in [min d, 1] and the
+// quotients share its reciprocal in the strict build.
+HTX_HD double teno_combine(unsigned keep, double p0, double p1, double p2, double d0, double d1, double d2) {
+    HTX_NO_CONTRACT
+    double w0 = (keep & 1u) ? d0 : 0.0, w1 = (keep & 2u) ? d1 : 0.0, w2 = (keep & 4u) ? d2 : 0.0;
+    double ws = (w0 + w1) + w2;
+#if HTX_EXACT_DIVISION
+    const DivBy D = div_by(ws);
+    w0 = quot(D, w0);
+    w1 = quot(D, w1);
+    w2 = quot(D, w2);
+    double out = (p0 * w0 + p1 * w1) + p2 * w2;
+#else
+    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * drcp(ws);
+#endif
+    return out != out ? 1e-15 : out;  // catchnan
+}
+
 // 1 / x where the reference divides: the correctly rounded quotient in the
 // strict build, drcp in the fast one
 HTX_HD double inv(double x) {
@@ -184,27 +311,11 @@ HTX_HD double teno5_side(double v0, double v1, double v2, double v3, double v4,
                  v3 * (104963.0 * v3 - 299076.0 * v2 + 179098.0 * v1 - 38947.0 * v0) +
                  v2 * (231153.0 * v2 - 299076.0 * v1 + 67923.0 * v0) +
                  v1 * (104963.0 * v1 - 51001.0 * v0) + 6908.0 * v0 * v0;
-    b35 = b35 / 5040.0;
-    double tau = fabs(b35 - (s0 + s2 + 4.0 * s1) / 6.0);
-    double g0 = 1.0 + tau / (s0 + 1e-40);
-    double g1 = 1.0 + tau / (s1 + 1e-40);
-    double g2 = 1.0 + tau / (s2 + 1e-40);
-    g0 = g0 * g0 * g0; g0 = g0 * g0;
-    g1 = g1 * g1 * g1; g1 = g1 * g1;
-    g2 = g2 * g2 * g2; g2 = g2 * g2;
-    double gs = (g0 + g1) + g2;
-    // cutOff: w/sum < C_T -> discard.  NaN (inf/inf) compares false -> kept.
-    bool k0 = !(g0 / gs < 5e-4), k1 = !(g1 / gs < 5e-4), k2 = !(g2 / gs < 5e-4);
-    double w0 = k0 ? d0 : 0.0, w1 = k1 ? d1 : 0.0, w2 = k2 ? d2 : 0.0;
-    double ws = (w0 + w1) + w2;
-#if HTX_EXACT_DIVISION
-    w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
-    double out = (p0 * w0 + p1 * w1) + p2 * w2;
-#else
-    double out = ((p0 * w0 + p1 * w1) + p2 * w2) * drcp(ws);  // all cut: drcp(0) is NaN -> catchnan
-#endif
-    if (cut) *cut = (k0 ? 0u : 1u) | (k1 ? 0u : 2u) | (k2 ? 0u : 4u);
-    return out != out ? 1e-15 : out;  // catchnan
+    b35 = quot_const(b35, 5040.0, 1.0 / 5040.0);
+    double tau = fabs(b35 - quot_const((s0 + s2 + 4.0 * s1), 6.0, 1.0 / 6.0));
+    const unsigned keep = teno_keep(tau, s0, s1, s2, 5e-4);
+    if (cut) *cut = keep ^ 7u;
+    return teno_combine(keep, p0, p1, p2, d0, d1, d2);
 }
 
 // Mixed-precision TENO5 side: the same cut-off rule evaluated in FP32 on
@@ -316,6 +427,24 @@ HTX_HD void teno5_mixed(const double u[6], double& um, double& up) {
     up = teno5_side_mixed(u[1], u[2], u[3], u[4], u[5], q0, q1, q2, 0.1, 0.6, 0.3);
 }
 
+// Normalised WENO weights lin_k / e_k / sum_j (lin_j / e_j) with the
+// divisions of weno5JS3PointsQuadrature (Schemes_Lagrange.cpp:1178-1210):
+// each lin_k / e_k, then each weight by their sum; the e_k reciprocals come
+// shared across the quadrature's four weight sets.  Valid (the correctly
+// rounded quotients) for lin_k in (0, 1] and e_k in [1e-12, 2^200]: the
+// weights lie in [2^-210, 2^40], their sum in the same range and the
+// normalised weights in [2^-250, 1].
+HTX_HD void weno_weights_shared(const DivBy E[3], const double lin[3], double& w0, double& w1, double& w2) {
+    HTX_NO_CONTRACT
+    w0 = quot(E[0], lin[0]);
+    w1 = quot(E[1], lin[1]);
+    w2 = quot(E[2], lin[2]);
+    const DivBy S = div_by((w0 + w1) + w2);
+    w0 = quot(S, w0);
+    w1 = quot(S, w1);
+    w2 = quot(S, w2);
+}
+
 // Schemes::Lagrange::weno5JS3PointsQuadrature + smoothnessMeasure3L
 // (Schemes_Lagrange.cpp:1155-1217): 5 transverse values -> values at the
 // Gauss points (+xi, 0, -xi).
@@ -332,6 +461,27 @@ HTX_HD void weno3q(const QuadConsts& C, double u0, double u1, double u2, double 
     e0 = e0 * e0; e1 = e1 * e1; e2 = e2 * e2;
     double w0, w1, w2, ws, p0, p1, p2;
 #if HTX_EXACT_DIVISION
+    if (e0 <= 0x1p200 && e1 <= 0x1p200 && e2 <= 0x1p200) {
+        // every division in the shared-reciprocal range (NaN fails the test)
+        const DivBy E[3] = {div_by(e0), div_by(e1), div_by(e2)};
+        p0 = (C.gp[0][0] * u0 + C.gp[1][0] * u1) + C.gp[2][0] * u2;
+        p1 = (C.gp[0][1] * u1 + C.gp[1][1] * u2) + C.gp[2][1] * u3;
+        p2 = (C.gp[0][2] * u2 + C.gp[1][2] * u3) + C.gp[2][2] * u4;
+        weno_weights_shared(E, C.linwp, w0, w1, w2);
+        o0 = (p0 * w0 + p1 * w1) + p2 * w2;
+        p0 = quot_const(-u0 + 2.0 * u1 + 23.0 * u2, 24.0, 1.0 / 24.0);
+        p1 = quot_const(-u1 + 26.0 * u2 - u3, 24.0, 1.0 / 24.0);
+        p2 = quot_const(23.0 * u2 + 2.0 * u3 - u4, 24.0, 1.0 / 24.0);
+        weno_weights_shared(E, C.linw0p, w0, w1, w2);
+        o1 = C.sumw0p * ((p0 * w0 + p1 * w1) + p2 * w2);
+        weno_weights_shared(E, C.linw0m, w0, w1, w2);
+        o1 -= C.sumw0m * ((p0 * w0 + p1 * w1) + p2 * w2);
+        p0 = (C.gm[0][0] * u0 + C.gm[1][0] * u1) + C.gm[2][0] * u2;
+        p1 = (C.gm[0][1] * u1 + C.gm[1][1] * u2) + C.gm[2][1] * u3;
+        p2 = (C.gm[0][2] * u2 + C.gm[1][2] * u3) + C.gm[2][2] * u4;
+        weno_weights_shared(E, C.linwm, w0, w1, w2);
+        o2 = (p0 * w0 + p1 * w1) + p2 * w2;
+    } else {
     // +xi
     p0 = (C.gp[0][0] * u0 + C.gp[1][0] * u1) + C.gp[2][0] * u2;
     p1 = (C.gp[0][1] * u1 + C.gp[1][1] * u2) + C.gp[2][1] * u3;
@@ -360,6 +510,7 @@ HTX_HD void weno3q(const QuadConsts& C, double u0, double u1, double u2, double 
     ws = (w0 + w1) + w2;
     w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
     o2 = (p0 * w0 + p1 * w1) + p2 * w2;
+    }
 #else
     // 3 reciprocals of (eps + beta_k)^2 shared by the four weight sets, one
     // reciprocal per normalisation
@@ -468,11 +619,25 @@ HTX_HD void weno3q_gp3(const QuadConsts& C, double u0, double u1, double u2, dou
     e0 = e0 * e0; e1 = e1 * e1; e2 = e2 * e2;
 #if !HTX_EXACT_DIVISION
     const double r0 = drcp(e0), r1 = drcp(e1), r2 = drcp(e2);
+#else
+    // shared reciprocals when every division is in their range (see
+    // weno_weights_shared; NaN fails the test)
+    const bool shared = e0 <= 0x1p200 && e1 <= 0x1p200 && e2 <= 0x1p200;
+    const DivBy E[3] = {div_by(e0), div_by(e1), div_by(e2)};
 #endif
     double w0, w1, w2, ws, p0, p1, p2, v;
     // centre
     {
 #if HTX_EXACT_DIVISION
+        if (shared) {
+            p0 = quot_const(-u0 + 2.0 * u1 + 23.0 * u2, 24.0, 1.0 / 24.0);
+            p1 = quot_const(-u1 + 26.0 * u2 - u3, 24.0, 1.0 / 24.0);
+            p2 = quot_const(23.0 * u2 + 2.0 * u3 - u4, 24.0, 1.0 / 24.0);
+            weno_weights_shared(E, C.linw0p, w0, w1, w2);
+            v = C.sumw0p * ((p0 * w0 + p1 * w1) + p2 * w2);
+            weno_weights_shared(E, C.linw0m, w0, w1, w2);
+            v -= C.sumw0m * ((p0 * w0 + p1 * w1) + p2 * w2);
+        } else {
         p0 = (-u0 + 2.0 * u1 + 23.0 * u2) / 24.0;
         p1 = (-u1 + 26.0 * u2 - u3) / 24.0;
         p2 = (23.0 * u2 + 2.0 * u3 - u4) / 24.0;
@@ -484,6 +649,7 @@ HTX_HD void weno3q_gp3(const QuadConsts& C, double u0, double u1, double u2, dou
         ws = (w0 + w1) + w2;
         w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
         v -= C.sumw0m * ((p0 * w0 + p1 * w1) + p2 * w2);
+        }
 #else
         const double r24 = 1.0 / 24.0;
         p0 = (-u0 + 2.0 * u1 + 23.0 * u2) * r24;
@@ -506,9 +672,13 @@ HTX_HD void weno3q_gp3(const QuadConsts& C, double u0, double u1, double u2, dou
         p1 = (cc[0][1] * u1 + cc[1][1] * u2) + cc[2][1] * u3;
         p2 = (cc[0][2] * u2 + cc[1][2] * u3) + cc[2][2] * u4;
 #if HTX_EXACT_DIVISION
-        w0 = lw[0] / e0; w1 = lw[1] / e1; w2 = lw[2] / e2;
-        ws = (w0 + w1) + w2;
-        w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
+        if (shared) {
+            weno_weights_shared(E, lw, w0, w1, w2);
+        } else {
+            w0 = lw[0] / e0; w1 = lw[1] / e1; w2 = lw[2] / e2;
+            ws = (w0 + w1) + w2;
+            w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
+        }
         v = (p0 * w0 + p1 * w1) + p2 * w2;
 #else
         w0 = lw[0] * r0; w1 = lw[1] * r1; w2 = lw[2] * r2;

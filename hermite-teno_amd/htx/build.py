@@ -65,12 +65,19 @@ def build_htx(extra=()):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     objdir = os.path.join(PKG_DIR, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in HIP_SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         flags = [f for f in HIPCC_FLAGS if f != "-shared"] + PER_FILE_FLAGS.get(src, [])
-        _run([hipcc, *flags, *extra, "-c", "-o", obj, os.path.join(CSRC, src)])
+        cmds.append([hipcc, *flags, *extra, "-c", "-o", obj, os.path.join(CSRC, src)])
         objs.append(obj)
+    # the translation units are independent: compile them side by side
+    from concurrent.futures import ThreadPoolExecutor
+
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
     tmp = LIBHTX + ".tmp"
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIBHTX)
@@ -130,9 +137,22 @@ def build_examples():
     return EXAMPLE_BIN
 
 
+GPU_HARNESS_DIV = os.path.join(ROOT, "tests", "gpu_harness", "libdivcheck.so")
+
+
+def build_gpu_harness():
+    """GPU test harnesses (tests/gpu_harness): the shared-reciprocal division
+    check, built like the product's strict kernels."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    flags = [f for f in HIPCC_FLAGS if not f.startswith("-ffp-contract")] + ["-ffp-contract=off"]
+    _run([hipcc, *flags, "-o", GPU_HARNESS_DIV, os.path.join(ROOT, "tests", "gpu_harness", "divcheck.hip")])
+    return GPU_HARNESS_DIV
+
+
 def build_all():
     build_oracle()
     build_host_harness()
+    build_gpu_harness()
     build_htx()
     build_examples()
 
@@ -141,4 +161,5 @@ if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
         {"all": build_all, "htx": build_htx, "oracle": build_oracle, "harness": build_host_harness,
+         "gpu_harness": build_gpu_harness,
          "examples": build_examples}[w]()

@@ -117,7 +117,7 @@ class System1D:
     def troubledCells(self):
         """KXRCF troubled-cell flags (uint8, nx; 1 = discontinuous) of the
         current state, as FV::System1D::rhs computes them
-        (FV_System1D.cpp:166-192); cells 0 and nx-1 are never flagged."""
+        (FV_System1D.cpp:166-192); the first and last cells read the ghost cells."""
         f = np.zeros(self.nx, dtype=np.uint8)
         check(_lib.load().htx_kxrcf_flags(self._h, f.ctypes.data))
         return f

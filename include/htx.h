@@ -270,8 +270,9 @@ int htx_classify(htx_solver* h, int dir, float* logits, float* feat);
  * radius circleRadMod = (dx/2)^1.5, :19), here for the current state: the
  * density at the face upwind of the momentum's sign, reconstructed from the
  * 3-cell stencils of the two cells sharing it, jumps by more than radius
- * |rho|.  flags holds nx bytes; cells 0 and nx-1, which the reference never
- * visits, are 0.  The reference's 1D scheme ignores the flags (both of its
+ * |rho|.  flags holds nx bytes, one per cell; the first and last cells read
+ * the ghost cells of the boundary condition, as the reference's loop
+ * (i = s+1 .. nxx-s-2, s = 2) does.  The reference's 1D scheme ignores the flags (both of its
  * branches take teno5i); they are reported as a classical detector's flag
  * rate. */
 int htx_kxrcf_flags(htx_solver* h, unsigned char* flags);

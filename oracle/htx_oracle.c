@@ -312,6 +312,31 @@ void orc_weno5js3q(const double u[5], double out[3]) {
     for (int k = 0; k < 3; ++k) out[k] = catchnan(out[k]);
 }
 
+/* weno5JS3PointsQuadrature's linear limit (Schemes_Lagrange.cpp:1171-1217
+ * with every nonlinear weight replaced by its linear weight, :86-105): the
+ * candidates of each Gauss point combined with linwp, linw0p / linw0m (the
+ * centre's positive and negative groups) and linwm.  No reference routine
+ * computes it; the learned path's transverse quadrature on smooth windows is
+ * this combination. */
+void orc_weno5js3q_linear(const double u[5], double out[3]) {
+    double p[3], c[3];
+    lag_init();
+    for (int k = 0; k < 3; ++k) {
+        for (int r = 0; r < 3; ++r) c[r] = LC.crjgp3L[r][k];
+        p[k] = dot3(c, u + k);
+    }
+    out[0] = dot3(p, LC.linwp);
+    p[0] = (-u[0] + 2 * u[1] + 23.0 * u[2]) / 24.0;
+    p[1] = (-u[1] + 26.0 * u[2] - u[3]) / 24.0;
+    p[2] = (23.0 * u[2] + 2.0 * u[3] - u[4]) / 24.0;
+    out[1] = LC.sumw0p * dot3(p, LC.linw0p) - LC.sumw0m * dot3(p, LC.linw0m);
+    for (int k = 0; k < 3; ++k) {
+        for (int r = 0; r < 3; ++r) c[r] = LC.crjgm3L[r][k];
+        p[k] = dot3(c, u + k);
+    }
+    out[2] = dot3(p, LC.linwm);
+}
+
 /* Gauss-Legendre 9-point rule in the reference's node order
  * (Schemes_Lagrange.cpp:22-42): 0, -a1, +a1, -a2, +a2, -a3, +a3, -a4, +a4. */
 void orc_gauss9(double nodes[9], double weights[9]) {
@@ -591,7 +616,17 @@ static inline void cell_q(const padded* P, int i, int j, double out[4]) {
  * values of rows j-2..j+2 (:402-423), back-projection with the face's own R
  * (:425-430, FV_System2D.h:50-58), flux at the 3 Gauss points weighted by
  * (5/18, 4/9, 5/18) (:432-497), divergence (:562-572). */
-void orc_fv2d_rhs(const orc_params* p, const double* q, double* rhs) {
+/* linear = 1: FV::System2D::rhs with Lagrange::linear (Schemes_Lagrange.cpp
+ * :1219-1233) in place of teno5 and orc_weno5js3q_linear in place of the
+ * transverse WENO quadrature (what the learned path computes where every
+ * face is flagged smooth). */
+static void fv2d_rhs_impl(const orc_params* p, const double* q, double* rhs, int linear);
+
+void orc_fv2d_rhs(const orc_params* p, const double* q, double* rhs) { fv2d_rhs_impl(p, q, rhs, 0); }
+
+void orc_fv2d_rhs_linear(const orc_params* p, const double* q, double* rhs) { fv2d_rhs_impl(p, q, rhs, 1); }
+
+static void fv2d_rhs_impl(const orc_params* p, const double* q, double* rhs, int linear) {
     lag_init();
     padded P;
     int nx = p->nx, ny = p->ny, k = FV_K, s = FV_S;
@@ -625,7 +660,10 @@ void orc_fv2d_rhs(const orc_params* p, const double* q, double* rhs) {
                 for (int e = 0; e < 4; ++e) C[e][m] = y[e];
             }
             for (int e = 0; e < 4; ++e) {
-                teno5_impl(C[e], out, NULL);
+                if (linear)
+                    orc_linear5(C[e], out);
+                else
+                    teno5_impl(C[e], out, NULL);
                 uxl[e * nt + PIX(&P, i, j)] = out[0];
                 uxr[e * nt + PIX(&P, i, j)] = out[1];
             }
@@ -646,7 +684,10 @@ void orc_fv2d_rhs(const orc_params* p, const double* q, double* rhs) {
                 for (int e = 0; e < 4; ++e) C[e][m] = y[e];
             }
             for (int e = 0; e < 4; ++e) {
-                teno5_impl(C[e], out, NULL);
+                if (linear)
+                    orc_linear5(C[e], out);
+                else
+                    teno5_impl(C[e], out, NULL);
                 uyl[e * nt + PIX(&P, i, j)] = out[0];
                 uyr[e * nt + PIX(&P, i, j)] = out[1];
             }
@@ -671,8 +712,13 @@ void orc_fv2d_rhs(const orc_params* p, const double* q, double* rhs) {
                         a5[m] = (dir == 1 ? uxl : uyl)[e * nt + id];
                         b5[m] = (dir == 1 ? uxr : uyr)[e * nt + id];
                     }
-                    orc_weno5js3q(a5, gl[e]);
-                    orc_weno5js3q(b5, gr[e]);
+                    if (linear) {
+                        orc_weno5js3q_linear(a5, gl[e]);
+                        orc_weno5js3q_linear(b5, gr[e]);
+                    } else {
+                        orc_weno5js3q(a5, gl[e]);
+                        orc_weno5js3q(b5, gr[e]);
+                    }
                 }
                 double tf[3][4];
                 for (int m = 0; m < 3; ++m) {
@@ -929,7 +975,7 @@ static double herm_brkd(const double* u, const double* d) {
 
 /* one HTENO side: tau from the full-stencil indicator, gamma^6, cutOff at
  * C_T = 1e-3 (:614-628), weights, catchnan */
-static double hteno_side(double brk, const double SI[3], const double p[3], const double d[3]) {
+static double hteno_side(double brk, const double SI[3], const double p[3], const double d[3], int* sig_out) {
     double w[3], sigm[3];
     double tau = fabs(brk - 0.25 * (SI[0] + SI[1] + 2.0 * SI[2]));
     for (int k = 0; k < 3; ++k) w[k] = pow(1.0 + tau / (SI[k] + 1E-40), 6);
@@ -939,13 +985,15 @@ static double hteno_side(double brk, const double SI[3], const double p[3], cons
     for (int k = 0; k < 3; ++k) w[k] = sigm[k] * d[k];
     s = sum3(w);
     for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    if (sig_out)
+        for (int k = 0; k < 3; ++k) sig_out[k] = (int)sigm[k];
     return catchnan(dot3(p, w));
 }
 
 /* Hermite::hteno, Schemes_Hermite.cpp:295-379: cell averages u[0..3] =
  * u_{i-1..i+2} and derivative moments d[0..3] -> (u^-, u^+, d^-, d^+) at
  * x_{i+1/2}. */
-static void hteno_impl(const herm_consts* H, const double u[4], const double d[4], double out[4]) {
+static void hteno_impl(const herm_consts* H, const double u[4], const double d[4], double out[4], int* flags) {
     double ud[4], p[3], SI[3];
     for (int k = 0; k < 4; ++k) ud[k] = d[k] * H->dif;
     /* u^- :304-320 */
@@ -953,32 +1001,40 @@ static void hteno_impl(const herm_consts* H, const double u[4], const double d[4
     p[1] = H->crj[0][1] * u[1] + H->crj[1][1] * u[2] + H->crj[2][1] * d[2];
     p[2] = H->crj[0][2] * u[0] + H->crj[1][2] * u[1] + H->crj[2][2] * u[2];
     herm_si(u, ud, SI);
-    out[0] = hteno_side(herm_brk(u, ud), SI, p, H->dkm);
+    out[0] = hteno_side(herm_brk(u, ud), SI, p, H->dkm, flags ? flags : NULL);
     /* d^- :323-339 */
     p[0] = H->crjd[0][0] * u[0] + H->crjd[1][0] * u[1] + H->crjd[2][0] * d[0] + H->crjd[3][0] * d[1];
     p[1] = H->crjd[0][1] * u[1] + H->crjd[1][1] * u[2] + H->crjd[2][1] * d[1] + H->crjd[3][1] * d[2];
     p[2] = H->crjd[0][2] * u[0] + H->crjd[1][2] * u[1] + H->crjd[2][2] * u[2] + H->crjd[3][2] * d[1];
     herm_sid(u, ud, SI);
-    out[2] = hteno_side(herm_brkd(u, ud), SI, p, H->dkmd);
+    out[2] = hteno_side(herm_brkd(u, ud), SI, p, H->dkmd, flags ? flags + 6 : NULL);
     /* u^+ :342-357 */
     p[0] = H->crj[1][1] * u[1] + H->crj[0][1] * u[2] - H->crj[2][1] * d[1];
     p[1] = H->crj[1][0] * u[2] + H->crj[0][0] * u[3] - H->crj[2][0] * d[3];
     p[2] = H->crj[2][2] * u[1] + H->crj[1][2] * u[2] + H->crj[0][2] * u[3];
     herm_si(u + 1, ud + 1, SI);
-    out[1] = hteno_side(herm_brk(u + 1, ud + 1), SI, p, H->dkp);
+    out[1] = hteno_side(herm_brk(u + 1, ud + 1), SI, p, H->dkp, flags ? flags + 3 : NULL);
     /* d^+ :360-376 */
     p[0] = H->crjd[0][1] * u[1] + H->crjd[1][1] * u[2] + H->crjd[3][1] * d[1] + H->crjd[2][1] * d[2];
     p[1] = H->crjd[0][0] * u[2] + H->crjd[1][0] * u[3] + H->crjd[3][0] * d[2] + H->crjd[2][0] * d[3];
     p[2] = -(H->crjd[2][2] * u[1] + H->crjd[1][2] * u[2] + H->crjd[0][2] * u[3]) + H->crjd[3][2] * d[2];
     herm_sid(u + 1, ud + 1, SI);
-    out[3] = hteno_side(herm_brkd(u + 1, ud + 1), SI, p, H->dkpd);
+    out[3] = hteno_side(herm_brkd(u + 1, ud + 1), SI, p, H->dkpd, flags ? flags + 9 : NULL);
 }
 
 void orc_hteno(double dif, const double u[4], const double d[4], double out[4]) {
     herm_consts H;
     lag_init();
     herm_init(&H, dif);
-    hteno_impl(&H, u, d, out);
+    hteno_impl(&H, u, d, out, NULL);
+}
+
+void orc_hteno_flags(double dif, const double u[4], const double d[4], int flags[12]) {
+    herm_consts H;
+    double out[4];
+    lag_init();
+    herm_init(&H, dif);
+    hteno_impl(&H, u, d, out, flags);
 }
 
 /* Hermite::hermiteLimiter, Schemes_Hermite.cpp:634-662 */
@@ -1153,7 +1209,7 @@ static void fvh2d_rhs_impl(const orc_params* p, const double* q, double* rhs, do
             }
             size_t id = PIX(&P, i, j);
             for (int e = 0; e < 4; ++e) {
-                hteno_impl(&H, A[e], B[e], o4);
+                hteno_impl(&H, A[e], B[e], o4, NULL);
                 SER(FX, 0, e, id) = o4[0];
                 SER(FX, 1, e, id) = o4[1];
                 SER(FX, 2, e, id) = o4[2];
@@ -1186,7 +1242,7 @@ static void fvh2d_rhs_impl(const orc_params* p, const double* q, double* rhs, do
             }
             size_t id = PIX(&P, i, j);
             for (int e = 0; e < 4; ++e) {
-                hteno_impl(&H, A[e], B[e], o4);
+                hteno_impl(&H, A[e], B[e], o4, NULL);
                 SER(FY, 0, e, id) = o4[0];  /* uyl */
                 SER(FY, 1, e, id) = o4[1];  /* uyr */
                 SER(FY, 4, e, id) = o4[2];  /* wyl */
@@ -1570,11 +1626,13 @@ void orc_fv1d_rhs(int nx, double xl, double xr, double gamma, int recon, int flu
 }
 
 /* The KXRCF troubled-cell flags of FV::System1D::rhs (FV_System1D.cpp:166-192,
- * radius circleRadMod = (dx/2)^1.5 from :19): for each cell i = k+1 .. nxx-k-2
- * of the zero-gradient ghost-padded density, the value at the face upwind
- * of the momentum's sign reconstructed from the stencils of the two cells
- * sharing it; flagged when |h1 - h2| / (radius |rho_i|) > 1.  The reference
- * never visits the first and last interior cells: they come out 0. */
+ * radius circleRadMod = (dx/2)^1.5 from :19): the reference's loop runs over
+ * i = s+1 .. nxx-s-2 with s = 2 (:15, :166), i.e. every interior cell and
+ * two ghost cells on each side, of the zero-gradient ghost-padded density;
+ * for each cell the value at the face upwind of the momentum's sign is
+ * reconstructed from the stencils of the two cells sharing it, and the cell
+ * is flagged when |h1 - h2| / (radius |rho_i|) > 1.  The interior cells'
+ * flags are returned (the first and last cells read the ghosts). */
 void orc_fv1d_kxrcf(int nx, double xl, double xr, const double* q, unsigned char* flags) {
     int k = FV1_K, nxx = nx + 2 * k;
     double dx = (xr - xl) / nx, radius = pow(dx / 2.0, 1.5);
@@ -1590,8 +1648,8 @@ void orc_fv1d_kxrcf(int nx, double xl, double xr, const double* q, unsigned char
         m[k - 1 - g] = m[k];
         m[nxx - 1 - g] = m[nxx - k - 1];
     }
-    for (int i = 0; i < nx; ++i) flags[i] = 0;
-    for (int i = k + 1; i < nxx - k - 1; ++i) {
+    const int s = 2; /* FV_System1D.cpp:15 */
+    for (int i = s + 1; i < nxx - s - 1; ++i) {
         double h1, h2;
         if (m[i] < 0.0) {
             h1 = (2.0 * r[i - 1] + 5.0 * r[i] - r[i + 1]) / 6.0;
@@ -1600,7 +1658,8 @@ void orc_fv1d_kxrcf(int nx, double xl, double xr, const double* q, unsigned char
             h2 = (2.0 * r[i] + 5.0 * r[i + 1] - r[i + 2]) / 6.0;
             h1 = (-r[i - 1] + 5.0 * r[i] + 2.0 * r[i + 1]) / 6.0;
         }
-        flags[i - k] = fabs(h1 - h2) / (radius * fabs(r[i])) > 1.0;
+        const int fl = fabs(h1 - h2) / (radius * fabs(r[i])) > 1.0;
+        if (i >= k && i < nxx - k) flags[i - k] = (unsigned char)fl;
     }
     free(r);
     free(m);

@@ -35,6 +35,7 @@ void orc_teno5(const double u[6], double out[2]);
 void orc_linear5(const double u[6], double out[2]);
 void orc_teno5_flags(const double u[6], int flags[6]);
 void orc_weno5js3q(const double u[5], double out[3]);
+void orc_weno5js3q_linear(const double u[5], double out[3]);
 void orc_primitive(double gamma, const double q[4], double w[4]);
 void orc_conservative(double gamma, const double w[4], double q[4]);
 void orc_eigen(double gamma, int dir, const double qa[4], const double qb[4],
@@ -50,6 +51,9 @@ void orc_gauss9(double nodes[9], double weights[9]);
 
 /* ---- 2D FV (TENO5 + 3-point transverse quadrature) ---- */
 void orc_fv2d_rhs(const orc_params* p, const double* q, double* rhs);
+/* the same residual with linear (optimal-weight) reconstruction and linear
+ * transverse quadrature on every face */
+void orc_fv2d_rhs_linear(const orc_params* p, const double* q, double* rhs);
 /* TENO5 cut-off decisions per face (label 1 = nothing cut), dir 1 x, 2 y */
 void orc_face_labels(const orc_params* p, const double* q, int dir, unsigned char* labels);
 void orc_fv2d_max_speed(const orc_params* p, const double* q, double* sx, double* sy);
@@ -60,6 +64,9 @@ int orc_fv2d_advance(const orc_params* p, double* q, double cfl, double t0, doub
 
 /* ---- 2D Hermite-TENO (FVHermite::System2D): 12 fields u, <dq/dx>, <dq/dy> ---- */
 void orc_hteno(double dif, const double u[4], const double d[4], double out[4]);
+/* cut-off flags (1 = kept) of hteno's four reconstructions: u^- [0..2],
+ * u^+ [3..5], d^- [6..8], d^+ [9..11] */
+void orc_hteno_flags(double dif, const double u[4], const double d[4], int flags[12]);
 double orc_hermite_limiter(double dif, const double u[3], const double d[3]);
 void orc_dflux(double gamma, int dir, const double q[4], const double w[4], double M[16]);
 void orc_fvh2d_rhs(const orc_params* p, const double* q, double* rhs);

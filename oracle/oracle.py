@@ -42,6 +42,9 @@ _SIG = {
     "orc_rk3_combine": (None, [_I, _DP, _DP, _DP, _D, _DP, ctypes.c_long]),
     "orc_gauss9": (None, [_DP, _DP]),
     "orc_fv2d_rhs": (None, [ctypes.POINTER(OrcParams), _DP, _DP]),
+    "orc_fv2d_rhs_linear": (None, [ctypes.POINTER(OrcParams), _DP, _DP]),
+    "orc_weno5js3q_linear": (None, [_DP, _DP]),
+    "orc_hteno_flags": (None, [_D, _DP, _DP, _IP]),
     "orc_fv2d_max_speed": (None, [ctypes.POINTER(OrcParams), _DP, _DP, _DP]),
     "orc_face_labels": (None, [ctypes.POINTER(OrcParams), _DP, _I, ctypes.POINTER(ctypes.c_ubyte)]),
     "orc_fv2d_dt": (_D, [ctypes.POINTER(OrcParams), _DP, _D, _D, _D]),
@@ -124,6 +127,12 @@ class Oracle:
         self.lib.orc_weno5js3q(pu, out.ctypes.data_as(_DP))
         return out
 
+    def weno3q_linear(self, u):
+        u, pu = _p(u)
+        out = np.empty(3)
+        self.lib.orc_weno5js3q_linear(pu, out.ctypes.data_as(_DP))
+        return out
+
     def primitive(self, gamma, q):
         q, pq = _p(q)
         w = np.empty(4)
@@ -185,6 +194,14 @@ class Oracle:
         self.lib.orc_fv2d_rhs(ctypes.byref(prm), pq, r.ctypes.data_as(_DP))
         return r
 
+    def fv2d_rhs_linear(self, prm, q):
+        """The residual with linear reconstruction and linear transverse
+        quadrature on every face."""
+        q, pq = _p(q)
+        r = np.empty_like(q)
+        self.lib.orc_fv2d_rhs_linear(ctypes.byref(prm), pq, r.ctypes.data_as(_DP))
+        return r
+
     def face_labels(self, prm, q, direction):
         """TENO5 cut-off label (1 = nothing cut) of every x (direction 1) or
         y (direction 2) face, indexed [j, i]: the face of cell (i, j) with
@@ -232,6 +249,14 @@ class Oracle:
         out = np.empty(4)
         self.lib.orc_hteno(dif, pu, pd, out.ctypes.data_as(_DP))
         return out
+
+    def hteno_flags(self, dif, u, d):
+        """Cut-off flags (1 = kept): u^- [0:3], u^+ [3:6], d^- [6:9], d^+ [9:12]."""
+        u, pu = _p(u)
+        d, pd = _p(d)
+        f = np.zeros(12, dtype=np.int32)
+        self.lib.orc_hteno_flags(dif, pu, pd, f.ctypes.data_as(_IP))
+        return f
 
     def hermite_limiter(self, dif, u, d):
         u, pu = _p(u)

@@ -119,6 +119,17 @@ def test_kxrcf_flags_match_oracle(orc):
     f = s.troubledCells()
     np.testing.assert_array_equal(f, orc.fv1d_kxrcf(nx, 0.0, 1.0, q))
     assert 0 < f.sum() < nx
+    # jumps next to both boundaries: the first and last cells see the ghosts
+    for mom in (0.3, -0.3):
+        q = np.empty((3, nx))
+        q[0] = 1.0
+        q[0, 0], q[0, -1] = 3.0, 0.2
+        q[1] = mom
+        q[2] = 2.5 + 0.5 * mom * mom / q[0]
+        s.set_state(q)
+        f = s.troubledCells()
+        np.testing.assert_array_equal(f, orc.fv1d_kxrcf(nx, 0.0, 1.0, q))
+        assert f[0] == 1 and f[-1] == 1
     s.close()
     t = htx.System2D(0.0, 1.0, 16, 0.0, 1.0, 16, 0.5, 1.0, 0.0, GAMMA)
     with pytest.raises(HtxError):

@@ -232,6 +232,38 @@ def test_learned_all_smooth_matches_cpu_tiling():
     assert err < 1e-12
 
 
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+def test_learned_all_smooth_matches_oracle_linear_residual(orc, flux):
+    """Every face flagged smooth (threshold -1e30): on the smooth vortex no
+    positivity fallback acts, so the learned residual is the reference's
+    residual with Lagrange::linear (Schemes_Lagrange.cpp:1219-1233) on every
+    face and the transverse quadrature at its linear weights, which the
+    oracle computes independently (orc_fv2d_rhs_linear: linear on the six
+    characteristic values, the quadrature's candidates with the linear
+    weights); they agree to rounding."""
+    import htx
+    from conftest import rel_l1
+    from htx.ic import vortex_modes
+
+    n = 48
+    print(f"[case] learned all-smooth vs oracle linear residual, vortex n={n} {flux}", flush=True)
+    prm = params(n, n, 0.0, 10.0, 0.0, 10.0, GAMMA, "periodic", flux, True)
+    q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
+    s = htx.System2D(0.0, 10.0, n, 0.0, 10.0, n, 0.5, 2.0, 0.0, GAMMA, flux=flux, bc="periodic",
+                     precision="learned")
+    s.load_classifier(np.random.default_rng(3).normal(0, 0.3, 5057).astype(np.float32), -1e30)
+    s.set_state(q)
+    r = s.rhs()
+    f, g, lin = s.classifier_stats(reset=True)
+    err = rel_l1(r, orc.fv2d_rhs_linear(prm, q))
+    print(f"faces {f} flagged {g} linear {lin}; rel L1 {err:.3e}", flush=True)
+    s.close()
+    assert f == g == lin == 2 * n * n
+    assert err < 1e-12
+    # and it is not the TENO5 residual (the comparison can tell them apart)
+    assert rel_l1(r, orc.fv2d_rhs(prm, q)) > 1e-6
+
+
 @pytest.mark.parametrize("case", ["shock_blobs", "quadrant"])
 def test_face_flags_match_standalone_and_drive_the_stage(case):
     """The per-step face-flag kernel's flags equal the standalone inference's

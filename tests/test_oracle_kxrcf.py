@@ -3,7 +3,9 @@ pinned by what its reconstructions fix: the two 3-cell reconstructions of a
 face agree on cell averages of a quadratic (no flags away from the
 zero-gradient ghosts), and a single density
 jump flags exactly the cells whose 4-cell window straddles it (the window
-leans upwind of the momentum's sign), never the first or last cell."""
+leans upwind of the momentum's sign); the reference's loop (i = s+1 ..
+nxx-s-2 with s = 2, FV_System1D.cpp:15,166) visits every cell, the first
+and last ones against the zero-gradient ghosts."""
 import numpy as np
 import pytest
 
@@ -29,7 +31,7 @@ def test_quadratic_profile_is_never_flagged(orc):
         assert not orc.fv1d_kxrcf(N, 0.0, 1.0, _state(rho, np.full(N, mom)))[2:N - 2].any()
 
 
-@pytest.mark.parametrize("k", [2, 57, 200, 398])
+@pytest.mark.parametrize("k", [1, 2, 57, 200, 398, 399])
 @pytest.mark.parametrize("sign", [1.0, -1.0])
 def test_single_jump_flags_its_window(orc, k, sign):
     rho = np.where(np.arange(N) < k, 1.0, 0.125)
@@ -37,7 +39,7 @@ def test_single_jump_flags_its_window(orc, k, sign):
     # momentum >= 0: window i-1 .. i+2; < 0: window i-2 .. i+1
     lo, hi = (k - 2, k) if sign > 0 else (k - 1, k + 1)
     expect = np.zeros(N, dtype=np.uint8)
-    expect[max(lo, 1):min(hi, N - 2) + 1] = 1
+    expect[max(lo, 0):min(hi, N - 1) + 1] = 1
     assert np.array_equal(f, expect), np.nonzero(f)[0]
 
 
@@ -47,12 +49,19 @@ def test_sod_initial_state(orc):
     assert list(np.nonzero(f)[0]) == [198, 199, 200]
 
 
-def test_first_and_last_cells_are_not_visited(orc):
+def test_first_and_last_cells_see_the_ghosts(orc):
+    """Cells 0 and nx-1 are visited and read the zero-gradient ghosts
+    (rho_-1 = rho_0, rho_nx = rho_nx-1): a jump between cells 0 and 1 flags
+    cells 0 and 1 (momentum >= 0: window i-1 .. i+2), one between nx-2 and
+    nx-1 flags nx-3 .. nx-1."""
     rho = np.ones(N)
     rho[0], rho[-1] = 3.0, 0.2
     f = orc.fv1d_kxrcf(N, 0.0, 1.0, _state(rho, np.zeros(N)))
-    assert f[0] == 0 and f[-1] == 0
-    assert list(np.nonzero(f)[0]) == [1, N - 3, N - 2]
+    assert list(np.nonzero(f)[0]) == [0, 1, N - 3, N - 2, N - 1]
+    # momentum < 0: window i-2 .. i+1, so the jump at 0|1 flags cells 0 .. 2
+    # and the jump at nx-2|nx-1 flags nx-2 and nx-1
+    f = orc.fv1d_kxrcf(N, 0.0, 1.0, _state(rho, np.full(N, -0.5)))
+    assert list(np.nonzero(f)[0]) == [0, 1, 2, N - 2, N - 1]
 
 
 @pytest.mark.parametrize("jump, cells", [(5e-4, [198, 199, 200]), (2e-4, [199])])

@@ -277,6 +277,32 @@ def test_fp32_mixed_cutoff_never_overflows(harness_fast):
         assert np.all(np.abs(out - ref) < 1e-9 * u[0]), (out, ref)
 
 
+def test_teno5_decision_on_the_threshold_uses_pow(orc, harness):
+    """A stencil whose third left-candidate weight ratio lies on C_T = 5e-4 to
+    the last bit: the reference's (a)^6 by pow gives 0.0004999999999999999
+    (cut), a cube squared gives exactly 0.0005 (kept).  The strict build
+    forms the powers as cubes squared but recomputes them with pow where a
+    ratio lies within 1e-13 of C_T, so it reconstructs what the oracle (and
+    the reference) do."""
+    import math
+
+    u = np.array([float.fromhex(h) for h in (
+        "0x1.46cb19df16295p+2", "0x1.34e5a6f5e6534p-2", "-0x1.7d87893f7dc1cp-1", "-0x1.7f2e1e186e38ep+1",
+        "0x1.842efadc39048p+1", "0x1.2d452282a474ep+1")])
+    SI, b = orc.teno5_indicators(u)
+    s = list(SI[:3])
+    tau = abs(b[0] - (s[0] + s[2] + 4.0 * s[1]) / 6.0)
+    a = [1.0 + tau / (x + 1e-40) for x in s]
+    gp = [math.pow(x, 6.0) for x in a]
+    gc = [(x * x * x) * (x * x * x) for x in a]
+    assert gp[2] / ((gp[0] + gp[1]) + gp[2]) < 5e-4 <= gc[2] / ((gc[0] + gc[1]) + gc[2])
+    assert orc.teno5_flags(u)[2] == 0
+    harness.fv2dh_teno5.argtypes = [DP, ctypes.c_int, DP]
+    out = np.empty(2)
+    harness.fv2dh_teno5(u.ctypes.data_as(DP), 0, out.ctypes.data_as(DP))
+    assert out.tobytes() == orc.teno5(u).tobytes()
+
+
 def test_linear_quadrature_exact_for_quartics(harness_fast):
     """The learned path's transverse quadrature on smooth windows (weno3q
     with linear weights, folded into one 5-point filter per Gauss point):
@@ -333,6 +359,28 @@ def test_learned_all_smooth_residual(orc, harness_fast, flux):
     n = 40
     u = np.ones((4, n, n)) * np.array([1.2, 0.3, -0.4, 2.5])[:, None, None]
     assert np.abs(_rhs_prec(harness_fast, n, n, dom, "periodic", flux, 2, u)).max() < 1e-13
+
+
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+def test_learned_all_smooth_equals_oracle_linear_residual(orc, harness_fast, flux):
+    """With every face flagged smooth the tile phases compute the oracle's
+    all-linear residual (orc_fv2d_rhs_linear: Lagrange::linear on the
+    characteristic values, the transverse quadrature at its linear weights)
+    to rounding: the kernel reconstructs the conserved variables and projects
+    the two face states, and folds the linear quadrature into one filter per
+    Gauss point, which are the same linear maps.  (Periodic boundaries: with
+    zero-gradient ones the faces past the boundary keep TENO5, and the
+    wavefronts whose transverse windows reach them the nonlinear
+    quadrature.)"""
+    from htx.ic import vortex_modes
+
+    dom = (0.0, 10.0, 0.0, 10.0)
+    n, bc = 40, "periodic"
+    prm = params(n, n, *dom, 1.4, bc, flux, True)
+    q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
+    r = _rhs_prec(harness_fast, n, n, dom, bc, flux, 2, q)
+    rl = orc.fv2d_rhs_linear(prm, q)
+    assert np.abs(r - rl).sum() / np.abs(rl).sum() < 1e-12
 
 
 def _rhs_flags(lib, nx, ny, dom, bc, flux, q, fx, fy):
