@@ -20,6 +20,8 @@ hipError_t launch_ic_vortex_hermite(const VortexArgs& a, hipStream_t s);
 // strict IEEE build of the same kernels (htx_strict.hip)
 hipError_t htx_strict_launch_fv2d_stage(const void* a, int flux, int tile_x, int tile_y, hipStream_t s);
 hipError_t htx_strict_launch_fvh2d_stage(const void* a, int flux, hipStream_t s);
+hipError_t htx_strict_launch_max_speed(const double* q, int nx, int ny, int pitch, long long fstride, double gam,
+                                       int quirks, void* ctl, hipStream_t s);
 hipError_t htx_strict_launch_fv1d_stage(const void* a, hipStream_t s);
 hipError_t htx_strict_launch_fv1d_max_speed(const double* q, int nx, double gam, void* ctl, hipStream_t s);
 hipError_t htx_strict_launch_fv1d_kxrcf(const double* q, int nx, int bc, double radius, unsigned char* flags,
@@ -281,8 +283,13 @@ int refresh_cfl(htx_solver* h) {
         HIP_TRY(htx_strict_launch_fv1d_max_speed(h->dA, h->nx, h->p.gamma, h->ctl, h->stream));
         return HTX_OK;
     }
-    HIP_TRY(launch_max_speed(h->dA, h->nx, h->ny, h->pitch, h->fstride, h->p.gamma, h->p.ref_quirks ? 1 : 0,
-                             h->ctl, h->stream));
+    // (the 12-field Hermite state's first four fields are the cell averages)
+    if (h->p.precision == HTX_PREC_FP64)
+        HIP_TRY(htx_strict_launch_max_speed(h->dA, h->nx, h->ny, h->pitch, h->fstride, h->p.gamma,
+                                            h->p.ref_quirks ? 1 : 0, h->ctl, h->stream));
+    else
+        HIP_TRY(launch_max_speed(h->dA, h->nx, h->ny, h->pitch, h->fstride, h->p.gamma, h->p.ref_quirks ? 1 : 0,
+                                 h->ctl, h->stream));
     return HTX_OK;
 }
 

@@ -17,6 +17,14 @@ hipError_t htx_strict_launch_fv2d_stage(const void* a, int flux, int tile_x, int
     return htx_strict::launch_fv2d_stage(*static_cast<const htx_strict::Fv2dArgs*>(a), flux, tile_x, tile_y, s);
 }
 
+// the CFL maxima of a 2D state with the strict build's sound speed (the
+// drop-in's first dt after a state is set); ctl points to a Ctl
+hipError_t htx_strict_launch_max_speed(const double* q, int nx, int ny, int pitch, long long fstride, double gam,
+                                       int quirks, void* ctl, hipStream_t s) {
+    return htx_strict::launch_max_speed(q, nx, ny, pitch, fstride, gam, quirks, static_cast<htx_strict::Ctl*>(ctl),
+                                        s);
+}
+
 hipError_t htx_strict_launch_fvh2d_stage(const void* a, int flux, hipStream_t s) {
     return htx_strict::launch_fvh2d_stage(*static_cast<const htx_strict::FvhArgs*>(a), flux, s);
 }
