@@ -110,6 +110,7 @@ SIGNATURES = {
     "htx_batch_use_graph": (_I, [_P, _I]),
     "htx_batch_set_tile": (_I, [_P, _I, _I]),
     "htx_batch_tune": (_I, [_P, _I, _IP, _IP, _DP]),
+    "htx_step_noalloc": (_I, [ctypes.POINTER(HtxParams), _P, _D, _P, _P]),
     "htx_last_error": (ctypes.c_char_p, []),
     "htx_abi_version": (_I, []),
 }

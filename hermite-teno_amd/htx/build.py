@@ -25,7 +25,7 @@ LIBORACLE = os.path.join(ORACLE, "liboracle.so")
 LIBORACLE_OMP = os.path.join(ORACLE, "liboracle_omp.so")
 
 HIP_SOURCES = ["htx_fv2d.hip", "htx_fvh2d.hip", "htx_strict.hip", "htx_ic.hip", "htx_learn.hip", "htx_batch.hip",
-               "htx_api.hip"]
+               "htx_api.hip", "htx_noalloc.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

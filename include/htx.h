@@ -343,6 +343,30 @@ int htx_batch_set_tile(htx_batch* b, int tile_x, int tile_y);
  * three shapes in the order 16x16, 32x8, 8x32. */
 int htx_batch_tune(htx_batch* b, int steps, int* tile_x, int* tile_y, double* ms);
 
+/* ------------------------------------------------------------------------
+ * One step without allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).
+ *
+ * htx_step_noalloc performs what htx_step does for a 2D TENO5 solver (one
+ * SSP-RK3 step with the CFL dt of the state, clamped to p->tmax; the same
+ * kernels, the same results) on a state q in caller-owned device memory
+ * (4 fields, field-major, pitch nx: q[(f * ny + j) * nx + i]), in place, at
+ * time t; it writes the dt taken (0 when t >= p->tmax) to the device double
+ * dt_taken (may be null).  Precisions fp64, fp64_fast and fp32_mixed.  The
+ * work is enqueued on `stream` (a hipStream_t; null = the default stream)
+ * and the call returns without waiting.
+ *
+ * The two intermediate stage states and the control block live in a fixed
+ * device array of the library (one per GPU, 64 MiB, held until the process
+ * ends): nx * ny may not exceed HTX_NOALLOC_MAX_CELLS (HTX_EINVAL).  The call
+ * allocates and frees nothing.  Every call on a GPU uses that GPU's array:
+ * calls that overlap on one GPU (other host threads, or streams not ordered
+ * against each other) race and must be serialised by the caller.
+ * ------------------------------------------------------------------------ */
+#define HTX_NOALLOC_MAX_CELLS (1024LL * 1024LL)
+int htx_step_noalloc(const htx_params* p, double* q, double t, double* dt_taken, void* stream);
+
 /* Message of the last failed call on this host thread. */
 const char* htx_last_error(void);
 
