@@ -277,6 +277,24 @@ int read_ctl(htx_solver* h, Ctl* c) {
     return HTX_OK;
 }
 
+// HTX_ENONFINITE when the control block records a failure (a non-finite
+// conserved value met by a stage-3 epilogue or the CFL kernel, or a
+// non-finite / non-positive dt).  The reference has no such check: its CFL
+// loop takes fmax over the cells, which skips NaN, and it steps on.
+int status_error(const Ctl& c) {
+    if (!c.status) return HTX_OK;
+    char buf[200];
+    std::snprintf(buf, sizeof buf, "%s%s at t = %.17g after %d steps",
+                  (c.status & CTL_NONFINITE_STATE) ? "the state holds a NaN or infinite value" : "",
+                  (c.status & CTL_NONFINITE_STATE) ? "" : "the CFL time step is not finite and positive", c.t, c.nsteps);
+    return fail(HTX_ENONFINITE, buf);
+}
+
+int read_ctl_checked(htx_solver* h, Ctl* c) {
+    int rc = read_ctl(h, c);
+    return rc ? rc : status_error(*c);
+}
+
 int refresh_cfl(htx_solver* h) {
     HIP_TRY(launch_reset_smax(h->ctl, h->stream));
     if (h->p.dim == 1) {
@@ -417,9 +435,9 @@ int htx_set_state(htx_solver* h, const double* q, size_t ld) {
                              (size_t)h->ny * h->nfield, hipMemcpyHostToDevice, h->stream));
     int rc = refresh_cfl(h);
     if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(h->stream));
     h->have_state = true;
-    return HTX_OK;
+    Ctl c;
+    return read_ctl_checked(h, &c);
 }
 
 int htx_get_state(htx_solver* h, double* q, size_t ld) {
@@ -459,9 +477,9 @@ int htx_init_vortex(htx_solver* h, double cx, double cy, const double* modes, in
         HIP_TRY(launch_ic_vortex(a, h->stream));
     int rc = refresh_cfl(h);
     if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(h->stream));
     h->have_state = true;
-    return HTX_OK;
+    Ctl c;
+    return read_ctl_checked(h, &c);
 }
 
 namespace {
@@ -486,8 +504,8 @@ int finish_ic(htx_solver* h) {
     h->have_state = true;
     int rc = refresh_cfl(h);
     if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    return HTX_OK;
+    Ctl c;
+    return read_ctl_checked(h, &c);
 }
 }  // namespace
 
@@ -540,13 +558,16 @@ int htx_compute_dt(htx_solver* h, double* dt) {
     if (!h || !dt) return fail(HTX_EINVAL, "null argument");
     if (!h->have_state) return fail(HTX_ESTATE, "no state set");
     Ctl c;
-    int rc = read_ctl(h, &c);
+    int rc = read_ctl_checked(h, &c);
     if (rc) return rc;
     double sx = 0, sy = 0;
     std::memcpy(&sx, &c.smax_x, sizeof(double));
     std::memcpy(&sy, &c.smax_y, sizeof(double));
     double d = c.cfl * std::fmin(h->dx, h->dy) / std::fmax(sx, sy);
-    *dt = (c.t + d > c.tfinal) ? c.tfinal - c.t : d;
+    d = (c.t + d > c.tfinal) ? c.tfinal - c.t : d;
+    if (c.t < c.tfinal && !(d > 0.0 && std::isfinite(d)))
+        return fail(HTX_ENONFINITE, "the CFL time step is not finite and positive");
+    *dt = d;
     return HTX_OK;
 }
 
@@ -556,7 +577,7 @@ int htx_step(htx_solver* h, double* dt_taken) {
     int rc = enqueue_step(h);
     if (rc) return rc;
     Ctl c;
-    if ((rc = read_ctl(h, &c))) return rc;
+    if ((rc = read_ctl_checked(h, &c))) return rc;
     if (dt_taken) *dt_taken = c.active ? c.dt : 0.0;
     return HTX_OK;
 }
@@ -576,7 +597,7 @@ int htx_advance(htx_solver* h, double tfinal, int* nsteps) {
     if (!h) return fail(HTX_EINVAL, "null handle");
     if (!h->have_state) return fail(HTX_ESTATE, "no state set");
     Ctl c;
-    int rc = read_ctl(h, &c);
+    int rc = read_ctl_checked(h, &c);
     if (rc) return rc;
     int n0 = c.nsteps;
     c.tfinal = tfinal;
@@ -588,7 +609,7 @@ int htx_advance(htx_solver* h, double tfinal, int* nsteps) {
     while (c.t < tfinal) {
         for (int k = 0; k < batch; ++k)
             if ((rc = enqueue_step(h))) return rc;
-        if ((rc = read_ctl(h, &c))) return rc;
+        if ((rc = read_ctl_checked(h, &c))) return rc;
         if (batch < 64) batch *= 2;
     }
     if (nsteps) *nsteps = c.nsteps - n0;
@@ -700,6 +721,18 @@ int htx_load(htx_solver* h, const char* path) {
         return fail(HTX_EIO, "truncated snapshot header");
     if (hd.nfield != h->nfield || hd.nx != h->nx || hd.ny != h->ny || hd.scheme != h->p.scheme)
         return fail(HTX_EINVAL, "snapshot grid or scheme differs from the solver's");
+    // the physics of the run must be the solver's; precision, CFL number and
+    // end time belong to how it is continued and may differ
+    {
+        const char* field = nullptr;
+        if (hd.bc != h->p.bc) field = "bc";
+        else if (hd.flux != h->p.flux) field = "flux";
+        else if (hd.ref_quirks != h->p.ref_quirks) field = "ref_quirks";
+        else if (hd.gamma != h->p.gamma) field = "gamma";
+        else if (hd.xl != h->p.xl || hd.xr != h->p.xr) field = "domain (xl, xr)";
+        else if (h->p.dim == 2 && (hd.yb != h->p.yb || hd.yt != h->p.yt)) field = "domain (yb, yt)";
+        if (field) return fail(HTX_EINVAL, std::string("snapshot ") + field + " differs from the solver's");
+    }
     std::vector<double> q((size_t)h->nfield * h->nx * h->ny);
     unsigned long long hsh = 0;
     if (std::fread(q.data(), sizeof(double), q.size(), fc.f) != q.size() || std::fread(&hsh, sizeof(hsh), 1, fc.f) != 1)
@@ -718,7 +751,9 @@ int htx_load(htx_solver* h, const char* path) {
 int htx_sync(htx_solver* h) {
     if (!h) return fail(HTX_EINVAL, "null handle");
     HIP_TRY(hipStreamSynchronize(h->stream));
-    return HTX_OK;
+    if (!h->have_state) return HTX_OK;
+    Ctl c;
+    return read_ctl_checked(h, &c);
 }
 
 int htx_timing_enable(htx_solver* h, int enable) {

@@ -109,8 +109,10 @@ __global__ void __launch_bounds__(256) kb_fvh_update(BatchHermArgs B) {
     long long n = (long long)a.nx * a.ny;
     long long t = lb * 256ll + threadIdx.x;
     double smx = 0.0, smy = 0.0;
-    if (t < n) fvh_update_cell(a, (int)(t % a.nx), (int)(t / a.nx), dt, smx, smy);
+    bool bad = false;
+    if (t < n) fvh_update_cell(a, (int)(t % a.nx), (int)(t / a.nx), dt, smx, smy, &bad);
     if (a.mode == 3) {
+        flag_nonfinite(bad, a.ctl);
         smx = wave_max(smx);
         smy = wave_max(smy);
         if ((threadIdx.x & 63) == 0) {
@@ -391,6 +393,27 @@ int check_index(htx_batch* b, int i) {
     return HTX_OK;
 }
 
+// HTX_ENONFINITE naming the first instance whose control block records a
+// failure (non-finite state or dt; that instance stops stepping)
+int status_error(const Ctl* c, int n, int first = 0) {
+    for (int i = 0; i < n; ++i)
+        if (c[i].status)
+            return htx_internal_fail(HTX_ENONFINITE,
+                                     "instance " + std::to_string(first + i) +
+                                         ((c[i].status & CTL_NONFINITE_STATE)
+                                              ? ": the state holds a NaN or infinite value"
+                                              : ": the CFL time step is not finite and positive") +
+                                         " at t = " + std::to_string(c[i].t));
+    return HTX_OK;
+}
+
+int instance_status(htx_batch* b, int i) {
+    Ctl c{};
+    BHIP(hipMemcpyAsync(&c, b->ctl + i, sizeof(Ctl), hipMemcpyDeviceToHost, b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
+    return status_error(&c, 1, i);
+}
+
 }  // namespace
 
 extern "C" {
@@ -499,8 +522,7 @@ int htx_batch_set_state(htx_batch* b, int i, const double* q, size_t ld) {
     BHIP(hipMemcpy2DAsync(inst_state(b, 0, i), sizeof(double) * p.nx, q, sizeof(double) * ld, sizeof(double) * p.nx,
                           (size_t)p.ny * nf, hipMemcpyHostToDevice, b->stream));
     if ((rc = refresh_instance_cfl(b, i))) return rc;
-    BHIP(hipStreamSynchronize(b->stream));
-    return HTX_OK;
+    return instance_status(b, i);
 }
 
 int htx_batch_get_state(htx_batch* b, int i, double* q, size_t ld) {
@@ -539,8 +561,7 @@ int htx_batch_init_quadrant(htx_batch* b, int i, double xs, double ys, const dou
         for (int e = 0; e < 4; ++e) a.st[k][e] = states[4 * k + e];
     BHIP(launch_ic_piecewise(a, b->stream));
     if ((rc = refresh_instance_cfl(b, i))) return rc;
-    BHIP(hipStreamSynchronize(b->stream));
-    return HTX_OK;
+    return instance_status(b, i);
 }
 
 int htx_batch_load_classifier(htx_batch* b, const float* params, double threshold) {
@@ -587,6 +608,10 @@ int htx_batch_advance(htx_batch* b, int max_steps, int* nsteps) {
         done += k;
         BHIP(hipMemcpyAsync(c.data(), b->ctl, sizeof(Ctl) * b->n, hipMemcpyDeviceToHost, b->stream));
         BHIP(hipStreamSynchronize(b->stream));
+        if ((rc = status_error(c.data(), b->n))) {
+            if (nsteps) *nsteps = done;
+            return rc;
+        }
         bool all = true;
         for (int i = 0; i < b->n; ++i) all = all && !(c[i].t < c[i].tfinal);
         if (all) break;
@@ -609,8 +634,10 @@ int htx_batch_get_time(htx_batch* b, int i, double* t, int* nsteps) {
 
 int htx_batch_sync(htx_batch* b) {
     if (!b) return htx_internal_fail(HTX_EINVAL, "null batch");
+    std::vector<Ctl> c(b->n);
+    BHIP(hipMemcpyAsync(c.data(), b->ctl, sizeof(Ctl) * b->n, hipMemcpyDeviceToHost, b->stream));
     BHIP(hipStreamSynchronize(b->stream));
-    return HTX_OK;
+    return status_error(c.data(), b->n);
 }
 
 int htx_batch_use_graph(htx_batch* b, int enable) {
@@ -636,49 +663,73 @@ int htx_batch_set_tile(htx_batch* b, int tile_x, int tile_y) {
 int htx_batch_tune(htx_batch* b, int steps, int* tile_x, int* tile_y, double* ms) {
     if (!b) return htx_internal_fail(HTX_EINVAL, "null batch");
     if (steps < 1) return htx_internal_fail(HTX_EINVAL, "steps >= 1");
-    // time every built shape on the batch itself, from the same state
-    double* saved = nullptr;
-    Ctl* saved_ctl = nullptr;
-    if (hipMalloc(&saved, sizeof(double) * (size_t)b->total) != hipSuccess ||
-        hipMalloc(&saved_ctl, sizeof(Ctl) * (size_t)b->n) != hipSuccess) {
-        if (saved) (void)hipFree(saved);
+    // Everything the timing changes is put back on every way out: the state
+    // and control blocks (saved on the device first), the graph setting, the
+    // tile shape with its launch tables (the original one unless a fastest
+    // shape was found), and the buffers and events are released.
+    struct Guard {
+        htx_batch* b;
+        double* saved = nullptr;
+        Ctl* saved_ctl = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        bool graph;
+        int shape;
+        bool restore = false;  // saved holds the batch's state
+        explicit Guard(htx_batch* b_) : b(b_), graph(b_->use_graph), shape(b_->shape) {}
+        ~Guard() {
+            if (restore) {
+                (void)hipMemcpyAsync(b->state, saved, sizeof(double) * (size_t)b->total, hipMemcpyDeviceToDevice,
+                                     b->stream);
+                (void)hipMemcpyAsync(b->ctl, saved_ctl, sizeof(Ctl) * (size_t)b->n, hipMemcpyDeviceToDevice,
+                                     b->stream);
+            }
+            (void)hipStreamSynchronize(b->stream);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (saved) (void)hipFree(saved);
+            if (saved_ctl) (void)hipFree(saved_ctl);
+            b->use_graph = graph;
+            if (b->shape != shape) {
+                b->shape = shape;
+                (void)build_teno_tables(b);
+            }
+            (void)drop_graph(b);
+        }
+    } g(b);
+    if (hipMalloc(&g.saved, sizeof(double) * (size_t)b->total) != hipSuccess ||
+        hipMalloc(&g.saved_ctl, sizeof(Ctl) * (size_t)b->n) != hipSuccess)
         return htx_internal_fail(HTX_ENOMEM, "device allocation failed");
-    }
-    hipEvent_t e0, e1;
-    BHIP(hipEventCreate(&e0));
-    BHIP(hipEventCreate(&e1));
-    BHIP(hipMemcpyAsync(saved, b->state, sizeof(double) * (size_t)b->total, hipMemcpyDeviceToDevice, b->stream));
-    BHIP(hipMemcpyAsync(saved_ctl, b->ctl, sizeof(Ctl) * (size_t)b->n, hipMemcpyDeviceToDevice, b->stream));
-    const bool graph = b->use_graph;
+    BHIP(hipEventCreate(&g.e0));
+    BHIP(hipEventCreate(&g.e1));
+    BHIP(hipMemcpyAsync(g.saved, b->state, sizeof(double) * (size_t)b->total, hipMemcpyDeviceToDevice, b->stream));
+    BHIP(hipMemcpyAsync(g.saved_ctl, b->ctl, sizeof(Ctl) * (size_t)b->n, hipMemcpyDeviceToDevice, b->stream));
+    g.restore = true;
     b->use_graph = false;
-    int best = b->shape, rc = HTX_OK;
+    int best = b->shape, rc;
     double best_ms = 1e300;
-    for (int k = 0; k < NSHAPES && !rc; ++k) {
+    for (int k = 0; k < NSHAPES; ++k) {
         b->shape = k;
-        if ((rc = build_teno_tables(b)) || (rc = enqueue_batch_step(b, b->stream))) break;  // warm-up
-        BHIP(hipEventRecord(e0, b->stream));
-        for (int s = 0; s < steps && !rc; ++s) rc = enqueue_batch_step(b, b->stream);
-        BHIP(hipEventRecord(e1, b->stream));
-        BHIP(hipEventSynchronize(e1));
+        if ((rc = build_teno_tables(b)) || (rc = enqueue_batch_step(b, b->stream))) return rc;  // warm-up
+        BHIP(hipEventRecord(g.e0, b->stream));
+        for (int s = 0; s < steps; ++s)
+            if ((rc = enqueue_batch_step(b, b->stream))) return rc;
+        BHIP(hipEventRecord(g.e1, b->stream));
+        BHIP(hipEventSynchronize(g.e1));
         float t = 0.f;
-        BHIP(hipEventElapsedTime(&t, e0, e1));
+        BHIP(hipEventElapsedTime(&t, g.e0, g.e1));
         if (ms) ms[k] = t / steps;
         if (t < best_ms) {
             best_ms = t;
             best = k;
         }
-        BHIP(hipMemcpyAsync(b->state, saved, sizeof(double) * (size_t)b->total, hipMemcpyDeviceToDevice, b->stream));
-        BHIP(hipMemcpyAsync(b->ctl, saved_ctl, sizeof(Ctl) * (size_t)b->n, hipMemcpyDeviceToDevice, b->stream));
+        BHIP(hipMemcpyAsync(b->state, g.saved, sizeof(double) * (size_t)b->total, hipMemcpyDeviceToDevice, b->stream));
+        BHIP(hipMemcpyAsync(b->ctl, g.saved_ctl, sizeof(Ctl) * (size_t)b->n, hipMemcpyDeviceToDevice, b->stream));
     }
-    BHIP(hipStreamSynchronize(b->stream));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(saved);
-    (void)hipFree(saved_ctl);
-    b->use_graph = graph;
-    if (rc) return rc;
+    // keep the fastest shape: the guard rebuilds its tables on the way out;
+    // build them here once more so that a failure is reported
+    g.shape = best;
     b->shape = best;
-    if ((rc = build_teno_tables(b)) || (rc = drop_graph(b))) return rc;
+    if ((rc = build_teno_tables(b))) return rc;
     if (tile_x) *tile_x = SHAPES[best].tx;
     if (tile_y) *tile_y = SHAPES[best].ty;
     return HTX_OK;

@@ -46,25 +46,35 @@ __global__ void __launch_bounds__(FV1_T) k_fv1d_stage(Fv1dArgs a) {
     __syncthreads();
     const int c = c0 + threadIdx.x;
     double sm = 0.0;
+    bool bad = false;
     if (c < a.nx) {
         double u[3];
         for (int e = 0; e < 3; ++e) {
             u[e] = fv1d_stage_value(a, c, e, fl[e][threadIdx.x], fl[e][threadIdx.x + 1], dt);
             a.qout[e * a.nx + c] = u[e];
         }
-        if (a.mode == 3) sm = fv1d_max_speed(a.gam, a.g1, u);
+        if (a.mode == 3) {
+            sm = fv1d_max_speed(a.gam, a.g1, u);
+            bad = !all_finite(u, 3);
+        }
     }
-    if (a.mode == 3) block_max_to_ctl(sm, a.ctl);
+    if (a.mode == 3) {
+        flag_nonfinite(bad, a.ctl);
+        block_max_to_ctl(sm, a.ctl);
+    }
 }
 
 // wave-speed maximum of a state (after a state is set)
 __global__ void __launch_bounds__(FV1_T) k_fv1d_max_speed(const double* q, int nx, double gam, Ctl* ctl) {
     const int c = blockIdx.x * FV1_T + threadIdx.x;
     double sm = 0.0;
+    bool bad = false;
     if (c < nx) {
         const double u[3] = {q[c], q[nx + c], q[2 * nx + c]};
         sm = fv1d_max_speed(gam, gam - 1.0, u);
+        bad = !all_finite(u, 3);
     }
+    flag_nonfinite(bad, ctl);
     block_max_to_ctl(sm, ctl);
 }
 

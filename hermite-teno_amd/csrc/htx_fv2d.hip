@@ -37,9 +37,11 @@ __global__ void k_dt(Ctl* c, double dx, double dy) { ctl_dt(c, dx, dy); }
 
 __global__ void k_tadvance(Ctl* c) { ctl_tadvance(c); }
 
+// a new state: fresh CFL maxima and no failure record
 __global__ void k_reset_smax(Ctl* c) {
     c->smax_x = 0ull;
     c->smax_y = 0ull;
+    c->status = 0u;
 }
 
 __global__ void __launch_bounds__(256) k_max_speed(const double* q, int nx, int ny, int pitch,
@@ -47,11 +49,14 @@ __global__ void __launch_bounds__(256) k_max_speed(const double* q, int nx, int 
     __shared__ double red[2][4];
     double g1 = gam - 1.0;
     double mx = 0.0, my = 0.0;
+    bool bad = false;
     long long n = (long long)nx * ny;
     for (long long id = blockIdx.x * 256ll + threadIdx.x; id < n; id += (long long)gridDim.x * 256) {
         int i = (int)(id % nx), j = (int)(id / nx);
         long long off = (long long)j * pitch + i;
         double q0 = q[off], q1 = q[off + fs], q2 = q[off + 2 * fs], q3 = q[off + 3 * fs];
+        const double qq[4] = {q0, q1, q2, q3};
+        bad = bad || !all_finite(qq, 4);
         double wu = q1 / q0, wv = q2 / q0;
         double p = (q3 - 0.5 * (q1 * wu + q2 * wv)) * g1;
         if (quirks) {
@@ -62,6 +67,7 @@ __global__ void __launch_bounds__(256) k_max_speed(const double* q, int nx, int 
             my = fmax(my, max_speed<false>(gam, wv, p, q0));
         }
     }
+    flag_nonfinite(bad, c);
     mx = wave_max(mx);
     my = wave_max(my);
     if ((threadIdx.x & 63) == 0) {

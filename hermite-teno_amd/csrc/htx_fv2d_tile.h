@@ -349,7 +349,7 @@ HTX_HD void ph_xdiv(const Fv2dArgs& a, const Fv2dTile<TX, TY>& s, int tid, doubl
 // (:623-631 with Equations_Euler2D.cpp:207-219)
 template <int TX, int TY>
 HTX_HD void ph_epilogue(const Fv2dArgs& a, const Fv2dTile<TX, TY>& s, int i0, int j0, int tid,
-                        const double dxp[][4], double dt, double& smx, double& smy) {
+                        const double dxp[][4], double dt, double& smx, double& smy, bool* bad = nullptr) {
     using T = Fv2dTile<TX, TY>;
     const long long fs = a.fstride;
 #pragma unroll
@@ -383,6 +383,7 @@ HTX_HD void ph_epilogue(const Fv2dArgs& a, const Fv2dTile<TX, TY>& s, int i0, in
             a.qout[off + e * fs] = v;
         }
         if (a.mode == 3) {
+            if (bad && !all_finite(unew, 4)) *bad = true;
 #if HTX_EXACT_DIVISION
             double wu = unew[1] / unew[0], wv = unew[2] / unew[0];
 #else
@@ -449,8 +450,10 @@ __device__ __forceinline__ void stage_tile(const Fv2dArgs& a, Fv2dTile<TX, TY>& 
     ph_flux<TX, TY, true, FLUX, MIXED, LEARNED>(a, s, tid);
     __syncthreads();
     double smx = 0.0, smy = 0.0;
-    ph_epilogue<TX, TY>(a, s, i0, j0, tid, dxp, dt, smx, smy);
+    bool bad = false;
+    ph_epilogue<TX, TY>(a, s, i0, j0, tid, dxp, dt, smx, smy, &bad);
     if (a.mode == 3) {
+        flag_nonfinite(bad, a.ctl);
         smx = wave_max(smx);
         smy = wave_max(smy);
         if ((tid & 63) == 0) {

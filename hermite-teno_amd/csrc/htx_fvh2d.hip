@@ -62,8 +62,10 @@ __global__ void __launch_bounds__(256) k_fvh_update(FvhArgs a) {
     long long n = (long long)a.nx * a.ny;
     long long t = blockIdx.x * 256ll + threadIdx.x;
     double smx = 0.0, smy = 0.0;
-    if (t < n) fvh_update_cell(a, (int)(t % a.nx), (int)(t / a.nx), dt, smx, smy);
+    bool bad = false;
+    if (t < n) fvh_update_cell(a, (int)(t % a.nx), (int)(t / a.nx), dt, smx, smy, &bad);
     if (a.mode == 3) {
+        flag_nonfinite(bad, a.ctl);
         smx = wave_max_fvh(smx);
         smy = wave_max_fvh(smy);
         if ((threadIdx.x & 63) == 0) {

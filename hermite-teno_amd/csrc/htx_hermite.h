@@ -809,12 +809,14 @@ __device__ void fvh_flux_block(const FvhArgs& a, long long t0, long long nf, dou
 // stage (:733-782) and, in stage 3, the CFL maxima of u.  In stage 1 the
 // base state u^n of v and w is the limited one (save() runs after rhs()
 // limited them in place), so it is also written back into q0.
-HTX_HD void fvh_update_cell(const FvhArgs& a, int i, int j, double dt, double& smx, double& smy) {
+HTX_HD void fvh_update_cell(const FvhArgs& a, int i, int j, double dt, double& smx, double& smy,
+                            bool* bad = nullptr) {
     const long long fs = a.fstride;
     long long off = (long long)j * a.pitch + i;
     long long gxs = (long long)a.ny * (a.nx + 1), gys = (long long)(a.ny + 1) * a.nx;
     long long bx = (long long)j * (a.nx + 1) + i, by = (long long)j * a.nx + i;
     double unew[4];
+    bool fin = true;
     for (int c = 0; c < 12; ++c) {
         double rhs = ((a.gx[c * gxs + bx] - a.gx[c * gxs + bx + 1]) * a.dy +
                       (a.gy[c * gys + by] - a.gy[c * gys + by + a.nx]) * a.dx) /
@@ -833,9 +835,11 @@ HTX_HD void fvh_update_cell(const FvhArgs& a, int i, int j, double dt, double& s
             v = (a.q0[off + c * fs] + 2.0 * (us + dt * rhs)) / 3.0;
         }
         if (c < 4) unew[c] = v;
+        fin = fin && fabs(v) <= 1.7976931348623157e308;
         a.qout[off + c * fs] = v;
     }
     if (a.mode == 3) {
+        if (bad && !fin) *bad = true;
         double wu = unew[1] / unew[0], wv = unew[2] / unew[0];
         double p = (unew[3] - 0.5 * (unew[1] * wu + unew[2] * wv)) * a.g1;
         if (a.quirks) {

@@ -15,6 +15,11 @@ struct Ctl {
     unsigned long long smax_x, smax_y;  // bit patterns of non-negative doubles
     int active;
     int nsteps;
+    // failure record (bits): CTL_NONFINITE_STATE when a stage-3 epilogue or
+    // the CFL kernel met a NaN or infinite conserved value, CTL_BAD_DT when
+    // the CFL dt came out non-finite or non-positive (that step and every
+    // later one are skipped); cleared only when a new state is set
+    unsigned int status;
     // learned precision, each face once per stage since the last reset: faces
     // reconstructed, faces the classifier flagged smooth, flagged faces that
     // kept the linear scheme (the rest failed the positivity check)
@@ -22,6 +27,21 @@ struct Ctl {
 };
 
 enum FluxKind { FLUX_RUSANOV_CHAR = 0, FLUX_HLLC = 1, FLUX_HLLC_TORO = 2 };
+
+enum CtlStatus : unsigned { CTL_NONFINITE_STATE = 1u, CTL_BAD_DT = 2u };
+
+// every component finite (NaN and infinities fail the comparison)
+HTX_HD bool all_finite(const double* u, int n) {
+    bool ok = true;
+    for (int e = 0; e < n; ++e) ok = ok && fabs(u[e]) <= 1.7976931348623157e308;
+    return ok;
+}
+
+// One atomic per wavefront that met a non-finite value; called by every lane
+// of the wavefront.
+__device__ __forceinline__ void flag_nonfinite(bool bad, Ctl* c) {
+    if (__any(bad ? 1 : 0) && (threadIdx.x & 63) == 0) atomicOr(&c->status, (unsigned)CTL_NONFINITE_STATE);
+}
 
 // dt of the next step from the accumulated max wave speeds
 // (FV_System2D.cpp:633,645); a step with t >= tFinal is marked inactive and
@@ -36,6 +56,13 @@ __device__ inline void ctl_dt(Ctl* c, double dx, double dy) {
         double sy = __longlong_as_double((long long)c->smax_y);
         double dt = c->cfl * fmin(dx, dy) / fmax(sx, sy);
         dt = t + dt > tf ? tf - t : dt;
+        if (!(dt > 0.0 && dt <= 1.7976931348623157e308) || (c->status & CTL_NONFINITE_STATE)) {
+            // a NaN or infinite wave speed or state: stop stepping
+            c->status |= CTL_BAD_DT;
+            c->dt = 0.0;
+            c->active = 0;
+            return;
+        }
         c->dt = dt;
         c->active = 1;
         c->smax_x = 0ull;

@@ -12,7 +12,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhtx.so")
 
 HTX_OK = 0
-ERRORS = {-1: "HTX_EINVAL", -2: "HTX_EDEVICE", -3: "HTX_ENOMEM", -4: "HTX_ESTATE", -5: "HTX_ENOTSUP", -6: "HTX_EIO"}
+ERRORS = {-1: "HTX_EINVAL", -2: "HTX_EDEVICE", -3: "HTX_ENOMEM", -4: "HTX_ESTATE", -5: "HTX_ENOTSUP", -6: "HTX_EIO",
+          -7: "HTX_ENONFINITE"}
 
 SCHEMES = {"teno5": 0, "hteno": 1, "teno5i": 2, "weno5js": 3, "weno5z": 4}
 FLUXES = {"rusanov": 0, "rusanov_char": 0, "hllc": 1}

@@ -38,7 +38,10 @@ enum htx_error {
     HTX_ENOMEM = -3,   /* device allocation failed */
     HTX_ESTATE = -4,   /* call out of order (no state set) */
     HTX_ENOTSUP = -5,  /* combination not built yet */
-    HTX_EIO = -6       /* snapshot file missing, unreadable, truncated or corrupt */
+    HTX_EIO = -6,      /* snapshot file missing, unreadable, truncated or corrupt */
+    HTX_ENONFINITE = -7 /* the state holds a NaN or infinite value, or the CFL dt came out
+                           non-finite or non-positive; stepping stops (steps become no-ops)
+                           until a new state is set */
 };
 
 /* HTX_SCHEME_TENO5I: Lagrange::teno5i (Schemes_Lagrange.cpp:1235-1263), the
@@ -175,10 +178,11 @@ int htx_sync(htx_solver* h);
  *   double q[nfield][ny][nx] (field-major interior cells, as htx_get_state);
  *   uint64 checksum: sum over the words w_i of q (as uint64) of
  *   w_i (2 i + 1), modulo 2^64.
- * htx_load needs a solver of the same grid and scheme; it restores the
- * state, t and the step count (the solver keeps its own flux, boundary
- * condition, precision, CFL number and end time), so that continuing from
- * a snapshot reproduces the uninterrupted run.  HTX_EIO for a missing,
+ * htx_load needs a solver of the same grid, scheme, boundary condition,
+ * flux, quirk setting, gamma and domain (HTX_EINVAL naming the field that
+ * differs); it restores the state, t and the step count (the solver keeps
+ * its own precision, CFL number and end time), so that continuing from a
+ * snapshot reproduces the uninterrupted run.  HTX_EIO for a missing,
  * truncated or corrupt file. */
 int htx_save(htx_solver* h, const char* path);
 int htx_load(htx_solver* h, const char* path);

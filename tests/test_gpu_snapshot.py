@@ -48,10 +48,21 @@ def test_load_host_written_and_refusals(tmp_path):
     s = _solver("teno5", n=32)
     q = s.get_state() * 1.01
     p = tmp_path / "h.snap"
-    snapshot.write(p, q, t=0.25, nsteps=5, scheme=0)
+    cfg = dict(scheme=0, flux=1, bc=1, precision=0, ref_quirks=1, xl=0.0, xr=10.0, yb=0.0, yt=10.0, cfl=0.5,
+               tmax=10.0, tinit=0.0, gamma=1.4)
+    snapshot.write(p, q, t=0.25, nsteps=5, **cfg)
     s.load(p)
     np.testing.assert_array_equal(s.get_state(), q)
     assert s.time == 0.25 and s.nsteps == 5
+    # another precision, CFL number and end time may continue the run; other
+    # physics may not (the message names the field)
+    snapshot.write(tmp_path / "p.snap", q, t=0.25, nsteps=5, **dict(cfg, precision=1, cfl=0.3, tmax=1.0))
+    s.load(tmp_path / "p.snap")
+    for field, value in (("bc", 0), ("gamma", 5.0 / 3.0), ("flux", 0), ("xr", 12.0), ("ref_quirks", 0)):
+        snapshot.write(tmp_path / "x.snap", q, t=0.25, nsteps=5, **dict(cfg, **{field: value}))
+        with pytest.raises(htx.HtxError) as ei:
+            s.load(tmp_path / "x.snap")
+        assert ei.value.code == -1 and field.rstrip("r") in str(ei.value)
     other = _solver("teno5", n=24)
     with pytest.raises(htx.HtxError) as ei:
         other.load(p)
