@@ -83,6 +83,7 @@ struct htx_solver {
     QuadConsts qc{};
     bool have_state = false;
     bool timing = false;
+    int next_stage = 1;  // htx_stage: the stage that may run next
     // learned classifier (precision HTX_PREC_LEARNED): packed bf16 fragments,
     // FP32 biases and output weights, logit threshold
     unsigned short* dFrag = nullptr;
@@ -262,6 +263,7 @@ int launch_stage(htx_solver* h, const double* qin, const double* q0, double* qou
 }
 
 int enqueue_step(htx_solver* h) {
+    h->next_stage = 1;
     HIP_TRY(launch_dt(h->ctl, h->dx, h->dy, h->stream));
     int rc;
     if ((rc = launch_stage(h, h->dA, h->dA, h->dB, 1))) return rc;
@@ -436,6 +438,7 @@ int htx_set_state(htx_solver* h, const double* q, size_t ld) {
     int rc = refresh_cfl(h);
     if (rc) return rc;
     h->have_state = true;
+    h->next_stage = 1;
     Ctl c;
     return read_ctl_checked(h, &c);
 }
@@ -550,6 +553,38 @@ int htx_rhs(htx_solver* h, double* rhs, size_t ld) {
     if (rc) return rc;
     HIP_TRY(hipMemcpy2DAsync(rhs, sizeof(double) * ld, h->dR, sizeof(double) * h->pitch, sizeof(double) * h->nx,
                              (size_t)h->ny * h->nfield, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return HTX_OK;
+}
+
+int htx_stage(htx_solver* h, int stage, double dt) {
+    if (!h) return fail(HTX_EINVAL, "null handle");
+    if (!h->have_state) return fail(HTX_ESTATE, "no state set");
+    if (stage < 1 || stage > 3) return fail(HTX_EINVAL, "stage must be 1, 2 or 3");
+    if (stage != h->next_stage)
+        return fail(HTX_ESTATE, "stage " + std::to_string(stage) + " called where stage " +
+                                    std::to_string(h->next_stage) + " is due");
+    if (!std::isfinite(dt)) return fail(HTX_EINVAL, "dt must be finite");
+    HIP_TRY(launch_set_stage_dt(h->ctl, dt, stage == 1 ? 1 : 0, h->stream));
+    int rc;
+    if (stage == 1)
+        rc = launch_stage(h, h->dA, h->dA, h->dB, 1);
+    else if (stage == 2)
+        rc = launch_stage(h, h->dB, h->dA, h->dC, 2);
+    else
+        rc = launch_stage(h, h->dC, h->dA, h->dA, 3);
+    if (rc) return rc;
+    h->next_stage = stage == 3 ? 1 : stage + 1;
+    return HTX_OK;
+}
+
+int htx_get_stage_state(htx_solver* h, int stage, double* q, size_t ld) {
+    if (!h || !q) return fail(HTX_EINVAL, "null argument");
+    if (stage != 1 && stage != 2) return fail(HTX_EINVAL, "stage must be 1 or 2");
+    if (ld < (size_t)h->nx) return fail(HTX_EINVAL, "ld < nx");
+    if (!h->have_state) return fail(HTX_ESTATE, "no state set");
+    HIP_TRY(hipMemcpy2DAsync(q, sizeof(double) * ld, stage == 1 ? h->dB : h->dC, sizeof(double) * h->pitch,
+                             sizeof(double) * h->nx, (size_t)h->ny * h->nfield, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     return HTX_OK;
 }

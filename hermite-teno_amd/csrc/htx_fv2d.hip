@@ -37,6 +37,17 @@ __global__ void k_dt(Ctl* c, double dx, double dy) { ctl_dt(c, dx, dy); }
 
 __global__ void k_tadvance(Ctl* c) { ctl_tadvance(c); }
 
+// an explicit stage (htx_stage): the stage kernels take dt from the control
+// block and run; stage 1 starts the CFL maxima afresh, as ctl_dt does
+__global__ void k_set_stage_dt(Ctl* c, double dt, int fresh) {
+    c->dt = dt;
+    c->active = 1;
+    if (fresh) {
+        c->smax_x = 0ull;
+        c->smax_y = 0ull;
+    }
+}
+
 // a new state: fresh CFL maxima and no failure record
 __global__ void k_reset_smax(Ctl* c) {
     c->smax_x = 0ull;
@@ -171,6 +182,11 @@ hipError_t launch_dt(Ctl* c, double dx, double dy, hipStream_t s) {
 
 hipError_t launch_tadvance(Ctl* c, hipStream_t s) {
     hipLaunchKernelGGL(k_tadvance, dim3(1), dim3(1), 0, s, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_stage_dt(Ctl* c, double dt, int fresh, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_stage_dt, dim3(1), dim3(1), 0, s, c, dt, fresh);
     return hipGetLastError();
 }
 

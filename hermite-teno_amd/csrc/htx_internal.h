@@ -163,6 +163,7 @@ hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y
 hipError_t launch_dt(Ctl* c, double dx, double dy, hipStream_t s);
 hipError_t launch_tadvance(Ctl* c, hipStream_t s);
 hipError_t launch_reset_smax(Ctl* c, hipStream_t s);
+hipError_t launch_set_stage_dt(Ctl* c, double dt, int fresh, hipStream_t s);
 hipError_t launch_max_speed(const double* q, int nx, int ny, int pitch, long long fstride, double gam,
                             int quirks, Ctl* c, hipStream_t s);
 hipError_t launch_ic_vortex(const VortexArgs& a, hipStream_t s);

@@ -71,6 +71,8 @@ SIGNATURES = {
     "htx_init_shock_blobs": (_I, [_P, _D, _D, _D, _D, _DP, _DP, _DP, _I]),
     "htx_rhs": (_I, [_P, _DP, _S]),
     "htx_compute_dt": (_I, [_P, _DP]),
+    "htx_stage": (_I, [_P, _I, _D]),
+    "htx_get_stage_state": (_I, [_P, _I, _DP, _S]),
     "htx_step": (_I, [_P, _DP]),
     "htx_steps_async": (_I, [_P, _I]),
     "htx_advance": (_I, [_P, _D, _IP]),

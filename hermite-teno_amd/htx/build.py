@@ -25,7 +25,7 @@ LIBORACLE = os.path.join(ORACLE, "liboracle.so")
 LIBORACLE_OMP = os.path.join(ORACLE, "liboracle_omp.so")
 
 HIP_SOURCES = ["htx_fv2d.hip", "htx_fvh2d.hip", "htx_strict.hip", "htx_ic.hip", "htx_learn.hip", "htx_batch.hip",
-               "htx_api.hip", "htx_noalloc.hip"]
+               "htx_api.hip", "htx_noalloc.hip", "htx_classes.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -58,6 +58,7 @@ def _run(cmd, cwd=None):
 # (the classifier's matrix-core products write ordinary VGPRs, which spares
 # the accumulator-register reads before the GELU)
 PER_FILE_FLAGS = {"htx_strict.hip": ["-ffp-contract=off"], "htx_ic.hip": ["-ffp-contract=off"],
+                  "htx_classes.hip": ["-ffp-contract=off"],
                   "htx_learn.hip": ["-ffp-contract=off", "-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
@@ -128,9 +129,15 @@ EXAMPLES = os.path.join(os.path.dirname(PKG_DIR), "examples")
 EXAMPLE_BIN = os.path.join(EXAMPLES, "build", "main_fv2d")
 
 
+CLASSES_CHECK = os.path.join(ROOT, "tests", "host", "classes_check")
+
+
 def build_examples():
-    """The main.cpp-style C++ driver over the facade headers (C++11, g++)."""
+    """The main.cpp-style C++ driver over the facade headers (C++11, g++),
+    and the test driver of the physics and scheme classes."""
     os.makedirs(os.path.dirname(EXAMPLE_BIN), exist_ok=True)
+    _run(["g++", "-std=c++11", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", CLASSES_CHECK,
+          os.path.join(ROOT, "tests", "host", "classes_check.cpp"), "-L", PKG_DIR, "-lhtx", "-Wl,-rpath," + PKG_DIR])
     _run(["g++", "-std=c++11", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", EXAMPLE_BIN,
           os.path.join(EXAMPLES, "main_fv2d.cpp"), "-L", PKG_DIR, "-lhtx", "-Wl,-rpath," + PKG_DIR,
           "-Wl,-rpath,$ORIGIN/../../htx"])

@@ -165,6 +165,16 @@ class System2D:
         check(_lib.load().htx_rhs(self._h, dptr(r), self.nx))
         return r
 
+    def stage(self, k, dt):
+        """One SSP-RK3 stage (1, 2 or 3) with the given dt (htx_stage)."""
+        check(_lib.load().htx_stage(self._h, int(k), float(dt)))
+
+    def stage_state(self, k):
+        """u1 (k = 1) or u2 (k = 2) of the last stages run."""
+        q = np.empty((self.nfield, self.ny, self.nx))
+        check(_lib.load().htx_get_stage_state(self._h, int(k), dptr(q), self.nx))
+        return q
+
     def compute_dt(self):
         d = ctypes.c_double()
         check(_lib.load().htx_compute_dt(self._h, ctypes.byref(d)))

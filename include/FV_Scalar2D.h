@@ -1,0 +1,23 @@
+// FV_Scalar2D.h -- declaration of the reference's FV::Scalar2D (FV_Scalar2D.h:8-60), the
+// scalar (linear advection / Burgers) test solver.  Outside the rebuilt path:
+// declared so that a driver written against the reference's headers (its
+// main.cpp includes every solver header) compiles against these; no
+// definitions are built, so a program that constructs one does not link.
+#pragma once
+
+namespace FV {
+
+class Scalar2D {
+public:
+    Scalar2D(const double& xl, const double& xr, const int& nx, const double& yb, const double& yt, const int& ny,
+             const double& cflnumber, const double& tmax, const double& tInit);
+    void initialize();
+    void periodicBc();
+    void rhs();
+    void RungeKutta3();
+    void timeStepping();
+    void Exact();
+    void printResult();
+};
+
+}  // namespace FV

@@ -61,6 +61,9 @@ public:
     // Snapshot of the state, time, step count and configuration (htx_save),
     // and its restoration into a solver of the same grid and scheme.
     void saveSnapshot(const std::string& path) { check(htx_save(h_, path.c_str())); }
+    // One SSP-RK3 stage with a given dt (the stage loop of RungeKutta3,
+    // FV_System2D.cpp:575-612, private in the reference): stages 1, 2, 3 in order.
+    void stage(int k, double dt) { check(htx_stage(h_, k, dt)); }
     void loadSnapshot(const std::string& path) { check(htx_load(h_, path.c_str())); }
 
 protected:

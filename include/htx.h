@@ -149,6 +149,24 @@ int htx_init_shock_blobs(htx_solver* h, double xs, double ys, double nx, double 
  * (FV_System2D.cpp:363-573); written to host memory in the state layout. */
 int htx_rhs(htx_solver* h, double* rhs, size_t ld);
 
+/* One SSP-RK3 stage with a given dt: the stage loop of
+ * FV::System2D::RungeKutta3 (FV_System2D.cpp:575-612;
+ * FVHermite_System2D.cpp:733-782), which the reference keeps private, exposed
+ * as a per-stage hook.  Stage 1 forms u1 = un + dt L(un) from the current
+ * state (for the Hermite scheme it also limits the derivative moments of un
+ * in place, as the reference's first rhs() does), stage 2
+ * u2 = 3/4 un + 1/4 (u1 + dt L(u1)), stage 3 (un + 2 (u2 + dt L(u2))) / 3,
+ * which replaces the current state.  Time and step count do not change
+ * (htx_step advances them); stage 3 leaves the CFL maxima of the new state
+ * for the next dt.  Stages run in the order 1, 2, 3 (HTX_ESTATE otherwise;
+ * setting a state or stepping starts the order again).  Enqueued without
+ * blocking. */
+int htx_stage(htx_solver* h, int stage, double dt);
+
+/* The intermediate state u1 (stage 1) or u2 (stage 2) of the last stages
+ * run, in the state layout (blocks). */
+int htx_get_stage_state(htx_solver* h, int stage, double* q, size_t ld);
+
 /* The dt that the next step would take (FV_System2D.cpp:623-645). */
 int htx_compute_dt(htx_solver* h, double* dt);
 

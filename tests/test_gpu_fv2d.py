@@ -375,3 +375,44 @@ def test_tile_shapes_bit_identical(precision, tile):
         s.close()
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+def test_stage_hook_parity(orc, flux):
+    """htx_stage, the per-stage hook of RungeKutta3 (FV_System2D.cpp:575-612):
+    each of the three stages with a given dt equals the oracle's stage
+    combination of its residual (Schemes_SSPRK.cpp:3-15), bit for bit; the
+    three stages together are htx_step's state; stages out of order are
+    refused."""
+    import htx
+
+    n = 40
+    s, prm = make(n, n, "zero_grad", flux)
+    q = quadrant_state(n, n, seed=11)
+    s.set_state(q)
+    dt = 0.6 * orc.fv2d_dt(prm, q, 0.5, 0.0, 10.0)
+    with pytest.raises(htx.HtxError) as ei:
+        s.stage(2, dt)
+    assert ei.value.code == -4
+    s.stage(1, dt)
+    u1 = orc.rk3_combine(1, q, q, orc.fv2d_rhs(prm, q), dt)
+    np.testing.assert_array_equal(s.stage_state(1), u1)
+    s.stage(2, dt)
+    u2 = orc.rk3_combine(2, q, u1, orc.fv2d_rhs(prm, u1), dt)
+    np.testing.assert_array_equal(s.stage_state(2), u2)
+    s.stage(3, dt)
+    u3 = orc.rk3_combine(3, q, u2, orc.fv2d_rhs(prm, u2), dt)
+    np.testing.assert_array_equal(s.get_state(), u3)
+    assert s.time == 0.0 and s.nsteps == 0
+    # the CFL maxima stage 3 left give htx_step's next dt
+    dt_next = s.compute_dt()
+    assert dt_next == orc.fv2d_dt(prm, u3, 0.5, 0.0, 10.0)
+    t, prm2 = make(n, n, "zero_grad", flux)
+    t.set_state(q)
+    dt_full = t.step()
+    s.set_state(q)
+    for k in (1, 2, 3):
+        s.stage(k, dt_full)
+    np.testing.assert_array_equal(s.get_state(), t.get_state())
+    s.close()
+    t.close()
