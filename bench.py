@@ -122,7 +122,7 @@ def study():
     w4 = shock_blob_workload(4096)
     sb = {"workload": f"planar Mach {w4['mach']:.2f} shock into 32 seeded Gaussian density blobs, 4096 x 4096 on "
                       f"[0,4] x [0,1], TENO5 + HLLC, transmissive, 20 steps from t = 0 after 2 warm-up steps"}
-    for p in ("fp64", "fp64_fast", "fp32_mixed", "learned"):
+    for p in ("fp64", "fp64_fast", "fp32_mixed", "learned", "fp32", "learned_fp32"):
         log(f"[study] shock-blob throughput {p}")
         sb[p] = throughput(w4, 4096, 4096, p, steps=20, warmup=2)
     sb["learned_vs_fp64"] = sb["learned"]["value"] / sb["fp64"]["value"]
@@ -232,7 +232,7 @@ def time_variant(n, steps, warmup, **kw):
     s.isentropicVortex(w["centre"], w["modes"])
     s.steps_async(warmup)
     s.sync()
-    learned = kw.get("precision") == "learned"
+    learned = str(kw.get("precision", "")).startswith("learned")
     if learned:
         s.classifier_stats(reset=True)
     t0 = time.perf_counter()
@@ -350,6 +350,8 @@ def main():
         for name, kw in [("teno5_hllc_fp64_fast", dict(flux=args.flux, precision="fp64_fast")),
                          ("teno5_hllc_fp32_mixed", dict(flux=args.flux, precision="fp32_mixed")),
                          ("teno5_hllc_learned", dict(flux=args.flux, precision="learned")),
+                         ("teno5_hllc_fp32", dict(flux=args.flux, precision="fp32")),
+                         ("teno5_hllc_learned_fp32", dict(flux=args.flux, precision="learned_fp32")),
                          ("hteno_hllc_fp64", dict(flux=args.flux, scheme="hteno")),
                          ("hteno_hllc_fp64_fast", dict(flux=args.flux, scheme="hteno", precision="fp64_fast")),
                          ("hteno_hllc_fp32_mixed", dict(flux=args.flux, scheme="hteno", precision="fp32_mixed")),

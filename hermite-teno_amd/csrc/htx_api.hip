@@ -99,6 +99,9 @@ struct htx_solver {
 
 namespace {
 
+bool is_learned(int precision) { return precision == HTX_PREC_LEARNED || precision == HTX_PREC_LEARNED_FP32; }
+bool is_bulk32(int precision) { return precision == HTX_PREC_FP32 || precision == HTX_PREC_LEARNED_FP32; }
+
 int check_params(const htx_params* p) {
     if (!p) return fail(HTX_EINVAL, "params is null");
     if (p->struct_size != (int)sizeof(htx_params)) return fail(HTX_EINVAL, "params.struct_size mismatch");
@@ -120,11 +123,12 @@ int check_params(const htx_params* p) {
     if (p->scheme < HTX_SCHEME_TENO5 || p->scheme > HTX_SCHEME_WENO5Z) return fail(HTX_EINVAL, "unknown scheme");
     if (p->flux != HTX_FLUX_RUSANOV_CHAR && p->flux != HTX_FLUX_HLLC) return fail(HTX_EINVAL, "unknown flux");
     if (p->bc != HTX_BC_ZERO_GRAD && p->bc != HTX_BC_PERIODIC) return fail(HTX_EINVAL, "unknown bc");
-    if (p->precision != HTX_PREC_FP64 && p->precision != HTX_PREC_FP32_MIXED && p->precision != HTX_PREC_LEARNED &&
-        p->precision != HTX_PREC_FP64_FAST)
+    if (p->precision < HTX_PREC_FP64 || p->precision > HTX_PREC_LEARNED_FP32)
         return fail(HTX_EINVAL, "unknown precision");
-    if (p->precision == HTX_PREC_LEARNED && p->scheme != HTX_SCHEME_TENO5)
+    if (is_learned(p->precision) && p->scheme != HTX_SCHEME_TENO5)
         return fail(HTX_ENOTSUP, "the learned precision is built for the TENO5 scheme only");
+    if (p->precision == HTX_PREC_FP32 && p->scheme != HTX_SCHEME_TENO5)
+        return fail(HTX_ENOTSUP, "the fp32 bulk precision is built for the TENO5 scheme only");
     return HTX_OK;
 }
 
@@ -141,8 +145,9 @@ Fv2dArgs stage_args(htx_solver* h, const double* qin, const double* q0, double* 
     a.bc = h->p.bc;
     a.mode = mode;
     a.quirks = h->p.ref_quirks ? 1 : 0;
-    a.mixed = (h->p.precision == HTX_PREC_FP32_MIXED || h->p.precision == HTX_PREC_LEARNED) ? 1 : 0;
-    a.learned = h->p.precision == HTX_PREC_LEARNED ? 1 : 0;
+    a.mixed = (h->p.precision == HTX_PREC_FP32_MIXED || is_learned(h->p.precision) || is_bulk32(h->p.precision)) ? 1 : 0;
+    a.learned = is_learned(h->p.precision) ? 1 : 0;
+    a.bulk32 = is_bulk32(h->p.precision) ? 1 : 0;
     a.face_flags_x = h->dFlagsX;
     a.face_flags_y = h->dFlagsY;
     a.gam = h->p.gamma;
@@ -191,7 +196,7 @@ FvhArgs hermite_args(htx_solver* h, const double* qin, const double* q0, double*
 }
 
 int launch_stage(htx_solver* h, const double* qin, const double* q0, double* qout, int mode) {
-    if (h->p.precision == HTX_PREC_LEARNED && !h->have_classifier)
+    if (is_learned(h->p.precision) && !h->have_classifier)
         return fail(HTX_ESTATE, "precision 'learned' needs a classifier (htx_load_classifier)");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (h->timing) {
@@ -206,7 +211,7 @@ int launch_stage(htx_solver* h, const double* qin, const double* q0, double* qou
         HIP_TRY(hipEventRecord(e0, h->stream));
     }
     const bool strict = h->p.precision == HTX_PREC_FP64;
-    if (h->p.precision == HTX_PREC_LEARNED && mode <= 1) {
+    if (is_learned(h->p.precision) && mode <= 1) {
         // classify every face of u^n once per step; stages 2 and 3 reuse the flags
         LrnFlagsArgs f{};
         f.q = qin;
@@ -375,7 +380,7 @@ int htx_create(const htx_params* p, htx_solver** out) {
         htx_destroy(h);
         return fail(HTX_ENOMEM, "device allocation failed");
     }
-    if (p->precision == HTX_PREC_LEARNED && (hipMalloc(&h->dFlagsX, lrn_flag_bytes_x(p->nx, p->ny)) != hipSuccess ||
+    if (is_learned(p->precision) && (hipMalloc(&h->dFlagsX, lrn_flag_bytes_x(p->nx, p->ny)) != hipSuccess ||
                                              hipMalloc(&h->dFlagsY, lrn_flag_bytes_y(p->nx, p->ny)) != hipSuccess)) {
         htx_destroy(h);
         return fail(HTX_ENOMEM, "device allocation failed");
@@ -1176,7 +1181,7 @@ int htx_kxrcf_flags(htx_solver* h, unsigned char* flags) {
 
 int htx_classifier_flags(htx_solver* h, unsigned char* fx, unsigned char* fy) {
     if (!h || !fx || !fy) return fail(HTX_EINVAL, "null argument");
-    if (h->p.precision != HTX_PREC_LEARNED || !h->dFlagsX)
+    if (!is_learned(h->p.precision) || !h->dFlagsX)
         return fail(HTX_ESTATE, "the solver does not run the learned precision");
     HIP_TRY(copy_sync(fx, h->dFlagsX, lrn_flag_bytes_x(h->nx, h->ny), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(copy_sync(fy, h->dFlagsY, lrn_flag_bytes_y(h->nx, h->ny), hipMemcpyDeviceToHost, h->stream));

@@ -15,13 +15,18 @@
 //            speed (wavefront shuffle reduction, one atomic per workgroup).
 // Face states never leave the chip; HBM sees the state read, u^n read
 // (stages 2-3) and the stage result write.
+#include <type_traits>
+
 #include "htx_fv2d_tile.h"
 
 namespace HTX_NS {
 
-template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED>
-__global__ void __launch_bounds__(256, 2) fv2d_stage_kernel(Fv2dArgs a) {
-    using T = Fv2dTile<TX, TY>;
+// BULK32: the FP32 bulk precision (face values stored in FP32, quadrature,
+// back-projection and flux in FP32): a smaller tile in LDS and fewer
+// registers, three workgroups per CU
+template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED, bool BULK32 = false>
+__global__ void __launch_bounds__(256, BULK32 ? 3 : 2) fv2d_stage_kernel(Fv2dArgs a) {
+    using T = Fv2dTile<TX, TY, typename std::conditional<BULK32, float, double>::type>;
     __shared__ T s;
     double dt = 0.0;
     if (a.mode > 0) {
@@ -136,27 +141,34 @@ __global__ void __launch_bounds__(256) k_ic_vortex(VortexArgs a) {
     for (int e = 0; e < 4; ++e) a.q[off + e * a.fstride] = acc[e];
 }
 
-template <int TX, int TY, bool MIXED, bool LEARNED>
+template <int TX, int TY, bool MIXED, bool LEARNED, bool BULK32 = false>
 static hipError_t launch_tile(const Fv2dArgs& a0, int flux, hipStream_t s) {
     Fv2dArgs a = a0;
     a.tiles_x = (a.nx + TX - 1) / TX;
     a.tiles_y = (a.ny + TY - 1) / TY;
     dim3 grid(a.tiles_x * a.tiles_y), block(256);
     if (flux == FLUX_RUSANOV_CHAR)
-        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_RUSANOV_CHAR, MIXED, LEARNED>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_RUSANOV_CHAR, MIXED, LEARNED, BULK32>), grid, block, 0, s, a);
     else if (flux == FLUX_HLLC)
-        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC, MIXED, LEARNED>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC, MIXED, LEARNED, BULK32>), grid, block, 0, s, a);
     else
-        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC_TORO, MIXED, LEARNED>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC_TORO, MIXED, LEARNED, BULK32>), grid, block, 0, s, a);
     return hipGetLastError();
 }
 
 template <int TX, int TY>
 static hipError_t launch_prec(const Fv2dArgs& a, int flux, hipStream_t s) {
-    if (a.learned) {
-        if (!a.face_flags_x || !a.face_flags_y) return hipErrorInvalidValue;
-        return launch_tile<TX, TY, true, true>(a, flux, s);
+    if (a.learned && (!a.face_flags_x || !a.face_flags_y)) return hipErrorInvalidValue;
+#if !HTX_EXACT_DIVISION
+    // the FP32 bulk precision is built for the 16 x 16 and 16 x 8 tiles
+    if constexpr (TX == 16) {
+        if (a.bulk32) {
+            if (a.learned) return launch_tile<TX, TY, true, true, true>(a, flux, s);
+            return launch_tile<TX, TY, true, false, true>(a, flux, s);
+        }
     }
+#endif
+    if (a.learned) return launch_tile<TX, TY, true, true>(a, flux, s);
     if (a.mixed) return launch_tile<TX, TY, true, false>(a, flux, s);
     return launch_tile<TX, TY, false, false>(a, flux, s);
 }
@@ -167,7 +179,7 @@ static hipError_t launch_prec(const Fv2dArgs& a, int flux, hipStream_t s) {
 hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y, hipStream_t s) {
 #if !HTX_EXACT_DIVISION
     if (tile_x == 16 && tile_y == 8) return launch_prec<16, 8>(a, flux, s);
-    if (tile_x == 32 && tile_y == 8) return launch_prec<32, 8>(a, flux, s);
+    if (tile_x == 32 && tile_y == 8 && !a.bulk32) return launch_prec<32, 8>(a, flux, s);
 #else
     (void)tile_x;
     (void)tile_y;

@@ -13,7 +13,9 @@
 
 namespace HTX_NS {
 
-template <int TX, int TY>
+// FT: storage type of the characteristic face values (double; float in the
+// FP32 bulk precision, which halves their LDS footprint)
+template <int TX, int TY, typename FT = double>
 struct Fv2dTile {
     static constexpr int NT = 256;
     static constexpr int HX = TX + 6, HY = TY + 6, HC = HX * HY;
@@ -27,7 +29,7 @@ struct Fv2dTile {
     static constexpr int KC = (NC + NT - 1) / NT;
     double sQ[4][HC];  // conserved state
     double sP[4][HC];  // sqrt|rho|, u, v, H
-    double sF[8][NF];  // characteristic face values (left/right per field)
+    FT sF[8][NF];      // characteristic face values (left/right per field)
     double sFl[4][NQ]; // face fluxes
     double sRed[2][NT / 64];
     unsigned char sFlag[2][NF];  // learned precision: 1 = linear scheme on this recon face
@@ -36,9 +38,9 @@ struct Fv2dTile {
 
 // halo load with the boundary condition applied through the index, plus the
 // per-cell Roe inputs of calcPrimitiveVariable (FV_System2D.cpp:226-263)
-template <int TX, int TY>
-HTX_HD void ph_load(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int i0, int j0, int tid) {
-    using T = Fv2dTile<TX, TY>;
+template <int TX, int TY, typename FT>
+HTX_HD void ph_load(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int i0, int j0, int tid) {
+    using T = Fv2dTile<TX, TY, FT>;
     const long long fs = a.fstride;
     for (int w = tid; w < T::HC; w += T::NT) {
         int lx = w % T::HX, ly = w / T::HX;
@@ -58,8 +60,8 @@ HTX_HD void ph_load(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int i0, int j0, int 
     }
 }
 
-template <int TX, int TY>
-HTX_HD Prim prim_at(const Fv2dTile<TX, TY>& s, int id) {
+template <int TX, int TY, typename FT>
+HTX_HD Prim prim_at(const Fv2dTile<TX, TY, FT>& s, int id) {
     return Prim{s.sP[0][id], s.sP[1][id], s.sP[2][id], s.sP[3][id]};
 }
 
@@ -81,9 +83,9 @@ HTX_HD unsigned char line_flag(const unsigned char* line, int g, int n, int bc, 
 // Learned precision: the flags of the tile's recon faces, from the face-flag
 // arrays of the step (rows of x faces and columns of y faces outside the grid
 // map like the cells their stencils read).
-template <int TX, int TY>
-HTX_HD void ph_flags_load(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int i0, int j0, int tid) {
-    using T = Fv2dTile<TX, TY>;
+template <int TX, int TY, typename FT>
+HTX_HD void ph_flags_load(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int i0, int j0, int tid) {
+    using T = Fv2dTile<TX, TY, FT>;
     for (int w = tid; w < T::NXF; w += T::NT) {
         int gi = i0 - 1 + w % (TX + 1), gj = map_index(j0 - 2 + w / (TX + 1), a.ny, a.bc);
         s.sFlag[0][w] = line_flag(a.face_flags_x + (long long)gj * (a.nx + 2), gi, a.nx, a.bc, 1);
@@ -121,10 +123,10 @@ HTX_HD bool recon_face_owned(int fi, int fj, int i0, int j0, int nx, int ny) {
 // have positive density and pressure on both sides (checked in FP64; else
 // TENO5); the face's flag is then overwritten with the scheme it took (1 =
 // linear), and cnt accumulates (owned faces, flagged, linear) of this thread.
-template <int TX, int TY, bool YDIR, bool MIXED = false, bool LEARNED = false>
-HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid, int i0 = 0, int j0 = 0,
+template <int TX, int TY, bool YDIR, bool MIXED = false, bool LEARNED = false, typename FT = double>
+HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int tid, int i0 = 0, int j0 = 0,
                      unsigned* cnt = nullptr) {
-    using T = Fv2dTile<TX, TY>;
+    using T = Fv2dTile<TX, TY, FT>;
     constexpr int NW = YDIR ? T::NYF : T::NXF;
     constexpr int RW = YDIR ? (TX + 4) : (TX + 1);   // faces per row of the face grid
     constexpr int STEP = YDIR ? T::HX : 1;           // LDS stride along the normal
@@ -200,8 +202,8 @@ HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid, int i0 = 0
 // or column (y flux): their 6-cell stencils cover the window's transverse
 // extent, so a discontinuity running along the face normal through the
 // window is seen there.
-template <int TX, int TY, bool YDIR>
-HTX_HD bool flux_window_smooth(const Fv2dTile<TX, TY>& s, int fi, int fj) {
+template <int TX, int TY, bool YDIR, typename FT>
+HTX_HD bool flux_window_smooth(const Fv2dTile<TX, TY, FT>& s, int fi, int fj) {
     if (!YDIR) {
         const unsigned char* N = s.sFlag[0];
         const int b = fj * (TX + 1) + fi;
@@ -222,9 +224,9 @@ HTX_HD bool flux_window_smooth(const Fv2dTile<TX, TY>& s, int fi, int fj) {
 // item w = tid + k NT of ph_flux) has a smooth window.  On the device a
 // wavefront vote over its active lanes; on the host, where a phase runs item
 // by item, the same vote taken over the items of that wavefront.
-template <int TX, int TY, bool YDIR>
-HTX_HD bool window_smooth_wave(const Fv2dTile<TX, TY>& s, int w, int tid) {
-    using T = Fv2dTile<TX, TY>;
+template <int TX, int TY, bool YDIR, typename FT>
+HTX_HD bool window_smooth_wave(const Fv2dTile<TX, TY, FT>& s, int w, int tid) {
+    using T = Fv2dTile<TX, TY, FT>;
     constexpr int NW = YDIR ? T::NYQ : T::NXQ;
     constexpr int RW = YDIR ? TX : (TX + 1);
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -240,6 +242,103 @@ HTX_HD bool window_smooth_wave(const Fv2dTile<TX, TY>& s, int w, int tid) {
 #endif
 }
 
+// FP32 bulk precision of ph_flux: the transverse quadrature (or, on smooth
+// windows of the learned path, its linear limit), the back-projection with
+// the face's R and the Riemann flux at the 3 Gauss points in FP32, the
+// Gauss-weighted sum in FP64.  A Gauss-point state whose density is not
+// positive and well above the cancellation of its back-projection
+// (rho > 2^-10 (|c0| + |c2| + |c3|)), or whose pressure is not positive and
+// at least 2^-10 of its energy (rho E - m^2 / 2 > 2^-10 rho E), sends the
+// face to the FP64 path: back-projection and flux recomputed in FP64 from
+// the same Gauss-point values (on the learned path's linear windows, first
+// the nonlinear quadrature).
+HTX_HD bool state_fp32_ok(const float q[4], float c0, float c2, float c3) {
+    const float re = q[3] * q[0];
+    return q[0] > 0x1p-10f * (fabsf(c0) + fabsf(c2) + fabsf(c3)) &&
+           re - 0.5f * (q[1] * q[1] + q[2] * q[2]) > 0x1p-10f * re;
+}
+
+template <int TX, int TY, bool YDIR, int FLUX, bool LEARNED>
+HTX_HD void ph_flux32(const Fv2dArgs& a, Fv2dTile<TX, TY, float>& s, int tid) {
+    using T = Fv2dTile<TX, TY, float>;
+    constexpr int NW = YDIR ? T::NYQ : T::NXQ;
+    constexpr int RW = YDIR ? TX : (TX + 1);
+    constexpr int RR = YDIR ? (TX + 4) : (TX + 1);
+    constexpr int TS = YDIR ? 1 : (TX + 1);
+    constexpr int STEP = YDIR ? T::HX : 1;
+    const float gam = (float)a.gam, g1 = (float)a.g1;
+    for (int w = tid; w < NW; w += T::NT) {
+        const int fi = w % RW, fj = w / RW;
+        const int b = fj * RR + fi;
+        const int ca = YDIR ? (2 + fj) * T::HX + (3 + fi) : (3 + fj) * T::HX + (2 + fi);
+        const Eig E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
+        const Eig32 E32 = eig32(E);
+        float gl[4][3], gr[4][3];
+        bool lin = LEARNED && window_smooth_wave<TX, TY, YDIR>(s, w, tid);
+        auto quadrature = [&](bool linear) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                const float* L = s.sF[2 * f];
+                const float* R = s.sF[2 * f + 1];
+                if (linear) {
+                    weno3q_linear_f32(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0],
+                                      gl[f][1], gl[f][2]);
+                    weno3q_linear_f32(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0],
+                                      gr[f][1], gr[f][2]);
+                } else {
+                    weno3q_f32(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
+                               gl[f][2]);
+                    weno3q_f32(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
+                               gr[f][2]);
+                }
+            }
+        };
+        quadrature(lin);
+        double acc[3][4];
+        bool ok = true;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            float ql[4], qr[4], fl[4];
+            eig_right_f32(E32, gl[0][m], gl[1][m], gl[2][m], gl[3][m], ql[0], ql[1], ql[2], ql[3]);
+            eig_right_f32(E32, gr[0][m], gr[1][m], gr[2][m], gr[3][m], qr[0], qr[1], qr[2], qr[3]);
+            ok = ok && state_fp32_ok(ql, gl[0][m], gl[2][m], gl[3][m]) && state_fp32_ok(qr, gr[0][m], gr[2][m], gr[3][m]);
+            if (FLUX == FLUX_RUSANOV_CHAR)
+                rusanov_char_f32(gam, g1, E32, ql, qr, fl);
+            else if (FLUX == FLUX_HLLC)
+                hllc_f32<true>(gam, g1, ql, qr, fl);
+            else
+                hllc_f32<false>(gam, g1, ql, qr, fl);
+            const double gw = (m == 1) ? 8.0 / 9.0 / 2.0 : 5.0 / 9.0 / 2.0;
+            acc[m][0] = gw * (double)fl[0];
+            acc[m][1] = gw * (double)(YDIR ? fl[2] : fl[1]);
+            acc[m][2] = gw * (double)(YDIR ? fl[1] : fl[2]);
+            acc[m][3] = gw * (double)fl[3];
+        }
+        if (!ok) {
+            if (lin) quadrature(false);
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                double ql[4], qr[4], fl[4];
+                eig_right(E, gl[0][m], gl[1][m], gl[2][m], gl[3][m], ql[0], ql[1], ql[2], ql[3]);
+                eig_right(E, gr[0][m], gr[1][m], gr[2][m], gr[3][m], qr[0], qr[1], qr[2], qr[3]);
+                if (FLUX == FLUX_RUSANOV_CHAR)
+                    rusanov_char<YDIR>(a.gam, a.g1, E, ql, qr, fl);
+                else if (FLUX == FLUX_HLLC)
+                    hllc<true>(a.gam, a.g1, ql, qr, fl);
+                else
+                    hllc<false>(a.gam, a.g1, ql, qr, fl);
+                const double gw = (m == 1) ? 8.0 / 9.0 / 2.0 : 5.0 / 9.0 / 2.0;
+                acc[m][0] = gw * fl[0];
+                acc[m][1] = gw * (YDIR ? fl[2] : fl[1]);
+                acc[m][2] = gw * (YDIR ? fl[1] : fl[2]);
+                acc[m][3] = gw * fl[3];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) s.sFl[e][w] = (acc[0][e] + acc[1][e]) + acc[2][e];
+    }
+}
+
 // transverse 3-point quadrature, back-projection with the face's own R,
 // Riemann flux at the Gauss points weighted by (5/18, 4/9, 5/18)
 // (FV_System2D.cpp:402-497); the flux is stored in (rho, x, y, E) order.
@@ -249,89 +348,93 @@ HTX_HD bool window_smooth_wave(const Fv2dTile<TX, TY>& s, int w, int tid) {
 // non-positive density or pressure (checked in FP64); every other face takes
 // the nonlinear quadrature.  The wavefront-wide decision keeps the two paths
 // from running one after the other in the same wavefront.
-template <int TX, int TY, bool YDIR, int FLUX, bool MIXED = false, bool LEARNED = false>
-HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tid) {
-    using T = Fv2dTile<TX, TY>;
-    constexpr int NW = YDIR ? T::NYQ : T::NXQ;
-    constexpr int RW = YDIR ? TX : (TX + 1);          // flux faces per row
-    constexpr int RR = YDIR ? (TX + 4) : (TX + 1);    // recon faces per row
-    constexpr int TS = YDIR ? 1 : (TX + 1);           // recon stride along the transverse direction
-    constexpr int STEP = YDIR ? T::HX : 1;
-    for (int w = tid; w < NW; w += T::NT) {
-        int fi = w % RW, fj = w / RW;
-        // first of the 5 transverse recon faces, and the face's left/lower cell
-        int b = YDIR ? fj * RR + fi : fj * RR + fi;
-        int ca = YDIR ? (2 + fj) * T::HX + (3 + fi) : (3 + fj) * T::HX + (2 + fi);
-        // the learned path needs the eigensystem for its positivity check;
-        // the others form it after the quadrature (fewer live registers)
-        Eig E;
-        if (LEARNED) E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
-        double gl[4][3], gr[4][3];
-        bool linq = false;
-        if (LEARNED && window_smooth_wave<TX, TY, YDIR>(s, w, tid)) {
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                const double* L = s.sF[2 * f];
-                const double* R = s.sF[2 * f + 1];
-                weno3q_linear(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
-                              gl[f][2]);
-                weno3q_linear(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
-                              gr[f][2]);
-            }
-            linq = true;
-#pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                double gm[4] = {gl[0][m], gl[1][m], gl[2][m], gl[3][m]};
-                double gp[4] = {gr[0][m], gr[1][m], gr[2][m], gr[3][m]};
-                linq = linq && face_state_positive(E, gm) && face_state_positive(E, gp);
-            }
-        }
-        if (!linq) {
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                const double* L = s.sF[2 * f];
-                const double* R = s.sF[2 * f + 1];
-                if (MIXED) {
-                    weno3q_mixed(a.qc, a.qc.lw32, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS],
-                                 gl[f][0], gl[f][1], gl[f][2]);
-                    weno3q_mixed(a.qc, a.qc.lw32, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS],
-                                 gr[f][0], gr[f][1], gr[f][2]);
-                } else {
-                    weno3q(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
-                           gl[f][2]);
-                    weno3q(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
-                           gr[f][2]);
+template <int TX, int TY, bool YDIR, int FLUX, bool MIXED = false, bool LEARNED = false, typename FT = double>
+HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int tid) {
+    if constexpr (sizeof(FT) == sizeof(float)) {
+        ph_flux32<TX, TY, YDIR, FLUX, LEARNED>(a, s, tid);
+    } else {
+        using T = Fv2dTile<TX, TY, FT>;
+        constexpr int NW = YDIR ? T::NYQ : T::NXQ;
+        constexpr int RW = YDIR ? TX : (TX + 1);          // flux faces per row
+        constexpr int RR = YDIR ? (TX + 4) : (TX + 1);    // recon faces per row
+        constexpr int TS = YDIR ? 1 : (TX + 1);           // recon stride along the transverse direction
+        constexpr int STEP = YDIR ? T::HX : 1;
+        for (int w = tid; w < NW; w += T::NT) {
+            int fi = w % RW, fj = w / RW;
+            // first of the 5 transverse recon faces, and the face's left/lower cell
+            int b = YDIR ? fj * RR + fi : fj * RR + fi;
+            int ca = YDIR ? (2 + fj) * T::HX + (3 + fi) : (3 + fj) * T::HX + (2 + fi);
+            // the learned path needs the eigensystem for its positivity check;
+            // the others form it after the quadrature (fewer live registers)
+            Eig E;
+            if (LEARNED) E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
+            double gl[4][3], gr[4][3];
+            bool linq = false;
+            if (LEARNED && window_smooth_wave<TX, TY, YDIR>(s, w, tid)) {
+    #pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    const double* L = s.sF[2 * f];
+                    const double* R = s.sF[2 * f + 1];
+                    weno3q_linear(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
+                                  gl[f][2]);
+                    weno3q_linear(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
+                                  gr[f][2]);
+                }
+                linq = true;
+    #pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    double gm[4] = {gl[0][m], gl[1][m], gl[2][m], gl[3][m]};
+                    double gp[4] = {gr[0][m], gr[1][m], gr[2][m], gr[3][m]};
+                    linq = linq && face_state_positive(E, gm) && face_state_positive(E, gp);
                 }
             }
+            if (!linq) {
+    #pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    const double* L = s.sF[2 * f];
+                    const double* R = s.sF[2 * f + 1];
+                    if (MIXED) {
+                        weno3q_mixed(a.qc, a.qc.lw32, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS],
+                                     gl[f][0], gl[f][1], gl[f][2]);
+                        weno3q_mixed(a.qc, a.qc.lw32, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS],
+                                     gr[f][0], gr[f][1], gr[f][2]);
+                    } else {
+                        weno3q(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
+                               gl[f][2]);
+                        weno3q(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
+                               gr[f][2]);
+                    }
+                }
+            }
+            if (!LEARNED) E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
+            double acc[3][4];
+    #pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                double ql[4], qr[4], fl[4];
+                eig_right(E, gl[0][m], gl[1][m], gl[2][m], gl[3][m], ql[0], ql[1], ql[2], ql[3]);
+                eig_right(E, gr[0][m], gr[1][m], gr[2][m], gr[3][m], qr[0], qr[1], qr[2], qr[3]);
+                if (FLUX == FLUX_RUSANOV_CHAR)
+                    rusanov_char<YDIR>(a.gam, a.g1, E, ql, qr, fl);
+                else if (FLUX == FLUX_HLLC)
+                    hllc<true>(a.gam, a.g1, ql, qr, fl);
+                else
+                    hllc<false>(a.gam, a.g1, ql, qr, fl);
+                const double gw = (m == 1) ? 8.0 / 9.0 / 2.0 : 5.0 / 9.0 / 2.0;
+                acc[m][0] = gw * fl[0];
+                acc[m][1] = gw * (YDIR ? fl[2] : fl[1]);
+                acc[m][2] = gw * (YDIR ? fl[1] : fl[2]);
+                acc[m][3] = gw * fl[3];
+            }
+    #pragma unroll
+            for (int e = 0; e < 4; ++e) s.sFl[e][w] = (acc[0][e] + acc[1][e]) + acc[2][e];
         }
-        if (!LEARNED) E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
-        double acc[3][4];
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            double ql[4], qr[4], fl[4];
-            eig_right(E, gl[0][m], gl[1][m], gl[2][m], gl[3][m], ql[0], ql[1], ql[2], ql[3]);
-            eig_right(E, gr[0][m], gr[1][m], gr[2][m], gr[3][m], qr[0], qr[1], qr[2], qr[3]);
-            if (FLUX == FLUX_RUSANOV_CHAR)
-                rusanov_char<YDIR>(a.gam, a.g1, E, ql, qr, fl);
-            else if (FLUX == FLUX_HLLC)
-                hllc<true>(a.gam, a.g1, ql, qr, fl);
-            else
-                hllc<false>(a.gam, a.g1, ql, qr, fl);
-            const double gw = (m == 1) ? 8.0 / 9.0 / 2.0 : 5.0 / 9.0 / 2.0;
-            acc[m][0] = gw * fl[0];
-            acc[m][1] = gw * (YDIR ? fl[2] : fl[1]);
-            acc[m][2] = gw * (YDIR ? fl[1] : fl[2]);
-            acc[m][3] = gw * fl[3];
-        }
-#pragma unroll
-        for (int e = 0; e < 4; ++e) s.sFl[e][w] = (acc[0][e] + acc[1][e]) + acc[2][e];
     }
 }
 
 // x part of the divergence, (fx_{i-1} - fx_i) * dy, kept per thread
-template <int TX, int TY>
-HTX_HD void ph_xdiv(const Fv2dArgs& a, const Fv2dTile<TX, TY>& s, int tid, double dxp[][4]) {
-    using T = Fv2dTile<TX, TY>;
+template <int TX, int TY, typename FT>
+HTX_HD void ph_xdiv(const Fv2dArgs& a, const Fv2dTile<TX, TY, FT>& s, int tid, double dxp[][4]) {
+    using T = Fv2dTile<TX, TY, FT>;
 #pragma unroll
     for (int k = 0; k < T::KC; ++k) {
         int w = tid + k * T::NT;
@@ -347,10 +450,10 @@ HTX_HD void ph_xdiv(const Fv2dArgs& a, const Fv2dTile<TX, TY>& s, int tid, doubl
 // residual (FV_System2D.cpp:562-572), SSP-RK3 stage (:575-612 with
 // Schemes_SSPRK.cpp:3-15) and, in stage 3, the per-thread CFL maxima
 // (:623-631 with Equations_Euler2D.cpp:207-219)
-template <int TX, int TY>
-HTX_HD void ph_epilogue(const Fv2dArgs& a, const Fv2dTile<TX, TY>& s, int i0, int j0, int tid,
+template <int TX, int TY, typename FT>
+HTX_HD void ph_epilogue(const Fv2dArgs& a, const Fv2dTile<TX, TY, FT>& s, int i0, int j0, int tid,
                         const double dxp[][4], double dt, double& smx, double& smy, bool* bad = nullptr) {
-    using T = Fv2dTile<TX, TY>;
+    using T = Fv2dTile<TX, TY, FT>;
     const long long fs = a.fstride;
 #pragma unroll
     for (int k = 0; k < T::KC; ++k) {
@@ -412,9 +515,9 @@ __device__ __forceinline__ double wave_max(double v) {
 // the tile's CFL maxima (one atomic per workgroup).  dt = the stage's time
 // step.  Ends with every thread past its last LDS access only for mode 3;
 // callers that reuse the tile storage synchronise before the next tile.
-template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED>
-__device__ __forceinline__ void stage_tile(const Fv2dArgs& a, Fv2dTile<TX, TY>& s, int tile, double dt, int tid) {
-    using T = Fv2dTile<TX, TY>;
+template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED, typename FT>
+__device__ __forceinline__ void stage_tile(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int tile, double dt, int tid) {
+    using T = Fv2dTile<TX, TY, FT>;
     const int i0 = (tile % a.tiles_x) * TX;
     const int j0 = (tile / a.tiles_x) * TY;
     double dxp[T::KC][4];

@@ -92,6 +92,7 @@ struct Fv2dArgs {
     int quirks;         // reference maxSpeed quirk in the CFL epilogue
     int mixed;          // 1: cut-off decisions and nonlinear weights in FP32
     int learned;        // 1: classifier flags pick linear or TENO5 per face (implies mixed)
+    int bulk32;         // 1: face values, quadrature, back-projection and flux in FP32 (implies mixed)
     int tiles_x, tiles_y;
     double gam, g1, dx, dy, area;
     QuadConsts qc;

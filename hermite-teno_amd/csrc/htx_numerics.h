@@ -52,6 +52,9 @@ struct QuadConsts {
     // one 5-point filter per Gauss point (+xi, centre, -xi): the learned path's
     // transverse quadrature on smooth windows
     double lq[3][5];
+    // FP32 bulk precision: the candidate coefficients and linear filters in
+    // FP32, the centre's group sums
+    float gp32[3][3], gm32[3][3], lq32[3][5], sumw0p32, sumw0m32;
 };
 
 inline void fill_quad_consts(QuadConsts& C) {
@@ -115,6 +118,15 @@ inline void fill_quad_consts(QuadConsts& C) {
             C.lq[1][k + j] += lin0[k] * cc[j][k];
             C.lq[2][k + j] += C.linwm[k] * C.gm[j][k];
         }
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) {
+            C.gp32[r][k] = (float)C.gp[r][k];
+            C.gm32[r][k] = (float)C.gm[r][k];
+        }
+    for (int m = 0; m < 3; ++m)
+        for (int i = 0; i < 5; ++i) C.lq32[m][i] = (float)C.lq[m][i];
+    C.sumw0p32 = (float)C.sumw0p;
+    C.sumw0m32 = (float)C.sumw0m;
 }
 
 // Fast FP32 reciprocal and base-2 logarithm on the device (one hardware
@@ -906,6 +918,184 @@ HTX_HD void hllc(double gam, double g1, const double ql[4], const double qr[4],
         f[2] = qr[1] * wrt;
         f[3] = wrn * (qr[3] + pr);
     }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 bulk precision (transverse quadrature, back-projection and Riemann
+// flux in FP32).  On CDNA4 a non-packed FP32 FMA issues at the FP64 rate, so
+// FP32 pays through single-instruction reciprocals and square roots, half
+// the registers and half the LDS bytes of the face values, not through FMA
+// throughput.  Every formula is written relative to a reference value of its
+// stencil so that constant data stay exact in FP32 (free streams are
+// preserved to the last bit).
+// ---------------------------------------------------------------------------
+
+HTX_HD float fsqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+
+// FP32 weno3q: values at the Gauss points (+xi, 0, -xi) of the 5 transverse
+// values (Schemes_Lagrange.cpp:1155-1217).  Smoothness and weights as
+// weno3q_mixed; each candidate as u2 plus its combination of the differences
+// u_j - u2 (the candidates' coefficients sum to one), each weighted sum as
+// u2 plus the weighted candidate differences, the centre's split groups
+// recombined as u2 + sumw0p (...) - sumw0m (...) (sumw0p - sumw0m = 1).
+HTX_HD void weno3q_f32(const QuadConsts& C, float u0, float u1, float u2, float u3, float u4, float& o0, float& o1,
+                       float& o2) {
+    const float c1 = 13.0f / 12.0f;
+    const float d1 = u1 - u0, d2 = u2 - u1, d3 = u3 - u2, d4 = u4 - u3;
+    const float bg0 = 3.0f * d2 - d1, cg0 = d2 - d1;
+    const float bg1 = -(d2 + d3), cg1 = d3 - d2;
+    const float bg2 = d4 - 3.0f * d3, cg2 = d4 - d3;
+    const float e0 = 1e-6f + (0.25f * (bg0 * bg0) + c1 * (cg0 * cg0));
+    const float e1 = 1e-6f + (0.25f * (bg1 * bg1) + c1 * (cg1 * cg1));
+    const float e2 = 1e-6f + (0.25f * (bg2 * bg2) + c1 * (cg2 * cg2));
+    const float r0 = frcp(e0 * e0), r1 = frcp(e1 * e1), r2 = frcp(e2 * e2);
+    // differences to the centre value
+    const float a0 = u0 - u2, a1 = u1 - u2, a3 = u3 - u2, a4 = u4 - u2;
+    const float* lw = C.lw32;
+    auto mix = [&](int k, float q0, float q1, float q2) {
+        const float w0 = lw[3 * k] * r0, w1 = lw[3 * k + 1] * r1, w2 = lw[3 * k + 2] * r2;
+        return ((w0 * q0 + w1 * q1) + w2 * q2) * frcp((w0 + w1) + w2);
+    };
+    float q0 = C.gp32[0][0] * a0 + C.gp32[1][0] * a1;
+    float q1 = C.gp32[0][1] * a1 + C.gp32[2][1] * a3;
+    float q2 = C.gp32[1][2] * a3 + C.gp32[2][2] * a4;
+    o0 = u2 + mix(0, q0, q1, q2);
+    const float r24 = 1.0f / 24.0f;
+    q0 = (2.0f * a1 - a0) * r24;
+    q1 = -(a1 + a3) * r24;
+    q2 = (2.0f * a3 - a4) * r24;
+    o1 = u2 + (C.sumw0p32 * mix(1, q0, q1, q2) - C.sumw0m32 * mix(2, q0, q1, q2));
+    q0 = C.gm32[0][0] * a0 + C.gm32[1][0] * a1;
+    q1 = C.gm32[0][1] * a1 + C.gm32[2][1] * a3;
+    q2 = C.gm32[1][2] * a3 + C.gm32[2][2] * a4;
+    o2 = u2 + mix(3, q0, q1, q2);
+    o0 = o0 != o0 ? 1e-15f : o0;  // catchnan
+    o1 = o1 != o1 ? 1e-15f : o1;
+    o2 = o2 != o2 ? 1e-15f : o2;
+}
+
+// FP32 linear limit of the quadrature (the learned path's smooth windows):
+// u2 plus each Gauss point's filter applied to the differences u_j - u2
+HTX_HD void weno3q_linear_f32(const QuadConsts& C, float u0, float u1, float u2, float u3, float u4, float& o0,
+                              float& o1, float& o2) {
+    const float a0 = u0 - u2, a1 = u1 - u2, a3 = u3 - u2, a4 = u4 - u2;
+    o0 = u2 + (((C.lq32[0][0] * a0 + C.lq32[0][1] * a1) + C.lq32[0][3] * a3) + C.lq32[0][4] * a4);
+    o1 = u2 + (((C.lq32[1][0] * a0 + C.lq32[1][1] * a1) + C.lq32[1][3] * a3) + C.lq32[1][4] * a4);
+    o2 = u2 + (((C.lq32[2][0] * a0 + C.lq32[2][1] * a1) + C.lq32[2][3] * a3) + C.lq32[2][4] * a4);
+}
+
+// The face eigensystem in FP32 (formed in FP64 by roe_eig, rounded once)
+struct Eig32 {
+    float vn, vt, hm, qm, cm, rcm, b1, b2, t2;
+};
+
+HTX_HD Eig32 eig32(const Eig& E) {
+    return Eig32{(float)E.vn, (float)E.vt, (float)E.hm, (float)E.qm, (float)E.cm,
+                 (float)E.rcm, (float)E.b1, (float)E.b2, (float)E.t2};
+}
+
+// R c in (rho, normal, tangential, E) order, FP32
+HTX_HD void eig_right_f32(const Eig32& E, float c0, float c1, float c2, float c3, float& q0, float& qn, float& qt,
+                          float& q3) {
+    const float tc = E.vn * E.cm;
+    q0 = (c0 + c2) + c3;
+    qn = ((E.vn - E.cm) * c0 + E.vn * c2) + (E.vn + E.cm) * c3;
+    qt = ((E.vt * c0 + c1) + E.vt * c2) + E.vt * c3;
+    q3 = (((E.hm - tc) * c0 + E.vt * c1) + E.qm * c2) + (E.hm + tc) * c3;
+}
+
+// L q (normal, tangential momenta order), FP32
+HTX_HD void eig_left_nt_f32(const Eig32& E, float q0, float qn, float qt, float q3, float& c0, float& c1, float& c2,
+                            float& c3) {
+    const float tr = E.vn * E.rcm, tb = E.b1 * E.vn;
+    const float ln0 = -0.5f * (tb + E.rcm), lt0 = -E.t2 * E.vt, ln3 = -0.5f * (tb - E.rcm), lt2 = E.b1 * E.vt;
+    c0 = ((0.5f * (E.b2 + tr) * q0 + ln0 * qn) + lt0 * qt) + E.t2 * q3;
+    c1 = -E.vt * q0 + qt;
+    c2 = (((1.0f - E.b2) * q0 + tb * qn) + lt2 * qt) + (-E.b1) * q3;
+    c3 = ((0.5f * (E.b2 - tr) * q0 + ln3 * qn) + lt0 * qt) + E.t2 * q3;
+}
+
+// density and pressure (gamma - 1)(E - m^2 / 2 rho) of a state positive,
+// tested as rho E - m^2 / 2 > 0 (same sign for rho > 0)
+HTX_HD bool state_positive_f32(const float q[4]) {
+    return q[0] > 0.0f && q[3] * q[0] - 0.5f * (q[1] * q[1] + q[2] * q[2]) > 0.0f;
+}
+
+// HLLC (Equations_Euler2D.cpp:372-428) in FP32, (rho, n, t, E) order; QUIRKS
+// as hllc<>
+template <bool QUIRKS>
+HTX_HD void hllc_f32(float gam, float g1, const float ql[4], const float qr[4], float f[4]) {
+    const float il = frcp(ql[0]), ir = frcp(qr[0]);
+    const float wl0 = ql[0], wln = ql[1] * il, wlt = ql[2] * il;
+    const float wr0 = qr[0], wrn = qr[1] * ir, wrt = qr[2] * ir;
+    const float pl = (ql[3] - 0.5f * (ql[1] * wln + ql[2] * wlt)) * g1;
+    const float pr = (qr[3] - 0.5f * (qr[1] * wrn + qr[2] * wrt)) * g1;
+    const float hl = (ql[3] + pl) * il, hr = (qr[3] + pr) * ir;
+    const float rho_ratio = fsqrt(fabsf(wr0 * il));
+    const float cl = QUIRKS ? fsqrt(fabsf(pl * il)) : fsqrt(fabsf(gam * pl * il));
+    const float cr = QUIRKS ? fsqrt(fabsf(pr * ir)) : fsqrt(fabsf(gam * pr * ir));
+    const float rrr = frcp(1.0f + rho_ratio);
+    const float vn = (wln + wrn * rho_ratio) * rrr, vt = (wlt + wrt * rho_ratio) * rrr;
+    const float qm = 0.5f * (vn * vn + vt * vt);
+    const float hm = (hl + hr * rho_ratio) * rrr;
+    const float cm = fsqrt(fabsf(g1 * (hm - qm)));
+    const float sl = fminf(wln - cl, vn - cm), sr = fmaxf(wrn + cr, vn + cm);
+    const float sm = (qr[1] * (sr - wrn) - ql[1] * (sl - wln) + pl - pr) * frcp(wr0 * (sr - wrn) - wl0 * (sl - wln));
+    const float ps = pl + wl0 * (wln - sl) * (wln - sm);
+    if (sl > 0.0f) {
+        f[0] = ql[1];
+        f[1] = ql[1] * wln + pl;
+        f[2] = ql[1] * wlt;
+        f[3] = wln * (ql[3] + pl);
+    } else if (sl <= 0.0f && sm > 0.0f) {
+        const float nom = sl - wln, id = frcp(sl - sm);
+        f[0] = ql[1] + sl * (wl0 * nom * id - ql[0]);
+        f[1] = (ql[1] * wln + pl) + sl * ((nom * ql[1] + ps - pl) * id - ql[1]);
+        f[2] = ql[1] * wlt + sl * (nom * ql[2] * id - ql[2]);
+        f[3] = wln * (ql[3] + pl) + sl * ((nom * ql[3] - pl * wln + ps * sm) * id - ql[3]);
+    } else if (sm <= 0.0f && sr >= 0.0f) {
+        const float nom = sr - wrn, id = frcp(sr - sm), pe = QUIRKS ? pl : pr;
+        f[0] = qr[1] + sr * (wr0 * nom * id - qr[0]);
+        f[1] = (qr[1] * wrn + pr) + sr * ((nom * qr[1] + ps - pr) * id - qr[1]);
+        f[2] = qr[1] * wrt + sr * (nom * qr[2] * id - qr[2]);
+        f[3] = wrn * (qr[3] + pr) + sr * ((nom * qr[3] - pe * wrn + ps * sm) * id - qr[3]);
+    } else {
+        f[0] = qr[1];
+        f[1] = qr[1] * wrn + pr;
+        f[2] = qr[1] * wrt;
+        f[3] = wrn * (qr[3] + pr);
+    }
+}
+
+// characteristic Rusanov (Equations_Euler2D.cpp:221-246, 310-330) in FP32,
+// (rho, n, t, E) order
+HTX_HD void rusanov_char_f32(float gam, float g1, const Eig32& E, const float ql[4], const float qr[4], float f[4]) {
+    const float il = frcp(ql[0]), ir = frcp(qr[0]);
+    const float wnl = ql[1] * il, wtl = ql[2] * il, wnr = qr[1] * ir, wtr = qr[2] * ir;
+    const float pl = (ql[3] - 0.5f * (ql[1] * wnl + ql[2] * wtl)) * g1;
+    const float pr = (qr[3] - 0.5f * (qr[1] * wnr + qr[2] * wtr)) * g1;
+    const float cl = fsqrt(fabsf(gam * pl * il)), cr = fsqrt(fabsf(gam * pr * ir));
+    const float a0 = fmaxf(fabsf(wnl - cl), fabsf(wnr - cr));
+    const float a1 = fmaxf(fabsf(wnl), fabsf(wnr));
+    const float a3 = fmaxf(fabsf(wnl + cl), fabsf(wnr + cr));
+    float d0, d1, d2, d3;
+    eig_left_nt_f32(E, qr[0] - ql[0], qr[1] - ql[1], qr[2] - ql[2], qr[3] - ql[3], d0, d1, d2, d3);
+    d0 *= a0;
+    d1 *= a1;
+    d2 *= a1;
+    d3 *= a3;
+    float r0, rn, rt, r3;
+    eig_right_f32(E, d0, d1, d2, d3, r0, rn, rt, r3);
+    f[0] = 0.5f * ((ql[1] + qr[1]) - r0);
+    f[1] = 0.5f * (((ql[1] * wnl + pl) + (qr[1] * wnr + pr)) - rn);
+    f[2] = 0.5f * ((ql[1] * wtl + qr[1] * wtr) - rt);
+    f[3] = 0.5f * ((wnl * (ql[3] + pl) + wnr * (qr[3] + pr)) - r3);
 }
 
 // Euler2D::maxSpeed (Equations_Euler2D.cpp:207-219); QUIRKS drops the fabs

@@ -18,7 +18,7 @@ ERRORS = {-1: "HTX_EINVAL", -2: "HTX_EDEVICE", -3: "HTX_ENOMEM", -4: "HTX_ESTATE
 SCHEMES = {"teno5": 0, "hteno": 1, "teno5i": 2, "weno5js": 3, "weno5z": 4}
 FLUXES = {"rusanov": 0, "rusanov_char": 0, "hllc": 1}
 BCS = {"zero_grad": 0, "zerograd": 0, "periodic": 1}
-PRECISIONS = {"fp64": 0, "fp32_mixed": 1, "learned": 2, "fp64_fast": 3}
+PRECISIONS = {"fp64": 0, "fp32_mixed": 1, "learned": 2, "fp64_fast": 3, "fp32": 4, "learned_fp32": 5}
 
 
 class HtxParams(ctypes.Structure):

@@ -45,7 +45,7 @@ class System2D:
         self.gamma = p.gamma
         self.tFinal = p.tmax
         self.exact = None
-        if precision == "learned":
+        if precision in ("learned", "learned_fp32"):
             from .learn import default_classifier
 
             prm, thr = classifier if classifier is not None else default_classifier()

@@ -33,13 +33,14 @@ def run(w, nx, ny, precision, tend, flux="hllc"):
     steps = s.timeStepping(tend)
     s.sync()
     el = time.perf_counter() - t0
-    stats = s.classifier_stats() if precision == "learned" else None
+    stats = s.classifier_stats() if precision.startswith("learned") else None
     q = s.get_state()
     s.close()
     return q, el, steps, stats
 
 
-def matched_accuracy(w, nx, ny, factor=4, precisions=("fp64", "fp32_mixed", "learned"), tend=None, flux="hllc"):
+def matched_accuracy(w, nx, ny, factor=4, precisions=("fp64", "fp32_mixed", "fp32", "learned", "learned_fp32"),
+                     tend=None, flux="hllc"):
     tend = w["tmax"] if tend is None else tend
     qf, elf, stf, _ = run(w, factor * nx, factor * ny, "fp64_fast", tend, flux)
     ref = block_average(qf, factor)
@@ -69,14 +70,14 @@ def throughput(w, nx, ny, precision, steps=20, warmup=2, flux="hllc"):
     s.init_workload(w)
     s.steps_async(warmup)
     s.sync()
-    if precision == "learned":
+    if precision.startswith("learned"):
         s.classifier_stats(reset=True)
     t0 = time.perf_counter()
     s.steps_async(steps)
     s.sync()
     el = time.perf_counter() - t0
     r = {"value": nx * ny * steps / el, "unit": "cell-updates/s", "steps": steps}
-    if precision == "learned":
+    if precision.startswith("learned"):
         f, g, l = s.classifier_stats()
         r["flagged_fraction"] = g / max(f, 1)
         r["linear_fraction"] = l / max(f, 1)
@@ -84,7 +85,7 @@ def throughput(w, nx, ny, precision, steps=20, warmup=2, flux="hllc"):
     return r
 
 
-def precision_report(w, nx, ny, precisions=("fp32_mixed", "learned"), tend=None, flux="hllc"):
+def precision_report(w, nx, ny, precisions=("fp32_mixed", "fp32", "learned", "learned_fp32"), tend=None, flux="hllc"):
     """Error of each reduced precision against the plain FP64 GPU solution of
     the same run (same grid, same end time): per conserved variable the
     relative L1 (sum |q - q64| / sum |q64|) and the relative Linf

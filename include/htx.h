@@ -72,8 +72,25 @@ enum htx_bc { HTX_BC_ZERO_GRAD = 0, HTX_BC_PERIODIC = 1 };
  *   (Lagrange::linear), guarded by an FP64 positivity check, and the rest
  *   to TENO5; the classifier runs once per step, on the state the step
  *   starts from, and its flags serve all three stages; needs
- *   htx_load_classifier. */
-enum htx_precision { HTX_PREC_FP64 = 0, HTX_PREC_FP32_MIXED = 1, HTX_PREC_LEARNED = 2, HTX_PREC_FP64_FAST = 3 };
+ *   htx_load_classifier.
+ * HTX_PREC_FP32 (TENO5): FP32 bulk arithmetic.  The FP32-mixed cut-off
+ *   decisions; the characteristic face values stored in FP32; the transverse
+ *   quadrature, back-projection and HLLC / Rusanov flux in FP32; the Roe
+ *   eigensystem and characteristic projection, the flux divergence, the RK
+ *   accumulation and the CFL reduction in FP64.  A face whose FP32
+ *   Gauss-point states have a density or pressure that is not positive or
+ *   too small to be resolved in FP32 has its back-projection and flux
+ *   recomputed in FP64.
+ * HTX_PREC_LEARNED_FP32: HTX_PREC_LEARNED on the FP32 bulk arithmetic of
+ *   HTX_PREC_FP32 (needs htx_load_classifier). */
+enum htx_precision {
+    HTX_PREC_FP64 = 0,
+    HTX_PREC_FP32_MIXED = 1,
+    HTX_PREC_LEARNED = 2,
+    HTX_PREC_FP64_FAST = 3,
+    HTX_PREC_FP32 = 4,
+    HTX_PREC_LEARNED_FP32 = 5
+};
 
 typedef struct htx_params {
     int struct_size;    /* sizeof(htx_params), for ABI checks */

@@ -14,9 +14,9 @@ using namespace htx;
 // ph_flags_load from face-flag arrays in the kernel's layout (fx: x faces
 // i = -1 .. nx of each row at [j (nx + 2) + i + 1]; fy: y faces j = -1 .. ny
 // of each column at [(j + 1) nx + i]); null arrays flag every face smooth.
-template <int TX, int TY, bool MIXED = false, bool LEARNED = false>
+template <int TX, int TY, bool MIXED = false, bool LEARNED = false, typename FT = double>
 static void run(Fv2dArgs a, int flux, const unsigned char* fx = nullptr, const unsigned char* fy = nullptr) {
-    using T = Fv2dTile<TX, TY>;
+    using T = Fv2dTile<TX, TY, FT>;
     T* s = new T();
     a.tiles_x = (a.nx + TX - 1) / TX;
     a.tiles_y = (a.ny + TY - 1) / TY;
@@ -50,7 +50,8 @@ static void run(Fv2dArgs a, int flux, const unsigned char* fx = nullptr, const u
     delete s;
 }
 
-// mixed: 0 FP64, 1 FP32-mixed, 2 learned with every face flagged smooth
+// mixed: 0 FP64, 1 FP32-mixed, 2 learned with every face flagged smooth,
+// 3 FP32 bulk, 4 learned on the FP32 bulk arithmetic, every face flagged
 extern "C" int fv2dh_rhs_prec(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc,
                               int flux, int quirks, int mixed, const double* q, double* rhs) {
     Fv2dArgs a{};
@@ -71,7 +72,11 @@ extern "C" int fv2dh_rhs_prec(int nx, int ny, double xl, double xr, double yb, d
     a.area = a.dx * a.dy;
     fill_quad_consts(a.qc);
     int fk = flux == 0 ? 0 : (quirks ? 1 : 2);
-    if (mixed == 2)
+    if (mixed == 4)
+        run<16, 16, true, true, float>(a, fk);
+    else if (mixed == 3)
+        run<16, 16, true, false, float>(a, fk);
+    else if (mixed == 2)
         run<16, 16, true, true>(a, fk);
     else if (mixed)
         run<16, 16, true>(a, fk);
