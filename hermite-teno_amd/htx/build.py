@@ -6,6 +6,9 @@ Everything builds in-tree with plain hipcc / gcc command lines:
   oracle/liboracle.so                the FP64 checker (-O2, no FP contraction)
   oracle/liboracle_omp.so            the same source, OpenMP, -O3, for the
                                      CPU baseline timing
+  oracle/liboracle_count.so          the same source as C++ with an
+                                     op-counting double (oracle/flop_count.h),
+                                     for the flop count of the path
   tests/host/libfv2d_host.so         CPU build of the stage kernel's tiling
                                      (test infrastructure)
   tests/host/libbatch_host.so        CPU walk of the batched launches'
@@ -23,6 +26,7 @@ ORACLE = os.path.join(ROOT, "oracle")
 LIBHTX = os.path.join(PKG_DIR, "libhtx.so")
 LIBORACLE = os.path.join(ORACLE, "liboracle.so")
 LIBORACLE_OMP = os.path.join(ORACLE, "liboracle_omp.so")
+LIBORACLE_COUNT = os.path.join(ORACLE, "liboracle_count.so")
 
 HIP_SOURCES = ["htx_fv2d.hip", "htx_fvh2d.hip", "htx_strict.hip", "htx_ic.hip", "htx_learn.hip", "htx_batch.hip",
                "htx_api.hip", "htx_noalloc.hip", "htx_classes.hip"]
@@ -90,6 +94,8 @@ def build_oracle():
     _run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-std=c11", "-o", LIBORACLE, src, "-lm"])
     _run(["gcc", "-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared", "-std=c11", "-o", LIBORACLE_OMP, src,
           "-lm"])
+    _run(["g++", "-O1", "-ffp-contract=off", "-fPIC", "-shared", "-std=c++17", "-include",
+          os.path.join(ORACLE, "flop_count.h"), "-x", "c++", "-o", LIBORACLE_COUNT, src])
     return LIBORACLE
 
 
