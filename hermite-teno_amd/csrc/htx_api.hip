@@ -368,6 +368,11 @@ int htx_create(const htx_params* p, htx_solver** out) {
     // cfl min(dx, dy) / max speed with dy = dx
     h->dy = p->dim == 1 ? h->dx : (p->yt - p->yb) / p->ny;
     fill_quad_consts(h->qc);
+    if (!check_quad_consts()) {
+        delete h;
+        return fail(HTX_EDEVICE, "the kernels' compile-time quadrature table differs from the host's sqrt(15) "
+                                 "expressions (host math library or compiler rounding)");
+    }
     fill_herm_consts(h->hc, h->dx);  // Hermite::setGridSize(dx), FVHermite_System2D.cpp:28
     size_t bytes = sizeof(double) * (size_t)h->fstride * h->nfield;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {

@@ -281,14 +281,14 @@ HTX_HD void ph_flux32(const Fv2dArgs& a, Fv2dTile<TX, TY, float>& s, int tid) {
                 const float* L = s.sF[2 * f];
                 const float* R = s.sF[2 * f + 1];
                 if (linear) {
-                    weno3q_linear_f32(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0],
+                    weno3q_linear_f32(kQuad, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0],
                                       gl[f][1], gl[f][2]);
-                    weno3q_linear_f32(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0],
+                    weno3q_linear_f32(kQuad, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0],
                                       gr[f][1], gr[f][2]);
                 } else {
-                    weno3q_f32(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
+                    weno3q_f32(kQuad, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
                                gl[f][2]);
-                    weno3q_f32(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
+                    weno3q_f32(kQuad, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
                                gr[f][2]);
                 }
             }
@@ -375,9 +375,9 @@ HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int tid) {
                 for (int f = 0; f < 4; ++f) {
                     const double* L = s.sF[2 * f];
                     const double* R = s.sF[2 * f + 1];
-                    weno3q_linear(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
+                    weno3q_linear(kQuad, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
                                   gl[f][2]);
-                    weno3q_linear(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
+                    weno3q_linear(kQuad, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
                                   gr[f][2]);
                 }
                 linq = true;
@@ -394,14 +394,14 @@ HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int tid) {
                     const double* L = s.sF[2 * f];
                     const double* R = s.sF[2 * f + 1];
                     if (MIXED) {
-                        weno3q_mixed(a.qc, a.qc.lw32, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS],
+                        weno3q_mixed(kQuad, kQuad.lw32, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS],
                                      gl[f][0], gl[f][1], gl[f][2]);
-                        weno3q_mixed(a.qc, a.qc.lw32, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS],
+                        weno3q_mixed(kQuad, kQuad.lw32, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS],
                                      gr[f][0], gr[f][1], gr[f][2]);
                     } else {
-                        weno3q(a.qc, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
+                        weno3q(kQuad, L[b], L[b + TS], L[b + 2 * TS], L[b + 3 * TS], L[b + 4 * TS], gl[f][0], gl[f][1],
                                gl[f][2]);
-                        weno3q(a.qc, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
+                        weno3q(kQuad, R[b], R[b + TS], R[b + 2 * TS], R[b + 3 * TS], R[b + 4 * TS], gr[f][0], gr[f][1],
                                gr[f][2]);
                     }
                 }

@@ -747,7 +747,7 @@ HTX_HD void fvh_flux_face(const FvhArgs& a, int i, int j, const double* F, long 
     double c[3][6][4];
     for (int s = 0; s < 24; ++s) {
         double g[3];
-        fvh_series_gp3<MIXED>(a.qc, F, fstr, s, fid, g);
+        fvh_series_gp3<MIXED>(kQuad, F, fstr, s, fid, g);
         for (int m = 0; m < 3; ++m) c[m][s / 4][s % 4] = g[m];
     }
     for (int m = 0; m < 3; ++m) fvh_flux_at<YDIR, FLUX>(a, i, j, m, c[m], out[m]);
@@ -781,7 +781,7 @@ __device__ void fvh_flux_block(const FvhArgs& a, long long t0, long long nf, dou
         for (int q = 0; q < 8; ++q) {
             const int s = 8 * m + q;
             double g[3];
-            fvh_series_gp3<MIXED>(a.qc, F, nf, s, fid, g);
+            fvh_series_gp3<MIXED>(kQuad, F, nf, s, fid, g);
 #pragma unroll
             for (int p = 0; p < 3; ++p) lds[(s * 3 + p) * 64 + x] = g[p];
         }

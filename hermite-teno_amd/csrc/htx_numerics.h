@@ -9,6 +9,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 // The kernels are compiled twice into libhtx.so: once as the fast FP64 /
 // mixed-precision build (namespace htx) and once as the strict IEEE build
 // (htx_strict.hip: no contraction, division by division), each in its own
@@ -39,9 +41,13 @@
 
 namespace HTX_NS {
 
-// Runtime-computed coefficients of Schemes::Lagrange that involve sqrt(15)
-// (Schemes_Lagrange.cpp:64-107).  Filled on the host with the reference's own
-// expressions and passed to kernels by value.
+// Coefficients of Schemes::Lagrange that involve sqrt(15)
+// (Schemes_Lagrange.cpp:64-107), formed with the reference's own expressions
+// from sqrt(15).  The kernels read the compile-time copy kQuad (literal
+// operands the compiler rematerialises, instead of kernel arguments held in
+// scalar registers across the whole tile); the host forms the same table at
+// run time from sqrt(15.0) and the library refuses to start if the two differ
+// in any bit (check_quad_consts).
 struct QuadConsts {
     double gp[3][3];   // crjgp3L(r, k)
     double gm[3][3];   // crjgm3L(r, k)
@@ -57,9 +63,9 @@ struct QuadConsts {
     float gp32[3][3], gm32[3][3], lq32[3][5], sumw0p32, sumw0m32;
 };
 
-inline void fill_quad_consts(QuadConsts& C) {
-    // Schemes_Lagrange.cpp:64-105, same expressions, evaluated on the host.
-    double sq15 = sqrt(15.0);
+constexpr QuadConsts make_quad_consts(double sq15) {
+    // Schemes_Lagrange.cpp:64-105, same expressions
+    QuadConsts C{};
     C.gp[0][0] = 1.0 / 30.0 + sq15 / 20.0;
     C.gp[1][0] = -1.0 / 15.0 - sq15 / 5.0;
     C.gp[2][0] = 31.0 / 30.0 + 3.0 * sq15 / 20.0;
@@ -78,10 +84,10 @@ inline void fill_quad_consts(QuadConsts& C) {
     C.gm[0][2] = 31.0 / 30.0 + 3.0 * sq15 / 20.0;
     C.gm[1][2] = -1.0 / 15.0 - sq15 / 5.0;
     C.gm[2][2] = 1.0 / 30.0 + sq15 / 20.0;
-    double lin0[3] = {-9.0 / 80.0, 49.0 / 40.0, -9.0 / 80.0};
-    double p[3], m[3];
+    const double lin0[3] = {-9.0 / 80.0, 49.0 / 40.0, -9.0 / 80.0};
+    double p[3] = {0.0, 0.0, 0.0}, m[3] = {0.0, 0.0, 0.0};
     for (int i = 0; i < 3; ++i) {
-        p[i] = 0.5 * (lin0[i] + 3.0 * fabs(lin0[i]));
+        p[i] = 0.5 * (lin0[i] + 3.0 * (lin0[i] < 0.0 ? -lin0[i] : lin0[i]));
         m[i] = p[i] - lin0[i];
     }
     C.sumw0p = (p[0] + p[1]) + p[2];
@@ -127,6 +133,26 @@ inline void fill_quad_consts(QuadConsts& C) {
         for (int i = 0; i < 5; ++i) C.lq32[m][i] = (float)C.lq[m][i];
     C.sumw0p32 = (float)C.sumw0p;
     C.sumw0m32 = (float)C.sumw0m;
+    return C;
+}
+
+// sqrt(15) correctly rounded (IEEE sqrt is, so the host's sqrt(15.0) equals it)
+constexpr double kSqrt15 = 0x1.efbdeb14f4edap+1;
+constexpr QuadConsts kQuad = make_quad_consts(kSqrt15);
+
+inline void fill_quad_consts(QuadConsts& C) { C = make_quad_consts(sqrt(15.0)); }
+
+// true when the run-time table equals the compile-time one in every bit
+inline bool check_quad_consts() {
+    QuadConsts C;
+    fill_quad_consts(C);
+    const unsigned char* a = reinterpret_cast<const unsigned char*>(&C);
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(&kQuad);
+    // the members are contiguous; the tail padding is not compared
+    static_assert(offsetof(QuadConsts, sumw0m32) + sizeof(float) + 4 == sizeof(QuadConsts), "layout");
+    for (unsigned k = 0; k < offsetof(QuadConsts, sumw0m32) + sizeof(float); ++k)
+        if (a[k] != b[k]) return false;
+    return true;
 }
 
 // Fast FP32 reciprocal and base-2 logarithm on the device (one hardware
