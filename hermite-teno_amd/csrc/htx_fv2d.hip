@@ -23,10 +23,15 @@ namespace HTX_NS {
 
 // BULK32: the FP32 bulk precision (face values stored in FP32, quadrature,
 // back-projection and flux in FP32): a smaller tile in LDS and fewer
-// registers, three workgroups per CU
+// registers, three workgroups per CU.
+// Tiles that form the cells' Roe inputs per face (tile_prim_on_the_fly) also
+// fit three per CU.
+
 template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED, bool BULK32 = false>
-__global__ void __launch_bounds__(256, BULK32 ? 3 : 2) fv2d_stage_kernel(Fv2dArgs a) {
-    using T = Fv2dTile<TX, TY, typename std::conditional<BULK32, float, double>::type>;
+__global__ void __launch_bounds__(256, (BULK32 || tile_prim_on_the_fly<MIXED, LEARNED, BULK32>()) ? 3 : 2)
+    fv2d_stage_kernel(Fv2dArgs a) {
+    using T = Fv2dTile<TX, TY, typename std::conditional<BULK32, float, double>::type,
+                       tile_prim_on_the_fly<MIXED, LEARNED, BULK32>()>;
     __shared__ T s;
     double dt = 0.0;
     if (a.mode > 0) {

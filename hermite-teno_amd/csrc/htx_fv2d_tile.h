@@ -15,7 +15,7 @@ namespace HTX_NS {
 
 // FT: storage type of the characteristic face values (double; float in the
 // FP32 bulk precision, which halves their LDS footprint)
-template <int TX, int TY, typename FT = double>
+template <int TX, int TY, typename FT = double, bool OF = false>
 struct Fv2dTile {
     static constexpr int NT = 256;
     static constexpr int HX = TX + 6, HY = TY + 6, HC = HX * HY;
@@ -28,7 +28,10 @@ struct Fv2dTile {
     static constexpr int NC = TX * TY;
     static constexpr int KC = (NC + NT - 1) / NT;
     double sQ[4][HC];  // conserved state
-    double sP[4][HC];  // sqrt|rho|, u, v, H
+    // sqrt|rho|, u, v, H of each cell; with OF the tile keeps none and forms
+    // them from sQ where a face needs them (a third less LDS: three
+    // workgroups per CU instead of two, for a few more operations per face)
+    double sP[OF ? 1 : 4][OF ? 1 : HC];
     FT sF[8][NF];      // characteristic face values (left/right per field)
     double sFl[4][NQ]; // face fluxes
     double sRed[2][NT / 64];
@@ -36,11 +39,21 @@ struct Fv2dTile {
     unsigned int sCnt[3][NT / 64];
 };
 
+// Which stage kernels keep no per-cell Roe inputs in LDS: the strict build
+// and the learned FP64 path, whose registers fit the three-workgroup budget
+// (168 VGPRs) that the smaller tile opens.  The fp64_fast and fp32_mixed
+// kernels need more registers than that and keep the stored inputs at two
+// workgroups per CU.
+template <bool MIXED, bool LEARNED, bool BULK32>
+constexpr bool tile_prim_on_the_fly() {
+    return HTX_EXACT_DIVISION || (LEARNED && !BULK32);
+}
+
 // halo load with the boundary condition applied through the index, plus the
 // per-cell Roe inputs of calcPrimitiveVariable (FV_System2D.cpp:226-263)
-template <int TX, int TY, typename FT>
-HTX_HD void ph_load(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int i0, int j0, int tid) {
-    using T = Fv2dTile<TX, TY, FT>;
+template <int TX, int TY, typename FT, bool OF>
+HTX_HD void ph_load(const Fv2dArgs& a, Fv2dTile<TX, TY, FT, OF>& s, int i0, int j0, int tid) {
+    using T = Fv2dTile<TX, TY, FT, OF>;
     const long long fs = a.fstride;
     for (int w = tid; w < T::HC; w += T::NT) {
         int lx = w % T::HX, ly = w / T::HX;
@@ -52,17 +65,22 @@ HTX_HD void ph_load(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int i0, int j0, 
         s.sQ[1][w] = q1;
         s.sQ[2][w] = q2;
         s.sQ[3][w] = q3;
-        Prim P = cell_prim(a.g1, q0, q1, q2, q3);
-        s.sP[0][w] = P.sr;
-        s.sP[1][w] = P.u;
-        s.sP[2][w] = P.v;
-        s.sP[3][w] = P.h;
+        if constexpr (!OF) {
+            Prim P = cell_prim(a.g1, q0, q1, q2, q3);
+            s.sP[0][w] = P.sr;
+            s.sP[1][w] = P.u;
+            s.sP[2][w] = P.v;
+            s.sP[3][w] = P.h;
+        }
     }
 }
 
-template <int TX, int TY, typename FT>
-HTX_HD Prim prim_at(const Fv2dTile<TX, TY, FT>& s, int id) {
-    return Prim{s.sP[0][id], s.sP[1][id], s.sP[2][id], s.sP[3][id]};
+template <int TX, int TY, typename FT, bool OF>
+HTX_HD Prim prim_at(const Fv2dTile<TX, TY, FT, OF>& s, int id, double g1) {
+    if constexpr (OF)
+        return cell_prim(g1, s.sQ[0][id], s.sQ[1][id], s.sQ[2][id], s.sQ[3][id]);
+    else
+        return Prim{s.sP[0][id], s.sP[1][id], s.sP[2][id], s.sP[3][id]};
 }
 
 // Learned precision: the classifier flag of face g (cells g | g+1 along a
@@ -83,9 +101,9 @@ HTX_HD unsigned char line_flag(const unsigned char* line, int g, int n, int bc, 
 // Learned precision: the flags of the tile's recon faces, from the face-flag
 // arrays of the step (rows of x faces and columns of y faces outside the grid
 // map like the cells their stencils read).
-template <int TX, int TY, typename FT>
-HTX_HD void ph_flags_load(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int i0, int j0, int tid) {
-    using T = Fv2dTile<TX, TY, FT>;
+template <int TX, int TY, typename FT, bool OF>
+HTX_HD void ph_flags_load(const Fv2dArgs& a, Fv2dTile<TX, TY, FT, OF>& s, int i0, int j0, int tid) {
+    using T = Fv2dTile<TX, TY, FT, OF>;
     for (int w = tid; w < T::NXF; w += T::NT) {
         int gi = i0 - 1 + w % (TX + 1), gj = map_index(j0 - 2 + w / (TX + 1), a.ny, a.bc);
         s.sFlag[0][w] = line_flag(a.face_flags_x + (long long)gj * (a.nx + 2), gi, a.nx, a.bc, 1);
@@ -123,17 +141,17 @@ HTX_HD bool recon_face_owned(int fi, int fj, int i0, int j0, int nx, int ny) {
 // have positive density and pressure on both sides (checked in FP64; else
 // TENO5); the face's flag is then overwritten with the scheme it took (1 =
 // linear), and cnt accumulates (owned faces, flagged, linear) of this thread.
-template <int TX, int TY, bool YDIR, bool MIXED = false, bool LEARNED = false, typename FT = double>
-HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int tid, int i0 = 0, int j0 = 0,
+template <int TX, int TY, bool YDIR, bool MIXED = false, bool LEARNED = false, typename FT = double, bool OF = false>
+HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY, FT, OF>& s, int tid, int i0 = 0, int j0 = 0,
                      unsigned* cnt = nullptr) {
-    using T = Fv2dTile<TX, TY, FT>;
+    using T = Fv2dTile<TX, TY, FT, OF>;
     constexpr int NW = YDIR ? T::NYF : T::NXF;
     constexpr int RW = YDIR ? (TX + 4) : (TX + 1);   // faces per row of the face grid
     constexpr int STEP = YDIR ? T::HX : 1;           // LDS stride along the normal
     for (int w = tid; w < NW; w += T::NT) {
         int fi = w % RW, fj = w / RW;
         int ca = YDIR ? (2 + fj) * T::HX + (1 + fi) : (1 + fj) * T::HX + (2 + fi);
-        Eig E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
+        Eig E = roe_eig(a.g1, YDIR, prim_at(s, ca, a.g1), prim_at(s, ca + STEP, a.g1));
         bool lin = false;
         if (LEARNED) {
             const bool flag = s.sFlag[YDIR ? 1 : 0][w] != 0;
@@ -202,8 +220,8 @@ HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int tid, int i0
 // or column (y flux): their 6-cell stencils cover the window's transverse
 // extent, so a discontinuity running along the face normal through the
 // window is seen there.
-template <int TX, int TY, bool YDIR, typename FT>
-HTX_HD bool flux_window_smooth(const Fv2dTile<TX, TY, FT>& s, int fi, int fj) {
+template <int TX, int TY, bool YDIR, typename FT, bool OF>
+HTX_HD bool flux_window_smooth(const Fv2dTile<TX, TY, FT, OF>& s, int fi, int fj) {
     if (!YDIR) {
         const unsigned char* N = s.sFlag[0];
         const int b = fj * (TX + 1) + fi;
@@ -224,9 +242,9 @@ HTX_HD bool flux_window_smooth(const Fv2dTile<TX, TY, FT>& s, int fi, int fj) {
 // item w = tid + k NT of ph_flux) has a smooth window.  On the device a
 // wavefront vote over its active lanes; on the host, where a phase runs item
 // by item, the same vote taken over the items of that wavefront.
-template <int TX, int TY, bool YDIR, typename FT>
-HTX_HD bool window_smooth_wave(const Fv2dTile<TX, TY, FT>& s, int w, int tid) {
-    using T = Fv2dTile<TX, TY, FT>;
+template <int TX, int TY, bool YDIR, typename FT, bool OF>
+HTX_HD bool window_smooth_wave(const Fv2dTile<TX, TY, FT, OF>& s, int w, int tid) {
+    using T = Fv2dTile<TX, TY, FT, OF>;
     constexpr int NW = YDIR ? T::NYQ : T::NXQ;
     constexpr int RW = YDIR ? TX : (TX + 1);
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -258,9 +276,9 @@ HTX_HD bool state_fp32_ok(const float q[4], float c0, float c2, float c3) {
            re - 0.5f * (q[1] * q[1] + q[2] * q[2]) > 0x1p-10f * re;
 }
 
-template <int TX, int TY, bool YDIR, int FLUX, bool LEARNED>
-HTX_HD void ph_flux32(const Fv2dArgs& a, Fv2dTile<TX, TY, float>& s, int tid) {
-    using T = Fv2dTile<TX, TY, float>;
+template <int TX, int TY, bool YDIR, int FLUX, bool LEARNED, bool OF>
+HTX_HD void ph_flux32(const Fv2dArgs& a, Fv2dTile<TX, TY, float, OF>& s, int tid) {
+    using T = Fv2dTile<TX, TY, float, OF>;
     constexpr int NW = YDIR ? T::NYQ : T::NXQ;
     constexpr int RW = YDIR ? TX : (TX + 1);
     constexpr int RR = YDIR ? (TX + 4) : (TX + 1);
@@ -271,7 +289,7 @@ HTX_HD void ph_flux32(const Fv2dArgs& a, Fv2dTile<TX, TY, float>& s, int tid) {
         const int fi = w % RW, fj = w / RW;
         const int b = fj * RR + fi;
         const int ca = YDIR ? (2 + fj) * T::HX + (3 + fi) : (3 + fj) * T::HX + (2 + fi);
-        const Eig E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
+        const Eig E = roe_eig(a.g1, YDIR, prim_at(s, ca, a.g1), prim_at(s, ca + STEP, a.g1));
         const Eig32 E32 = eig32(E);
         float gl[4][3], gr[4][3];
         bool lin = LEARNED && window_smooth_wave<TX, TY, YDIR>(s, w, tid);
@@ -348,12 +366,12 @@ HTX_HD void ph_flux32(const Fv2dArgs& a, Fv2dTile<TX, TY, float>& s, int tid) {
 // non-positive density or pressure (checked in FP64); every other face takes
 // the nonlinear quadrature.  The wavefront-wide decision keeps the two paths
 // from running one after the other in the same wavefront.
-template <int TX, int TY, bool YDIR, int FLUX, bool MIXED = false, bool LEARNED = false, typename FT = double>
-HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int tid) {
+template <int TX, int TY, bool YDIR, int FLUX, bool MIXED = false, bool LEARNED = false, typename FT = double, bool OF = false>
+HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY, FT, OF>& s, int tid) {
     if constexpr (sizeof(FT) == sizeof(float)) {
         ph_flux32<TX, TY, YDIR, FLUX, LEARNED>(a, s, tid);
     } else {
-        using T = Fv2dTile<TX, TY, FT>;
+        using T = Fv2dTile<TX, TY, FT, OF>;
         constexpr int NW = YDIR ? T::NYQ : T::NXQ;
         constexpr int RW = YDIR ? TX : (TX + 1);          // flux faces per row
         constexpr int RR = YDIR ? (TX + 4) : (TX + 1);    // recon faces per row
@@ -367,7 +385,7 @@ HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int tid) {
             // the learned path needs the eigensystem for its positivity check;
             // the others form it after the quadrature (fewer live registers)
             Eig E;
-            if (LEARNED) E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
+            if (LEARNED) E = roe_eig(a.g1, YDIR, prim_at(s, ca, a.g1), prim_at(s, ca + STEP, a.g1));
             double gl[4][3], gr[4][3];
             bool linq = false;
             if (LEARNED && window_smooth_wave<TX, TY, YDIR>(s, w, tid)) {
@@ -406,7 +424,7 @@ HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int tid) {
                     }
                 }
             }
-            if (!LEARNED) E = roe_eig(a.g1, YDIR, prim_at(s, ca), prim_at(s, ca + STEP));
+            if (!LEARNED) E = roe_eig(a.g1, YDIR, prim_at(s, ca, a.g1), prim_at(s, ca + STEP, a.g1));
             double acc[3][4];
     #pragma unroll
             for (int m = 0; m < 3; ++m) {
@@ -432,9 +450,9 @@ HTX_HD void ph_flux(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int tid) {
 }
 
 // x part of the divergence, (fx_{i-1} - fx_i) * dy, kept per thread
-template <int TX, int TY, typename FT>
-HTX_HD void ph_xdiv(const Fv2dArgs& a, const Fv2dTile<TX, TY, FT>& s, int tid, double dxp[][4]) {
-    using T = Fv2dTile<TX, TY, FT>;
+template <int TX, int TY, typename FT, bool OF>
+HTX_HD void ph_xdiv(const Fv2dArgs& a, const Fv2dTile<TX, TY, FT, OF>& s, int tid, double dxp[][4]) {
+    using T = Fv2dTile<TX, TY, FT, OF>;
 #pragma unroll
     for (int k = 0; k < T::KC; ++k) {
         int w = tid + k * T::NT;
@@ -450,10 +468,10 @@ HTX_HD void ph_xdiv(const Fv2dArgs& a, const Fv2dTile<TX, TY, FT>& s, int tid, d
 // residual (FV_System2D.cpp:562-572), SSP-RK3 stage (:575-612 with
 // Schemes_SSPRK.cpp:3-15) and, in stage 3, the per-thread CFL maxima
 // (:623-631 with Equations_Euler2D.cpp:207-219)
-template <int TX, int TY, typename FT>
-HTX_HD void ph_epilogue(const Fv2dArgs& a, const Fv2dTile<TX, TY, FT>& s, int i0, int j0, int tid,
+template <int TX, int TY, typename FT, bool OF>
+HTX_HD void ph_epilogue(const Fv2dArgs& a, const Fv2dTile<TX, TY, FT, OF>& s, int i0, int j0, int tid,
                         const double dxp[][4], double dt, double& smx, double& smy, bool* bad = nullptr) {
-    using T = Fv2dTile<TX, TY, FT>;
+    using T = Fv2dTile<TX, TY, FT, OF>;
     const long long fs = a.fstride;
 #pragma unroll
     for (int k = 0; k < T::KC; ++k) {
@@ -515,9 +533,9 @@ __device__ __forceinline__ double wave_max(double v) {
 // the tile's CFL maxima (one atomic per workgroup).  dt = the stage's time
 // step.  Ends with every thread past its last LDS access only for mode 3;
 // callers that reuse the tile storage synchronise before the next tile.
-template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED, typename FT>
-__device__ __forceinline__ void stage_tile(const Fv2dArgs& a, Fv2dTile<TX, TY, FT>& s, int tile, double dt, int tid) {
-    using T = Fv2dTile<TX, TY, FT>;
+template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED, typename FT, bool OF>
+__device__ __forceinline__ void stage_tile(const Fv2dArgs& a, Fv2dTile<TX, TY, FT, OF>& s, int tile, double dt, int tid) {
+    using T = Fv2dTile<TX, TY, FT, OF>;
     const int i0 = (tile % a.tiles_x) * TX;
     const int j0 = (tile / a.tiles_x) * TY;
     double dxp[T::KC][4];

@@ -16,7 +16,8 @@ using namespace htx;
 // of each column at [(j + 1) nx + i]); null arrays flag every face smooth.
 template <int TX, int TY, bool MIXED = false, bool LEARNED = false, typename FT = double>
 static void run(Fv2dArgs a, int flux, const unsigned char* fx = nullptr, const unsigned char* fy = nullptr) {
-    using T = Fv2dTile<TX, TY, FT>;
+    // the tile layout the product kernel of this precision uses
+    using T = Fv2dTile<TX, TY, FT, tile_prim_on_the_fly<MIXED, LEARNED, sizeof(FT) == sizeof(float)>()>;
     T* s = new T();
     a.tiles_x = (a.nx + TX - 1) / TX;
     a.tiles_y = (a.ny + TY - 1) / TY;
