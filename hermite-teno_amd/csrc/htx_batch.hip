@@ -22,8 +22,14 @@
 
 #include "../../include/htx.h"
 #include "htx_batch_tables.h"
+#include "htx_learn_api.h"
 
 int htx_internal_fail(int code, const std::string& msg);
+// the strict FP64 build's single-problem launchers (htx_strict.hip)
+hipError_t htx_strict_launch_fv2d_stage(const void* a, int flux, int tile_x, int tile_y, hipStream_t s);
+hipError_t htx_strict_launch_fvh2d_stage(const void* a, int flux, hipStream_t s);
+hipError_t htx_strict_launch_max_speed(const double* q, int nx, int ny, int pitch, long long fstride, double gam,
+                                       int quirks, void* ctl, hipStream_t s);
 
 namespace htx {
 
@@ -215,13 +221,21 @@ struct htx_batch {
     Ctl* ctl = nullptr;
     double* dDxdy = nullptr;
     // TENO5 launch
-    int teno_flux = FLUX_HLLC, teno_prec = 1;  // prec 0 fp64_fast, 1 fp32_mixed
+    int teno_flux = FLUX_HLLC, teno_prec = BP_FP32_MIXED;  // a BatchPrec code
     int shape = 0, ntiles = 0, nblk_teno = 0;
     Fv2dArgs* dTenoArgs = nullptr;
     int *dTileInst = nullptr, *dTile0 = nullptr;
+    // direct TENO5 launches (batch_prec_direct): one 16 x 16 stage launch per
+    // instance from these host argument blocks, and for the learned
+    // precisions one face-flag launch per instance and step
+    bool teno_direct = false;
+    std::vector<Fv2dArgs> teno_args;        // [mode - 1][TENO instance]
+    std::vector<unsigned char*> dFlagsX, dFlagsY;
     // Hermite launches
     int herm_flux = FLUX_RUSANOV_CHAR;
     bool herm_mixed = false;                // Hermite instances in fp32_mixed
+    bool herm_strict = false;               // Hermite instances in FP64 (strict build, one launch per instance)
+    std::vector<FvhArgs> herm_args;         // [mode - 1][Hermite instance], host copy
     FvhArgs* dHermArgs = nullptr;
     int* dHermMap = nullptr;
     int herm_blocks[6] = {0, 0, 0, 0, 0, 0};
@@ -247,12 +261,6 @@ namespace {
         if (e_ != hipSuccess) return htx_internal_fail(HTX_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-int prec_code(int precision) {
-    if (precision == HTX_PREC_FP64_FAST) return 0;
-    if (precision == HTX_PREC_FP32_MIXED) return 1;
-    if (precision == HTX_PREC_LEARNED) return 2;
-    return -1;
-}
 
 int flux_code(const htx_params& p) {
     if (p.flux == HTX_FLUX_RUSANOV_CHAR) return FLUX_RUSANOV_CHAR;
@@ -277,7 +285,15 @@ int build_teno_tables(htx_batch* b) {
     const int nt = (int)b->teno.size();
     if (nt == 0) return HTX_OK;
     BatchLayout L = layout_of(b);
-    TenoTables T = make_teno_tables(L, b->state, b->ctl, SHAPES[b->shape].tx, SHAPES[b->shape].ty, b->teno_prec);
+    const bool lrn = batch_prec_learned(b->teno_prec);
+    TenoTables T = make_teno_tables(L, b->state, b->ctl, SHAPES[b->shape].tx, SHAPES[b->shape].ty, b->teno_prec,
+                                    lrn ? b->dFlagsX.data() : nullptr, lrn ? b->dFlagsY.data() : nullptr);
+    if (b->teno_direct) {
+        // the single-problem launchers size their own grids from these blocks
+        b->teno_args = T.args;
+        b->ntiles = T.ntiles;
+        return HTX_OK;
+    }
     const int g = T.ntiles;
     b->ntiles = g;
     // the old tables may still be read by launches in flight
@@ -300,12 +316,26 @@ int build_teno_tables(htx_batch* b) {
     return HTX_OK;
 }
 
+// the learned precisions' face-flag arrays, one pair per TENO5 instance
+int alloc_teno_flags(htx_batch* b) {
+    const int nt = (int)b->teno.size();
+    b->dFlagsX.assign(nt, nullptr);
+    b->dFlagsY.assign(nt, nullptr);
+    for (int k = 0; k < nt; ++k) {
+        const htx_params& p = b->p[b->teno[k]];
+        BHIP(hipMalloc(&b->dFlagsX[k], lrn_flag_bytes_x(p.nx, p.ny)));
+        BHIP(hipMalloc(&b->dFlagsY[k], lrn_flag_bytes_y(p.nx, p.ny)));
+    }
+    return HTX_OK;
+}
+
 int build_herm_tables(htx_batch* b) {
     const int nh = (int)b->herm.size();
     if (nh == 0) return HTX_OK;
     BatchLayout L = layout_of(b);
     BHIP(hipMalloc(&b->herm_scratch, sizeof(double) * (size_t)herm_scratch_size(L)));
     HermTables H = make_herm_tables(L, b->state, b->ctl, b->herm_scratch);
+    b->herm_args = H.args;
     long long msize = 0;
     for (int k = 0; k < 12; ++k) {
         b->herm_map_off[k] = msize;
@@ -326,7 +356,35 @@ int enqueue_batch_step(htx_batch* b, hipStream_t s) {
     const int n = b->n;
     hipLaunchKernelGGL(k_batch_dt, dim3((n + 255) / 256), dim3(256), 0, s, b->ctl, b->dDxdy, n);
     for (int mode = 1; mode <= 3; ++mode) {
-        if (!b->teno.empty()) {
+        if (!b->teno.empty() && b->teno_direct) {
+            const int nt = (int)b->teno.size();
+            for (int k = 0; k < nt; ++k) {
+                const Fv2dArgs& a = b->teno_args[(size_t)(mode - 1) * nt + k];
+                if (a.learned && mode == 1) {
+                    // classify the faces of u^n once per step (nothing when the
+                    // instance is past its end time)
+                    LrnFlagsArgs f{};
+                    f.q = a.qin;
+                    f.nx = a.nx;
+                    f.ny = a.ny;
+                    f.pitch = a.pitch;
+                    f.bc = a.bc;
+                    f.fstride = a.fstride;
+                    f.gam = a.gam;
+                    f.frag = b->dFrag;
+                    f.cvec = b->dCvec;
+                    f.threshold = b->cls_thr;
+                    f.ctl = a.ctl;
+                    f.flags_x = b->dFlagsX[k];
+                    f.flags_y = b->dFlagsY[k];
+                    BHIP(lrn_launch_flags(f, s));
+                }
+                if (b->teno_prec == BP_FP64)
+                    BHIP(htx_strict_launch_fv2d_stage(&a, b->teno_flux, 16, 16, s));
+                else
+                    BHIP(launch_fv2d_stage(a, b->teno_flux, 16, 16, s));
+            }
+        } else if (!b->teno.empty()) {
             BatchTenoArgs T{};
             T.args = b->dTenoArgs;
             T.tile_inst = b->dTileInst;
@@ -336,7 +394,11 @@ int enqueue_batch_step(htx_batch* b, hipStream_t s) {
             T.mode = mode;
             BHIP(launch_teno(b->shape, T, b->teno_flux, b->teno_prec, b->nblk_teno, s));
         }
-        if (!b->herm.empty()) {
+        if (!b->herm.empty() && b->herm_strict) {
+            const int nh = (int)b->herm.size();
+            for (int k = 0; k < nh; ++k)
+                BHIP(htx_strict_launch_fvh2d_stage(&b->herm_args[(size_t)(mode - 1) * nh + k], b->herm_flux, s));
+        } else if (!b->herm.empty()) {
             BatchHermArgs H{};
             H.args = b->dHermArgs;
             for (int k = 0; k < 6; ++k) {
@@ -379,11 +441,17 @@ int capture_graph(htx_batch* b) {
     return HTX_OK;
 }
 
+// CFL maxima of a newly set state; FP64 instances take the strict build's
+// sound speed, as the single-problem solver does
 int refresh_instance_cfl(htx_batch* b, int i) {
     const htx_params& p = b->p[i];
     BHIP(launch_reset_smax(b->ctl + i, b->stream));
-    BHIP(launch_max_speed(inst_state(b, 0, i), p.nx, p.ny, p.nx, (long long)p.nx * p.ny, p.gamma,
-                          p.ref_quirks ? 1 : 0, b->ctl + i, b->stream));
+    if (p.precision == HTX_PREC_FP64)
+        BHIP(htx_strict_launch_max_speed(inst_state(b, 0, i), p.nx, p.ny, p.nx, (long long)p.nx * p.ny, p.gamma,
+                                         p.ref_quirks ? 1 : 0, b->ctl + i, b->stream));
+    else
+        BHIP(launch_max_speed(inst_state(b, 0, i), p.nx, p.ny, p.nx, (long long)p.nx * p.ny, p.gamma,
+                              p.ref_quirks ? 1 : 0, b->ctl + i, b->stream));
     return HTX_OK;
 }
 
@@ -431,21 +499,22 @@ int htx_batch_create(const htx_params* ps, int n, htx_batch** out) {
             (p.flux != HTX_FLUX_RUSANOV_CHAR && p.flux != HTX_FLUX_HLLC))
             return htx_internal_fail(HTX_EINVAL, "instance " + std::to_string(i) + ": invalid parameters");
         if (p.scheme == HTX_SCHEME_TENO5) {
-            int pc = prec_code(p.precision), fc = flux_code(p);
-            // The learned precision's persistent launch (several tiles per
-            // workgroup, and the 32x8 / 8x32 tile shapes) has faulted the GPU
-            // with an illegal address; it stays off until that is resolved.
-            if (pc < 0 || pc == 2)
-                return htx_internal_fail(HTX_ENOTSUP, "batched TENO5 runs fp64_fast or fp32_mixed precision");
-            if (fc == FLUX_HLLC_TORO) return htx_internal_fail(HTX_ENOTSUP, "batched HLLC keeps the reference quirks");
+            int pc = batch_prec_code(p.precision), fc = flux_code(p);
+            if (pc < 0) return htx_internal_fail(HTX_EINVAL, "instance " + std::to_string(i) + ": unknown precision");
+            // the persistent launch is built for the reference's HLLC and
+            // the characteristic Rusanov flux; the direct launches take all
+            if (fc == FLUX_HLLC_TORO && !batch_prec_direct(pc))
+                return htx_internal_fail(HTX_ENOTSUP, "batched fp64_fast / fp32_mixed HLLC keeps the reference quirks");
             if ((teno_prec >= 0 && pc != teno_prec) || (teno_flux >= 0 && fc != teno_flux))
                 return htx_internal_fail(HTX_ENOTSUP, "the TENO5 instances of a batch share flux and precision");
             teno_prec = pc;
             teno_flux = fc;
         } else if (p.scheme == HTX_SCHEME_HTENO) {
-            if (p.precision != HTX_PREC_FP64_FAST && p.precision != HTX_PREC_FP32_MIXED)
-                return htx_internal_fail(HTX_ENOTSUP, "batched Hermite instances run fp64_fast or fp32_mixed precision");
-            int fc = flux_code(p), pm = p.precision == HTX_PREC_FP32_MIXED;
+            if (p.precision != HTX_PREC_FP64 && p.precision != HTX_PREC_FP64_FAST &&
+                p.precision != HTX_PREC_FP32_MIXED)
+                return htx_internal_fail(HTX_ENOTSUP,
+                                         "batched Hermite instances run fp64, fp64_fast or fp32_mixed precision");
+            int fc = flux_code(p), pm = p.precision == HTX_PREC_FP32_MIXED ? 1 : (p.precision == HTX_PREC_FP64 ? 2 : 0);
             if ((herm_flux >= 0 && fc != herm_flux) || (herm_mixed >= 0 && pm != herm_mixed))
                 return htx_internal_fail(HTX_ENOTSUP, "the Hermite instances of a batch share flux and precision");
             herm_flux = fc;
@@ -461,8 +530,10 @@ int htx_batch_create(const htx_params* ps, int n, htx_batch** out) {
     b->p.assign(ps, ps + n);
     if (teno_prec >= 0) b->teno_prec = teno_prec;
     if (teno_flux >= 0) b->teno_flux = teno_flux;
+    b->teno_direct = batch_prec_direct(b->teno_prec);
     if (herm_flux >= 0) b->herm_flux = herm_flux;
     b->herm_mixed = herm_mixed == 1;
+    b->herm_strict = herm_mixed == 2;
     BatchLayout L = make_layout(ps, n);
     b->off = L.off;
     b->dxdy = L.dxdy;
@@ -493,7 +564,8 @@ int htx_batch_create(const htx_params* ps, int n, htx_batch** out) {
         htx_batch_destroy(b);
         return htx_internal_fail(HTX_EDEVICE, "batch initialisation failed");
     }
-    if ((rc = build_teno_tables(b)) || (rc = build_herm_tables(b))) {
+    if ((batch_prec_learned(b->teno_prec) && !b->teno.empty() && (rc = alloc_teno_flags(b))) ||
+        (rc = build_teno_tables(b)) || (rc = build_herm_tables(b))) {
         htx_batch_destroy(b);
         return rc;
     }
@@ -508,6 +580,10 @@ void htx_batch_destroy(htx_batch* b) {
     for (void* m : {(void*)b->state, (void*)b->ctl, (void*)b->dDxdy, (void*)b->dTenoArgs, (void*)b->dTileInst,
                     (void*)b->dTile0, (void*)b->dHermArgs, (void*)b->dHermMap,
                     (void*)b->herm_scratch, (void*)b->dFrag, (void*)b->dCvec})
+        if (m) (void)hipFree(m);
+    for (unsigned char* m : b->dFlagsX)
+        if (m) (void)hipFree(m);
+    for (unsigned char* m : b->dFlagsY)
         if (m) (void)hipFree(m);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -587,6 +663,8 @@ int htx_batch_load_classifier(htx_batch* b, const float* params, double threshol
 int htx_batch_steps(htx_batch* b, int nsteps) {
     if (!b) return htx_internal_fail(HTX_EINVAL, "null batch");
     if (nsteps < 0) return htx_internal_fail(HTX_EINVAL, "nsteps < 0");
+    if (!b->teno.empty() && batch_prec_learned(b->teno_prec) && !b->have_cls)
+        return htx_internal_fail(HTX_ESTATE, "the learned precision needs htx_batch_load_classifier");
     int rc;
     if (b->use_graph && !b->graph && (rc = capture_graph(b))) return rc;
     for (int k = 0; k < nsteps; ++k) {
@@ -653,6 +731,8 @@ int htx_batch_set_tile(htx_batch* b, int tile_x, int tile_y) {
     for (int s = 0; s < NSHAPES; ++s)
         if (SHAPES[s].tx == tile_x && SHAPES[s].ty == tile_y) k = s;
     if (k < 0) return htx_internal_fail(HTX_EINVAL, "tile shape not built (16x16, 32x8, 8x32)");
+    if (b->teno_direct && k != 0)
+        return htx_internal_fail(HTX_ENOTSUP, "the direct TENO5 launches of this precision use 16x16 tiles");
     BHIP(hipStreamSynchronize(b->stream));
     b->shape = k;
     int rc = build_teno_tables(b);
@@ -663,6 +743,10 @@ int htx_batch_set_tile(htx_batch* b, int tile_x, int tile_y) {
 int htx_batch_tune(htx_batch* b, int steps, int* tile_x, int* tile_y, double* ms) {
     if (!b) return htx_internal_fail(HTX_EINVAL, "null batch");
     if (steps < 1) return htx_internal_fail(HTX_EINVAL, "steps >= 1");
+    if (!b->teno.empty() && batch_prec_learned(b->teno_prec) && !b->have_cls)
+        return htx_internal_fail(HTX_ESTATE, "the learned precision needs htx_batch_load_classifier");
+    if (ms)
+        for (int k = 0; k < NSHAPES; ++k) ms[k] = -1.0;  // shapes not timed stay -1
     // Everything the timing changes is put back on every way out: the state
     // and control blocks (saved on the device first), the graph setting, the
     // tile shape with its launch tables (the original one unless a fastest
@@ -707,7 +791,9 @@ int htx_batch_tune(htx_batch* b, int steps, int* tile_x, int* tile_y, double* ms
     b->use_graph = false;
     int best = b->shape, rc;
     double best_ms = 1e300;
-    for (int k = 0; k < NSHAPES; ++k) {
+    // the direct TENO5 launches have one shape: time it alone
+    const int nshapes = b->teno_direct ? 1 : NSHAPES;
+    for (int k = 0; k < nshapes; ++k) {
         b->shape = k;
         if ((rc = build_teno_tables(b)) || (rc = enqueue_batch_step(b, b->stream))) return rc;  // warm-up
         BHIP(hipEventRecord(g.e0, b->stream));

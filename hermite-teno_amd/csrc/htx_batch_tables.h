@@ -55,9 +55,34 @@ struct TenoTables {
     int ntiles = 0;
 };
 
+// precision codes of a batch's TENO5 instances
+enum BatchPrec { BP_FP64_FAST = 0, BP_FP32_MIXED = 1, BP_LEARNED = 2, BP_FP64 = 3, BP_FP32 = 4, BP_LEARNED_FP32 = 5 };
+
+inline int batch_prec_code(int precision) {
+    switch (precision) {
+        case HTX_PREC_FP64_FAST: return BP_FP64_FAST;
+        case HTX_PREC_FP32_MIXED: return BP_FP32_MIXED;
+        case HTX_PREC_LEARNED: return BP_LEARNED;
+        case HTX_PREC_FP64: return BP_FP64;
+        case HTX_PREC_FP32: return BP_FP32;
+        case HTX_PREC_LEARNED_FP32: return BP_LEARNED_FP32;
+        default: return -1;
+    }
+}
+
+// Precisions whose TENO5 instances run one stage launch per instance (the
+// single-problem kernels: the strict FP64 build, the FP32 bulk build and the
+// learned path with its per-step face-flag kernel) instead of the persistent
+// launch over all instances' tiles, which is built for fp64_fast and
+// fp32_mixed only.
+inline bool batch_prec_direct(int code) { return code >= BP_LEARNED; }
+inline bool batch_prec_learned(int code) { return code == BP_LEARNED || code == BP_LEARNED_FP32; }
+
 // state = base of the three state copies (A | B | C, each `total` doubles)
-// prec: 0 fp64_fast, 1 fp32_mixed
-inline TenoTables make_teno_tables(const BatchLayout& L, double* state, Ctl* ctl, int TX, int TY, int prec) {
+// prec: a BatchPrec code; flags_x / flags_y: the learned precisions' face
+// flag arrays per TENO5 instance (null otherwise)
+inline TenoTables make_teno_tables(const BatchLayout& L, double* state, Ctl* ctl, int TX, int TY, int prec,
+                                   unsigned char* const* flags_x = nullptr, unsigned char* const* flags_y = nullptr) {
     TenoTables T;
     const int nt = (int)L.teno.size();
     T.args.resize(3 * (size_t)nt);
@@ -88,8 +113,11 @@ inline TenoTables make_teno_tables(const BatchLayout& L, double* state, Ctl* ctl
             a.bc = p.bc;
             a.mode = mode;
             a.quirks = p.ref_quirks ? 1 : 0;
-            a.mixed = prec >= 1;
-            a.learned = 0;
+            a.mixed = (prec != BP_FP64_FAST && prec != BP_FP64) ? 1 : 0;
+            a.learned = batch_prec_learned(prec) ? 1 : 0;
+            a.bulk32 = (prec == BP_FP32 || prec == BP_LEARNED_FP32) ? 1 : 0;
+            a.face_flags_x = flags_x ? flags_x[k] : nullptr;
+            a.face_flags_y = flags_y ? flags_y[k] : nullptr;
             a.tiles_x = tx;
             a.tiles_y = ty;
             a.gam = p.gamma;

@@ -335,13 +335,17 @@ int htx_classifier_stats(htx_solver* h, long long* n_faces, long long* n_flagged
  * reference counterpart; the reference runs one System2D at a time).
  *
  * Every instance has its own parameter block (size, domain, end time tmax,
- * CFL, boundary condition, scheme) and its own dt.  TENO5 instances run in
- * one persistent launch per RK stage and must share flux and precision
- * (fp64_fast or fp32_mixed; the learned precision is not available in
- * batches and htx_batch_create returns HTX_ENOTSUP for it); Hermite instances
- * run batched Hermite launches and must share flux and precision (fp64_fast
- * or fp32_mixed).  A step can be captured
- * once in a hipGraph and replayed.  Instances past their tmax are skipped.
+ * CFL, boundary condition, scheme) and its own dt.  The TENO5 instances of
+ * a batch share flux and precision: in fp64_fast and fp32_mixed they run in
+ * one persistent launch per RK stage over all their tiles (reference HLLC or
+ * characteristic Rusanov); in fp64 (the drop-in's strict build), fp32,
+ * learned and learned_fp32 each instance runs the single-problem stage
+ * kernel of that precision, one launch per instance and stage, the learned
+ * ones after a per-step face-flag launch (these give each instance the
+ * single problem's results bit for bit).  Hermite instances share flux and
+ * precision: fp64_fast or fp32_mixed in batched Hermite launches, fp64 one
+ * strict launch sequence per instance.  A step can be captured once in a
+ * hipGraph and replayed.  Instances past their tmax are skipped.
  * ------------------------------------------------------------------------ */
 typedef struct htx_batch htx_batch;
 
@@ -355,8 +359,8 @@ int htx_batch_get_state(htx_batch* b, int i, double* q, size_t ld);
 /* Four-quadrant Riemann data for instance i, as htx_init_quadrant. */
 int htx_batch_init_quadrant(htx_batch* b, int i, double xs, double ys, const double* states);
 
-/* Classifier of the learned precision, as htx_load_classifier (kept for
- * the learned batches; no batch runs that precision at present). */
+/* Classifier of the learned precisions, as htx_load_classifier: required
+ * before a batch with learned TENO5 instances steps (HTX_ESTATE otherwise). */
 int htx_batch_load_classifier(htx_batch* b, const float* params, double threshold);
 
 /* Enqueues n steps of every instance (without blocking). */
@@ -373,13 +377,15 @@ int htx_batch_sync(htx_batch* b);
  * it; 0: launch kernel by kernel. */
 int htx_batch_use_graph(htx_batch* b, int enable);
 
-/* Tile shape of the TENO5 launch: 16x16, 32x8 or 8x32. */
+/* Tile shape of the persistent TENO5 launch: 16x16, 32x8 or 8x32 (the
+ * direct launches take 16x16 only: HTX_ENOTSUP for the others). */
 int htx_batch_set_tile(htx_batch* b, int tile_x, int tile_y);
 
 /* Autotuner: times `steps` steps of the batch with every built tile shape
  * (from the current state, which is restored afterwards), keeps the fastest
  * and returns it; ms (may be null) receives the per-step milliseconds of the
- * three shapes in the order 16x16, 32x8, 8x32. */
+ * three shapes in the order 16x16, 32x8, 8x32 (-1 for a shape not timed:
+ * batches with direct TENO5 launches time 16x16 only). */
 int htx_batch_tune(htx_batch* b, int steps, int* tile_x, int* tile_y, double* ms);
 
 /* ------------------------------------------------------------------------

@@ -77,17 +77,25 @@ def test_python_mirror_raises_typed_error():
     assert ei.value.code == -1
 
 
-def test_batch_rejects_learned_precision():
-    """Batches run TENO5 in fp64_fast or fp32_mixed; the learned precision
-    is refused (HTX_ENOTSUP) before any device work."""
+def test_batch_rejects_unbuilt_combinations():
+    """The TENO5 instances of a batch share flux and precision, and the
+    persistent launch (fp64_fast, fp32_mixed) is built for the reference's
+    HLLC and the characteristic Rusanov flux only: both are refused
+    (HTX_ENOTSUP) before any device work."""
     import htx
     from htx.batch import Batch, params_for
 
-    ps = [params_for(0.0, 1.0, 32, 0.0, 1.0, 32, 0.5, 0.1, 1.4, precision="learned")]
+    ps = [params_for(0.0, 1.0, 32, 0.0, 1.0, 32, 0.5, 0.1, 1.4, precision="learned"),
+          params_for(0.0, 1.0, 40, 0.0, 1.0, 24, 0.5, 0.1, 1.4, precision="fp32_mixed")]
     with pytest.raises(htx.HtxError) as ei:
         Batch(ps)
     assert ei.value.code == -5
-    assert "fp64_fast or fp32_mixed" in str(ei.value)
+    assert "share flux and precision" in str(ei.value)
+    ps = [params_for(0.0, 1.0, 32, 0.0, 1.0, 32, 0.5, 0.1, 1.4, precision="fp32_mixed", ref_quirks=False)]
+    with pytest.raises(htx.HtxError) as ei:
+        Batch(ps)
+    assert ei.value.code == -5
+    assert "reference quirks" in str(ei.value)
 
 
 def test_hermite_rejects_learned_precision():

@@ -1,7 +1,9 @@
-"""Batches of independent problems: the persistent TENO5 launch and the
-batched Hermite launches reproduce single-instance solvers, every built tile
-shape gives the same result, graph replays equal eager launches, the
-autotuner restores the state it times from."""
+"""Batches of independent problems: the persistent TENO5 launch, the direct
+per-instance launches (strict FP64, FP32 bulk, learned) and the batched
+Hermite launches reproduce single-instance solvers, the strict FP64 batch
+reproduces the oracle, every built tile shape gives the same result, graph
+replays equal eager launches, the autotuner restores the state it times
+from."""
 import numpy as np
 import pytest
 
@@ -102,12 +104,7 @@ def test_batch_tile_shapes_agree():
         b.close()
 
 
-LEARNED_OFF = pytest.mark.skip(
-    reason="batched learned precision is switched off (htx_batch_create returns HTX_ENOTSUP): its persistent "
-           "launch faulted the GPU with an illegal address inside htx_batch_tune, cause not yet found")
-
-
-@pytest.mark.parametrize("precision", ["fp64_fast", pytest.param("learned", marks=LEARNED_OFF)])
+@pytest.mark.parametrize("precision", ["fp64_fast", "learned"])
 def test_batch_graph_replays_match_eager(precision):
     """A captured step replayed several times, with eager work and
     synchronisation between the replays, equals eager launches after every
@@ -115,7 +112,7 @@ def test_batch_graph_replays_match_eager(precision):
     print(f"[case] batch graph replay vs eager ({precision})", flush=True)
     g, inst = make_batch(precision, tmax=10.0)
     e, _ = make_batch(precision, tmax=10.0)
-    if precision == "learned":
+    if precision.startswith("learned"):
         g.load_classifier()
         e.load_classifier()
     g.use_graph(True)
@@ -148,7 +145,7 @@ def test_batch_advance_and_tune():
     assert n >= 1
 
 
-@pytest.mark.parametrize("precision", ["fp64_fast", "fp32_mixed", pytest.param("learned", marks=LEARNED_OFF)])
+@pytest.mark.parametrize("precision", ["fp64_fast", "fp32_mixed", "learned", "learned_fp32", "fp32"])
 def test_batch_many_tiles_per_workgroup(precision):
     """More TENO5 tiles than the persistent grid holds (3 x 19 x 17 tiles of
     16 x 16 against 2 workgroups per CU), so each workgroup walks several
@@ -171,7 +168,7 @@ def test_batch_many_tiles_per_workgroup(precision):
     b = Batch(ps)
     for i, d in enumerate(inst):
         b.quadrant(i, d["xs"], d["ys"], d["states"])
-    if precision == "learned":
+    if precision.startswith("learned"):
         b.load_classifier()
     shape, _ = b.tune(steps=1)
     print(f"autotuner: {shape}", flush=True)
@@ -195,4 +192,97 @@ def test_batch_many_tiles_per_workgroup(precision):
         # the same tile code; FP64 rounding-level differences between them
         # can flip an FP32 cut-off decision of the mixed precision
         assert err < (1e-10 if precision == "fp32_mixed" and d["scheme"] == "teno5" else 1e-12)
+    b.close()
+
+
+def test_batch_strict_fp64_reproduces_the_oracle(orc):
+    """The drop-in precision in a batch: TENO5 instances of uneven size run
+    the strict stage kernel one launch per instance and equal the oracle's
+    SSP-RK3 steps (own dt per instance) bit for bit; strict Hermite
+    instances equal their single-instance strict solver bit for bit."""
+    import htx
+    from htx.batch import Batch, params_for
+    from htx.ic import SplitMix64, quadrant_params
+    from oracle import params
+
+    print("[case] strict FP64 batch (3 TENO5, 2 Hermite) vs oracle / single strict solvers, 3 steps", flush=True)
+    rng = SplitMix64(91)
+    inst = []
+    for nx, ny, scheme in [(40, 24, "teno5"), (17, 33, "teno5"), (36, 36, "hteno"), (64, 20, "teno5"),
+                           (20, 28, "hteno")]:
+        xs, ys, st = quadrant_params(rng)
+        m = max(nx, ny)
+        inst.append(dict(nx=nx, ny=ny, scheme=scheme, xr=nx / m, yt=ny / m, xs=xs * nx / m, ys=ys * ny / m,
+                         states=st, flux="hllc" if scheme == "teno5" else "rusanov"))
+    ps = [params_for(0.0, d["xr"], d["nx"], 0.0, d["yt"], d["ny"], 0.5, 10.0, GAMMA, scheme=d["scheme"],
+                     flux=d["flux"], precision="fp64") for d in inst]
+    b = Batch(ps)
+    q0 = []
+    for i, d in enumerate(inst):
+        b.quadrant(i, d["xs"], d["ys"], d["states"])
+        q0.append(b.get_state(i))
+    b.use_graph(True)
+    b.steps(3)
+    for i, d in enumerate(inst):
+        q = b.get_state(i)
+        t, n = b.time(i)
+        assert n == 3
+        if d["scheme"] == "teno5":
+            prm = params(d["nx"], d["ny"], 0.0, d["xr"], 0.0, d["yt"], GAMMA, "zero_grad", "hllc", True)
+            qc, tc, k = orc.fv2d_advance(prm, q0[i], 0.5, 0.0, 10.0, 3)
+            print(f"instance {i} teno5 {d['nx']}x{d['ny']}: max |q - oracle| {np.abs(q - qc).max():.3e}", flush=True)
+            assert t == tc and k == 3
+            np.testing.assert_array_equal(q, qc)
+        else:
+            s = htx.System2D(0.0, d["xr"], d["nx"], 0.0, d["yt"], d["ny"], 0.5, 10.0, 0.0, GAMMA, scheme="hteno",
+                             flux="rusanov", bc="zero_grad", precision="fp64")
+            s.set_state(q0[i])
+            s.steps_async(3)
+            q_ref, t_ref = s.get_state(), s.time
+            s.close()
+            print(f"instance {i} hteno {d['nx']}x{d['ny']}: max |q - single| {np.abs(q - q_ref).max():.3e}",
+                  flush=True)
+            assert t == t_ref
+            np.testing.assert_array_equal(q, q_ref)
+    b.close()
+
+
+def test_batch_serving_sweep_against_fp64():
+    """The serving-style sweep (seeded four-quadrant problems, uneven
+    log-uniform sizes, TENO5 + HLLC or Hermite-TENO + Rusanov with
+    probability 1/2 each) at reduced count, sizes and end time, run in the
+    sweep's precision: TENO5 learned_fp32 (FP32 bulk arithmetic plus the
+    learned indicator), Hermite fp32_mixed.  Every instance reaches its end
+    time and stays within 1e-3 relative L1 of the oracle-pinned
+    single-instance FP64 solver."""
+    import htx
+    from htx.batch import Batch, params_for
+    from htx.ic import batch_workload
+
+    ws = batch_workload(count=12, lo=32, hi=160, tend=0.03)
+    print(f"[case] serving sweep: {len(ws)} instances, learned_fp32 / fp32_mixed vs strict FP64", flush=True)
+    ps = [params_for(w["xl"], w["xr"], w["nx"], w["yb"], w["yt"], w["ny"], w["cfl"], w["tmax"], w["gamma"],
+                     scheme=w["scheme"], flux=w["flux"], bc=w["bc"],
+                     precision="learned_fp32" if w["scheme"] == "teno5" else "fp32_mixed") for w in ws]
+    b = Batch(ps)
+    b.load_classifier()
+    for i, w in enumerate(ws):
+        b.quadrant(i, w["xs"], w["ys"], w["states"])
+    b.use_graph(True)
+    b.advance()
+    worst = 0.0
+    for i, w in enumerate(ws):
+        t, n = b.time(i)
+        assert t == pytest.approx(w["tmax"], rel=1e-14)
+        s = htx.System2D(w["xl"], w["xr"], w["nx"], w["yb"], w["yt"], w["ny"], w["cfl"], w["tmax"], 0.0, w["gamma"],
+                         scheme=w["scheme"], flux=w["flux"], bc=w["bc"], precision="fp64")
+        s.quadrant(w["xs"], w["ys"], w["states"])
+        s.timeStepping(w["tmax"])
+        q_ref = s.get_state()
+        s.close()
+        err = rel_l1(b.get_state(i), q_ref)
+        worst = max(worst, err)
+        print(f"instance {i} {w['scheme']} {w['nx']}x{w['ny']}: {n} steps, rel L1 vs FP64 {err:.3e}", flush=True)
+        assert err < 1e-3
+    print(f"worst rel L1 {worst:.3e}", flush=True)
     b.close()
