@@ -185,16 +185,18 @@ def precision_study(n=2048):
     return r
 
 
-def batch_study(count=256, steps=20, t_compare=0.02):
+def batch_study(count=256, steps=20, t_compare=0.02, teno_precision="learned_fp32", herm_precision="fp32_mixed"):
     """Serving-style sweep: `count` seeded quadrant problems with sides
     log-uniform in [64, 1024], half TENO5 + HLLC, half Hermite-TENO + Rusanov,
-    all in the FP32-mixed precision (FP32 cut-off decisions and nonlinear
-    weights, FP64 values, fluxes and RK accumulation), advanced together; aggregate
-    cell-updates/s over `steps` steps from t = 0 (every instance active;
-    graph replay, autotuned tile shape) and each instance's relative L1
-    density difference at t = t_compare against the plain FP64 drop-in solver
-    run alone (t_compare shortens the workload's t_end = 0.2 to bound the
-    bench's run time)."""
+    advanced together with the TENO5 instances in `teno_precision` and the
+    Hermite instances in `herm_precision` (default: FP32 bulk arithmetic plus
+    the learned indicator for TENO5, FP32 decisions with FP64 values for
+    Hermite; RK accumulation in FP64 throughout); aggregate cell-updates/s
+    over `steps` steps from t = 0 (every instance active; eager and graph
+    replay, autotuned tile shape) and, with t_compare, each instance's
+    relative L1 density difference at t = t_compare against the plain FP64
+    drop-in solver run alone (t_compare shortens the workload's t_end = 0.2 to
+    bound the bench's run time)."""
     import numpy as np
 
     import htx
@@ -205,8 +207,11 @@ def batch_study(count=256, steps=20, t_compare=0.02):
 
     def make(tmax):
         ps = [params_for(w["xl"], w["xr"], w["nx"], w["yb"], w["yt"], w["ny"], w["cfl"], tmax, w["gamma"],
-                         scheme=w["scheme"], flux=w["flux"], precision="fp32_mixed") for w in ws]
+                         scheme=w["scheme"], flux=w["flux"],
+                         precision=teno_precision if w["scheme"] == "teno5" else herm_precision) for w in ws]
         bb = Batch(ps)
+        if teno_precision.startswith("learned"):
+            bb.load_classifier()
         for i, w in enumerate(ws):
             bb.quadrant(i, w["xs"], w["ys"], w["states"])
         return bb
@@ -216,11 +221,12 @@ def batch_study(count=256, steps=20, t_compare=0.02):
     out = {"instances": count, "cells": cells,
            "teno5_instances": sum(w["scheme"] == "teno5" for w in ws),
            "workload": f"{count} seeded quadrant problems, sides log-uniform in [64, 1024], TENO5+HLLC "
-                       f"or Hermite-TENO+Rusanov with probability 1/2, fp32_mixed, t_end 0.2"}
+                       f"or Hermite-TENO+Rusanov with probability 1/2, t_end 0.2",
+           "teno5_precision": teno_precision, "hermite_precision": herm_precision}
     log("[batch] autotuning the TENO5 tile shape")
     shape, ms = b.tune(steps=2)
     out["tile"] = f"{shape[0]}x{shape[1]}"
-    out["tune_ms_per_step"] = {"16x16": ms[0], "32x8": ms[1], "8x32": ms[2]}
+    out["tune_ms_per_step"] = {k: v for k, v in zip(("16x16", "32x8", "8x32"), ms) if v >= 0}
     for graph in (False, True):
         b.use_graph(graph)
         b.steps(2)
@@ -408,7 +414,10 @@ def main():
         except Exception as e:
             log(f"[gpu] precision study failed: {e}")
         try:
+            # the sweep's own precision (learned indicator + FP32 bulk), then
+            # the persistent launch's fp32_mixed for comparison (timing only)
             out["batch"] = batch_study()
+            out["batch_fp32_mixed"] = batch_study(t_compare=0.0, teno_precision="fp32_mixed")
         except Exception as e:
             log(f"[gpu] batch study failed: {e}")
     if not args.no_cpu:
