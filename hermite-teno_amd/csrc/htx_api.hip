@@ -24,6 +24,8 @@ hipError_t htx_strict_launch_max_speed(const double* q, int nx, int ny, int pitc
                                        int quirks, void* ctl, hipStream_t s);
 hipError_t htx_strict_launch_fv1d_stage(const void* a, hipStream_t s);
 hipError_t htx_strict_launch_fv1d_max_speed(const double* q, int nx, double gam, void* ctl, hipStream_t s);
+hipError_t htx_strict_launch_fv1d_lmpe(const double* q, int nx, int bc, double dx, unsigned char* flags,
+                                       hipStream_t s);
 hipError_t htx_strict_launch_fv1d_kxrcf(const double* q, int nx, int bc, double radius, unsigned char* flags,
                                         hipStream_t s);
 
@@ -91,6 +93,13 @@ struct htx_solver {
     unsigned char* dFlagsX = nullptr;  // classifier flags of the x and y faces (learned precision)
     unsigned char* dFlagsY = nullptr;
     unsigned char* dTroubled = nullptr;  // 1D KXRCF cell flags (allocated on first use)
+    unsigned char* dLmpe = nullptr;      // 1D LMPE flags, 3 fields (allocated on first use)
+    // the analytic initial condition last set (htx_init_vortex,
+    // htx_init_entropy_wave) for htx_l1_density_error: its exact solution
+    // is the initial field advected with velocity (adv_u, adv_v)
+    ExactArgs exact{};
+    double adv_u = 0.0, adv_v = 0.0;
+    double* dPartial = nullptr;  // per-workgroup partial sums + the total (allocated on first use)
     float cls_threshold = 0.0f;
     bool have_classifier = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -436,6 +445,8 @@ void htx_destroy(htx_solver* h) {
     if (h->dFlagsX) (void)hipFree(h->dFlagsX);
     if (h->dFlagsY) (void)hipFree(h->dFlagsY);
     if (h->dTroubled) (void)hipFree(h->dTroubled);
+    if (h->dLmpe) (void)hipFree(h->dLmpe);
+    if (h->dPartial) (void)hipFree(h->dPartial);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -449,6 +460,7 @@ int htx_set_state(htx_solver* h, const double* q, size_t ld) {
     if (rc) return rc;
     h->have_state = true;
     h->next_stage = 1;
+    h->exact.kind = EXACT_NONE;
     Ctl c;
     return read_ctl_checked(h, &c);
 }
@@ -491,8 +503,86 @@ int htx_init_vortex(htx_solver* h, double cx, double cy, const double* modes, in
     int rc = refresh_cfl(h);
     if (rc) return rc;
     h->have_state = true;
+    // the vortex rides on the background flow u = v = 1
+    h->exact = ExactArgs{};
+    h->exact.kind = EXACT_VORTEX;
+    h->exact.cx = cx;
+    h->exact.cy = cy;
+    h->exact.nmodes = nmodes;
+    for (int m = 0; m < nmodes; ++m)
+        for (int k = 0; k < 6; ++k) h->exact.modes[m][k] = a.modes[m][k];
+    h->adv_u = h->adv_v = 1.0;
     Ctl c;
     return read_ctl_checked(h, &c);
+}
+
+int htx_init_entropy_wave(htx_solver* h, double rho, double u, double v, double p, double amplitude) {
+    if (!h) return fail(HTX_EINVAL, "null handle");
+    if (h->p.dim != 2) return fail(HTX_ENOTSUP, "the entropy wave is a 2D initial condition");
+    if (!(rho > 0.0) || !(p > 0.0) || !std::isfinite(u) || !std::isfinite(v) || !std::isfinite(amplitude) ||
+        !(std::fabs(amplitude) < rho))
+        return fail(HTX_EINVAL, "entropy wave needs rho > |amplitude|, p > 0 and finite u, v");
+    EntropyArgs a{};
+    a.q = h->dA;
+    a.nx = h->nx;
+    a.ny = h->ny;
+    a.pitch = h->pitch;
+    a.nfield = h->nfield;
+    a.fstride = h->fstride;
+    a.xl = h->p.xl;
+    a.yb = h->p.yb;
+    a.dx = h->dx;
+    a.dy = h->dy;
+    a.gam = h->p.gamma;
+    a.rho = rho;
+    a.u = u;
+    a.v = v;
+    a.p = p;
+    a.amp = amplitude;
+    gauss9(a.gx, a.gw);
+    HIP_TRY(launch_ic_entropy(a, h->stream));
+    int rc = refresh_cfl(h);
+    if (rc) return rc;
+    h->have_state = true;
+    h->next_stage = 1;
+    h->exact = ExactArgs{};
+    h->exact.kind = EXACT_ENTROPY;
+    h->exact.rho = rho;
+    h->exact.amp = amplitude;
+    h->adv_u = u;
+    h->adv_v = v;
+    Ctl c;
+    return read_ctl_checked(h, &c);
+}
+
+int htx_l1_density_error(htx_solver* h, double t_shift, double* err) {
+    if (!h || !err) return fail(HTX_EINVAL, "null argument");
+    if (h->p.dim != 2) return fail(HTX_ENOTSUP, "the exact solutions are 2D");
+    if (!h->have_state || h->exact.kind == EXACT_NONE)
+        return fail(HTX_ESTATE, "no analytic initial condition set (htx_init_vortex, htx_init_entropy_wave)");
+    if (!std::isfinite(t_shift)) return fail(HTX_EINVAL, "t_shift must be finite");
+    const int nb = exact_l1_blocks(h->nx, h->ny);
+    if (!h->dPartial) HIP_TRY(hipMalloc(&h->dPartial, sizeof(double) * ((size_t)nb + 1)));
+    ExactArgs a = h->exact;
+    a.q = h->dA;
+    a.nx = h->nx;
+    a.ny = h->ny;
+    a.pitch = h->pitch;
+    a.xl = h->p.xl;
+    a.yb = h->p.yb;
+    a.dx = h->dx;
+    a.dy = h->dy;
+    a.gam = h->p.gamma;
+    a.sx = h->adv_u * t_shift;
+    a.sy = h->adv_v * t_shift;
+    gauss9(a.gx, a.gw);
+    a.partial = h->dPartial;
+    a.out = h->dPartial + nb;
+    HIP_TRY(launch_exact_l1(a, h->stream));
+    double sum = 0.0;
+    HIP_TRY(copy_sync(&sum, a.out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    *err = sum / ((double)h->nx * h->ny);
+    return HTX_OK;
 }
 
 namespace {
@@ -515,6 +605,7 @@ PiecewiseArgs piecewise_args(const htx_solver* h, int kind) {
 
 int finish_ic(htx_solver* h) {
     h->have_state = true;
+    h->exact.kind = EXACT_NONE;
     int rc = refresh_cfl(h);
     if (rc) return rc;
     Ctl c;
@@ -1181,6 +1272,16 @@ int htx_kxrcf_flags(htx_solver* h, unsigned char* flags) {
     const double radius = std::pow(h->dx / 2.0, 1.5);
     HIP_TRY(htx_strict_launch_fv1d_kxrcf(h->dA, h->nx, h->p.bc, radius, h->dTroubled, h->stream));
     HIP_TRY(copy_sync(flags, h->dTroubled, (size_t)h->nx, hipMemcpyDeviceToHost, h->stream));
+    return HTX_OK;
+}
+
+int htx_lmpe_flags(htx_solver* h, unsigned char* flags) {
+    if (!h || !flags) return fail(HTX_EINVAL, "null argument");
+    if (h->p.dim != 1) return fail(HTX_ENOTSUP, "the LMPE detector belongs to the 1D solvers");
+    if (!h->have_state) return fail(HTX_ESTATE, "the solver has no state");
+    if (!h->dLmpe) HIP_TRY(hipMalloc(&h->dLmpe, 3 * (size_t)h->nx));
+    HIP_TRY(htx_strict_launch_fv1d_lmpe(h->dA, h->nx, h->p.bc, h->dx, h->dLmpe, h->stream));
+    HIP_TRY(copy_sync(flags, h->dLmpe, 3 * (size_t)h->nx, hipMemcpyDeviceToHost, h->stream));
     return HTX_OK;
 }
 

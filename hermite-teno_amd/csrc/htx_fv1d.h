@@ -307,4 +307,37 @@ HTX_HD bool fv1d_kxrcf(const double* rho, double mom, int c, int nx, int bc, dou
     return fabs(h1 - h2) / (radius * fabs(r0)) > 1.0;
 }
 
+// LMPE troubled-cell detector of FVHermite::System1D::rhs
+// (FVHermite_System1D.cpp:239-257), from a reset state.  The test at cell g
+// of one field u (ghosts by the boundary condition):
+//     A = |-u_{g-1} + 2 u_g - u_{g+1}| / 2 > dx / 2  and
+//     0 <= xc <= 3 dx,  xc = (-2 u_{g-1} + 3 u_g - u_{g+1}) / (-u_{g-1} + 2 u_g - u_{g+1}) dx.
+HTX_HD bool fv1d_lmpe_test(const double* u, int g, int nx, int bc, double dx) {
+    HTX_NO_CONTRACT
+    const double um = fv1d_rho_at(u, g - 1, nx, bc), u0 = fv1d_rho_at(u, g, nx, bc), up = fv1d_rho_at(u, g + 1, nx, bc);
+    const double A = fabs(-um + 2.0 * u0 - up) / 2.0;
+    if (!(A > 0.5 * dx)) return false;
+    const double xc = (-2. * um + 3. * u0 - up) / (-um + 2.0 * u0 - up) * dx;
+    return 0.0 <= xc && xc <= 3. * dx;
+}
+
+// The reference sweeps g = -2 .. nx+1 in order (k = 4 ghosts, s = 1) and
+// tests a cell only when it is not flagged yet; a successful test flags
+// g-1, g, g+1.  From a reset state a cell is flagged before its turn only by
+// its left neighbour's test, so along a run of consecutive successful tests
+// starting at r the tests at r, r+2, r+4, ... act and the others are
+// skipped: active(g) = test(g) and (g - r) even.
+HTX_HD bool fv1d_lmpe_active(const double* u, int g, int nx, int bc, double dx) {
+    if (g < -2 || g > nx + 1 || !fv1d_lmpe_test(u, g, nx, bc, dx)) return false;
+    int r = g;
+    while (r - 1 >= -2 && fv1d_lmpe_test(u, r - 1, nx, bc, dx)) --r;
+    return ((g - r) & 1) == 0;
+}
+
+// the flag of interior cell c after the sweep
+HTX_HD bool fv1d_lmpe(const double* u, int c, int nx, int bc, double dx) {
+    return fv1d_lmpe_active(u, c - 1, nx, bc, dx) || fv1d_lmpe_active(u, c, nx, bc, dx) ||
+           fv1d_lmpe_active(u, c + 1, nx, bc, dx);
+}
+
 }  // namespace HTX_NS

@@ -85,6 +85,18 @@ __global__ void __launch_bounds__(FV1_T) k_fv1d_kxrcf(const double* q, int nx, i
     if (c < nx) flags[c] = fv1d_kxrcf(q, q[nx + c], c, nx, bc, radius) ? 1 : 0;
 }
 
+// LMPE flags of the three fields: flags[eq nx + c]
+__global__ void __launch_bounds__(FV1_T) k_fv1d_lmpe(const double* q, int nx, int bc, double dx,
+                                                     unsigned char* flags) {
+    const int c = blockIdx.x * FV1_T + threadIdx.x, eq = blockIdx.y;
+    if (c < nx) flags[eq * nx + c] = fv1d_lmpe(q + (long long)eq * nx, c, nx, bc, dx) ? 1 : 0;
+}
+
+hipError_t launch_fv1d_lmpe(const double* q, int nx, int bc, double dx, unsigned char* flags, hipStream_t s) {
+    hipLaunchKernelGGL(k_fv1d_lmpe, dim3((nx + FV1_T - 1) / FV1_T, 3), dim3(FV1_T), 0, s, q, nx, bc, dx, flags);
+    return hipGetLastError();
+}
+
 hipError_t launch_fv1d_stage(const Fv1dArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_fv1d_stage, dim3((a.nx + FV1_T - 1) / FV1_T), dim3(FV1_T), 0, s, a);
     return hipGetLastError();

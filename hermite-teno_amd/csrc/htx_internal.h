@@ -114,6 +114,39 @@ struct VortexArgs {
     double gx[9], gw[9];
 };
 
+// Entropy wave (Equations_Euler2D.cpp:430-511): rho = rho_inf + A sin(pi (x
+// + y)), u, v, p constant; 9x9 Gauss cell averages of the conserved state
+// and, with nfield 12, of its x and y derivatives (the Hermite moments).
+struct EntropyArgs {
+    double* q;
+    int nx, ny, pitch, nfield;
+    long long fstride;
+    double xl, yb, dx, dy, gam;
+    double rho, u, v, p, amp;
+    double gx[9], gw[9];
+};
+
+// Exact solutions for the L1 density error of printResult
+// (FV_System2D.cpp:653-666, FVHermite_System2D.cpp:821-835): the initial
+// field of an analytic condition evaluated at (x - sx, y - sy), cell-averaged
+// with the 9x9 Gauss rule; |rho - rho_exact| summed per workgroup into
+// partial[blockIdx.x] in a fixed order.
+enum ExactKind { EXACT_NONE = 0, EXACT_ENTROPY = 1, EXACT_VORTEX = 2 };
+struct ExactArgs {
+    const double* q;  // density field of the state
+    int nx, ny, pitch;
+    double xl, yb, dx, dy, gam;
+    int kind;
+    double sx, sy;
+    double rho, amp;               // entropy wave
+    double cx, cy;                 // vortex centre
+    int nmodes;
+    double modes[16][6];           // vortex density modes (k, l, a, phi, Lx, Ly)
+    double gx[9], gw[9];
+    double* partial;               // [gridDim.x]
+    double* out;                   // the sum of the partials
+};
+
 enum IcKind { IC_QUADRANT = 0, IC_SHOCK_BLOBS = 1 };
 constexpr int IC_MAX_BLOBS = 64;
 
@@ -170,5 +203,8 @@ hipError_t launch_max_speed(const double* q, int nx, int ny, int pitch, long lon
 hipError_t launch_ic_vortex(const VortexArgs& a, hipStream_t s);
 // htx_ic.hip (fast build only)
 hipError_t launch_ic_piecewise(const PiecewiseArgs& a, hipStream_t s);
+hipError_t launch_ic_entropy(const EntropyArgs& a, hipStream_t s);
+int exact_l1_blocks(int nx, int ny);
+hipError_t launch_exact_l1(const ExactArgs& a, hipStream_t s);
 
 }  // namespace HTX_NS

@@ -43,3 +43,8 @@ hipError_t htx_strict_launch_fv1d_kxrcf(const double* q, int nx, int bc, double 
                                         hipStream_t s) {
     return htx_strict::launch_fv1d_kxrcf(q, nx, bc, radius, flags, s);
 }
+
+hipError_t htx_strict_launch_fv1d_lmpe(const double* q, int nx, int bc, double dx, unsigned char* flags,
+                                       hipStream_t s) {
+    return htx_strict::launch_fv1d_lmpe(q, nx, bc, dx, flags, s);
+}

@@ -114,6 +114,9 @@ SIGNATURES = {
     "htx_batch_set_tile": (_I, [_P, _I, _I]),
     "htx_batch_tune": (_I, [_P, _I, _IP, _IP, _DP]),
     "htx_step_noalloc": (_I, [ctypes.POINTER(HtxParams), _P, _D, _P, _P]),
+    "htx_init_entropy_wave": (_I, [_P, _D, _D, _D, _D, _D]),
+    "htx_l1_density_error": (_I, [_P, _D, _DP]),
+    "htx_lmpe_flags": (_I, [_P, ctypes.c_void_p]),
     "htx_last_error": (ctypes.c_char_p, []),
     "htx_abi_version": (_I, []),
 }

@@ -97,6 +97,22 @@ class System2D:
         ptr = dptr(mm) if mm.size else None
         check(_lib.load().htx_init_vortex(self._h, float(centre[0]), float(centre[1]), ptr, int(m.shape[0])))
 
+    def entropyWave(self, rho=1.0, u=1.0, v=1.0, p=1.0, amplitude=0.5):
+        """FV::System2D::entropyWave / FVHermite::System2D::entropyWave
+        (FV_System2D.cpp:49-74, FVHermite_System2D.cpp:52-86), generated on
+        the device: rho + amplitude sin(pi (x + y)) at constant (u, v, p);
+        the reference's case is the default."""
+        check(_lib.load().htx_init_entropy_wave(self._h, float(rho), float(u), float(v), float(p), float(amplitude)))
+
+    def l1_density_error(self, t_shift=None):
+        """printResult's numerical error on the device (FV_System2D.cpp:
+        653-666): mean |rho - rho_exact| against the last analytic initial
+        condition advected by t_shift (default: the current time)."""
+        err = ctypes.c_double()
+        t = self.time if t_shift is None else float(t_shift)
+        check(_lib.load().htx_l1_density_error(self._h, t, ctypes.byref(err)))
+        return err.value
+
     def quadrant(self, xs, ys, states):
         """Four-quadrant Riemann data (primitive rows (rho, u, v, p) for
         k = (x >= xs) + 2 (y >= ys)), point-sampled on the device as

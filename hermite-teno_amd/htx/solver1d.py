@@ -122,6 +122,14 @@ class System1D:
         check(_lib.load().htx_kxrcf_flags(self._h, f.ctypes.data))
         return f
 
+    def lmpeCells(self):
+        """LMPE troubled-cell flags (uint8, (3, nx); 1 = discontinuous) of the
+        current state per field, the detector of FVHermite::System1D::rhs
+        (FVHermite_System1D.cpp:239-257) swept from clear flags."""
+        f = np.zeros((3, self.nx), dtype=np.uint8)
+        check(_lib.load().htx_lmpe_flags(self._h, f.ctypes.data))
+        return f
+
     # -- time stepping ----------------------------------------------------
     def rhs(self):
         r = np.empty((3, self.nx), dtype=np.float64)

@@ -145,6 +145,29 @@ int htx_get_state(htx_solver* h, double* q, size_t ld);
  * nmodes <= 16. */
 int htx_init_vortex(htx_solver* h, double cx, double cy, const double* modes, int nmodes);
 
+/* FV::System2D::entropyWave / FVHermite::System2D::entropyWave
+ * (FV_System2D.cpp:49-74, FVHermite_System2D.cpp:52-86 with
+ * Equations_Euler2D.cpp:430-511), generated on the device: density
+ * rho + amplitude sin(pi (x + y)) at constant velocity (u, v) and pressure p,
+ * 9x9 Gauss cell averages of the conserved state and, for the Hermite scheme,
+ * of its x and y derivatives.  The reference's case is (1, 1, 1, 1, 0.5).
+ * Requires rho > |amplitude| and p > 0. */
+int htx_init_entropy_wave(htx_solver* h, double rho, double u, double v, double p, double amplitude);
+
+/* printResult's numerical error (FV_System2D.cpp:653-666,
+ * FVHermite_System2D.cpp:821-835) computed on the device: the mean over the
+ * cells of |rho - rho_exact|, where rho_exact is the 9x9 Gauss cell average
+ * of the last analytic initial condition (htx_init_vortex or
+ * htx_init_entropy_wave, whose exact solutions are their initial fields
+ * advected with the background velocity: (1, 1) for the vortex, (u, v) for
+ * the wave) shifted by velocity x t_shift.  The reference evaluates the
+ * entropy wave and the Hermite vortex at t_shift = tFinal and the FV vortex
+ * at t_shift = 0.  HTX_ESTATE when no such condition was set (any other
+ * initial condition or htx_set_state clears it).  The cell sums are added
+ * in a fixed tree order (the reference adds them in row order; the two agree
+ * to rounding). */
+int htx_l1_density_error(htx_solver* h, double t_shift, double* err);
+
 /* Four-quadrant Riemann data, generated on the device and point-sampled at
  * cell centres as FV::System2D::Riemann2D does (FV_System2D.cpp:101-111 with
  * the quadrant template of Equations_Euler2D.cpp:530-617), with the split
@@ -315,6 +338,18 @@ int htx_classify(htx_solver* h, int dir, float* logits, float* feat);
  * branches take teno5i); they are reported as a classical detector's flag
  * rate. */
 int htx_kxrcf_flags(htx_solver* h, unsigned char* flags);
+
+/* 1D only: the LMPE troubled-cell flags of FVHermite::System1D::rhs
+ * (FVHermite_System1D.cpp:239-257) for the current state, per field:
+ * flags[eq nx + c], 3 nx bytes.  The reference sweeps the cells in order
+ * (two ghost cells past each end, filled by the boundary condition) and tests
+ * a cell not yet flagged: A = |-u_{i-1} + 2 u_i - u_{i+1}| / 2 > dx / 2 and
+ * 0 <= xc <= 3 dx with xc = (-2 u_{i-1} + 3 u_i - u_{i+1}) /
+ * (-u_{i-1} + 2 u_i - u_{i+1}) dx flag cells i-1, i and i+1.  Here the sweep
+ * starts from clear flags (the reference's reset loop is commented out, so
+ * its flags accumulate across steps); the sequential skip rule is kept
+ * exactly.  Reported as a classical detector's flag rate. */
+int htx_lmpe_flags(htx_solver* h, unsigned char* flags);
 
 /* Learned precision: the classifier flags (1 = smooth) of the last step's
  * state (or of the last htx_rhs) that the stage kernels read: x faces

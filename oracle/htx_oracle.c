@@ -904,6 +904,51 @@ void orc_ic_vortex(const orc_params* p, double cx, double cy, const double* mode
         }
 }
 
+/* Euler2D::entropyWaves, entropyWavesdx, entropyWavesdy
+ * (Equations_Euler2D.cpp:430-511): 9x9 Gauss cell averages of the entropy
+ * wave's conserved state (rho + amp sin(pi (x + y)), u, v, p) and, with
+ * hermite, of its derivative (fields 4..7 and 8..11, equal: the density
+ * depends on x + y only).  The cell centre is shifted by (-sx, -sy) before
+ * the Gauss points are spread, as the reference forms its exact solution
+ * (FV_System2D.cpp:64-71). */
+void orc_ic_entropy(const orc_params* p, double rho, double u, double v, double pr, double amp, double sx,
+                    double sy, int hermite, double* q) {
+    double a[9], wt[9];
+    orc_gauss9(a, wt);
+    const double pi = 4.0 * atan(1.0), g1 = p->gamma - 1.0;
+    int nx = p->nx, ny = p->ny;
+    double dx = (p->xr - p->xl) / nx, dy = (p->yt - p->yb) / ny;
+    size_t n = (size_t)nx * ny;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double x = ((double)i + 0.5) * dx + p->xl - sx;
+            double y = ((double)j + 0.5) * dy + p->yb - sy;
+            double acc[4] = {0, 0, 0, 0}, dacc[4] = {0, 0, 0, 0};
+            for (int ii = 0; ii < 9; ++ii)
+                for (int jj = 0; jj < 9; ++jj) {
+                    double px = x + dx * a[ii] / 2.0, py = y + dy * a[jj] / 2.0;
+                    double w[4] = {rho + amp * sin(pi * (px + py)), u, v, pr}, qq[4], d[4];
+                    orc_conservative(p->gamma, w, qq);
+                    (void)g1;
+                    d[0] = amp * cos(pi * (px + py)) * pi;
+                    d[1] = u * d[0];
+                    d[2] = v * d[0];
+                    d[3] = (0.5 * (u * u + v * v)) * d[0];
+                    for (int e = 0; e < 4; ++e) {
+                        acc[e] += wt[ii] * wt[jj] * qq[e] / 4.0;
+                        dacc[e] += wt[ii] * wt[jj] * d[e] / 4.0;
+                    }
+                }
+            for (int e = 0; e < 4; ++e) {
+                q[e * n + (size_t)j * nx + i] = acc[e];
+                if (hermite) {
+                    q[(4 + e) * n + (size_t)j * nx + i] = dacc[e];
+                    q[(8 + e) * n + (size_t)j * nx + i] = dacc[e];
+                }
+            }
+        }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Schemes::Hermite (Schemes_Hermite.cpp)                                    */
 /* ------------------------------------------------------------------------ */
@@ -1663,6 +1708,45 @@ void orc_fv1d_kxrcf(int nx, double xl, double xr, const double* q, unsigned char
     }
     free(r);
     free(m);
+}
+
+/* The LMPE troubled-cell detector of FVHermite::System1D::rhs
+ * (FVHermite_System1D.cpp:239-257; k = 4 ghost cells, s = 1, :20-21), from
+ * a reset state (all flags clear; the reference's reset loop at :214-216 is
+ * commented out, so there the flags accumulate over steps).  For each field
+ * eq and each cell i = s+1 .. nxx-s-2 in order (two ghost cells on each
+ * side, filled by the boundary condition: periodicBc :77-86 or zero
+ * gradient), when cell i is not yet flagged:
+ *     A  = |-u_{i-1} + 2 u_i - u_{i+1}| / 2,
+ *     xc = (-2 u_{i-1} + 3 u_i - u_{i+1}) / (-u_{i-1} + 2 u_i - u_{i+1}) dx;
+ * A > dx / 2 and 0 <= xc <= 3 dx flag cells i-1, i, i+1 (so a cell flagged
+ * by its left neighbour is not tested itself).  flags[eq nx + c] receives
+ * the interior cells' flags.  bc: 1 periodic, 0 zero gradient. */
+void orc_fv1d_lmpe(int nx, double xl, double xr, int bc, const double* q, unsigned char* flags) {
+    const int k = 4, s = 1, nxx = nx + 2 * k;
+    const double dx = (xr - xl) / nx;
+    double* u = (double*)calloc((size_t)nxx, sizeof(double));
+    unsigned char* f = (unsigned char*)calloc((size_t)nxx, 1);
+    for (int eq = 0; eq < 3; ++eq) {
+        for (int i = 0; i < nxx; ++i) {
+            int g = i - k;
+            if (bc == 1) g = ((g % nx) + nx) % nx;
+            else g = g < 0 ? 0 : (g >= nx ? nx - 1 : g);
+            u[i] = q[eq * nx + g];
+            f[i] = 0;
+        }
+        for (int i = s + 1; i < nxx - s - 1; ++i) {
+            if (f[i]) continue;
+            double A = fabs(-u[i - 1] + 2.0 * u[i] - u[i + 1]) / 2.0;
+            if (A > 0.5 * dx) {
+                double xc = (-2. * u[i - 1] + 3. * u[i] - u[i + 1]) / (-u[i - 1] + 2.0 * u[i] - u[i + 1]) * dx;
+                if (0.0 <= xc && xc <= 3. * dx) f[i] = f[i - 1] = f[i + 1] = 1;
+            }
+        }
+        for (int c = 0; c < nx; ++c) flags[eq * nx + c] = f[k + c];
+    }
+    free(u);
+    free(f);
 }
 
 /* FV::System1D::timeStepping + RungeKutta3, FV_System1D.cpp:243-292 */

@@ -87,6 +87,7 @@ int orc_fv1d_advance(int nx, double xl, double xr, double gamma, int recon, int 
                      double* q, double cfl, double t0, double tfinal, int max_steps, double* t_out);
 void orc_ic_sod(int nx, double xl, double xr, double gamma, double* q);
 void orc_fv1d_kxrcf(int nx, double xl, double xr, const double* q, unsigned char* flags);
+void orc_fv1d_lmpe(int nx, double xl, double xr, int bc, const double* q, unsigned char* flags);
 
 /* ---- test hooks: smoothness indicators ---- */
 void orc_teno5_indicators(const double u[6], double SI[6], double b35[2]);
@@ -95,6 +96,8 @@ void orc_hteno_indicators(const double u[3], const double ud[3], double SI[3], d
 /* ---- initial conditions (restated generators) ---- */
 void orc_ic_vortex(const orc_params* p, double cx, double cy, const double* modes,
                    int nmodes, double* q);
+void orc_ic_entropy(const orc_params* p, double rho, double u, double v, double pr, double amp, double sx,
+                    double sy, int hermite, double* q);
 
 #ifdef __cplusplus
 }

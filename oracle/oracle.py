@@ -51,6 +51,7 @@ _SIG = {
     "orc_fv2d_rk3": (None, [ctypes.POINTER(OrcParams), _DP, _D]),
     "orc_fv2d_advance": (_I, [ctypes.POINTER(OrcParams), _DP, _D, _D, _D, _I, _DP]),
     "orc_ic_vortex": (None, [ctypes.POINTER(OrcParams), _D, _D, _DP, _I, _DP]),
+    "orc_ic_entropy": (None, [ctypes.POINTER(OrcParams), _D, _D, _D, _D, _D, _D, _D, _I, _DP]),
     "orc_hteno": (None, [_D, _DP, _DP, _DP]),
     "orc_hermite_limiter": (_D, [_D, _DP, _DP]),
     "orc_dflux": (None, [_D, _I, _DP, _DP, _DP]),
@@ -65,6 +66,7 @@ _SIG = {
     "orc_fv1d_advance": (_I, [_I, _D, _D, _D, _I, _I, _I, _DP, _D, _D, _D, _I, _DP]),
     "orc_ic_sod": (None, [_I, _D, _D, _D, _DP]),
     "orc_fv1d_kxrcf": (None, [_I, _D, _D, _DP, ctypes.POINTER(ctypes.c_ubyte)]),
+    "orc_fv1d_lmpe": (None, [_I, _D, _D, _I, _DP, ctypes.POINTER(ctypes.c_ubyte)]),
     "orc_weno5": (None, [_DP, _I, _DP]),
     "orc_teno5_indicators": (None, [_DP, _DP, _DP]),
     "orc_hteno_indicators": (None, [_DP, _DP, _DP, _DP, _DP]),
@@ -234,6 +236,15 @@ class Oracle:
                                       ctypes.byref(t))
         return q, t.value, n
 
+    def ic_entropy(self, prm, rho=1.0, u=1.0, v=1.0, p=1.0, amp=0.5, shift=(0.0, 0.0), hermite=False):
+        """Entropy-wave cell averages (4 fields, or 12 with the derivative
+        moments), centres shifted by -shift (the exact solution at t is
+        shift = (u t, v t))."""
+        q = np.empty((12 if hermite else 4, prm.ny, prm.nx))
+        self.lib.orc_ic_entropy(ctypes.byref(prm), rho, u, v, p, amp, shift[0], shift[1], 1 if hermite else 0,
+                                q.ctypes.data_as(_DP))
+        return q
+
     def ic_vortex(self, prm, centre=(0.0, 0.0), modes=None):
         m = np.zeros((0, 6)) if modes is None else np.asarray(modes, dtype=np.float64).reshape(-1, 6)
         mm = np.ascontiguousarray(m.reshape(-1))
@@ -347,6 +358,15 @@ class Oracle:
         q = np.ascontiguousarray(q, dtype=np.float64)
         f = np.zeros(nx, dtype=np.uint8)
         self.lib.orc_fv1d_kxrcf(nx, xl, xr, q.ctypes.data_as(_DP), f.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+        return f
+
+    def fv1d_lmpe(self, nx, xl, xr, q, bc="periodic"):
+        """LMPE troubled-cell flags (uint8, (3, nx)) of a 1D state (3, nx),
+        from a reset state."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        f = np.zeros((3, nx), dtype=np.uint8)
+        self.lib.orc_fv1d_lmpe(nx, xl, xr, {"zero_grad": 0, "periodic": 1}[bc], q.ctypes.data_as(_DP),
+                               f.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
         return f
 
     def ic_sod(self, nx, xl=0.0, xr=1.0, gamma=1.4):

@@ -36,3 +36,11 @@ extern "C" void fv1dh_stage(int nx, double xl, double xr, double gamma, int reco
                 fv1d_stage_value(a, c, e, F[e * (size_t)(nx + 1) + c], F[e * (size_t)(nx + 1) + c + 1], dt);
     for (size_t k = 0; k < res.size(); ++k) out[k] = res[k];
 }
+
+// LMPE flags (per field, flags[eq nx + c]) by the device's run-parity form
+// of the reference's sequential sweep
+extern "C" void fv1dh_lmpe(int nx, double xl, double xr, int bc, const double* q, unsigned char* flags) {
+    const double dx = (xr - xl) / nx;
+    for (int eq = 0; eq < 3; ++eq)
+        for (int c = 0; c < nx; ++c) flags[eq * nx + c] = fv1d_lmpe(q + (size_t)eq * nx, c, nx, bc, dx) ? 1 : 0;
+}

@@ -135,3 +135,52 @@ def test_kxrcf_flags_match_oracle(orc):
     with pytest.raises(HtxError):
         htx._lib.check(htx._lib.load().htx_kxrcf_flags(t._h, np.zeros(16, np.uint8).ctypes.data))
     t.close()
+
+
+@pytest.mark.parametrize("bc", ["zero_grad", "periodic"])
+def test_lmpe_flags_match_oracle(orc, bc):
+    """The GPU LMPE detector (run-parity form of the reference's sequential
+    sweep) equals the oracle's literal sweep bit for bit, per field, on the
+    Sod states of a 40-step run and on seeded staircases with long runs of
+    successful tests; the 2D solver refuses the call.  The flag rates of LMPE
+    and KXRCF over the Sod run are printed side by side."""
+    import htx
+    from htx._lib import HtxError
+
+    nx = 400
+    print(f"[case] LMPE flags, Sod N=400 and seeded staircases ({bc})", flush=True)
+    s = htx.System1D(0.0, 1.0, nx, 0.5, 0.2, 0.0, GAMMA, scheme="teno5", flux="hllc", bc=bc)
+    s.sodShock()
+    lm, kx = [], []
+    for k in range(41):
+        q = s.get_state()
+        f = s.lmpeCells()
+        np.testing.assert_array_equal(f, orc.fv1d_lmpe(nx, 0.0, 1.0, q, bc=bc))
+        lm.append(f.any(axis=0).mean())
+        kx.append(s.troubledCells().mean())
+        if k < 40:
+            s.step()
+    print(f"flag rate over the Sod run: LMPE {min(lm):.4f} .. {max(lm):.4f}, KXRCF {min(kx):.4f} .. {max(kx):.4f}",
+          flush=True)
+    rng = np.random.default_rng(9)
+    for _ in range(5):
+        steps = np.cumsum(rng.integers(0, 2, nx) * rng.uniform(-0.6, 0.6, nx))
+        q = np.stack([2.0 + steps, 0.1 * steps, 3.0 + np.abs(steps)])
+        alt = np.where(np.arange(nx) % 2 == 0, 0.5, 1.0)
+        q[1, 100:180] = alt[100:180]
+        s.set_state(q)
+        np.testing.assert_array_equal(s.lmpeCells(), orc.fv1d_lmpe(nx, 0.0, 1.0, q, bc=bc))
+    s.close()
+    t = htx.System2D(0.0, 1.0, 16, 0.0, 1.0, 16, 0.5, 0.1, 0.0, GAMMA)
+    with pytest.raises(HtxError) as ei:
+        _lib_lmpe(t)
+    assert ei.value.code == -5
+    t.close()
+
+
+def _lib_lmpe(solver2d):
+    from htx import _lib
+    from htx._lib import check
+
+    f = np.zeros(3 * 16, dtype=np.uint8)
+    check(_lib.load().htx_lmpe_flags(solver2d._h, f.ctypes.data))

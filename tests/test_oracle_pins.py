@@ -317,3 +317,35 @@ def test_linear_residual_converges_and_approaches_teno(orc):
     assert errs[0] / errs[1] > 8.0 and errs[1] / errs[2] > 4.0
     assert all(e <= et * 1.01 for e, et in zip(errs, errs_t))
     assert diffs[0] > diffs[1] > diffs[2]
+
+
+def test_entropy_wave_averages_closed_form(orc):
+    """The oracle's 9x9 Gauss entropy-wave averages against the closed form:
+    the average of sin(pi (x + y)) over a dx x dy cell is
+    sin(pi (xc + yc)) S(dx) S(dy) with S(h) = sin(pi h / 2) / (pi h / 2), the
+    derivative moments the same with pi cos, momentum and energy linear in
+    the density; the exact-solution shift moves the centres by -shift."""
+    import math
+
+    nx, ny = 7, 5
+    prm = params(nx, ny, 0.0, 1.4, 0.0, 1.5, 1.4, "periodic", "hllc", True)
+    u, v, p, amp, rho, g1 = 0.7, -0.4, 0.9, 0.3, 1.0, 0.4
+    dx, dy = 1.4 / nx, 1.5 / ny
+
+    def S(h):
+        return math.sin(math.pi * h / 2) / (math.pi * h / 2)
+
+    for shift in ((0.0, 0.0), (0.37, -0.11)):
+        q = orc.ic_entropy(prm, rho, u, v, p, amp, shift=shift, hermite=True)
+        xc = (np.arange(nx) + 0.5) * dx - shift[0]
+        yc = (np.arange(ny) + 0.5) * dy - shift[1]
+        arg = math.pi * (xc[None, :] + yc[:, None])
+        r = rho + amp * np.sin(arg) * S(dx) * S(dy)
+        d = amp * math.pi * np.cos(arg) * S(dx) * S(dy)
+        np.testing.assert_allclose(q[0], r, rtol=1e-14, atol=1e-15)
+        np.testing.assert_allclose(q[1], u * r, rtol=1e-14, atol=1e-15)
+        np.testing.assert_allclose(q[2], v * r, rtol=1e-14, atol=1e-15)
+        np.testing.assert_allclose(q[3], p / g1 + 0.5 * (u * u + v * v) * r, rtol=1e-14)
+        np.testing.assert_allclose(q[4], d, rtol=1e-13, atol=1e-14)
+        np.testing.assert_allclose(q[8:12], q[4:8], rtol=0, atol=0)
+        np.testing.assert_allclose(q[7], 0.5 * (u * u + v * v) * d, rtol=1e-13, atol=1e-14)
