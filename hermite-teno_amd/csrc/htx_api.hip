@@ -124,8 +124,10 @@ int check_params(const htx_params* p) {
             p->scheme != HTX_SCHEME_WENO5Z)
             return fail(HTX_EINVAL, "1D scheme must be teno5, teno5i, weno5js or weno5z");
         if (p->precision != HTX_PREC_FP64) return fail(HTX_ENOTSUP, "the 1D path is built in fp64 precision");
-    } else if (p->scheme == HTX_SCHEME_TENO5I || p->scheme == HTX_SCHEME_WENO5JS || p->scheme == HTX_SCHEME_WENO5Z) {
-        return fail(HTX_ENOTSUP, "teno5i, weno5js and weno5z are 1D reconstructions");
+    } else if ((p->scheme == HTX_SCHEME_TENO5I || p->scheme == HTX_SCHEME_WENO5JS || p->scheme == HTX_SCHEME_WENO5Z) &&
+               p->precision != HTX_PREC_FP64 && p->precision != HTX_PREC_FP64_FAST) {
+        // the 2D path takes them in place of TENO5 (FV_System2D.cpp:379, :394)
+        return fail(HTX_ENOTSUP, "the 2D teno5i, weno5js and weno5z reconstructions are built in fp64 and fp64_fast");
     }
     if (!(p->gamma > 1.0)) return fail(HTX_EINVAL, "gamma must exceed 1");
     if (!(p->cfl > 0.0)) return fail(HTX_EINVAL, "cfl must be positive");
@@ -157,6 +159,10 @@ Fv2dArgs stage_args(htx_solver* h, const double* qin, const double* q0, double* 
     a.mixed = (h->p.precision == HTX_PREC_FP32_MIXED || is_learned(h->p.precision) || is_bulk32(h->p.precision)) ? 1 : 0;
     a.learned = is_learned(h->p.precision) ? 1 : 0;
     a.bulk32 = is_bulk32(h->p.precision) ? 1 : 0;
+    a.recon = h->p.scheme == HTX_SCHEME_TENO5I    ? 1
+              : h->p.scheme == HTX_SCHEME_WENO5JS ? 2
+              : h->p.scheme == HTX_SCHEME_WENO5Z  ? 3
+                                                  : 0;
     a.face_flags_x = h->dFlagsX;
     a.face_flags_y = h->dFlagsY;
     a.gam = h->p.gamma;

@@ -129,99 +129,6 @@ HTX_HD void e1_rusanov_char(double gam, double g1, const double L[9], const doub
     for (int e = 0; e < 3; ++e) f[e] = 0.5 * ((fl[e] + fr[e]) - rd[e]);
 }
 
-// Lagrange::teno5i (Schemes_Lagrange.cpp:1235-1263): teno5 with tau the
-// largest smoothness measure of the three candidates
-HTX_HD double teno5i_side(double v0, double v1, double v2, double v3, double v4, double p0, double p1, double p2,
-                          double d0, double d1, double d2) {
-    HTX_NO_CONTRACT
-    const double c1 = 13.0 / 12.0;
-    double c_0 = v2 - 2.0 * v3 + v4, b_0 = (3.0 * v2 - 4.0 * v3 + v4) / 2.0;
-    double c_1 = v1 - 2.0 * v2 + v3, b_1 = (v1 - v3) / 2.0;
-    double c_2 = v0 - 2.0 * v1 + v2, b_2 = (v0 - 4.0 * v1 + 3.0 * v2) / 2.0;
-    double s0 = b_0 * b_0 + c1 * c_0 * c_0;
-    double s1 = b_1 * b_1 + c1 * c_1 * c_1;
-    double s2 = b_2 * b_2 + c1 * c_2 * c_2;
-    double tau = s0;
-    if (s1 > tau) tau = s1;
-    if (s2 > tau) tau = s2;
-    double g0 = 1.0 + tau / (s0 + 1e-40);
-    double g1 = 1.0 + tau / (s1 + 1e-40);
-    double g2 = 1.0 + tau / (s2 + 1e-40);
-    g0 = g0 * g0 * g0; g0 = g0 * g0;
-    g1 = g1 * g1 * g1; g1 = g1 * g1;
-    g2 = g2 * g2 * g2; g2 = g2 * g2;
-    double gs = (g0 + g1) + g2;
-    bool k0 = !(g0 / gs < 5e-4), k1 = !(g1 / gs < 5e-4), k2 = !(g2 / gs < 5e-4);
-    double w0 = k0 ? d0 : 0.0, w1 = k1 ? d1 : 0.0, w2 = k2 ? d2 : 0.0;
-    double ws = (w0 + w1) + w2;
-    w0 = w0 / ws; w1 = w1 / ws; w2 = w2 / ws;
-    double out = (p0 * w0 + p1 * w1) + p2 * w2;
-    return out != out ? 1e-15 : out;
-}
-
-HTX_HD void teno5i(const double u[6], double& um, double& up) {
-    HTX_NO_CONTRACT
-    const double a = 1.0 / 3.0, b = 5.0 / 6.0, c = -1.0 / 6.0, d = 11.0 / 6.0, e = -7.0 / 6.0;
-    double p0 = (a * u[2] + b * u[3]) + c * u[4];
-    double p1 = (c * u[1] + b * u[2]) + a * u[3];
-    double p2 = (a * u[0] + e * u[1]) + d * u[2];
-    um = teno5i_side(u[0], u[1], u[2], u[3], u[4], p0, p1, p2, 0.3, 0.6, 0.1);
-    double q0 = (d * u[3] + e * u[4]) + a * u[5];
-    double q1 = (a * u[2] + b * u[3]) + c * u[4];
-    double q2 = (c * u[1] + b * u[2]) + a * u[3];
-    up = teno5i_side(u[1], u[2], u[3], u[4], u[5], q0, q1, q2, 0.1, 0.6, 0.3);
-}
-
-// One side of Lagrange::weno5JS (Schemes_Lagrange.cpp:358-381; weights
-// d_k / (1e-6 + SI_k)^2) or weno5Z (:383-408; weights d_k (1 + tau / (SI_k
-// + 1e-40)), tau = |SI_2 - SI_0|), normalised, on the window v0..v4 with
-// candidates p0..p2; the smoothness measures of teno5_side
-// (Schemes_Lagrange.cpp:1013-1026).  A NaN result becomes 1e-15 (catchnan).
-HTX_HD double weno5_side(double v0, double v1, double v2, double v3, double v4, double p0, double p1, double p2,
-                         double d0, double d1, double d2, bool z) {
-    HTX_NO_CONTRACT
-    const double c1 = 13.0 / 12.0;
-    double c_0 = v2 - 2.0 * v3 + v4, b_0 = (3.0 * v2 - 4.0 * v3 + v4) / 2.0;
-    double c_1 = v1 - 2.0 * v2 + v3, b_1 = (v1 - v3) / 2.0;
-    double c_2 = v0 - 2.0 * v1 + v2, b_2 = (v0 - 4.0 * v1 + 3.0 * v2) / 2.0;
-    double s0 = b_0 * b_0 + c1 * c_0 * c_0;
-    double s1 = b_1 * b_1 + c1 * c_1 * c_1;
-    double s2 = b_2 * b_2 + c1 * c_2 * c_2;
-    double w0, w1, w2;
-    if (z) {
-        const double tau = fabs(s2 - s0);
-        w0 = d0 * (1.0 + tau / (s0 + 1e-40));
-        w1 = d1 * (1.0 + tau / (s1 + 1e-40));
-        w2 = d2 * (1.0 + tau / (s2 + 1e-40));
-    } else {
-        const double e0 = 1e-6 + s0, e1 = 1e-6 + s1, e2 = 1e-6 + s2;
-        w0 = d0 / (e0 * e0);
-        w1 = d1 / (e1 * e1);
-        w2 = d2 / (e2 * e2);
-    }
-    const double sum = (w0 + w1) + w2;
-    w0 = w0 / sum;
-    w1 = w1 / sum;
-    w2 = w2 / sum;
-    const double r = (p0 * w0 + p1 * w1) + p2 * w2;
-    return r != r ? 1e-15 : r;
-}
-
-// Lagrange::weno5JS (z false) / weno5Z on u_{i-2..i+3}: (u-, u+) at x_{i+1/2},
-// with the candidates and linear weights of teno5.
-HTX_HD void weno5(const double u[6], bool z, double& um, double& up) {
-    HTX_NO_CONTRACT
-    const double a = 1.0 / 3.0, b = 5.0 / 6.0, c = -1.0 / 6.0, d = 11.0 / 6.0, e = -7.0 / 6.0;
-    const double p0 = (a * u[2] + b * u[3]) + c * u[4];
-    const double p1 = (c * u[1] + b * u[2]) + a * u[3];
-    const double p2 = (a * u[0] + e * u[1]) + d * u[2];
-    um = weno5_side(u[0], u[1], u[2], u[3], u[4], p0, p1, p2, 0.3, 0.6, 0.1, z);
-    const double q0 = (d * u[3] + e * u[4]) + a * u[5];
-    const double q1 = (a * u[2] + b * u[3]) + c * u[4];
-    const double q2 = (c * u[1] + b * u[2]) + a * u[3];
-    up = weno5_side(u[1], u[2], u[3], u[4], u[5], q0, q1, q2, 0.1, 0.6, 0.3, z);
-}
-
 // Flux through the face between cells c and c + 1 (c = -1 .. nx - 1):
 // characteristic reconstruction (FV_System1D.cpp:194-208), back to
 // conservative variables (FV_System1D.h:48-51), and the flux (:210-223).

@@ -27,7 +27,9 @@ namespace HTX_NS {
 // Tiles that form the cells' Roe inputs per face (tile_prim_on_the_fly) also
 // fit three per CU.
 
-template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED, bool BULK32 = false>
+// RECON: the characteristic reconstruction (0 TENO5, the reference's; 1
+// teno5i, 2 weno5JS, 3 weno5Z), built for the FP64 precisions
+template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED, bool BULK32 = false, int RECON = 0>
 __global__ void __launch_bounds__(256, (BULK32 || tile_prim_on_the_fly<MIXED, LEARNED, BULK32>()) ? 3 : 2)
     fv2d_stage_kernel(Fv2dArgs a) {
     using T = Fv2dTile<TX, TY, typename std::conditional<BULK32, float, double>::type,
@@ -39,7 +41,7 @@ __global__ void __launch_bounds__(256, (BULK32 || tile_prim_on_the_fly<MIXED, LE
         dt = a.ctl->dt;
     }
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    stage_tile<TX, TY, FLUX, MIXED, LEARNED>(a, s, tile, dt, threadIdx.x);
+    stage_tile<TX, TY, FLUX, MIXED, LEARNED, RECON>(a, s, tile, dt, threadIdx.x);
 }
 
 // one-thread control kernels around a step (ctl_dt, ctl_tadvance)
@@ -146,24 +148,36 @@ __global__ void __launch_bounds__(256) k_ic_vortex(VortexArgs a) {
     for (int e = 0; e < 4; ++e) a.q[off + e * a.fstride] = acc[e];
 }
 
-template <int TX, int TY, bool MIXED, bool LEARNED, bool BULK32 = false>
+template <int TX, int TY, bool MIXED, bool LEARNED, bool BULK32 = false, int RECON = 0>
 static hipError_t launch_tile(const Fv2dArgs& a0, int flux, hipStream_t s) {
     Fv2dArgs a = a0;
     a.tiles_x = (a.nx + TX - 1) / TX;
     a.tiles_y = (a.ny + TY - 1) / TY;
     dim3 grid(a.tiles_x * a.tiles_y), block(256);
     if (flux == FLUX_RUSANOV_CHAR)
-        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_RUSANOV_CHAR, MIXED, LEARNED, BULK32>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_RUSANOV_CHAR, MIXED, LEARNED, BULK32, RECON>), grid, block, 0,
+                           s, a);
     else if (flux == FLUX_HLLC)
-        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC, MIXED, LEARNED, BULK32>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC, MIXED, LEARNED, BULK32, RECON>), grid, block, 0, s, a);
     else
-        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC_TORO, MIXED, LEARNED, BULK32>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((fv2d_stage_kernel<TX, TY, FLUX_HLLC_TORO, MIXED, LEARNED, BULK32, RECON>), grid, block, 0, s,
+                           a);
     return hipGetLastError();
 }
 
 template <int TX, int TY>
 static hipError_t launch_prec(const Fv2dArgs& a, int flux, hipStream_t s) {
     if (a.learned && (!a.face_flags_x || !a.face_flags_y)) return hipErrorInvalidValue;
+    // the alternative reconstructions: FP64 precisions, 16 x 16 tiles
+    if (a.recon != 0) {
+        if constexpr (TX == 16 && TY == 16) {
+            if (a.mixed || a.learned || a.bulk32) return hipErrorInvalidValue;
+            if (a.recon == 1) return launch_tile<TX, TY, false, false, false, 1>(a, flux, s);
+            if (a.recon == 2) return launch_tile<TX, TY, false, false, false, 2>(a, flux, s);
+            if (a.recon == 3) return launch_tile<TX, TY, false, false, false, 3>(a, flux, s);
+        }
+        return hipErrorInvalidValue;
+    }
 #if !HTX_EXACT_DIVISION
     // the FP32 bulk precision is built for the 16 x 16 and 16 x 8 tiles
     if constexpr (TX == 16) {
@@ -182,6 +196,7 @@ static hipError_t launch_prec(const Fv2dArgs& a, int flux, hipStream_t s) {
 // (smaller LDS footprints, more workgroups per CU for kernels with few
 // registers); any other request takes the default.
 hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y, hipStream_t s) {
+    if (a.recon != 0) return launch_prec<16, 16>(a, flux, s);
 #if !HTX_EXACT_DIVISION
     if (tile_x == 16 && tile_y == 8) return launch_prec<16, 8>(a, flux, s);
     if (tile_x == 32 && tile_y == 8 && !a.bulk32) return launch_prec<32, 8>(a, flux, s);

@@ -141,7 +141,8 @@ HTX_HD bool recon_face_owned(int fi, int fj, int i0, int j0, int nx, int ny) {
 // have positive density and pressure on both sides (checked in FP64; else
 // TENO5); the face's flag is then overwritten with the scheme it took (1 =
 // linear), and cnt accumulates (owned faces, flagged, linear) of this thread.
-template <int TX, int TY, bool YDIR, bool MIXED = false, bool LEARNED = false, typename FT = double, bool OF = false>
+template <int TX, int TY, bool YDIR, bool MIXED = false, bool LEARNED = false, int RECON = 0, typename FT = double,
+          bool OF = false>
 HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY, FT, OF>& s, int tid, int i0 = 0, int j0 = 0,
                      unsigned* cnt = nullptr) {
     using T = Fv2dTile<TX, TY, FT, OF>;
@@ -203,6 +204,10 @@ HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY, FT, OF>& s, int tid, in
                 double um, up;
                 if (MIXED)
                     teno5_mixed(c[f], um, up);
+                else if (RECON == 1)
+                    teno5i(c[f], um, up);
+                else if (RECON == 2 || RECON == 3)
+                    weno5(c[f], RECON == 3, um, up);
                 else
                     teno5(c[f], um, up);
                 s.sF[2 * f][w] = um;
@@ -533,7 +538,7 @@ __device__ __forceinline__ double wave_max(double v) {
 // the tile's CFL maxima (one atomic per workgroup).  dt = the stage's time
 // step.  Ends with every thread past its last LDS access only for mode 3;
 // callers that reuse the tile storage synchronise before the next tile.
-template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED, typename FT, bool OF>
+template <int TX, int TY, int FLUX, bool MIXED, bool LEARNED, int RECON = 0, typename FT, bool OF>
 __device__ __forceinline__ void stage_tile(const Fv2dArgs& a, Fv2dTile<TX, TY, FT, OF>& s, int tile, double dt, int tid) {
     using T = Fv2dTile<TX, TY, FT, OF>;
     const int i0 = (tile % a.tiles_x) * TX;
@@ -544,13 +549,13 @@ __device__ __forceinline__ void stage_tile(const Fv2dArgs& a, Fv2dTile<TX, TY, F
     if (LEARNED) ph_flags_load<TX, TY>(a, s, i0, j0, tid);
     __syncthreads();
     unsigned cnt[3] = {0, 0, 0};
-    ph_recon<TX, TY, false, MIXED, LEARNED>(a, s, tid, i0, j0, cnt);
+    ph_recon<TX, TY, false, MIXED, LEARNED, RECON>(a, s, tid, i0, j0, cnt);
     __syncthreads();
     ph_flux<TX, TY, false, FLUX, MIXED, LEARNED>(a, s, tid);
     __syncthreads();
     ph_xdiv<TX, TY>(a, s, tid, dxp);
     __syncthreads();
-    ph_recon<TX, TY, true, MIXED, LEARNED>(a, s, tid, i0, j0, cnt);
+    ph_recon<TX, TY, true, MIXED, LEARNED, RECON>(a, s, tid, i0, j0, cnt);
     if (LEARNED) {
         for (int k = 0; k < 3; ++k) {
             unsigned v = cnt[k];

@@ -93,6 +93,7 @@ struct Fv2dArgs {
     int mixed;          // 1: cut-off decisions and nonlinear weights in FP32
     int learned;        // 1: classifier flags pick linear or TENO5 per face (implies mixed)
     int bulk32;         // 1: face values, quadrature, back-projection and flux in FP32 (implies mixed)
+    int recon;          // characteristic reconstruction: 0 teno5 (the reference's), 1 teno5i, 2 weno5JS, 3 weno5Z
     int tiles_x, tiles_y;
     double gam, g1, dx, dy, area;
     QuadConsts qc;

@@ -45,9 +45,12 @@ enum htx_error {
 };
 
 /* HTX_SCHEME_TENO5I: Lagrange::teno5i (Schemes_Lagrange.cpp:1235-1263), the
- * reconstruction FV::System1D ships with; 1D only.  HTX_SCHEME_WENO5JS /
+ * reconstruction FV::System1D ships with.  HTX_SCHEME_WENO5JS /
  * HTX_SCHEME_WENO5Z: Lagrange::weno5JS / weno5Z (Schemes_Lagrange.cpp:358-408),
- * the reference's alternatives for the 1D solver's reconstruction; 1D only. */
+ * the reference's alternatives for the 1D solver's reconstruction.  In 2D
+ * each takes TENO5's place in the characteristic reconstruction of the x and
+ * y faces (FV_System2D.cpp:379, :394, where a user of the reference swaps the
+ * call), everything else unchanged; 2D builds: fp64 and fp64_fast. */
 enum htx_scheme {
     HTX_SCHEME_TENO5 = 0,
     HTX_SCHEME_HTENO = 1,

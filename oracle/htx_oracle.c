@@ -621,6 +621,22 @@ static inline void cell_q(const padded* P, int i, int j, double out[4]) {
  * transverse WENO quadrature (what the learned path computes where every
  * face is flagged smooth). */
 static void fv2d_rhs_impl(const orc_params* p, const double* q, double* rhs, int linear);
+static void teno5i_impl(const double u[6], double out[2]);
+static void weno5_impl(const double u[6], double out[2], int z);
+
+/* the reconstruction FV_System2D.cpp:379 / :394 calls (lagrangeObj.teno5),
+ * or one of the alternatives a user of the reference swaps in there
+ * (Schemes_Lagrange.cpp:358-408, :1235-1263) */
+static void recon2d(int recon, int linear, const double u[6], double out[2]) {
+    if (linear)
+        orc_linear5(u, out);
+    else if (recon == ORC_RECON_TENO5I)
+        teno5i_impl(u, out);
+    else if (recon == ORC_RECON_WENO5JS || recon == ORC_RECON_WENO5Z)
+        weno5_impl(u, out, recon == ORC_RECON_WENO5Z);
+    else
+        teno5_impl(u, out, NULL);
+}
 
 void orc_fv2d_rhs(const orc_params* p, const double* q, double* rhs) { fv2d_rhs_impl(p, q, rhs, 0); }
 
@@ -660,10 +676,7 @@ static void fv2d_rhs_impl(const orc_params* p, const double* q, double* rhs, int
                 for (int e = 0; e < 4; ++e) C[e][m] = y[e];
             }
             for (int e = 0; e < 4; ++e) {
-                if (linear)
-                    orc_linear5(C[e], out);
-                else
-                    teno5_impl(C[e], out, NULL);
+                recon2d(p->recon, linear, C[e], out);
                 uxl[e * nt + PIX(&P, i, j)] = out[0];
                 uxr[e * nt + PIX(&P, i, j)] = out[1];
             }
@@ -684,10 +697,7 @@ static void fv2d_rhs_impl(const orc_params* p, const double* q, double* rhs, int
                 for (int e = 0; e < 4; ++e) C[e][m] = y[e];
             }
             for (int e = 0; e < 4; ++e) {
-                if (linear)
-                    orc_linear5(C[e], out);
-                else
-                    teno5_impl(C[e], out, NULL);
+                recon2d(p->recon, linear, C[e], out);
                 uyl[e * nt + PIX(&P, i, j)] = out[0];
                 uyr[e * nt + PIX(&P, i, j)] = out[1];
             }

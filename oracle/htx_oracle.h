@@ -28,7 +28,10 @@ typedef struct {
     int bc;          /* ORC_BC_* */
     int flux;        /* ORC_FLUX_* */
     int quirks;      /* 1: reproduce reference quirks (HLLC c, HLLC E*, maxSpeed) */
+    int recon;       /* 2D characteristic reconstruction: ORC_RECON_* (the reference's is TENO5) */
 } orc_params;
+
+enum { ORC_RECON_TENO5 = 0, ORC_RECON_TENO5I = 1, ORC_RECON_WENO5JS = 2, ORC_RECON_WENO5Z = 3 };
 
 /* ---- scalar building blocks (unit-test surface) ---- */
 void orc_teno5(const double u[6], double out[2]);

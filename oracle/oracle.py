@@ -24,7 +24,7 @@ class OrcParams(ctypes.Structure):
         ("nx", _I), ("ny", _I),
         ("xl", _D), ("xr", _D), ("yb", _D), ("yt", _D),
         ("gamma", _D),
-        ("bc", _I), ("flux", _I), ("quirks", _I),
+        ("bc", _I), ("flux", _I), ("quirks", _I), ("recon", _I),
     ]
 
 
@@ -95,9 +95,11 @@ def _p(a):
     return a, a.ctypes.data_as(_DP)
 
 
-def params(nx, ny, xl=0.0, xr=1.0, yb=0.0, yt=1.0, gamma=1.4, bc="zero_grad", flux="hllc", quirks=True):
+def params(nx, ny, xl=0.0, xr=1.0, yb=0.0, yt=1.0, gamma=1.4, bc="zero_grad", flux="hllc", quirks=True,
+           recon="teno5"):
     return OrcParams(nx, ny, xl, xr, yb, yt, gamma, {"zero_grad": 0, "periodic": 1}[bc],
-                     {"rusanov": 0, "hllc": 1}[flux], 1 if quirks else 0)
+                     {"rusanov": 0, "hllc": 1}[flux], 1 if quirks else 0,
+                     {"teno5": 0, "teno5i": 1, "weno5js": 2, "weno5z": 3}[recon])
 
 
 class Oracle:

@@ -14,7 +14,7 @@ using namespace htx;
 // ph_flags_load from face-flag arrays in the kernel's layout (fx: x faces
 // i = -1 .. nx of each row at [j (nx + 2) + i + 1]; fy: y faces j = -1 .. ny
 // of each column at [(j + 1) nx + i]); null arrays flag every face smooth.
-template <int TX, int TY, bool MIXED = false, bool LEARNED = false, typename FT = double>
+template <int TX, int TY, bool MIXED = false, bool LEARNED = false, typename FT = double, int RECON = 0>
 static void run(Fv2dArgs a, int flux, const unsigned char* fx = nullptr, const unsigned char* fy = nullptr) {
     // the tile layout the product kernel of this precision uses
     using T = Fv2dTile<TX, TY, FT, tile_prim_on_the_fly<MIXED, LEARNED, sizeof(FT) == sizeof(float)>()>;
@@ -30,14 +30,14 @@ static void run(Fv2dArgs a, int flux, const unsigned char* fx = nullptr, const u
         for (int t = 0; t < T::NT; ++t) ph_load<TX, TY>(a, *s, i0, j0, t);
         if (LEARNED)
             for (int t = 0; t < T::NT; ++t) ph_flags_load<TX, TY>(a, *s, i0, j0, t);
-        for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, false, MIXED, LEARNED>(a, *s, t);
+        for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, false, MIXED, LEARNED, RECON>(a, *s, t);
         for (int t = 0; t < T::NT; ++t) {
             if (flux == 0) ph_flux<TX, TY, false, 0, MIXED, LEARNED>(a, *s, t);
             else if (flux == 1) ph_flux<TX, TY, false, 1, MIXED, LEARNED>(a, *s, t);
             else ph_flux<TX, TY, false, 2, MIXED, LEARNED>(a, *s, t);
         }
         for (int t = 0; t < T::NT; ++t) ph_xdiv<TX, TY>(a, *s, t, (double(*)[4]) & dxp[(size_t)t * T::KC * 4]);
-        for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, true, MIXED, LEARNED>(a, *s, t);
+        for (int t = 0; t < T::NT; ++t) ph_recon<TX, TY, true, MIXED, LEARNED, RECON>(a, *s, t);
         for (int t = 0; t < T::NT; ++t) {
             if (flux == 0) ph_flux<TX, TY, true, 0, MIXED, LEARNED>(a, *s, t);
             else if (flux == 1) ph_flux<TX, TY, true, 1, MIXED, LEARNED>(a, *s, t);
@@ -52,7 +52,8 @@ static void run(Fv2dArgs a, int flux, const unsigned char* fx = nullptr, const u
 }
 
 // mixed: 0 FP64, 1 FP32-mixed, 2 learned with every face flagged smooth,
-// 3 FP32 bulk, 4 learned on the FP32 bulk arithmetic, every face flagged
+// 3 FP32 bulk, 4 learned on the FP32 bulk arithmetic, every face flagged;
+// 5, 6, 7: FP64 with teno5i, weno5JS, weno5Z in place of TENO5
 extern "C" int fv2dh_rhs_prec(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc,
                               int flux, int quirks, int mixed, const double* q, double* rhs) {
     Fv2dArgs a{};
@@ -73,7 +74,13 @@ extern "C" int fv2dh_rhs_prec(int nx, int ny, double xl, double xr, double yb, d
     a.area = a.dx * a.dy;
     fill_quad_consts(a.qc);
     int fk = flux == 0 ? 0 : (quirks ? 1 : 2);
-    if (mixed == 4)
+    if (mixed == 5)
+        run<16, 16, false, false, double, 1>(a, fk);
+    else if (mixed == 6)
+        run<16, 16, false, false, double, 2>(a, fk);
+    else if (mixed == 7)
+        run<16, 16, false, false, double, 3>(a, fk);
+    else if (mixed == 4)
         run<16, 16, true, true, float>(a, fk);
     else if (mixed == 3)
         run<16, 16, true, false, float>(a, fk);

@@ -143,7 +143,10 @@ def test_one_dimensional_parameters():
 
     assert create(dim=1, precision=_lib.PRECISIONS["fp32_mixed"]) == -5
     assert create(dim=1, scheme=_lib.SCHEMES["hteno"]) == -1
-    assert create(dim=2, scheme=_lib.SCHEMES["teno5i"]) == -5
+    # the 2D reconstruction variants are built in the FP64 precisions only
+    for sch in ("teno5i", "weno5js", "weno5z"):
+        assert create(dim=2, scheme=_lib.SCHEMES[sch], precision=_lib.PRECISIONS["fp32_mixed"]) == -5
+        assert create(dim=2, scheme=_lib.SCHEMES[sch], precision=_lib.PRECISIONS["learned"]) == -5
     assert create(dim=3) == -1
     with pytest.raises(htx.HtxError):
         htx.System1D(0.0, 1.0, 0, 0.5, 0.2, 0.0, 1.4)

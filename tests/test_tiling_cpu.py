@@ -461,3 +461,28 @@ def test_learned_flag_placement_is_shift_equivariant(orc, harness_fast):
     # and the flags matter: all-TENO flags give another residual
     r0 = _rhs_flags(harness_fast, nx, ny, dom, "periodic", "hllc", q, *_periodic_flag_arrays(0 * cx, 0 * cy))
     assert np.abs(r0 - r).max() > 0.0
+
+
+@pytest.mark.parametrize("recon,mode", [("teno5i", 5), ("weno5js", 6), ("weno5z", 7)])
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+def test_tiling_bitwise_reconstruction_variants(orc, harness, recon, mode, flux):
+    """The 2D path with teno5i, weno5JS or weno5Z in TENO5's place: the
+    strict CPU build of the tile phases equals the oracle's residual with
+    the same reconstruction bit for bit, on the vortex (periodic) and on
+    quadrant data with seeded noise (zero gradient); on the noisy data the
+    residual differs from the TENO5 one (on clean jumps teno5i makes TENO5's
+    cut-off decisions)."""
+    from htx.ic import vortex_modes
+
+    rng = np.random.default_rng(4)
+    for nx, ny, bc, dom in [(48, 40, "periodic", (0.0, 10.0, 0.0, 10.0)), (40, 36, "zero_grad", (0.0, 1.0, 0.0, 1.0))]:
+        prm = params(nx, ny, *dom, 1.4, bc, flux, True, recon=recon)
+        if bc == "periodic":
+            q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
+        else:
+            q = quadrant(nx, ny, 3)
+            q[0] *= 1.0 + 0.05 * rng.random((ny, nx))
+        got = _rhs_prec(harness, nx, ny, dom, bc, flux, mode, q)
+        np.testing.assert_array_equal(got, orc.fv2d_rhs(prm, q))
+        if bc == "zero_grad":
+            assert not np.array_equal(got, orc.fv2d_rhs(params(nx, ny, *dom, 1.4, bc, flux, True), q))
