@@ -36,7 +36,7 @@ FLOPS_PER_CELL_UPDATE = {"hllc": 17416.6, "rusanov": 17575.0}
 # HBM bytes per stage-kernel launch at 1024^2, FP64, measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled against a 256 MiB device copy
 # (profiles/fv2d_stage_kernel_vortex1024_fp64_hbm_traffic.md); per cell-stage.
-MEASURED_TRAFFIC_PER_CELL_STAGE = {"fp64": 99.55e6 / (1024 * 1024)}
+MEASURED_TRAFFIC_PER_CELL_STAGE = {"fp64": 93.29e6 / (1024 * 1024)}
 
 
 def log(*a):
