@@ -225,7 +225,7 @@ struct htx_batch {
     int shape = 0, ntiles = 0, nblk_teno = 0;
     Fv2dArgs* dTenoArgs = nullptr;
     int *dTileInst = nullptr, *dTile0 = nullptr;
-    // direct TENO5 launches (batch_prec_direct): one 16 x 16 stage launch per
+    // direct TENO5 launches (batch_prec_direct): one stage launch (default tile) per
     // instance from these host argument blocks, and for the learned
     // precisions one face-flag launch per instance and step
     bool teno_direct = false;
@@ -380,9 +380,9 @@ int enqueue_batch_step(htx_batch* b, hipStream_t s) {
                     BHIP(lrn_launch_flags(f, s));
                 }
                 if (b->teno_prec == BP_FP64)
-                    BHIP(htx_strict_launch_fv2d_stage(&a, b->teno_flux, 16, 16, s));
+                    BHIP(htx_strict_launch_fv2d_stage(&a, b->teno_flux, 0, 0, s));
                 else
-                    BHIP(launch_fv2d_stage(a, b->teno_flux, 16, 16, s));
+                    BHIP(launch_fv2d_stage(a, b->teno_flux, 0, 0, s));
             }
         } else if (!b->teno.empty()) {
             BatchTenoArgs T{};
@@ -732,7 +732,7 @@ int htx_batch_set_tile(htx_batch* b, int tile_x, int tile_y) {
         if (SHAPES[s].tx == tile_x && SHAPES[s].ty == tile_y) k = s;
     if (k < 0) return htx_internal_fail(HTX_EINVAL, "tile shape not built (16x16, 32x8, 8x32)");
     if (b->teno_direct && k != 0)
-        return htx_internal_fail(HTX_ENOTSUP, "the direct TENO5 launches of this precision use 16x16 tiles");
+        return htx_internal_fail(HTX_ENOTSUP, "the direct TENO5 launches of this precision use the default tile");
     BHIP(hipStreamSynchronize(b->stream));
     b->shape = k;
     int rc = build_teno_tables(b);

@@ -109,8 +109,11 @@ typedef struct htx_params {
     int bc;             /* htx_bc: FV_System2D.cpp:365-366 */
     int precision;      /* htx_precision */
     int ref_quirks;     /* 1: reproduce the reference's HLLC and maxSpeed quirks */
-    int tile_x, tile_y; /* TENO5 stage tile: 0 = 16 x 16; the FP64_FAST, FP32_MIXED and
-                           LEARNED precisions also build 16 x 8 and 32 x 8 */
+    int tile_x, tile_y; /* TENO5 stage tile: 0 = the default (16 x 15; 16 x 16 for the learned
+                           precisions); 16 x 16 in every precision; the FP64_FAST, FP32_MIXED
+                           and LEARNED precisions also build 16 x 8 and 32 x 8.  Every shape
+                           gives the same results bit for bit, except that the learned
+                           precisions decide their linear transverse quadrature per wavefront */
 } htx_params;
 
 typedef struct htx_solver htx_solver;
@@ -415,15 +418,18 @@ int htx_batch_sync(htx_batch* b);
  * it; 0: launch kernel by kernel. */
 int htx_batch_use_graph(htx_batch* b, int enable);
 
-/* Tile shape of the persistent TENO5 launch: 16x16, 32x8 or 8x32 (the
- * direct launches take 16x16 only: HTX_ENOTSUP for the others). */
+/* Tile shape of the persistent TENO5 launch: 16x16, 32x8 or 8x32.  The
+ * direct launches always use the single-problem kernels' default tile:
+ * 16x16 is accepted for them and changes nothing, the others return
+ * HTX_ENOTSUP. */
 int htx_batch_set_tile(htx_batch* b, int tile_x, int tile_y);
 
 /* Autotuner: times `steps` steps of the batch with every built tile shape
  * (from the current state, which is restored afterwards), keeps the fastest
  * and returns it; ms (may be null) receives the per-step milliseconds of the
  * three shapes in the order 16x16, 32x8, 8x32 (-1 for a shape not timed:
- * batches with direct TENO5 launches time 16x16 only). */
+ * batches with direct TENO5 launches time their one shape, reported as
+ * 16x16). */
 int htx_batch_tune(htx_batch* b, int steps, int* tile_x, int* tile_y, double* ms);
 
 /* ------------------------------------------------------------------------

@@ -53,7 +53,8 @@ static void run(Fv2dArgs a, int flux, const unsigned char* fx = nullptr, const u
 
 // mixed: 0 FP64, 1 FP32-mixed, 2 learned with every face flagged smooth,
 // 3 FP32 bulk, 4 learned on the FP32 bulk arithmetic, every face flagged;
-// 5, 6, 7: FP64 with teno5i, weno5JS, weno5Z in place of TENO5
+// 5, 6, 7: FP64 with teno5i, weno5JS, weno5Z in place of TENO5; 8, 9: FP64 on
+// 16 x 15 and 15 x 16 tiles
 extern "C" int fv2dh_rhs_prec(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc,
                               int flux, int quirks, int mixed, const double* q, double* rhs) {
     Fv2dArgs a{};
@@ -74,7 +75,12 @@ extern "C" int fv2dh_rhs_prec(int nx, int ny, double xl, double xr, double yb, d
     a.area = a.dx * a.dy;
     fill_quad_consts(a.qc);
     int fk = flux == 0 ? 0 : (quirks ? 1 : 2);
-    if (mixed == 5)
+    if (mixed == 8)
+        run<16, 15>(a, fk);
+    else if (mixed == 9)
+        run<15, 16>(a, fk);
+
+    else if (mixed == 5)
         run<16, 16, false, false, double, 1>(a, fk);
     else if (mixed == 6)
         run<16, 16, false, false, double, 2>(a, fk);

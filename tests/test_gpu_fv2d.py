@@ -353,13 +353,14 @@ def test_consecutive_advances_match_one_trajectory(orc, scheme):
         assert err < TOL_END
 
 
-@pytest.mark.parametrize("precision", ["fp64_fast", "fp32_mixed"])
-@pytest.mark.parametrize("tile", [(16, 8), (32, 8)])
+@pytest.mark.parametrize("precision,tile", [("fp64_fast", (16, 8)), ("fp64_fast", (32, 8)), ("fp64_fast", (16, 16)),
+                                            ("fp32_mixed", (16, 8)), ("fp32_mixed", (32, 8)), ("fp32_mixed", (16, 16)),
+                                            ("fp64", (16, 16)), ("fp32", (16, 16)), ("fp32", (16, 8))])
 def test_tile_shapes_bit_identical(precision, tile):
     """Every face is reconstructed by one owner thread with the same
-    arithmetic whatever the tile, so the 16 x 8 and 32 x 8 tiles give
-    bit-identical residuals and steps to the default 16 x 16 tile, also on a
-    ragged grid that leaves partial tiles on both edges."""
+    arithmetic whatever the tile, so the 16 x 16, 16 x 8 and 32 x 8 tiles
+    give bit-identical residuals and steps to the default 16 x 15 tile, also
+    on a ragged grid that leaves partial tiles on both edges."""
     import htx
 
     nx, ny = 83, 61

@@ -486,3 +486,19 @@ def test_tiling_bitwise_reconstruction_variants(orc, harness, recon, mode, flux)
         np.testing.assert_array_equal(got, orc.fv2d_rhs(prm, q))
         if bc == "zero_grad":
             assert not np.array_equal(got, orc.fv2d_rhs(params(nx, ny, *dom, 1.4, bc, flux, True), q))
+
+
+@pytest.mark.parametrize("mode,shape", [(8, "16x15"), (9, "15x16")])
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+def test_tiling_bitwise_other_tile_shapes(orc, harness, mode, shape, flux):
+    """The default 16 x 15 tile (and its transpose) of the strict build:
+    residual bit-identical to the oracle on ragged grids of both boundary
+    conditions, including grids smaller than one tile."""
+    from htx.ic import vortex_modes
+
+    for nx, ny, bc in [(48, 40, "periodic"), (33, 31, "zero_grad"), (17, 46, "periodic"), (5, 3, "zero_grad"),
+                       (40, 70, "zero_grad")]:
+        dom = (0.0, 10.0, 0.0, 10.0)
+        prm = params(nx, ny, *dom, 1.4, bc, flux, True)
+        q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
+        np.testing.assert_array_equal(_rhs_prec(harness, nx, ny, dom, bc, flux, mode, q), orc.fv2d_rhs(prm, q))
