@@ -8,7 +8,6 @@
 #define HTX_NS htx_strict
 #define HTX_EXACT_DIVISION 1
 #include "htx_fv2d.hip"
-#include "htx_fvh2d.hip"
 #include "htx_fv1d.hip"
 
 // entry points for htx_api.hip, which only sees the fast build's types (the
@@ -25,9 +24,6 @@ hipError_t htx_strict_launch_max_speed(const double* q, int nx, int ny, int pitc
                                         s);
 }
 
-hipError_t htx_strict_launch_fvh2d_stage(const void* a, int flux, hipStream_t s) {
-    return htx_strict::launch_fvh2d_stage(*static_cast<const htx_strict::FvhArgs*>(a), flux, s);
-}
 
 // the 1D path (built here only); a points to an Fv1dArgs, whose layout does
 // not depend on the build

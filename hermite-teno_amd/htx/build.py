@@ -28,8 +28,8 @@ LIBORACLE = os.path.join(ORACLE, "liboracle.so")
 LIBORACLE_OMP = os.path.join(ORACLE, "liboracle_omp.so")
 LIBORACLE_COUNT = os.path.join(ORACLE, "liboracle_count.so")
 
-HIP_SOURCES = ["htx_fv2d.hip", "htx_fvh2d.hip", "htx_strict.hip", "htx_ic.hip", "htx_learn.hip", "htx_batch.hip",
-               "htx_api.hip", "htx_noalloc.hip", "htx_classes.hip"]
+HIP_SOURCES = ["htx_fv2d.hip", "htx_fvh2d.hip", "htx_strict.hip", "htx_strict_fvh.hip", "htx_ic.hip", "htx_learn.hip",
+               "htx_batch.hip", "htx_api.hip", "htx_noalloc.hip", "htx_classes.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -62,6 +62,7 @@ def _run(cmd, cwd=None):
 # (the classifier's matrix-core products write ordinary VGPRs, which spares
 # the accumulator-register reads before the GELU)
 PER_FILE_FLAGS = {"htx_strict.hip": ["-ffp-contract=off"], "htx_ic.hip": ["-ffp-contract=off"],
+                  "htx_strict_fvh.hip": ["-ffp-contract=off", "-mllvm", "-disable-machine-licm"],
                   "htx_classes.hip": ["-ffp-contract=off"],
                   "htx_learn.hip": ["-ffp-contract=off", "-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
