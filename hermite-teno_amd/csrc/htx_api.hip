@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/htx.h"
+#include "htx_schemes.h"
 #include "htx_fv1d.h"
 #include "htx_hermite.h"
 #include "htx_learn.h"
@@ -118,20 +119,21 @@ int check_params(const htx_params* p) {
     if (p->nx <= 0 || (p->dim == 2 && p->ny <= 0)) return fail(HTX_EINVAL, "grid size must be positive");
     if (!(p->xr > p->xl) || (p->dim == 2 && !(p->yt > p->yb)))
         return fail(HTX_EINVAL, "domain must have positive extent");
+    if (p->scheme < HTX_SCHEME_TENO5 || p->scheme > HTX_SCHEME_HLINEAR) return fail(HTX_EINVAL, "unknown scheme");
     if (p->dim == 1) {
-        // FV::System1D: TENO5, the shipped TENO5i, WENO5-JS or WENO5-Z, FP64
-        if (p->scheme != HTX_SCHEME_TENO5 && p->scheme != HTX_SCHEME_TENO5I && p->scheme != HTX_SCHEME_WENO5JS &&
-            p->scheme != HTX_SCHEME_WENO5Z)
-            return fail(HTX_EINVAL, "1D scheme must be teno5, teno5i, weno5js or weno5z");
+        // FV::System1D: TENO5, the shipped TENO5i or any other 5-point
+        // reconstruction of Schemes::Lagrange, FP64
+        if (htx_is_hermite(p->scheme))
+            return fail(HTX_EINVAL, "the 1D solver takes a 5-point reconstruction (teno5, teno5i, weno5js, ...)");
         if (p->precision != HTX_PREC_FP64) return fail(HTX_ENOTSUP, "the 1D path is built in fp64 precision");
-    } else if ((p->scheme == HTX_SCHEME_TENO5I || p->scheme == HTX_SCHEME_WENO5JS || p->scheme == HTX_SCHEME_WENO5Z) &&
+    } else if (p->scheme != HTX_SCHEME_TENO5 && p->scheme != HTX_SCHEME_HTENO &&
                p->precision != HTX_PREC_FP64 && p->precision != HTX_PREC_FP64_FAST) {
         // the 2D path takes them in place of TENO5 (FV_System2D.cpp:379, :394)
-        return fail(HTX_ENOTSUP, "the 2D teno5i, weno5js and weno5z reconstructions are built in fp64 and fp64_fast");
+        // or of HTENO (FVHermite_System2D.cpp:459, :486)
+        return fail(HTX_ENOTSUP, "the alternative 2D reconstructions are built in fp64 and fp64_fast");
     }
     if (!(p->gamma > 1.0)) return fail(HTX_EINVAL, "gamma must exceed 1");
     if (!(p->cfl > 0.0)) return fail(HTX_EINVAL, "cfl must be positive");
-    if (p->scheme < HTX_SCHEME_TENO5 || p->scheme > HTX_SCHEME_WENO5Z) return fail(HTX_EINVAL, "unknown scheme");
     if (p->flux != HTX_FLUX_RUSANOV_CHAR && p->flux != HTX_FLUX_HLLC) return fail(HTX_EINVAL, "unknown flux");
     if (p->bc != HTX_BC_ZERO_GRAD && p->bc != HTX_BC_PERIODIC) return fail(HTX_EINVAL, "unknown bc");
     if (p->precision < HTX_PREC_FP64 || p->precision > HTX_PREC_LEARNED_FP32)
@@ -159,10 +161,7 @@ Fv2dArgs stage_args(htx_solver* h, const double* qin, const double* q0, double* 
     a.mixed = (h->p.precision == HTX_PREC_FP32_MIXED || is_learned(h->p.precision) || is_bulk32(h->p.precision)) ? 1 : 0;
     a.learned = is_learned(h->p.precision) ? 1 : 0;
     a.bulk32 = is_bulk32(h->p.precision) ? 1 : 0;
-    a.recon = h->p.scheme == HTX_SCHEME_TENO5I    ? 1
-              : h->p.scheme == HTX_SCHEME_WENO5JS ? 2
-              : h->p.scheme == HTX_SCHEME_WENO5Z  ? 3
-                                                  : 0;
+    a.recon = htx_recon_code(h->p.scheme);
     a.face_flags_x = h->dFlagsX;
     a.face_flags_y = h->dFlagsY;
     a.gam = h->p.gamma;
@@ -200,6 +199,7 @@ FvhArgs hermite_args(htx_solver* h, const double* qin, const double* q0, double*
     a.quirks = h->p.ref_quirks ? 1 : 0;
     a.flux = h->p.flux;
     a.mixed = h->p.precision == HTX_PREC_FP32_MIXED;
+    a.hrecon = htx_hrecon_code(h->p.scheme);
     a.gam = h->p.gamma;
     a.g1 = h->p.gamma - 1.0;
     a.dx = h->dx;
@@ -253,16 +253,13 @@ int launch_stage(htx_solver* h, const double* qin, const double* q0, double* qou
         a.nx = h->nx;
         a.bc = h->p.bc;
         a.mode = mode;
-        a.recon = h->p.scheme == HTX_SCHEME_TENO5I    ? 1
-                  : h->p.scheme == HTX_SCHEME_WENO5JS ? 2
-                  : h->p.scheme == HTX_SCHEME_WENO5Z  ? 3
-                                                      : 0;
+        a.recon = htx_recon_code(h->p.scheme);
         a.flux = flux_kind(h);
         a.gam = h->p.gamma;
         a.g1 = h->p.gamma - 1.0;
         a.dx = h->dx;
         HIP_TRY(htx_strict_launch_fv1d_stage(&a, h->stream));
-    } else if (h->p.scheme == HTX_SCHEME_HTENO) {
+    } else if (htx_is_hermite(h->p.scheme)) {
         FvhArgs a = hermite_args(h, qin, q0, qout, mode);
         if (strict)
             HIP_TRY(htx_strict_launch_fvh2d_stage(&a, flux_kind(h), h->stream));
@@ -375,7 +372,7 @@ int htx_create(const htx_params* p, htx_solver** out) {
     h->p = *p;
     h->nx = p->nx;
     h->ny = p->dim == 1 ? 1 : p->ny;
-    h->nfield = p->dim == 1 ? 3 : (p->scheme == HTX_SCHEME_HTENO ? 12 : 4);
+    h->nfield = p->dim == 1 ? 3 : (htx_is_hermite(p->scheme) ? 12 : 4);
     h->pitch = p->nx;
     h->fstride = (long long)h->pitch * h->ny;
     h->dx = (p->xr - p->xl) / p->nx;
@@ -405,7 +402,7 @@ int htx_create(const htx_params* p, htx_solver** out) {
         htx_destroy(h);
         return fail(HTX_ENOMEM, "device allocation failed");
     }
-    if (p->scheme == HTX_SCHEME_HTENO) {
+    if (htx_is_hermite(p->scheme)) {
         size_t nxf = (size_t)(p->nx + 1) * p->ny, nyf = (size_t)p->nx * (p->ny + 1);
         if (hipMalloc(&h->dLV, sizeof(double) * 8 * (size_t)h->fstride) != hipSuccess ||
             hipMalloc(&h->dFX, sizeof(double) * 24 * nxf) != hipSuccess ||
@@ -502,7 +499,7 @@ int htx_init_vortex(htx_solver* h, double cx, double cy, const double* modes, in
     for (int m = 0; m < nmodes; ++m)
         for (int k = 0; k < 6; ++k) a.modes[m][k] = modes[6 * m + k];
     gauss9(a.gx, a.gw);
-    if (h->p.scheme == HTX_SCHEME_HTENO)
+    if (htx_is_hermite(h->p.scheme))
         HIP_TRY(launch_ic_vortex_hermite(a, h->stream));
     else
         HIP_TRY(launch_ic_vortex(a, h->stream));

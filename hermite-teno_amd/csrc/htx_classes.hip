@@ -21,11 +21,6 @@
 
 namespace {
 
-[[noreturn]] void not_built(const char* what) {
-    throw std::logic_error(std::string(what) +
-                           ": not built (no configuration of the rebuilt Hermite-TENO path uses it)");
-}
-
 void check_dir(int dir) {
     if (dir != 1 && dir != 2) throw std::invalid_argument("Euler2D: dir must be 1 (x) or 2 (y)");
 }
@@ -663,32 +658,76 @@ Vector3d Lagrange::weno5JS3PointsQuadrature(const Vector5d& u) const {
     return o;
 }
 
-Vector2d Lagrange::weno3JS(const Vector4d&) const { not_built("Lagrange::weno3JS"); }
-Vector2d Lagrange::weno3JS2PointsQuadrature(const Vector3d&) const { not_built("Lagrange::weno3JS2PointsQuadrature"); }
-Vector2d Lagrange::weno5Z(const Vector6d&, const double&) const { not_built("Lagrange::weno5Z(u, diff)"); }
-Vector2d Lagrange::weno5N(const Vector6d&) const { not_built("Lagrange::weno5N"); }
-Vector2d Lagrange::teno5a(const Vector6d&) const { not_built("Lagrange::teno5a"); }
-Vector2d Lagrange::teno5(const Vector6d&, const double&) const { not_built("Lagrange::teno5(u, diff)"); }
-Vector2d Lagrange::teno5N(const Vector6d&) const { not_built("Lagrange::teno5N"); }
-Vector2d Lagrange::weno5IS(const Vector6d&) const { not_built("Lagrange::weno5IS"); }
-Vector2d Lagrange::weno5ISmod(const Vector6d&, const double&) const { not_built("Lagrange::weno5ISmod"); }
-Vector2d Lagrange::weno5IS_Extended(const Vector6d&) const { not_built("Lagrange::weno5IS_Extended"); }
-Vector2d Lagrange::weno5JS2PointsQuadrature(const Vector5d&) const { not_built("Lagrange::weno5JS2PointsQuadrature"); }
-Vector2d Lagrange::weno5Z2PointsQuadrature(const Vector5d&) const { not_built("Lagrange::weno5Z2PointsQuadrature"); }
-Vector2d Lagrange::teno5th2PointsQuadrature(const Vector5d&) const { not_built("Lagrange::teno5th2PointsQuadrature"); }
-Vector4d Lagrange::weno5JS4PointsGaussLobatto(const Vector5d&) const {
-    not_built("Lagrange::weno5JS4PointsGaussLobatto");
+// The remaining reconstructions and quadratures (htx_variants.h, each citing
+// its reference routine)
+Vector2d Lagrange::weno3JS(const Vector4d& u) const {
+    double a, b;
+    htx_host::weno3js(u.data(), a, b);
+    return pair(a, b);
 }
-Vector4d Lagrange::weno5Z4PointsGaussLobatto(const Vector5d&) const { not_built("Lagrange::weno5Z4PointsGaussLobatto"); }
-Vector4d Lagrange::teno5th4PointsGaussLobatto(const Vector5d&) const {
-    not_built("Lagrange::teno5th4PointsGaussLobatto");
+
+Vector2d Lagrange::weno3JS2PointsQuadrature(const Vector3d& u) const {
+    double a, b;
+    htx_host::weno3js_2q(u.data(), a, b);
+    return pair(a, b);
 }
-Vector4d Lagrange::weno5JS4PointsGaussLegendre(const Vector5d&) const {
-    not_built("Lagrange::weno5JS4PointsGaussLegendre");
+
+static Vector2d recon_variant(int kind, const Vector6d& u, double diff) {
+    double um, up;
+    htx_host::recon5_variant(kind, u.data(), diff, um, up);
+    return pair(um, up);
 }
-Vector4d Lagrange::weno5Z4PointsGaussLegendre(const Vector5d&) const {
-    not_built("Lagrange::weno5Z4PointsGaussLegendre");
+
+Vector2d Lagrange::weno5Z(const Vector6d& u, const double& diff) const {
+    return recon_variant(htx_host::RECON_WENO5Z_DIFF, u, diff);
 }
+Vector2d Lagrange::weno5N(const Vector6d& u) const { return recon_variant(htx_host::RECON_WENO5N, u, 0.0); }
+Vector2d Lagrange::teno5a(const Vector6d& u) const { return recon_variant(htx_host::RECON_TENO5A, u, 0.0); }
+Vector2d Lagrange::teno5(const Vector6d& u, const double& diff) const {
+    return recon_variant(htx_host::RECON_TENO5_DIFF, u, diff);
+}
+Vector2d Lagrange::teno5N(const Vector6d& u) const { return recon_variant(htx_host::RECON_TENO5N, u, 0.0); }
+Vector2d Lagrange::weno5IS(const Vector6d& u) const { return recon_variant(htx_host::RECON_WENO5IS, u, 0.0); }
+Vector2d Lagrange::weno5ISmod(const Vector6d& u, const double& diff) const {
+    return recon_variant(htx_host::RECON_WENO5ISMOD, u, diff);
+}
+Vector2d Lagrange::weno5IS_Extended(const Vector6d& u) const {
+    return recon_variant(htx_host::RECON_WENO5IS_EXT, u, 0.0);
+}
+
+static const htx_host::QuadVar& quad_var() {
+    static const htx_host::QuadVar Q = [] {
+        htx_host::QuadVar q;
+        htx_host::fill_quad_var(q);
+        return q;
+    }();
+    return Q;
+}
+
+static Vector2d quad2(int kind, const Vector5d& u) {
+    double a, b;
+    htx_host::weno5_2q(quad_var(), kind, u.data(), a, b);
+    return pair(a, b);
+}
+
+Vector2d Lagrange::weno5JS2PointsQuadrature(const Vector5d& u) const { return quad2(0, u); }
+Vector2d Lagrange::weno5Z2PointsQuadrature(const Vector5d& u) const { return quad2(1, u); }
+Vector2d Lagrange::teno5th2PointsQuadrature(const Vector5d& u) const { return quad2(2, u); }
+
+static Vector4d quad4(bool lobatto, int kind, const Vector5d& u) {
+    Vector4d o;
+    if (lobatto)
+        htx_host::weno5_4lobatto(quad_var(), kind, u.data(), o.data());
+    else
+        htx_host::weno5_4legendre(quad_var(), kind, u.data(), o.data());
+    return o;
+}
+
+Vector4d Lagrange::weno5JS4PointsGaussLobatto(const Vector5d& u) const { return quad4(true, 0, u); }
+Vector4d Lagrange::weno5Z4PointsGaussLobatto(const Vector5d& u) const { return quad4(true, 1, u); }
+Vector4d Lagrange::teno5th4PointsGaussLobatto(const Vector5d& u) const { return quad4(true, 2, u); }
+Vector4d Lagrange::weno5JS4PointsGaussLegendre(const Vector5d& u) const { return quad4(false, 0, u); }
+Vector4d Lagrange::weno5Z4PointsGaussLegendre(const Vector5d& u) const { return quad4(false, 1, u); }
 
 // ---------------------------------------------------------------------------
 // Schemes::Hermite (Schemes_Hermite.cpp)
@@ -727,12 +766,43 @@ bool Hermite::KXRCF(const double& uj, const double& unbj, const double& unorm) c
     return ik > 1.0;
 }
 
-Vector4d Hermite::hweno(const Vector4d&, const Vector4d&) const { not_built("Hermite::hweno"); }
-Vector4d Hermite::linear(const Vector4d&, const Vector4d&) const { not_built("Hermite::linear"); }
-Vector4d Hermite::htenoi(const Vector4d&, const Vector4d&) const { not_built("Hermite::htenoi"); }
-Vector4d Hermite::hweno2PointsGLQ(const Vector3d&, const Vector3d&) const { not_built("Hermite::hweno2PointsGLQ"); }
-Matrix<double, 8, 1> Hermite::hweno4PointsGLQ(const Vector3d&, const Vector3d&) const {
-    not_built("Hermite::hweno4PointsGLQ");
+Vector4d Hermite::hweno(const Vector4d& u, const Vector4d& d) const {
+    const htx_host::HermConsts H = herm_consts(dif);
+    Vector4d o;
+    htx_host::hweno(H, u.data(), d.data(), o.data());
+    return o;
+}
+
+Vector4d Hermite::linear(const Vector4d& u, const Vector4d& d) const {
+    const htx_host::HermConsts H = herm_consts(dif);
+    Vector4d o;
+    htx_host::herm_linear(H, u.data(), d.data(), o.data());
+    return o;
+}
+
+Vector4d Hermite::htenoi(const Vector4d& u, const Vector4d& d) const {
+    const htx_host::HermConsts H = herm_consts(dif);
+    Vector4d o;
+    htx_host::htenoi(H, u.data(), d.data(), o.data());
+    return o;
+}
+
+Vector4d Hermite::hweno2PointsGLQ(const Vector3d& u, const Vector3d& d) const {
+    (void)herm_consts(dif);
+    htx_host::HermGLQ G;
+    htx_host::fill_herm_glq(G, dif);
+    Vector4d o;
+    htx_host::hweno_2q(G, dif, u.data(), d.data(), o.data());
+    return o;
+}
+
+Matrix<double, 8, 1> Hermite::hweno4PointsGLQ(const Vector3d& u, const Vector3d& d) const {
+    const htx_host::HermConsts H = herm_consts(dif);
+    htx_host::HermGLQ G;
+    htx_host::fill_herm_glq(G, dif);
+    Matrix<double, 8, 1> o;
+    htx_host::hweno_4q(H, G, u.data(), d.data(), o.data());
+    return o;
 }
 
 }  // namespace Schemes

@@ -13,7 +13,8 @@ struct Fv1dArgs {
     double* qout;       // residual (mode 0) or the stage result
     Ctl* ctl;
     int nx, bc, mode;   // bc: 0 zero-gradient, 1 periodic
-    int recon;          // 0 teno5, 1 teno5i (the reference's shipped 1D scheme), 2 weno5JS, 3 weno5Z
+    int recon;          // 0 teno5, 1 teno5i (the reference's shipped 1D scheme), 2 weno5JS, 3 weno5Z,
+                        // RECON_WENO5N .. RECON_LAST the variants of recon5_variant (diff = dx)
     int flux;           // FLUX_RUSANOV_CHAR, FLUX_HLLC (quirks), FLUX_HLLC_TORO
     double gam, g1, dx;
 };
@@ -155,6 +156,8 @@ HTX_HD void fv1d_face_flux(const Fv1dArgs& a, int c, double F[3]) {
             teno5i(C[e], ul[e], ur[e]);
         else if (a.recon == 2 || a.recon == 3)
             weno5(C[e], a.recon == 3, ul[e], ur[e]);
+        else if (a.recon >= RECON_WENO5N)
+            recon5_variant(a.recon, C[e], a.dx, ul[e], ur[e]);
         else
             teno5(C[e], ul[e], ur[e]);
     }

@@ -175,6 +175,8 @@ static hipError_t launch_prec(const Fv2dArgs& a, int flux, hipStream_t s) {
             if (a.recon == 1) return launch_tile<TX, TY, false, false, false, 1>(a, flux, s);
             if (a.recon == 2) return launch_tile<TX, TY, false, false, false, 2>(a, flux, s);
             if (a.recon == 3) return launch_tile<TX, TY, false, false, false, 3>(a, flux, s);
+            if (a.recon >= RECON_WENO5N && a.recon <= RECON_LAST)
+                return launch_tile<TX, TY, false, false, false, RECON_WENO5N>(a, flux, s);
         }
         return hipErrorInvalidValue;
     }

@@ -208,6 +208,8 @@ HTX_HD void ph_recon(const Fv2dArgs& a, Fv2dTile<TX, TY, FT, OF>& s, int tid, in
                     teno5i(c[f], um, up);
                 else if (RECON == 2 || RECON == 3)
                     weno5(c[f], RECON == 3, um, up);
+                else if (RECON == RECON_WENO5N)  // the other variants, chosen at run time
+                    recon5_variant(a.recon, c[f], YDIR ? a.dy : a.dx, um, up);
                 else
                     teno5(c[f], um, up);
                 s.sF[2 * f][w] = um;

@@ -26,22 +26,22 @@ __device__ __forceinline__ double wave_max_fvh(double v) {
 // The limiter: fvh_limit_block (htx_hermite.h), 64 chains per workgroup.
 __global__ void __launch_bounds__(LIM_THREADS) k_fvh_limit(FvhArgs a, int g0) { fvh_limit_block(a, blockIdx.x, g0); }
 
-template <bool MIXED>
+template <bool MIXED, bool VARIANT = false>
 __global__ void __launch_bounds__(256) k_fvh_recon_x(FvhArgs a) {
     long long nf = (long long)(a.nx + 1) * a.ny;
     long long t = blockIdx.x * 256ll + threadIdx.x;
     if (t >= nf) return;
     int fi = (int)(t % (a.nx + 1)), j = (int)(t / (a.nx + 1));
-    fvh_recon_face<false, MIXED>(a, fi - 1, j, a.fx, t, nf);
+    fvh_recon_face<false, MIXED, VARIANT>(a, fi - 1, j, a.fx, t, nf);
 }
 
-template <bool MIXED>
+template <bool MIXED, bool VARIANT = false>
 __global__ void __launch_bounds__(256) k_fvh_recon_y(FvhArgs a) {
     long long nf = (long long)a.nx * (a.ny + 1);
     long long t = blockIdx.x * 256ll + threadIdx.x;
     if (t >= nf) return;
     int i = (int)(t % a.nx), fj = (int)(t / a.nx);
-    fvh_recon_face<true, MIXED>(a, i, fj - 1, a.fy, t, nf);
+    fvh_recon_face<true, MIXED, VARIANT>(a, i, fj - 1, a.fy, t, nf);
 }
 
 // block (64 faces) x (3 Gauss points)
@@ -152,8 +152,13 @@ __global__ void __launch_bounds__(256) k_ic_vortex_hermite(VortexArgs a) {
 template <bool MIXED>
 static void launch_recon_flux(const FvhArgs& a, int flux, hipStream_t s) {
     long long nxf = (long long)(a.nx + 1) * a.ny, nyf = (long long)a.nx * (a.ny + 1);
-    hipLaunchKernelGGL(k_fvh_recon_x<MIXED>, dim3((unsigned)((nxf + 255) / 256)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_fvh_recon_y<MIXED>, dim3((unsigned)((nyf + 255) / 256)), dim3(256), 0, s, a);
+    if (!MIXED && a.hrecon != 0) {
+        hipLaunchKernelGGL((k_fvh_recon_x<false, true>), dim3((unsigned)((nxf + 255) / 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_fvh_recon_y<false, true>), dim3((unsigned)((nyf + 255) / 256)), dim3(256), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(k_fvh_recon_x<MIXED>, dim3((unsigned)((nxf + 255) / 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_fvh_recon_y<MIXED>, dim3((unsigned)((nyf + 255) / 256)), dim3(256), 0, s, a);
+    }
     dim3 fb(64, 3), gx((unsigned)((nxf + 63) / 64)), gy((unsigned)((nyf + 63) / 64));
     if (flux == FLUX_RUSANOV_CHAR) {
         hipLaunchKernelGGL((k_fvh_flux<false, FLUX_RUSANOV_CHAR, MIXED>), gx, fb, 0, s, a);

@@ -5,6 +5,7 @@
 // and a CPU build of them run the same code.
 #pragma once
 #include "htx_internal.h"
+#include "htx_variants.h"
 
 namespace HTX_NS {
 
@@ -162,6 +163,308 @@ HTX_HD void hteno(const HermConsts& H, const double u[4], const double d[4], dou
                         H.dkpd[2]);
 }
 
+// The four candidate sets of a Hermite face (the same for hteno, hweno and
+// htenoi): u^- (p[0]), u^+ (p[1]), d^- (p[2]), d^+ (p[3]) at x_{i+1/2} from
+// u[0..3] = u_{i-1..i+2} and their derivative moments d
+// (Schemes_Hermite.cpp:240-290)
+HTX_HD void herm_cands(const HermConsts& H, const double u[4], const double d[4], double p[4][3]) {
+    HTX_NO_CONTRACT
+    p[0][0] = H.crj[0][0] * u[0] + H.crj[1][0] * u[1] + H.crj[2][0] * d[0];
+    p[0][1] = H.crj[0][1] * u[1] + H.crj[1][1] * u[2] + H.crj[2][1] * d[2];
+    p[0][2] = H.crj[0][2] * u[0] + H.crj[1][2] * u[1] + H.crj[2][2] * u[2];
+    p[1][0] = H.crj[1][1] * u[1] + H.crj[0][1] * u[2] - H.crj[2][1] * d[1];
+    p[1][1] = H.crj[1][0] * u[2] + H.crj[0][0] * u[3] - H.crj[2][0] * d[3];
+    p[1][2] = H.crj[2][2] * u[1] + H.crj[1][2] * u[2] + H.crj[0][2] * u[3];
+    p[2][0] = H.crjd[0][0] * u[0] + H.crjd[1][0] * u[1] + H.crjd[2][0] * d[0] + H.crjd[3][0] * d[1];
+    p[2][1] = H.crjd[0][1] * u[1] + H.crjd[1][1] * u[2] + H.crjd[2][1] * d[1] + H.crjd[3][1] * d[2];
+    p[2][2] = H.crjd[0][2] * u[0] + H.crjd[1][2] * u[1] + H.crjd[2][2] * u[2] + H.crjd[3][2] * d[1];
+    p[3][0] = H.crjd[0][1] * u[1] + H.crjd[1][1] * u[2] + H.crjd[3][1] * d[1] + H.crjd[2][1] * d[2];
+    p[3][1] = H.crjd[0][0] * u[2] + H.crjd[1][0] * u[3] + H.crjd[3][0] * d[2] + H.crjd[2][0] * d[3];
+    p[3][2] = -(H.crjd[2][2] * u[1] + H.crjd[1][2] * u[2] + H.crjd[0][2] * u[3]) + H.crjd[3][2] * d[2];
+}
+
+// Hermite::hweno (Schemes_Hermite.cpp:234-293): the candidates of hteno with
+// WENO-JS weights d_k / (s_k + 1e-6)^2, normalised, then catchnan.
+HTX_HD void hweno(const HermConsts& H, const double u[4], const double d[4], double out[4]) {
+    HTX_NO_CONTRACT
+    const double dif = H.dif;
+    const double ud0 = d[0] * dif, ud1 = d[1] * dif, ud2 = d[2] * dif, ud3 = d[3] * dif;
+    double p[4][3], s[3], w[3];
+    herm_cands(H, u, d, p);
+    for (int o = 0; o < 4; ++o) {
+        const int b = o & 1;  // 0: cells i-1..i+1, 1: cells i..i+2
+        if (o < 2)
+            herm_si(u[b], u[b + 1], u[b + 2], b ? ud1 : ud0, b ? ud3 : ud2, s[0], s[1], s[2]);
+        else
+            herm_sid(u[b], u[b + 1], u[b + 2], b ? ud1 : ud0, b ? ud2 : ud1, b ? ud3 : ud2, s[0], s[1], s[2]);
+        const double* dk = o == 0 ? H.dkm : o == 1 ? H.dkp : o == 2 ? H.dkmd : H.dkpd;
+        for (int k = 0; k < 3; ++k) {
+            const double e = s[k] + 1e-6;
+            w[k] = dk[k] / (e * e);
+        }
+        const double ws = (w[0] + w[1]) + w[2];
+        for (int k = 0; k < 3; ++k) w[k] = w[k] / ws;
+        const double r = (p[o][0] * w[0] + p[o][1] * w[1]) + p[o][2] * w[2];
+        out[o] = r != r ? 1e-15 : r;
+    }
+}
+
+// Hermite::htenoi (Schemes_Hermite.cpp:670-755): hteno's candidates with tau
+// the largest smoothness measure, weights (1 + tau / (s_k + 1e-40))^6 cut off
+// at 1e-3 (cutOff / cutOffd, :614-628), kept linear weights normalised,
+// catchnan.  The sixth power is rounded once (pow_n).
+HTX_HD void htenoi(const HermConsts& H, const double u[4], const double d[4], double out[4]) {
+    HTX_NO_CONTRACT
+    const double dif = H.dif;
+    const double ud0 = d[0] * dif, ud1 = d[1] * dif, ud2 = d[2] * dif, ud3 = d[3] * dif;
+    double p[4][3], s[3], g[3], w[3];
+    herm_cands(H, u, d, p);
+    for (int o = 0; o < 4; ++o) {
+        const int b = o & 1;
+        if (o < 2)
+            herm_si(u[b], u[b + 1], u[b + 2], b ? ud1 : ud0, b ? ud3 : ud2, s[0], s[1], s[2]);
+        else
+            herm_sid(u[b], u[b + 1], u[b + 2], b ? ud1 : ud0, b ? ud2 : ud1, b ? ud3 : ud2, s[0], s[1], s[2]);
+        double tau = s[0];
+        if (s[1] > tau) tau = s[1];
+        if (s[2] > tau) tau = s[2];
+        for (int k = 0; k < 3; ++k) g[k] = pow_n(1.0 + tau / (s[k] + 1e-40), 6);
+        const double gs = (g[0] + g[1]) + g[2];
+        const double* dk = o == 0 ? H.dkm : o == 1 ? H.dkp : o == 2 ? H.dkmd : H.dkpd;
+        for (int k = 0; k < 3; ++k) w[k] = (g[k] / gs < 1e-3 ? 0.0 : 1.0) * dk[k];
+        const double ws = (w[0] + w[1]) + w[2];
+        for (int k = 0; k < 3; ++k) w[k] = w[k] / ws;
+        const double r = (p[o][0] * w[0] + p[o][1] * w[1]) + p[o][2] * w[2];
+        out[o] = r != r ? 1e-15 : r;
+    }
+}
+
+// Hermite::linear (Schemes_Hermite.cpp:757-767 with crjlin / crjlind of
+// setGridSize :67-92): the optimal 5th-order Hermite interpolants
+// (u^-, u^+, d^-, d^+).
+HTX_HD void herm_linear(const HermConsts& H, const double u[4], const double d[4], double out[4]) {
+    HTX_NO_CONTRACT
+    const double dif = H.dif;
+    out[0] = ((-23.0 / 120.0) * u[0] + (19.0 / 30.0) * u[1]) + (67.0 / 120.0) * u[2] + (-3.0 / 40.0 * dif) * d[0] +
+             (-7.0 / 40.0 * dif) * d[2];
+    out[1] = ((67.0 / 120.0) * u[1] + (19.0 / 30.0) * u[2]) + (-23.0 / 120.0) * u[3] + (7.0 / 40.0 * dif) * d[1] +
+             (3.0 / 40.0 * dif) * d[3];
+    out[2] = (((0.25 / dif) * u[0] + (-2.0 / dif) * u[1]) + (1.75 / dif) * u[2]) +
+             (((1.0 / 12.0) * d[0] + (-1.0 / 6.0) * d[1]) + (-5.0 / 12.0) * d[2]);
+    out[3] = (((-7.0 / 4.0 / dif) * u[1] + (2.0 / dif) * u[2]) + (-0.25 / dif) * u[3]) +
+             (((-5.0 / 12.0) * d[1] + (-1.0 / 6.0) * d[2]) + (1.0 / 12.0) * d[3]);
+}
+
+// The Hermite face reconstruction of the 2D Hermite solver
+// (FVHermite_System2D.cpp:447-470 calls hteno): 0 hteno, 1 hweno, 2 htenoi,
+// 3 linear, each on (L u, L d_normal) of a characteristic field.
+HTX_HD void herm_recon(int kind, const HermConsts& H, const double u[4], const double d[4], double out[4]) {
+    if (kind == 1)
+        hweno(H, u, d, out);
+    else if (kind == 2)
+        htenoi(H, u, d, out);
+    else if (kind == 3)
+        herm_linear(H, u, d, out);
+    else
+        hteno<false>(H, u, d, out);
+}
+
+// Coefficients of the Hermite 2- and 4-point Gauss-Legendre quadratures
+// (setGridSize, Schemes_Hermite.cpp:93-231) for grid size dif, with the
+// reference's expressions.
+struct HermGLQ {
+    double crjgp[3][3], crjgm[3][3], crjdgp[4][3], crjdgm[4][3];
+    double dkpg[3], dkmg[3], dkpdg[3], dkmdg[3];
+    double crjg4p[3][3], crjg4m[3][3], crjdg4p[4][3], crjdg4m[4][3];
+    double dkpg4[3], dkmg4[3], dkpdg4[3], dkmdg4[3];
+    double c2, c3, c4, c5, c6, c7;
+};
+
+inline void fill_herm_glq(HermGLQ& G, double dif) {
+    const double sq3 = sqrt(3.0), sq5 = sqrt(5.0);
+    G.c2 = sqrt(3.0) / 3.0;
+    G.c3 = 37.0 * sqrt(3.0) / 108.0;
+    G.c4 = sqrt(3.0) / 3.0;
+    G.c5 = 37.0 * sqrt(3.0) / 108.0;
+    G.c6 = sqrt(5.0) / 5.0;
+    G.c7 = 61.0 * sqrt(5.0) / 300.0;
+    G.crjgp[0][0] = -sq3 / 3.0; G.crjgp[1][0] = 1.0 + sq3 / 3.0; G.crjgp[2][0] = -dif * sq3 / 6.0;
+    G.crjgp[0][1] = 1.0 - sq3 / 3.0; G.crjgp[1][1] = sq3 / 3.0; G.crjgp[2][1] = -dif * sq3 / 6.0;
+    G.crjgp[0][2] = -sq3 / 12.0; G.crjgp[1][2] = 1.0; G.crjgp[2][2] = sq3 / 12.0;
+    G.crjgm[0][0] = sq3 / 3.0; G.crjgm[1][0] = 1.0 - sq3 / 3.0; G.crjgm[2][0] = dif * sq3 / 6.0;
+    G.crjgm[0][1] = 1.0 + sq3 / 3.0; G.crjgm[1][1] = -sq3 / 3.0; G.crjgm[2][1] = dif * sq3 / 6.0;
+    G.crjgm[0][2] = sq3 / 12.0; G.crjgm[1][2] = 1.0; G.crjgm[2][2] = -sq3 / 12.0;
+    G.dkpg[0] = (-sq3 + 105.0) / 360.0; G.dkpg[1] = (sq3 + 105.0) / 360.0; G.dkpg[2] = 5.0 / 12.0;
+    G.dkmg[0] = (sq3 + 105.0) / 360.0; G.dkmg[1] = (-sq3 + 105.0) / 360.0; G.dkmg[2] = 5.0 / 12.0;
+    G.crjdgp[0][0] = sq3 / dif; G.crjdgp[1][0] = -sq3 / dif; G.crjdgp[2][0] = sq3 / 3.0;
+    G.crjdgp[3][0] = (2.0 * sq3 + 3.0) / 3.0;
+    G.crjdgp[0][1] = -sq3 / dif; G.crjdgp[1][1] = sq3 / dif; G.crjdgp[2][1] = (-2.0 * sq3 + 3.0) / 3.0;
+    G.crjdgp[3][1] = -sq3 / 3.0;
+    G.crjdgp[0][2] = sq3 / 6.0 / dif; G.crjdgp[1][2] = -sq3 / 3.0 / dif; G.crjdgp[2][2] = sq3 / 6.0 / dif;
+    G.crjdgp[3][2] = 1.0;
+    G.crjdgm[0][0] = -sq3 / dif; G.crjdgm[1][0] = sq3 / dif; G.crjdgm[2][0] = -sq3 / 3.0;
+    G.crjdgm[3][0] = (-2.0 * sq3 + 3.0) / 3.0;
+    G.crjdgm[0][1] = sq3 / dif; G.crjdgm[1][1] = -sq3 / dif; G.crjdgm[2][1] = (2.0 * sq3 + 3.0) / 3.0;
+    G.crjdgm[3][1] = sq3 / 3.0;
+    G.crjdgm[0][2] = -sq3 / 6.0 / dif; G.crjdgm[1][2] = sq3 / 3.0 / dif; G.crjdgm[2][2] = -sq3 / 6.0 / dif;
+    G.crjdgm[3][2] = 1.0;
+    G.dkpdg[0] = -sq3 / 144.0 + 1.0 / 3.0; G.dkpdg[1] = sq3 / 144.0 + 1.0 / 3.0; G.dkpdg[2] = 1.0 / 3.0;
+    G.dkmdg[0] = sq3 / 144.0 + 1.0 / 3.0; G.dkmdg[1] = -sq3 / 144.0 + 1.0 / 3.0; G.dkmdg[2] = 1.0 / 3.0;
+    G.crjg4p[0][0] = -sq5 / 5.0 + 1.0 / 30.0; G.crjg4p[1][0] = sq5 / 5.0 + 29.0 / 30.0;
+    G.crjg4p[2][0] = (-sq5 / 10.0 + 1.0 / 30.0) * dif;
+    G.crjg4p[0][1] = -sq5 / 5.0 + 29.0 / 30.0; G.crjg4p[1][1] = sq5 / 5.0 + 1.0 / 30.0;
+    G.crjg4p[2][1] = -(sq5 / 10.0 + 1.0 / 30.0) * dif;
+    G.crjg4p[0][2] = -(sq5 / 20.0 + 1.0 / 60.0); G.crjg4p[1][2] = 31.0 / 30.0; G.crjg4p[2][2] = sq5 / 20.0 - 1.0 / 60.0;
+    G.crjg4m[0][0] = sq5 / 5.0 + 1.0 / 30.0; G.crjg4m[1][0] = -sq5 / 5.0 + 29.0 / 30.0;
+    G.crjg4m[2][0] = (sq5 / 10.0 + 1.0 / 30.0) * dif;
+    G.crjg4m[0][1] = sq5 / 5.0 + 29.0 / 30.0; G.crjg4m[1][1] = -sq5 / 5.0 + 1.0 / 30.0;
+    G.crjg4m[2][1] = (sq5 / 10.0 - 1.0 / 30.0) * dif;
+    G.crjg4m[0][2] = sq5 / 20.0 - 1.0 / 60.0; G.crjg4m[1][2] = 31.0 / 30.0; G.crjg4m[2][2] = -(sq5 / 20.0 + 1.0 / 60.0);
+    double den = 330.0 * sq5 - 110.0;
+    G.dkpg4[0] = (99.0 * sq5 + 33.0) / den; G.dkpg4[1] = (108.0 * sq5 - 102.0) / den;
+    G.dkpg4[2] = (123.0 * sq5 - 41.0) / den;
+    den = 330.0 * sq5 + 110.0;
+    G.dkmg4[0] = (99.0 * sq5 - 33.0) / den; G.dkmg4[1] = (108.0 * sq5 + 102.0) / den;
+    G.dkmg4[2] = (123.0 * sq5 + 41.0) / den;
+    G.crjdg4p[0][0] = (6.0 * sq5 - 2.0) / 10.0 / dif; G.crjdg4p[1][0] = (-6.0 * sq5 + 2.0) / 10.0 / dif;
+    G.crjdg4p[2][0] = 0.2 * sq5 - 0.1; G.crjdg4p[3][0] = 0.4 * sq5 + 0.9;
+    G.crjdg4p[0][1] = -(6.0 * sq5 + 2.0) / 10.0 / dif; G.crjdg4p[1][1] = (6.0 * sq5 + 2.0) / 10.0 / dif;
+    G.crjdg4p[2][1] = -0.4 * sq5 + 0.9; G.crjdg4p[3][1] = -0.2 * sq5 - 0.1;
+    G.crjdg4p[0][2] = (2.0 * sq5 + 1.0) / 20.0 / dif; G.crjdg4p[1][2] = -sq5 / 5.0 / dif;
+    G.crjdg4p[2][2] = (2.0 * sq5 - 1.0) / 20.0 / dif; G.crjdg4p[3][2] = 1.1;
+    G.crjdg4m[0][0] = -(6.0 * sq5 + 2.0) / 10.0 / dif; G.crjdg4m[1][0] = (6.0 * sq5 + 2.0) / 10.0 / dif;
+    G.crjdg4m[2][0] = -0.2 * sq5 - 0.1; G.crjdg4m[3][0] = -0.4 * sq5 + 0.9;
+    G.crjdg4m[0][1] = (6.0 * sq5 - 2.0) / 10.0 / dif; G.crjdg4m[1][1] = (-6.0 * sq5 + 2.0) / 10.0 / dif;
+    G.crjdg4m[2][1] = 0.4 * sq5 + 0.9; G.crjdg4m[3][1] = 0.2 * sq5 - 0.1;
+    G.crjdg4m[0][2] = (-2.0 * sq5 + 1.0) / 20.0 / dif; G.crjdg4m[1][2] = sq5 / 5.0 / dif;
+    G.crjdg4m[2][2] = (-2.0 * sq5 - 1.0) / 20.0 / dif; G.crjdg4m[3][2] = 1.1;
+    den = 1140.0 * sq5 - 570.0;
+    G.dkpdg4[0] = (399.0 * sq5 + 190.0) / den; G.dkpdg4[1] = (481.0 * sq5 - 630.0) / den;
+    G.dkpdg4[2] = (260.0 * sq5 - 130.0) / den;
+    den = 1140.0 * sq5 + 570.0;
+    G.dkmdg4[0] = (399.0 * sq5 - 190.0) / den; G.dkmdg4[1] = (481.0 * sq5 + 630.0) / den;
+    G.dkmdg4[2] = (260.0 * sq5 + 130.0) / den;
+}
+
+// smoothnessMeasure2PointsGLQ / 4PointsGLQ (Schemes_Hermite.cpp:543-581) and
+// their derivative forms (:556-595): the squared slope and curvature terms
+// b_k^2, c_k^2 of the three Hermite stencils of u0..u2 with moments ud
+HTX_HD void herm_bc2(const double u[3], const double ud[3], bool deriv, double b2[3], double c2[3]) {
+    HTX_NO_CONTRACT
+    double b[3], c[3];
+    if (!deriv) {
+        b[0] = -2.0 * (u[0] - u[1]) - ud[0];
+        c[0] = -2.0 * (u[0] - u[1]) - 2.0 * ud[0];
+        b[1] = -2.0 * (u[1] - u[2]) - ud[2];
+        c[1] = 2.0 * (u[1] - u[2]) + 2.0 * ud[2];
+        b[2] = -(u[0] - u[2]) / 2.0;
+        c[2] = u[0] - 2.0 * u[1] + u[2];
+    } else {
+        b[0] = 6.0 * (u[0] - u[1]) + (2.0 * ud[0] + 4.0 * ud[1]);
+        c[0] = 12.0 * (u[0] - u[1]) + 6.0 * (ud[0] + ud[1]);
+        b[1] = 6.0 * (u[2] - u[1]) - (4.0 * ud[1] + 2.0 * ud[2]);
+        c[1] = 12.0 * (u[1] - u[2]) + 6.0 * (ud[1] + ud[2]);
+        b[2] = u[0] - 2.0 * u[1] + u[2];
+        c[2] = 3.0 * (u[2] - u[0]) - 6.0 * ud[1];
+    }
+    for (int k = 0; k < 3; ++k) {
+        b2[k] = b[k] * b[k];
+        c2[k] = c[k] * c[k];
+    }
+}
+
+// WENO-JS value of one point: weights d_k / (s_k + 1e-6)^2 normalised, no
+// catchnan (the quadrature routines apply none)
+HTX_HD double herm_js_point(const double p[3], const double dk[3], const double s[3]) {
+    HTX_NO_CONTRACT
+    double w[3];
+    for (int k = 0; k < 3; ++k) {
+        const double e = s[k] + 1e-6;
+        w[k] = dk[k] / (e * e);
+    }
+    const double ws = (w[0] + w[1]) + w[2];
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / ws;
+    return (p[0] * w[0] + p[1] * w[1]) + p[2] * w[2];
+}
+
+// u-type candidates at one point: C(0, k) u_{a} + C(1, k) u_{b} + C(2, k) m
+HTX_HD void herm_ucand(const double C[3][3], const double u[3], const double d[3], double p[3]) {
+    HTX_NO_CONTRACT
+    p[0] = C[0][0] * u[0] + C[1][0] * u[1] + C[2][0] * d[0];
+    p[1] = C[0][1] * u[1] + C[1][1] * u[2] + C[2][1] * d[2];
+    p[2] = C[0][2] * u[0] + C[1][2] * u[1] + C[2][2] * u[2];
+}
+
+// derivative-type candidates at one point (the crjd pattern of :270-276)
+HTX_HD void herm_dcand(const double C[4][3], const double u[3], const double d[3], double p[3]) {
+    HTX_NO_CONTRACT
+    p[0] = C[0][0] * u[0] + C[1][0] * u[1] + C[2][0] * d[0] + C[3][0] * d[1];
+    p[1] = C[0][1] * u[1] + C[1][1] * u[2] + C[2][1] * d[1] + C[3][1] * d[2];
+    p[2] = C[0][2] * u[0] + C[1][2] * u[1] + C[2][2] * u[2] + C[3][2] * d[1];
+}
+
+// Hermite::hweno2PointsGLQ (Schemes_Hermite.cpp:381-425): u at x_i + sqrt(3)/6
+// dx and x_i - sqrt(3)/6 dx, then the derivative moment at the same points,
+// from cells i-1..i+1
+HTX_HD void hweno_2q(const HermGLQ& G, double dif, const double u[3], const double d[3], double out[4]) {
+    HTX_NO_CONTRACT
+    const double ud[3] = {d[0] * dif, d[1] * dif, d[2] * dif};
+    double b2[3], c2[3], s[3], p[3];
+    herm_bc2(u, ud, false, b2, c2);
+    for (int k = 0; k < 3; ++k) s[k] = G.c2 * b2[k] + G.c3 * c2[k];
+    herm_ucand(G.crjgp, u, d, p);
+    out[0] = herm_js_point(p, G.dkpg, s);
+    herm_ucand(G.crjgm, u, d, p);
+    out[1] = herm_js_point(p, G.dkmg, s);
+    herm_bc2(u, ud, true, b2, c2);
+    for (int k = 0; k < 3; ++k) s[k] = G.c4 * b2[k] + G.c5 * c2[k];
+    herm_dcand(G.crjdgp, u, d, p);
+    out[2] = herm_js_point(p, G.dkpdg, s);
+    herm_dcand(G.crjdgm, u, d, p);
+    out[3] = herm_js_point(p, G.dkmdg, s);
+}
+
+// Hermite::hweno4PointsGLQ (Schemes_Hermite.cpp:427-513): u at x_{i+1/2},
+// x_{i-1/2}, x_i + sqrt(5)/10 dx, x_i - sqrt(5)/10 dx (out[0..3]), then the
+// derivative moment at the same four points (out[4..7])
+HTX_HD void hweno_4q(const HermConsts& H, const HermGLQ& G, const double u[3], const double d[3], double out[8]) {
+    HTX_NO_CONTRACT
+    const double dif = H.dif;
+    const double c1 = 13.0 / 12.0;
+    const double ud[3] = {d[0] * dif, d[1] * dif, d[2] * dif};
+    double b2[3], c2[3], s[3], s4[3], p[3];
+    herm_bc2(u, ud, false, b2, c2);
+    for (int k = 0; k < 3; ++k) {
+        s[k] = c1 * c2[k] + b2[k];
+        s4[k] = G.c6 * b2[k] + G.c7 * c2[k];
+    }
+    herm_ucand(H.crj, u, d, p);
+    out[0] = herm_js_point(p, H.dkm, s);
+    p[0] = H.crj[1][1] * u[0] + H.crj[0][1] * u[1] - H.crj[2][1] * d[0];
+    p[1] = H.crj[1][0] * u[1] + H.crj[0][0] * u[2] - H.crj[2][0] * d[2];
+    p[2] = H.crj[2][2] * u[0] + H.crj[1][2] * u[1] + H.crj[0][2] * u[2];
+    out[1] = herm_js_point(p, H.dkp, s);
+    herm_ucand(G.crjg4p, u, d, p);
+    out[2] = herm_js_point(p, G.dkpg4, s4);
+    herm_ucand(G.crjg4m, u, d, p);
+    out[3] = herm_js_point(p, G.dkmg4, s4);
+    herm_bc2(u, ud, true, b2, c2);
+    for (int k = 0; k < 3; ++k) {
+        s[k] = c1 * c2[k] + b2[k];
+        s4[k] = G.c6 * b2[k] + G.c7 * c2[k];
+    }
+    herm_dcand(H.crjd, u, d, p);
+    out[4] = herm_js_point(p, H.dkmd, s);
+    p[0] = H.crjd[0][1] * u[0] + H.crjd[1][1] * u[1] + H.crjd[3][1] * d[0] + H.crjd[2][1] * d[1];
+    p[1] = H.crjd[0][0] * u[1] + H.crjd[1][0] * u[2] + H.crjd[3][0] * d[1] + H.crjd[2][0] * d[2];
+    p[2] = -(H.crjd[2][2] * u[0] + H.crjd[1][2] * u[1] + H.crjd[0][2] * u[2]) + H.crjd[3][2] * d[1];
+    out[5] = herm_js_point(p, H.dkpd, s);
+    herm_dcand(G.crjdg4p, u, d, p);
+    out[6] = herm_js_point(p, G.dkpdg4, s4);
+    herm_dcand(G.crjdg4m, u, d, p);
+    out[7] = herm_js_point(p, G.dkmdg4, s4);
+}
+
 // Hermite::hermiteLimiter (Schemes_Hermite.cpp:634-662): u and derivative
 // moments of cells i-1, i, i+1 -> the limited moment of cell i.
 HTX_HD double herm_limiter(double dif, double u0, double u1, double u2, double d0, double d2) {
@@ -228,6 +531,7 @@ struct FvhArgs {
     long long fstride;   // per field of the state
     int bc, mode, quirks, flux;
     int mixed;           // 1: FP32 cut-off rules and transverse weights (fast build only)
+    int hrecon;          // face reconstruction (herm_recon): 0 hteno, 1 hweno, 2 htenoi, 3 linear
     double gam, g1, dx, dy, area;
     QuadConsts qc;
     HermConsts hc;
@@ -558,7 +862,7 @@ HTX_HD double fvh_lim(const FvhArgs& a, int comp, int i, int j) { return a.lv[co
 // characteristic face values at face index fid of the face buffer.  MIXED:
 // the HTENO and TENO5 cut-off rules in FP32 (hteno_side_mixed,
 // teno5_side_mixed), every value and projection in FP64.
-template <bool YDIR, bool MIXED = false>
+template <bool YDIR, bool MIXED = false, bool VARIANT = false>
 HTX_HD void fvh_recon_face(const FvhArgs& a, int i, int j, double* F, long long fid, long long fstr) {
     const long long fs = a.fstride;
     auto cellq = [&](int k, double q[4]) {  // k: offset along the normal
@@ -601,7 +905,10 @@ HTX_HD void fvh_recon_face(const FvhArgs& a, int i, int j, double* F, long long 
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         double o[4], um, up;
-        hteno<MIXED>(a.hc, A[e], B[e], o);
+        if (VARIANT)  // hweno, htenoi or linear in hteno's place (FVHermite_System2D.cpp:459, :486)
+            herm_recon(a.hrecon, a.hc, A[e], B[e], o);
+        else
+            hteno<MIXED>(a.hc, A[e], B[e], o);
         if (MIXED)
             teno5_mixed(C[e], um, up);
         else

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "htx_numerics.h"
+#include "htx_variants.h"
 
 namespace HTX_NS {
 
@@ -93,7 +94,8 @@ struct Fv2dArgs {
     int mixed;          // 1: cut-off decisions and nonlinear weights in FP32
     int learned;        // 1: classifier flags pick linear or TENO5 per face (implies mixed)
     int bulk32;         // 1: face values, quadrature, back-projection and flux in FP32 (implies mixed)
-    int recon;          // characteristic reconstruction: 0 teno5 (the reference's), 1 teno5i, 2 weno5JS, 3 weno5Z
+    int recon;          // characteristic reconstruction: 0 teno5 (the reference's), 1 teno5i, 2 weno5JS, 3 weno5Z,
+                        // RECON_WENO5N .. RECON_LAST the variants of recon5_variant (htx_variants.h)
     int tiles_x, tiles_y;
     double gam, g1, dx, dy, area;
     QuadConsts qc;

@@ -15,7 +15,11 @@ HTX_OK = 0
 ERRORS = {-1: "HTX_EINVAL", -2: "HTX_EDEVICE", -3: "HTX_ENOMEM", -4: "HTX_ESTATE", -5: "HTX_ENOTSUP", -6: "HTX_EIO",
           -7: "HTX_ENONFINITE"}
 
-SCHEMES = {"teno5": 0, "hteno": 1, "teno5i": 2, "weno5js": 3, "weno5z": 4}
+SCHEMES = {"teno5": 0, "hteno": 1, "teno5i": 2, "weno5js": 3, "weno5z": 4, "weno5n": 5, "teno5n": 6, "teno5a": 7,
+           "weno5is": 8, "weno5is_ext": 9, "weno5ismod": 10, "weno5z_diff": 11, "teno5_diff": 12, "linear": 13,
+           "hweno": 14, "htenoi": 15, "hlinear": 16}
+# the schemes of the Hermite solver (12 fields: u and the x, y derivative moments)
+HERMITE_SCHEMES = ("hteno", "hweno", "htenoi", "hlinear")
 FLUXES = {"rusanov": 0, "rusanov_char": 0, "hllc": 1}
 BCS = {"zero_grad": 0, "zerograd": 0, "periodic": 1}
 PRECISIONS = {"fp64": 0, "fp32_mixed": 1, "learned": 2, "fp64_fast": 3, "fp32": 4, "learned_fp32": 5}

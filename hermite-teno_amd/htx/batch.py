@@ -48,7 +48,7 @@ class Batch:
             pass
 
     def nfield(self, i):
-        return 12 if self.params[i].scheme == _lib.SCHEMES["hteno"] else 4
+        return 12 if self.params[i].scheme in [_lib.SCHEMES[k] for k in _lib.HERMITE_SCHEMES] else 4
 
     def set_state(self, i, q):
         p = self.params[i]

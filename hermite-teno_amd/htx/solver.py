@@ -39,7 +39,7 @@ class System2D:
         check(lib.htx_create(ctypes.byref(p), ctypes.byref(h)))
         self._h = h
         self.nx, self.ny = int(nx), int(ny)
-        self.nfield = 12 if scheme == "hteno" else 4
+        self.nfield = 12 if scheme in _lib.HERMITE_SCHEMES else 4
         self.dx = (p.xr - p.xl) / p.nx
         self.dy = (p.yt - p.yb) / p.ny
         self.gamma = p.gamma

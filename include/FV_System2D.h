@@ -33,7 +33,17 @@ struct Options {
     int precision = HTX_PREC_FP64;
     int ref_quirks = 1;
     std::string output_path;           // write(): empty -> no file
+    // the reconstruction a user of the reference swaps in for teno5
+    // (FV_System2D.cpp:379, :394: an htx_scheme of the 5-point family, e.g.
+    // HTX_SCHEME_WENO5Z) or for hteno (FVHermite_System2D.cpp:459, :486:
+    // HTX_SCHEME_HWENO, HTX_SCHEME_HTENOI, HTX_SCHEME_HLINEAR); -1 keeps the
+    // class's own (teno5 / hteno)
+    int reconstruction = -1;
 };
+
+inline bool is_hermite_scheme(int s) {
+    return s == HTX_SCHEME_HTENO || s == HTX_SCHEME_HWENO || s == HTX_SCHEME_HTENOI || s == HTX_SCHEME_HLINEAR;
+}
 
 inline void check(int rc) {
     if (rc != HTX_OK) throw std::runtime_error(std::string("htx: ") + htx_last_error());
@@ -69,8 +79,13 @@ public:
 protected:
     SolverBase(double xl, double xr, int nx, double yb, double yt, int ny, double cfl, double tmax, double tinit,
                double gamma, int scheme, const Options& o)
-        : nx_(nx), ny_(ny), nf_(scheme == HTX_SCHEME_HTENO ? 12 : 4), xl_(xl), yb_(yb), gamma_(gamma),
+        : nx_(nx), ny_(ny), nf_(is_hermite_scheme(scheme) ? 12 : 4), xl_(xl), yb_(yb), gamma_(gamma),
           tFinal_(tmax), opt_(o) {
+        if (o.reconstruction >= 0) {
+            if (is_hermite_scheme(o.reconstruction) != is_hermite_scheme(scheme))
+                throw std::invalid_argument("htx: Options::reconstruction belongs to the other solver class");
+            scheme = o.reconstruction;
+        }
         htx_params p;
         check(htx_params_default(&p));
         p.xl = xl; p.xr = xr; p.nx = nx;

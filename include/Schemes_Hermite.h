@@ -9,8 +9,8 @@
 //
 // setGridSize(0) throws std::invalid_argument where the reference prints and
 // exits (Schemes_Hermite.cpp:16-25); calling a method before setGridSize
-// throws std::logic_error.  Reconstructions no configuration of the rebuilt
-// path uses are declared and throw std::logic_error naming them.
+// throws std::logic_error.  hweno, htenoi and linear are also selectable as the
+// face reconstruction of the 2D Hermite GPU solver (htx_scheme in htx.h).
 #pragma once
 #include "htx_linalg.h"
 
@@ -20,15 +20,15 @@ class Hermite {
 public:
     Hermite();
     void setGridSize(const double& gridSize);
-    Vector4d hweno(const Vector4d& u, const Vector4d& derivatives) const;    // not built
+    Vector4d hweno(const Vector4d& u, const Vector4d& derivatives) const;    // :234-293
     Vector4d hteno(const Vector4d& u, const Vector4d& derivatives) const;    // :295-379
-    Vector4d linear(const Vector4d& u, const Vector4d& derivatives) const;   // not built
+    Vector4d linear(const Vector4d& u, const Vector4d& derivatives) const;   // :757-767
 
-    Vector4d htenoi(const Vector4d& u, const Vector4d& derivatives) const;   // not built
+    Vector4d htenoi(const Vector4d& u, const Vector4d& derivatives) const;   // :670-755
 
     double hermiteLimiter(const Vector3d& u, const Vector3d& derivatives) const;  // :634-662
-    Vector4d hweno2PointsGLQ(const Vector3d& u, const Vector3d& derivatives) const;           // not built
-    Matrix<double, 8, 1> hweno4PointsGLQ(const Vector3d& u, const Vector3d& derivatives) const;  // not built
+    Vector4d hweno2PointsGLQ(const Vector3d& u, const Vector3d& derivatives) const;           // :381-425
+    Matrix<double, 8, 1> hweno4PointsGLQ(const Vector3d& u, const Vector3d& derivatives) const;  // :427-513
     bool KXRCF(const double& u0, const double& ul, const double& unorm) const;  // :664-668
 
     double gridSize() const { return dif; }

@@ -47,16 +47,38 @@ enum htx_error {
 /* HTX_SCHEME_TENO5I: Lagrange::teno5i (Schemes_Lagrange.cpp:1235-1263), the
  * reconstruction FV::System1D ships with.  HTX_SCHEME_WENO5JS /
  * HTX_SCHEME_WENO5Z: Lagrange::weno5JS / weno5Z (Schemes_Lagrange.cpp:358-408),
- * the reference's alternatives for the 1D solver's reconstruction.  In 2D
- * each takes TENO5's place in the characteristic reconstruction of the x and
- * y faces (FV_System2D.cpp:379, :394, where a user of the reference swaps the
- * call), everything else unchanged; 2D builds: fp64 and fp64_fast. */
+ * the reference's alternatives for the 1D solver's reconstruction.
+ * HTX_SCHEME_WENO5N .. HTX_SCHEME_LINEAR: the remaining 5-point routines of
+ * Schemes::Lagrange, weno5N (:440-478), teno5N (:553-593), teno5a
+ * (:1265-1294), weno5IS (:596-620), weno5IS_Extended (:649-717),
+ * weno5ISmod(u, diff) (:622-647), weno5Z(u, diff) (:410-438), teno5(u, diff)
+ * (:514-551) and linear (:1219-1233); diff is the cell size normal to the
+ * face.  Each of these takes TENO5's place in the characteristic
+ * reconstruction (FV_System1D.cpp:200-201 in 1D; FV_System2D.cpp:379, :394
+ * in 2D, where a user of the reference swaps the call), everything else
+ * unchanged; 2D builds: fp64 and fp64_fast, 1D: fp64.
+ * HTX_SCHEME_HTENO: FVHermite::System2D (Hermite::hteno).  HTX_SCHEME_HWENO,
+ * HTX_SCHEME_HTENOI, HTX_SCHEME_HLINEAR: the same solver with Hermite::hweno
+ * (Schemes_Hermite.cpp:234-293), htenoi (:670-755) or linear (:757-767) in
+ * hteno's place (FVHermite_System2D.cpp:459, :486); fp64 and fp64_fast. */
 enum htx_scheme {
     HTX_SCHEME_TENO5 = 0,
     HTX_SCHEME_HTENO = 1,
     HTX_SCHEME_TENO5I = 2,
     HTX_SCHEME_WENO5JS = 3,
-    HTX_SCHEME_WENO5Z = 4
+    HTX_SCHEME_WENO5Z = 4,
+    HTX_SCHEME_WENO5N = 5,
+    HTX_SCHEME_TENO5N = 6,
+    HTX_SCHEME_TENO5A = 7,
+    HTX_SCHEME_WENO5IS = 8,
+    HTX_SCHEME_WENO5IS_EXT = 9,
+    HTX_SCHEME_WENO5ISMOD = 10,
+    HTX_SCHEME_WENO5Z_DIFF = 11,
+    HTX_SCHEME_TENO5_DIFF = 12,
+    HTX_SCHEME_LINEAR = 13,
+    HTX_SCHEME_HWENO = 14,
+    HTX_SCHEME_HTENOI = 15,
+    HTX_SCHEME_HLINEAR = 16
 };
 enum htx_flux { HTX_FLUX_RUSANOV_CHAR = 0, HTX_FLUX_HLLC = 1 };
 enum htx_bc { HTX_BC_ZERO_GRAD = 0, HTX_BC_PERIODIC = 1 };

@@ -337,6 +337,598 @@ void orc_weno5js3q_linear(const double u[5], double out[3]) {
     out[2] = dot3(p, LC.linwm);
 }
 
+/* ------------------------------------------------------------------------ */
+/* The remaining Schemes::Lagrange routines (Schemes_Lagrange.cpp:315-1004,  */
+/* 1036-1149, 1265-1294), restated in the member-state style of the          */
+/* reference: SI, weights, sigm, tau are locals standing for its members.    */
+/* ------------------------------------------------------------------------ */
+
+/* the 5th-order candidates of one side: col c of crj applied to the 3
+ * values starting at u[off[c]] (the p(0..2) assignments of teno5, :482-484
+ * and :495-497) */
+static void cand_side(const double u[6], int plus, double p[3]) {
+    double c[3];
+    const int cm[3] = {1, 2, 3}, om[3] = {2, 1, 0};
+    const int cp[3] = {0, 1, 2}, op[3] = {3, 2, 1};
+    for (int k = 0; k < 3; ++k) {
+        int col = plus ? cp[k] : cm[k], off = plus ? op[k] : om[k];
+        for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][col];
+        p[k] = dot3(c, u + off);
+    }
+}
+
+/* cutOff (:1141-1149) with ct, then sigm.cwiseProduct(d) normalised and
+ * the candidates combined, catchnan */
+static double cut_and_combine(double weights[3], const double p[3], const double d[3], double ct) {
+    double sigm[3], s = sum3(weights);
+    for (int k = 0; k < 3; ++k) weights[k] = weights[k] / s;
+    for (int k = 0; k < 3; ++k) sigm[k] = weights[k] < ct ? 0.0 : 1.0;
+    for (int k = 0; k < 3; ++k) weights[k] = sigm[k] * d[k];
+    s = sum3(weights);
+    for (int k = 0; k < 3; ++k) weights[k] = weights[k] / s;
+    return catchnan(dot3(p, weights));
+}
+
+static double normalise_and_combine(double weights[3], const double p[3]) {
+    double s = sum3(weights);
+    for (int k = 0; k < 3; ++k) weights[k] = weights[k] / s;
+    return catchnan(dot3(p, weights));
+}
+
+/* the SI and tau of weno5N / teno5N (:445-453, :558-566) on window v */
+static void si_n(const double* v, double SI[3], double* fourth) {
+    const double theta = 0.1;
+    SI[2] = theta * fabs(v[0] - 3.0 * v[1] + 2.0 * v[2]) + fabs(v[0] - 2.0 * v[1] + v[2]);
+    SI[1] = theta * fabs(v[3] - v[2]) + fabs(v[1] - 2.0 * v[2] + v[3]);
+    SI[0] = theta * fabs(v[3] - v[2]) + fabs(v[2] - 2.0 * v[3] + v[4]);
+    *fourth = fabs(v[0] - 4.0 * v[1] + 6.0 * v[2] - 4.0 * v[3] + v[4]);
+}
+
+/* Lagrange::weno5N :440-478 */
+static void weno5n_impl(const double u[6], double out[2]) {
+    double p[3], SI[3], w[3], f;
+    for (int side = 0; side < 2; ++side) {
+        cand_side(u, side, p);
+        si_n(u + side, SI, &f);
+        double tau = pow(f, 2.5);
+        const double* d = side ? LC.dkp : LC.dkm;
+        for (int k = 0; k < 3; ++k) w[k] = d[k] * (1.0 + (tau / pow(SI[k] + LC.eps, 2)));
+        out[side] = normalise_and_combine(w, p);
+    }
+}
+
+/* Lagrange::teno5N :553-593 */
+static void teno5n_impl(const double u[6], double out[2]) {
+    double p[3], SI[3], w[3], f;
+    for (int side = 0; side < 2; ++side) {
+        cand_side(u, side, p);
+        si_n(u + side, SI, &f);
+        double tau = pow(f, 2);
+        for (int k = 0; k < 3; ++k) w[k] = pow(1.0 + tau / pow(SI[k] + LC.eps, 2), 6);
+        out[side] = cut_and_combine(w, p, side ? LC.dkp : LC.dkm, LC.ct);
+    }
+}
+
+/* Lagrange::teno5a :1265-1294 */
+static void teno5a_impl(const double u[6], double out[2]) {
+    double p[3], SI[3], w[3];
+    for (int side = 0; side < 2; ++side) {
+        cand_side(u, side, p);
+        smoothness5(u + side, SI);
+        double tau = 1.0;
+        for (int k = 0; k < 3; ++k) w[k] = pow(tau / (SI[k] + LC.epsz), 6);
+        out[side] = cut_and_combine(w, p, side ? LC.dkp : LC.dkm, LC.ct);
+    }
+}
+
+/* Lagrange::teno5(u, diff) :514-551 */
+static void teno5diff_impl(const double u[6], double diff, double out[2]) {
+    double p[3], SI[3], w[3];
+    double df = pow(diff, 4);
+    for (int side = 0; side < 2; ++side) {
+        cand_side(u, side, p);
+        smoothness5(u + side, SI);
+        double tau = fabs(b35_of(u + side) - (SI[0] + SI[2] + 4.0 * SI[1]) / 6.0);
+        for (int k = 0; k < 3; ++k) w[k] = pow(1.0 + pow(tau / (SI[k] + df), 2), 3);
+        out[side] = cut_and_combine(w, p, side ? LC.dkp : LC.dkm, LC.ct);
+    }
+}
+
+/* Lagrange::weno5Z(u, diff) :410-438 (both sides take fullpointsSmootness
+ * of u.segment(0, 5), :419 and :431; diff is unused) */
+static void weno5zdiff_impl(const double u[6], double out[2]) {
+    double p[3], SI[3], w[3];
+    for (int side = 0; side < 2; ++side) {
+        cand_side(u, side, p);
+        smoothness5(u + side, SI);
+        double tau = fabs(b35_of(u) - (SI[0] + SI[2] + 4.0 * SI[1]) / 6.0);
+        const double* d = side ? LC.dkp : LC.dkm;
+        for (int k = 0; k < 3; ++k) w[k] = (1.0 + tau / pow(SI[k] + LC.epsz, 0.75)) * d[k];
+        out[side] = normalise_and_combine(w, p);
+    }
+}
+
+/* smoothnessMeasureIS :1036-1050 (mod = 0) / smoothnessMeasureISmod
+ * :1052-1066 (mod = 1, offsets df) -> weightsis */
+static void si_is(const double* u, int mod, double df, double weightsis[4]) {
+    double SI4is[4], b12, tau;
+    SI4is[0] = pow(u[3] - u[2], 2);
+    SI4is[1] = pow(u[2] - u[1], 2);
+    SI4is[2] = 0.25 * pow(3.0 * u[2] - 4.0 * u[3] + u[4], 2) + LC.c1 * pow(u[2] - 2.0 * u[3] + u[4], 2);
+    SI4is[3] = 0.25 * pow(u[0] - 4.0 * u[1] + 3.0 * u[2], 2) + LC.c1 * pow(u[0] - 2.0 * u[1] + u[2], 2);
+    b12 = 0.25 * pow(u[1] - u[3], 2) + LC.c1 * pow(u[1] - 2.0 * u[2] + u[3], 2);
+    tau = 0.25 * pow(u[4] - 2.0 * (u[3] - u[1]) - u[0], 2) +
+          LC.c1 * pow(u[4] - 4.0 * u[3] + 6.0 * u[2] - 4.0 * u[1] + u[0], 2);
+    if (!mod) {
+        weightsis[0] = 1.0 + pow(tau, 2) / (SI4is[0] + LC.eps) / (b12 + LC.eps);
+        weightsis[1] = 1.0 + pow(tau, 2) / (SI4is[1] + LC.eps) / (b12 + LC.eps);
+        weightsis[2] = 1.0 + tau / (SI4is[2] + LC.eps);
+        weightsis[3] = 1.0 + tau / (SI4is[3] + LC.eps);
+    } else {
+        weightsis[0] = 1.0 + tau * tau / (SI4is[0] * b12 + df);
+        weightsis[1] = 1.0 + tau * tau / (SI4is[1] * b12 + df);
+        weightsis[2] = 1.0 + tau / (SI4is[2] + df);
+        weightsis[3] = 1.0 + tau / (SI4is[3] + df);
+    }
+}
+
+/* Lagrange::weno5IS :596-620 and weno5ISmod(u, diff) :622-647 (dkmis /
+ * dkpis :144-152) */
+static void weno5is_impl(const double u[6], int mod, double diff, double out[2]) {
+    const double dkmis[4] = {0.4, 0.2, 0.3, 0.1}, dkpis[4] = {0.2, 0.4, 0.1, 0.3};
+    double df = mod ? pow(diff, 3) : 0.0;
+    double pis[4], w[4], c[3];
+    for (int side = 0; side < 2; ++side) {
+        if (!side) {
+            pis[0] = 0.5 * (u[2] + u[3]);
+            pis[1] = -0.5 * u[1] + 1.5 * u[2];
+            for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][1];
+            pis[2] = dot3(c, u + 2);
+            for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][3];
+            pis[3] = dot3(c, u + 0);
+        } else {
+            pis[0] = 1.5 * u[3] - 0.5 * u[4];
+            pis[1] = 0.5 * (u[2] + u[3]);
+            for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][0];
+            pis[2] = dot3(c, u + 3);
+            for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][2];
+            pis[3] = dot3(c, u + 1);
+        }
+        si_is(u + side, mod, df, w);
+        const double* d = side ? dkpis : dkmis;
+        for (int k = 0; k < 4; ++k) w[k] = d[k] * w[k];
+        double s = ((w[0] + w[1]) + w[2]) + w[3];
+        for (int k = 0; k < 4; ++k) w[k] = w[k] / s;
+        out[side] = catchnan(((pis[0] * w[0] + pis[1] * w[1]) + pis[2] * w[2]) + pis[3] * w[3]);
+    }
+}
+
+static double sum6(const double a[6]) {
+    double s = a[0];
+    for (int i = 1; i < 6; ++i) s = s + a[i];
+    return s;
+}
+static double dot6(const double a[6], const double b[6]) {
+    double s = a[0] * b[0];
+    for (int i = 1; i < 6; ++i) s = s + a[i] * b[i];
+    return s;
+}
+
+/* Lagrange::weno5IS_Extended :649-717 with smoothnessMeasureISExtended
+ * :1068-1084 and the split optimal weights :288-303 */
+static void weno5isext_impl(const double u[6], double out[2]) {
+    const double dkmise[6] = {-1.0 / 45.0, 11.0 / 90.0, 1.0 / 5.0, 2.0 / 5.0, 1.0 / 5.0, 1.0 / 10.0};
+    const double dkpise[6] = {1.0 / 10.0, 1.0 / 5.0, 4.0 / 10.0, 2.0 / 10.0, 11.0 / 90.0, -1.0 / 45.0};
+    for (int side = 0; side < 2; ++side) {
+        const double* dk = side ? dkpise : dkmise;
+        double pos[6], neg[6], pos_sum, neg_sum;
+        for (int i = 0; i < 6; ++i) {
+            pos[i] = 0.5 * (dk[i] + 3.0 * fabs(dk[i]));
+            neg[i] = pos[i] - dk[i];
+        }
+        pos_sum = sum6(pos);
+        neg_sum = sum6(neg);
+        for (int i = 0; i < 6; ++i) {
+            pos[i] = pos[i] / pos_sum;
+            neg[i] = neg[i] / neg_sum;
+        }
+        double pise[6];
+        if (!side) {
+            pise[0] = -1.5 * u[0] + 2.5 * u[1];
+            pise[1] = -0.5 * u[1] + 1.5 * u[2];
+            pise[2] = pise[1];
+            pise[3] = 0.5 * (u[2] + u[3]);
+            pise[4] = pise[3];
+            pise[5] = 1.5 * u[3] - 0.5 * u[4];
+        } else {
+            pise[0] = -0.5 * u[1] + 1.5 * u[2];
+            pise[1] = 0.5 * (u[2] + u[3]);
+            pise[2] = pise[1];
+            pise[3] = 1.5 * u[3] - 0.5 * u[4];
+            pise[4] = pise[3];
+            pise[5] = 2.5 * u[4] - 1.5 * u[5];
+        }
+        const double* v = u + side;
+        double c[3], b[3], SI[3], SI4is[4], tau;
+        c[2] = v[2] - 2.0 * v[3] + v[4];
+        b[2] = 3.0 * v[2] - 4.0 * v[3] + v[4];
+        c[1] = v[1] - 2.0 * v[2] + v[3];
+        b[1] = v[1] - v[3];
+        c[0] = v[0] - 2.0 * v[1] + v[2];
+        b[0] = v[0] - 4.0 * v[1] + 3.0 * v[2];
+        for (int k = 0; k < 3; ++k) {
+            b[k] = b[k] / 2.0;
+            SI[k] = b[k] * b[k] + LC.c1 * c[k] * c[k];
+        }
+        for (int i = 0; i < 4; ++i) SI4is[i] = pow(v[i + 1] - v[i], 2);
+        tau = 0.25 * pow(v[4] - 2.0 * (v[3] - v[1]) - v[0], 2) +
+              LC.c1 * pow(v[4] - 4.0 * v[3] + 6.0 * v[2] - 4.0 * v[1] + v[0], 2);
+        tau *= tau;
+        double w[6], wp[6], wn[6];
+        w[0] = 1.0 + tau / (SI[0] + LC.eps) / (SI4is[0] + LC.eps);
+        w[1] = 1.0 + tau / (SI[0] + LC.eps) / (SI4is[1] + LC.eps);
+        w[2] = 1.0 + tau / (SI[1] + LC.eps) / (SI4is[1] + LC.eps);
+        w[3] = 1.0 + tau / (SI[1] + LC.eps) / (SI4is[2] + LC.eps);
+        w[4] = 1.0 + tau / (SI[2] + LC.eps) / (SI4is[2] + LC.eps);
+        w[5] = 1.0 + tau / (SI[2] + LC.eps) / (SI4is[3] + LC.eps);
+        for (int i = 0; i < 6; ++i) w[i] = pow(w[i], 4);
+        double s = sum6(w);
+        for (int i = 0; i < 6; ++i) w[i] = w[i] / s;
+        for (int i = 0; i < 6; ++i) w[i] = w[i] < 1E-5 ? 0.0 : 1.0;
+        for (int i = 0; i < 6; ++i) {
+            wp[i] = pos[i] * w[i];
+            wn[i] = neg[i] * w[i];
+        }
+        double sp = sum6(wp), sn = sum6(wn);
+        for (int i = 0; i < 6; ++i) {
+            wp[i] = wp[i] / sp;
+            wn[i] = wn[i] / sn;
+        }
+        out[side] = catchnan(pos_sum * dot6(pise, wp) - neg_sum * dot6(pise, wn));
+    }
+}
+
+static void teno5i_impl(const double u[6], double out[2]);
+static void weno5_impl(const double u[6], double out[2], int z);
+
+/* The characteristic reconstruction a solver calls (FV_System1D.cpp:200-201,
+ * FV_System2D.cpp:379, :394) with one of the alternatives swapped in:
+ * ORC_RECON_* (diff: the cell size normal to the face, for the routines
+ * that take one). */
+void orc_recon(int recon, const double u[6], double diff, double out[2]) {
+    lag_init();
+    switch (recon) {
+        case ORC_RECON_TENO5I: teno5i_impl(u, out); break;
+        case ORC_RECON_WENO5JS: weno5_impl(u, out, 0); break;
+        case ORC_RECON_WENO5Z: weno5_impl(u, out, 1); break;
+        case ORC_RECON_WENO5N: weno5n_impl(u, out); break;
+        case ORC_RECON_TENO5N: teno5n_impl(u, out); break;
+        case ORC_RECON_TENO5A: teno5a_impl(u, out); break;
+        case ORC_RECON_WENO5IS: weno5is_impl(u, 0, 0.0, out); break;
+        case ORC_RECON_WENO5IS_EXT: weno5isext_impl(u, out); break;
+        case ORC_RECON_WENO5ISMOD: weno5is_impl(u, 1, diff, out); break;
+        case ORC_RECON_WENO5Z_DIFF: weno5zdiff_impl(u, out); break;
+        case ORC_RECON_TENO5_DIFF: teno5diff_impl(u, diff, out); break;
+        case ORC_RECON_LINEAR: orc_linear5(u, out); break;
+        default: teno5_impl(u, out, NULL); break;
+    }
+}
+
+/* Lagrange::weno3JS :315-335 (its last line stores catchnan(out(0)) in
+ * out(1)) and weno3JS2PointsQuadrature :337-356 */
+void orc_weno3js(const double u[4], double out[2]) {
+    const double eps = 1E-6;
+    const double crjm3rd[2][2] = {{-0.5, 0.5}, {1.5, 0.5}}; /* (r, c) :105-108 */
+    const double dkm3rd[2] = {1.0 / 3.0, 2.0 / 3.0};
+    double crjp3rd[2][2], dkp3rd[2] = {dkm3rd[1], dkm3rd[0]};
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c) crjp3rd[r][c] = crjm3rd[1 - r][1 - c];
+    double p3rd[2], SI3rd[2], w[2], s;
+    p3rd[0] = crjm3rd[0][0] * u[0] + crjm3rd[1][0] * u[1];
+    p3rd[1] = crjm3rd[0][1] * u[1] + crjm3rd[1][1] * u[2];
+    SI3rd[0] = pow(u[0] - u[1], 2);
+    SI3rd[1] = pow(u[1] - u[2], 2);
+    for (int k = 0; k < 2; ++k) w[k] = dkm3rd[k] / pow(eps + SI3rd[k], 2);
+    s = w[0] + w[1];
+    for (int k = 0; k < 2; ++k) w[k] = w[k] / s;
+    out[0] = catchnan(p3rd[0] * w[0] + p3rd[1] * w[1]);
+    p3rd[0] = crjp3rd[0][0] * u[1] + crjp3rd[1][0] * u[2];
+    p3rd[1] = crjp3rd[0][1] * u[2] + crjp3rd[1][1] * u[3];
+    SI3rd[0] = pow(u[1] - u[2], 2);
+    SI3rd[1] = pow(u[2] - u[3], 2);
+    for (int k = 0; k < 2; ++k) w[k] = dkp3rd[k] / pow(eps + SI3rd[k], 2);
+    s = w[0] + w[1];
+    for (int k = 0; k < 2; ++k) w[k] = w[k] / s;
+    out[1] = p3rd[0] * w[0] + p3rd[1] * w[1];
+    out[1] = catchnan(out[0]);
+}
+
+void orc_weno3js_2q(const double u[3], double out[2]) {
+    const double eps = 1E-6, sq3 = sqrt(3.0), c6 = sq3 / 3.0;
+    double crjp3rdg[2][2], crjm3rdg[2][2];
+    crjp3rdg[0][0] = -sq3 / 6.0;
+    crjp3rdg[1][0] = sq3 / 6.0 + 1.0;
+    crjp3rdg[0][1] = -sq3 / 6.0 + 1.0;
+    crjp3rdg[1][1] = sq3 / 6.0;
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c) crjm3rdg[r][c] = crjp3rdg[1 - r][1 - c];
+    double SI3rd[2], p3rd[2], w[2], s;
+    SI3rd[0] = u[0] - u[1];
+    SI3rd[1] = u[1] - u[2];
+    for (int k = 0; k < 2; ++k) SI3rd[k] = c6 * pow(SI3rd[k], 2);
+    p3rd[0] = crjp3rdg[0][0] * u[0] + crjp3rdg[1][0] * u[1];
+    p3rd[1] = crjp3rdg[0][1] * u[1] + crjp3rdg[1][1] * u[2];
+    for (int k = 0; k < 2; ++k) w[k] = 0.5 / pow(eps + SI3rd[k], 2);
+    s = w[0] + w[1];
+    for (int k = 0; k < 2; ++k) w[k] = w[k] / s;
+    out[0] = p3rd[0] * w[0] + p3rd[1] * w[1];
+    p3rd[0] = crjm3rdg[0][0] * u[0] + crjm3rdg[1][0] * u[1];
+    p3rd[1] = crjm3rdg[0][1] * u[1] + crjm3rdg[1][1] * u[2];
+    for (int k = 0; k < 2; ++k) w[k] = 0.5 / pow(eps + SI3rd[k], 2);
+    s = w[0] + w[1];
+    for (int k = 0; k < 2; ++k) w[k] = w[k] / s;
+    out[1] = p3rd[0] * w[0] + p3rd[1] * w[1];
+}
+
+/* coefficient tables of the 2- and 4-point quadratures (constructor,
+ * Schemes_Lagrange.cpp:9-20, :154-286) */
+typedef struct {
+    double crjgp[3][3], crjgm[3][3], dkpg[3], dkmg[3];
+    double crjgpLobatto[3][3], crjgmLobatto[3][3], dkpLobatto[3], dkmLobatto[3];
+    double crjgpLegendre_12[3][3], crjgmLegendre_12[3][3], dkpLegendre_12[3], dkmLegendre_12[3];
+    double crjgpLegendre_34[3][3], crjgmLegendre_34[3][3], dkpLegendre_34[3], dkmLegendre_34[3];
+    double c2, c3, c4, c5, c7, c8, c9, c10;
+} quad_tables;
+
+/* M = P with rows 0 and 2 swapped, then columns 0 and 2 */
+static void swap02(const double P[3][3], double M[3][3]) {
+    double T[3][3];
+    for (int c = 0; c < 3; ++c) {
+        T[0][c] = P[2][c];
+        T[1][c] = P[1][c];
+        T[2][c] = P[0][c];
+    }
+    for (int r = 0; r < 3; ++r) {
+        M[r][0] = T[r][2];
+        M[r][1] = T[r][1];
+        M[r][2] = T[r][0];
+    }
+}
+
+static void quad_tables_init(quad_tables* Q) {
+    double sq3 = sqrt(3.0), sq5 = sqrt(5.0), denom;
+    Q->c2 = sq3 / 12.0;
+    Q->c3 = 37.0 / 108.0 * sq3;
+    Q->c4 = sq5 / 20.0;
+    Q->c5 = 61.0 * sq5 / 300.0;
+    Q->c7 = sqrt(0.525e3 - 0.70e2 * sqrt(0.30e2)) / 0.140e3;
+    Q->c8 = pow(0.525e3 - 0.70e2 * sqrt(0.30e2), 0.3e1 / 0.2e1) / 0.514500e6 +
+            sqrt(0.525e3 - 0.70e2 * sqrt(0.30e2)) / 0.35e2;
+    Q->c9 = sqrt(0.525e3 + 0.70e2 * sqrt(0.30e2)) / 0.140e3;
+    Q->c10 = pow(0.525e3 + 0.70e2 * sqrt(0.30e2), 0.3e1 / 0.2e1) / 0.514500e6 +
+             sqrt(0.525e3 + 0.70e2 * sqrt(0.30e2)) / 0.35e2;
+    Q->crjgp[0][0] = sq3 / 12.0;
+    Q->crjgp[1][0] = -sq3 / 3.0;
+    Q->crjgp[2][0] = 1.0 + sq3 / 4.0;
+    Q->crjgp[0][1] = -sq3 / 12.0;
+    Q->crjgp[1][1] = 1.0;
+    Q->crjgp[2][1] = sq3 / 12.0;
+    Q->crjgp[0][2] = -sq3 / 4.0 + 1.0;
+    Q->crjgp[1][2] = sq3 / 3.0;
+    Q->crjgp[2][2] = -sq3 / 12.0;
+    swap02(Q->crjgp, Q->crjgm);
+    Q->crjgpLobatto[0][0] = sq5 / 20.0 - 1.0 / 60.0;
+    Q->crjgpLobatto[1][0] = -sq5 / 5.0 + 1.0 / 30.0;
+    Q->crjgpLobatto[2][0] = 3.0 * sq5 / 20.0 + 59 / 60.0;
+    Q->crjgpLobatto[0][1] = -sq5 / 20.0 - 1.0 / 60.0;
+    Q->crjgpLobatto[1][1] = 31.0 / 30.0;
+    Q->crjgpLobatto[2][1] = sq5 / 20.0 - 1.0 / 60.0;
+    Q->crjgpLobatto[0][2] = -3.0 * sq5 / 20.0 + 59.0 / 60.0;
+    Q->crjgpLobatto[1][2] = sq5 / 5.0 + 1.0 / 30.0;
+    Q->crjgpLobatto[2][2] = -sq5 / 20.0 - 1.0 / 60.0;
+    swap02(Q->crjgpLobatto, Q->crjgmLobatto);
+    denom = 660.0 * sq5 - 220.0;
+    Q->dkpLobatto[0] = 132.0 * sq5 + 22.0;
+    Q->dkpLobatto[1] = 387.0 * sq5 - 129.0;
+    Q->dkpLobatto[2] = 141.0 * sq5 - 113.0;
+    for (int k = 0; k < 3; ++k) Q->dkpLobatto[k] = Q->dkpLobatto[k] / denom;
+    denom = 660.0 * sq5 + 220.0;
+    Q->dkmLobatto[0] = 132.0 * sq5 - 22.0;
+    Q->dkmLobatto[1] = 387.0 * sq5 + 129.0;
+    Q->dkmLobatto[2] = 141.0 * sq5 + 113.0;
+    for (int k = 0; k < 3; ++k) Q->dkmLobatto[k] = Q->dkmLobatto[k] / denom;
+    denom = 1080;
+    Q->dkpg[0] = (210.0 - sq3) / denom;
+    Q->dkpg[1] = 11.0 / 18.0;
+    Q->dkpg[2] = (210.0 + sq3) / denom;
+    Q->dkmg[0] = Q->dkpg[2];
+    Q->dkmg[1] = Q->dkpg[1];
+    Q->dkmg[2] = Q->dkpg[0];
+    const double r30 = sqrt(0.30e2);
+    const double am = sqrt(0.525e3 - 0.70e2 * sqrt(0.30e2)), ap = sqrt(0.525e3 + 0.70e2 * sqrt(0.30e2));
+    const double bm = 0.525e3 - 0.70e2 * sqrt(0.30e2), bp = 0.525e3 + 0.70e2 * sqrt(0.30e2);
+    Q->crjgpLegendre_12[0][0] = am / 0.140e3 - r30 / 0.140e3 + 0.1e1 / 0.84e2;
+    Q->crjgpLegendre_12[1][0] = -am / 0.35e2 + r30 / 0.70e2 - 0.1e1 / 0.42e2;
+    Q->crjgpLegendre_12[2][0] = 0.3e1 / 0.140e3 * am - r30 / 0.140e3 + 0.85e2 / 0.84e2;
+    Q->crjgpLegendre_12[0][1] = -am / 0.140e3 - r30 / 0.140e3 + 0.1e1 / 0.84e2;
+    Q->crjgpLegendre_12[1][1] = r30 / 0.70e2 + 0.41e2 / 0.42e2;
+    Q->crjgpLegendre_12[2][1] = am / 0.140e3 - r30 / 0.140e3 + 0.1e1 / 0.84e2;
+    Q->crjgpLegendre_12[0][2] = -0.3e1 / 0.140e3 * am - r30 / 0.140e3 + 0.85e2 / 0.84e2;
+    Q->crjgpLegendre_12[1][2] = am / 0.35e2 + r30 / 0.70e2 - 0.1e1 / 0.42e2;
+    Q->crjgpLegendre_12[2][2] = -am / 0.140e3 - r30 / 0.140e3 + 0.1e1 / 0.84e2;
+    swap02(Q->crjgpLegendre_12, Q->crjgmLegendre_12);
+    denom = (0.1234800000e10 * r30 - 0.9116940000e10) * am + 0.11174940000e11 * r30 - 0.52238900000e11;
+    Q->dkpLegendre_12[2] = ((0.257250000e9 * r30 - 0.1899362500e10) * am +
+                            (-0.42000e5 * r30 + 0.310100e6) * pow(bm, 0.3e1 / 0.2e1) - 0.1197315000e10 * r30 +
+                            0.5597025000e10) / denom;
+    Q->dkpLegendre_12[1] = ((0.1761795000e10 * r30 - 0.13038532500e11) * am +
+                            (-0.88200e5 * r30 + 0.16096500e8) * pow(bm, 0.3e1 / 0.2e1) -
+                            0.2520e4 * pow(bm, 0.5e1 / 0.2e1) + 0.6757590000e10 * r30 - 0.32331425000e11) / denom;
+    Q->dkpLegendre_12[0] = ((0.26092500e8 * r30 - 0.256576250e9) * am +
+                            (-0.2100e4 * r30 - 0.3838100e7) * pow(bm, 0.3e1 / 0.2e1) +
+                            0.630e3 * pow(bm, 0.5e1 / 0.2e1) + 0.5614665000e10 * r30 - 0.25504500000e11) / denom;
+    Q->dkmLegendre_12[0] = Q->dkpLegendre_12[2];
+    Q->dkmLegendre_12[1] = Q->dkpLegendre_12[1];
+    Q->dkmLegendre_12[2] = Q->dkpLegendre_12[0];
+    Q->crjgpLegendre_34[0][0] = ap / 0.140e3 + r30 / 0.140e3 + 0.1e1 / 0.84e2;
+    Q->crjgpLegendre_34[1][0] = -ap / 0.35e2 - r30 / 0.70e2 - 0.1e1 / 0.42e2;
+    Q->crjgpLegendre_34[2][0] = 0.3e1 / 0.140e3 * ap + r30 / 0.140e3 + 0.85e2 / 0.84e2;
+    Q->crjgpLegendre_34[0][1] = -ap / 0.140e3 + r30 / 0.140e3 + 0.1e1 / 0.84e2;
+    Q->crjgpLegendre_34[1][1] = -r30 / 0.70e2 + 0.41e2 / 0.42e2;
+    Q->crjgpLegendre_34[2][1] = ap / 0.140e3 + r30 / 0.140e3 + 0.1e1 / 0.84e2;
+    Q->crjgpLegendre_34[0][2] = -0.3e1 / 0.140e3 * ap + r30 / 0.140e3 + 0.85e2 / 0.84e2;
+    Q->crjgpLegendre_34[1][2] = ap / 0.35e2 - r30 / 0.70e2 - 0.1e1 / 0.42e2;
+    Q->crjgpLegendre_34[2][2] = -ap / 0.140e3 + r30 / 0.140e3 + 0.1e1 / 0.84e2;
+    swap02(Q->crjgpLegendre_34, Q->crjgmLegendre_34);
+    denom = (0.1234800000e10 * r30 + 0.9116940000e10) * ap - 0.11174940000e11 * r30 - 0.52238900000e11;
+    Q->dkpLegendre_34[2] = ((0.26092500e8 * r30 + 0.256576250e9) * ap +
+                            (-0.2100e4 * r30 + 0.3838100e7) * pow(bp, 0.3e1 / 0.2e1) -
+                            0.630e3 * pow(bp, 0.5e1 / 0.2e1) - 0.5614665000e10 * r30 - 0.25504500000e11) / denom;
+    Q->dkpLegendre_34[1] = ((0.1761795000e10 * r30 + 0.13038532500e11) * ap +
+                            (-0.88200e5 * r30 - 0.16096500e8) * pow(bp, 0.3e1 / 0.2e1) +
+                            0.2520e4 * pow(bp, 0.5e1 / 0.2e1) - 0.6757590000e10 * r30 - 0.32331425000e11) / denom;
+    Q->dkpLegendre_34[0] = ((0.257250000e9 * r30 + 0.1899362500e10) * ap +
+                            (-0.42000e5 * r30 - 0.310100e6) * pow(bp, 0.3e1 / 0.2e1) + 0.1197315000e10 * r30 +
+                            0.5597025000e10) / denom;
+    Q->dkmLegendre_34[0] = Q->dkpLegendre_34[2];
+    Q->dkmLegendre_34[1] = Q->dkpLegendre_34[1];
+    Q->dkmLegendre_34[2] = Q->dkpLegendre_34[0];
+}
+
+/* p(k) = C.col(k).dot(u.segment(k, 3)) */
+static void cand3(const double C[3][3], const double u[5], double p[3]) {
+    double c[3];
+    for (int k = 0; k < 3; ++k) {
+        for (int r = 0; r < 3; ++r) c[r] = C[r][k];
+        p[k] = dot3(c, u + k);
+    }
+}
+
+/* bg, cg of smoothnessMeasure2PointsGaussLegendre :1086-1099 (and the 4-point
+ * forms :1101-1133), squared */
+static void bg_cg2(const double u[5], double bg[3], double cg[3]) {
+    bg[0] = u[0] - 4.0 * u[1] + 3.0 * u[2];
+    cg[0] = u[0] - 2.0 * u[1] + u[2];
+    bg[1] = u[1] - u[3];
+    cg[1] = u[1] - 2.0 * u[2] + u[3];
+    bg[2] = 3.0 * u[2] - 4.0 * u[3] + u[4];
+    cg[2] = u[2] - 2.0 * u[3] + u[4];
+    for (int k = 0; k < 3; ++k) {
+        bg[k] = pow(bg[k], 2);
+        cg[k] = pow(cg[k], 2);
+    }
+}
+
+/* weights of one quadrature point: 0 JS, 1 Z (SZG), 2 TENO (sigm) */
+static double qpoint(int kind, const double p[3], const double d[3], const double SI[3], double tau,
+                     const double sigm[3], int nan_guard) {
+    double w[3], s;
+    for (int k = 0; k < 3; ++k) {
+        if (kind == 0)
+            w[k] = d[k] / pow(LC.eps + SI[k], 2);
+        else if (kind == 1)
+            w[k] = d[k] * (1.0 + tau / (SI[k] + LC.epsz));
+        else
+            w[k] = sigm[k] * d[k];
+    }
+    s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    double r = dot3(p, w);
+    return nan_guard ? catchnan(r) : r;
+}
+
+/* cutOff of (1 + tau / (SI + epsz))^6; prenorm: normalised once before
+ * cutOff normalises (:962-963) */
+static void qsigm(const double SI[3], double tau, int prenorm, double sigm[3]) {
+    double w[3], s;
+    for (int k = 0; k < 3; ++k) w[k] = pow(1.0 + tau / (SI[k] + LC.epsz), 6);
+    if (prenorm) {
+        s = sum3(w);
+        for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    }
+    s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    for (int k = 0; k < 3; ++k) sigm[k] = w[k] < LC.ct ? 0.0 : 1.0;
+}
+
+/* weno5JS2PointsQuadrature (kind 0, :719-740), weno5Z2PointsQuadrature (1,
+ * :742-765), teno5th2PointsQuadrature (2, :767-791) */
+void orc_quad2(int kind, const double u[5], double out[2]) {
+    quad_tables Q;
+    double bg[3], cg[3], SIGLQ[3], p[3], sigm[3] = {1.0, 1.0, 1.0};
+    lag_init();
+    quad_tables_init(&Q);
+    bg_cg2(u, bg, cg);
+    for (int k = 0; k < 3; ++k) SIGLQ[k] = Q.c2 * bg[k] + Q.c3 * cg[k];
+    double tau = fabs(SIGLQ[0] - SIGLQ[2]);
+    if (kind == 2) qsigm(SIGLQ, tau, 0, sigm);
+    cand3(Q.crjgp, u, p);
+    out[0] = qpoint(kind, p, Q.dkpg, SIGLQ, tau, sigm, 0);
+    cand3(Q.crjgm, u, p);
+    out[1] = qpoint(kind, p, Q.dkmg, SIGLQ, tau, sigm, 0);
+}
+
+/* weno5JS4PointsGaussLobatto (0, :793-831), weno5Z4PointsGaussLobatto (1,
+ * :914-955, no catchnan), teno5th4PointsGaussLobatto (2, :957-1004) */
+void orc_quad4_lobatto(int kind, const double u[5], double out[4]) {
+    quad_tables Q;
+    double bg[3], cg[3], SIA[3], SIB[3], p[3], c[3], sigm[3] = {1.0, 1.0, 1.0};
+    lag_init();
+    quad_tables_init(&Q);
+    bg_cg2(u, bg, cg);
+    for (int k = 0; k < 3; ++k) {
+        SIA[k] = 0.25 * bg[k] + Q.c3 * cg[k];
+        SIB[k] = Q.c4 * bg[k] + Q.c5 * cg[k];
+    }
+    int ng = kind != 1;
+    double tau = fabs(SIA[0] - SIA[2]);
+    if (kind == 2) qsigm(SIA, tau, 1, sigm);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][1];
+    p[2] = dot3(c, u + 2);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][2];
+    p[1] = dot3(c, u + 1);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][3];
+    p[0] = dot3(c, u + 0);
+    out[0] = qpoint(kind, p, LC.dkm, SIA, tau, sigm, ng);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][0];
+    p[2] = dot3(c, u + 2);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][1];
+    p[1] = dot3(c, u + 1);
+    for (int r = 0; r < 3; ++r) c[r] = LC.crj[r][2];
+    p[0] = dot3(c, u + 0);
+    out[1] = qpoint(kind, p, LC.dkp, SIA, tau, sigm, ng);
+    tau = fabs(SIB[0] - SIB[2]);
+    if (kind == 2) qsigm(SIB, tau, 1, sigm);
+    cand3(Q.crjgpLobatto, u, p);
+    out[2] = qpoint(kind, p, Q.dkpLobatto, SIB, tau, sigm, ng);
+    cand3(Q.crjgmLobatto, u, p);
+    out[3] = qpoint(kind, p, Q.dkmLobatto, SIB, tau, sigm, ng);
+}
+
+/* weno5JS4PointsGaussLegendre (0, :833-869), weno5Z4PointsGaussLegendre (1,
+ * :871-912) */
+void orc_quad4_legendre(int kind, const double u[5], double out[4]) {
+    quad_tables Q;
+    double bg[3], cg[3], SIA[3], SIB[3], p[3], sigm[3] = {1.0, 1.0, 1.0};
+    lag_init();
+    quad_tables_init(&Q);
+    bg_cg2(u, bg, cg);
+    for (int k = 0; k < 3; ++k) {
+        SIA[k] = Q.c7 * bg[k] + Q.c8 * cg[k];
+        SIB[k] = Q.c9 * bg[k] + Q.c10 * cg[k];
+    }
+    double tau = fabs(SIA[0] - SIA[2]);
+    cand3(Q.crjgpLegendre_12, u, p);
+    out[0] = qpoint(kind, p, Q.dkpLegendre_12, SIA, tau, sigm, 1);
+    cand3(Q.crjgmLegendre_12, u, p);
+    out[1] = qpoint(kind, p, Q.dkmLegendre_12, SIA, tau, sigm, 1);
+    tau = fabs(SIB[0] - SIB[2]);
+    cand3(Q.crjgpLegendre_34, u, p);
+    out[2] = qpoint(kind, p, Q.dkpLegendre_34, SIB, tau, sigm, 1);
+    cand3(Q.crjgmLegendre_34, u, p);
+    out[3] = qpoint(kind, p, Q.dkmLegendre_34, SIB, tau, sigm, 1);
+}
+
 /* Gauss-Legendre 9-point rule in the reference's node order
  * (Schemes_Lagrange.cpp:22-42): 0, -a1, +a1, -a2, +a2, -a3, +a3, -a4, +a4. */
 void orc_gauss9(double nodes[9], double weights[9]) {
@@ -627,15 +1219,11 @@ static void weno5_impl(const double u[6], double out[2], int z);
 /* the reconstruction FV_System2D.cpp:379 / :394 calls (lagrangeObj.teno5),
  * or one of the alternatives a user of the reference swaps in there
  * (Schemes_Lagrange.cpp:358-408, :1235-1263) */
-static void recon2d(int recon, int linear, const double u[6], double out[2]) {
+static void recon2d(int recon, int linear, const double u[6], double diff, double out[2]) {
     if (linear)
         orc_linear5(u, out);
-    else if (recon == ORC_RECON_TENO5I)
-        teno5i_impl(u, out);
-    else if (recon == ORC_RECON_WENO5JS || recon == ORC_RECON_WENO5Z)
-        weno5_impl(u, out, recon == ORC_RECON_WENO5Z);
     else
-        teno5_impl(u, out, NULL);
+        orc_recon(recon, u, diff, out);
 }
 
 void orc_fv2d_rhs(const orc_params* p, const double* q, double* rhs) { fv2d_rhs_impl(p, q, rhs, 0); }
@@ -676,7 +1264,7 @@ static void fv2d_rhs_impl(const orc_params* p, const double* q, double* rhs, int
                 for (int e = 0; e < 4; ++e) C[e][m] = y[e];
             }
             for (int e = 0; e < 4; ++e) {
-                recon2d(p->recon, linear, C[e], out);
+                recon2d(p->recon, linear, C[e], dx, out);
                 uxl[e * nt + PIX(&P, i, j)] = out[0];
                 uxr[e * nt + PIX(&P, i, j)] = out[1];
             }
@@ -697,7 +1285,7 @@ static void fv2d_rhs_impl(const orc_params* p, const double* q, double* rhs, int
                 for (int e = 0; e < 4; ++e) C[e][m] = y[e];
             }
             for (int e = 0; e < 4; ++e) {
-                recon2d(p->recon, linear, C[e], out);
+                recon2d(p->recon, linear, C[e], dy, out);
                 uyl[e * nt + PIX(&P, i, j)] = out[0];
                 uyr[e * nt + PIX(&P, i, j)] = out[1];
             }
@@ -1092,6 +1680,327 @@ void orc_hteno_flags(double dif, const double u[4], const double d[4], int flags
     hteno_impl(&H, u, d, out, flags);
 }
 
+/* Hermite::hweno, Schemes_Hermite.cpp:234-293: hteno's candidates with
+ * weights dk / (SI + eps)^2 normalised, catchnan; eps = 1e-6 (:5-6) */
+static void hweno_impl(const herm_consts* H, const double u[4], const double d[4], double out[4]) {
+    double ud[4], p[3], SI[3], w[3], s;
+    const double eps = 1E-6;
+    for (int k = 0; k < 4; ++k) ud[k] = d[k] * H->dif;
+    for (int o = 0; o < 4; ++o) {
+        const double* dk;
+        if (o == 0) { /* u^- :238-251 */
+            p[0] = H->crj[0][0] * u[0] + H->crj[1][0] * u[1] + H->crj[2][0] * d[0];
+            p[1] = H->crj[0][1] * u[1] + H->crj[1][1] * u[2] + H->crj[2][1] * d[2];
+            p[2] = H->crj[0][2] * u[0] + H->crj[1][2] * u[1] + H->crj[2][2] * u[2];
+            herm_si(u, ud, SI);
+            dk = H->dkm;
+        } else if (o == 1) { /* u^+ :254-266 */
+            p[0] = H->crj[1][1] * u[1] + H->crj[0][1] * u[2] - H->crj[2][1] * d[1];
+            p[1] = H->crj[1][0] * u[2] + H->crj[0][0] * u[3] - H->crj[2][0] * d[3];
+            p[2] = H->crj[2][2] * u[1] + H->crj[1][2] * u[2] + H->crj[0][2] * u[3];
+            herm_si(u + 1, ud + 1, SI);
+            dk = H->dkp;
+        } else if (o == 2) { /* d^- :269-280 */
+            p[0] = H->crjd[0][0] * u[0] + H->crjd[1][0] * u[1] + H->crjd[2][0] * d[0] + H->crjd[3][0] * d[1];
+            p[1] = H->crjd[0][1] * u[1] + H->crjd[1][1] * u[2] + H->crjd[2][1] * d[1] + H->crjd[3][1] * d[2];
+            p[2] = H->crjd[0][2] * u[0] + H->crjd[1][2] * u[1] + H->crjd[2][2] * u[2] + H->crjd[3][2] * d[1];
+            herm_sid(u, ud, SI);
+            dk = H->dkmd;
+        } else { /* d^+ :283-290 */
+            p[0] = H->crjd[0][1] * u[1] + H->crjd[1][1] * u[2] + H->crjd[3][1] * d[1] + H->crjd[2][1] * d[2];
+            p[1] = H->crjd[0][0] * u[2] + H->crjd[1][0] * u[3] + H->crjd[3][0] * d[2] + H->crjd[2][0] * d[3];
+            p[2] = -(H->crjd[2][2] * u[1] + H->crjd[1][2] * u[2] + H->crjd[0][2] * u[3]) + H->crjd[3][2] * d[2];
+            herm_sid(u + 1, ud + 1, SI);
+            dk = H->dkpd;
+        }
+        for (int k = 0; k < 3; ++k) w[k] = dk[k] / pow(SI[k] + eps, 2);
+        s = sum3(w);
+        for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+        out[o] = catchnan(dot3(p, w));
+    }
+}
+
+/* Hermite::htenoi, :670-755: tau = SI.maxCoeff(), weights (C + tau / (SI +
+ * epsz))^6, C = 1, cutOff / cutOffd at ct = ctd = 1e-3 (:614-628) */
+static void htenoi_impl(const herm_consts* H, const double u[4], const double d[4], double out[4]) {
+    double ud[4], p[3], SI[3], w[3], sigm[3], s, tau;
+    for (int k = 0; k < 4; ++k) ud[k] = d[k] * H->dif;
+    const int order[4] = {0, 2, 1, 3}; /* u^-, d^-, u^+, d^+ as the source runs them */
+    for (int n = 0; n < 4; ++n) {
+        int o = order[n];
+        const double* dk;
+        if (o == 0) {
+            p[0] = H->crj[0][0] * u[0] + H->crj[1][0] * u[1] + H->crj[2][0] * d[0];
+            p[1] = H->crj[0][1] * u[1] + H->crj[1][1] * u[2] + H->crj[2][1] * d[2];
+            p[2] = H->crj[0][2] * u[0] + H->crj[1][2] * u[1] + H->crj[2][2] * u[2];
+            herm_si(u, ud, SI);
+            dk = H->dkm;
+        } else if (o == 2) {
+            p[0] = H->crjd[0][0] * u[0] + H->crjd[1][0] * u[1] + H->crjd[2][0] * d[0] + H->crjd[3][0] * d[1];
+            p[1] = H->crjd[0][1] * u[1] + H->crjd[1][1] * u[2] + H->crjd[2][1] * d[1] + H->crjd[3][1] * d[2];
+            p[2] = H->crjd[0][2] * u[0] + H->crjd[1][2] * u[1] + H->crjd[2][2] * u[2] + H->crjd[3][2] * d[1];
+            herm_sid(u, ud, SI);
+            dk = H->dkmd;
+        } else if (o == 1) {
+            p[0] = H->crj[1][1] * u[1] + H->crj[0][1] * u[2] - H->crj[2][1] * d[1];
+            p[1] = H->crj[1][0] * u[2] + H->crj[0][0] * u[3] - H->crj[2][0] * d[3];
+            p[2] = H->crj[2][2] * u[1] + H->crj[1][2] * u[2] + H->crj[0][2] * u[3];
+            herm_si(u + 1, ud + 1, SI);
+            dk = H->dkp;
+        } else {
+            p[0] = H->crjd[0][1] * u[1] + H->crjd[1][1] * u[2] + H->crjd[3][1] * d[1] + H->crjd[2][1] * d[2];
+            p[1] = H->crjd[0][0] * u[2] + H->crjd[1][0] * u[3] + H->crjd[3][0] * d[2] + H->crjd[2][0] * d[3];
+            p[2] = -(H->crjd[2][2] * u[1] + H->crjd[1][2] * u[2] + H->crjd[0][2] * u[3]) + H->crjd[3][2] * d[2];
+            herm_sid(u + 1, ud + 1, SI);
+            dk = H->dkpd;
+        }
+        tau = SI[0];
+        for (int k = 1; k < 3; ++k) tau = SI[k] > tau ? SI[k] : tau;
+        for (int k = 0; k < 3; ++k) w[k] = pow(1.0 + tau / (SI[k] + 1E-40), 6);
+        s = sum3(w);
+        for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+        for (int k = 0; k < 3; ++k) sigm[k] = w[k] < 1E-3 ? 0.0 : 1.0;
+        for (int k = 0; k < 3; ++k) w[k] = sigm[k] * dk[k];
+        s = sum3(w);
+        for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+        out[o] = catchnan(dot3(p, w));
+    }
+}
+
+/* Hermite::linear, :757-767, with crjlin / crjlind of setGridSize :67-92 */
+static void hlinear_impl(double dif, const double u[4], const double d[4], double out[4]) {
+    double crjlin[5][2], crjlind[6][2];
+    crjlin[0][0] = -23.0 / 120.0;
+    crjlin[1][0] = 19.0 / 30.0;
+    crjlin[2][0] = 67.0 / 120.0;
+    crjlin[3][0] = -3.0 / 40.0 * dif;
+    crjlin[4][0] = -7.0 / 40.0 * dif;
+    crjlin[0][1] = 67.0 / 120.0;
+    crjlin[1][1] = 19.0 / 30.0;
+    crjlin[2][1] = -23.0 / 120.0;
+    crjlin[3][1] = 7.0 / 40.0 * dif;
+    crjlin[4][1] = 3.0 / 40.0 * dif;
+    crjlind[0][0] = 0.25 / dif;
+    crjlind[1][0] = -2.0 / dif;
+    crjlind[2][0] = 1.75 / dif;
+    crjlind[3][0] = 1.0 / 12.0;
+    crjlind[4][0] = -1.0 / 6.0;
+    crjlind[5][0] = -5.0 / 12.0;
+    crjlind[0][1] = -7.0 / 4.0 / dif;
+    crjlind[1][1] = 2.0 / dif;
+    crjlind[2][1] = -0.25 / dif;
+    crjlind[3][1] = -5.0 / 12.0;
+    crjlind[4][1] = -1.0 / 6.0;
+    crjlind[5][1] = 1.0 / 12.0;
+    double c[3], e[3];
+    for (int r = 0; r < 3; ++r) c[r] = crjlin[r][0];
+    out[0] = dot3(c, u) + crjlin[3][0] * d[0] + crjlin[4][0] * d[2];
+    for (int r = 0; r < 3; ++r) c[r] = crjlin[r][1];
+    out[1] = dot3(c, u + 1) + crjlin[3][1] * d[1] + crjlin[4][1] * d[3];
+    for (int r = 0; r < 3; ++r) {
+        c[r] = crjlind[r][0];
+        e[r] = crjlind[3 + r][0];
+    }
+    out[2] = dot3(c, u) + dot3(e, d);
+    for (int r = 0; r < 3; ++r) {
+        c[r] = crjlind[r][1];
+        e[r] = crjlind[3 + r][1];
+    }
+    out[3] = dot3(c, u + 1) + dot3(e, d + 1);
+}
+
+/* the Hermite face reconstruction FVHermite_System2D.cpp:459 / :486 calls
+ * (hermiteObj.hteno), or an alternative swapped in: ORC_HRECON_* */
+static void herm_recon(const herm_consts* H, int kind, const double u[4], const double d[4], double out[4]) {
+    if (kind == ORC_HRECON_HWENO)
+        hweno_impl(H, u, d, out);
+    else if (kind == ORC_HRECON_HTENOI)
+        htenoi_impl(H, u, d, out);
+    else if (kind == ORC_HRECON_LINEAR)
+        hlinear_impl(H->dif, u, d, out);
+    else
+        hteno_impl(H, u, d, out, NULL);
+}
+
+void orc_herm_recon(int kind, double dif, const double u[4], const double d[4], double out[4]) {
+    herm_consts H;
+    lag_init();
+    herm_init(&H, dif);
+    herm_recon(&H, kind, u, d, out);
+}
+
+/* b, c of smoothnessMeasure2PointsGLQ / 4PointsGLQ (:543-581; deriv = 0)
+ * and of their derivative forms (:556-595; deriv = 1), squared */
+static void herm_bc(const double u[3], const double udf[3], int deriv, double b2[3], double c2[3]) {
+    double b[3], c[3];
+    if (!deriv) {
+        b[0] = -2.0 * (u[0] - u[1]) - udf[0];
+        c[0] = -2.0 * (u[0] - u[1]) - 2.0 * udf[0];
+        b[1] = -2.0 * (u[1] - u[2]) - udf[2];
+        c[1] = 2.0 * (u[1] - u[2]) + 2.0 * udf[2];
+        b[2] = -(u[0] - u[2]) / 2.0;
+        c[2] = u[0] - 2.0 * u[1] + u[2];
+    } else {
+        b[0] = 6.0 * (u[0] - u[1]) + (2.0 * udf[0] + 4.0 * udf[1]);
+        c[0] = 12.0 * (u[0] - u[1]) + 6.0 * (udf[0] + udf[1]);
+        b[1] = 6.0 * (u[2] - u[1]) - (4.0 * udf[1] + 2.0 * udf[2]);
+        c[1] = 12.0 * (u[1] - u[2]) + 6.0 * (udf[1] + udf[2]);
+        b[2] = u[0] - 2.0 * u[1] + u[2];
+        c[2] = 3.0 * (u[2] - u[0]) - 6.0 * udf[1];
+    }
+    for (int k = 0; k < 3; ++k) {
+        b2[k] = pow(b[k], 2);
+        c2[k] = pow(c[k], 2);
+    }
+}
+
+static double hjs(const double p[3], const double dk[3], const double SI[3]) {
+    double w[3], s;
+    for (int k = 0; k < 3; ++k) w[k] = dk[k] / pow(SI[k] + 1E-6, 2);
+    s = sum3(w);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / s;
+    return dot3(p, w);
+}
+
+/* Hermite::hweno2PointsGLQ :381-425 and hweno4PointsGLQ :427-513, with the
+ * coefficient tables of setGridSize :93-231 and c2..c7 of the constructor
+ * (:5-13) */
+void orc_hweno_glq(int npts, double dif, const double u[3], const double d[3], double* out) {
+    const double sq3 = sqrt(3.0), sq5 = sqrt(5.0);
+    const double c1 = 13.0 / 12.0, c2 = sqrt(3.0) / 3.0, c3 = 37.0 * sqrt(3.0) / 108.0;
+    const double c4 = sqrt(3.0) / 3.0, c5 = 37.0 * sqrt(3.0) / 108.0;
+    const double c6 = sqrt(5.0) / 5.0, c7 = 61.0 * sqrt(5.0) / 300.0;
+    double udglq[3], b2[3], c2v[3], SI[3], SI4[3], p[3], denom;
+    herm_consts H;
+    lag_init();
+    herm_init(&H, dif);
+    for (int k = 0; k < 3; ++k) udglq[k] = d[k] * dif;
+    if (npts == 2) {
+        double crjgp[3][3], crjgm[3][3], crjdgp[4][3], crjdgm[4][3], dkpg[3], dkmg[3], dkpdg[3], dkmdg[3];
+        crjgp[0][0] = -sq3 / 3.0; crjgp[1][0] = 1.0 + sq3 / 3.0; crjgp[2][0] = -dif * sq3 / 6.0;
+        crjgp[0][1] = 1.0 - sq3 / 3.0; crjgp[1][1] = sq3 / 3.0; crjgp[2][1] = -dif * sq3 / 6.0;
+        crjgp[0][2] = -sq3 / 12.0; crjgp[1][2] = 1.0; crjgp[2][2] = sq3 / 12.0;
+        crjgm[0][0] = sq3 / 3.0; crjgm[1][0] = 1.0 - sq3 / 3.0; crjgm[2][0] = dif * sq3 / 6.0;
+        crjgm[0][1] = 1.0 + sq3 / 3.0; crjgm[1][1] = -sq3 / 3.0; crjgm[2][1] = dif * sq3 / 6.0;
+        crjgm[0][2] = sq3 / 12.0; crjgm[1][2] = 1.0; crjgm[2][2] = -sq3 / 12.0;
+        denom = 360.0;
+        dkpg[0] = (-sq3 + 105.0) / denom; dkpg[1] = (sq3 + 105.0) / denom; dkpg[2] = 5.0 / 12.0;
+        dkmg[0] = (sq3 + 105.0) / denom; dkmg[1] = (-sq3 + 105.0) / denom; dkmg[2] = 5.0 / 12.0;
+        crjdgp[0][0] = sq3 / dif; crjdgp[1][0] = -sq3 / dif; crjdgp[2][0] = sq3 / 3.0;
+        crjdgp[3][0] = (2.0 * sq3 + 3.0) / 3.0;
+        crjdgp[0][1] = -sq3 / dif; crjdgp[1][1] = sq3 / dif; crjdgp[2][1] = (-2.0 * sq3 + 3.0) / 3.0;
+        crjdgp[3][1] = -sq3 / 3.0;
+        crjdgp[0][2] = sq3 / 6.0 / dif; crjdgp[1][2] = -sq3 / 3.0 / dif; crjdgp[2][2] = sq3 / 6.0 / dif;
+        crjdgp[3][2] = 1.0;
+        crjdgm[0][0] = -sq3 / dif; crjdgm[1][0] = sq3 / dif; crjdgm[2][0] = -sq3 / 3.0;
+        crjdgm[3][0] = (-2.0 * sq3 + 3.0) / 3.0;
+        crjdgm[0][1] = sq3 / dif; crjdgm[1][1] = -sq3 / dif; crjdgm[2][1] = (2.0 * sq3 + 3.0) / 3.0;
+        crjdgm[3][1] = sq3 / 3.0;
+        crjdgm[0][2] = -sq3 / 6.0 / dif; crjdgm[1][2] = sq3 / 3.0 / dif; crjdgm[2][2] = -sq3 / 6.0 / dif;
+        crjdgm[3][2] = 1.0;
+        dkpdg[0] = -sq3 / 144.0 + 1.0 / 3.0; dkpdg[1] = sq3 / 144.0 + 1.0 / 3.0; dkpdg[2] = 1.0 / 3.0;
+        dkmdg[0] = sq3 / 144.0 + 1.0 / 3.0; dkmdg[1] = -sq3 / 144.0 + 1.0 / 3.0; dkmdg[2] = 1.0 / 3.0;
+        herm_bc(u, udglq, 0, b2, c2v);
+        for (int k = 0; k < 3; ++k) SI[k] = c2 * b2[k] + c3 * c2v[k];
+        p[0] = crjgp[0][0] * u[0] + crjgp[1][0] * u[1] + crjgp[2][0] * d[0];
+        p[1] = crjgp[0][1] * u[1] + crjgp[1][1] * u[2] + crjgp[2][1] * d[2];
+        p[2] = crjgp[0][2] * u[0] + crjgp[1][2] * u[1] + crjgp[2][2] * u[2];
+        out[0] = hjs(p, dkpg, SI);
+        p[0] = crjgm[0][0] * u[0] + crjgm[1][0] * u[1] + crjgm[2][0] * d[0];
+        p[1] = crjgm[0][1] * u[1] + crjgm[1][1] * u[2] + crjgm[2][1] * d[2];
+        p[2] = crjgm[0][2] * u[0] + crjgm[1][2] * u[1] + crjgm[2][2] * u[2];
+        out[1] = hjs(p, dkmg, SI);
+        herm_bc(u, udglq, 1, b2, c2v);
+        for (int k = 0; k < 3; ++k) SI[k] = c4 * b2[k] + c5 * c2v[k];
+        p[0] = crjdgp[0][0] * u[0] + crjdgp[1][0] * u[1] + crjdgp[2][0] * d[0] + crjdgp[3][0] * d[1];
+        p[1] = crjdgp[0][1] * u[1] + crjdgp[1][1] * u[2] + crjdgp[2][1] * d[1] + crjdgp[3][1] * d[2];
+        p[2] = crjdgp[0][2] * u[0] + crjdgp[1][2] * u[1] + crjdgp[2][2] * u[2] + crjdgp[3][2] * d[1];
+        out[2] = hjs(p, dkpdg, SI);
+        p[0] = crjdgm[0][0] * u[0] + crjdgm[1][0] * u[1] + crjdgm[2][0] * d[0] + crjdgm[3][0] * d[1];
+        p[1] = crjdgm[0][1] * u[1] + crjdgm[1][1] * u[2] + crjdgm[2][1] * d[1] + crjdgm[3][1] * d[2];
+        p[2] = crjdgm[0][2] * u[0] + crjdgm[1][2] * u[1] + crjdgm[2][2] * u[2] + crjdgm[3][2] * d[1];
+        out[3] = hjs(p, dkmdg, SI);
+        return;
+    }
+    double crjg4p[3][3], crjg4m[3][3], crjdg4p[4][3], crjdg4m[4][3], dkpg4[3], dkmg4[3], dkpdg4[3], dkmdg4[3];
+    crjg4p[0][0] = -sq5 / 5.0 + 1.0 / 30.0; crjg4p[1][0] = sq5 / 5.0 + 29.0 / 30.0;
+    crjg4p[2][0] = (-sq5 / 10.0 + 1.0 / 30.0) * dif;
+    crjg4p[0][1] = -sq5 / 5.0 + 29.0 / 30.0; crjg4p[1][1] = sq5 / 5.0 + 1.0 / 30.0;
+    crjg4p[2][1] = -(sq5 / 10.0 + 1.0 / 30.0) * dif;
+    crjg4p[0][2] = -(sq5 / 20.0 + 1.0 / 60.0); crjg4p[1][2] = 31.0 / 30.0; crjg4p[2][2] = sq5 / 20.0 - 1.0 / 60.0;
+    crjg4m[0][0] = sq5 / 5.0 + 1.0 / 30.0; crjg4m[1][0] = -sq5 / 5.0 + 29.0 / 30.0;
+    crjg4m[2][0] = (sq5 / 10.0 + 1.0 / 30.0) * dif;
+    crjg4m[0][1] = sq5 / 5.0 + 29.0 / 30.0; crjg4m[1][1] = -sq5 / 5.0 + 1.0 / 30.0;
+    crjg4m[2][1] = (sq5 / 10.0 - 1.0 / 30.0) * dif;
+    crjg4m[0][2] = sq5 / 20.0 - 1.0 / 60.0; crjg4m[1][2] = 31.0 / 30.0; crjg4m[2][2] = -(sq5 / 20.0 + 1.0 / 60.0);
+    denom = 330.0 * sq5 - 110.0;
+    dkpg4[0] = (99.0 * sq5 + 33.0) / denom; dkpg4[1] = (108.0 * sq5 - 102.0) / denom;
+    dkpg4[2] = (123.0 * sq5 - 41.0) / denom;
+    denom = 330.0 * sq5 + 110.0;
+    dkmg4[0] = (99.0 * sq5 - 33.0) / denom; dkmg4[1] = (108.0 * sq5 + 102.0) / denom;
+    dkmg4[2] = (123.0 * sq5 + 41.0) / denom;
+    crjdg4p[0][0] = (6.0 * sq5 - 2.0) / 10.0 / dif; crjdg4p[1][0] = (-6.0 * sq5 + 2.0) / 10.0 / dif;
+    crjdg4p[2][0] = 0.2 * sq5 - 0.1; crjdg4p[3][0] = 0.4 * sq5 + 0.9;
+    crjdg4p[0][1] = -(6.0 * sq5 + 2.0) / 10.0 / dif; crjdg4p[1][1] = (6.0 * sq5 + 2.0) / 10.0 / dif;
+    crjdg4p[2][1] = -0.4 * sq5 + 0.9; crjdg4p[3][1] = -0.2 * sq5 - 0.1;
+    crjdg4p[0][2] = (2.0 * sq5 + 1.0) / 20.0 / dif; crjdg4p[1][2] = -sq5 / 5.0 / dif;
+    crjdg4p[2][2] = (2.0 * sq5 - 1.0) / 20.0 / dif; crjdg4p[3][2] = 1.1;
+    crjdg4m[0][0] = -(6.0 * sq5 + 2.0) / 10.0 / dif; crjdg4m[1][0] = (6.0 * sq5 + 2.0) / 10.0 / dif;
+    crjdg4m[2][0] = -0.2 * sq5 - 0.1; crjdg4m[3][0] = -0.4 * sq5 + 0.9;
+    crjdg4m[0][1] = (6.0 * sq5 - 2.0) / 10.0 / dif; crjdg4m[1][1] = (-6.0 * sq5 + 2.0) / 10.0 / dif;
+    crjdg4m[2][1] = 0.4 * sq5 + 0.9; crjdg4m[3][1] = 0.2 * sq5 - 0.1;
+    crjdg4m[0][2] = (-2.0 * sq5 + 1.0) / 20.0 / dif; crjdg4m[1][2] = sq5 / 5.0 / dif;
+    crjdg4m[2][2] = (-2.0 * sq5 - 1.0) / 20.0 / dif; crjdg4m[3][2] = 1.1;
+    denom = 1140.0 * sq5 - 570.0;
+    dkpdg4[0] = (399.0 * sq5 + 190.0) / denom; dkpdg4[1] = (481.0 * sq5 - 630.0) / denom;
+    dkpdg4[2] = (260.0 * sq5 - 130.0) / denom;
+    denom = 1140.0 * sq5 + 570.0;
+    dkmdg4[0] = (399.0 * sq5 - 190.0) / denom; dkmdg4[1] = (481.0 * sq5 + 630.0) / denom;
+    dkmdg4[2] = (260.0 * sq5 + 130.0) / denom;
+    herm_bc(u, udglq, 0, b2, c2v);
+    for (int k = 0; k < 3; ++k) {
+        SI[k] = c1 * c2v[k] + b2[k];
+        SI4[k] = c6 * b2[k] + c7 * c2v[k];
+    }
+    p[0] = H.crj[0][0] * u[0] + H.crj[1][0] * u[1] + H.crj[2][0] * d[0];
+    p[1] = H.crj[0][1] * u[1] + H.crj[1][1] * u[2] + H.crj[2][1] * d[2];
+    p[2] = H.crj[0][2] * u[0] + H.crj[1][2] * u[1] + H.crj[2][2] * u[2];
+    out[0] = hjs(p, H.dkm, SI);
+    p[0] = H.crj[1][1] * u[0] + H.crj[0][1] * u[1] - H.crj[2][1] * d[0];
+    p[1] = H.crj[1][0] * u[1] + H.crj[0][0] * u[2] - H.crj[2][0] * d[2];
+    p[2] = H.crj[2][2] * u[0] + H.crj[1][2] * u[1] + H.crj[0][2] * u[2];
+    out[1] = hjs(p, H.dkp, SI);
+    p[0] = crjg4p[0][0] * u[0] + crjg4p[1][0] * u[1] + crjg4p[2][0] * d[0];
+    p[1] = crjg4p[0][1] * u[1] + crjg4p[1][1] * u[2] + crjg4p[2][1] * d[2];
+    p[2] = crjg4p[0][2] * u[0] + crjg4p[1][2] * u[1] + crjg4p[2][2] * u[2];
+    out[2] = hjs(p, dkpg4, SI4);
+    p[0] = crjg4m[0][0] * u[0] + crjg4m[1][0] * u[1] + crjg4m[2][0] * d[0];
+    p[1] = crjg4m[0][1] * u[1] + crjg4m[1][1] * u[2] + crjg4m[2][1] * d[2];
+    p[2] = crjg4m[0][2] * u[0] + crjg4m[1][2] * u[1] + crjg4m[2][2] * u[2];
+    out[3] = hjs(p, dkmg4, SI4);
+    herm_bc(u, udglq, 1, b2, c2v);
+    for (int k = 0; k < 3; ++k) {
+        SI[k] = c1 * c2v[k] + b2[k];
+        SI4[k] = c6 * b2[k] + c7 * c2v[k];
+    }
+    p[0] = H.crjd[0][0] * u[0] + H.crjd[1][0] * u[1] + H.crjd[2][0] * d[0] + H.crjd[3][0] * d[1];
+    p[1] = H.crjd[0][1] * u[1] + H.crjd[1][1] * u[2] + H.crjd[2][1] * d[1] + H.crjd[3][1] * d[2];
+    p[2] = H.crjd[0][2] * u[0] + H.crjd[1][2] * u[1] + H.crjd[2][2] * u[2] + H.crjd[3][2] * d[1];
+    out[4] = hjs(p, H.dkmd, SI);
+    p[0] = H.crjd[0][1] * u[0] + H.crjd[1][1] * u[1] + H.crjd[3][1] * d[0] + H.crjd[2][1] * d[1];
+    p[1] = H.crjd[0][0] * u[1] + H.crjd[1][0] * u[2] + H.crjd[3][0] * d[1] + H.crjd[2][0] * d[2];
+    p[2] = -(H.crjd[2][2] * u[0] + H.crjd[1][2] * u[1] + H.crjd[0][2] * u[2]) + H.crjd[3][2] * d[1];
+    out[5] = hjs(p, H.dkpd, SI);
+    p[0] = crjdg4p[0][0] * u[0] + crjdg4p[1][0] * u[1] + crjdg4p[2][0] * d[0] + crjdg4p[3][0] * d[1];
+    p[1] = crjdg4p[0][1] * u[1] + crjdg4p[1][1] * u[2] + crjdg4p[2][1] * d[1] + crjdg4p[3][1] * d[2];
+    p[2] = crjdg4p[0][2] * u[0] + crjdg4p[1][2] * u[1] + crjdg4p[2][2] * u[2] + crjdg4p[3][2] * d[1];
+    out[6] = hjs(p, dkpdg4, SI4);
+    p[0] = crjdg4m[0][0] * u[0] + crjdg4m[1][0] * u[1] + crjdg4m[2][0] * d[0] + crjdg4m[3][0] * d[1];
+    p[1] = crjdg4m[0][1] * u[1] + crjdg4m[1][1] * u[2] + crjdg4m[2][1] * d[1] + crjdg4m[3][1] * d[2];
+    p[2] = crjdg4m[0][2] * u[0] + crjdg4m[1][2] * u[1] + crjdg4m[2][2] * u[2] + crjdg4m[3][2] * d[1];
+    out[7] = hjs(p, dkmdg4, SI4);
+}
+
 /* Hermite::hermiteLimiter, Schemes_Hermite.cpp:634-662 */
 static double herm_limiter(double dif, const double u[3], const double d[3]) {
     double p[3], lin[3], sm[3], w[3], ud[3];
@@ -1264,7 +2173,7 @@ static void fvh2d_rhs_impl(const orc_params* p, const double* q, double* rhs, do
             }
             size_t id = PIX(&P, i, j);
             for (int e = 0; e < 4; ++e) {
-                hteno_impl(&H, A[e], B[e], o4, NULL);
+                herm_recon(&H, p->recon, A[e], B[e], o4);
                 SER(FX, 0, e, id) = o4[0];
                 SER(FX, 1, e, id) = o4[1];
                 SER(FX, 2, e, id) = o4[2];
@@ -1297,7 +2206,7 @@ static void fvh2d_rhs_impl(const orc_params* p, const double* q, double* rhs, do
             }
             size_t id = PIX(&P, i, j);
             for (int e = 0; e < 4; ++e) {
-                hteno_impl(&H, A[e], B[e], o4, NULL);
+                herm_recon(&H, p->recon, A[e], B[e], o4);
                 SER(FY, 0, e, id) = o4[0];  /* uyl */
                 SER(FY, 1, e, id) = o4[1];  /* uyr */
                 SER(FY, 4, e, id) = o4[2];  /* wyl */
@@ -1642,12 +2551,7 @@ void orc_fv1d_rhs(int nx, double xl, double xr, double gamma, int recon, int flu
             for (int e = 0; e < 3; ++e) C[e][m] = y[e];
         }
         for (int e = 0; e < 3; ++e) {
-            if (recon == 1)
-                teno5i_impl(C[e], o);
-            else if (recon == 2 || recon == 3)
-                weno5_impl(C[e], o, recon == 3);
-            else
-                teno5_impl(C[e], o, NULL);
+            orc_recon(recon, C[e], dx, o);
             ul[e] = o[0];
             ur[e] = o[1];
         }

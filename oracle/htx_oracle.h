@@ -31,7 +31,14 @@ typedef struct {
     int recon;       /* 2D characteristic reconstruction: ORC_RECON_* (the reference's is TENO5) */
 } orc_params;
 
-enum { ORC_RECON_TENO5 = 0, ORC_RECON_TENO5I = 1, ORC_RECON_WENO5JS = 2, ORC_RECON_WENO5Z = 3 };
+enum {
+    ORC_RECON_TENO5 = 0, ORC_RECON_TENO5I = 1, ORC_RECON_WENO5JS = 2, ORC_RECON_WENO5Z = 3,
+    ORC_RECON_WENO5N = 4, ORC_RECON_TENO5N = 5, ORC_RECON_TENO5A = 6, ORC_RECON_WENO5IS = 7,
+    ORC_RECON_WENO5IS_EXT = 8, ORC_RECON_WENO5ISMOD = 9, ORC_RECON_WENO5Z_DIFF = 10, ORC_RECON_TENO5_DIFF = 11,
+    ORC_RECON_LINEAR = 12
+};
+/* the Hermite solver's face reconstruction (orc_fvh2d_*: recon) */
+enum { ORC_HRECON_HTENO = 0, ORC_HRECON_HWENO = 1, ORC_HRECON_HTENOI = 2, ORC_HRECON_LINEAR = 3 };
 
 /* ---- scalar building blocks (unit-test surface) ---- */
 void orc_teno5(const double u[6], double out[2]);
@@ -83,6 +90,16 @@ void orc_ic_vortex_hermite(const orc_params* p, double cx, double cy, const doub
 /* ---- 1D FV (FV::System1D): 3 fields ---- */
 void orc_teno5i(const double u[6], double out[2]);
 void orc_weno5(const double u[6], int z, double out[2]); /* z: 0 weno5JS, 1 weno5Z */
+/* the remaining Lagrange and Hermite routines (Schemes_Lagrange.cpp:315-1004,
+ * :1265-1294; Schemes_Hermite.cpp:234-293, :381-513, :670-767) */
+void orc_recon(int recon, const double u[6], double diff, double out[2]);
+void orc_weno3js(const double u[4], double out[2]);
+void orc_weno3js_2q(const double u[3], double out[2]);
+void orc_quad2(int kind, const double u[5], double out[2]);          /* 0 JS, 1 Z, 2 TENO */
+void orc_quad4_lobatto(int kind, const double u[5], double out[4]);  /* 0 JS, 1 Z, 2 TENO */
+void orc_quad4_legendre(int kind, const double u[5], double out[4]); /* 0 JS, 1 Z */
+void orc_herm_recon(int kind, double dif, const double u[4], const double d[4], double out[4]);
+void orc_hweno_glq(int npts, double dif, const double u[3], const double d[3], double* out); /* npts 2 or 4 */
 void orc_e1_hllc(double gamma, int quirks, const double ql[3], const double qr[3], double f[3]);
 void orc_fv1d_rhs(int nx, double xl, double xr, double gamma, int recon, int flux, int quirks,
                   const double* q, double* rhs);

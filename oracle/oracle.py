@@ -70,9 +70,23 @@ _SIG = {
     "orc_weno5": (None, [_DP, _I, _DP]),
     "orc_teno5_indicators": (None, [_DP, _DP, _DP]),
     "orc_hteno_indicators": (None, [_DP, _DP, _DP, _DP, _DP]),
+    "orc_recon": (None, [_I, _DP, _D, _DP]),
+    "orc_weno3js": (None, [_DP, _DP]),
+    "orc_weno3js_2q": (None, [_DP, _DP]),
+    "orc_quad2": (None, [_I, _DP, _DP]),
+    "orc_quad4_lobatto": (None, [_I, _DP, _DP]),
+    "orc_quad4_legendre": (None, [_I, _DP, _DP]),
+    "orc_herm_recon": (None, [_I, _D, _DP, _DP, _DP]),
+    "orc_hweno_glq": (None, [_I, _D, _DP, _DP, _DP]),
 }
 
 _libs = {}
+
+# reconstruction names: the 5-point routines a 1D or 2D solver can take
+# (ORC_RECON_*), and the Hermite face reconstructions (ORC_HRECON_*)
+RECON = {"teno5": 0, "teno5i": 1, "weno5js": 2, "weno5z": 3, "weno5n": 4, "teno5n": 5, "teno5a": 6,
+         "weno5is": 7, "weno5is_ext": 8, "weno5ismod": 9, "weno5z_diff": 10, "teno5_diff": 11, "linear": 12}
+HRECON = {"hteno": 0, "hweno": 1, "htenoi": 2, "hlinear": 3}
 
 
 def load(omp=False):
@@ -97,9 +111,11 @@ def _p(a):
 
 def params(nx, ny, xl=0.0, xr=1.0, yb=0.0, yt=1.0, gamma=1.4, bc="zero_grad", flux="hllc", quirks=True,
            recon="teno5"):
+    """recon: a RECON name for the TENO5 solver, an HRECON name for the
+    Hermite solver (orc_fvh2d_*)."""
     return OrcParams(nx, ny, xl, xr, yb, yt, gamma, {"zero_grad": 0, "periodic": 1}[bc],
                      {"rusanov": 0, "hllc": 1}[flux], 1 if quirks else 0,
-                     {"teno5": 0, "teno5i": 1, "weno5js": 2, "weno5z": 3}[recon])
+                     HRECON[recon] if recon in HRECON else RECON[recon])
 
 
 class Oracle:
@@ -334,7 +350,7 @@ class Oracle:
     def fv1d_rhs(self, nx, xl, xr, gamma, q, recon="teno5", flux="hllc", quirks=True):
         q, pq = _p(q)
         r = np.empty_like(q)
-        self.lib.orc_fv1d_rhs(nx, xl, xr, gamma, {"teno5": 0, "teno5i": 1, "weno5js": 2, "weno5z": 3}[recon], {"rusanov": 0, "hllc": 1}[flux],
+        self.lib.orc_fv1d_rhs(nx, xl, xr, gamma, RECON[recon], {"rusanov": 0, "hllc": 1}[flux],
                               1 if quirks else 0, pq, r.ctypes.data_as(_DP))
         return r
 
@@ -342,7 +358,7 @@ class Oracle:
                      max_steps=0):
         q = np.array(q, dtype=np.float64, copy=True, order="C")
         t = _D()
-        n = self.lib.orc_fv1d_advance(nx, xl, xr, gamma, {"teno5": 0, "teno5i": 1, "weno5js": 2, "weno5z": 3}[recon],
+        n = self.lib.orc_fv1d_advance(nx, xl, xr, gamma, RECON[recon],
                                       {"rusanov": 0, "hllc": 1}[flux], 1 if quirks else 0, q.ctypes.data_as(_DP),
                                       cfl, t0, tfinal, max_steps, ctypes.byref(t))
         return q, t.value, n
@@ -390,3 +406,52 @@ class Oracle:
         SI, SId, b = np.empty(3), np.empty(3), np.empty(2)
         self.lib.orc_hteno_indicators(pu, pd, SI.ctypes.data_as(_DP), SId.ctypes.data_as(_DP), b.ctypes.data_as(_DP))
         return SI, SId, b
+
+    # the remaining Lagrange / Hermite routines
+    def recon(self, kind, u, diff=0.0):
+        """A 5-point reconstruction (RECON name) of a 6-value window:
+        (u-, u+) at its centre face; diff for the routines that take one."""
+        u, pu = _p(u)
+        out = np.empty(2)
+        self.lib.orc_recon(RECON[kind], pu, diff, out.ctypes.data_as(_DP))
+        return out
+
+    def weno3js(self, u):
+        u, pu = _p(u)
+        out = np.empty(2)
+        self.lib.orc_weno3js(pu, out.ctypes.data_as(_DP))
+        return out
+
+    def weno3js_2q(self, u):
+        u, pu = _p(u)
+        out = np.empty(2)
+        self.lib.orc_weno3js_2q(pu, out.ctypes.data_as(_DP))
+        return out
+
+    def quad2(self, kind, u):
+        """kind 0 weno5JS2PointsQuadrature, 1 weno5Z2..., 2 teno5th2..."""
+        u, pu = _p(u)
+        out = np.empty(2)
+        self.lib.orc_quad2(kind, pu, out.ctypes.data_as(_DP))
+        return out
+
+    def quad4(self, kind, u, lobatto=True):
+        u, pu = _p(u)
+        out = np.empty(4)
+        (self.lib.orc_quad4_lobatto if lobatto else self.lib.orc_quad4_legendre)(kind, pu, out.ctypes.data_as(_DP))
+        return out
+
+    def herm_recon(self, kind, dif, u, d):
+        """A Hermite face reconstruction (HRECON name): (u-, u+, d-, d+)."""
+        u, pu = _p(u)
+        d, pd = _p(d)
+        out = np.empty(4)
+        self.lib.orc_herm_recon(HRECON[kind], dif, pu, pd, out.ctypes.data_as(_DP))
+        return out
+
+    def hweno_glq(self, npts, dif, u, d):
+        u, pu = _p(u)
+        d, pd = _p(d)
+        out = np.empty(4 if npts == 2 else 8)
+        self.lib.orc_hweno_glq(npts, dif, pu, pd, out.ctypes.data_as(_DP))
+        return out

@@ -94,8 +94,48 @@ int main() {
         for (int m = 0; m < 5; ++m) v(m) = st[k](m);
         put(("lag.quad3." + n).c_str(), lag.weno5JS3PointsQuadrature(v));
     }
-    expect_throw("lag.teno5a", [&] { lag.teno5a(s1); });
-    expect_throw("lag.weno5N", [&] { lag.weno5N(s1); });
+    // the remaining reconstructions and quadratures on eight stencils: the
+    // two above, a flat one, a linear one, and four drawn from a fixed LCG
+    Vector6d sv[8];
+    sv[0] = s1;
+    sv[1] = s2;
+    sv[2] << 2.0, 2.0, 2.0, 2.0, 2.0, 2.0;
+    sv[3] << 0.5, 0.75, 1.0, 1.25, 1.5, 1.75;
+    unsigned long long lcg = 0x48544E4F00ull;
+    for (int k = 4; k < 8; ++k)
+        for (int m = 0; m < 6; ++m) {
+            lcg = lcg * 6364136223846793005ull + 1442695040888963407ull;
+            sv[k](m) = 0.5 + (double)(lcg >> 11) / 9007199254740992.0 * (k == 7 ? 1e-3 : 1.0);
+        }
+    for (int k = 0; k < 8; ++k) {
+        std::string n = "." + std::to_string(k);
+        const Vector6d& u = sv[k];
+        put(("lag.in" + n).c_str(), u);
+        put(("lag.weno5N" + n).c_str(), lag.weno5N(u));
+        put(("lag.teno5N" + n).c_str(), lag.teno5N(u));
+        put(("lag.teno5a" + n).c_str(), lag.teno5a(u));
+        put(("lag.weno5IS" + n).c_str(), lag.weno5IS(u));
+        put(("lag.weno5ISmod" + n).c_str(), lag.weno5ISmod(u, 0.05));
+        put(("lag.weno5IS_Extended" + n).c_str(), lag.weno5IS_Extended(u));
+        put(("lag.weno5Zd" + n).c_str(), lag.weno5Z(u, 0.05));
+        put(("lag.teno5d" + n).c_str(), lag.teno5(u, 0.05));
+        Vector4d u4;
+        Vector3d u3;
+        Vector5d u5;
+        for (int m = 0; m < 5; ++m) u5(m) = u(m);
+        for (int m = 0; m < 4; ++m) u4(m) = u(m);
+        for (int m = 0; m < 3; ++m) u3(m) = u(m);
+        put(("lag.weno3JS" + n).c_str(), lag.weno3JS(u4));
+        put(("lag.weno3JS2q" + n).c_str(), lag.weno3JS2PointsQuadrature(u3));
+        put(("lag.q2js" + n).c_str(), lag.weno5JS2PointsQuadrature(u5));
+        put(("lag.q2z" + n).c_str(), lag.weno5Z2PointsQuadrature(u5));
+        put(("lag.q2teno" + n).c_str(), lag.teno5th2PointsQuadrature(u5));
+        put(("lag.q4lojs" + n).c_str(), lag.weno5JS4PointsGaussLobatto(u5));
+        put(("lag.q4loz" + n).c_str(), lag.weno5Z4PointsGaussLobatto(u5));
+        put(("lag.q4loteno" + n).c_str(), lag.teno5th4PointsGaussLobatto(u5));
+        put(("lag.q4lejs" + n).c_str(), lag.weno5JS4PointsGaussLegendre(u5));
+        put(("lag.q4lez" + n).c_str(), lag.weno5Z4PointsGaussLegendre(u5));
+    }
 
     Schemes::Hermite her;
     expect_throw("her.unset", [&] { her.hteno(Vector4d(), Vector4d()); });
@@ -109,7 +149,28 @@ int main() {
     ld << 0.5, 1.0, -2.0;
     put("her.limiter", her.hermiteLimiter(lu, ld));
     put("her.kxrcf", her.KXRCF(1.0, 1.001, 1.0) ? 1.0 : 0.0);
-    expect_throw("her.hweno", [&] { her.hweno(hu, hd); });
+    // the Hermite alternatives on the stencils above (cell averages sv[k](0..3),
+    // moments from sv[(k + 3) % 8] scaled)
+    for (int k = 0; k < 8; ++k) {
+        std::string n = "." + std::to_string(k);
+        Vector4d u4, d4;
+        Vector3d u3, d3;
+        for (int m = 0; m < 4; ++m) {
+            u4(m) = sv[k](m);
+            d4(m) = 10.0 * (sv[(k + 3) % 8](m + 1) - 1.0);
+        }
+        for (int m = 0; m < 3; ++m) {
+            u3(m) = u4(m);
+            d3(m) = d4(m);
+        }
+        put(("her.hweno" + n).c_str(), her.hweno(u4, d4));
+        put(("her.htenoi" + n).c_str(), her.htenoi(u4, d4));
+        put(("her.linear" + n).c_str(), her.linear(u4, d4));
+        put(("her.glq2" + n).c_str(), her.hweno2PointsGLQ(u3, d3));
+        put(("her.glq4" + n).c_str(), her.hweno4PointsGLQ(u3, d3));
+        put(("her.in" + n).c_str(), u4);
+        put(("her.din" + n).c_str(), d4);
+    }
     try {
         her.setGridSize(0.0);
         std::printf("her.zero no-throw\n");

@@ -54,7 +54,8 @@ static void run(Fv2dArgs a, int flux, const unsigned char* fx = nullptr, const u
 // mixed: 0 FP64, 1 FP32-mixed, 2 learned with every face flagged smooth,
 // 3 FP32 bulk, 4 learned on the FP32 bulk arithmetic, every face flagged;
 // 5, 6, 7: FP64 with teno5i, weno5JS, weno5Z in place of TENO5; 8, 9: FP64 on
-// 16 x 15 and 15 x 16 tiles
+// 16 x 15 and 15 x 16 tiles; 100 + k: FP64 with reconstruction code k
+// (RECON_WENO5N .. RECON_LAST, htx_variants.h) in place of TENO5
 extern "C" int fv2dh_rhs_prec(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc,
                               int flux, int quirks, int mixed, const double* q, double* rhs) {
     Fv2dArgs a{};
@@ -80,7 +81,10 @@ extern "C" int fv2dh_rhs_prec(int nx, int ny, double xl, double xr, double yb, d
     else if (mixed == 9)
         run<15, 16>(a, fk);
 
-    else if (mixed == 5)
+    else if (mixed >= 100) {  // 100 + k: the run-time variant k (RECON_WENO5N ..) on the product's 16 x 15 tiles
+        a.recon = mixed - 100;
+        run<16, 15, false, false, double, RECON_WENO5N>(a, fk);
+    } else if (mixed == 5)
         run<16, 16, false, false, double, 1>(a, fk);
     else if (mixed == 6)
         run<16, 16, false, false, double, 2>(a, fk);

@@ -8,9 +8,9 @@
 
 using namespace htx;
 
-template <bool MIXED>
+template <bool MIXED, bool VARIANT = false>
 static int rhs_impl(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc, int flux,
-                    int quirks, const double* q, double* rhs, double* qlim) {
+                    int quirks, const double* q, double* rhs, double* qlim, int hrecon = 0) {
     FvhArgs a{};
     size_t n = (size_t)nx * ny;
     std::vector<double> lv(8 * n), fx(24 * (size_t)ny * (nx + 1)), fy(24 * (size_t)(ny + 1) * nx),
@@ -32,6 +32,7 @@ static int rhs_impl(int nx, int ny, double xl, double xr, double yb, double yt, 
     a.mode = 0;
     a.quirks = quirks;
     a.mixed = MIXED;
+    a.hrecon = hrecon;
     a.gam = gamma;
     a.g1 = gamma - 1.0;
     a.dx = (xr - xl) / nx;
@@ -47,9 +48,10 @@ static int rhs_impl(int nx, int ny, double xl, double xr, double yb, double yt, 
     long long fxs = (long long)ny * (nx + 1), fys = (long long)(ny + 1) * nx;
     for (int j = 0; j < ny; ++j)
         for (int fi = 0; fi <= nx; ++fi)
-            fvh_recon_face<false, MIXED>(a, fi - 1, j, a.fx, (long long)j * (nx + 1) + fi, fxs);
+            fvh_recon_face<false, MIXED, VARIANT>(a, fi - 1, j, a.fx, (long long)j * (nx + 1) + fi, fxs);
     for (int fj = 0; fj <= ny; ++fj)
-        for (int i = 0; i < nx; ++i) fvh_recon_face<true, MIXED>(a, i, fj - 1, a.fy, (long long)fj * nx + i, fys);
+        for (int i = 0; i < nx; ++i)
+            fvh_recon_face<true, MIXED, VARIANT>(a, i, fj - 1, a.fy, (long long)fj * nx + i, fys);
     double o[3][12];
     for (int j = 0; j < ny; ++j)
         for (int fi = 0; fi <= nx; ++fi) {
@@ -78,6 +80,13 @@ static int rhs_impl(int nx, int ny, double xl, double xr, double yb, double yt, 
 extern "C" int fvhh_rhs(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc, int flux,
                         int quirks, const double* q, double* rhs, double* qlim) {
     return rhs_impl<false>(nx, ny, xl, xr, yb, yt, gamma, bc, flux, quirks, q, rhs, qlim);
+}
+
+// hweno (1), htenoi (2) or linear (3) in hteno's place (the kernels' run-time
+// variant instantiation)
+extern "C" int fvhh_rhs_variant(int hrecon, int nx, int ny, double xl, double xr, double yb, double yt, double gamma,
+                                int bc, int flux, int quirks, const double* q, double* rhs, double* qlim) {
+    return rhs_impl<false, true>(nx, ny, xl, xr, yb, yt, gamma, bc, flux, quirks, q, rhs, qlim, hrecon);
 }
 
 // the FP32-mixed precision's bodies (HTENO / TENO5 cut-offs and transverse

@@ -41,7 +41,7 @@ def test_abi_version_and_params_struct():
 
 
 @pytest.mark.parametrize("field,value", [("nx", 0), ("ny", -3), ("gamma", 1.0), ("cfl", 0.0),
-                                         ("flux", 7), ("bc", 9), ("scheme", 5), ("struct_size", 4)])
+                                         ("flux", 7), ("bc", 9), ("scheme", 17), ("scheme", -1), ("struct_size", 4)])
 def test_invalid_params_rejected_without_exit(field, value):
     """Where the reference calls exit(1) (Schemes_Hermite.cpp:16-25 on a zero
     grid size) or prints and continues (bad dir codes), the ABI returns

@@ -1,7 +1,8 @@
 """The 1D path's bodies (htx_fv1d.h) built for the CPU with the strict
 evaluation (tests/host/fv1d_host.hip): residual and SSP-RK3 stages bit for
 bit against the FP64 oracle's FV::System1D restatement, for the shipped
-TENO5i + characteristic Rusanov and the 1D configuration's TENO5 + HLLC."""
+TENO5i + characteristic Rusanov, the 1D configuration's TENO5 + HLLC and
+every other 5-point reconstruction of Schemes::Lagrange (diff = dx)."""
 import ctypes
 import os
 
@@ -9,6 +10,7 @@ import numpy as np
 import pytest
 
 from conftest import ROOT
+from oracle import RECON
 
 LIB = os.path.join(ROOT, "tests", "host", "libfv1d_host.so")
 DP = ctypes.POINTER(ctypes.c_double)
@@ -31,7 +33,7 @@ def _stage(lib, q, recon, flux, mode=0, dt=0.0, q0=None, quirks=1):
     q = np.ascontiguousarray(q)
     q0 = np.ascontiguousarray(q if q0 is None else q0)
     out = np.empty_like(q)
-    lib.fv1dh_stage(nx, 0.0, 1.0, GAMMA, {"teno5": 0, "teno5i": 1, "weno5js": 2, "weno5z": 3}[recon], {"rusanov": 0, "hllc": 1}[flux], quirks,
+    lib.fv1dh_stage(nx, 0.0, 1.0, GAMMA, RECON[recon], {"rusanov": 0, "hllc": 1}[flux], quirks,
                     mode, dt, q.ctypes.data_as(DP), q0.ctypes.data_as(DP), out.ctypes.data_as(DP))
     return out
 
@@ -51,7 +53,10 @@ def _states():
 
 
 @pytest.mark.parametrize("recon,flux", [("teno5i", "rusanov"), ("teno5", "hllc"), ("teno5", "rusanov"),
-                                        ("teno5i", "hllc"), ("weno5js", "hllc"), ("weno5z", "rusanov")])
+                                        ("teno5i", "hllc"), ("weno5js", "hllc"), ("weno5z", "rusanov"),
+                                        ("weno5n", "hllc"), ("teno5n", "rusanov"), ("teno5a", "hllc"),
+                                        ("weno5is", "rusanov"), ("weno5is_ext", "hllc"), ("weno5ismod", "hllc"),
+                                        ("weno5z_diff", "rusanov"), ("teno5_diff", "hllc"), ("linear", "rusanov")])
 def test_residual_bitwise(orc, harness, recon, flux):
     for name, q in _states().items():
         r = _stage(harness, q, recon, flux)

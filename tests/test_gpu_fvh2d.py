@@ -162,3 +162,33 @@ def test_hermite_fp32_mixed_accuracy(case):
         print(f"{name}: L1 error vs 4x finer FP64: fp64 {e64:.4e}, fp32_mixed {e32:.4e}, ratio {e32 / e64:.4f}; "
               f"fp32_mixed - fp64 {rel_l1(q32[sl], q64[sl]):.3e}", flush=True)
         assert e32 <= 1.10 * e64
+
+
+@pytest.mark.parametrize("recon", ["hweno", "htenoi", "hlinear"])
+@pytest.mark.parametrize("precision", ["fp64", "fp64_fast"])
+def test_hermite_variants_parity(orc, recon, precision):
+    """hweno, htenoi or the linear Hermite scheme in hteno's place
+    (FVHermite_System2D.cpp:459, :486): residual within 1e-12 and 3 SSP-RK3
+    steps within 1e-9 of the oracle's (the parity tolerances of the Hermite
+    path), both builds."""
+    import htx
+    from htx.ic import vortex_modes
+
+    n = 32
+    print(f"[case] {recon} n={n} {precision}", flush=True)
+    s = htx.System2D(0.0, 10.0, n, 0.0, 10.0, n, 0.4, 10.0, 0.0, GAMMA, scheme=recon, flux="rusanov", bc="periodic",
+                     precision=precision)
+    prm = params(n, n, 0.0, 10.0, 0.0, 10.0, GAMMA, "periodic", "rusanov", True, recon=recon)
+    q = orc.ic_vortex_hermite(prm, (5.0, 5.0), vortex_modes())
+    s.set_state(q)
+    r, rc = s.rhs(), orc.fvh2d_rhs(prm, q)
+    print(f"residual rel L1 {rel_l1(r, rc):.3e}, bit-identical {np.array_equal(r, rc)}", flush=True)
+    assert rel_l1(r, rc) < 1e-12
+    for _ in range(3):
+        s.step()
+    qc, tc, k = orc.fvh2d_advance(prm, q, 0.4, 0.0, 10.0, 3)
+    g = s.get_state()
+    print(f"3 steps rel L1 {rel_l1(g, qc):.3e}, bit-identical {np.array_equal(g, qc)}", flush=True)
+    assert k == 3 and s.time == pytest.approx(tc, rel=1e-13)
+    assert rel_l1(g, qc) < 1e-9
+    s.close()

@@ -136,6 +136,36 @@ def test_hermite_kernels_cpu_build(orc, harness_hermite, nx, ny, bc, flux):
     np.testing.assert_array_equal(r, rc)
 
 
+@pytest.mark.parametrize("recon,code", [("hweno", 1), ("htenoi", 2), ("hlinear", 3)])
+@pytest.mark.parametrize("bc,flux", [("periodic", "rusanov"), ("zero_grad", "hllc")])
+def test_hermite_variants_cpu_build(orc, harness_hermite, recon, code, bc, flux):
+    """The Hermite solver with hweno, htenoi or the linear Hermite scheme in
+    hteno's place: the strict CPU build of the kernels' bodies equals the
+    oracle's residual and limited moments bit for bit, on the vortex with
+    density modes and on quadrant data."""
+    from htx.ic import vortex_modes
+
+    lib = harness_hermite
+    lib.fvhh_rhs_variant.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, D, D, D, D, D, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, DP, DP, DP]
+    nx, ny = (24, 20) if bc == "periodic" else (19, 16)
+    dom = (0.0, 10.0, 0.0, 10.0)
+    prm = params(nx, ny, *dom, 1.4, bc, flux, True, recon=recon)
+    q = orc.ic_vortex_hermite(prm, (5.0, 5.0), vortex_modes())
+    if bc == "zero_grad":
+        q[:4] *= np.where(np.arange(nx) < nx // 2, 1.0, 0.6)[None, None, :]
+    q = np.ascontiguousarray(q)
+    r, ql = np.empty_like(q), np.empty_like(q)
+    lib.fvhh_rhs_variant(code, nx, ny, *dom, 1.4, {"zero_grad": 0, "periodic": 1}[bc],
+                         {"rusanov": 0, "hllc": 1}[flux], 1, q.ctypes.data_as(DP), r.ctypes.data_as(DP),
+                         ql.ctypes.data_as(DP))
+    rc, qlc = orc.fvh2d_rhs_lim(prm, q)
+    np.testing.assert_array_equal(ql, qlc)
+    np.testing.assert_array_equal(r, rc)
+    # and it is not the hteno residual
+    assert not np.array_equal(r, orc.fvh2d_rhs(params(nx, ny, *dom, 1.4, bc, flux, True), q))
+
+
 def test_hermite_limiter_split_is_bit_identical(harness_hermite):
     """The limiter split at its chain dependence (the chain-independent
     subexpressions computed ahead, the rest per step) performs the same
@@ -486,6 +516,32 @@ def test_tiling_bitwise_reconstruction_variants(orc, harness, recon, mode, flux)
         np.testing.assert_array_equal(got, orc.fv2d_rhs(prm, q))
         if bc == "zero_grad":
             assert not np.array_equal(got, orc.fv2d_rhs(params(nx, ny, *dom, 1.4, bc, flux, True), q))
+
+
+VARIANTS = ["weno5n", "teno5n", "teno5a", "weno5is", "weno5is_ext", "weno5ismod", "weno5z_diff", "teno5_diff",
+            "linear"]
+
+
+@pytest.mark.parametrize("recon", VARIANTS)
+def test_tiling_bitwise_remaining_reconstructions(orc, harness, recon):
+    """The 2D path with each remaining 5-point routine of Schemes::Lagrange in
+    TENO5's place (the run-time variant instantiation of the stage kernel, on
+    its 16 x 15 tiles): the strict CPU build of the tile phases equals the
+    oracle's residual bit for bit, periodic vortex and noisy quadrants."""
+    from htx.ic import vortex_modes
+    from oracle import RECON
+
+    rng = np.random.default_rng(5)
+    for nx, ny, bc, dom, flux in [(48, 40, "periodic", (0.0, 10.0, 0.0, 10.0), "hllc"),
+                                  (40, 36, "zero_grad", (0.0, 1.0, 0.0, 1.0), "rusanov")]:
+        prm = params(nx, ny, *dom, 1.4, bc, flux, True, recon=recon)
+        if bc == "periodic":
+            q = orc.ic_vortex(prm, (5.0, 5.0), vortex_modes())
+        else:
+            q = quadrant(nx, ny, 3)
+            q[0] *= 1.0 + 0.05 * rng.random((ny, nx))
+        got = _rhs_prec(harness, nx, ny, dom, bc, flux, 100 + RECON[recon], q)
+        np.testing.assert_array_equal(got, orc.fv2d_rhs(prm, q))
 
 
 @pytest.mark.parametrize("mode,shape", [(8, "16x15"), (9, "15x16")])
