@@ -134,7 +134,9 @@ int check_params(const htx_params* p) {
     }
     if (!(p->gamma > 1.0)) return fail(HTX_EINVAL, "gamma must exceed 1");
     if (!(p->cfl > 0.0)) return fail(HTX_EINVAL, "cfl must be positive");
-    if (p->flux != HTX_FLUX_RUSANOV_CHAR && p->flux != HTX_FLUX_HLLC) return fail(HTX_EINVAL, "unknown flux");
+    if (p->flux != HTX_FLUX_RUSANOV_CHAR && p->flux != HTX_FLUX_HLLC && p->flux != HTX_FLUX_CU)
+        return fail(HTX_EINVAL, "unknown flux");
+    if (p->flux == HTX_FLUX_CU && p->dim != 1) return fail(HTX_ENOTSUP, "the central-upwind flux is built for 1D");
     if (p->bc != HTX_BC_ZERO_GRAD && p->bc != HTX_BC_PERIODIC) return fail(HTX_EINVAL, "unknown bc");
     if (p->precision < HTX_PREC_FP64 || p->precision > HTX_PREC_LEARNED_FP32)
         return fail(HTX_EINVAL, "unknown precision");
@@ -175,6 +177,7 @@ Fv2dArgs stage_args(htx_solver* h, const double* qin, const double* q0, double* 
 
 int flux_kind(const htx_solver* h) {
     if (h->p.flux == HTX_FLUX_RUSANOV_CHAR) return FLUX_RUSANOV_CHAR;
+    if (h->p.flux == HTX_FLUX_CU) return FLUX_CU;
     return h->p.ref_quirks ? FLUX_HLLC : FLUX_HLLC_TORO;
 }
 

@@ -497,9 +497,10 @@ Vector3d Euler1D::Rusanov(const Vector3d& ql, const Vector3d& qr, const Vector3d
     return 0.5 * (flux(ql, wl) + flux(qr, wr) - R * a.cwiseProduct(L * (qr - ql)));
 }
 
-// Kurganov's central-upwind flux (:137-154): one-sided local speeds a+ >= 0,
-// a- <= 0 from the componentwise max / min of the two states, the HLL-type
-// average of the fluxes and the characteristic anti-diffusion term
+// Kurganov's central-upwind flux (:137-154): the 1D kernels' e1_cuflux
+// (one-sided local speeds a+ >= 0, a- <= 0 from the componentwise max / min
+// of the two states, the HLL-type average of the fluxes and the
+// characteristic anti-diffusion term)
 static void cu_speeds(const Euler1D& e, const Vector3d& ql, const Vector3d& qr, Vector3d& ap, Vector3d& am) {
     const Vector3d smax = e.localWaveSpeed(e.primitive(ql.cwiseMax(qr)));
     const Vector3d smin = e.localWaveSpeed(e.primitive(ql.cwiseMin(qr)));
@@ -508,11 +509,14 @@ static void cu_speeds(const Euler1D& e, const Vector3d& ql, const Vector3d& qr, 
 }
 
 Vector3d Euler1D::CUFlux(const Vector3d& ql, const Vector3d& qr, const Matrix3d& L, const Matrix3d& R) const {
-    Vector3d ap, am;
-    cu_speeds(*this, ql, qr, ap, am);
-    const Vector3d inv = (ap - am).cwiseInverse();
-    Vector3d f = (ap.cwiseProduct(flux(ql, primitive(ql))) - am.cwiseProduct(flux(qr, primitive(qr)))).cwiseProduct(inv);
-    f += R * (ap.cwiseProduct(am)).cwiseProduct(inv).cwiseProduct(L * (qr - ql));
+    double Lr[9], Rr[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            Lr[3 * r + c] = L(r, c);
+            Rr[3 * r + c] = R(r, c);
+        }
+    Vector3d f;
+    htx_host::e1_cuflux(gamma, gamma_1, Lr, Rr, ql.data(), qr.data(), f.data());
     return f;
 }
 

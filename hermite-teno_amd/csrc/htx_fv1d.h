@@ -15,7 +15,7 @@ struct Fv1dArgs {
     int nx, bc, mode;   // bc: 0 zero-gradient, 1 periodic
     int recon;          // 0 teno5, 1 teno5i (the reference's shipped 1D scheme), 2 weno5JS, 3 weno5Z,
                         // RECON_WENO5N .. RECON_LAST the variants of recon5_variant (diff = dx)
-    int flux;           // FLUX_RUSANOV_CHAR, FLUX_HLLC (quirks), FLUX_HLLC_TORO
+    int flux;           // FLUX_RUSANOV_CHAR, FLUX_HLLC (quirks), FLUX_HLLC_TORO, FLUX_CU
     double gam, g1, dx;
 };
 
@@ -130,6 +130,44 @@ HTX_HD void e1_rusanov_char(double gam, double g1, const double L[9], const doub
     for (int e = 0; e < 3; ++e) f[e] = 0.5 * ((fl[e] + fr[e]) - rd[e]);
 }
 
+// Euler1D::CUFlux (Equations_Euler1D.cpp:137-154), the central-upwind flux
+// the reference's 1D solver offers in a comment (FV_System1D.cpp:234): the
+// one-sided speeds a+ = max(s_max, s_min, 0), a- = min(s_max, s_min, 0) of the
+// componentwise larger and smaller state, s = localWaveSpeed (:108-119:
+// |u - c|, |u|, |u + c|), then
+//   F = (a+ f(q_L) - a- f(q_R)) / (a+ - a-) + R (a+ a- / (a+ - a-) L (q_R - q_L)),
+// each division a multiplication by the inverse as the reference writes it.
+// With the absolute speeds a- is zero, so F = (a+ f(q_L)) (1 / a+).
+HTX_HD void e1_cuflux(double gam, double g1, const double L[9], const double R[9], const double ql[3],
+                      const double qr[3], double f[3]) {
+    HTX_NO_CONTRACT
+    double qmax[3], qmin[3], wmax[3], wmin[3], wl[3], wr[3], fl[3], fr[3], ap[3], am[3], inv[3], dd[3], ld[3], rd[3];
+    for (int e = 0; e < 3; ++e) {
+        qmax[e] = ql[e] < qr[e] ? qr[e] : ql[e];  // cwiseMax / cwiseMin
+        qmin[e] = qr[e] < ql[e] ? qr[e] : ql[e];
+    }
+    e1_prim(g1, qmax, wmax);
+    e1_prim(g1, qmin, wmin);
+    const double cx = sqrt(fabs(gam * wmax[2] / wmax[0])), cn = sqrt(fabs(gam * wmin[2] / wmin[0]));
+    const double sx[3] = {fabs(wmax[1] - cx), fabs(wmax[1]), fabs(wmax[1] + cx)};
+    const double sn[3] = {fabs(wmin[1] - cn), fabs(wmin[1]), fabs(wmin[1] + cn)};
+    for (int e = 0; e < 3; ++e) {
+        const double hi = sx[e] < sn[e] ? sn[e] : sx[e], lo = sn[e] < sx[e] ? sn[e] : sx[e];
+        ap[e] = hi < 0.0 ? 0.0 : hi;
+        am[e] = 0.0 < lo ? 0.0 : lo;
+        inv[e] = 1.0 / (ap[e] - am[e]);
+    }
+    e1_prim(g1, ql, wl);
+    e1_prim(g1, qr, wr);
+    e1_phys_flux(ql, wl, fl);
+    e1_phys_flux(qr, wr, fr);
+    for (int e = 0; e < 3; ++e) dd[e] = qr[e] - ql[e];
+    e1_mv(L, dd, ld);
+    for (int e = 0; e < 3; ++e) ld[e] = ((ap[e] * am[e]) * inv[e]) * ld[e];
+    e1_mv(R, ld, rd);
+    for (int e = 0; e < 3; ++e) f[e] = (ap[e] * fl[e] - am[e] * fr[e]) * inv[e] + rd[e];
+}
+
 // Flux through the face between cells c and c + 1 (c = -1 .. nx - 1):
 // characteristic reconstruction (FV_System1D.cpp:194-208), back to
 // conservative variables (FV_System1D.h:48-51), and the flux (:210-223).
@@ -168,6 +206,8 @@ HTX_HD void fv1d_face_flux(const Fv1dArgs& a, int c, double F[3]) {
         e1_hllc<true>(a.gam, a.g1, ql, qr, F);
     else if (a.flux == FLUX_HLLC_TORO)
         e1_hllc<false>(a.gam, a.g1, ql, qr, F);
+    else if (a.flux == FLUX_CU)
+        e1_cuflux(a.gam, a.g1, L, R, ql, qr, F);
     else
         e1_rusanov_char(a.gam, a.g1, L, R, ql, qr, F);
 }

@@ -27,7 +27,7 @@ struct Ctl {
     unsigned long long n_faces, n_flagged, n_linear;
 };
 
-enum FluxKind { FLUX_RUSANOV_CHAR = 0, FLUX_HLLC = 1, FLUX_HLLC_TORO = 2 };
+enum FluxKind { FLUX_RUSANOV_CHAR = 0, FLUX_HLLC = 1, FLUX_HLLC_TORO = 2, FLUX_CU = 3 };  // FLUX_CU: 1D only
 
 enum CtlStatus : unsigned { CTL_NONFINITE_STATE = 1u, CTL_BAD_DT = 2u };
 

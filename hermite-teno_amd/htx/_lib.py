@@ -20,7 +20,7 @@ SCHEMES = {"teno5": 0, "hteno": 1, "teno5i": 2, "weno5js": 3, "weno5z": 4, "weno
            "hweno": 14, "htenoi": 15, "hlinear": 16}
 # the schemes of the Hermite solver (12 fields: u and the x, y derivative moments)
 HERMITE_SCHEMES = ("hteno", "hweno", "htenoi", "hlinear")
-FLUXES = {"rusanov": 0, "rusanov_char": 0, "hllc": 1}
+FLUXES = {"rusanov": 0, "rusanov_char": 0, "hllc": 1, "cu": 2}
 BCS = {"zero_grad": 0, "zerograd": 0, "periodic": 1}
 PRECISIONS = {"fp64": 0, "fp32_mixed": 1, "learned": 2, "fp64_fast": 3, "fp32": 4, "learned_fp32": 5}
 

@@ -80,7 +80,9 @@ enum htx_scheme {
     HTX_SCHEME_HTENOI = 15,
     HTX_SCHEME_HLINEAR = 16
 };
-enum htx_flux { HTX_FLUX_RUSANOV_CHAR = 0, HTX_FLUX_HLLC = 1 };
+/* HTX_FLUX_CU: Euler1D::CUFlux (Equations_Euler1D.cpp:137-154), the 1D
+ * solver's commented alternative (FV_System1D.cpp:234); dim = 1 only. */
+enum htx_flux { HTX_FLUX_RUSANOV_CHAR = 0, HTX_FLUX_HLLC = 1, HTX_FLUX_CU = 2 };
 enum htx_bc { HTX_BC_ZERO_GRAD = 0, HTX_BC_PERIODIC = 1 };
 /* HTX_PREC_FP64: the drop-in; every operation is one IEEE operation in the
  *   reference's order (no FMA contraction, divisions kept), so each step is

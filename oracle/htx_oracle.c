@@ -2470,6 +2470,52 @@ void orc_e1_hllc(double gamma, int quirks, const double ql[3], const double qr[3
     e1_hllc(gamma, quirks, ql, qr, f);
 }
 
+/* Euler1D::localWaveSpeed, Equations_Euler1D.cpp:108-119 */
+static void e1_local_speed(double gamma, const double w[3], double s[3]) {
+    double c = sqrt(fabs(gamma * w[2] / w[0]));
+    s[0] = fabs(w[1] - c);
+    s[1] = fabs(w[1]);
+    s[2] = fabs(w[1] + c);
+}
+
+static inline double cmax(double a, double b) { return (a < b) ? b : a; } /* cwiseMax */
+static inline double cmin(double a, double b) { return (b < a) ? b : a; } /* cwiseMin */
+
+/* Euler1D::CUFlux, Equations_Euler1D.cpp:137-154 (L, R row-major) */
+static void e1_cuflux(double gamma, const double L[9], const double R[9], const double ql[3], const double qr[3],
+                      double f[3]) {
+    double g1 = gamma - 1.0, qmax[3], qmin[3], w[3], spdmax[3], spdmin[3], aplus[3], aminus[3];
+    for (int e = 0; e < 3; ++e) {
+        qmax[e] = cmax(ql[e], qr[e]);
+        qmin[e] = cmin(ql[e], qr[e]);
+    }
+    e1_primitive(g1, qmax, w);
+    e1_local_speed(gamma, w, spdmax);
+    e1_primitive(g1, qmin, w);
+    e1_local_speed(gamma, w, spdmin);
+    for (int e = 0; e < 3; ++e) {
+        aplus[e] = cmax(cmax(spdmax[e], spdmin[e]), 0.0);
+        aminus[e] = cmin(cmin(spdmax[e], spdmin[e]), 0.0);
+    }
+    double wl[3], wr[3], fl[3], fr[3], inv[3], dq[3], Ldq[3], t[3], Rt[3];
+    e1_primitive(g1, ql, wl);
+    e1_primitive(g1, qr, wr);
+    e1_flux(ql, wl, fl);
+    e1_flux(qr, wr, fr);
+    for (int e = 0; e < 3; ++e) inv[e] = 1.0 / (aplus[e] - aminus[e]);
+    for (int e = 0; e < 3; ++e) f[e] = (aplus[e] * fl[e] - aminus[e] * fr[e]) * inv[e];
+    for (int e = 0; e < 3; ++e) dq[e] = qr[e] - ql[e];
+    matvec3(L, dq, Ldq);
+    for (int e = 0; e < 3; ++e) t[e] = ((aplus[e] * aminus[e]) * inv[e]) * Ldq[e];
+    matvec3(R, t, Rt);
+    for (int e = 0; e < 3; ++e) f[e] += Rt[e];
+}
+
+void orc_e1_cuflux(double gamma, const double L[9], const double R[9], const double ql[3], const double qr[3],
+                   double f[3]) {
+    e1_cuflux(gamma, L, R, ql, qr, f);
+}
+
 /* Lagrange::teno5i, Schemes_Lagrange.cpp:1235-1263: tau = max(beta) */
 static double teno5i_side(const double* v, const double p[3], const double d[3]) {
     double SI[3], w[3], sigm[3];
@@ -2560,6 +2606,8 @@ void orc_fv1d_rhs(int nx, double xl, double xr, double gamma, int recon, int flu
         matvec3(R, ur, qr);
         if (flux == ORC_FLUX_HLLC) {
             e1_hllc(gamma, quirks, ql, qr, F);
+        } else if (flux == ORC_FLUX_CU) { /* the commented alternative, FV_System1D.cpp:234 */
+            e1_cuflux(gamma, L, R, ql, qr, F);
         } else {
             double wl[3], wr[3], fl[3], fr[3], a[3], dd[3], ld[3], rd[3];
             e1_primitive(g1, ql, wl);

@@ -19,7 +19,7 @@ extern "C" {
 #endif
 
 enum { ORC_BC_ZERO_GRAD = 0, ORC_BC_PERIODIC = 1 };
-enum { ORC_FLUX_RUSANOV_CHAR = 0, ORC_FLUX_HLLC = 1 };
+enum { ORC_FLUX_RUSANOV_CHAR = 0, ORC_FLUX_HLLC = 1, ORC_FLUX_CU = 2 /* 1D only */ };
 
 typedef struct {
     int nx, ny;
@@ -93,6 +93,8 @@ void orc_weno5(const double u[6], int z, double out[2]); /* z: 0 weno5JS, 1 weno
 /* the remaining Lagrange and Hermite routines (Schemes_Lagrange.cpp:315-1004,
  * :1265-1294; Schemes_Hermite.cpp:234-293, :381-513, :670-767) */
 void orc_recon(int recon, const double u[6], double diff, double out[2]);
+void orc_e1_cuflux(double gamma, const double L[9], const double R[9], const double ql[3], const double qr[3],
+                   double f[3]);
 void orc_weno3js(const double u[4], double out[2]);
 void orc_weno3js_2q(const double u[3], double out[2]);
 void orc_quad2(int kind, const double u[5], double out[2]);          /* 0 JS, 1 Z, 2 TENO */

@@ -71,6 +71,7 @@ _SIG = {
     "orc_teno5_indicators": (None, [_DP, _DP, _DP]),
     "orc_hteno_indicators": (None, [_DP, _DP, _DP, _DP, _DP]),
     "orc_recon": (None, [_I, _DP, _D, _DP]),
+    "orc_e1_cuflux": (None, [_D, _DP, _DP, _DP, _DP, _DP]),
     "orc_weno3js": (None, [_DP, _DP]),
     "orc_weno3js_2q": (None, [_DP, _DP]),
     "orc_quad2": (None, [_I, _DP, _DP]),
@@ -350,7 +351,7 @@ class Oracle:
     def fv1d_rhs(self, nx, xl, xr, gamma, q, recon="teno5", flux="hllc", quirks=True):
         q, pq = _p(q)
         r = np.empty_like(q)
-        self.lib.orc_fv1d_rhs(nx, xl, xr, gamma, RECON[recon], {"rusanov": 0, "hllc": 1}[flux],
+        self.lib.orc_fv1d_rhs(nx, xl, xr, gamma, RECON[recon], {"rusanov": 0, "hllc": 1, "cu": 2}[flux],
                               1 if quirks else 0, pq, r.ctypes.data_as(_DP))
         return r
 
@@ -359,7 +360,7 @@ class Oracle:
         q = np.array(q, dtype=np.float64, copy=True, order="C")
         t = _D()
         n = self.lib.orc_fv1d_advance(nx, xl, xr, gamma, RECON[recon],
-                                      {"rusanov": 0, "hllc": 1}[flux], 1 if quirks else 0, q.ctypes.data_as(_DP),
+                                      {"rusanov": 0, "hllc": 1, "cu": 2}[flux], 1 if quirks else 0, q.ctypes.data_as(_DP),
                                       cfl, t0, tfinal, max_steps, ctypes.byref(t))
         return q, t.value, n
 
@@ -455,3 +456,13 @@ class Oracle:
         out = np.empty(4 if npts == 2 else 8)
         self.lib.orc_hweno_glq(npts, dif, pu, pd, out.ctypes.data_as(_DP))
         return out
+
+    def e1_cuflux(self, gamma, L, R, ql, qr):
+        """Euler1D::CUFlux with row-major 3x3 L, R."""
+        L, pL = _p(L)
+        R, pR = _p(R)
+        ql, pl = _p(ql)
+        qr, pr = _p(qr)
+        f = np.empty(3)
+        self.lib.orc_e1_cuflux(gamma, pL, pR, pl, pr, f.ctypes.data_as(_DP))
+        return f

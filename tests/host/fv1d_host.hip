@@ -7,7 +7,8 @@
 
 using namespace htx;
 
-// recon: 0 teno5, 1 teno5i, 2 weno5JS, 3 weno5Z; flux: 0 Rusanov, 1 HLLC; mode 0 residual,
+// recon: the kernels' codes (0 teno5, 1 teno5i, 2 weno5JS, 3 weno5Z, RECON_WENO5N ..); flux: 0 Rusanov,
+// 1 HLLC, 2 central-upwind; mode 0 residual,
 // 1..3 stage with dt (q0 = u^n)
 extern "C" void fv1dh_stage(int nx, double xl, double xr, double gamma, int recon, int flux, int quirks, int mode,
                             double dt, const double* q, const double* q0, double* out) {
@@ -19,7 +20,7 @@ extern "C" void fv1dh_stage(int nx, double xl, double xr, double gamma, int reco
     a.bc = 0;
     a.mode = mode;
     a.recon = recon;
-    a.flux = flux == 0 ? FLUX_RUSANOV_CHAR : (quirks ? FLUX_HLLC : FLUX_HLLC_TORO);
+    a.flux = flux == 0 ? FLUX_RUSANOV_CHAR : flux == 2 ? FLUX_CU : (quirks ? FLUX_HLLC : FLUX_HLLC_TORO);
     a.gam = gamma;
     a.g1 = gamma - 1.0;
     a.dx = (xr - xl) / nx;

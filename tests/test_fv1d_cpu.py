@@ -33,7 +33,7 @@ def _stage(lib, q, recon, flux, mode=0, dt=0.0, q0=None, quirks=1):
     q = np.ascontiguousarray(q)
     q0 = np.ascontiguousarray(q if q0 is None else q0)
     out = np.empty_like(q)
-    lib.fv1dh_stage(nx, 0.0, 1.0, GAMMA, RECON[recon], {"rusanov": 0, "hllc": 1}[flux], quirks,
+    lib.fv1dh_stage(nx, 0.0, 1.0, GAMMA, RECON[recon], {"rusanov": 0, "hllc": 1, "cu": 2}[flux], quirks,
                     mode, dt, q.ctypes.data_as(DP), q0.ctypes.data_as(DP), out.ctypes.data_as(DP))
     return out
 
@@ -56,7 +56,8 @@ def _states():
                                         ("teno5i", "hllc"), ("weno5js", "hllc"), ("weno5z", "rusanov"),
                                         ("weno5n", "hllc"), ("teno5n", "rusanov"), ("teno5a", "hllc"),
                                         ("weno5is", "rusanov"), ("weno5is_ext", "hllc"), ("weno5ismod", "hllc"),
-                                        ("weno5z_diff", "rusanov"), ("teno5_diff", "hllc"), ("linear", "rusanov")])
+                                        ("weno5z_diff", "rusanov"), ("teno5_diff", "hllc"), ("linear", "rusanov"),
+                                        ("teno5", "cu"), ("teno5i", "cu")])
 def test_residual_bitwise(orc, harness, recon, flux):
     for name, q in _states().items():
         r = _stage(harness, q, recon, flux)

@@ -29,7 +29,7 @@ def _smooth_state(nx, seed=5):
 
 
 @pytest.mark.parametrize("recon", ["teno5i", "teno5", "weno5js", "weno5z"])
-@pytest.mark.parametrize("flux", ["rusanov", "hllc"])
+@pytest.mark.parametrize("flux", ["rusanov", "hllc", "cu"])
 @pytest.mark.parametrize("case", ["sod", "smooth", "ragged"])
 def test_rhs_parity(orc, recon, flux, case):
     import htx
@@ -74,7 +74,7 @@ def test_rhs_parity_remaining_reconstructions(orc, recon, case):
 
 @pytest.mark.parametrize("recon,flux", [("teno5i", "rusanov"), ("teno5", "hllc"), ("weno5js", "hllc"),
                                         ("weno5z", "rusanov"), ("teno5n", "hllc"), ("weno5is_ext", "rusanov"),
-                                        ("teno5a", "rusanov")])
+                                        ("teno5a", "rusanov"), ("teno5", "cu"), ("teno5i", "cu")])
 def test_steps_parity(orc, recon, flux):
     """Ten SSP-RK3 steps of the Sod problem: dt and state of every step
     equal the oracle's."""

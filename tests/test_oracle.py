@@ -389,3 +389,24 @@ def test_rhs_converges_for_smooth_vortex(orc):
         dq = (q2 - q) / eps
         errs.append(np.abs(r - dq).mean())
     assert errs[0] / errs[1] > 8.0
+
+
+def test_cuflux_reduces_to_the_left_flux(orc):
+    """Euler1D::CUFlux (Equations_Euler1D.cpp:137-154) takes its one-sided
+    speeds from localWaveSpeed, which returns absolute values (:108-119), so
+    a- = min(s_max, s_min, 0) = 0 and the flux is (a+ f(q_L)) (1 / a+): the
+    left state's physical flux to rounding, for any right state, and the
+    anti-diffusion term vanishes.  Pinned on random admissible states and
+    Roe eigenvectors of them."""
+    rng = np.random.default_rng(17)
+    g = 1.4
+    for _ in range(200):
+        wl = [rng.uniform(0.1, 3), rng.uniform(-2, 2), rng.uniform(0.1, 3)]
+        wr = [rng.uniform(0.1, 3), rng.uniform(-2, 2), rng.uniform(0.1, 3)]
+        ql = np.array([wl[0], wl[0] * wl[1], wl[2] / (g - 1) + 0.5 * wl[0] * wl[1] ** 2])
+        qr = np.array([wr[0], wr[0] * wr[1], wr[2] / (g - 1) + 0.5 * wr[0] * wr[1] ** 2])
+        L = rng.normal(size=9)
+        R = rng.normal(size=9)
+        f = orc.e1_cuflux(g, L, R, ql, qr)
+        fl = np.array([ql[1], ql[1] * wl[1] + wl[2], wl[1] * (ql[2] + wl[2])])
+        np.testing.assert_allclose(f, fl, rtol=4e-16 * 4, atol=1e-15)
