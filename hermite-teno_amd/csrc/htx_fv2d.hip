@@ -196,8 +196,8 @@ static hipError_t launch_prec(const Fv2dArgs& a, int flux, hipStream_t s) {
 
 // Tile shapes: 16 x 15 (the default: its four face loops, 323 + 320
 // reconstructed and 255 + 256 flux faces, fill 19 wavefront passes with 94 %
-// of the lanes busy, against 22 passes and 87 % for 16 x 16 cells), 16 x 16
-// (the learned precisions' default, equally fast for them), and in the fast
+// of the lanes busy, against 22 passes and 87 % for 16 x 16 cells; the
+// learned precisions gain 8-15 % from it too), 16 x 16, and in the fast
 // build 16 x 8 and 32 x 8 (smaller LDS footprints, more workgroups per CU for
 // kernels with few registers); any other request takes the default.  Every
 // shape gives the same results bit for bit, except in the learned
@@ -205,8 +205,7 @@ static hipError_t launch_prec(const Fv2dArgs& a, int flux, hipStream_t s) {
 // and so depends on which faces share one.
 hipError_t launch_fv2d_stage(const Fv2dArgs& a, int flux, int tile_x, int tile_y, hipStream_t s) {
     if (a.recon != 0) return launch_prec<16, 15>(a, flux, s);
-    if ((tile_x == 16 && tile_y == 16) || (a.learned && tile_x == 0 && tile_y == 0))
-        return launch_prec<16, 16>(a, flux, s);
+    if (tile_x == 16 && tile_y == 16) return launch_prec<16, 16>(a, flux, s);
 #if !HTX_EXACT_DIVISION
     if (tile_x == 16 && tile_y == 8) return launch_prec<16, 8>(a, flux, s);
     if (tile_x == 32 && tile_y == 8 && !a.bulk32) return launch_prec<32, 8>(a, flux, s);

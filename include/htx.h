@@ -133,8 +133,8 @@ typedef struct htx_params {
     int bc;             /* htx_bc: FV_System2D.cpp:365-366 */
     int precision;      /* htx_precision */
     int ref_quirks;     /* 1: reproduce the reference's HLLC and maxSpeed quirks */
-    int tile_x, tile_y; /* TENO5 stage tile: 0 = the default (16 x 15; 16 x 16 for the learned
-                           precisions); 16 x 16 in every precision; the FP64_FAST, FP32_MIXED
+    int tile_x, tile_y; /* TENO5 stage tile: 0 = the default (16 x 15); 16 x 16 in every
+                           precision; the FP64_FAST, FP32_MIXED
                            and LEARNED precisions also build 16 x 8 and 32 x 8.  Every shape
                            gives the same results bit for bit, except that the learned
                            precisions decide their linear transverse quadrature per wavefront */

@@ -91,11 +91,11 @@ extern "C" int fv2dh_rhs_prec(int nx, int ny, double xl, double xr, double yb, d
     else if (mixed == 7)
         run<16, 16, false, false, double, 3>(a, fk);
     else if (mixed == 4)
-        run<16, 16, true, true, float>(a, fk);
+        run<16, 15, true, true, float>(a, fk);
     else if (mixed == 3)
         run<16, 16, true, false, float>(a, fk);
     else if (mixed == 2)
-        run<16, 16, true, true>(a, fk);
+        run<16, 15, true, true>(a, fk);
     else if (mixed)
         run<16, 16, true>(a, fk);
     else
@@ -146,7 +146,7 @@ extern "C" int fv2dh_rhs_flags(int nx, int ny, double xl, double xr, double yb, 
     a.dy = (yt - yb) / ny;
     a.area = a.dx * a.dy;
     fill_quad_consts(a.qc);
-    run<16, 16, true, true>(a, flux == 0 ? 0 : (quirks ? 1 : 2), fx, fy);
+    run<16, 15, true, true>(a, flux == 0 ? 0 : (quirks ? 1 : 2), fx, fy);
     return 0;
 }
 

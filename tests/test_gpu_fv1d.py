@@ -19,6 +19,18 @@ def _sod_state(nx):
     return np.stack([rho, np.zeros(nx), p / (GAMMA - 1.0)])
 
 
+def _supersonic_state(nx):
+    """Sod's densities and pressures moving right at u = 2.5, faster than
+    the sound speed on both sides: every wave speed is positive and no cell
+    is at rest (the reference's CUFlux divides 0 by 0 in the middle field
+    where u = 0, Equations_Euler1D.cpp:144-150)."""
+    x = (np.arange(nx) + 0.5) / nx
+    rho = np.where(x < 0.5, 1.0, 0.125)
+    p = np.where(x < 0.5, 1.0, 0.1)
+    u = 2.5
+    return np.stack([rho, rho * u, p / (GAMMA - 1.0) + 0.5 * rho * u * u])
+
+
 def _smooth_state(nx, seed=5):
     rng = np.random.default_rng(seed)
     x = (np.arange(nx) + 0.5) / nx
@@ -35,13 +47,40 @@ def test_rhs_parity(orc, recon, flux, case):
     import htx
 
     nx = {"sod": 400, "smooth": 256, "ragged": 77}[case]
-    q = _sod_state(nx) if case == "sod" else _smooth_state(nx)
+    if case == "sod":
+        q = _supersonic_state(nx) if flux == "cu" else _sod_state(nx)
+    else:
+        q = _smooth_state(nx)
     print(f"[case] 1d rhs {case} nx={nx} {recon} {flux}", flush=True)
     s = htx.System1D(0.0, 1.0, nx, 0.5, 0.2, 0.0, GAMMA, scheme=recon, flux=flux)
     s.set_state(q)
     r = s.rhs()
     rc = orc.fv1d_rhs(nx, 0.0, 1.0, GAMMA, q, recon=recon, flux=flux)
+    assert np.isfinite(r).all()
     np.testing.assert_array_equal(r, rc)
+    s.close()
+
+
+def test_cu_flux_at_rest_is_nonfinite_like_the_reference(orc):
+    """CUFlux where the gas is at rest (Sod's initial state): the middle
+    field's speeds |u| are 0 on both sides, a+ = a- = 0, and the reference's
+    (a+ f_L - a- f_R) / (a+ - a-) is 0 / 0 (Equations_Euler1D.cpp:144-150).
+    The GPU residual is NaN in exactly the oracle's cells, and a step
+    reports the non-finite state as HTX_ENONFINITE instead of running on."""
+    import htx
+    from htx._lib import HtxError
+
+    nx = 200
+    q = _sod_state(nx)
+    print("[case] 1d cu flux on a state at rest", flush=True)
+    s = htx.System1D(0.0, 1.0, nx, 0.5, 0.2, 0.0, GAMMA, scheme="teno5", flux="cu")
+    s.set_state(q)
+    r = s.rhs()
+    rc = orc.fv1d_rhs(nx, 0.0, 1.0, GAMMA, q, recon="teno5", flux="cu")
+    assert np.isnan(rc).any()
+    np.testing.assert_array_equal(np.isnan(r), np.isnan(rc))
+    with pytest.raises(HtxError, match="HTX_ENONFINITE"):
+        s.step()
     s.close()
 
 
@@ -76,12 +115,14 @@ def test_rhs_parity_remaining_reconstructions(orc, recon, case):
                                         ("weno5z", "rusanov"), ("teno5n", "hllc"), ("weno5is_ext", "rusanov"),
                                         ("teno5a", "rusanov"), ("teno5", "cu"), ("teno5i", "cu")])
 def test_steps_parity(orc, recon, flux):
-    """Ten SSP-RK3 steps of the Sod problem: dt and state of every step
-    equal the oracle's."""
+    """Ten SSP-RK3 steps of the Sod problem (moving right supersonically for
+    the central-upwind flux, which fails where the gas is at rest): dt and
+    state of every step equal the oracle's."""
     import htx
 
     nx = 400
-    q = _sod_state(nx)
+    q = _supersonic_state(nx) if flux == "cu" else _sod_state(nx)
+    print(f"[case] 1d steps nx={nx} {recon} {flux}", flush=True)
     s = htx.System1D(0.0, 1.0, nx, 0.5, 0.2, 0.0, GAMMA, scheme=recon, flux=flux)
     s.set_state(q)
     qc, t = q.copy(), 0.0
@@ -90,6 +131,7 @@ def test_steps_parity(orc, recon, flux):
         qn, tn, n = orc.fv1d_advance(nx, 0.0, 1.0, GAMMA, qc, 0.5, t, 0.2, recon=recon, flux=flux, max_steps=1)
         assert n == 1
         assert s.time == tn
+        assert np.isfinite(qn).all()
         np.testing.assert_array_equal(s.get_state(), qn)
         qc, t = qn, tn
     s.close()

@@ -442,7 +442,7 @@ def test_learned_flag_reaches_its_face(harness_fast, bc):
     i+1 of rows j-2 .. j+2, through the face's own flux and the transverse
     quadratures of its neighbours; a y face: the transpose), and most in the
     two cells of the face itself (the centre of its own transverse window),
-    wherever the face sits relative to the 16 x 16 tiles and the boundary."""
+    wherever the face sits relative to the 16 x 15 tiles and the boundary."""
     nx, ny, dom = 48, 40, (0.0, 1.0, 0.0, 1.0)
     rng = np.random.default_rng(21)
     w = np.array([1.0, 0.3, -0.2, 1.0])[:, None, None] * (1.0 + 0.03 * rng.standard_normal((4, ny, nx)))
