@@ -876,8 +876,11 @@ HTX_HD void fvh_recon_face(const FvhArgs& a, int i, int j, double* F, long long 
     // hteno on (L u, L d_normal) over cells -1..2, teno5 on L d_tangential over -2..3
     double A[4][4], B[4][4], C[4][6];
     const int nc = YDIR ? 4 : 0;  // normal-derivative block: v (x) or w (y) -> limited comp offset
-    // the loops are unrolled so that A, B, C stay in registers
-#pragma unroll
+    // FP32-mixed: the loops are unrolled so that A, B, C stay in registers
+    // (125 VGPRs); FP64: they stay loops over A, B, C in scratch, 164 VGPRs
+    // and three waves per SIMD instead of 262 and one (+13 % for fp64_fast;
+    // the strict build's compiler makes that choice by itself)
+#pragma unroll(MIXED ? 6 : 1)
     for (int m = 0; m < 6; ++m) {
         int k = m - 2;
         int ci = YDIR ? i : i + k, cj = YDIR ? j + k : j;
@@ -902,7 +905,7 @@ HTX_HD void fvh_recon_face(const FvhArgs& a, int i, int j, double* F, long long 
 #pragma unroll
         for (int e = 0; e < 4; ++e) C[e][m] = c[e];
     }
-#pragma unroll
+#pragma unroll(MIXED ? 4 : 1)
     for (int e = 0; e < 4; ++e) {
         double o[4], um, up;
         if (VARIANT)  // hweno, htenoi or linear in hteno's place (FVHermite_System2D.cpp:459, :486)
