@@ -172,10 +172,18 @@ struct PiecewiseArgs {
 // Boundary conditions are applied through the index: zero gradient clamps,
 // periodic wraps.  This is what zeroGradBc / periodicBc (FV_System2D.cpp
 // :114-152) leave in every ghost cell a stencil can reach.
+// Every stencil reaches less than one grid width past an edge, so one add
+// or subtract wraps it; the integer remainder (some thirty instructions on
+// the device) is left to indices further out (grids narrower than a
+// stencil).
 HTX_HD int map_index(int g, int n, int bc) {
     if (bc == 1) {
-        g = g % n;
-        return g < 0 ? g + n : g;
+        g = g < 0 ? g + n : (g >= n ? g - n : g);
+        if (g < 0 || g >= n) {
+            g = g % n;
+            g = g < 0 ? g + n : g;
+        }
+        return g;
     }
     return g < 0 ? 0 : (g >= n ? n - 1 : g);
 }
