@@ -299,6 +299,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--scheme", default="teno5", choices=("teno5", "hteno"),
+                    help="hteno times the Hermite-TENO solver on the same workload (no roofline or CPU leg)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device copy (for scaling byte counters)")
     ap.add_argument("--no-study", action="store_true",
@@ -307,7 +309,7 @@ def main():
 
     if args.gpus != 1:
         print(json.dumps({"error": "this path runs on one GPU; multi-GPU sharding is designed "
-                                   "(see DESIGN.md) but not built, so --gpus must be 1"}))
+                                   "but not built, so --gpus must be 1"}))
         sys.exit(2)
 
     import htx
@@ -317,9 +319,10 @@ def main():
     w = vortex_workload(n)
     if args.calibrate_copy:
         copy_calibration()
-    log(f"[gpu] creating {n}x{n} solver ({args.flux}, {args.precision})")
+    herm = args.scheme == "hteno"
+    log(f"[gpu] creating {n}x{n} {args.scheme} solver ({args.flux}, {args.precision})")
     s = htx.System2D(w["xl"], w["xr"], n, w["yb"], w["yt"], n, w["cfl"], w["tmax"], 0.0, w["gamma"],
-                     flux=args.flux, bc="periodic", precision=args.precision)
+                     flux=args.flux, bc="periodic", precision=args.precision, scheme=args.scheme)
     s.isentropicVortex(w["centre"], w["modes"])
     log(f"[gpu] warm-up {args.warmup} steps")
     s.steps_async(args.warmup)
@@ -361,10 +364,13 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded isentropic vortex + Fourier density modes, generated on device)",
         "config": {"workload": f"2D Euler periodic isentropic vortex + 8 seeded Fourier modes, {n}x{n} on "
-                               f"[0,10]^2, TENO5 char. + 3-pt transverse quadrature + {args.flux.upper()}, "
+                               f"[0,10]^2, "
+                               + ("Hermite-TENO (HTENO char. + derivative moments + limiter)" if herm else
+                                  "TENO5 char.")
+                               + f" + 3-pt transverse quadrature + {args.flux.upper()}, "
                                f"SSP-RK3 CFL 0.5, precision {args.precision}"
                                + (" (plain drop-in path)" if args.precision == "fp64" else ""),
-                   "nx": n, "ny": n, "scheme": "teno5", "flux": args.flux, "precision": args.precision,
+                   "nx": n, "ny": n, "scheme": args.scheme, "flux": args.flux, "precision": args.precision,
                    "parallelism": "single GPU"},
         "roofline": {
             # the stage kernel does about 68 FP64 flops per algorithmic HBM byte,
@@ -387,6 +393,8 @@ def main():
         "cpu_baseline": None,
         "variants": None,
     }
+    if herm:  # five launches per stage: no single kernel's roofline
+        out["roofline"] = None
     s.close()
     if not args.no_variants:
         out["variants"] = {}
@@ -420,7 +428,7 @@ def main():
             out["batch_fp32_mixed"] = batch_study(t_compare=0.0, teno_precision="fp32_mixed")
         except Exception as e:
             log(f"[gpu] batch study failed: {e}")
-    if not args.no_cpu:
+    if not args.no_cpu and not herm:
         try:
             out["cpu_baseline"], out["parity_vs_cpu"] = cpu_baseline(n, args.cpu_seconds)
         except Exception as e:  # the baseline is reported, never required for the GPU figure

@@ -23,14 +23,24 @@ def rel_l1(a, b):
     return float(np.abs(a - b).sum() / np.abs(b).sum())
 
 
-def run(w, nx, ny, precision, tend, flux="hllc"):
+def run(w, nx, ny, precision, tend, flux="hllc", segments=0):
     """End state, wall seconds, steps, and for precision 'learned' the
-    classifier statistics (faces, flagged, linear)."""
+    classifier statistics (faces, flagged, linear).  segments > 0 advances
+    in that many pieces of equal time, each ending exactly at its time (so a
+    few more steps), and prints the progress after each."""
     s = System2D(w["xl"], w["xr"], nx, w["yb"], w["yt"], ny, w["cfl"], tend, 0.0, w["gamma"], flux=flux,
                  bc=w["bc"], precision=precision)
     s.init_workload(w)
     t0 = time.perf_counter()
-    steps = s.timeStepping(tend)
+    if segments > 0:
+        steps = 0
+        for k in range(1, segments + 1):
+            steps += s.timeStepping(tend * k / segments)
+            s.sync()
+            print(f"{nx} x {ny} {precision}: t = {s.time:.6g} of {tend:.6g}, {steps} steps, "
+                  f"{time.perf_counter() - t0:.1f} s", flush=True)
+    else:
+        steps = s.timeStepping(tend)
     s.sync()
     el = time.perf_counter() - t0
     stats = s.classifier_stats() if precision.startswith("learned") else None
@@ -40,9 +50,12 @@ def run(w, nx, ny, precision, tend, flux="hllc"):
 
 
 def matched_accuracy(w, nx, ny, factor=4, precisions=("fp64", "fp32_mixed", "fp32", "learned", "learned_fp32"),
-                     tend=None, flux="hllc"):
+                     tend=None, flux="hllc", segments=0):
+    """Relative L1 error of each precision at nx x ny against the factor-times
+    finer fp64_fast solution averaged onto the grid (segments: progress
+    pieces of the finer run, see run)."""
     tend = w["tmax"] if tend is None else tend
-    qf, elf, stf, _ = run(w, factor * nx, factor * ny, "fp64_fast", tend, flux)
+    qf, elf, stf, _ = run(w, factor * nx, factor * ny, "fp64_fast", tend, flux, segments)
     ref = block_average(qf, factor)
     out = {"reference": {"nx": factor * nx, "ny": factor * ny, "precision": "fp64_fast", "steps": stf,
                          "seconds": elf}, "nx": nx, "ny": ny, "t_end": tend}
@@ -56,6 +69,8 @@ def matched_accuracy(w, nx, ny, factor=4, precisions=("fp64", "fp32_mixed", "fp3
             r["linear_fraction"] = l / max(f, 1)
             r["teno_fallback_fraction"] = 1.0 - l / max(f, 1)
         out[p] = r
+        if segments > 0:
+            print(f"{nx} x {ny} {p}: l1_rho {r['l1_rho']:.4e}, {st} steps, {el:.1f} s", flush=True)
     if "fp64" in out:
         for p in precisions:
             out[p]["l1_rho_vs_fp64"] = out[p]["l1_rho"] / out["fp64"]["l1_rho"]
