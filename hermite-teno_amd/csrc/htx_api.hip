@@ -142,8 +142,8 @@ int check_params(const htx_params* p) {
         return fail(HTX_EINVAL, "unknown precision");
     if (is_learned(p->precision) && p->scheme != HTX_SCHEME_TENO5)
         return fail(HTX_ENOTSUP, "the learned precision is built for the TENO5 scheme only");
-    if (p->precision == HTX_PREC_FP32 && p->scheme != HTX_SCHEME_TENO5)
-        return fail(HTX_ENOTSUP, "the fp32 bulk precision is built for the TENO5 scheme only");
+    if (p->precision == HTX_PREC_FP32 && p->scheme != HTX_SCHEME_TENO5 && p->scheme != HTX_SCHEME_HTENO)
+        return fail(HTX_ENOTSUP, "the fp32 bulk precision is built for the TENO5 and Hermite-TENO schemes");
     return HTX_OK;
 }
 
@@ -201,7 +201,8 @@ FvhArgs hermite_args(htx_solver* h, const double* qin, const double* q0, double*
     a.mode = mode;
     a.quirks = h->p.ref_quirks ? 1 : 0;
     a.flux = h->p.flux;
-    a.mixed = h->p.precision == HTX_PREC_FP32_MIXED;
+    a.mixed = (h->p.precision == HTX_PREC_FP32_MIXED || h->p.precision == HTX_PREC_FP32) ? 1 : 0;
+    a.bulk32 = h->p.precision == HTX_PREC_FP32 ? 1 : 0;
     a.hrecon = htx_hrecon_code(h->p.scheme);
     a.gam = h->p.gamma;
     a.g1 = h->p.gamma - 1.0;

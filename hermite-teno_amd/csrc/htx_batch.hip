@@ -79,7 +79,8 @@ __global__ void __launch_bounds__(LIM_THREADS) kb_fvh_limit(BatchHermArgs B) {
     fvh_limit_block(a, lb, (4 * a.ny + 63) / 64);
 }
 
-template <bool YDIR, bool MIXED>
+// FT: the face values' storage type (float in the FP32 bulk precision)
+template <bool YDIR, bool MIXED, typename FT = double>
 __global__ void __launch_bounds__(256) kb_fvh_recon(BatchHermArgs B) {
     int lb;
     const FvhArgs& a = herm_args(B, YDIR ? HK_RECON_Y : HK_RECON_X, lb);
@@ -89,10 +90,10 @@ __global__ void __launch_bounds__(256) kb_fvh_recon(BatchHermArgs B) {
     if (t >= nf) return;
     if (YDIR) {
         int i = (int)(t % a.nx), fj = (int)(t / a.nx);
-        fvh_recon_face<true, MIXED>(a, i, fj - 1, a.fy, t, nf);
+        fvh_recon_face<true, MIXED>(a, i, fj - 1, reinterpret_cast<FT*>(a.fy), t, nf);
     } else {
         int fi = (int)(t % (a.nx + 1)), j = (int)(t / (a.nx + 1));
-        fvh_recon_face<false, MIXED>(a, fi - 1, j, a.fx, t, nf);
+        fvh_recon_face<false, MIXED>(a, fi - 1, j, reinterpret_cast<FT*>(a.fx), t, nf);
     }
 }
 
@@ -104,6 +105,16 @@ __global__ void __launch_bounds__(192) kb_fvh_flux(BatchHermArgs B) {
     if (!a.ctl->active) return;
     long long nf = YDIR ? (long long)a.nx * (a.ny + 1) : (long long)(a.nx + 1) * a.ny;
     fvh_flux_block<YDIR, FLUX, MIXED>(a, lb * 64ll, nf, lds);
+}
+
+template <bool YDIR, int FLUX>
+__global__ void __launch_bounds__(192, 3) kb_fvh_flux_f32(BatchHermArgs B) {
+    __shared__ double lds[12 * 3 * 64];
+    int lb;
+    const FvhArgs& a = herm_args(B, YDIR ? HK_FLUX_Y : HK_FLUX_X, lb);
+    if (!a.ctl->active) return;
+    long long nf = YDIR ? (long long)a.nx * (a.ny + 1) : (long long)(a.nx + 1) * a.ny;
+    fvh_flux_block_f32<YDIR, FLUX>(a, lb * 64ll, nf, lds);
 }
 
 __global__ void __launch_bounds__(256) kb_fvh_update(BatchHermArgs B) {
@@ -194,10 +205,31 @@ void launch_herm_faces(const BatchHermArgs& B, const int nblocks[6], int flux, h
     }
 }
 
-// mixed: the Hermite instances' shared precision is fp32_mixed
-hipError_t launch_herm(const BatchHermArgs& B, const int nblocks[6], int flux, bool mixed, hipStream_t s) {
+// FP32 bulk precision: FP32 face values from the FP32-mixed reconstruction,
+// FP32 fluxes
+void launch_herm_faces_f32(const BatchHermArgs& B, const int nblocks[6], int flux, hipStream_t s) {
+    hipLaunchKernelGGL((kb_fvh_recon<false, true, float>), dim3(nblocks[HK_RECON_X]), dim3(256), 0, s, B);
+    hipLaunchKernelGGL((kb_fvh_recon<true, true, float>), dim3(nblocks[HK_RECON_Y]), dim3(256), 0, s, B);
+    dim3 fb(64, 3), gx(nblocks[HK_FLUX_X]), gy(nblocks[HK_FLUX_Y]);
+    if (flux == FLUX_RUSANOV_CHAR) {
+        hipLaunchKernelGGL((kb_fvh_flux_f32<false, FLUX_RUSANOV_CHAR>), gx, fb, 0, s, B);
+        hipLaunchKernelGGL((kb_fvh_flux_f32<true, FLUX_RUSANOV_CHAR>), gy, fb, 0, s, B);
+    } else if (flux == FLUX_HLLC) {
+        hipLaunchKernelGGL((kb_fvh_flux_f32<false, FLUX_HLLC>), gx, fb, 0, s, B);
+        hipLaunchKernelGGL((kb_fvh_flux_f32<true, FLUX_HLLC>), gy, fb, 0, s, B);
+    } else {
+        hipLaunchKernelGGL((kb_fvh_flux_f32<false, FLUX_HLLC_TORO>), gx, fb, 0, s, B);
+        hipLaunchKernelGGL((kb_fvh_flux_f32<true, FLUX_HLLC_TORO>), gy, fb, 0, s, B);
+    }
+}
+
+// prec: the Hermite instances' shared precision, 0 fp64_fast, 1 fp32_mixed,
+// 3 fp32 (bulk)
+hipError_t launch_herm(const BatchHermArgs& B, const int nblocks[6], int flux, int prec, hipStream_t s) {
     hipLaunchKernelGGL(kb_fvh_limit, dim3(nblocks[HK_LIMIT]), dim3(LIM_THREADS), 0, s, B);
-    if (mixed)
+    if (prec == 3)
+        launch_herm_faces_f32(B, nblocks, flux, s);
+    else if (prec == 1)
         launch_herm_faces<true>(B, nblocks, flux, s);
     else
         launch_herm_faces<false>(B, nblocks, flux, s);
@@ -233,7 +265,7 @@ struct htx_batch {
     std::vector<unsigned char*> dFlagsX, dFlagsY;
     // Hermite launches
     int herm_flux = FLUX_RUSANOV_CHAR;
-    bool herm_mixed = false;                // Hermite instances in fp32_mixed
+    int herm_prec = 0;                      // Hermite instances: 0 fp64_fast, 1 fp32_mixed, 3 fp32 (bulk)
     bool herm_strict = false;               // Hermite instances in FP64 (strict build, one launch per instance)
     std::vector<FvhArgs> herm_args;         // [mode - 1][Hermite instance], host copy
     FvhArgs* dHermArgs = nullptr;
@@ -407,7 +439,7 @@ int enqueue_batch_step(htx_batch* b, hipStream_t s) {
             }
             H.ninst = (int)b->herm.size();
             H.mode = mode;
-            BHIP(launch_herm(H, b->herm_blocks, b->herm_flux, b->herm_mixed, s));
+            BHIP(launch_herm(H, b->herm_blocks, b->herm_flux, b->herm_prec, s));
         }
     }
     hipLaunchKernelGGL(k_batch_tadvance, dim3((n + 255) / 256), dim3(256), 0, s, b->ctl, n);
@@ -490,7 +522,7 @@ int htx_batch_create(const htx_params* ps, int n, htx_batch** out) {
     if (!out) return htx_internal_fail(HTX_EINVAL, "out is null");
     *out = nullptr;
     if (!ps || n < 1) return htx_internal_fail(HTX_EINVAL, "need n >= 1 parameter blocks");
-    int teno_flux = -1, teno_prec = -1, herm_flux = -1, herm_mixed = -1;
+    int teno_flux = -1, teno_prec = -1, herm_flux = -1, herm_pc = -1;
     for (int i = 0; i < n; ++i) {
         const htx_params& p = ps[i];
         if (p.struct_size != (int)sizeof(htx_params)) return htx_internal_fail(HTX_EINVAL, "params.struct_size mismatch");
@@ -511,14 +543,18 @@ int htx_batch_create(const htx_params* ps, int n, htx_batch** out) {
             teno_flux = fc;
         } else if (p.scheme == HTX_SCHEME_HTENO) {
             if (p.precision != HTX_PREC_FP64 && p.precision != HTX_PREC_FP64_FAST &&
-                p.precision != HTX_PREC_FP32_MIXED)
+                p.precision != HTX_PREC_FP32_MIXED && p.precision != HTX_PREC_FP32)
                 return htx_internal_fail(HTX_ENOTSUP,
-                                         "batched Hermite instances run fp64, fp64_fast or fp32_mixed precision");
-            int fc = flux_code(p), pm = p.precision == HTX_PREC_FP32_MIXED ? 1 : (p.precision == HTX_PREC_FP64 ? 2 : 0);
-            if ((herm_flux >= 0 && fc != herm_flux) || (herm_mixed >= 0 && pm != herm_mixed))
+                                         "batched Hermite instances run fp64, fp64_fast, fp32_mixed or fp32 precision");
+            int fc = flux_code(p);
+            int pm = p.precision == HTX_PREC_FP32_MIXED ? 1
+                   : p.precision == HTX_PREC_FP64       ? 2
+                   : p.precision == HTX_PREC_FP32       ? 3
+                                                        : 0;
+            if ((herm_flux >= 0 && fc != herm_flux) || (herm_pc >= 0 && pm != herm_pc))
                 return htx_internal_fail(HTX_ENOTSUP, "the Hermite instances of a batch share flux and precision");
             herm_flux = fc;
-            herm_mixed = pm;
+            herm_pc = pm;
         } else {
             return htx_internal_fail(HTX_EINVAL, "unknown scheme");
         }
@@ -532,8 +568,8 @@ int htx_batch_create(const htx_params* ps, int n, htx_batch** out) {
     if (teno_flux >= 0) b->teno_flux = teno_flux;
     b->teno_direct = batch_prec_direct(b->teno_prec);
     if (herm_flux >= 0) b->herm_flux = herm_flux;
-    b->herm_mixed = herm_mixed == 1;
-    b->herm_strict = herm_mixed == 2;
+    b->herm_prec = herm_pc == 2 || herm_pc < 0 ? 0 : herm_pc;
+    b->herm_strict = herm_pc == 2;
     BatchLayout L = make_layout(ps, n);
     b->off = L.off;
     b->dxdy = L.dxdy;

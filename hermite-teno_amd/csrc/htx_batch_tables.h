@@ -186,7 +186,8 @@ inline HermTables make_herm_tables(const BatchLayout& L, double* state, Ctl* ctl
         a.bc = p.bc;
         a.quirks = p.ref_quirks ? 1 : 0;
         a.flux = p.flux;
-        a.mixed = p.precision == HTX_PREC_FP32_MIXED;
+        a.mixed = (p.precision == HTX_PREC_FP32_MIXED || p.precision == HTX_PREC_FP32) ? 1 : 0;
+        a.bulk32 = p.precision == HTX_PREC_FP32 ? 1 : 0;
         a.gam = p.gamma;
         a.g1 = p.gamma - 1.0;
         a.dx = L.dxdy[2 * i];

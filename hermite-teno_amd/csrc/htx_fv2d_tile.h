@@ -276,13 +276,7 @@ HTX_HD bool window_smooth_wave(const Fv2dTile<TX, TY, FT, OF>& s, int w, int tid
 // at least 2^-10 of its energy (rho E - m^2 / 2 > 2^-10 rho E), sends the
 // face to the FP64 path: back-projection and flux recomputed in FP64 from
 // the same Gauss-point values (on the learned path's linear windows, first
-// the nonlinear quadrature).
-HTX_HD bool state_fp32_ok(const float q[4], float c0, float c2, float c3) {
-    const float re = q[3] * q[0];
-    return q[0] > 0x1p-10f * (fabsf(c0) + fabsf(c2) + fabsf(c3)) &&
-           re - 0.5f * (q[1] * q[1] + q[2] * q[2]) > 0x1p-10f * re;
-}
-
+// the nonlinear quadrature).  The margins are state_fp32_ok's.
 template <int TX, int TY, bool YDIR, int FLUX, bool LEARNED, bool OF>
 HTX_HD void ph_flux32(const Fv2dArgs& a, Fv2dTile<TX, TY, float, OF>& s, int tid) {
     using T = Fv2dTile<TX, TY, float, OF>;

@@ -26,22 +26,23 @@ __device__ __forceinline__ double wave_max_fvh(double v) {
 // The limiter: fvh_limit_block (htx_hermite.h), 64 chains per workgroup.
 __global__ void __launch_bounds__(LIM_THREADS) k_fvh_limit(FvhArgs a, int g0) { fvh_limit_block(a, blockIdx.x, g0); }
 
-template <bool MIXED, bool VARIANT = false>
+// FT: the face values' storage type (float in the FP32 bulk precision)
+template <bool MIXED, bool VARIANT = false, typename FT = double>
 __global__ void __launch_bounds__(256) k_fvh_recon_x(FvhArgs a) {
     long long nf = (long long)(a.nx + 1) * a.ny;
     long long t = blockIdx.x * 256ll + threadIdx.x;
     if (t >= nf) return;
     int fi = (int)(t % (a.nx + 1)), j = (int)(t / (a.nx + 1));
-    fvh_recon_face<false, MIXED, VARIANT>(a, fi - 1, j, a.fx, t, nf);
+    fvh_recon_face<false, MIXED, VARIANT>(a, fi - 1, j, reinterpret_cast<FT*>(a.fx), t, nf);
 }
 
-template <bool MIXED, bool VARIANT = false>
+template <bool MIXED, bool VARIANT = false, typename FT = double>
 __global__ void __launch_bounds__(256) k_fvh_recon_y(FvhArgs a) {
     long long nf = (long long)a.nx * (a.ny + 1);
     long long t = blockIdx.x * 256ll + threadIdx.x;
     if (t >= nf) return;
     int i = (int)(t % a.nx), fj = (int)(t / a.nx);
-    fvh_recon_face<true, MIXED, VARIANT>(a, i, fj - 1, a.fy, t, nf);
+    fvh_recon_face<true, MIXED, VARIANT>(a, i, fj - 1, reinterpret_cast<FT*>(a.fy), t, nf);
 }
 
 // block (64 faces) x (3 Gauss points)
@@ -51,6 +52,16 @@ __global__ void __launch_bounds__(192) k_fvh_flux(FvhArgs a) {
     long long nf = YDIR ? (long long)a.nx * (a.ny + 1) : (long long)(a.nx + 1) * a.ny;
     fvh_flux_block<YDIR, FLUX, MIXED>(a, blockIdx.x * 64ll, nf, lds);
 }
+
+#if !HTX_EXACT_DIVISION
+// FP32 bulk precision: 18 KB of LDS per 64 faces x 3 Gauss points
+template <bool YDIR, int FLUX>
+__global__ void __launch_bounds__(192, 3) k_fvh_flux_f32(FvhArgs a) {
+    __shared__ double lds[12 * 3 * 64];
+    long long nf = YDIR ? (long long)a.nx * (a.ny + 1) : (long long)(a.nx + 1) * a.ny;
+    fvh_flux_block_f32<YDIR, FLUX>(a, blockIdx.x * 64ll, nf, lds);
+}
+#endif
 
 __global__ void __launch_bounds__(256) k_fvh_update(FvhArgs a) {
     __shared__ double red[2][4];
@@ -172,13 +183,36 @@ static void launch_recon_flux(const FvhArgs& a, int flux, hipStream_t s) {
     }
 }
 
+#if !HTX_EXACT_DIVISION
+// FP32 bulk precision: the FP32-mixed reconstruction writing FP32 face values,
+// then the FP32 flux kernels
+static void launch_recon_flux_f32(const FvhArgs& a, int flux, hipStream_t s) {
+    long long nxf = (long long)(a.nx + 1) * a.ny, nyf = (long long)a.nx * (a.ny + 1);
+    hipLaunchKernelGGL((k_fvh_recon_x<true, false, float>), dim3((unsigned)((nxf + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_fvh_recon_y<true, false, float>), dim3((unsigned)((nyf + 255) / 256)), dim3(256), 0, s, a);
+    dim3 fb(64, 3), gx((unsigned)((nxf + 63) / 64)), gy((unsigned)((nyf + 63) / 64));
+    if (flux == FLUX_RUSANOV_CHAR) {
+        hipLaunchKernelGGL((k_fvh_flux_f32<false, FLUX_RUSANOV_CHAR>), gx, fb, 0, s, a);
+        hipLaunchKernelGGL((k_fvh_flux_f32<true, FLUX_RUSANOV_CHAR>), gy, fb, 0, s, a);
+    } else if (flux == FLUX_HLLC) {
+        hipLaunchKernelGGL((k_fvh_flux_f32<false, FLUX_HLLC>), gx, fb, 0, s, a);
+        hipLaunchKernelGGL((k_fvh_flux_f32<true, FLUX_HLLC>), gy, fb, 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_fvh_flux_f32<false, FLUX_HLLC_TORO>), gx, fb, 0, s, a);
+        hipLaunchKernelGGL((k_fvh_flux_f32<true, FLUX_HLLC_TORO>), gy, fb, 0, s, a);
+    }
+}
+#endif
+
 hipError_t launch_fvh2d_stage(const FvhArgs& a, int flux, hipStream_t s) {
     hipLaunchKernelGGL(k_fvh_limit, dim3(lim_blocks(a.nx, a.ny)), dim3(LIM_THREADS), 0, s, a, (4 * a.ny + 63) / 64);
 #if HTX_EXACT_DIVISION
-    if (a.mixed) return hipErrorInvalidValue;  // the strict build is FP64 only
+    if (a.mixed || a.bulk32) return hipErrorInvalidValue;  // the strict build is FP64 only
     launch_recon_flux<false>(a, flux, s);
 #else
-    if (a.mixed)
+    if (a.bulk32)
+        launch_recon_flux_f32(a, flux, s);
+    else if (a.mixed)
         launch_recon_flux<true>(a, flux, s);
     else
         launch_recon_flux<false>(a, flux, s);

@@ -531,6 +531,7 @@ struct FvhArgs {
     long long fstride;   // per field of the state
     int bc, mode, quirks, flux;
     int mixed;           // 1: FP32 cut-off rules and transverse weights (fast build only)
+    int bulk32;          // 1: FP32 face values and fluxes (with mixed; fast build only)
     int hrecon;          // face reconstruction (herm_recon): 0 hteno, 1 hweno, 2 htenoi, 3 linear
     double gam, g1, dx, dy, area;
     QuadConsts qc;
@@ -862,8 +863,8 @@ HTX_HD double fvh_lim(const FvhArgs& a, int comp, int i, int j) { return a.lv[co
 // characteristic face values at face index fid of the face buffer.  MIXED:
 // the HTENO and TENO5 cut-off rules in FP32 (hteno_side_mixed,
 // teno5_side_mixed), every value and projection in FP64.
-template <bool YDIR, bool MIXED = false, bool VARIANT = false>
-HTX_HD void fvh_recon_face(const FvhArgs& a, int i, int j, double* F, long long fid, long long fstr) {
+template <bool YDIR, bool MIXED = false, bool VARIANT = false, typename FT = double>
+HTX_HD void fvh_recon_face(const FvhArgs& a, int i, int j, FT* F, long long fid, long long fstr) {
     const long long fs = a.fstride;
     auto cellq = [&](int k, double q[4]) {  // k: offset along the normal
         long long o = YDIR ? fvh_off(a, i, j + k) : fvh_off(a, i + k, j);
@@ -917,12 +918,12 @@ HTX_HD void fvh_recon_face(const FvhArgs& a, int i, int j, double* F, long long 
         else
             teno5(C[e], um, up);
         // series: 0 u^-, 1 u^+, 2 vn^-, 3 vn^+, 4 vt^-, 5 vt^+ (vn: normal moment)
-        F[(0 * 4 + e) * fstr + fid] = o[0];
-        F[(1 * 4 + e) * fstr + fid] = o[1];
-        F[(2 * 4 + e) * fstr + fid] = o[2];
-        F[(3 * 4 + e) * fstr + fid] = o[3];
-        F[(4 * 4 + e) * fstr + fid] = um;
-        F[(5 * 4 + e) * fstr + fid] = up;
+        F[(0 * 4 + e) * fstr + fid] = (FT)o[0];
+        F[(1 * 4 + e) * fstr + fid] = (FT)o[1];
+        F[(2 * 4 + e) * fstr + fid] = (FT)o[2];
+        F[(3 * 4 + e) * fstr + fid] = (FT)o[3];
+        F[(4 * 4 + e) * fstr + fid] = (FT)um;
+        F[(5 * 4 + e) * fstr + fid] = (FT)up;
     }
 }
 
@@ -995,19 +996,22 @@ HTX_HD void fvh_series_gp3(const QuadConsts& qc, const double* F, long long fstr
         weno3q_gp3(qc, S[fid[0]], S[fid[1]], S[fid[2]], S[fid[3]], S[fid[4]], g);
 }
 
-template <bool YDIR, int FLUX>
-HTX_HD void fvh_flux_at(const FvhArgs& a, int i, int j, int m, const double c[6][4], double out[12]) {
+// the face's Roe eigensystem from its two cells
+template <bool YDIR>
+HTX_HD Eig fvh_face_eig(const FvhArgs& a, int i, int j) {
     double qa[4], qb[4];
-    {
-        const long long fs = a.fstride;
-        long long oa = fvh_off(a, i, j), ob = YDIR ? fvh_off(a, i, j + 1) : fvh_off(a, i + 1, j);
+    const long long fs = a.fstride;
+    long long oa = fvh_off(a, i, j), ob = YDIR ? fvh_off(a, i, j + 1) : fvh_off(a, i + 1, j);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            qa[e] = a.q[oa + e * fs];
-            qb[e] = a.q[ob + e * fs];
-        }
+    for (int e = 0; e < 4; ++e) {
+        qa[e] = a.q[oa + e * fs];
+        qb[e] = a.q[ob + e * fs];
     }
-    Eig E = roe_eig(a.g1, YDIR, cell_prim(a.g1, qa[0], qa[1], qa[2], qa[3]), cell_prim(a.g1, qb[0], qb[1], qb[2], qb[3]));
+    return roe_eig(a.g1, YDIR, cell_prim(a.g1, qa[0], qa[1], qa[2], qa[3]), cell_prim(a.g1, qb[0], qb[1], qb[2], qb[3]));
+}
+
+template <bool YDIR, int FLUX>
+HTX_HD void fvh_flux_at_e(const FvhArgs& a, const Eig& E, int m, const double c[6][4], double out[12]) {
     double Q[6][4];  // unrolled loops keep Q in registers
 #pragma unroll
     for (int sr = 0; sr < 6; ++sr) eig_right(E, c[sr][0], c[sr][1], c[sr][2], c[sr][3], Q[sr][0], Q[sr][1], Q[sr][2], Q[sr][3]);
@@ -1042,6 +1046,117 @@ HTX_HD void fvh_flux_at(const FvhArgs& a, int i, int j, int m, const double c[6]
         out[4 * g + 1] = gw * (YDIR ? f[2] : f[1]);
         out[4 * g + 2] = gw * (YDIR ? f[1] : f[2]);
         out[4 * g + 3] = gw * f[3];
+    };
+    put(0, fu);
+    put(1, fv);
+    put(2, fw);
+}
+
+template <bool YDIR, int FLUX>
+HTX_HD void fvh_flux_at(const FvhArgs& a, int i, int j, int m, const double c[6][4], double out[12]) {
+    fvh_flux_at_e<YDIR, FLUX>(a, fvh_face_eig<YDIR>(a, i, j), m, c, out);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 bulk precision of the fluxes: the face values come in FP32, the
+// transverse quadrature (weno3q_f32), the back-projection, the u flux and the
+// moment fluxes run in FP32 from the face's FP64 Roe eigensystem rounded
+// once; the Gauss-weighted sums are FP64.  A Gauss point whose FP32 u states
+// fail state_fp32_ok is recomputed in FP64 from the same Gauss-point values.
+
+template <bool YDIR>
+HTX_HD void dflux_mul_f32(float gam, float g1, float q3, float rho, float u, float v, float p, const float d[4],
+                          float o[4]) {
+    const float g3 = gam - 3.0f;
+    const float h = (q3 + p) * frcp(rho);
+    if (!YDIR) {
+        const float sq = u * u, dyn = 0.5f * g1 * (sq + v * v);
+        o[0] = d[1];
+        o[1] = (((dyn - sq) * d[0] + (-g3 * u) * d[1]) + (-g1 * v) * d[2]) + g1 * d[3];
+        o[2] = ((-u * v) * d[0] + v * d[1]) + u * d[2];
+        o[3] = ((((dyn - h) * u) * d[0] + (h - g1 * sq) * d[1]) + (-g1 * u * v) * d[2]) + (gam * u) * d[3];
+    } else {
+        const float sq = v * v, dyn = 0.5f * g1 * (sq + u * u);
+        o[0] = d[2];
+        o[1] = ((-u * v) * d[0] + v * d[1]) + u * d[2];
+        o[2] = (((dyn - sq) * d[0] + (-g1 * u) * d[1]) + (-g3 * v) * d[2]) + g1 * d[3];
+        o[3] = ((((dyn - h) * v) * d[0] + (-g1 * u * v) * d[1]) + (h - g1 * sq) * d[2]) + (gam * v) * d[3];
+    }
+}
+
+// drusanov in FP32 (Equations_Euler2D.cpp:352-360), the same (rho, n, t, E)
+// conventions
+template <bool YDIR>
+HTX_HD void drusanov_f32(float gam, float g1, const Eig32& E, const float ql[4], const float qr[4], float pl,
+                         float pr, const float a[4], const float dql[4], const float dqr[4], float f[4]) {
+    const int n = YDIR ? 2 : 1, t = YDIR ? 1 : 2;
+    float dl[4], dr[4], jl[4], jr[4];
+    // natural (rho, x, y, E) order for the Jacobian products
+    dl[0] = dql[0], dl[n] = dql[1], dl[t] = dql[2], dl[3] = dql[3];
+    dr[0] = dqr[0], dr[n] = dqr[1], dr[t] = dqr[2], dr[3] = dqr[3];
+    const float il = frcp(ql[0]), ir = frcp(qr[0]);
+    const float unl = ql[1] * il, utl = ql[2] * il, unr = qr[1] * ir, utr = qr[2] * ir;
+    dflux_mul_f32<YDIR>(gam, g1, ql[3], ql[0], YDIR ? utl : unl, YDIR ? unl : utl, pl, dl, jl);
+    dflux_mul_f32<YDIR>(gam, g1, qr[3], qr[0], YDIR ? utr : unr, YDIR ? unr : utr, pr, dr, jr);
+    float c0, c1, c2, c3, r0, rn, rt, r3;
+    eig_left_nt_f32(E, dr[0] - dl[0], dr[n] - dl[n], dr[t] - dl[t], dr[3] - dl[3], c0, c1, c2, c3);
+    eig_right_f32(E, a[0] * c0, a[1] * c1, a[1] * c2, a[3] * c3, r0, rn, rt, r3);
+    f[0] = 0.5f * ((jl[0] + jr[0]) - r0);
+    f[1] = 0.5f * ((jl[n] + jr[n]) - rn);
+    f[2] = 0.5f * ((jl[t] + jr[t]) - rt);
+    f[3] = 0.5f * ((jl[3] + jr[3]) - r3);
+}
+
+HTX_HD void fvh_series_gp3_f32(const QuadConsts& qc, const float* F, long long fstr, int series, const long long fid[5],
+                               float g[3]) {
+    const float* S = F + (long long)series * fstr;
+    weno3q_f32(qc, S[fid[0]], S[fid[1]], S[fid[2]], S[fid[3]], S[fid[4]], g[0], g[1], g[2]);
+}
+
+template <bool YDIR, int FLUX>
+HTX_HD void fvh_flux_at_f32(const FvhArgs& a, int i, int j, int m, const float c[6][4], double out[12]) {
+    const Eig E = fvh_face_eig<YDIR>(a, i, j);
+    const Eig32 E32 = eig32(E);
+    float Q[6][4];
+#pragma unroll
+    for (int sr = 0; sr < 6; ++sr)
+        eig_right_f32(E32, c[sr][0], c[sr][1], c[sr][2], c[sr][3], Q[sr][0], Q[sr][1], Q[sr][2], Q[sr][3]);
+    if (!(state_fp32_ok(Q[0], c[0][0], c[0][2], c[0][3]) && state_fp32_ok(Q[1], c[1][0], c[1][2], c[1][3]))) {
+        double cd[6][4];
+#pragma unroll
+        for (int sr = 0; sr < 6; ++sr)
+#pragma unroll
+            for (int f = 0; f < 4; ++f) cd[sr][f] = (double)c[sr][f];
+        fvh_flux_at_e<YDIR, FLUX>(a, E, m, cd, out);
+        return;
+    }
+    const float gam = (float)a.gam, g1 = (float)a.g1;
+    const float il = frcp(Q[0][0]), ir = frcp(Q[1][0]);
+    const float wnl = Q[0][1] * il, wtl = Q[0][2] * il, wnr = Q[1][1] * ir, wtr = Q[1][2] * ir;
+    const float pl = (Q[0][3] - 0.5f * (Q[0][1] * wnl + Q[0][2] * wtl)) * g1;
+    const float pr = (Q[1][3] - 0.5f * (Q[1][1] * wnr + Q[1][2] * wtr)) * g1;
+    const float cl = fsqrt(fabsf(gam * pl * il)), cr = fsqrt(fabsf(gam * pr * ir));
+    float am[4];
+    am[0] = fmaxf(fabsf(wnl - cl), fabsf(wnr - cr));
+    am[1] = fmaxf(fabsf(wnl), fabsf(wnr));
+    am[2] = am[1];
+    am[3] = fmaxf(fabsf(wnl + cl), fabsf(wnr + cr));
+    float fu[4], fv[4], fw[4];
+    if (FLUX == FLUX_RUSANOV_CHAR)
+        rusanov_char_f32(gam, g1, E32, Q[0], Q[1], fu);
+    else if (FLUX == FLUX_HLLC)
+        hllc_f32<true>(gam, g1, Q[0], Q[1], fu);
+    else
+        hllc_f32<false>(gam, g1, Q[0], Q[1], fu);
+    const int sv = YDIR ? 4 : 2, sw = YDIR ? 2 : 4;
+    drusanov_f32<YDIR>(gam, g1, E32, Q[0], Q[1], pl, pr, am, Q[sv], Q[sv + 1], fv);
+    drusanov_f32<YDIR>(gam, g1, E32, Q[0], Q[1], pl, pr, am, Q[sw], Q[sw + 1], fw);
+    const double gw = (m == 1) ? 8.0 / 9.0 / 2.0 : 5.0 / 9.0 / 2.0;
+    auto put = [&](int g, const float f[4]) {
+        out[4 * g + 0] = gw * (double)f[0];
+        out[4 * g + 1] = gw * (double)(YDIR ? f[2] : f[1]);
+        out[4 * g + 2] = gw * (double)(YDIR ? f[1] : f[2]);
+        out[4 * g + 3] = gw * (double)f[3];
     };
     put(0, fu);
     put(1, fv);
@@ -1113,6 +1228,69 @@ __device__ void fvh_flux_block(const FvhArgs& a, long long t0, long long nf, dou
 #pragma unroll
         for (int k = 0; k < 12; ++k) G[k * nf + t] = (lds[k * 64 + x] + lds[(12 + k) * 64 + x]) + lds[(24 + k) * 64 + x];
     }
+}
+
+// FP32 bulk form of fvh_flux_block: the series values go through LDS in FP32
+// (24 x 3 x 64 floats), the weighted fluxes in FP64 (12 x 3 x 64 doubles) in
+// the same 18 KB.
+template <bool YDIR, int FLUX>
+__device__ void fvh_flux_block_f32(const FvhArgs& a, long long t0, long long nf, double* lds) {
+    float* ldsf = reinterpret_cast<float*>(lds);
+    const int x = threadIdx.x, m = threadIdx.y;
+    const long long t = t0 + x;
+    const bool valid = t < nf;
+    int i = 0, j = 0;
+    if (YDIR) {
+        i = (int)(t % a.nx);
+        j = (int)(t / a.nx) - 1;
+    } else {
+        i = (int)(t % (a.nx + 1)) - 1;
+        j = (int)(t / (a.nx + 1));
+    }
+    const float* F = reinterpret_cast<const float*>(YDIR ? a.fy : a.fx);
+    if (valid) {
+        long long fid[5];
+        fvh_flux_fids<YDIR>(a, i, j, fid);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int s = 8 * m + q;
+            float g[3];
+            fvh_series_gp3_f32(kQuad, F, nf, s, fid, g);
+#pragma unroll
+            for (int p = 0; p < 3; ++p) ldsf[(s * 3 + p) * 64 + x] = g[p];
+        }
+    }
+    __syncthreads();
+    float c[6][4];
+#pragma unroll
+    for (int s = 0; s < 24; ++s) c[s / 4][s % 4] = valid ? ldsf[(s * 3 + m) * 64 + x] : 0.0f;
+    __syncthreads();
+    if (valid) {
+        double o[12];
+        fvh_flux_at_f32<YDIR, FLUX>(a, i, j, m, c, o);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) lds[(m * 12 + k) * 64 + x] = o[k];
+    }
+    __syncthreads();
+    if (valid && m == 0) {
+        double* G = YDIR ? a.gy : a.gx;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) G[k * nf + t] = (lds[k * 64 + x] + lds[(12 + k) * 64 + x]) + lds[(24 + k) * 64 + x];
+    }
+}
+
+// All of one face on one work item in the FP32 bulk precision (the CPU form)
+template <bool YDIR, int FLUX>
+HTX_HD void fvh_flux_face_f32(const FvhArgs& a, int i, int j, const float* F, long long fstr, double out[3][12]) {
+    long long fid[5];
+    fvh_flux_fids<YDIR>(a, i, j, fid);
+    float c[3][6][4];
+    for (int s = 0; s < 24; ++s) {
+        float g[3];
+        fvh_series_gp3_f32(kQuad, F, fstr, s, fid, g);
+        for (int m = 0; m < 3; ++m) c[m][s / 4][s % 4] = g[m];
+    }
+    for (int m = 0; m < 3; ++m) fvh_flux_at_f32<YDIR, FLUX>(a, i, j, m, c[m], out[m]);
 }
 
 // divergence of the 12 fluxes (FVHermite_System2D.cpp:710-730), SSP-RK3

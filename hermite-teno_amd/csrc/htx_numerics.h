@@ -1047,6 +1047,17 @@ HTX_HD void eig_left_nt_f32(const Eig32& E, float q0, float qn, float qt, float 
     c3 = ((0.5f * (E.b2 - tr) * q0 + ln3 * qn) + lt0 * qt) + E.t2 * q3;
 }
 
+// An FP32 back-projected state (rho, m_n, m_t, E) from characteristic values
+// c is usable when its density is positive and well above the cancellation
+// of the back-projection (rho > 2^-10 (|c0| + |c2| + |c3|)) and its pressure
+// is positive and at least 2^-10 of its energy (rho E - m^2 / 2 > 2^-10
+// rho E); else the FP32 bulk paths recompute in FP64.
+HTX_HD bool state_fp32_ok(const float q[4], float c0, float c2, float c3) {
+    const float re = q[3] * q[0];
+    return q[0] > 0x1p-10f * (fabsf(c0) + fabsf(c2) + fabsf(c3)) &&
+           re - 0.5f * (q[1] * q[1] + q[2] * q[2]) > 0x1p-10f * re;
+}
+
 // density and pressure (gamma - 1)(E - m^2 / 2 rho) of a state positive,
 // tested as rho E - m^2 / 2 > 0 (same sign for rho > 0)
 HTX_HD bool state_positive_f32(const float q[4]) {

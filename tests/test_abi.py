@@ -99,7 +99,7 @@ def test_batch_rejects_unbuilt_combinations():
 
 
 def test_hermite_rejects_learned_precision():
-    """Hermite-TENO is built in fp64, fp64_fast and fp32_mixed; the learned
+    """Hermite-TENO is built in fp64, fp64_fast, fp32_mixed and fp32; the learned
     precision (a TENO5 classifier) is refused before any device work, for a
     solver and for batched Hermite instances, as are Hermite instances of
     one batch that mix precisions."""

@@ -86,6 +86,29 @@ def test_batch_fp32_mixed_matches_single_instances():
     b.close()
 
 
+def test_batch_fp32_bulk_matches_single_instances():
+    """Both schemes in the FP32 bulk precision (TENO5 one launch per
+    instance, Hermite-TENO in the multi-instance kernels with FP32 face values
+    and fluxes): each instance matches its solver run alone (5 steps), the
+    TENO5 ones bit for bit (the same kernel), the Hermite ones to FP32
+    rounding (the same bodies compiled into other kernels, where the FP32
+    contractions may fall differently)."""
+    print("[case] batch of 6 in fp32 vs single solvers, 5 steps", flush=True)
+    b, inst = make_batch("fp32", tmax=10.0, herm_precision="fp32")
+    b.steps(5)
+    for i, d in enumerate(inst):
+        q_ref, t_ref = single(d, "fp32", 5, herm_precision="fp32")
+        q = b.get_state(i)
+        t, n = b.time(i)
+        err = rel_l1(q, q_ref)
+        print(f"instance {i} {d['scheme']} {d['nx']}x{d['ny']}: rel L1 {err:.3e}", flush=True)
+        assert np.isfinite(q).all()
+        assert n == 5
+        assert t == pytest.approx(t_ref, rel=1e-9)
+        assert err < (1e-6 if d["scheme"] == "hteno" else 1e-12)
+    b.close()
+
+
 def test_batch_tile_shapes_agree():
     """The tile shape changes where faces are computed, not what: every shape
     gives the same state bit for bit."""

@@ -113,6 +113,36 @@ def test_hermite_fp32_mixed_residual(orc, flux):
     assert err < 2e-8
 
 
+@pytest.mark.parametrize("flux", ["rusanov", "hllc"])
+@pytest.mark.parametrize("bc", ["periodic", "zero_grad"])
+def test_hermite_fp32_bulk_residual(orc, flux, bc):
+    """FP32 bulk Hermite residual (FP32 face values, transverse quadrature,
+    back-projection, u and moment fluxes; FP64 Roe eigensystem, Gauss sums,
+    divergence): each of the u, v, w field groups within 1e-4 relative L1 of
+    the FP64 oracle on the vortex, the u fields conserved on the periodic
+    grid, and a uniform state's residual exactly zero."""
+    from htx.ic import vortex_modes
+
+    s, prm = make(48, bc, flux, precision="fp32")
+    q = orc.ic_vortex_hermite(prm, (5.0, 5.0), vortex_modes())
+    s.set_state(q)
+    r = s.rhs()
+    rc = orc.fvh2d_rhs(prm, q)
+    assert np.isfinite(r).all()
+    for g in range(0, 12, 4):
+        err = rel_l1(r[g:g + 4], rc[g:g + 4])
+        print(f"hermite fp32 bulk fields {g}..{g + 3}: rel L1 {err:.3e}", flush=True)
+        assert 0.0 < err < 1e-4
+    if bc == "periodic":
+        for e in range(4):
+            assert abs(r[e].sum()) <= 1e-12 * np.abs(r[e]).sum()
+    u = np.zeros((12, 48, 48))
+    u[:4] = np.array([1.2, 0.3, -0.4, 2.5])[:, None, None]
+    s.set_state(u)
+    assert np.abs(s.rhs()).max() == 0.0
+    s.close()
+
+
 def _hermite_run(n, q0, dom, bc, flux, tend, precision):
     import htx
 
@@ -130,9 +160,10 @@ def _hermite_run(n, q0, dom, bc, flux, tend, precision):
     return out
 
 
+@pytest.mark.parametrize("precision", ["fp32_mixed", "fp32"])
 @pytest.mark.parametrize("case", ["quadrant", "vortex"])
-def test_hermite_fp32_mixed_accuracy(case):
-    """Hermite FP32-mixed at matched accuracy: against a 4x finer FP64
+def test_hermite_fp32_mixed_accuracy(case, precision):
+    """Hermite FP32-mixed and FP32 bulk at matched accuracy: against a 4x finer FP64
     solution averaged back onto the grid, the L1 error of the cell averages
     is within 10 % of the plain FP64 scheme's error at the same resolution
     (density, and all four conserved fields).  Quadrant: seeded Riemann data
@@ -151,16 +182,16 @@ def test_hermite_fp32_mixed_accuracy(case):
     else:
         dom, bc, tend = (0.0, 10.0, 0.0, 10.0), "periodic", 0.5
         q0 = q0f = None
-    print(f"[case] hermite fp32_mixed accuracy {case} n={n} t={tend}", flush=True)
+    print(f"[case] hermite {precision} accuracy {case} n={n} t={tend}", flush=True)
     q64 = _hermite_run(n, q0, dom, bc, flux, tend, "fp64")[:4]
-    q32 = _hermite_run(n, q0, dom, bc, flux, tend, "fp32_mixed")[:4]
+    q32 = _hermite_run(n, q0, dom, bc, flux, tend, precision)[:4]
     qf = _block_average(_hermite_run(4 * n, q0f, dom, bc, flux, tend, "fp64")[:4], 4)
     assert np.isfinite(q32).all()
     for name, sl in [("rho", slice(0, 1)), ("all", slice(0, 4))]:
         e64 = rel_l1(q64[sl], qf[sl])
         e32 = rel_l1(q32[sl], qf[sl])
-        print(f"{name}: L1 error vs 4x finer FP64: fp64 {e64:.4e}, fp32_mixed {e32:.4e}, ratio {e32 / e64:.4f}; "
-              f"fp32_mixed - fp64 {rel_l1(q32[sl], q64[sl]):.3e}", flush=True)
+        print(f"{name}: L1 error vs 4x finer FP64: fp64 {e64:.4e}, {precision} {e32:.4e}, ratio {e32 / e64:.4f}; "
+              f"{precision} - fp64 {rel_l1(q32[sl], q64[sl]):.3e}", flush=True)
         assert e32 <= 1.10 * e64
 
 
