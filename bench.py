@@ -185,13 +185,13 @@ def precision_study(n=2048):
     return r
 
 
-def batch_study(count=256, steps=20, t_compare=0.02, teno_precision="learned_fp32", herm_precision="fp32_mixed"):
+def batch_study(count=256, steps=20, t_compare=0.02, teno_precision="learned_fp32", herm_precision="fp32"):
     """Serving-style sweep: `count` seeded quadrant problems with sides
     log-uniform in [64, 1024], half TENO5 + HLLC, half Hermite-TENO + Rusanov,
     advanced together with the TENO5 instances in `teno_precision` and the
-    Hermite instances in `herm_precision` (default: FP32 bulk arithmetic plus
-    the learned indicator for TENO5, FP32 decisions with FP64 values for
-    Hermite; RK accumulation in FP64 throughout); aggregate cell-updates/s
+    Hermite instances in `herm_precision` (default: FP32 bulk arithmetic, plus
+    the learned indicator for TENO5; RK accumulation in FP64 throughout);
+    aggregate cell-updates/s
     over `steps` steps from t = 0 (every instance active; eager and graph
     replay, autotuned tile shape) and, with t_compare, each instance's
     relative L1 density difference at t = t_compare against the plain FP64
@@ -425,7 +425,7 @@ def main():
             # the sweep's own precision (learned indicator + FP32 bulk), then
             # the persistent launch's fp32_mixed for comparison (timing only)
             out["batch"] = batch_study()
-            out["batch_fp32_mixed"] = batch_study(t_compare=0.0, teno_precision="fp32_mixed")
+            out["batch_fp32_mixed"] = batch_study(t_compare=0.0, teno_precision="fp32_mixed", herm_precision="fp32_mixed")
         except Exception as e:
             log(f"[gpu] batch study failed: {e}")
     if not args.no_cpu and not herm:

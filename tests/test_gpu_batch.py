@@ -275,7 +275,7 @@ def test_batch_serving_sweep_against_fp64():
     log-uniform sizes, TENO5 + HLLC or Hermite-TENO + Rusanov with
     probability 1/2 each) at reduced count, sizes and end time, run in the
     sweep's precision: TENO5 learned_fp32 (FP32 bulk arithmetic plus the
-    learned indicator), Hermite fp32_mixed.  Every instance reaches its end
+    learned indicator), Hermite fp32 (FP32 bulk).  Every instance reaches its end
     time and stays within 1e-3 relative L1 of the oracle-pinned
     single-instance FP64 solver."""
     import htx
@@ -283,10 +283,10 @@ def test_batch_serving_sweep_against_fp64():
     from htx.ic import batch_workload
 
     ws = batch_workload(count=12, lo=32, hi=160, tend=0.03)
-    print(f"[case] serving sweep: {len(ws)} instances, learned_fp32 / fp32_mixed vs strict FP64", flush=True)
+    print(f"[case] serving sweep: {len(ws)} instances, learned_fp32 / fp32 vs strict FP64", flush=True)
     ps = [params_for(w["xl"], w["xr"], w["nx"], w["yb"], w["yt"], w["ny"], w["cfl"], w["tmax"], w["gamma"],
                      scheme=w["scheme"], flux=w["flux"], bc=w["bc"],
-                     precision="learned_fp32" if w["scheme"] == "teno5" else "fp32_mixed") for w in ws]
+                     precision="learned_fp32" if w["scheme"] == "teno5" else "fp32") for w in ws]
     b = Batch(ps)
     b.load_classifier()
     for i, w in enumerate(ws):
