@@ -406,6 +406,7 @@ def main():
                          ("hteno_hllc_fp64", dict(flux=args.flux, scheme="hteno")),
                          ("hteno_hllc_fp64_fast", dict(flux=args.flux, scheme="hteno", precision="fp64_fast")),
                          ("hteno_hllc_fp32_mixed", dict(flux=args.flux, scheme="hteno", precision="fp32_mixed")),
+                         ("hteno_hllc_fp32", dict(flux=args.flux, scheme="hteno", precision="fp32")),
                          ("teno5_rusanov_fp64", dict(flux="rusanov"))]:
             try:
                 log(f"[gpu] variant {name}")

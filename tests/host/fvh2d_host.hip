@@ -2,13 +2,15 @@
 // bodies (htx_hermite.h): the limiter chains, face reconstructions, Gauss-point
 // fluxes and the cell update run here in loops in the order the GPU launches
 // them.  Test infrastructure, compared with the FP64 oracle.
+#include <type_traits>
 #include <vector>
 
 #include "../../hermite-teno_amd/csrc/htx_hermite.h"
 
 using namespace htx;
 
-template <bool MIXED, bool VARIANT = false>
+// BULK32: the FP32 bulk precision (with MIXED): FP32 face values and fluxes
+template <bool MIXED, bool VARIANT = false, bool BULK32 = false>
 static int rhs_impl(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc, int flux,
                     int quirks, const double* q, double* rhs, double* qlim, int hrecon = 0) {
     FvhArgs a{};
@@ -32,6 +34,7 @@ static int rhs_impl(int nx, int ny, double xl, double xr, double yb, double yt, 
     a.mode = 0;
     a.quirks = quirks;
     a.mixed = MIXED;
+    a.bulk32 = BULK32;
     a.hrecon = hrecon;
     a.gam = gamma;
     a.g1 = gamma - 1.0;
@@ -46,25 +49,36 @@ static int rhs_impl(int nx, int ny, double xl, double xr, double yb, double yt, 
         for (int i = 0; i < nx; ++i) fvh_limit_line(a, 1, i, e);
     }
     long long fxs = (long long)ny * (nx + 1), fys = (long long)(ny + 1) * nx;
+    using FT = typename std::conditional<BULK32, float, double>::type;
+    FT* Fx = reinterpret_cast<FT*>(a.fx);
+    FT* Fy = reinterpret_cast<FT*>(a.fy);
     for (int j = 0; j < ny; ++j)
         for (int fi = 0; fi <= nx; ++fi)
-            fvh_recon_face<false, MIXED, VARIANT>(a, fi - 1, j, a.fx, (long long)j * (nx + 1) + fi, fxs);
+            fvh_recon_face<false, MIXED, VARIANT>(a, fi - 1, j, Fx, (long long)j * (nx + 1) + fi, fxs);
     for (int fj = 0; fj <= ny; ++fj)
         for (int i = 0; i < nx; ++i)
-            fvh_recon_face<true, MIXED, VARIANT>(a, i, fj - 1, a.fy, (long long)fj * nx + i, fys);
+            fvh_recon_face<true, MIXED, VARIANT>(a, i, fj - 1, Fy, (long long)fj * nx + i, fys);
     double o[3][12];
+    auto face = [&](auto ydir, int i, int j, const FT* F, long long fstr) {
+        constexpr bool Y = decltype(ydir)::value;
+        if constexpr (BULK32) {
+            if (fk == 0) fvh_flux_face_f32<Y, 0>(a, i, j, F, fstr, o);
+            else if (fk == 1) fvh_flux_face_f32<Y, 1>(a, i, j, F, fstr, o);
+            else fvh_flux_face_f32<Y, 2>(a, i, j, F, fstr, o);
+        } else {
+            if (fk == 0) fvh_flux_face<Y, 0, MIXED>(a, i, j, F, fstr, o);
+            else if (fk == 1) fvh_flux_face<Y, 1, MIXED>(a, i, j, F, fstr, o);
+            else fvh_flux_face<Y, 2, MIXED>(a, i, j, F, fstr, o);
+        }
+    };
     for (int j = 0; j < ny; ++j)
         for (int fi = 0; fi <= nx; ++fi) {
-            if (fk == 0) fvh_flux_face<false, 0, MIXED>(a, fi - 1, j, a.fx, fxs, o);
-            else if (fk == 1) fvh_flux_face<false, 1, MIXED>(a, fi - 1, j, a.fx, fxs, o);
-            else fvh_flux_face<false, 2, MIXED>(a, fi - 1, j, a.fx, fxs, o);
+            face(std::false_type{}, fi - 1, j, Fx, fxs);
             for (int c = 0; c < 12; ++c) a.gx[c * fxs + (long long)j * (nx + 1) + fi] = (o[0][c] + o[1][c]) + o[2][c];
         }
     for (int fj = 0; fj <= ny; ++fj)
         for (int i = 0; i < nx; ++i) {
-            if (fk == 0) fvh_flux_face<true, 0, MIXED>(a, i, fj - 1, a.fy, fys, o);
-            else if (fk == 1) fvh_flux_face<true, 1, MIXED>(a, i, fj - 1, a.fy, fys, o);
-            else fvh_flux_face<true, 2, MIXED>(a, i, fj - 1, a.fy, fys, o);
+            face(std::true_type{}, i, fj - 1, Fy, fys);
             for (int c = 0; c < 12; ++c) a.gy[c * fys + (long long)fj * nx + i] = (o[0][c] + o[1][c]) + o[2][c];
         }
     double smx = 0, smy = 0;
@@ -94,6 +108,13 @@ extern "C" int fvhh_rhs_variant(int hrecon, int nx, int ny, double xl, double xr
 extern "C" int fvhh_rhs_mixed(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc,
                               int flux, int quirks, const double* q, double* rhs, double* qlim) {
     return rhs_impl<true>(nx, ny, xl, xr, yb, yt, gamma, bc, flux, quirks, q, rhs, qlim);
+}
+
+// the FP32 bulk precision's bodies (FP32 face values, quadrature, projections
+// and fluxes; FP64 Roe eigensystem, Gauss sums and update)
+extern "C" int fvhh_rhs_bulk32(int nx, int ny, double xl, double xr, double yb, double yt, double gamma, int bc,
+                               int flux, int quirks, const double* q, double* rhs, double* qlim) {
+    return rhs_impl<true, false, true>(nx, ny, xl, xr, yb, yt, gamma, bc, flux, quirks, q, rhs, qlim);
 }
 
 // The limiter in one piece (herm_limiter) and split at its chain dependence

@@ -253,6 +253,51 @@ def test_hermite_fp32_mixed_residual_close_to_fp64(orc, flux):
 
 
 @pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+def test_hermite_fp32_bulk_residual_near_fp64(orc, flux):
+    """The Hermite FP32 bulk bodies (FP32 face values, transverse quadrature,
+    back-projection, u and moment fluxes; FP64 Roe eigensystem, Gauss sums,
+    update): each field group of the residual within 1e-4 relative L1 of
+    the FP64 oracle on the vortex and on quadrant data with zero moments,
+    and a uniform state's residual exactly zero."""
+    if not os.path.exists(LIB_HERM_FAST):
+        pytest.skip("tests/host/libfvh2d_host_fast.so not built (run __graft_entry__.build())")
+    from htx.ic import vortex_modes
+
+    lib = ctypes.CDLL(LIB_HERM_FAST)
+    lib.fvhh_rhs_bulk32.argtypes = [ctypes.c_int, ctypes.c_int, D, D, D, D, D, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, DP, DP, DP]
+
+    def rhs(nx, ny, dom, bc, q):
+        q = np.ascontiguousarray(q)
+        r = np.empty_like(q)
+        ql = np.empty_like(q)
+        lib.fvhh_rhs_bulk32(nx, ny, *dom, 1.4, {"zero_grad": 0, "periodic": 1}[bc], {"rusanov": 0, "hllc": 1}[flux],
+                            1, q.ctypes.data_as(DP), r.ctypes.data_as(DP), ql.ctypes.data_as(DP))
+        return r
+
+    for case in ("vortex", "quadrant"):
+        if case == "vortex":
+            nx, ny, dom, bc = 32, 28, (0.0, 10.0, 0.0, 10.0), "periodic"
+            prm = params(nx, ny, *dom, 1.4, bc, flux, True)
+            q = orc.ic_vortex_hermite(prm, (5.0, 5.0), vortex_modes())
+        else:
+            nx, ny, dom, bc = 40, 36, (0.0, 1.0, 0.0, 1.0), "zero_grad"
+            prm = params(nx, ny, *dom, 1.4, bc, flux, True)
+            q = np.concatenate([quadrant(nx, ny, 9), np.zeros((8, ny, nx))])
+        r = rhs(nx, ny, dom, bc, q)
+        rc = orc.fvh2d_rhs(prm, q)
+        for g in range(0, 12, 4):
+            if not np.abs(rc[g:g + 4]).any():
+                assert np.abs(r[g:g + 4]).max() < 1e-6 * np.abs(rc[:4]).max(), (case, g)
+                continue
+            err = np.abs(r[g:g + 4] - rc[g:g + 4]).sum() / np.abs(rc[g:g + 4]).sum()
+            assert 0.0 < err < 1e-4, (case, g, err)
+    u = np.zeros((12, 20, 24))
+    u[:4] = np.array([1.2, 0.3, -0.4, 2.5])[:, None, None]
+    assert np.abs(rhs(24, 20, (0.0, 1.0, 0.0, 1.0), "periodic", u)).max() == 0.0
+
+
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
 def test_fp32_mixed_residual_close_to_fp64(orc, harness_fast, flux):
     """FP32-mixed mode (cut-off decisions and nonlinear weights in FP32,
     values, projections, fluxes in FP64): the residual stays within 1e-8
