@@ -100,14 +100,16 @@ enum htx_bc { HTX_BC_ZERO_GRAD = 0, HTX_BC_PERIODIC = 1 };
  *   to TENO5; the classifier runs once per step, on the state the step
  *   starts from, and its flags serve all three stages; needs
  *   htx_load_classifier.
- * HTX_PREC_FP32 (TENO5): FP32 bulk arithmetic.  The FP32-mixed cut-off
- *   decisions; the characteristic face values stored in FP32; the transverse
- *   quadrature, back-projection and HLLC / Rusanov flux in FP32; the Roe
- *   eigensystem and characteristic projection, the flux divergence, the RK
- *   accumulation and the CFL reduction in FP64.  A face whose FP32
- *   Gauss-point states have a density or pressure that is not positive or
- *   too small to be resolved in FP32 has its back-projection and flux
- *   recomputed in FP64.
+ * HTX_PREC_FP32 (TENO5, Hermite-TENO): FP32 bulk arithmetic.  The
+ *   FP32-mixed cut-off decisions; the characteristic face values stored in
+ *   FP32; the transverse quadrature, back-projection and HLLC / Rusanov flux
+ *   (Hermite-TENO: also the derivative fluxes) in FP32; the Roe eigensystem
+ *   and characteristic projection, the flux divergence, the RK accumulation
+ *   and the CFL reduction in FP64 (Hermite-TENO: the Gauss sums and the
+ *   derivative limiter too).  A face (Hermite-TENO: a Gauss point) whose FP32
+ *   states have a density or pressure that is not positive or too small to
+ *   be resolved in FP32 has its back-projection and flux recomputed in
+ *   FP64.
  * HTX_PREC_LEARNED_FP32: HTX_PREC_LEARNED on the FP32 bulk arithmetic of
  *   HTX_PREC_FP32 (needs htx_load_classifier). */
 enum htx_precision {
