@@ -246,6 +246,7 @@ int launch_stage(htx_solver* h, const double* qin, const double* q0, double* qou
         f.ctl = mode == 1 ? h->ctl : nullptr;
         f.flags_x = h->dFlagsX;
         f.flags_y = h->dFlagsY;
+        f.bulk32 = h->p.precision == HTX_PREC_LEARNED_FP32;
         HIP_TRY(lrn_launch_flags(f, h->stream));
     }
     if (h->p.dim == 1) {

@@ -409,6 +409,7 @@ int enqueue_batch_step(htx_batch* b, hipStream_t s) {
                     f.ctl = a.ctl;
                     f.flags_x = b->dFlagsX[k];
                     f.flags_y = b->dFlagsY[k];
+                    f.bulk32 = a.bulk32;
                     BHIP(lrn_launch_flags(f, s));
                 }
                 if (b->teno_prec == BP_FP64)

@@ -124,16 +124,33 @@ HTX_HD float log_amp(float amp) {
 //             ut (scales rf, pf, cf, cf): 0 for a flat stencil, about 1 for
 //             an O(1) jump
 //   x[8..11]  TENO contrasts of r, p, un, ut (eps = 1e-12 scale^2)
+// floor > 0 (states advanced in FP32 bulk arithmetic, whose rounding leaves
+// noise of about 1e-7 of the state between neighbouring cells): a
+// difference within floor x its scale counts as zero, so that a stencil that
+// is flat up to that noise has the features of a flat one.
 HTX_HD void face_features(const double r[6], const double un[6], const double ut[6], const double p[6],
-                          double gam, float x[CLS_IN]) {
+                          double gam, float x[CLS_IN], float floor = 0.0f) {
     HTX_NO_CONTRACT
     float dr[5], dp[5], du[5], dv[5];
     float sr = 0.0f, sp = 0.0f, ar = 0.0f, ap = 0.0f, au = 0.0f, at = 0.0f;
+    float zr = 0.0f, zp = 0.0f, zc = 0.0f;
+    if (floor > 0.0f) {
+        const float fr = (float)(0.5 * (fabs(r[2]) + fabs(r[3]))), fp = (float)(0.5 * (fabs(p[2]) + fabs(p[3])));
+        zr = floor * fr;
+        zp = floor * fp;
+        zc = floor * sqrtf((float)gam * fp * frcp(fr + 1e-30f));
+    }
     for (int k = 0; k < 5; ++k) {
         dr[k] = (float)(r[k + 1] - r[k]);
         dp[k] = (float)(p[k + 1] - p[k]);
         du[k] = (float)(un[k + 1] - un[k]);
         dv[k] = (float)(ut[k + 1] - ut[k]);
+        if (floor > 0.0f) {
+            dr[k] = fabsf(dr[k]) <= zr ? 0.0f : dr[k];
+            dp[k] = fabsf(dp[k]) <= zp ? 0.0f : dp[k];
+            du[k] = fabsf(du[k]) <= zc ? 0.0f : du[k];
+            dv[k] = fabsf(dv[k]) <= zc ? 0.0f : dv[k];
+        }
         sr += fabsf(dr[k]);
         sp += fabsf(dp[k]);
         ar = fmaxf(ar, fabsf(dr[k]));

@@ -337,6 +337,14 @@ constexpr int FLAG_NCELL_MAX = FlagBlock<false>::NCELL > FlagBlock<true>::NCELL 
 // a = -1 .. n_along; its flag goes to flags_x[b][a + 1] (x faces, a row of
 // nx + 2) or flags_y[a + 1][b] (y faces, a row of nx).  The features are
 // those of stencil_features on the same six cells, bit for bit.
+// the features' noise floor on FP32-bulk states, relative to each quantity's
+// scale: the rounding noise of those states is about 1e-7 to 1e-6 of the
+// state, and a real jump of 1e-5 of the state moves a linear face value by
+// less than that again
+#ifndef HTX_LRN_FP32_FLOOR
+#define HTX_LRN_FP32_FLOOR 1e-5f
+#endif
+
 template <bool YDIR>
 __global__ void __launch_bounds__(256) k_face_flags(LrnFlagsArgs a) {
     using FB_ = FlagBlock<YDIR>;
@@ -396,7 +404,7 @@ __global__ void __launch_bounds__(256) k_face_flags(LrnFlagsArgs a) {
             ut[m] = cell[5][c[m]];
             p[m] = cell[6][c[m]];
         }
-        face_features(r, un, ut, p, a.gam, x);
+        face_features(r, un, ut, p, a.gam, x, a.bulk32 ? HTX_LRN_FP32_FLOOR : 0.0f);
     }
     cls_stage_features(xs[tid >> 6], x);
     __syncthreads();

@@ -78,6 +78,7 @@ struct LrnFlagsArgs {
     const void* ctl;            // Ctl* of the solver, or null
     unsigned char* flags_x;
     unsigned char* flags_y;
+    int bulk32;                 // the state is advanced in FP32 bulk arithmetic (noise floor in the features)
 };
 
 hipError_t lrn_launch_extract(const LrnExtractArgs& a, int dir, hipStream_t s);

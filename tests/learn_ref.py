@@ -83,8 +83,11 @@ def features(q, gamma, bc, direction, flag_faces=False):
     return stencil_features(st, gamma, direction)
 
 
-def stencil_features(st, gamma, direction):
-    """The 12 inputs of the faces of stencils st (6, 4, ...)."""
+def stencil_features(st, gamma, direction, floor=0.0):
+    """The 12 inputs of the faces of stencils st (6, 4, ...).  floor > 0:
+    differences within floor x their scale (density, pressure, sound speed
+    of the face) count as zero (the features of states advanced in FP32 bulk
+    arithmetic)."""
     g1 = gamma - 1.0
     r = st[:, 0]
     qn, qt = (st[:, 1], st[:, 2]) if direction == 0 else (st[:, 2], st[:, 1])
@@ -94,6 +97,13 @@ def stencil_features(st, gamma, direction):
     dp = [F32(p[k + 1] - p[k]) for k in range(5)]
     du = [F32(un[k + 1] - un[k]) for k in range(5)]
     dv = [F32(ut[k + 1] - ut[k]) for k in range(5)]
+    if floor > 0.0:
+        fr = F32(0.5 * (np.abs(r[2]) + np.abs(r[3])))
+        fp = F32(0.5 * (np.abs(p[2]) + np.abs(p[3])))
+        zr, zp = F(floor) * fr, F(floor) * fp
+        zc = F(floor) * np.sqrt(F(gamma) * fp * (F(1.0) / (fr + F(1e-30))))
+        zero = lambda d, z: [np.where(np.abs(x) <= z, F(0.0), x) for x in d]
+        dr, dp, du, dv = zero(dr, zr), zero(dp, zp), zero(du, zc), zero(dv, zc)
     sr = sp = np.zeros(r.shape[1:], dtype=np.float32)
     ar = ap = au = at = np.zeros(r.shape[1:], dtype=np.float32)
     for k in range(5):
