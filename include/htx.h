@@ -111,7 +111,10 @@ enum htx_bc { HTX_BC_ZERO_GRAD = 0, HTX_BC_PERIODIC = 1 };
  *   be resolved in FP32 has its back-projection and flux recomputed in
  *   FP64.
  * HTX_PREC_LEARNED_FP32: HTX_PREC_LEARNED on the FP32 bulk arithmetic of
- *   HTX_PREC_FP32 (needs htx_load_classifier). */
+ *   HTX_PREC_FP32 (needs htx_load_classifier); the classifier's features
+ *   count a stencil difference within 1e-5 of its scale (the face's density,
+ *   pressure or sound speed) as zero, below which FP32-bulk states carry
+ *   rounding noise. */
 enum htx_precision {
     HTX_PREC_FP64 = 0,
     HTX_PREC_FP32_MIXED = 1,
