@@ -877,10 +877,13 @@ HTX_HD void fvh_recon_face(const FvhArgs& a, int i, int j, FT* F, long long fid,
     // hteno on (L u, L d_normal) over cells -1..2, teno5 on L d_tangential over -2..3
     double A[4][4], B[4][4], C[4][6];
     const int nc = YDIR ? 4 : 0;  // normal-derivative block: v (x) or w (y) -> limited comp offset
-    // FP32-mixed: the loops are unrolled so that A, B, C stay in registers
-    // (125 VGPRs); FP64: they stay loops over A, B, C in scratch, 164 VGPRs
-    // and three waves per SIMD instead of 262 and one (+13 % for fp64_fast;
-    // the strict build's compiler makes that choice by itself)
+    // FP32-mixed (and FP32 bulk): the loops are unrolled so that A, B, C stay
+    // in registers (236 VGPRs, two waves per SIMD; rolled, 129 VGPRs and
+    // three waves, the Hermite stage ran 6 % slower, and with a four-wave
+    // launch bound 13 % slower); FP64: they stay loops over A, B, C in
+    // scratch, 164 VGPRs and three waves per SIMD instead of 262 and one
+    // (+13 % for fp64_fast; the strict build's compiler makes that choice by
+    // itself)
 #pragma unroll(MIXED ? 6 : 1)
     for (int m = 0; m < 6; ++m) {
         int k = m - 2;
